@@ -1,0 +1,104 @@
+"""Problem set-up: initial conditions of the reference's drivers, restated.
+
+* ``riemann_prim`` follows ``IC_for_riemann_2d`` (reference
+  ``src/2Dproblem.cpp:345-385``): primitive states by quadrant for interior
+  cells, zeros in the ghost layers, with the quadrant tests written against
+  ``discon/dx`` on both axes exactly as the reference does.
+* ``riemann_states(seed)`` draws the four quadrant states of the seeded
+  four-quadrant Riemann workload with ``std::mt19937_64(seed)`` and
+  ``std::uniform_real_distribution<double>`` semantics (libstdc++
+  ``generate_canonical``), in the order zone 1..4, each (rho, u, v, p), with
+  rho in [0.1, 1.5], u, v in [-0.8, 0.8], p in [0.02, 1.5].
+"""
+import numpy as np
+
+_MASK64 = (1 << 64) - 1
+
+
+class MT19937_64:
+    """The 64-bit Mersenne Twister (std::mt19937_64)."""
+
+    def __init__(self, seed=5489):
+        self.mt = [0] * 312
+        self.mt[0] = seed & _MASK64
+        for i in range(1, 312):
+            self.mt[i] = (6364136223846793005 * (self.mt[i - 1] ^ (self.mt[i - 1] >> 62)) + i) & _MASK64
+        self.idx = 312
+
+    def _twist(self):
+        mt = self.mt
+        for i in range(312):
+            x = (mt[i] & 0xFFFFFFFF80000000) | (mt[(i + 1) % 312] & 0x7FFFFFFF)
+            xa = x >> 1
+            if x & 1:
+                xa ^= 0xB5026F5AA96619E9
+            mt[i] = mt[(i + 156) % 312] ^ xa
+        self.idx = 0
+
+    def __call__(self):
+        if self.idx >= 312:
+            self._twist()
+        y = self.mt[self.idx]
+        self.idx += 1
+        y ^= (y >> 29) & 0x5555555555555555
+        y ^= (y << 17) & 0x71D67FFFEDA60000
+        y ^= (y << 37) & 0xFFF7EEE000000000
+        y ^= y >> 43
+        return y & _MASK64
+
+    def uniform(self, a, b):
+        # generate_canonical<double, 53> with a 64-bit engine: one draw
+        r = float(self()) / 18446744073709551616.0
+        if r >= 1.0:
+            r = np.nextafter(1.0, 0.0)
+        return r * (b - a) + a
+
+
+RIEMANN_RANGES = ((0.1, 1.5), (-0.8, 0.8), (-0.8, 0.8), (0.02, 1.5))
+
+
+def riemann_states(seed):
+    """Four quadrant primitive states (zone1..zone4, each rho,u,v,p)."""
+    g = MT19937_64(seed)
+    return np.array([[g.uniform(*RIEMANN_RANGES[k]) for k in range(4)] for _ in range(4)])
+
+
+def riemann_prim(p, zones, discon=(0.5, 0.5)):
+    """Primitive field [4, nx, ny] of IC_for_riemann_2d (ghosts zero)."""
+    G, nx, ny = p.ghost, p.nx, p.ny
+    dx = (p.right - p.left) / p.nodex
+    prim = np.zeros((4, nx, ny))
+    zones = np.asarray(zones, dtype=np.float64)
+    for i in range(G, nx - G):
+        xi = (i - G + 1) < discon[0] / dx
+        for j in range(G, ny - G):
+            yj = (j - G + 1) < discon[1] / dx
+            if xi and yj:
+                z = 0
+            elif (not xi) and yj:
+                z = 1
+            elif (not xi) and (not yj):
+                z = 2
+            else:
+                z = 3
+            prim[:, i, j] = zones[z]
+    return prim
+
+
+def prim_to_convar(prim, gamma):
+    """Primvar_to_convar_2D (reference src/basic_function.cpp:860-866) on a
+    [4, ...] array; ghost cells with rho=0 give NaN energy as in the
+    reference."""
+    r, u, v, pr = prim
+    with np.errstate(invalid="ignore", divide="ignore"):
+        e = r * (pr / r / (gamma - 1) + 0.5 * (u * u + v * v))
+    return np.stack([r, r * u, r * v, e])
+
+
+# RM2 configuration used by PlanarShock (reference src/2Dproblem.cpp:86-92)
+PLANAR_SHOCK_ZONES = np.array([
+    [0.138, 1.206, 1.206, 0.029],
+    [0.5323, 0.0, 1.206, 0.3],
+    [1.5, 0.0, 0.0, 1.5],
+    [0.5323, 1.206, 0.0, 0.3],
+])

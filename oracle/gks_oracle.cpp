@@ -1,0 +1,203 @@
+// gks_oracle.cpp — C API over the CPU restatement (oracle).  TEST
+// INFRASTRUCTURE ONLY: loaded by tests/, __graft_entry__.smoke() and the
+// cpu_baseline leg of bench.py as the checker; the product never links it.
+#include <cstdint>
+#include <cstring>
+
+#include "solver2d.hpp"
+
+namespace orc {
+template <> double Solver2d<double>::to_double(const double& r) { return r; }
+}  // namespace orc
+
+using S2 = orc::Solver2d<double>;
+
+static bool params_ok(const gks2d_params* p) {
+  if (!p) return false;
+  if (p->nodex < 5 || p->nodey < 5 || p->ghost != 5) return false;
+  if (p->weno_order != 5 || p->gausspoint != 2) return false;
+  return true;
+}
+
+extern "C" {
+
+void* orc2d_create(const gks2d_params* p) {
+  if (!params_ok(p)) return nullptr;
+  return new S2(*p);
+}
+void orc2d_destroy(void* h) { delete static_cast<S2*>(h); }
+
+// Sets primvar of all nx*ny cells (SoA [4][nx*ny]) and convar =
+// Primvar_to_convar_2D(primvar) for every cell, as IC_for_riemann_2d does
+// (2Dproblem.cpp:345-385).
+int orc2d_set_prim(void* h, const double* prim) {
+  S2* s = static_cast<S2*>(h);
+  size_t nc = (size_t)s->nx * s->ny;
+  for (size_t c = 0; c < nc; c++) {
+    double p[4], u[4];
+    for (int v = 0; v < 4; v++) p[v] = prim[v * nc + c];
+    orc::primvar_to_convar_2d(u, p, s->g);
+    for (int v = 0; v < 4; v++) {
+      s->prim[v][c] = p[v];
+      s->U[v][c] = u[v];
+    }
+  }
+  return GKS_OK;
+}
+int orc2d_set_convar(void* h, const double* conv) {
+  S2* s = static_cast<S2*>(h);
+  size_t nc = (size_t)s->nx * s->ny;
+  for (int v = 0; v < 4; v++) std::memcpy(s->U[v].data(), conv + v * nc, nc * 8);
+  return GKS_OK;
+}
+int orc2d_set_alpha(void* h, const double* ax, const double* ay) {
+  S2* s = static_cast<S2*>(h);
+  size_t nc = (size_t)s->nx * s->ny;
+  std::memcpy(s->ax.data(), ax, nc * 8);
+  std::memcpy(s->ay.data(), ay, nc * 8);
+  return GKS_OK;
+}
+int orc2d_get(void* h, double* conv, double* prim, double* ax, double* ay, double* t,
+              int32_t* step) {
+  S2* s = static_cast<S2*>(h);
+  size_t nc = (size_t)s->nx * s->ny;
+  for (int v = 0; v < 4; v++) {
+    if (conv) std::memcpy(conv + v * nc, s->U[v].data(), nc * 8);
+    if (prim) std::memcpy(prim + v * nc, s->prim[v].data(), nc * 8);
+  }
+  if (ax) std::memcpy(ax, s->ax.data(), nc * 8);
+  if (ay) std::memcpy(ay, s->ay.data(), nc * 8);
+  if (t) *t = s->t;
+  if (step) *step = s->step;
+  return GKS_OK;
+}
+int orc2d_set_time(void* h, double t, int32_t step) {
+  S2* s = static_cast<S2*>(h);
+  s->t = t;
+  s->step = step;
+  return GKS_OK;
+}
+int orc2d_run(void* h, const int32_t* bc, double tstop, int32_t max_steps, double* dt_hist) {
+  S2* s = static_cast<S2*>(h);
+  int b[4] = {bc[0], bc[1], bc[2], bc[3]};
+  return s->run(b, tstop, max_steps, dt_hist);
+}
+int orc2d_copy_old(void* h) {
+  static_cast<S2*>(h)->copy_old();
+  return GKS_OK;
+}
+int orc2d_cfl(void* h, double tstop, double* dt) {
+  *dt = static_cast<S2*>(h)->cfl(tstop);
+  return GKS_OK;
+}
+int orc2d_stage(void* h, int32_t stage, double dt, const int32_t* bc) {
+  S2* s = static_cast<S2*>(h);
+  int b[4] = {bc[0], bc[1], bc[2], bc[3]};
+  return s->stage_run(stage, dt, b);
+}
+// fluxes of `stage`: xf (nx+1)*ny faces, yf nx*(ny+1) faces, gp*8 doubles each
+int orc2d_get_face_fluxes(void* h, int32_t stage, double* xf, double* yf) {
+  S2* s = static_cast<S2*>(h);
+  int gp = s->gp;
+  for (int i = 0; i <= s->nx; i++)
+    for (int j = 0; j < s->ny; j++)
+      std::memcpy(xf + ((size_t)(i * s->ny + j) * gp) * 8,
+                  &s->xfl[stage][(size_t)s->face(i, j) * gp * 8], gp * 8 * 8);
+  for (int i = 0; i < s->nx; i++)
+    for (int j = 0; j <= s->ny; j++)
+      std::memcpy(yf + ((size_t)(i * (s->ny + 1) + j) * gp) * 8,
+                  &s->yfl[stage][(size_t)s->face(i, j) * gp * 8], gp * 8 * 8);
+  return GKS_OK;
+}
+// Gauss-point states: xg (nx+1)*ny faces, yg nx*(ny+1) faces, each
+// gp x [left 12, right 12]
+int orc2d_get_gauss(void* h, double* xg, double* yg) {
+  S2* s = static_cast<S2*>(h);
+  int gp = s->gp;
+  auto put = [&](double* dst, const orc::Pt<double>& a) {
+    std::memcpy(dst, a.convar, 32);
+    std::memcpy(dst + 4, a.der1x, 32);
+    std::memcpy(dst + 8, a.der1y, 32);
+  };
+  for (int i = 0; i <= s->nx; i++)
+    for (int j = 0; j < s->ny; j++)
+      for (int q = 0; q < gp; q++) {
+        size_t k = (size_t)s->face(i, j) * gp + q;
+        double* d = xg + (((size_t)(i * s->ny + j) * gp + q) * 24);
+        put(d, s->xgL[k]);
+        put(d + 12, s->xgR[k]);
+      }
+  for (int i = 0; i < s->nx; i++)
+    for (int j = 0; j <= s->ny; j++)
+      for (int q = 0; q < gp; q++) {
+        size_t k = (size_t)s->face(i, j) * gp + q;
+        double* d = yg + (((size_t)(i * (s->ny + 1) + j) * gp + q) * 24);
+        put(d, s->ygL[k]);
+        put(d + 12, s->ygR[k]);
+      }
+  return GKS_OK;
+}
+
+// ------------------------------------------------------------ leaf entries
+static void load_pt(orc::Pt<double>& p, const double* s) {
+  std::memcpy(p.convar, s, 32);
+  std::memcpy(p.der1x, s + 4, 32);
+  std::memcpy(p.der1y, s + 8, 32);
+}
+static void store_pt(double* d, const orc::Pt<double>& p) {
+  std::memcpy(d, p.convar, 32);
+  std::memcpy(d + 4, p.der1x, 32);
+  std::memcpy(d + 8, p.der1y, 32);
+}
+// centre state + GKS2D for n points (left/right/center n x 12, f/derf n x 4)
+int orc2d_flux_points(const gks2d_params* p, int64_t n, const double* left, const double* right,
+                      double dt, double* center, double* f, double* derf) {
+  orc::Gas g = orc::gas_from(*p);
+#pragma omp parallel for
+  for (int64_t k = 0; k < n; k++) {
+    orc::Pt<double> L, R, C;
+    load_pt(L, left + k * 12);
+    load_pt(R, right + k * 12);
+    orc::center_collision(C, L, R, g);
+    if (center) store_pt(center + k * 12, C);
+    orc::gks2d_flux(f + k * 4, derf + k * 4, L, R, C, dt, g);
+  }
+  return GKS_OK;
+}
+// WENO5_AO_with_df for one cell: w5 = 5 x 4 convar (x-like frame), alpha5,
+// out: left8 / right8 = convar[4], der1x[4] at the cell's left/right face
+int orc2d_weno5_normal(const gks2d_params* p, int64_t n, const double* w5, const double* alpha5,
+                       double h, double* left8, double* right8) {
+  orc::Gas g = orc::gas_from(*p);
+  for (int64_t k = 0; k < n; k++) {
+    orc::Pt<double> L, R;
+    const double* w = w5 + k * 20;
+    orc::weno5_df_normal(L, R, alpha5 + k * 5, w, w + 4, w + 8, w + 12, w + 16, h, p->recon_variable,
+                         g);
+    std::memcpy(left8 + k * 8, L.convar, 32);
+    std::memcpy(left8 + k * 8 + 4, L.der1x, 32);
+    std::memcpy(right8 + k * 8, R.convar, 32);
+    std::memcpy(right8 + k * 8 + 4, R.der1x, 32);
+  }
+  return GKS_OK;
+}
+// weno_5th_ao_with_df_tangential, one side: line5 = 5 x 8 (convar, der1x),
+// alpha5; out gp2 = 2 x 12 Gauss-point states
+int orc2d_weno5_tangent(const gks2d_params* p, int64_t n, const double* line5,
+                        const double* alpha5, double h, double* gp2) {
+  orc::Gas g = orc::gas_from(*p);
+  for (int64_t k = 0; k < n; k++) {
+    orc::Pt<double> line[5], out[2];
+    for (int s = 0; s < 5; s++) {
+      std::memcpy(line[s].convar, line5 + k * 40 + s * 8, 32);
+      std::memcpy(line[s].der1x, line5 + k * 40 + s * 8 + 4, 32);
+      for (int v = 0; v < 4; v++) line[s].der1y[v] = 0.0;
+    }
+    orc::weno5_df_tangent_side(out, line, alpha5 + k * 5, h, p->recon_variable, g);
+    store_pt(gp2 + k * 24, out[0]);
+    store_pt(gp2 + k * 24 + 12, out[1]);
+  }
+  return GKS_OK;
+}
+
+}  // extern "C"

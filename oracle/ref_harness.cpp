@@ -1,0 +1,321 @@
+// ref_harness.cpp — drives the compiled reference solver (its own sources,
+// built by oracle/Makefile into oracle/_ref/) through the same C API as the
+// oracle (orc2d_* <-> ref2d_*), so the tests can check the oracle against the
+// reference on identical inputs.  TEST INFRASTRUCTURE ONLY.
+//
+// It sets the reference's globals the way PlanarShock does
+// (2Dproblem.cpp:8-53), allocates with the reference's Set* functions and
+// calls the reference's sweeps in the driver's order (2Dproblem.cpp:109-161),
+// without the stdin prompt and with std::cout silenced.
+#include <cstdint>
+#include <cstring>
+#include <iostream>
+
+#include "2Dproblem.h"
+#include "../include/gks2d.h"
+
+namespace {
+
+struct RefCtx {
+  gks2d_params p;
+  Block2d block;
+  Fluid2d* fluids = nullptr;
+  Interface2d* xi = nullptr;
+  Interface2d* yi = nullptr;
+  Flux2d_gauss** xf = nullptr;
+  Flux2d_gauss** yf = nullptr;
+  Fluid2d bcvalue[4];
+};
+
+void set_globals(const gks2d_params* p) {
+  K = p->K;
+  Gamma = p->gamma;
+  Mu = p->mu;
+  Nu = p->nu;
+  Pr = p->pr;
+  c1_euler = p->c1_euler;
+  c2_euler = p->c2_euler;
+  tau_type = p->tau_type == GKS_TAU_NS ? NS : Euler;
+  is_Prandtl_fix = p->prandtl_fix != 0;
+  reconstruction_variable = p->recon_variable == GKS_VAR_CHARACTERISTIC ? characteristic : conservative;
+  wenotype = wenoz;
+  is_reduce_order_warning = false;
+  is_using_df_factor = p->use_df != 0;
+  g0reconstruction_2D_normal = Center_do_nothing_normal;
+  g0reconstruction_2D_tangent = Center_all_collision_multi;
+  gks2dsolver = gks2nd_2d;
+  flux_function_2d = GKS2D;
+  timecoe_list_2d = S2O4_2D;
+  if (p->weno_order == 5) {
+    cellreconstruction_2D_normal = p->use_df ? WENO5_AO_with_df_normal : WENO5_AO_normal;
+    cellreconstruction_2D_tangent = p->use_df ? WENO5_AO_with_df_tangent : WENO5_AO_tangent;
+  } else if (p->weno_order == 7) {
+    cellreconstruction_2D_normal = WENO7_AO_with_df_normal;
+    cellreconstruction_2D_tangent = WENO7_AO_with_df_tangent;
+  } else {
+    cellreconstruction_2D_normal = WENO9_AO_with_df_normal;
+    cellreconstruction_2D_tangent = WENO9_AO_with_df_tangent;
+  }
+}
+
+BoundaryCondition2d bc_fn(int side, int kind) {
+  static const BoundaryCondition2d tab[3][4] = {
+      {free_boundary_left, free_boundary_right, free_boundary_down, free_boundary_up},
+      {periodic_boundary_left, periodic_boundary_right, periodic_boundary_down, periodic_boundary_up},
+      {reflection_boundary_left, reflection_boundary_right, reflection_boundary_down,
+       reflection_boundary_up}};
+  return tab[kind][side];
+}
+
+void silence() {
+  static bool done = false;
+  if (!done) {
+    std::cout.rdbuf(nullptr);
+    done = true;
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+void* ref2d_create(const gks2d_params* p) {
+  silence();
+  RefCtx* c = new RefCtx();
+  c->p = *p;
+  set_globals(p);
+  gausspoint = p->gausspoint;
+  SetGuassPoint();
+  Block2d& b = c->block;
+  b.uniform = true;
+  b.nodex = p->nodex;
+  b.nodey = p->nodey;
+  b.ghost = p->ghost;
+  b.CFL = p->cfl;
+  Initial_stages(b);
+  b.nx = b.nodex + 2 * b.ghost;
+  b.ny = b.nodey + 2 * b.ghost;
+  c->fluids = Setfluid(b);
+  c->xi = Setinterface_array(b);
+  c->yi = Setinterface_array(b);
+  c->xf = Setflux_gauss_array(b);
+  c->yf = Setflux_gauss_array(b);
+  b.left = p->left;
+  b.right = p->right;
+  b.down = p->down;
+  b.up = p->up;
+  b.dx = (b.right - b.left) / b.nodex;
+  b.dy = (b.up - b.down) / b.nodey;
+  b.overdx = 1 / b.dx;
+  b.overdy = 1 / b.dy;
+  SetUniformMesh(b, c->fluids, c->xi, c->yi, c->xf, c->yf);
+  b.t = 0;
+  b.step = 0;
+  return c;
+}
+
+// the reference never frees its arrays either (2Dproblem.cpp); the harness
+// releases the top-level blocks only
+void ref2d_destroy(void* h) {
+  RefCtx* c = static_cast<RefCtx*>(h);
+  delete[] c->fluids;
+  delete c;
+}
+
+int ref2d_set_prim(void* h, const double* prim) {
+  RefCtx* c = static_cast<RefCtx*>(h);
+  size_t nc = (size_t)c->block.nx * c->block.ny;
+  for (size_t k = 0; k < nc; k++) {
+    for (int v = 0; v < 4; v++) c->fluids[k].primvar[v] = prim[v * nc + k];
+    Primvar_to_convar_2D(c->fluids[k].convar, c->fluids[k].primvar);
+  }
+  return GKS_OK;
+}
+int ref2d_set_convar(void* h, const double* conv) {
+  RefCtx* c = static_cast<RefCtx*>(h);
+  size_t nc = (size_t)c->block.nx * c->block.ny;
+  for (size_t k = 0; k < nc; k++)
+    for (int v = 0; v < 4; v++) c->fluids[k].convar[v] = conv[v * nc + k];
+  return GKS_OK;
+}
+int ref2d_set_alpha(void* h, const double* ax, const double* ay) {
+  RefCtx* c = static_cast<RefCtx*>(h);
+  size_t nc = (size_t)c->block.nx * c->block.ny;
+  for (size_t k = 0; k < nc; k++) {
+    c->fluids[k].alpha_x = ax[k];
+    c->fluids[k].alpha_y = ay[k];
+  }
+  return GKS_OK;
+}
+int ref2d_get(void* h, double* conv, double* prim, double* ax, double* ay, double* t,
+              int32_t* step) {
+  RefCtx* c = static_cast<RefCtx*>(h);
+  size_t nc = (size_t)c->block.nx * c->block.ny;
+  for (size_t k = 0; k < nc; k++) {
+    for (int v = 0; v < 4; v++) {
+      if (conv) conv[v * nc + k] = c->fluids[k].convar[v];
+      if (prim) prim[v * nc + k] = c->fluids[k].primvar[v];
+    }
+    if (ax) ax[k] = c->fluids[k].alpha_x;
+    if (ay) ay[k] = c->fluids[k].alpha_y;
+  }
+  if (t) *t = c->block.t;
+  if (step) *step = c->block.step;
+  return GKS_OK;
+}
+int ref2d_set_time(void* h, double t, int32_t step) {
+  RefCtx* c = static_cast<RefCtx*>(h);
+  c->block.t = t;
+  c->block.step = step;
+  return GKS_OK;
+}
+
+int ref2d_copy_old(void* h) {
+  RefCtx* c = static_cast<RefCtx*>(h);
+  CopyFluid_new_to_old(c->fluids, c->block);
+  return GKS_OK;
+}
+int ref2d_cfl(void* h, double tstop, double* dt) {
+  RefCtx* c = static_cast<RefCtx*>(h);
+  *dt = Get_CFL(c->block, c->fluids, tstop);
+  return GKS_OK;
+}
+int ref2d_stage(void* h, int32_t stage, double dt, const int32_t* bc) {
+  RefCtx* c = static_cast<RefCtx*>(h);
+  set_globals(&c->p);
+  Block2d& b = c->block;
+  b.dt = dt;
+  for (int s = 0; s < 4; s++) bc_fn(s, bc[s])(c->fluids, b, c->bcvalue[s]);
+  Convar_to_Primvar(c->fluids, b);
+  Reconstruction_within_cell(c->xi, c->yi, c->fluids, b);
+  Reconstruction_forg0(c->xi, c->yi, c->fluids, b);
+  Calculate_flux(c->xf, c->yf, c->xi, c->yi, b, stage);
+  Update(c->fluids, c->xf, c->yf, b, stage);
+  if (is_using_df_factor) Update_alpha(c->xi, c->yi, c->fluids, b);
+  return GKS_OK;
+}
+int ref2d_run(void* h, const int32_t* bc, double tstop, int32_t max_steps, double* dt_hist) {
+  RefCtx* c = static_cast<RefCtx*>(h);
+  Block2d& b = c->block;
+  int n = 0;
+  while (b.t < tstop && n < max_steps) {
+    CopyFluid_new_to_old(c->fluids, b);
+    b.dt = Get_CFL(b, c->fluids, tstop);
+    if (dt_hist) dt_hist[n] = b.dt;
+    for (int s = 0; s < b.stages; s++) ref2d_stage(h, s, b.dt, bc);
+    b.step++;
+    b.t = b.t + b.dt;
+    n++;
+  }
+  return GKS_OK;
+}
+int ref2d_get_face_fluxes(void* h, int32_t stage, double* xfo, double* yfo) {
+  RefCtx* c = static_cast<RefCtx*>(h);
+  int nx = c->block.nx, ny = c->block.ny, gp = gausspoint;
+  for (int i = 0; i <= nx; i++)
+    for (int j = 0; j < ny; j++)
+      for (int q = 0; q < gp; q++) {
+        Flux2d& f = c->xf[i * (ny + 1) + j][stage].gauss[q];
+        double* d = xfo + ((size_t)(i * ny + j) * gp + q) * 8;
+        std::memcpy(d, f.f, 32);
+        std::memcpy(d + 4, f.derf, 32);
+      }
+  for (int i = 0; i < nx; i++)
+    for (int j = 0; j <= ny; j++)
+      for (int q = 0; q < gp; q++) {
+        Flux2d& f = c->yf[i * (ny + 1) + j][stage].gauss[q];
+        double* d = yfo + ((size_t)(i * (ny + 1) + j) * gp + q) * 8;
+        std::memcpy(d, f.f, 32);
+        std::memcpy(d + 4, f.derf, 32);
+      }
+  return GKS_OK;
+}
+static void put_point(double* d, const Point2d& p) {
+  std::memcpy(d, p.convar, 32);
+  std::memcpy(d + 4, p.der1x, 32);
+  std::memcpy(d + 8, p.der1y, 32);
+}
+int ref2d_get_gauss(void* h, double* xg, double* yg) {
+  RefCtx* c = static_cast<RefCtx*>(h);
+  int nx = c->block.nx, ny = c->block.ny, gp = gausspoint;
+  for (int i = 0; i <= nx; i++)
+    for (int j = 0; j < ny; j++)
+      for (int q = 0; q < gp; q++) {
+        Recon2d& r = c->xi[i * (ny + 1) + j].gauss[q];
+        double* d = xg + ((size_t)(i * ny + j) * gp + q) * 24;
+        put_point(d, r.left);
+        put_point(d + 12, r.right);
+      }
+  for (int i = 0; i < nx; i++)
+    for (int j = 0; j <= ny; j++)
+      for (int q = 0; q < gp; q++) {
+        Recon2d& r = c->yi[i * (ny + 1) + j].gauss[q];
+        double* d = yg + ((size_t)(i * (ny + 1) + j) * gp + q) * 24;
+        put_point(d, r.left);
+        put_point(d + 12, r.right);
+      }
+  return GKS_OK;
+}
+
+static void get_point(Point2d& p, const double* s) {
+  std::memcpy(p.convar, s, 32);
+  std::memcpy(p.der1x, s + 4, 32);
+  std::memcpy(p.der1y, s + 8, 32);
+}
+int ref2d_flux_points(const gks2d_params* p, int64_t n, const double* left, const double* right,
+                      double dt, double* center, double* f, double* derf) {
+  silence();
+  set_globals(p);
+  for (int64_t k = 0; k < n; k++) {
+    Recon2d r;
+    get_point(r.left, left + k * 12);
+    get_point(r.right, right + k * 12);
+    Center_all_collision_2d_multi(r);
+    Flux2d fl;
+    GKS2D(fl, r, dt);
+    if (center) put_point(center + k * 12, r.center);
+    std::memcpy(f + k * 4, fl.f, 32);
+    std::memcpy(derf + k * 4, fl.derf, 32);
+  }
+  return GKS_OK;
+}
+int ref2d_weno5_normal(const gks2d_params* p, int64_t n, const double* w5, const double* alpha5,
+                       double h, double* left8, double* right8) {
+  silence();
+  set_globals(p);
+  for (int64_t k = 0; k < n; k++) {
+    Point2d L, R;
+    double w[5][4], a[5];
+    std::memcpy(w, w5 + k * 20, sizeof(w));
+    std::memcpy(a, alpha5 + k * 5, sizeof(a));
+    WENO5_AO_with_df(L, R, a, w[0], w[1], w[2], w[3], w[4], h);
+    std::memcpy(left8 + k * 8, L.convar, 32);
+    std::memcpy(left8 + k * 8 + 4, L.der1x, 32);
+    std::memcpy(right8 + k * 8, R.convar, 32);
+    std::memcpy(right8 + k * 8 + 4, R.der1x, 32);
+  }
+  return GKS_OK;
+}
+int ref2d_weno5_tangent(const gks2d_params* p, int64_t n, const double* line5,
+                        const double* alpha5, double h, double* gp2) {
+  silence();
+  set_globals(p);
+  gausspoint = 2;
+  for (int64_t k = 0; k < n; k++) {
+    Recon2d line[5], re[2];
+    for (int s = 0; s < 5; s++) {
+      std::memcpy(line[s].left.convar, line5 + k * 40 + s * 8, 32);
+      std::memcpy(line[s].left.der1x, line5 + k * 40 + s * 8 + 4, 32);
+      line[s].right = line[s].left;
+    }
+    double a1[5], a2[5];
+    std::memcpy(a1, alpha5 + k * 5, sizeof(a1));
+    std::memcpy(a2, alpha5 + k * 5, sizeof(a2));
+    weno_5th_ao_with_df_tangential(re, line[0], line[1], line[2], line[3], line[4], a1, a2, h);
+    put_point(gp2 + k * 24, re[0].left);
+    put_point(gp2 + k * 24 + 12, re[1].left);
+  }
+  return GKS_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,188 @@
+"""ctypes access to the oracle (oracle/liboracle.so, the CPU restatement) and,
+when it has been built, to the compiled reference (oracle/_ref/libgksref.so).
+Both export the same C API (orc2d_* / ref2d_*).  Test infrastructure only."""
+import ctypes
+import os
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+ORACLE_SO = os.path.join(ROOT, "oracle", "liboracle.so")
+REF_SO = os.path.join(ROOT, "oracle", "_ref", "libgksref.so")
+
+_P = ctypes.c_void_p
+_D = ctypes.POINTER(ctypes.c_double)
+_I = ctypes.POINTER(ctypes.c_int32)
+
+
+def dptr(a):
+    if a is None:
+        return None
+    assert a.dtype == np.float64 and a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data_as(_D)
+
+
+def _bind(lib, prefix):
+    from gks_amd.params import Params
+    pp = ctypes.POINTER(Params)
+    sig = {
+        "create": (_P, [pp]),
+        "destroy": (None, [_P]),
+        "set_prim": (ctypes.c_int, [_P, _D]),
+        "set_convar": (ctypes.c_int, [_P, _D]),
+        "set_alpha": (ctypes.c_int, [_P, _D, _D]),
+        "get": (ctypes.c_int, [_P, _D, _D, _D, _D, _D, _I]),
+        "set_time": (ctypes.c_int, [_P, ctypes.c_double, ctypes.c_int32]),
+        "run": (ctypes.c_int, [_P, _I, ctypes.c_double, ctypes.c_int32, _D]),
+        "copy_old": (ctypes.c_int, [_P]),
+        "cfl": (ctypes.c_int, [_P, ctypes.c_double, _D]),
+        "stage": (ctypes.c_int, [_P, ctypes.c_int32, ctypes.c_double, _I]),
+        "get_face_fluxes": (ctypes.c_int, [_P, ctypes.c_int32, _D, _D]),
+        "get_gauss": (ctypes.c_int, [_P, _D, _D]),
+        "flux_points": (ctypes.c_int, [pp, ctypes.c_int64, _D, _D, ctypes.c_double, _D, _D, _D]),
+        "weno5_normal": (ctypes.c_int, [pp, ctypes.c_int64, _D, _D, ctypes.c_double, _D, _D]),
+        "weno5_tangent": (ctypes.c_int, [pp, ctypes.c_int64, _D, _D, ctypes.c_double, _D]),
+    }
+    fns = {}
+    for name, (res, args) in sig.items():
+        f = getattr(lib, prefix + name)
+        f.restype = res
+        f.argtypes = args
+        fns[name] = f
+    return fns
+
+
+class CpuSolver:
+    """One 2D block in the oracle or in the compiled reference."""
+
+    def __init__(self, fns, p):
+        self.f = fns
+        self.p = p
+        self.h = fns["create"](ctypes.byref(p))
+        if not self.h:
+            raise ValueError("unsupported parameters")
+        self.nx, self.ny = p.nx, p.ny
+        self.gp = p.gausspoint
+
+    def close(self):
+        if self.h:
+            self.f["destroy"](self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def set_prim(self, prim):
+        prim = np.ascontiguousarray(prim, dtype=np.float64)
+        assert prim.shape == (4, self.nx, self.ny)
+        return self.f["set_prim"](self.h, dptr(prim))
+
+    def set_convar(self, u):
+        u = np.ascontiguousarray(u, dtype=np.float64)
+        return self.f["set_convar"](self.h, dptr(u))
+
+    def set_alpha(self, ax, ay):
+        ax = np.ascontiguousarray(ax, dtype=np.float64)
+        ay = np.ascontiguousarray(ay, dtype=np.float64)
+        return self.f["set_alpha"](self.h, dptr(ax), dptr(ay))
+
+    def get(self):
+        u = np.zeros((4, self.nx, self.ny))
+        pr = np.zeros((4, self.nx, self.ny))
+        ax = np.zeros((self.nx, self.ny))
+        ay = np.zeros((self.nx, self.ny))
+        t = ctypes.c_double()
+        s = ctypes.c_int32()
+        self.f["get"](self.h, dptr(u), dptr(pr), dptr(ax), dptr(ay), ctypes.byref(t), ctypes.byref(s))
+        return dict(convar=u, primvar=pr, alpha_x=ax, alpha_y=ay, t=t.value, step=s.value)
+
+    def set_time(self, t, step):
+        self.f["set_time"](self.h, t, step)
+
+    def run(self, bc, tstop, max_steps):
+        bca = (ctypes.c_int32 * 4)(*bc)
+        hist = np.zeros(max(1, min(max_steps, 10_000_000)))
+        rc = self.f["run"](self.h, bca, tstop, max_steps, dptr(hist))
+        return rc, hist
+
+    def copy_old(self):
+        self.f["copy_old"](self.h)
+
+    def cfl(self, tstop):
+        d = ctypes.c_double()
+        self.f["cfl"](self.h, tstop, ctypes.byref(d))
+        return d.value
+
+    def stage(self, stage, dt, bc):
+        bca = (ctypes.c_int32 * 4)(*bc)
+        return self.f["stage"](self.h, stage, dt, bca)
+
+    def face_fluxes(self, stage):
+        xf = np.zeros((self.nx + 1, self.ny, self.gp, 8))
+        yf = np.zeros((self.nx, self.ny + 1, self.gp, 8))
+        self.f["get_face_fluxes"](self.h, stage, dptr(xf), dptr(yf))
+        return xf, yf
+
+    def gauss(self):
+        xg = np.zeros((self.nx + 1, self.ny, self.gp, 2, 12))
+        yg = np.zeros((self.nx, self.ny + 1, self.gp, 2, 12))
+        self.f["get_gauss"](self.h, dptr(xg), dptr(yg))
+        return xg, yg
+
+
+class CpuLib:
+    def __init__(self, path, prefix):
+        self.lib = ctypes.CDLL(path)
+        self.fns = _bind(self.lib, prefix)
+
+    def solver(self, p):
+        return CpuSolver(self.fns, p)
+
+    def flux_points(self, p, left, right, dt):
+        left = np.ascontiguousarray(left, dtype=np.float64)
+        right = np.ascontiguousarray(right, dtype=np.float64)
+        n = left.shape[0]
+        c = np.zeros((n, 12))
+        f = np.zeros((n, 4))
+        d = np.zeros((n, 4))
+        self.fns["flux_points"](ctypes.byref(p), n, dptr(left), dptr(right), dt, dptr(c), dptr(f), dptr(d))
+        return c, f, d
+
+    def weno5_normal(self, p, w5, alpha5, h):
+        w5 = np.ascontiguousarray(w5, dtype=np.float64)
+        alpha5 = np.ascontiguousarray(alpha5, dtype=np.float64)
+        n = w5.shape[0]
+        lo = np.zeros((n, 8))
+        ro = np.zeros((n, 8))
+        self.fns["weno5_normal"](ctypes.byref(p), n, dptr(w5), dptr(alpha5), h, dptr(lo), dptr(ro))
+        return lo, ro
+
+    def weno5_tangent(self, p, line5, alpha5, h):
+        line5 = np.ascontiguousarray(line5, dtype=np.float64)
+        alpha5 = np.ascontiguousarray(alpha5, dtype=np.float64)
+        n = line5.shape[0]
+        out = np.zeros((n, 2, 12))
+        self.fns["weno5_tangent"](ctypes.byref(p), n, dptr(line5), dptr(alpha5), h, dptr(out))
+        return out
+
+
+_oracle = None
+_ref = None
+
+
+def oracle():
+    global _oracle
+    if _oracle is None:
+        _oracle = CpuLib(ORACLE_SO, "orc2d_")
+    return _oracle
+
+
+def have_ref():
+    return os.path.exists(REF_SO)
+
+
+def reference():
+    global _ref
+    if _ref is None:
+        _ref = CpuLib(REF_SO, "ref2d_")
+    return _ref
