@@ -1,0 +1,170 @@
+"""Throughput of the GKS hot path on one MI355X.
+
+Workload: the seeded four-quadrant Riemann problem on [0,1]^2 at 2048^2
+(5 ghost layers), WENO5-AO with the DF factor in characteristic variables,
+GKS2D (gks2nd_2d, Euler collision time c1=0.05, c2=1), S2O4, CFL 0.5,
+gamma 1.4, K=3, free boundaries.  A step is one full S2O4 step (two stages)
+of the whole grid; Mcell-updates/s = N^2 * steps / wall seconds / 1e6, with
+the state resident in HBM.
+
+Prints one JSON line (rank 0).  Also reports:
+  roofline      - the dominant kernel's algorithmic FP64 rate (FLOPs per
+                  launch from the oracle's operation count, duration from HIP
+                  events on the solver's stream over the timed steps);
+  cpu_baseline  - the compiled reference (oracle/_ref) or, if it is not
+                  built, the oracle, timed on this host's cores on a bounded
+                  sample of the same workload.
+"""
+import argparse
+import json
+import os
+import subprocess
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "high-order-solver_amd"))
+
+# Algorithmic FP64 operation counts (+ - * / sqrt between runtime values;
+# exp and erfc counted as one each) from the oracle's counting build
+# (oracle/flopcount), WENO5-AO-DF, characteristic variables, 2 Gauss points.
+FLOPS_PER_CELL_UPDATE = 57829.3 + 40.04 + 24.02  # N=1024 grid, one S2O4 step per interior cell
+FLOPS_FLUX_POINT = 3041 + 43 + 5 + 3  # centre + GKS2D + DF alpha_k per (face, Gauss point)
+FLOPS_TANGENT_SIDE = 2857  # tangential WENO per (face, side)
+FLOPS_NORMAL_CELL = 2253  # normal WENO per (cell, direction)
+PEAK_FP64_TFLOPS = 78.6  # MI355X FP64 (vector and matrix)
+PEAK_HBM_GBS = 8000.0
+
+
+def kernel_units(phase, n, gp=2):
+    """Work units one launch of `phase` processes on an n x n grid (G=5)."""
+    G = 5
+    nx = n + 2 * G
+    if phase in ("flux_x", "flux_y"):
+        return gp * (n + 1) * n, FLOPS_FLUX_POINT
+    if phase in ("tangent_x", "tangent_y"):
+        return 2 * (n + 1) * n, FLOPS_TANGENT_SIDE
+    if phase in ("normal_x", "normal_y"):
+        return (n + 2) * nx, FLOPS_NORMAL_CELL
+    return None, None
+
+
+def cpu_baseline(n, threads, sample_steps):
+    """Times the reference (or the oracle) on the host: `sample_steps` S2O4
+    steps of the same workload at n x n, in a child process."""
+    code = f"""
+import sys, time, json
+sys.path.insert(0, {os.path.join(ROOT, 'high-order-solver_amd')!r}); sys.path.insert(0, {os.path.join(ROOT, 'tests')!r})
+from oracle_lib import have_ref, reference, oracle
+from gks_amd.params import planar_shock_params
+from gks_amd.problems import riemann_prim, riemann_states
+kind = 'reference' if have_ref() else 'port'
+lib = reference() if kind == 'reference' else oracle()
+p = planar_shock_params({n})
+prim = riemann_prim(p, riemann_states(0))
+s = lib.solver(p)
+s.set_prim(prim)
+t0 = time.perf_counter()
+rc, _ = s.run([0, 0, 0, 0], 0.25, {sample_steps})
+t1 = time.perf_counter()
+print(json.dumps(dict(kind=kind, seconds=t1 - t0, rc=rc)))
+"""
+    env = dict(os.environ, OMP_NUM_THREADS=str(threads))
+    out = subprocess.run([sys.executable, "-c", code], env=env, capture_output=True, text=True, timeout=900)
+    if out.returncode != 0:
+        return None
+    r = json.loads(out.stdout.strip().splitlines()[-1])
+    if r["rc"] != 0:
+        return None
+    v = n * n * sample_steps / r["seconds"] / 1e6
+    return {"value": round(v, 4), "unit": "Mcell-updates/s", "cores": threads, "kind": r["kind"],
+            "sample": f"{sample_steps} S2O4 step(s) of the same workload at {n}x{n}, "
+                      f"{r['seconds']:.1f} s wall, OMP_NUM_THREADS={threads}"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=30)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--n", type=int, default=2048)
+    ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--variant", choices=["ieee", "contracted"], default="ieee")
+    ap.add_argument("--cpu-n", type=int, default=1024)
+    ap.add_argument("--cpu-steps", type=int, default=1)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus > 1 or world > 1:
+        print("bench.py: this build runs on one GPU; multi-GPU runs are not part of it "
+              f"(asked for --gpus {args.gpus}, WORLD_SIZE {world})", file=sys.stderr)
+        sys.exit(3)
+
+    import numpy as np  # noqa: F401
+
+    import gks_amd
+    from gks_amd.params import BC_FREE, planar_shock_params
+    from gks_amd.problems import riemann_prim, riemann_states
+
+    n = args.n
+    p = planar_shock_params(n, fast_math=1 if args.variant == "contracted" else 0)
+    prim = riemann_prim(p, riemann_states(args.seed))
+    s = gks_amd.Solver2D(p)
+    s.init_prim(prim)
+    bc = [BC_FREE] * 4
+    s.run(bc, 0.25, args.warmup)
+    s.sync()
+    s.timing(True)
+    t0 = time.perf_counter()
+    k = s.run(bc, 0.25, args.steps)
+    s.sync()
+    t1 = time.perf_counter()
+    tim = s.get_timing()
+    assert k == args.steps, (k, args.steps)
+    wall = t1 - t0
+    value = n * n * k / wall / 1e6
+
+    # dominant kernel
+    dom = max(tim, key=lambda ph: tim[ph][0])
+    units, fpu = kernel_units(dom, n)
+    avg_ms = tim[dom][0] / max(tim[dom][1], 1)
+    roof = None
+    if units:
+        achieved = units * fpu / (avg_ms * 1e-3) / 1e12
+        roof = {"bound": "mfma", "achieved": round(achieved, 3), "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
+                "frac": round(achieved / PEAK_FP64_TFLOPS, 4), "traffic": None, "kernel": dom,
+                "avg_ms": round(avg_ms, 4), "flops_per_launch": units * fpu}
+    total_dev_ms = sum(v[0] for v in tim.values())
+    step_flops = FLOPS_PER_CELL_UPDATE * n * n
+    out = {
+        "metric": "Mcell-updates/s FP64, 2D 2048² WENO5-AO GKS: learned vs plain at equal L1",
+        "value": round(value, 3),
+        "unit": "Mcell-updates/s",
+        "n_gpus": 1,
+        "steps": k,
+        "warmup": args.warmup,
+        "ms_per_step": round(wall / k * 1e3, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"four-quadrant Riemann {n}x{n} seed {args.seed}, WENO5-AO-DF characteristic, "
+                               f"GKS2D gks2nd, S2O4, CFL 0.5, free BCs (plain path, {args.variant} kernels)",
+                   "n": n, "path": "plain", "kernels": args.variant},
+        "roofline": roof,
+        "step_fp64_tflops": round(step_flops * k / wall / 1e12, 3),
+        "phase_ms_per_step": {ph: round(v[0] / k, 4) for ph, v in tim.items()},
+        "device_ms_per_step": round(total_dev_ms / k, 3),
+    }
+    if not args.no_cpu_baseline:
+        threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or (os.cpu_count() or 1)
+        out["cpu_baseline"] = cpu_baseline(args.cpu_n, threads, args.cpu_steps)
+    else:
+        out["cpu_baseline"] = None
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

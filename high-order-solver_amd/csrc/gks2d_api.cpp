@@ -1,0 +1,495 @@
+// gks2d_api.cpp — the extern "C" layer of include/gks2d.h over the HIP
+// kernels.  One context = one block = one HIP stream; a context is not
+// re-entrant (the reference is single-threaded at this level too).
+#include <hip/hip_runtime.h>
+
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/gks2d.h"
+#include "gks2d_launch.hpp"
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const char* what) {
+  g_err = what;
+  return code;
+}
+int hip_fail(hipError_t e, const char* where) {
+  g_err = std::string(where) + ": " + hipGetErrorString(e);
+  return GKS_ERR_HIP;
+}
+#define HIPCHK(call)                                   \
+  do {                                                 \
+    hipError_t e_ = (call);                            \
+    if (e_ != hipSuccess) return hip_fail(e_, #call);  \
+  } while (0)
+
+gksd::Gas gas_of(const gks2d_params& p) {
+  gksd::Gas g;
+  g.K = p.K;
+  g.gamma = p.gamma;
+  g.mu = p.mu;
+  g.nu = p.nu;
+  g.pr = p.pr;
+  g.c1 = p.c1_euler;
+  g.c2 = p.c2_euler;
+  g.tau_type = p.tau_type;
+  g.prandtl_fix = p.prandtl_fix;
+  return g;
+}
+
+int check_params(const gks2d_params* p) {
+  if (!p) return fail(GKS_ERR_ARG, "null params");
+  if (p->ghost != 5) return fail(GKS_ERR_UNSUPPORTED, "ghost must be 5");
+  if (p->nodex < 5 || p->nodey < 5) return fail(GKS_ERR_ARG, "nodex and nodey must be >= 5");
+  if (p->weno_order != 5 || p->gausspoint != 2)
+    return fail(GKS_ERR_UNSUPPORTED, "implemented: WENO5-AO with 2 Gauss points");
+  if (p->use_df != 1) return fail(GKS_ERR_UNSUPPORTED, "implemented: the DF (discontinuity feedback) variants");
+  if (p->recon_variable != GKS_VAR_CONSERVATIVE && p->recon_variable != GKS_VAR_CHARACTERISTIC)
+    return fail(GKS_ERR_ARG, "recon_variable");
+  if (p->right <= p->left || p->up <= p->down) return fail(GKS_ERR_ARG, "empty domain");
+  if (!(p->gamma > 1.0) || p->K < 0) return fail(GKS_ERR_ARG, "gamma / K");
+  return GKS_OK;
+}
+
+}  // namespace
+
+struct gks2d_ctx {
+  gks2d_params p;
+  Dev d;
+  const LaunchTable* lt = nullptr;
+  hipStream_t s = nullptr;
+  hipEvent_t ev = nullptr;
+  double* h_stat = nullptr;  // pinned copy of d.stat
+  double* d_prim = nullptr;  // staging for gks2d_set_cfl_primvar
+  std::vector<void*> allocs;
+  int last_slot = -1;        // slot of the last primvar / set_cfl_primvar
+  int last_flux_stage = -1;
+  // per-phase HIP-event timing (gks2d_timing)
+  bool timing = false;
+  struct Rec {
+    int phase;
+    hipEvent_t a, b;
+  };
+  std::vector<Rec> pending;
+  std::vector<hipEvent_t> pool;
+  size_t pool_used = 0;
+  double t_ms[PH_COUNT] = {};
+  long long t_n[PH_COUNT] = {};
+};
+
+static hipEvent_t pool_event(gks2d_ctx* c) {
+  if (c->pool_used == c->pool.size()) {
+    hipEvent_t e = nullptr;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    c->pool.push_back(e);
+  }
+  return c->pool[c->pool_used++];
+}
+// launches one kernel phase; with timing on, brackets it with events on the
+// context's stream
+template <class F> static int timed(gks2d_ctx* c, int phase, F&& launch) {
+  hipEvent_t a = nullptr, b = nullptr;
+  if (c->timing) {
+    a = pool_event(c);
+    b = pool_event(c);
+    if (!a || !b) return fail(GKS_ERR_HIP, "hipEventCreate");
+    HIPCHK(hipEventRecord(a, c->s));
+  }
+  HIPCHK(launch());
+  if (c->timing) {
+    HIPCHK(hipEventRecord(b, c->s));
+    c->pending.push_back({phase, a, b});
+  }
+  return GKS_OK;
+}
+// folds finished event pairs into the accumulators (stream must be idle)
+static int collect_timing(gks2d_ctx* c) {
+  for (auto& r : c->pending) {
+    float ms = 0.f;
+    HIPCHK(hipEventElapsedTime(&ms, r.a, r.b));
+    c->t_ms[r.phase] += ms;
+    c->t_n[r.phase]++;
+  }
+  c->pending.clear();
+  c->pool_used = 0;
+  return GKS_OK;
+}
+#define RC(x)              \
+  do {                     \
+    int rc_ = (x);         \
+    if (rc_) return rc_;   \
+  } while (0)
+
+static int dev_alloc(gks2d_ctx* c, double** out, size_t n) {
+  void* ptr = nullptr;
+  hipError_t e = hipMalloc(&ptr, n * sizeof(double));
+  if (e != hipSuccess) return hip_fail(e, "hipMalloc");
+  c->allocs.push_back(ptr);
+  e = hipMemset(ptr, 0, n * sizeof(double));
+  if (e != hipSuccess) return hip_fail(e, "hipMemset");
+  *out = static_cast<double*>(ptr);
+  return GKS_OK;
+}
+
+extern "C" {
+
+const char* gks2d_last_error(void) { return g_err.c_str(); }
+
+int gks2d_destroy(gks2d_ctx* c) {
+  if (!c) return GKS_OK;
+  if (c->s) (void)hipStreamSynchronize(c->s);
+  for (void* a : c->allocs) (void)hipFree(a);
+  for (hipEvent_t e : c->pool) (void)hipEventDestroy(e);
+  if (c->h_stat) (void)hipHostFree(c->h_stat);
+  if (c->ev) (void)hipEventDestroy(c->ev);
+  if (c->s) (void)hipStreamDestroy(c->s);
+  delete c;
+  return GKS_OK;
+}
+
+int gks2d_create(const gks2d_params* p, gks2d_ctx** out) {
+  if (!out) return fail(GKS_ERR_ARG, "null out");
+  *out = nullptr;
+  int rc = check_params(p);
+  if (rc) return rc;
+  gks2d_ctx* c = new gks2d_ctx();
+  c->p = *p;
+  Dev& d = c->d;
+  std::memset(&d, 0, sizeof(d));
+  d.NX = p->nodex;
+  d.NY = p->nodey;
+  d.G = p->ghost;
+  d.nx = d.NX + 2 * d.G;
+  d.ny = d.NY + 2 * d.G;
+  d.gp = p->gausspoint;
+  d.recon_var = p->recon_variable;
+  d.use_df = p->use_df;
+  d.dx = (p->right - p->left) / p->nodex;  // SetUniformMesh, basic_function.cpp:1411-1416
+  d.dy = (p->up - p->down) / p->nodey;
+  d.area = d.dx * d.dy;
+  d.gw[0] = 0.5;  // SetGuassPoint, fluid_mesh.cpp:19-23
+  d.gw[1] = 0.5;
+  d.gas = gas_of(*p);
+  d.nc = (long long)d.nx * d.ny;
+  d.nfx = (long long)(d.nx + 1) * d.ny;
+  d.nfy = (long long)d.nx * (d.ny + 1);
+  c->lt = p->fast_math ? &contracted::table : &ieee::table;
+  const long long nfm = d.nfx > d.nfy ? d.nfx : d.nfy;
+  (void)nfm;
+  hipError_t e = hipStreamCreateWithFlags(&c->s, hipStreamNonBlocking);
+  if (e != hipSuccess) {
+    gks2d_destroy(c);
+    return hip_fail(e, "hipStreamCreate");
+  }
+  e = hipEventCreateWithFlags(&c->ev, hipEventDisableTiming);
+  if (e == hipSuccess) e = hipHostMalloc((void**)&c->h_stat, 8 * sizeof(double));
+  if (e != hipSuccess) {
+    gks2d_destroy(c);
+    return hip_fail(e, "hipEventCreate/hipHostMalloc");
+  }
+  std::memset(c->h_stat, 0, 8 * sizeof(double));
+  const int gp = d.gp;
+  if ((rc = dev_alloc(c, &d.U, 4 * d.nc)) || (rc = dev_alloc(c, &d.Uold, 4 * d.nc)) ||
+      (rc = dev_alloc(c, &d.ax, d.nc)) || (rc = dev_alloc(c, &d.ay, d.nc)) ||
+      (rc = dev_alloc(c, &d.line_x, 16 * d.nfx)) || (rc = dev_alloc(c, &d.line_y, 16 * d.nfy)) ||
+      (rc = dev_alloc(c, &d.gst_x, 24 * gp * d.nfx)) || (rc = dev_alloc(c, &d.gst_y, 24 * gp * d.nfy)) ||
+      (rc = dev_alloc(c, &d.flx[0], 8 * gp * d.nfx)) || (rc = dev_alloc(c, &d.flx[1], 8 * gp * d.nfx)) ||
+      (rc = dev_alloc(c, &d.fly[0], 8 * gp * d.nfy)) || (rc = dev_alloc(c, &d.fly[1], 8 * gp * d.nfy)) ||
+      (rc = dev_alloc(c, &d.akx, gp * d.nfx)) || (rc = dev_alloc(c, &d.aky, gp * d.nfy)) ||
+      (rc = dev_alloc(c, &d.partial, 2 * 1024)) || (rc = dev_alloc(c, &d.stat, 8)) ||
+      (rc = dev_alloc(c, &c->d_prim, 4 * d.nc))) {
+    gks2d_destroy(c);
+    return rc;
+  }
+  *out = c;
+  return GKS_OK;
+}
+
+// ------------------------------------------------------------ host copies
+static void gather(std::vector<double>& soa, const double* base, long long nc, int nvar, int64_t cs, int64_t vs) {
+  const char* b = reinterpret_cast<const char*>(base);
+  for (long long k = 0; k < nc; k++)
+    for (int v = 0; v < nvar; v++) soa[v * nc + k] = *reinterpret_cast<const double*>(b + k * cs + v * vs);
+}
+static void scatter(const std::vector<double>& soa, double* base, long long nc, int nvar, int64_t cs, int64_t vs) {
+  char* b = reinterpret_cast<char*>(base);
+  for (long long k = 0; k < nc; k++)
+    for (int v = 0; v < nvar; v++) *reinterpret_cast<double*>(b + k * cs + v * vs) = soa[v * nc + k];
+}
+
+int gks2d_set_convar(gks2d_ctx* c, const double* base, int64_t cs, int64_t vs) {
+  if (!c || !base) return fail(GKS_ERR_ARG, "null");
+  std::vector<double> h(4 * c->d.nc);
+  gather(h, base, c->d.nc, 4, cs, vs);
+  HIPCHK(hipMemcpyAsync(c->d.U, h.data(), h.size() * 8, hipMemcpyHostToDevice, c->s));
+  HIPCHK(hipStreamSynchronize(c->s));
+  return GKS_OK;
+}
+int gks2d_get_convar(gks2d_ctx* c, double* base, int64_t cs, int64_t vs) {
+  if (!c || !base) return fail(GKS_ERR_ARG, "null");
+  std::vector<double> h(4 * c->d.nc);
+  HIPCHK(hipMemcpyAsync(h.data(), c->d.U, h.size() * 8, hipMemcpyDeviceToHost, c->s));
+  HIPCHK(hipStreamSynchronize(c->s));
+  scatter(h, base, c->d.nc, 4, cs, vs);
+  return GKS_OK;
+}
+int gks2d_set_alpha(gks2d_ctx* c, const double* ax, const double* ay, int64_t cs) {
+  if (!c || !ax || !ay) return fail(GKS_ERR_ARG, "null");
+  std::vector<double> h(c->d.nc);
+  gather(h, ax, c->d.nc, 1, cs, 0);
+  HIPCHK(hipMemcpyAsync(c->d.ax, h.data(), h.size() * 8, hipMemcpyHostToDevice, c->s));
+  HIPCHK(hipStreamSynchronize(c->s));
+  gather(h, ay, c->d.nc, 1, cs, 0);
+  HIPCHK(hipMemcpyAsync(c->d.ay, h.data(), h.size() * 8, hipMemcpyHostToDevice, c->s));
+  HIPCHK(hipStreamSynchronize(c->s));
+  return GKS_OK;
+}
+int gks2d_get_alpha(gks2d_ctx* c, double* ax, double* ay, int64_t cs) {
+  if (!c || !ax || !ay) return fail(GKS_ERR_ARG, "null");
+  std::vector<double> h(c->d.nc);
+  HIPCHK(hipMemcpyAsync(h.data(), c->d.ax, h.size() * 8, hipMemcpyDeviceToHost, c->s));
+  HIPCHK(hipStreamSynchronize(c->s));
+  scatter(h, ax, c->d.nc, 1, cs, 0);
+  HIPCHK(hipMemcpyAsync(h.data(), c->d.ay, h.size() * 8, hipMemcpyDeviceToHost, c->s));
+  HIPCHK(hipStreamSynchronize(c->s));
+  scatter(h, ay, c->d.nc, 1, cs, 0);
+  return GKS_OK;
+}
+int gks2d_set_cfl_primvar(gks2d_ctx* c, const double* base, int64_t cs, int64_t vs) {
+  if (!c || !base) return fail(GKS_ERR_ARG, "null");
+  std::vector<double> h(4 * c->d.nc);
+  gather(h, base, c->d.nc, 4, cs, vs);
+  HIPCHK(hipMemcpyAsync(c->d_prim, h.data(), h.size() * 8, hipMemcpyHostToDevice, c->s));
+  HIPCHK(c->lt->primvar(c->d, c->d_prim, 0, c->s));
+  HIPCHK(hipStreamSynchronize(c->s));
+  c->last_slot = 0;
+  return GKS_OK;
+}
+
+// ------------------------------------------------------------ sweeps
+int gks2d_copy_old(gks2d_ctx* c) {
+  if (!c) return fail(GKS_ERR_ARG, "null");
+  return timed(c, PH_COPY_OLD, [&] { return c->lt->copy_old(c->d, c->s); });
+}
+int gks2d_boundary(gks2d_ctx* c, int side, int kind) {
+  if (!c) return fail(GKS_ERR_ARG, "null");
+  if (side < 0 || side > 3 || kind < 0 || kind > 2) return fail(GKS_ERR_ARG, "side/kind");
+  return timed(c, PH_BC, [&] { return c->lt->bc(c->d, side, kind, c->s); });
+}
+int gks2d_primvar(gks2d_ctx* c) {
+  if (!c) return fail(GKS_ERR_ARG, "null");
+  RC(timed(c, PH_PRIMVAR, [&] { return c->lt->primvar(c->d, nullptr, 0, c->s); }));
+  c->last_slot = 0;
+  return GKS_OK;
+}
+int gks2d_reconstruct(gks2d_ctx* c) {
+  if (!c) return fail(GKS_ERR_ARG, "null");
+  RC(timed(c, PH_NORMAL_X, [&] { return c->lt->normal(c->d, 0, c->s); }));
+  RC(timed(c, PH_NORMAL_Y, [&] { return c->lt->normal(c->d, 1, c->s); }));
+  RC(timed(c, PH_TANGENT_X, [&] { return c->lt->tangent(c->d, 0, c->s); }));
+  return timed(c, PH_TANGENT_Y, [&] { return c->lt->tangent(c->d, 1, c->s); });
+}
+int gks2d_flux(gks2d_ctx* c, int stage, double dt) {
+  if (!c) return fail(GKS_ERR_ARG, "null");
+  if (stage < 0 || stage > 1) return fail(GKS_ERR_ARG, "stage");
+  RC(timed(c, PH_FLUX_X, [&] { return c->lt->flux(c->d, 0, stage, dt, c->s); }));
+  RC(timed(c, PH_FLUX_Y, [&] { return c->lt->flux(c->d, 1, stage, dt, c->s); }));
+  c->last_flux_stage = stage;
+  return GKS_OK;
+}
+int gks2d_update(gks2d_ctx* c, int stage) {
+  if (!c) return fail(GKS_ERR_ARG, "null");
+  if (stage < 0 || stage > 1) return fail(GKS_ERR_ARG, "stage");
+  return timed(c, PH_UPDATE, [&] { return c->lt->update(c->d, stage, c->s); });
+}
+int gks2d_update_alpha(gks2d_ctx* c) {
+  if (!c) return fail(GKS_ERR_ARG, "null");
+  return timed(c, PH_ALPHA, [&] { return c->lt->alpha(c->d, c->s); });
+}
+int gks2d_timing(gks2d_ctx* c, int enable) {
+  if (!c) return fail(GKS_ERR_ARG, "null");
+  HIPCHK(hipStreamSynchronize(c->s));
+  RC(collect_timing(c));
+  for (int k = 0; k < PH_COUNT; k++) {
+    c->t_ms[k] = 0.0;
+    c->t_n[k] = 0;
+  }
+  c->timing = enable != 0;
+  return GKS_OK;
+}
+int gks2d_get_timing(gks2d_ctx* c, double* ms, int64_t* launches) {
+  if (!c || !ms || !launches) return fail(GKS_ERR_ARG, "null");
+  HIPCHK(hipStreamSynchronize(c->s));
+  RC(collect_timing(c));
+  for (int k = 0; k < PH_COUNT; k++) {
+    ms[k] = c->t_ms[k];
+    launches[k] = c->t_n[k];
+  }
+  return GKS_OK;
+}
+int gks2d_sync(gks2d_ctx* c) {
+  if (!c) return fail(GKS_ERR_ARG, "null");
+  HIPCHK(hipStreamSynchronize(c->s));
+  return GKS_OK;
+}
+
+// Get_CFL / Dtx (time_advance.cpp:264-305) from the max speed over the
+// interior.  Equals the reference's sequential scan except when two cells'
+// speeds differ by one ulp (the scan may then keep the earlier one).
+static double dt_from_speed(const gks2d_ctx* c, double tmax, double t, double tstop) {
+  const gks2d_params& p = c->p;
+  double dt = c->d.dx;
+  const double hs[2] = {c->d.dx, c->d.dy};
+  for (double h : hs) {
+    if (tmax > p.cfl * h / dt) dt = p.cfl * h / tmax;
+    if (dt > 0.25 * p.cfl * h * h / p.mu && p.tau_type == GKS_TAU_NS && p.mu > 0) dt = 0.25 * p.cfl * h * h / p.mu;
+  }
+  if (t + dt > tstop) dt = tstop - t + 1e-16;
+  return dt;
+}
+
+int gks2d_cfl(gks2d_ctx* c, double t, double tstop, double* dt_out) {
+  if (!c || !dt_out) return fail(GKS_ERR_ARG, "null");
+  if (c->last_slot < 0) return fail(GKS_ERR_STATE, "no primitive state yet: call gks2d_set_cfl_primvar first");
+  HIPCHK(hipMemcpyAsync(c->h_stat, c->d.stat, 8 * sizeof(double), hipMemcpyDeviceToHost, c->s));
+  HIPCHK(hipStreamSynchronize(c->s));
+  if (c->h_stat[4] != 0.0) {
+    const double zero = 0.0;
+    HIPCHK(hipMemcpyAsync(c->d.stat + 4, &zero, sizeof(double), hipMemcpyHostToDevice, c->s));
+    HIPCHK(hipStreamSynchronize(c->s));
+    return fail(GKS_ERR_BLOWUP, "density or pressure negative or NaN");
+  }
+  *dt_out = dt_from_speed(c, c->h_stat[2 * c->last_slot], t, tstop);
+  return GKS_OK;
+}
+
+// one stage of the driver loop (2Dproblem.cpp:137-158)
+static int stage_launch(gks2d_ctx* c, int stage, double dt, const int32_t* bc, bool readback) {
+  for (int s = 0; s < 4; s++) RC(timed(c, PH_BC, [&] { return c->lt->bc(c->d, s, bc[s], c->s); }));
+  RC(timed(c, PH_PRIMVAR, [&] { return c->lt->primvar(c->d, nullptr, stage, c->s); }));
+  if (readback) {
+    HIPCHK(hipMemcpyAsync(c->h_stat, c->d.stat, 8 * sizeof(double), hipMemcpyDeviceToHost, c->s));
+    HIPCHK(hipEventRecord(c->ev, c->s));
+  }
+  RC(gks2d_reconstruct(c));
+  RC(gks2d_flux(c, stage, dt));
+  RC(gks2d_update(c, stage));
+  if (c->p.use_df) RC(gks2d_update_alpha(c));
+  return GKS_OK;
+}
+
+int gks2d_step(gks2d_ctx* c, double dt, const int32_t bc[4]) {
+  if (!c || !bc) return fail(GKS_ERR_ARG, "null");
+  for (int s = 0; s < 4; s++)
+    if (bc[s] < 0 || bc[s] > 2) return fail(GKS_ERR_ARG, "bc kind");
+  RC(gks2d_copy_old(c));
+  RC(stage_launch(c, 0, dt, bc, false));
+  RC(stage_launch(c, 1, dt, bc, false));
+  c->last_slot = 1;
+  c->last_flux_stage = 1;
+  return GKS_OK;
+}
+
+int gks2d_run(gks2d_ctx* c, const int32_t bc[4], double tstop, int32_t max_steps, double* t_inout,
+              int32_t* step_inout) {
+  if (!c || !bc || !t_inout || !step_inout) return fail(GKS_ERR_ARG, "null");
+  if (c->last_slot < 0) return fail(GKS_ERR_STATE, "no primitive state yet: call gks2d_set_cfl_primvar first");
+  for (int s = 0; s < 4; s++)
+    if (bc[s] < 0 || bc[s] > 2) return fail(GKS_ERR_ARG, "bc kind");
+  double t = *t_inout;
+  int32_t step = *step_inout;
+  // speed for the first dt: whatever the last primvar produced
+  HIPCHK(hipMemcpyAsync(c->h_stat, c->d.stat, 8 * sizeof(double), hipMemcpyDeviceToHost, c->s));
+  HIPCHK(hipStreamSynchronize(c->s));
+  double tmax = c->h_stat[2 * c->last_slot];
+  int n = 0;
+  while (t < tstop && n < max_steps) {
+    if (c->h_stat[4] != 0.0) break;
+    const double dt = dt_from_speed(c, tmax, t, tstop);
+    RC(gks2d_copy_old(c));
+    RC(stage_launch(c, 0, dt, bc, false));
+    // the stage-1 speed is the next step's (stale) CFL input: read it back
+    // while the rest of stage 1 runs
+    RC(stage_launch(c, 1, dt, bc, true));
+    c->last_slot = 1;
+    c->last_flux_stage = 1;
+    HIPCHK(hipEventSynchronize(c->ev));
+    if (c->timing) {
+      HIPCHK(hipStreamSynchronize(c->s));
+      RC(collect_timing(c));
+    }
+    if (c->h_stat[4] != 0.0) break;
+    tmax = c->h_stat[2];
+    t = t + dt;
+    step++;
+    n++;
+  }
+  *t_inout = t;
+  *step_inout = step;
+  if (c->h_stat[4] != 0.0) {
+    HIPCHK(hipStreamSynchronize(c->s));
+    const double zero = 0.0;
+    HIPCHK(hipMemcpyAsync(c->d.stat + 4, &zero, sizeof(double), hipMemcpyHostToDevice, c->s));
+    HIPCHK(hipStreamSynchronize(c->s));
+    c->h_stat[4] = 0.0;
+    return fail(GKS_ERR_BLOWUP, "density or pressure negative or NaN");
+  }
+  return GKS_OK;
+}
+
+int gks2d_get_face_fluxes(gks2d_ctx* c, double* xf, double* yf) {
+  if (!c || !xf || !yf) return fail(GKS_ERR_ARG, "null");
+  if (c->last_flux_stage < 0) return fail(GKS_ERR_STATE, "no flux computed yet");
+  const Dev& d = c->d;
+  const int st = c->last_flux_stage, gp = d.gp;
+  std::vector<double> h(8 * gp * d.nfx);
+  HIPCHK(hipMemcpyAsync(h.data(), d.flx[st], h.size() * 8, hipMemcpyDeviceToHost, c->s));
+  HIPCHK(hipStreamSynchronize(c->s));
+  for (long long f = 0; f < d.nfx; f++)
+    for (int q = 0; q < gp; q++)
+      for (int k = 0; k < 8; k++) xf[(f * gp + q) * 8 + k] = h[(q * 8 + k) * d.nfx + f];
+  h.assign(8 * gp * d.nfy, 0.0);
+  HIPCHK(hipMemcpyAsync(h.data(), d.fly[st], h.size() * 8, hipMemcpyDeviceToHost, c->s));
+  HIPCHK(hipStreamSynchronize(c->s));
+  for (long long f = 0; f < d.nfy; f++)
+    for (int q = 0; q < gp; q++)
+      for (int k = 0; k < 8; k++) yf[(f * gp + q) * 8 + k] = h[(q * 8 + k) * d.nfy + f];
+  return GKS_OK;
+}
+
+int gks2d_flux_points(const gks2d_params* p, int64_t n, const double* left, const double* right, double dt,
+                      double* center, double* f, double* derf) {
+  if (!p || !left || !right || !f || !derf) return fail(GKS_ERR_ARG, "null");
+  if (n < 0) return fail(GKS_ERR_ARG, "n < 0");
+  if (n == 0) return GKS_OK;
+  const LaunchTable* lt = p->fast_math ? &contracted::table : &ieee::table;
+  double *dl = nullptr, *dr = nullptr, *dc = nullptr, *dfo = nullptr, *dd = nullptr;
+  hipError_t e = hipMalloc(&dl, n * 12 * 8);
+  if (e == hipSuccess) e = hipMalloc(&dr, n * 12 * 8);
+  if (e == hipSuccess) e = hipMalloc(&dc, n * 12 * 8);
+  if (e == hipSuccess) e = hipMalloc(&dfo, n * 4 * 8);
+  if (e == hipSuccess) e = hipMalloc(&dd, n * 4 * 8);
+  if (e == hipSuccess) e = hipMemcpy(dl, left, n * 12 * 8, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(dr, right, n * 12 * 8, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = lt->flux_points(gas_of(*p), n, dl, dr, dt, dc, dfo, dd, nullptr);
+  if (e == hipSuccess) e = hipDeviceSynchronize();
+  if (e == hipSuccess && center) e = hipMemcpy(center, dc, n * 12 * 8, hipMemcpyDeviceToHost);
+  if (e == hipSuccess) e = hipMemcpy(f, dfo, n * 4 * 8, hipMemcpyDeviceToHost);
+  if (e == hipSuccess) e = hipMemcpy(derf, dd, n * 4 * 8, hipMemcpyDeviceToHost);
+  (void)hipFree(dl);
+  (void)hipFree(dr);
+  (void)hipFree(dc);
+  (void)hipFree(dfo);
+  (void)hipFree(dd);
+  if (e != hipSuccess) return hip_fail(e, "gks2d_flux_points");
+  return GKS_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,523 @@
+// gks2d_device.hpp — device numerics of the 2D GKS hot path (gfx950, FP64).
+//
+// Each function restates one reference leaf (cited file:line in the
+// reference repository) with the reference's floating-point operation order,
+// so that the IEEE build (-ffp-contract=off) differs from the reference only
+// through the device erfc/exp/sqrt.  The Maxwellian moment tables of the
+// reference (MMDF2d, 3 x 7x7x3 doubles) are not materialised: an entry
+// (i,j,k) is the product uplus[i]*vwhole[j]*xi_k formed where it is used,
+// which gives the same value the table holds and keeps the working set in
+// registers (4 length-7 vectors per Maxwellian).
+#pragma once
+#include <hip/hip_runtime.h>
+
+namespace gksd {
+
+struct Gas {
+  int K;
+  double gamma, mu, nu, pr, c1, c2;
+  int tau_type;
+  int prandtl_fix;
+};
+
+#define GKS_DI __device__ __forceinline__
+
+// ---------------------------------------------------------------- gas algebra
+// Convar_to_primvar_2D: basic_function.cpp:811-817 (+ U/V/Pressure :1137-1160)
+GKS_DI void convar_to_primvar(double* p, const double* c, const Gas& g) {
+  p[0] = c[0];
+  p[1] = c[1] / c[0];
+  p[2] = c[2] / c[0];
+  p[3] = (g.gamma - 1.0) * (c[3] - 0.5 * c[1] * c[1] / c[0] - 0.5 * c[2] * c[2] / c[0]);
+}
+// Lambda (2D): basic_function.cpp:1174-1177
+GKS_DI double lambda2(double rho, double u, double v, double rhoE, const Gas& g) {
+  return (g.K + 2.0) * 0.25 * (rho / (rhoE - 0.5 * rho * (u * u + v * v)));
+}
+// Convar_to_ULambda_2d: basic_function.cpp:835-842
+GKS_DI void convar_to_ulambda(double* p, const double* c, const Gas& g) {
+  p[0] = c[0];
+  p[1] = c[1] / c[0];
+  p[2] = c[2] / c[0];
+  p[3] = lambda2(c[0], p[1], p[2], c[3], g);
+}
+
+// Convar_to_char / Char_to_convar (2D): basic_function.cpp:923-992.  The
+// 4x4 projections are kept as written (zero and unit entries included) so
+// the IEEE build sums in the reference's order.
+struct CharBase {
+  double u, v, c, overc, alfa, q2, h;
+};
+GKS_DI void char_base(CharBase& b, const double* prim, const Gas& g) {
+  b.c = sqrt(g.gamma * prim[3] / prim[0]);
+  b.overc = 1.0 / b.c;
+  b.alfa = (g.gamma - 1.0) * 0.5 * b.overc * b.overc;
+  b.u = prim[1];
+  b.v = prim[2];
+  b.q2 = b.u * b.u + b.v * b.v;
+}
+GKS_DI void to_char(double* ch, const CharBase& b, const double* c) {
+  const double u = b.u, v = b.v, alfa = b.alfa, overc = b.overc;
+  double s0[4] = {1.0 - alfa * b.q2, 2.0 * alfa * u, 2.0 * alfa * v, -2.0 * alfa};
+  double s2[4] = {0.5 * (-u * overc + alfa * b.q2), 0.5 * overc - alfa * u, -alfa * v, alfa};
+  double s3[4] = {0.5 * (u * overc + alfa * b.q2), -0.5 * overc - alfa * u, -alfa * v, alfa};
+  double a;
+  a = 0.0;
+#pragma unroll
+  for (int j = 0; j < 4; j++) a = a + s0[j] * c[j];
+  ch[0] = a;
+  a = 0.0;
+  a = a + (-v) * c[0];
+  a = a + 0. * c[1];
+  a = a + 1. * c[2];
+  a = a + 0. * c[3];
+  ch[1] = a;
+  a = 0.0;
+#pragma unroll
+  for (int j = 0; j < 4; j++) a = a + s2[j] * c[j];
+  ch[2] = a;
+  a = 0.0;
+#pragma unroll
+  for (int j = 0; j < 4; j++) a = a + s3[j] * c[j];
+  ch[3] = a;
+}
+GKS_DI void from_char(double* c, const CharBase& b, const double* ch, const Gas& g) {
+  const double u = b.u, v = b.v, cs = b.c;
+  const double hq = 0.5 * (u * u + v * v);
+  const double s0[4] = {1., 0.0, 1.0, 1.0};
+  const double s1[4] = {u, 0., u + cs, u - cs};
+  const double s2[4] = {v, 1., v, v};
+  const double s3[4] = {hq, v, hq + cs * u + cs * cs / (g.gamma - 1.0), hq - cs * u + cs * cs / (g.gamma - 1.0)};
+  double a;
+  a = 0.0;
+#pragma unroll
+  for (int j = 0; j < 4; j++) a = a + s0[j] * ch[j];
+  c[0] = a;
+  a = 0.0;
+#pragma unroll
+  for (int j = 0; j < 4; j++) a = a + s1[j] * ch[j];
+  c[1] = a;
+  a = 0.0;
+#pragma unroll
+  for (int j = 0; j < 4; j++) a = a + s2[j] * ch[j];
+  c[2] = a;
+  a = 0.0;
+#pragma unroll
+  for (int j = 0; j < 4; j++) a = a + s3[j] * ch[j];
+  c[3] = a;
+}
+
+// --------------------------------------------------------- Maxwellian moments
+// MMDF2d::calcualte_MMDF: basic_function.cpp:110-187
+struct MM {
+  double up[7], un[7], vw[7], xi2, xi4;
+};
+GKS_DI void mmdf(MM& m, double u, double v, double lambda, const Gas& g) {
+  const double pi = 3.14159265358979323846;
+  double uw[7];
+  uw[0] = 1.0;
+  uw[1] = u;
+  m.up[0] = 0.5 * erfc(sqrt(lambda) * (-u));
+  m.un[0] = 1.0 - m.up[0];
+  m.up[1] = u * m.up[0] + 0.5 * (exp(-lambda * u * u) / sqrt(pi * lambda));
+  m.un[1] = u - m.up[1];
+  const double ol = 1.0 / lambda;
+#pragma unroll
+  for (int i = 2; i <= 6; i++) uw[i] = u * uw[i - 1] + 0.5 * (i - 1) * ol * uw[i - 2];
+#pragma unroll
+  for (int i = 2; i <= 6; i++) {
+    m.up[i] = u * m.up[i - 1] + 0.5 * (i - 1) * ol * m.up[i - 2];
+    m.un[i] = uw[i] - m.up[i];
+  }
+  m.vw[0] = 1.0;
+  m.vw[1] = v;
+#pragma unroll
+  for (int i = 2; i <= 6; i++) m.vw[i] = v * m.vw[i - 1] + 0.5 * (i - 1) * ol * m.vw[i - 2];
+  m.xi2 = 0.5 * g.K * ol;
+  m.xi4 = 0.25 * (g.K * g.K + 2 * g.K) * ol * ol;
+}
+// table entries upvxi / unvxi / uvxi [i][j][k]
+struct TabP {
+  const MM& m;
+  GKS_DI double operator()(int i, int j, int k) const {
+    return k == 0 ? m.up[i] * m.vw[j] : (k == 1 ? m.up[i] * m.vw[j] * m.xi2 : m.up[i] * m.vw[j] * m.xi4);
+  }
+};
+struct TabN {
+  const MM& m;
+  GKS_DI double operator()(int i, int j, int k) const {
+    return k == 0 ? m.un[i] * m.vw[j] : (k == 1 ? m.un[i] * m.vw[j] * m.xi2 : m.un[i] * m.vw[j] * m.xi4);
+  }
+};
+struct TabW {
+  const MM& m;
+  GKS_DI double operator()(int i, int j, int k) const { return TabP{m}(i, j, k) + TabN{m}(i, j, k); }
+};
+
+// GL_address / GR_address / G_address: basic_function.cpp:603-636
+template <class T> GKS_DI void contract(int u, int v, int x, double* psi, const double* a, const T& m) {
+  psi[0] = a[0] * m(u, v, x) + a[1] * m(u + 1, v, x) + a[2] * m(u, v + 1, x) +
+           a[3] * 0.5 * (m(u + 2, v, x) + m(u, v + 2, x) + m(u, v, x + 1));
+  psi[1] = a[0] * m(u + 1, v, x) + a[1] * m(u + 2, v, x) + a[2] * m(u + 1, v + 1, x) +
+           a[3] * 0.5 * (m(u + 3, v, x) + m(u + 1, v + 2, x) + m(u + 1, v, x + 1));
+  psi[2] = a[0] * m(u, v + 1, x) + a[1] * m(u + 1, v + 1, x) + a[2] * m(u, v + 2, x) +
+           a[3] * 0.5 * (m(u + 2, v + 1, x) + m(u, v + 3, x) + m(u, v + 1, x + 1));
+  psi[3] = 0.5 * (a[0] * (m(u + 2, v, x) + m(u, v + 2, x) + m(u, v, x + 1)) +
+                  a[1] * (m(u + 3, v, x) + m(u + 1, v + 2, x) + m(u + 1, v, x + 1)) +
+                  a[2] * (m(u + 2, v + 1, x) + m(u, v + 3, x) + m(u, v + 1, x + 1)) +
+                  a[3] * 0.5 *
+                      (m(u + 4, v, x) + m(u, v + 4, x) + m(u, v, x + 2) + 2 * m(u + 2, v + 2, x) +
+                       2 * m(u + 2, v, x + 1) + 2 * m(u, v + 2, x + 1)));
+}
+
+// A (2D): basic_function.cpp:571-583
+GKS_DI void micro_slope(double* a, const double* der, const double* prim, const Gas& g) {
+  double overden = 1.0 / prim[0];
+  double R4 = der[3] * overden -
+              0.5 * (prim[1] * prim[1] + prim[2] * prim[2] + 0.5 * (g.K + 2) / prim[3]) * der[0] * overden;
+  double R3 = (der[2] - prim[2] * der[0]) * overden;
+  double R2 = (der[1] - prim[1] * der[0]) * overden;
+  a[3] = (4.0 / (g.K + 2)) * prim[3] * prim[3] * (2 * R4 - 2 * prim[1] * R2 - 2 * prim[2] * R3);
+  a[2] = 2 * prim[3] * R3 - prim[2] * a[3];
+  a[1] = 2 * prim[3] * R2 - prim[1] * a[3];
+  a[0] = der[0] * overden - prim[1] * a[1] - prim[2] * a[2] -
+         0.5 * a[3] * (prim[1] * prim[1] + prim[2] * prim[2] + 0.5 * (g.K + 2) / prim[3]);
+}
+
+// Get_Tau_NS / Get_Tau: basic_function.cpp:638-708
+GKS_DI double tau_ns(double rho0, double lambda0, const Gas& g) {
+  if (g.tau_type == 0) return 0.0;
+  if (g.mu > 0.0) return 2.0 * g.mu * lambda0 / rho0;
+  if (g.nu > 0.0) return 2 * g.nu * lambda0;
+  return 0.0;
+}
+GKS_DI double tau_num(double rl, double rr, double r0, double ll, double lr, double l0, double dt,
+                      const Gas& g) {
+  if (g.tau_type == 0) {
+    if (g.c1 <= 0 && g.c2 <= 0) return 0.0;
+    double C = g.c2 * fabs(rl / ll - rr / lr) / fabs(rl / ll + rr / lr);
+    return g.c1 * dt + dt * C;
+  }
+  double tau_n = g.c2 * fabs(rl / ll - rr / lr) / fabs(rl / ll + rr / lr) * dt;
+  if (tau_n != tau_n) tau_n = 0.0;
+  if ((g.mu > 0.0 && g.nu > 0.0) || (g.mu < 0.0 && g.nu < 0.0)) return 0.0;
+  if (g.mu > 0.0) return tau_n + 2.0 * g.mu * l0 / r0;
+  if (g.nu > 0.0) return tau_n + 2 * g.nu * l0;
+  return 0.0;
+}
+// flux_function.cpp:410-424
+GKS_DI void time_coe(double* t, double tau, double tn, double dt) {
+  double eta = exp(-dt / tn);
+  t[0] = tn * (1 - eta);
+  t[1] = tn * (eta * (dt + tn) - tn) + tau * tn * (eta - 1);
+  t[2] = tau * tn * (eta - 1);
+  t[3] = tn * eta + dt - tn;
+  t[4] = tn * (tn - eta * (dt + tn) - tau * (eta - 1)) - dt * tau;
+  t[5] = 0.5 * dt * dt - tau * tn * (eta - 1) - tau * dt;
+}
+
+// One Gauss point: collision centre state (Center_all_collision_2d_multi,
+// reconstruction.cpp:3652-3687) followed by GKS2D with gks2nd_2d
+// (flux_function.cpp:345-698).  L/R: convar[0..3], der1x[4..7], der1y[8..11]
+// in the face-local frame.  Writes f[4], derf[4]; optionally the centre.
+GKS_DI void gks_point(double* f, double* derf, double* center, const double* L, const double* Rr,
+                      double dt, const Gas& g) {
+  double pl[4], pr[4];
+  convar_to_ulambda(pl, L, g);
+  convar_to_ulambda(pr, Rr, g);
+  MM ml, mr;
+  mmdf(ml, pl[1], pl[2], pl[3], g);
+  mmdf(mr, pr[1], pr[2], pr[3], g);
+  const TabP tl{ml};
+  const TabN tr{mr};
+  const TabW wl{ml}, wr{mr};
+
+  // --- centre state (Collision, basic_function.cpp:561-569)
+  double c0[4];
+  c0[0] = pl[0] * ml.up[0] + pr[0] * mr.un[0];
+  c0[1] = pl[0] * ml.up[1] + pr[0] * mr.un[1];
+  c0[2] = pl[0] * ml.vw[1] * ml.up[0] + pr[0] * mr.vw[1] * mr.un[0];
+  c0[3] = 0.5 * pl[0] * (ml.up[2] + ml.up[0] * ml.vw[2] + ml.up[0] * ml.xi2) +
+          0.5 * pr[0] * (mr.un[2] + mr.un[0] * mr.vw[2] + mr.un[0] * mr.xi2);
+  double alx[4], arx[4], aly[4], ary[4];
+  micro_slope(alx, L + 4, pl, g);
+  micro_slope(arx, Rr + 4, pr, g);
+  micro_slope(aly, L + 8, pl, g);
+  micro_slope(ary, Rr + 8, pr, g);
+  double c0x[4], c0y[4];
+  {
+    double al0[4], ar0[4];
+    contract(0, 0, 0, al0, alx, tl);
+    contract(0, 0, 0, ar0, arx, tr);
+#pragma unroll
+    for (int k = 0; k < 4; k++) c0x[k] = pl[0] * al0[k] + pr[0] * ar0[k];
+    contract(0, 0, 0, al0, aly, tl);
+    contract(0, 0, 0, ar0, ary, tr);
+#pragma unroll
+    for (int k = 0; k < 4; k++) c0y[k] = pl[0] * al0[k] + pr[0] * ar0[k];
+  }
+  if (center) {
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+      center[k] = c0[k];
+      center[4 + k] = c0x[k];
+      center[8 + k] = c0y[k];
+    }
+  }
+
+  // --- GKS2D
+  double p0[4];
+  convar_to_ulambda(p0, c0, g);
+  const double tau = tau_ns(p0[0], p0[3], g);
+  double tn = tau_num(pl[0], pr[0], p0[0], pl[3], pr[3], p0[3], dt, g);
+  double t[6];
+  time_coe(t, tau, tn, dt);
+
+  const double unit[4] = {1.0, 0.0, 0.0, 0.0};
+  double glu[4], gru[4];
+  contract(1, 0, 0, glu, unit, tl);
+  contract(1, 0, 0, gru, unit, tr);
+  double F0[4];
+#pragma unroll
+  for (int i = 0; i < 4; i++) F0[i] = pl[0] * t[0] * glu[i] + pr[0] * t[0] * gru[i];
+
+  MM m0;
+  mmdf(m0, p0[1], p0[2], p0[3], g);
+  const TabW w0{m0};
+  double g0u[4];
+  contract(1, 0, 0, g0u, unit, w0);
+#pragma unroll
+  for (int i = 0; i < 4; i++) F0[i] = F0[i] + p0[0] * t[3] * g0u[i];
+
+  double s1[4];  // pl0*(alxuul + alyuvl) + pr0*(arxuur + aryuvr)
+  {
+    double a[4], b[4], c[4], d[4];
+    contract(2, 0, 0, a, alx, tl);
+    contract(2, 0, 0, b, arx, tr);
+    contract(1, 1, 0, c, aly, tl);
+    contract(1, 1, 0, d, ary, tr);
+#pragma unroll
+    for (int i = 0; i < 4; i++) s1[i] = pl[0] * (a[i] + c[i]) + pr[0] * (b[i] + d[i]);
+  }
+#pragma unroll
+  for (int i = 0; i < 4; i++) F0[i] = F0[i] + t[1] * s1[i];
+
+  double Alul[4], Arur[4];
+  {
+    double xu[4], yv[4], der[4], A[4];
+    contract(1, 0, 0, xu, alx, wl);
+    contract(0, 1, 0, yv, aly, wl);
+#pragma unroll
+    for (int i = 0; i < 4; i++) der[i] = -pl[0] * (xu[i] + yv[i]);
+    micro_slope(A, der, pl, g);
+    contract(1, 0, 0, Alul, A, tl);
+    contract(1, 0, 0, xu, arx, wr);
+    contract(0, 1, 0, yv, ary, wr);
+#pragma unroll
+    for (int i = 0; i < 4; i++) der[i] = -pr[0] * (xu[i] + yv[i]);
+    micro_slope(A, der, pr, g);
+    contract(1, 0, 0, Arur, A, tr);
+  }
+#pragma unroll
+  for (int i = 0; i < 4; i++) F0[i] = F0[i] + t[2] * (pl[0] * Alul[i] + pr[0] * Arur[i]);
+
+  double s4[4], A0u[4];
+  double qs_raw = 0.0;  // Prandtl-fix heat-flux term (flux_function.cpp:630-636)
+  {
+    double a0x[4], a0y[4], a0xu[4], a0xuu[4], a0yv[4], a0yuv[4];
+    micro_slope(a0x, c0x, p0, g);
+    contract(1, 0, 0, a0xu, a0x, w0);
+    contract(2, 0, 0, a0xuu, a0x, w0);
+    micro_slope(a0y, c0y, p0, g);
+    contract(0, 1, 0, a0yv, a0y, w0);
+    contract(1, 1, 0, a0yuv, a0y, w0);
+#pragma unroll
+    for (int i = 0; i < 4; i++) s4[i] = a0xuu[i] + a0yuv[i];
+    double derA0[4], A0[4];
+#pragma unroll
+    for (int i = 0; i < 4; i++) derA0[i] = -p0[0] * (a0xu[i] + a0yv[i]);
+    micro_slope(A0, derA0, p0, g);
+    contract(1, 0, 0, A0u, A0, w0);
+    if (g.prandtl_fix)
+      qs_raw = a0xuu[3] + A0u[3] - p0[1] * (a0xuu[1] + A0u[1]) - p0[2] * (a0xuu[2] + A0u[2]);
+  }
+#pragma unroll
+  for (int i = 0; i < 4; i++) F0[i] = F0[i] + p0[0] * t[4] * s4[i];
+#pragma unroll
+  for (int i = 0; i < 4; i++) F0[i] = F0[i] + p0[0] * t[5] * (A0u[i]);
+  if (g.prandtl_fix) {
+    double qs = qs_raw;
+    qs *= -p0[0] * tau * dt;
+    F0[3] += (1.0 / g.pr - 1.0) * qs;
+  }
+
+  // --- second evaluation over dt/2 (flux_function.cpp:648-690)
+  const double dt2 = 0.5 * dt;
+  tn = tau_num(pl[0], pr[0], p0[0], pl[3], pr[3], p0[3], dt2, g);
+  time_coe(t, tau, tn, dt2);
+  double F1[4];
+#pragma unroll
+  for (int i = 0; i < 4; i++) {
+    F1[i] = t[0] * (pl[0] * glu[i] + pr[0] * gru[i]);
+    F1[i] = F1[i] + t[1] * s1[i];
+    F1[i] = F1[i] + t[2] * (pl[0] * Alul[i] + pr[0] * Arur[i]);
+    F1[i] = F1[i] + p0[0] * t[3] * g0u[i];
+    F1[i] = F1[i] + p0[0] * t[4] * s4[i];
+    F1[i] = F1[i] + p0[0] * t[5] * (A0u[i]);
+  }
+  if (g.prandtl_fix) {
+    double qs = qs_raw;
+    qs *= -p0[0] * tau * 0.5 * dt;
+    F1[3] += (1.0 / g.pr - 1.0) * qs;
+  }
+#pragma unroll
+  for (int i = 0; i < 4; i++) {
+    f[i] = (4.0 * F1[i] - F0[i]);
+    derf[i] = 4.0 * (F0[i] - 2.0 * F1[i]);
+  }
+}
+
+// ------------------------------------------------------------- WENO5-AO (+DF)
+// smoothness indicators and AO weights (reconstruction.cpp:480-511, 2742-2776)
+GKS_DI void weno5_weights(double* fw, double wn2, double wn1, double w0, double wp1, double wp2) {
+  const double dhi = 0.85, dlo = 0.85;
+  const double eps = 1e-6;
+  const double d[4] = {(1.0 - dhi) * (1.0 - dlo) / 2.0, (1.0 - dhi) * dlo, (1.0 - dhi) * (1.0 - dlo) / 2.0, dhi};
+  double q, q2, beta[4];
+  q = wn2 - 2.0 * wn1 + w0;
+  q2 = wn2 - 4.0 * wn1 + 3.0 * w0;
+  beta[0] = 13.0 / 12.0 * (q * q) + 0.25 * (q2 * q2);
+  q = wn1 - 2.0 * w0 + wp1;
+  q2 = wn1 - wp1;
+  beta[1] = 13.0 / 12.0 * (q * q) + 0.25 * (q2 * q2);
+  q = w0 - 2.0 * wp1 + wp2;
+  q2 = 3.0 * w0 - 4.0 * wp1 + wp2;
+  beta[2] = 13.0 / 12.0 * (q * q) + 0.25 * (q2 * q2);
+  beta[3] = 1.0 / 6.0 * (beta[0] + 4.0 * beta[1] + beta[2]) + fabs(beta[0] - beta[2]);
+  const double tau5 = 1.0 / 3.0 * (fabs(beta[3] - beta[0]) + fabs(beta[3] - beta[1]) + fabs(beta[3] - beta[2]));
+  double alpha[4], sum = 0.0;
+#pragma unroll
+  for (int i = 0; i < 4; i++) {
+    double gd = tau5 / (beta[i] + eps);
+    alpha[i] = d[i] * (1.0 + gd * gd);
+    sum += alpha[i];
+  }
+  double ww[4];
+#pragma unroll
+  for (int k = 0; k < 4; k++) ww[k] = alpha[k] / sum;
+  fw[3] = ww[3] / d[3];
+#pragma unroll
+  for (int k = 0; k < 3; k++) fw[k] = ww[k] - ww[3] / d[3] * d[k];
+}
+// weno_5th_ao_with_df_left: reconstruction.cpp:480-567
+GKS_DI void weno5_left(double& var, double& der1, double wn2, double wn1, double w0, double wp1, double wp2,
+                       const double* df, double h) {
+  double fw[4];
+  weno5_weights(fw, wn2, wn1, w0, wp1, wp2);
+  double p[4], px[4];
+  p[0] = w0 + df[0] * (-1.0 / 6.0 * wn2 + 5.0 / 6.0 * wn1 + 1.0 / 3.0 * w0 - w0);
+  p[1] = w0 + df[1] * (1.0 / 3.0 * wn1 + 5.0 / 6.0 * w0 - 1.0 / 6.0 * wp1 - w0);
+  p[2] = w0 + df[2] * (11.0 / 6.0 * w0 - 7.0 / 6.0 * wp1 + 1.0 / 3.0 * wp2 - w0);
+  p[3] = w0 + df[3] * ((1.0 / 60.0) * (47.0 * w0 + 27.0 * wn1 - 3.0 * wn2 - 13.0 * wp1 + 2.0 * wp2) - w0);
+  px[0] = df[0] * (w0 - wn1) / h;
+  px[1] = df[1] * (w0 - wn1) / h;
+  px[2] = -df[2] * ((2.0 * w0 - 3.0 * wp1 + wp2) / h);
+  px[3] = df[3] * (15.0 * w0 - 15.0 * wn1 + wn2 - wp1) / (12.0 * h);
+  var = 0.0;
+  der1 = 0.0;
+#pragma unroll
+  for (int k = 0; k < 4; k++) {
+    var += fw[k] * p[k];
+    der1 += fw[k] * px[k];
+  }
+}
+// weno_5th_ao_with_df_right: reconstruction.cpp:569-657
+GKS_DI void weno5_right(double& var, double& der1, double wn2, double wn1, double w0, double wp1, double wp2,
+                        const double* df, double h) {
+  double fw[4];
+  weno5_weights(fw, wn2, wn1, w0, wp1, wp2);
+  double p[4], px[4];
+  p[0] = w0 + df[0] * (1.0 / 3.0 * wn2 - 7.0 / 6.0 * wn1 + 11.0 / 6.0 * w0 - w0);
+  p[1] = w0 + df[1] * (-1.0 / 6.0 * wn1 + 5.0 / 6.0 * w0 + 1.0 / 3.0 * wp1 - w0);
+  p[2] = w0 + df[2] * (1.0 / 3.0 * w0 + 5.0 / 6.0 * wp1 - 1.0 / 6.0 * wp2 - w0);
+  p[3] = w0 + df[3] * ((1.0 / 60.0) * (47.0 * w0 - 13.0 * wn1 + 2.0 * wn2 + 27.0 * wp1 - 3.0 * wp2) - w0);
+  px[0] = df[0] * (2.0 * w0 - 3.0 * wn1 + wn2) / h;
+  px[1] = df[1] * (-w0 + wp1) / h;
+  px[2] = df[2] * (-w0 + wp1) / h;
+  px[3] = df[3] * (-15.0 * w0 + wn1 + 15.0 * wp1 - wp2) / (12.0 * h);
+  var = 0.0;
+  der1 = 0.0;
+#pragma unroll
+  for (int k = 0; k < 4; k++) {
+    var += fw[k] * p[k];
+    der1 += fw[k] * px[k];
+  }
+}
+// weno_5th_ao_with_df_2gauss: reconstruction.cpp:2742-2838
+GKS_DI void weno5_2gauss(double& g1, double& g1x, double& g2, double& g2x, const double* df, double wn2,
+                         double wn1, double w0, double wp1, double wp2, double h) {
+  double fw[4];
+  weno5_weights(fw, wn2, wn1, w0, wp1, wp2);
+  const double sqrt3 = 1.732050807568877294;
+  double p[4], px[4];
+  p[0] = w0 + df[0] * (w0 - (sqrt3 * w0) / 4 + (4 * wn1 - wn2) / (4 * sqrt3) - w0);
+  p[1] = w0 + df[1] * (w0 + (wn1 - wp1) / (4 * sqrt3) - w0);
+  p[2] = w0 + df[2] * ((1.0 / 12.0) * (3 * (4 + sqrt3) * w0 + sqrt3 * (-4 * wp1 + wp2)) - w0);
+  p[3] = w0 + df[3] * ((4314 * w0 + (4 + 500 * sqrt3) * wn1 - wn2 - 70 * sqrt3 * wn2 + 4 * wp1 -
+                        500 * sqrt3 * wp1 - wp2 + 70 * sqrt3 * wp2) / 4320 - w0);
+  px[0] = -df[0] * ((-9 + sqrt3) * w0 - 2 * (-6 + sqrt3) * wn1 + (-3 + sqrt3) * wn2) / (6 * h);
+  px[1] = -df[1] * (-2 * sqrt3 * w0 + (3 + sqrt3) * wn1 + (-3 + sqrt3) * wp1) / (6 * h);
+  px[2] = -df[2] * ((9 + sqrt3) * w0 - 2 * (6 + sqrt3) * wp1 + (3 + sqrt3) * wp2) / (6 * h);
+  px[3] = df[3] * (48 * sqrt3 * w0 - 72 * wn1 - 26 * sqrt3 * wn1 + 9 * wn2 + 2 * sqrt3 * wn2 + 72 * wp1 -
+                   26 * sqrt3 * wp1 - 9 * wp2 + 2 * sqrt3 * wp2) / (108 * h);
+  g1 = 0.0;
+  g1x = 0.0;
+#pragma unroll
+  for (int k = 0; k < 4; k++) {
+    g1 += fw[k] * p[k];
+    g1x += fw[k] * px[k];
+  }
+  p[0] = w0 + df[0] * ((1.0 / 12.0) * (3 * (4 + sqrt3) * w0 + sqrt3 * (-4 * wn1 + wn2)) - w0);
+  p[1] = w0 + df[1] * (w0 + (-wn1 + wp1) / (4 * sqrt3) - w0);
+  p[2] = w0 + df[2] * (w0 - (sqrt3 * w0) / 4 + (4 * wp1 - wp2) / (4 * sqrt3) - w0);
+  p[3] = w0 + df[3] * ((4314 * w0 + (4 - 500 * sqrt3) * wn1 - wn2 + 70 * sqrt3 * wn2 + 4 * wp1 +
+                        500 * sqrt3 * wp1 - wp2 - 70 * sqrt3 * wp2) / 4320 - w0);
+  px[0] = df[0] * ((9 + sqrt3) * w0 - 2 * (6 + sqrt3) * wn1 + (3 + sqrt3) * wn2) / (6 * h);
+  px[1] = df[1] * (-2 * sqrt3 * w0 + (-3 + sqrt3) * wn1 + (3 + sqrt3) * wp1) / (6 * h);
+  px[2] = df[2] * ((-9 + sqrt3) * w0 - 2 * (-6 + sqrt3) * wp1 + (-3 + sqrt3) * wp2) / (6 * h);
+  px[3] = -df[3] * ((48 * sqrt3 * w0 + 72 * wn1 - 26 * sqrt3 * wn1 - 9 * wn2 + 2 * sqrt3 * wn2 - 72 * wp1 -
+                     26 * sqrt3 * wp1 + 9 * wp2 + 2 * sqrt3 * wp2) / (108 * h));
+  g2 = 0.0;
+  g2x = 0.0;
+#pragma unroll
+  for (int k = 0; k < 4; k++) {
+    g2 += fw[k] * p[k];
+    g2x += fw[k] * px[k];
+  }
+}
+
+// DF factors: reconstruction.cpp:1733-1742
+GKS_DI void df5(double* df, const double* a) {
+  const double s[4] = {a[0] + a[2], a[1] + a[3], a[2] + a[4], a[0] + a[2] + a[4]};
+#pragma unroll
+  for (int i = 0; i < 4; i++) df[i] = s[i] < 2.0 ? 1.0 : 2.0 / s[i];
+}
+// Check_Order_Reduce_by_Lambda_2D: reconstruction.cpp:1439-1449
+GKS_DI bool reduce_order(const double* c, const Gas& g) {
+  double lam = lambda2(c[0], c[1] / c[0], c[2] / c[0], c[3], g);
+  return (lam <= 0.0 || !(lam == lam));
+}
+
+// Calculate_alpha_k: reconstruction.cpp:1451-1464 (inputs primitive)
+GKS_DI double alpha_k(const double* pl, const double* pr, const Gas& g) {
+  double cl = sqrt(g.gamma * pl[3] / pl[0]);
+  double cr = sqrt(g.gamma * pr[3] / pr[0]);
+  double mln = pl[1] / cl, mlt = pl[2] / cl;
+  double mrn = pr[1] / cr, mrt = pr[2] / cr;
+  double A = fabs(pl[3] - pr[3]) / pl[3] + fabs(pl[3] - pr[3]) / pr[3];
+  double dn = mln - mrn, dtg = mlt - mrt;
+  A = A + dn * dn + dtg * dtg;
+  return A;
+}
+
+}  // namespace gksd

@@ -1,0 +1,546 @@
+// gks2d_kernels.hpp — HIP kernels of the 2D GKS hot path and their launchers.
+//
+// Included by kernels_ieee.hip (built with -ffp-contract=off) and
+// kernels_contracted.hip (-ffp-contract=fast); GKS_VARIANT names the namespace.
+//
+// Device layout (all FP64, SoA, y fastest as in the reference):
+//   U, Uold      [4][nx*ny]          conservative state, cell i*ny+j
+//   ax, ay       [nx*ny]             DF shock sensor (ghost cells stay 0)
+//   line_x       [2 side][8][nfx]    x-face line states (convar, der1x);
+//                                    x face (i,j) = i*ny+j, i in [0,nx]
+//   line_y       [2][8][nfy]         y face (i,j) = i*(ny+1)+j, j in [0,ny]
+//   gst_x/_y     [2][gp][12][nf]     Gauss-point states (convar,der1x,der1y)
+//   flx/fly[s]   [gp][8][nf]         f[4], derf[4] of stage s (face frame)
+//   akx/aky      [gp][nf]            Calculate_alpha_k per Gauss point
+// side 0 = state left of the face ("line.left"), 1 = right ("line.right").
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "gks2d_device.hpp"
+#include "gks2d_launch.hpp"
+
+#ifndef GKS_VARIANT
+#error "GKS_VARIANT must name the kernel namespace"
+#endif
+
+namespace GKS_VARIANT {
+
+using namespace gksd;
+
+constexpr int kBlock = 256;
+constexpr int kPartialBlocks = 1024;
+
+__device__ __forceinline__ long long gtid() { return (long long)blockIdx.x * blockDim.x + threadIdx.x; }
+
+// ------------------------------------------------------------------ boundary
+// free / periodic / reflective fills of one side, in the reference's
+// semantics (boundary_condition.cpp:66-186, 367-442): left/right cover all
+// rows, down/up all columns (including the ghost columns just filled).
+__global__ void __launch_bounds__(kBlock) k_bc(Dev d, int side, int kind) {
+  long long t = gtid();
+  const int len = side < 2 ? d.ny : d.nx;
+  if (t >= (long long)d.G * len) return;
+  const int g = (int)(t / len), k = (int)(t % len);
+  int dst, src, neg;
+  if (side < 2) {
+    const int i = side == 0 ? d.G - 1 - g : d.nx - d.G + g;
+    int si;
+    if (kind == 0) si = side == 0 ? d.G : d.nx - d.G - 1;
+    else if (kind == 1) si = side == 0 ? i + d.NX : i - d.NX;
+    else si = side == 0 ? 2 * d.G - 1 - i : 2 * d.nx - 2 * d.G - 1 - i;
+    dst = i * d.ny + k;
+    src = si * d.ny + k;
+    neg = 1;
+  } else {
+    const int j = side == 2 ? d.G - 1 - g : d.ny - d.G + g;
+    int sj;
+    if (kind == 0) sj = side == 2 ? d.G : d.ny - d.G - 1;
+    else if (kind == 1) sj = side == 2 ? j + d.NY : j - d.NY;
+    else sj = side == 2 ? 2 * d.G - 1 - j : 2 * d.ny - 2 * d.G - 1 - j;
+    dst = k * d.ny + j;
+    src = k * d.ny + sj;
+    neg = 2;
+  }
+#pragma unroll
+  for (int v = 0; v < 4; v++) {
+    double val = d.U[v * d.nc + src];
+    if (kind == 2 && v == neg) val = -val;
+    d.U[v * d.nc + dst] = val;
+  }
+}
+
+// ------------------------------------------------------- primvar + CFL speed
+// Convar_to_Primvar (basic_function.cpp:752-773): blow-up test on all cells;
+// plus the Dtx speed sqrt(u^2+v^2)+c (time_advance.cpp:292-305) maxed over
+// the interior, which is what the next Get_CFL reads.  from_prim: take the
+// primitive variables from `prim` instead (step 0, initial condition).
+__global__ void __launch_bounds__(kBlock) k_primvar(Dev d, const double* prim, double* partial) {
+  __shared__ double smax[kBlock];
+  __shared__ int sbad[kBlock];
+  double tmax = 0.0;
+  int bad = 0;
+  for (long long c = gtid(); c < d.nc; c += (long long)gridDim.x * blockDim.x) {
+    double p[4];
+    if (prim) {
+#pragma unroll
+      for (int v = 0; v < 4; v++) p[v] = prim[v * d.nc + c];
+    } else {
+      double u[4];
+#pragma unroll
+      for (int v = 0; v < 4; v++) u[v] = d.U[v * d.nc + c];
+      convar_to_primvar(p, u, d.gas);
+      if (p[0] < 0 || p[3] < 0 || p[0] != p[0] || p[3] != p[3]) bad = 1;
+    }
+    const int i = (int)(c / d.ny), j = (int)(c % d.ny);
+    if (i >= d.G && i < d.nx - d.G && j >= d.G && j < d.ny - d.G) {
+      double tmp = sqrt(p[1] * p[1] + p[2] * p[2]) + sqrt(d.gas.gamma * p[3] / p[0]);
+      if (tmp > tmax) tmax = tmp;  // a NaN speed is skipped, as Dtx's comparison does
+    }
+  }
+  smax[threadIdx.x] = tmax;
+  sbad[threadIdx.x] = bad;
+  __syncthreads();
+  for (int s = kBlock / 2; s > 0; s >>= 1) {
+    if (threadIdx.x < s) {
+      double o = smax[threadIdx.x + s];
+      if (o > smax[threadIdx.x]) smax[threadIdx.x] = o;
+      sbad[threadIdx.x] |= sbad[threadIdx.x + s];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    partial[2 * blockIdx.x] = smax[0];
+    partial[2 * blockIdx.x + 1] = sbad[0] ? 1.0 : 0.0;
+  }
+}
+// final max / or over the partials into stat[slot*2 .. slot*2+1]; the flag
+// accumulates (cleared by the host when it reads it)
+__global__ void __launch_bounds__(kBlock) k_reduce(const double* partial, int n, double* stat, int slot) {
+  __shared__ double smax[kBlock];
+  __shared__ double sbad[kBlock];
+  double m = 0.0, b = 0.0;
+  for (int k = threadIdx.x; k < n; k += kBlock) {
+    double o = partial[2 * k];
+    if (o > m) m = o;
+    if (partial[2 * k + 1] != 0.0) b = 1.0;
+  }
+  smax[threadIdx.x] = m;
+  sbad[threadIdx.x] = b;
+  __syncthreads();
+  for (int s = kBlock / 2; s > 0; s >>= 1) {
+    if (threadIdx.x < s) {
+      double o = smax[threadIdx.x + s];
+      if (o > smax[threadIdx.x]) smax[threadIdx.x] = o;
+      if (sbad[threadIdx.x + s] != 0.0) sbad[threadIdx.x] = 1.0;
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    stat[2 * slot] = smax[0];
+    if (sbad[0] != 0.0) stat[4] = 1.0;  // sticky blow-up flag
+  }
+}
+
+// CopyFluid_new_to_old (basic_function.cpp:1571-1584), interior cells
+__global__ void __launch_bounds__(kBlock) k_copy_old(Dev d) {
+  long long t = gtid();
+  if (t >= (long long)d.NX * d.NY) return;
+  const int i = (int)(t / d.NY) + d.G, j = (int)(t % d.NY) + d.G;
+  const long long c = (long long)i * d.ny + j;
+#pragma unroll
+  for (int v = 0; v < 4; v++) d.Uold[v * d.nc + c] = d.U[v * d.nc + c];
+}
+
+// ------------------------------------------------------ normal reconstruction
+// WENO5_AO_with_df (reconstruction.cpp:1731-1859) for one cell along DIR;
+// the stencil is already in the x-like frame (YchangetoX for y).
+__device__ __forceinline__ void weno5_normal(double* lo, double* ro, const double (*w)[4], const double* alpha,
+                                             double h, int recon_var, const Gas& g) {
+  double df[4];
+  df5(df, alpha);
+  double pn1[4], p0[4], pp1[4];
+  convar_to_primvar(pn1, w[1], g);
+  convar_to_primvar(p0, w[2], g);
+  convar_to_primvar(pp1, w[3], g);
+#pragma unroll
+  for (int side = 0; side < 2; side++) {
+    double base[4];
+#pragma unroll
+    for (int i = 0; i < 4; i++) base[i] = side == 0 ? 0.5 * (pn1[i] + p0[i]) : 0.5 * (pp1[i] + p0[i]);
+    double c[5][4];
+    CharBase cb;
+    if (recon_var == 0) {
+#pragma unroll
+      for (int s = 0; s < 5; s++)
+#pragma unroll
+        for (int i = 0; i < 4; i++) c[s][i] = w[s][i];
+    } else {
+      char_base(cb, base, g);
+#pragma unroll
+      for (int s = 0; s < 5; s++) to_char(c[s], cb, w[s]);
+    }
+    double var[4], d1[4];
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+      if (side == 0) weno5_left(var[i], d1[i], c[0][i], c[1][i], c[2][i], c[3][i], c[4][i], df, h);
+      else weno5_right(var[i], d1[i], c[0][i], c[1][i], c[2][i], c[3][i], c[4][i], df, h);
+    }
+    double* out = side == 0 ? lo : ro;
+    if (recon_var == 0) {
+#pragma unroll
+      for (int i = 0; i < 4; i++) {
+        out[i] = var[i];
+        out[4 + i] = d1[i];
+      }
+    } else {
+      from_char(out, cb, var, g);
+      from_char(out + 4, cb, d1, g);
+    }
+  }
+  const bool rr = reduce_order(ro, g);
+  const bool rl = reduce_order(lo, g);
+  if (rl || rr) {
+#pragma unroll
+    for (int m = 0; m < 4; m++) {
+      ro[m] = w[2][m];
+      lo[m] = w[2][m];
+      ro[4 + m] = 0.0;
+      lo[4 + m] = 0.0;
+    }
+  }
+}
+
+// DIR 0: cells i in [G-1, nx-G], all j; writes x faces i (side 1) and i+1
+// (side 0).  DIR 1: cells j in [G-1, ny-G], all i; y faces j and j+1.
+// (reconstruction.cpp:1415-1425 with the guards of :1703, :1715)
+template <int DIR> __global__ void __launch_bounds__(kBlock) k_normal(Dev d) {
+  long long t = gtid();
+  int i, j;
+  if (DIR == 0) {
+    if (t >= (long long)(d.NX + 2) * d.ny) return;
+    i = (int)(t / d.ny) + d.G - 1;
+    j = (int)(t % d.ny);
+  } else {
+    if (t >= (long long)d.nx * (d.NY + 2)) return;
+    i = (int)(t / (d.NY + 2));
+    j = (int)(t % (d.NY + 2)) + d.G - 1;
+  }
+  double w[5][4], a[5];
+#pragma unroll
+  for (int s = 0; s < 5; s++) {
+    const long long c = DIR == 0 ? (long long)(i - 2 + s) * d.ny + j : (long long)i * d.ny + (j - 2 + s);
+    double u[4];
+#pragma unroll
+    for (int v = 0; v < 4; v++) u[v] = d.U[v * d.nc + c];
+    if (DIR == 0) {
+#pragma unroll
+      for (int v = 0; v < 4; v++) w[s][v] = u[v];
+      a[s] = d.ax[c];
+    } else {
+      w[s][0] = u[0];  // YchangetoX, basic_function.cpp:1192-1198
+      w[s][1] = u[2];
+      w[s][2] = -u[1];
+      w[s][3] = u[3];
+      a[s] = d.ay[c];
+    }
+  }
+  double lo[8], ro[8];
+  weno5_normal(lo, ro, w, a, DIR == 0 ? d.dx : d.dy, d.recon_var, d.gas);
+  double* line = DIR == 0 ? d.line_x : d.line_y;
+  const long long nf = DIR == 0 ? d.nfx : d.nfy;
+  const long long fa = DIR == 0 ? (long long)i * d.ny + j : (long long)i * (d.ny + 1) + j;
+  const long long fb = DIR == 0 ? (long long)(i + 1) * d.ny + j : (long long)i * (d.ny + 1) + j + 1;
+#pragma unroll
+  for (int k = 0; k < 8; k++) {
+    line[(8 + k) * nf + fa] = lo[k];  // side 1 (right state) of the face at the cell's left
+    line[k * nf + fb] = ro[k];        // side 0 (left state) of the face at the cell's right
+  }
+}
+
+// -------------------------------------------------- tangential reconstruction
+// weno_5th_ao_with_df_tangential, gausspoint 2 (reconstruction.cpp:2578-2740)
+// for one side of one face.  line[5][8]: convar, der1x of the 5 faces.
+__device__ __forceinline__ void weno5_tangent(double (*gp)[12], const double (*line)[8], const double* alpha,
+                                              double h, int recon_var, const Gas& g) {
+  double df[4];
+  df5(df, alpha);
+  if (recon_var == 0) {
+#pragma unroll
+    for (int i = 0; i < 4; i++)
+      weno5_2gauss(gp[0][i], gp[0][8 + i], gp[1][i], gp[1][8 + i], df, line[0][i], line[1][i], line[2][i],
+                   line[3][i], line[4][i], h);
+  } else {
+    double base[4];
+    convar_to_primvar(base, line[2], g);
+    CharBase cb;
+    char_base(cb, base, g);
+    double c[5][4];
+#pragma unroll
+    for (int s = 0; s < 5; s++) to_char(c[s], cb, line[s]);
+    double var[2][4], d1[2][4];
+#pragma unroll
+    for (int i = 0; i < 4; i++)
+      weno5_2gauss(var[0][i], d1[0][i], var[1][i], d1[1][i], df, c[0][i], c[1][i], c[2][i], c[3][i], c[4][i], h);
+#pragma unroll
+    for (int q = 0; q < 2; q++) {
+      from_char(gp[q], cb, var[q], g);
+      from_char(gp[q] + 8, cb, d1[q], g);
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < 4; i++) {
+    double t0, t1;
+    weno5_2gauss(gp[0][4 + i], t0, gp[1][4 + i], t1, df, line[0][4 + i], line[1][4 + i], line[2][4 + i],
+                 line[3][4 + i], line[4][4 + i], h);
+  }
+#pragma unroll
+  for (int q = 0; q < 2; q++) {
+    if (reduce_order(gp[q], g)) {
+#pragma unroll
+      for (int v = 0; v < 4; v++) {
+        gp[q][v] = line[2][v];
+        gp[q][4 + v] = line[2][4 + v];
+        gp[q][8 + v] = 0.0;
+      }
+    }
+  }
+}
+
+// Face ranges of the flux (flux_function.cpp:321-342):
+//   DIR 0: x faces i in [G, G+NX], j in [G, G+NY)
+//   DIR 1: y faces i in [G, G+NX), j in [G, G+NY]
+// Tangential stencils (reconstruction.cpp:2553-2576): x face (i,j) uses x
+// faces (i, j-2..j+2), alpha_y of cells (i-1, .) for side 0 and (i, .) for
+// side 1, h = dy; y face (i,j) uses y faces (i+2..i-2, j) (local tangent is
+// -x), alpha_x of cells (., j-1) / (., j), h = dx.
+template <int DIR> __global__ void __launch_bounds__(kBlock) k_tangent(Dev d) {
+  long long t = gtid();
+  const int ni = DIR == 0 ? d.NX + 1 : d.NX, nj = DIR == 0 ? d.NY : d.NY + 1;
+  if (t >= 2LL * ni * nj) return;
+  const int side = (int)(t / ((long long)ni * nj));
+  const long long r = t % ((long long)ni * nj);
+  const int i = (int)(r / nj) + d.G, j = (int)(r % nj) + d.G;
+  const double* line = DIR == 0 ? d.line_x : d.line_y;
+  const long long nf = DIR == 0 ? d.nfx : d.nfy;
+  double L[5][8], a[5];
+#pragma unroll
+  for (int s = 0; s < 5; s++) {
+    long long f, c;
+    if (DIR == 0) {
+      f = (long long)i * d.ny + (j - 2 + s);
+      c = (long long)(side == 0 ? i - 1 : i) * d.ny + (j - 2 + s);
+      a[s] = d.ay[c];
+    } else {
+      f = (long long)(i + 2 - s) * (d.ny + 1) + j;
+      c = (long long)(i + 2 - s) * d.ny + (side == 0 ? j - 1 : j);
+      a[s] = d.ax[c];
+    }
+#pragma unroll
+    for (int k = 0; k < 8; k++) L[s][k] = line[(side * 8 + k) * nf + f];
+  }
+  double gp[2][12];
+  weno5_tangent(gp, L, a, DIR == 0 ? d.dy : d.dx, d.recon_var, d.gas);
+  double* gst = DIR == 0 ? d.gst_x : d.gst_y;
+  const long long f0 = DIR == 0 ? (long long)i * d.ny + j : (long long)i * (d.ny + 1) + j;
+#pragma unroll
+  for (int q = 0; q < 2; q++)
+#pragma unroll
+    for (int k = 0; k < 12; k++) gst[((side * d.gp + q) * 12 + k) * nf + f0] = gp[q][k];
+}
+
+// -------------------------------------------------------------------- flux
+// Centre state + GKS2D per (face, Gauss point), and Calculate_alpha_k of the
+// point (reconstruction.cpp:1466-1495; y faces back to the global frame with
+// XchangetoY first).
+template <int DIR> __global__ void __launch_bounds__(kBlock) k_flux(Dev d, int stage, double dt) {
+  long long t = gtid();
+  const int ni = DIR == 0 ? d.NX + 1 : d.NX, nj = DIR == 0 ? d.NY : d.NY + 1;
+  if (t >= (long long)d.gp * ni * nj) return;
+  const int q = (int)(t / ((long long)ni * nj));
+  const long long r = t % ((long long)ni * nj);
+  const int i = (int)(r / nj) + d.G, j = (int)(r % nj) + d.G;
+  const long long nf = DIR == 0 ? d.nfx : d.nfy;
+  const long long f = DIR == 0 ? (long long)i * d.ny + j : (long long)i * (d.ny + 1) + j;
+  const double* gst = DIR == 0 ? d.gst_x : d.gst_y;
+  double L[12], R[12];
+#pragma unroll
+  for (int k = 0; k < 12; k++) {
+    L[k] = gst[((0 * d.gp + q) * 12 + k) * nf + f];
+    R[k] = gst[((1 * d.gp + q) * 12 + k) * nf + f];
+  }
+  double fo[4], dfo[4];
+  gks_point(fo, dfo, nullptr, L, R, dt, d.gas);
+  double* fl = DIR == 0 ? d.flx[stage] : d.fly[stage];
+#pragma unroll
+  for (int k = 0; k < 4; k++) {
+    fl[(q * 8 + k) * nf + f] = fo[k];
+    fl[(q * 8 + 4 + k) * nf + f] = dfo[k];
+  }
+  double pl[4], pr[4];
+  convar_to_primvar(pl, L, d.gas);
+  convar_to_primvar(pr, R, d.gas);
+  double a;
+  if (DIR == 0) {
+    a = alpha_k(pl, pr, d.gas);
+  } else {
+    const double tl[4] = {pl[0], -pl[2], pl[1], pl[3]};
+    const double tr[4] = {pr[0], -pr[2], pr[1], pr[3]};
+    a = alpha_k(tl, tr, d.gas);
+  }
+  (DIR == 0 ? d.akx : d.aky)[q * nf + f] = a;
+}
+
+// ------------------------------------------------------------------ update
+// S2O4 combination + Local_to_Global + cell update (time_advance.cpp:380-474)
+template <int STAGE>
+__device__ __forceinline__ void combined(double* x, const Dev& d, const double* const* fl, long long nf,
+                                         long long f, int q, bool yface) {
+  // S2O4_2D: c[0][0] = (0.5, 1/8); c[1][0] = (1, 1/6); c[1][1] = (0, 1/3)
+  const double c[2][2][2] = {{{0.5, 1.0 / 8.0}, {0.0, 0.0}}, {{1.0, 1.0 / 6.0}, {0.0, 1.0 / 3.0}}};
+#pragma unroll
+  for (int v = 0; v < 4; v++) {
+    double F = 0.0;
+#pragma unroll
+    for (int s = 0; s < STAGE + 1; s++)
+      F = F + d.gw[q] * (c[STAGE][s][0] * fl[s][(q * 8 + v) * nf + f] + c[STAGE][s][1] * fl[s][(q * 8 + 4 + v) * nf + f]);
+    x[v] = F;
+  }
+  const double n0 = yface ? 0.0 : 1.0, n1 = yface ? 1.0 : 0.0;
+  const double t0 = x[1], t1 = x[2];
+  x[1] = t0 * n0 - t1 * n1;
+  x[2] = t0 * n1 + t1 * n0;
+}
+template <int STAGE> __global__ void __launch_bounds__(kBlock) k_update(Dev d) {
+  long long t = gtid();
+  if (t >= (long long)d.NX * d.NY) return;
+  const int i = (int)(t / d.NY) + d.G, j = (int)(t % d.NY) + d.G;
+  const long long c = (long long)i * d.ny + j;
+  const long long fb = (long long)i * (d.ny + 1) + j, ft = fb + 1;
+  const long long fl_ = (long long)i * d.ny + j, fr = (long long)(i + 1) * d.ny + j;
+  double total[4] = {0.0, 0.0, 0.0, 0.0};
+  for (int q = 0; q < d.gp; q++) {
+    double xb[4], xt[4], xl[4], xr[4];
+    combined<STAGE>(xb, d, d.fly, d.nfy, fb, q, true);
+    combined<STAGE>(xt, d, d.fly, d.nfy, ft, q, true);
+    combined<STAGE>(xl, d, d.flx, d.nfx, fl_, q, false);
+    combined<STAGE>(xr, d, d.flx, d.nfx, fr, q, false);
+#pragma unroll
+    for (int v = 0; v < 4; v++) {
+      total[v] += d.dx * xb[v];
+      total[v] += -d.dx * xt[v];
+      total[v] += d.dy * xl[v];
+      total[v] += -d.dy * xr[v];
+    }
+  }
+#pragma unroll
+  for (int v = 0; v < 4; v++) {
+    double u = d.Uold[v * d.nc + c];
+    u += total[v] / d.area;
+    d.U[v * d.nc + c] = u;
+  }
+}
+
+// Update_alpha (reconstruction.cpp:1466-1534): the reference bounds j by
+// nodex (:1528); kept for nodey >= nodex, clipped to nodey otherwise.
+__global__ void __launch_bounds__(kBlock) k_alpha(Dev d) {
+  const int nj = d.NY < d.NX ? d.NY : d.NX;
+  long long t = gtid();
+  if (t >= (long long)d.NX * nj) return;
+  const int i = (int)(t / nj) + d.G, j = (int)(t % nj) + d.G;
+  auto face_alpha = [&](const double* ak, long long nf, long long f) {
+    double s;
+    if (d.gp == 2) s = 0.5 * (ak[f] + ak[nf + f]);
+    else if (d.gp == 3) s = (ak[f] + ak[nf + f] + ak[2 * nf + f]) / 3.0;
+    else s = 0.25 * (ak[f] + ak[nf + f] + ak[2 * nf + f] + ak[3 * nf + f]);
+    return s < 2.0 ? 0.0 : s;
+  };
+  const double al = face_alpha(d.akx, d.nfx, (long long)i * d.ny + j);
+  const double ar = face_alpha(d.akx, d.nfx, (long long)(i + 1) * d.ny + j);
+  const double ad = face_alpha(d.aky, d.nfy, (long long)i * (d.ny + 1) + j);
+  const double au = face_alpha(d.aky, d.nfy, (long long)i * (d.ny + 1) + j + 1);
+  const long long c = (long long)i * d.ny + j;
+  d.ax[c] = al + ar;
+  d.ay[c] = ad + au;
+}
+
+// leaf parity: centre + GKS2D over n independent points
+__global__ void __launch_bounds__(kBlock) k_flux_points(Gas g, long long n, const double* L, const double* R,
+                                                         double dt, double* C, double* F, double* D) {
+  long long t = gtid();
+  if (t >= n) return;
+  double l[12], r[12], c[12], f[4], df[4];
+#pragma unroll
+  for (int k = 0; k < 12; k++) {
+    l[k] = L[t * 12 + k];
+    r[k] = R[t * 12 + k];
+  }
+  gks_point(f, df, c, l, r, dt, g);
+#pragma unroll
+  for (int k = 0; k < 12; k++) C[t * 12 + k] = c[k];
+#pragma unroll
+  for (int k = 0; k < 4; k++) {
+    F[t * 4 + k] = f[k];
+    D[t * 4 + k] = df[k];
+  }
+}
+
+// ----------------------------------------------------------------- launchers
+#if !defined(__HIP_DEVICE_COMPILE__)
+static inline unsigned blocks_for(long long n) { return (unsigned)((n + kBlock - 1) / kBlock); }
+
+static hipError_t bc(const Dev& d, int side, int kind, hipStream_t s) {
+  const long long n = (long long)d.G * (side < 2 ? d.ny : d.nx);
+  hipLaunchKernelGGL(k_bc, dim3(blocks_for(n)), dim3(kBlock), 0, s, d, side, kind);
+  return hipGetLastError();
+}
+static hipError_t primvar(const Dev& d, const double* prim, int slot, hipStream_t s) {
+  long long nb = blocks_for(d.nc);
+  if (nb > kPartialBlocks) nb = kPartialBlocks;
+  hipLaunchKernelGGL(k_primvar, dim3((unsigned)nb), dim3(kBlock), 0, s, d, prim, d.partial);
+  hipLaunchKernelGGL(k_reduce, dim3(1), dim3(kBlock), 0, s, (const double*)d.partial, (int)nb, d.stat, slot);
+  return hipGetLastError();
+}
+static hipError_t copy_old(const Dev& d, hipStream_t s) {
+  hipLaunchKernelGGL(k_copy_old, dim3(blocks_for((long long)d.NX * d.NY)), dim3(kBlock), 0, s, d);
+  return hipGetLastError();
+}
+static hipError_t normal(const Dev& d, int dir, hipStream_t s) {
+  if (dir == 0) hipLaunchKernelGGL(k_normal<0>, dim3(blocks_for((long long)(d.NX + 2) * d.ny)), dim3(kBlock), 0, s, d);
+  else hipLaunchKernelGGL(k_normal<1>, dim3(blocks_for((long long)d.nx * (d.NY + 2))), dim3(kBlock), 0, s, d);
+  return hipGetLastError();
+}
+static hipError_t tangent(const Dev& d, int dir, hipStream_t s) {
+  if (dir == 0) hipLaunchKernelGGL(k_tangent<0>, dim3(blocks_for(2LL * (d.NX + 1) * d.NY)), dim3(kBlock), 0, s, d);
+  else hipLaunchKernelGGL(k_tangent<1>, dim3(blocks_for(2LL * d.NX * (d.NY + 1))), dim3(kBlock), 0, s, d);
+  return hipGetLastError();
+}
+static hipError_t flux(const Dev& d, int dir, int stage, double dt, hipStream_t s) {
+  if (dir == 0)
+    hipLaunchKernelGGL(k_flux<0>, dim3(blocks_for((long long)d.gp * (d.NX + 1) * d.NY)), dim3(kBlock), 0, s, d,
+                       stage, dt);
+  else
+    hipLaunchKernelGGL(k_flux<1>, dim3(blocks_for((long long)d.gp * d.NX * (d.NY + 1))), dim3(kBlock), 0, s, d,
+                       stage, dt);
+  return hipGetLastError();
+}
+static hipError_t update(const Dev& d, int stage, hipStream_t s) {
+  const unsigned nb = blocks_for((long long)d.NX * d.NY);
+  if (stage == 0) hipLaunchKernelGGL(k_update<0>, dim3(nb), dim3(kBlock), 0, s, d);
+  else hipLaunchKernelGGL(k_update<1>, dim3(nb), dim3(kBlock), 0, s, d);
+  return hipGetLastError();
+}
+static hipError_t alpha(const Dev& d, hipStream_t s) {
+  const int nj = d.NY < d.NX ? d.NY : d.NX;
+  hipLaunchKernelGGL(k_alpha, dim3(blocks_for((long long)d.NX * nj)), dim3(kBlock), 0, s, d);
+  return hipGetLastError();
+}
+static hipError_t flux_points(const Gas& g, long long n, const double* L, const double* R, double dt, double* C,
+                              double* F, double* D, hipStream_t s) {
+  hipLaunchKernelGGL(k_flux_points, dim3(blocks_for(n)), dim3(kBlock), 0, s, g, n, L, R, dt, C, F, D);
+  return hipGetLastError();
+}
+
+const LaunchTable table = {bc, primvar, copy_old, normal, tangent, flux, update, alpha, flux_points};
+#endif
+
+}  // namespace GKS_VARIANT

@@ -1,0 +1,50 @@
+// gks2d_launch.hpp — device-state descriptor shared by the kernel variants and
+// the C ABI layer, and the table of launchers each variant exports.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "gks2d_device.hpp"
+
+struct Dev {
+  int NX, NY, G, nx, ny, gp, recon_var, use_df;
+  double dx, dy, area;
+  double gw[4];
+  gksd::Gas gas;
+  long long nc, nfx, nfy;
+  double* U;
+  double* Uold;
+  double* ax;
+  double* ay;
+  double* line_x;
+  double* line_y;
+  double* gst_x;
+  double* gst_y;
+  double* flx[2];
+  double* fly[2];
+  double* akx;
+  double* aky;
+  double* partial;  // per-block (max speed, blow-up flag) pairs
+  double* stat;     // [0..3]: max speed of slot 0 / 1 (+pad); [4]: sticky blow-up flag
+};
+
+// device phases timed by gks2d_timing (order of GKS_PHASE_* in gks2d.h)
+enum Phase {
+  PH_BC = 0, PH_PRIMVAR, PH_NORMAL_X, PH_NORMAL_Y, PH_TANGENT_X, PH_TANGENT_Y,
+  PH_FLUX_X, PH_FLUX_Y, PH_UPDATE, PH_ALPHA, PH_COPY_OLD, PH_COUNT
+};
+
+struct LaunchTable {
+  hipError_t (*bc)(const Dev&, int side, int kind, hipStream_t);
+  hipError_t (*primvar)(const Dev&, const double* prim, int slot, hipStream_t);
+  hipError_t (*copy_old)(const Dev&, hipStream_t);
+  hipError_t (*normal)(const Dev&, int dir, hipStream_t);
+  hipError_t (*tangent)(const Dev&, int dir, hipStream_t);
+  hipError_t (*flux)(const Dev&, int dir, int stage, double dt, hipStream_t);
+  hipError_t (*update)(const Dev&, int stage, hipStream_t);
+  hipError_t (*alpha)(const Dev&, hipStream_t);
+  hipError_t (*flux_points)(const gksd::Gas&, long long n, const double* L, const double* R, double dt,
+                            double* C, double* F, double* D, hipStream_t);
+};
+
+namespace ieee { extern const LaunchTable table; }
+namespace contracted { extern const LaunchTable table; }

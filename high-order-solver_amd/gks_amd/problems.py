@@ -67,21 +67,14 @@ def riemann_prim(p, zones, discon=(0.5, 0.5)):
     """Primitive field [4, nx, ny] of IC_for_riemann_2d (ghosts zero)."""
     G, nx, ny = p.ghost, p.nx, p.ny
     dx = (p.right - p.left) / p.nodex
-    prim = np.zeros((4, nx, ny))
     zones = np.asarray(zones, dtype=np.float64)
-    for i in range(G, nx - G):
-        xi = (i - G + 1) < discon[0] / dx
-        for j in range(G, ny - G):
-            yj = (j - G + 1) < discon[1] / dx
-            if xi and yj:
-                z = 0
-            elif (not xi) and yj:
-                z = 1
-            elif (not xi) and (not yj):
-                z = 2
-            else:
-                z = 3
-            prim[:, i, j] = zones[z]
+    i = np.arange(G, nx - G)
+    j = np.arange(G, ny - G)
+    xi = ((i - G + 1) < discon[0] / dx)[:, None]
+    yj = ((j - G + 1) < discon[1] / dx)[None, :]
+    z = np.where(xi & yj, 0, np.where(~xi & yj, 1, np.where(~xi & ~yj, 2, 3)))
+    prim = np.zeros((4, nx, ny))
+    prim[:, G:nx - G, G:ny - G] = zones[z].transpose(2, 0, 1)
     return prim
 
 

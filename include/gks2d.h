@@ -128,6 +128,19 @@ int gks2d_run(gks2d_ctx* ctx, const int32_t bc[4], double tstop, int32_t max_ste
 /* Blocks until all queued device work of ctx is done. */
 int gks2d_sync(gks2d_ctx* ctx);
 
+/* Device time per kernel phase, measured with HIP events on the context's
+ * stream (no counterpart in the reference, whose only timer is clock() around
+ * the whole loop, 2Dproblem.cpp:164-167).  gks2d_timing(ctx, 1) resets and
+ * starts accumulating; gks2d_get_timing fills ms[GKS_NPHASE] (summed
+ * milliseconds) and launches[GKS_NPHASE]. */
+enum {
+  GKS_PHASE_BC = 0, GKS_PHASE_PRIMVAR, GKS_PHASE_NORMAL_X, GKS_PHASE_NORMAL_Y,
+  GKS_PHASE_TANGENT_X, GKS_PHASE_TANGENT_Y, GKS_PHASE_FLUX_X, GKS_PHASE_FLUX_Y,
+  GKS_PHASE_UPDATE, GKS_PHASE_ALPHA, GKS_PHASE_COPY_OLD, GKS_NPHASE
+};
+int gks2d_timing(gks2d_ctx* ctx, int enable);
+int gks2d_get_timing(gks2d_ctx* ctx, double* ms, int64_t* launches);
+
 /* Leaf parity entry: GKS2D (flux_function.cpp:345-698) preceded by the
  * collision centre state (Center_all_collision_2d_multi,
  * reconstruction.cpp:3652-3687) for n independent points.  left/right/center

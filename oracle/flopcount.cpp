@@ -135,14 +135,42 @@ int main(int argc, char** argv) {
     L.der1x[v] = 0.1; L.der1y[v] = 0.1; R.der1x[v] = 0.1; R.der1y[v] = 0.1;
   }
   long long f0 = cnt::n_arith, fe0 = cnt::n_exp, fr0 = cnt::n_erfc;
-  orc::center_collision(C, L, R, s.g);
+  orc::MM2<CD> ml, mr;
+  orc::center_collision(C, ml, mr, L, R, s.g);
   CD f[4], d[4];
-  orc::gks2d_flux(f, d, L, R, C, 1e-4, s.g);
-  printf("{\"N\": %d, \"flops_per_cell_update\": %.1f, \"exp_per_cell_update\": %.2f, "
+  orc::gks2d_flux(f, d, L, R, C, ml, mr, 1e-4, s.g);
+  const long long fp_arith = cnt::n_arith - f0, fp_exp = cnt::n_exp - fe0, fp_erfc = cnt::n_erfc - fr0;
+  // per-kernel units on developed data (face (10,10) of the last stage)
+  long long u0 = cnt::n_arith;
+  {
+    orc::Pt<CD> lo, ro;
+    CD w[5][4], a[5] = {0.0, 0.0, 3.0, 0.0, 0.0};
+    for (int q = 0; q < 5; q++) s.get_cell(w[q], (8 + q) * s.ny + 10);
+    orc::weno5_df_normal(lo, ro, a, w[0], w[1], w[2], w[3], w[4], s.dx, 1, s.g);
+  }
+  long long u1 = cnt::n_arith;
+  {
+    orc::Pt<CD> line[5], out[2];
+    CD a[5] = {0.0, 0.0, 3.0, 0.0, 0.0};
+    for (int q = 0; q < 5; q++) line[q] = s.xlL[s.face(10, 8 + q)];
+    orc::weno5_df_tangent_side(out, line, a, s.dy, 1, s.g);
+  }
+  long long u2 = cnt::n_arith;
+  {
+    CD pl[4], pr[4];
+    orc::convar_to_primvar_2d(pl, L.convar, s.g);
+    orc::convar_to_primvar_2d(pr, R.convar, s.g);
+    CD a = orc::alpha_k(pl, pr, s.g);
+    (void)a;
+  }
+  long long u3 = cnt::n_arith;
+  printf("{\"normal_cell_dir\": %lld, \"tangent_face_side\": %lld, \"alpha_point\": %lld, ",
+         u1 - u0, u2 - u1, u3 - u2);
+  printf("\"N\": %d, \"flops_per_cell_update\": %.1f, \"exp_per_cell_update\": %.2f, "
          "\"erfc_per_cell_update\": %.2f, \"primvar\": %.1f, \"reconstruct\": %.1f, \"flux\": %.1f, "
          "\"update\": %.1f, \"alpha\": %.1f, \"flux_point_flops\": %lld, \"flux_point_exp\": %lld, "
          "\"flux_point_erfc\": %lld}\n",
          N, arith, ex, er, part[0] / cells, part[1] / cells, part[2] / cells, part[3] / cells,
-         part[4] / cells, cnt::n_arith - f0, cnt::n_exp - fe0, cnt::n_erfc - fr0);
+         part[4] / cells, fp_arith, fp_exp, fp_erfc);
   return 0;
 }

@@ -262,12 +262,14 @@ inline R tau_num(R rl, R rr, R r0, R ll, R lr, R l0, double dt, const Gas& g) {
 
 // Center_all_collision_2d_multi: reconstruction.cpp:3652-3687 — collision
 // centre state of one Gauss point (convar, der1x, der1y).
+// ml / mr receive the left and right Maxwellian moments, which GKS2D needs
+// again (the reference recomputes them from the same inputs).
 template <class R>
-inline void center_collision(Pt<R>& center, const Pt<R>& left, const Pt<R>& right, const Gas& g) {
+inline void center_collision(Pt<R>& center, MM2<R>& ml, MM2<R>& mr, const Pt<R>& left, const Pt<R>& right,
+                             const Gas& g) {
   R pl[4], pr[4];
   convar_to_ulambda_2d(pl, left.convar, g);
   convar_to_ulambda_2d(pr, right.convar, g);
-  MM2<R> ml, mr;
   mmdf2(ml, pl[1], pl[2], pl[3], g);
   mmdf2(mr, pr[1], pr[2], pr[3], g);
   collision(center.convar, pl[0], pr[0], ml, mr);
@@ -299,19 +301,17 @@ template <class R> inline void time_coe(R* t, R tau, R tau_n, double dt) {
 }
 
 // GKS2D with gks2nd_2d and a two-stage (not S1O1) time scheme:
-// flux_function.cpp:345-698.  The centre state is taken from `center` (the
-// reference recomputes it identically at :369-401).  Outputs f = 4F(dt/2) -
+// flux_function.cpp:345-698.  The centre state and the left/right moments
+// are taken from center_collision (the reference recomputes both from the
+// same inputs at :363-401).  Outputs f = 4F(dt/2) -
 // F(dt) and derf = 4(F(dt) - 2F(dt/2)) in the face-local frame.
 template <class R>
 inline void gks2d_flux(R* f, R* derf, const Pt<R>& left, const Pt<R>& right, const Pt<R>& center,
-                       double dt, const Gas& g) {
+                       const MM2<R>& ml, const MM2<R>& mr, double dt, const Gas& g) {
   R Flux[2][4];
   R pl[4], pr[4], p0[4];
   convar_to_ulambda_2d(pl, left.convar, g);
   convar_to_ulambda_2d(pr, right.convar, g);
-  MM2<R> ml, mr;
-  mmdf2(ml, pl[1], pl[2], pl[3], g);
-  mmdf2(mr, pr[1], pr[2], pr[3], g);
   convar_to_ulambda_2d(p0, center.convar, g);
   R tau = tau_ns(p0[0], p0[3], g);
   R tn = tau_num(pl[0], pr[0], p0[0], pl[3], pr[3], p0[3], dt, g);

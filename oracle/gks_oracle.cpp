@@ -21,6 +21,21 @@ static bool params_ok(const gks2d_params* p) {
 
 extern "C" {
 
+// sizeof(gks2d_params) and the byte offset of each field, in declaration
+// order (checks the ctypes mirror against the C compiler)
+int64_t orc2d_params_layout(int64_t* off) {
+  const gks2d_params* z = nullptr;
+  const char* b = reinterpret_cast<const char*>(z);
+  int k = 0;
+#define OFF(f) off[k++] = reinterpret_cast<const char*>(&z->f) - b
+  OFF(nodex); OFF(nodey); OFF(ghost); OFF(gausspoint); OFF(weno_order); OFF(recon_variable);
+  OFF(use_df); OFF(tau_type); OFF(K); OFF(prandtl_fix); OFF(gamma); OFF(mu); OFF(nu); OFF(pr);
+  OFF(c1_euler); OFF(c2_euler); OFF(left); OFF(right); OFF(down); OFF(up); OFF(cfl);
+  OFF(fast_math); OFF(reserved);
+#undef OFF
+  return (int64_t)sizeof(gks2d_params);
+}
+
 void* orc2d_create(const gks2d_params* p) {
   if (!params_ok(p)) return nullptr;
   return new S2(*p);
@@ -156,11 +171,12 @@ int orc2d_flux_points(const gks2d_params* p, int64_t n, const double* left, cons
 #pragma omp parallel for
   for (int64_t k = 0; k < n; k++) {
     orc::Pt<double> L, R, C;
+    orc::MM2<double> ml, mr;
     load_pt(L, left + k * 12);
     load_pt(R, right + k * 12);
-    orc::center_collision(C, L, R, g);
+    orc::center_collision(C, ml, mr, L, R, g);
     if (center) store_pt(center + k * 12, C);
-    orc::gks2d_flux(f + k * 4, derf + k * 4, L, R, C, dt, g);
+    orc::gks2d_flux(f + k * 4, derf + k * 4, L, R, C, ml, mr, dt, g);
   }
   return GKS_OK;
 }

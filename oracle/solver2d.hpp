@@ -247,8 +247,9 @@ template <class R> struct Solver2d {
         for (int q = 0; q < gp; q++) {
           size_t k = (size_t)face(i, j) * gp + q;
           Pt<R> c;
-          center_collision(c, xgL[k], xgR[k], g);
-          gks2d_flux(&xfl[stage][k * 8], &xfl[stage][k * 8 + 4], xgL[k], xgR[k], c, dt, g);
+          MM2<R> ml, mr;
+          center_collision(c, ml, mr, xgL[k], xgR[k], g);
+          gks2d_flux(&xfl[stage][k * 8], &xfl[stage][k * 8 + 4], xgL[k], xgR[k], c, ml, mr, dt, g);
         }
 #pragma omp parallel for
     for (int i = G; i < P.nodex + G; i++)
@@ -256,8 +257,9 @@ template <class R> struct Solver2d {
         for (int q = 0; q < gp; q++) {
           size_t k = (size_t)face(i, j) * gp + q;
           Pt<R> c;
-          center_collision(c, ygL[k], ygR[k], g);
-          gks2d_flux(&yfl[stage][k * 8], &yfl[stage][k * 8 + 4], ygL[k], ygR[k], c, dt, g);
+          MM2<R> ml, mr;
+          center_collision(c, ml, mr, ygL[k], ygR[k], g);
+          gks2d_flux(&yfl[stage][k * 8], &yfl[stage][k * 8 + 4], ygL[k], ygR[k], c, ml, mr, dt, g);
         }
   }
 
