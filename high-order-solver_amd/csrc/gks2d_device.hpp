@@ -220,41 +220,74 @@ GKS_DI void time_coe(double* t, double tau, double tn, double dt) {
 // reconstruction.cpp:3652-3687) followed by GKS2D with gks2nd_2d
 // (flux_function.cpp:345-698).  L/R: convar[0..3], der1x[4..7], der1y[8..11]
 // in the face-local frame.  Writes f[4], derf[4]; optionally the centre.
+//
+// Register plan: every moment contraction that needs the left/right
+// Maxwellians is formed first (they die before the centre Maxwellian is
+// built); F(dt) and F(dt/2) are then summed from the stored 4-vectors, each in
+// the reference's own term order (flux_function.cpp:443-629 and 650-684).
 GKS_DI void gks_point(double* f, double* derf, double* center, const double* L, const double* Rr,
                       double dt, const Gas& g) {
   double pl[4], pr[4];
   convar_to_ulambda(pl, L, g);
   convar_to_ulambda(pr, Rr, g);
-  MM ml, mr;
-  mmdf(ml, pl[1], pl[2], pl[3], g);
-  mmdf(mr, pr[1], pr[2], pr[3], g);
-  const TabP tl{ml};
-  const TabN tr{mr};
-  const TabW wl{ml}, wr{mr};
-
-  // --- centre state (Collision, basic_function.cpp:561-569)
-  double c0[4];
-  c0[0] = pl[0] * ml.up[0] + pr[0] * mr.un[0];
-  c0[1] = pl[0] * ml.up[1] + pr[0] * mr.un[1];
-  c0[2] = pl[0] * ml.vw[1] * ml.up[0] + pr[0] * mr.vw[1] * mr.un[0];
-  c0[3] = 0.5 * pl[0] * (ml.up[2] + ml.up[0] * ml.vw[2] + ml.up[0] * ml.xi2) +
-          0.5 * pr[0] * (mr.un[2] + mr.un[0] * mr.vw[2] + mr.un[0] * mr.xi2);
-  double alx[4], arx[4], aly[4], ary[4];
-  micro_slope(alx, L + 4, pl, g);
-  micro_slope(arx, Rr + 4, pr, g);
-  micro_slope(aly, L + 8, pl, g);
-  micro_slope(ary, Rr + 8, pr, g);
-  double c0x[4], c0y[4];
+  double c0[4], c0x[4], c0y[4];
+  double glu[4], gru[4], s1[4], Alul[4], Arur[4];
   {
-    double al0[4], ar0[4];
-    contract(0, 0, 0, al0, alx, tl);
-    contract(0, 0, 0, ar0, arx, tr);
+    MM ml, mr;
+    mmdf(ml, pl[1], pl[2], pl[3], g);
+    mmdf(mr, pr[1], pr[2], pr[3], g);
+    const TabP tl{ml};
+    const TabN tr{mr};
+    const TabW wl{ml}, wr{mr};
+    // centre state (Collision, basic_function.cpp:561-569)
+    c0[0] = pl[0] * ml.up[0] + pr[0] * mr.un[0];
+    c0[1] = pl[0] * ml.up[1] + pr[0] * mr.un[1];
+    c0[2] = pl[0] * ml.vw[1] * ml.up[0] + pr[0] * mr.vw[1] * mr.un[0];
+    c0[3] = 0.5 * pl[0] * (ml.up[2] + ml.up[0] * ml.vw[2] + ml.up[0] * ml.xi2) +
+            0.5 * pr[0] * (mr.un[2] + mr.un[0] * mr.vw[2] + mr.un[0] * mr.xi2);
+    double alx[4], arx[4], aly[4], ary[4];
+    micro_slope(alx, L + 4, pl, g);
+    micro_slope(arx, Rr + 4, pr, g);
+    micro_slope(aly, L + 8, pl, g);
+    micro_slope(ary, Rr + 8, pr, g);
+    {
+      double al0[4], ar0[4];
+      contract(0, 0, 0, al0, alx, tl);
+      contract(0, 0, 0, ar0, arx, tr);
 #pragma unroll
-    for (int k = 0; k < 4; k++) c0x[k] = pl[0] * al0[k] + pr[0] * ar0[k];
-    contract(0, 0, 0, al0, aly, tl);
-    contract(0, 0, 0, ar0, ary, tr);
+      for (int k = 0; k < 4; k++) c0x[k] = pl[0] * al0[k] + pr[0] * ar0[k];
+      contract(0, 0, 0, al0, aly, tl);
+      contract(0, 0, 0, ar0, ary, tr);
 #pragma unroll
-    for (int k = 0; k < 4; k++) c0y[k] = pl[0] * al0[k] + pr[0] * ar0[k];
+      for (int k = 0; k < 4; k++) c0y[k] = pl[0] * al0[k] + pr[0] * ar0[k];
+    }
+    const double unit[4] = {1.0, 0.0, 0.0, 0.0};
+    contract(1, 0, 0, glu, unit, tl);
+    contract(1, 0, 0, gru, unit, tr);
+    {
+      double a[4], b[4], c[4], d[4];
+      contract(2, 0, 0, a, alx, tl);
+      contract(2, 0, 0, b, arx, tr);
+      contract(1, 1, 0, c, aly, tl);
+      contract(1, 1, 0, d, ary, tr);
+#pragma unroll
+      for (int i = 0; i < 4; i++) s1[i] = pl[0] * (a[i] + c[i]) + pr[0] * (b[i] + d[i]);
+    }
+    {
+      double xu[4], yv[4], der[4], A[4];
+      contract(1, 0, 0, xu, alx, wl);
+      contract(0, 1, 0, yv, aly, wl);
+#pragma unroll
+      for (int i = 0; i < 4; i++) der[i] = -pl[0] * (xu[i] + yv[i]);
+      micro_slope(A, der, pl, g);
+      contract(1, 0, 0, Alul, A, tl);
+      contract(1, 0, 0, xu, arx, wr);
+      contract(0, 1, 0, yv, ary, wr);
+#pragma unroll
+      for (int i = 0; i < 4; i++) der[i] = -pr[0] * (xu[i] + yv[i]);
+      micro_slope(A, der, pr, g);
+      contract(1, 0, 0, Arur, A, tr);
+    }
   }
   if (center) {
 #pragma unroll
@@ -265,65 +298,17 @@ GKS_DI void gks_point(double* f, double* derf, double* center, const double* L, 
     }
   }
 
-  // --- GKS2D
+  // equilibrium (centre) part
   double p0[4];
   convar_to_ulambda(p0, c0, g);
-  const double tau = tau_ns(p0[0], p0[3], g);
-  double tn = tau_num(pl[0], pr[0], p0[0], pl[3], pr[3], p0[3], dt, g);
-  double t[6];
-  time_coe(t, tau, tn, dt);
-
-  const double unit[4] = {1.0, 0.0, 0.0, 0.0};
-  double glu[4], gru[4];
-  contract(1, 0, 0, glu, unit, tl);
-  contract(1, 0, 0, gru, unit, tr);
-  double F0[4];
-#pragma unroll
-  for (int i = 0; i < 4; i++) F0[i] = pl[0] * t[0] * glu[i] + pr[0] * t[0] * gru[i];
-
-  MM m0;
-  mmdf(m0, p0[1], p0[2], p0[3], g);
-  const TabW w0{m0};
-  double g0u[4];
-  contract(1, 0, 0, g0u, unit, w0);
-#pragma unroll
-  for (int i = 0; i < 4; i++) F0[i] = F0[i] + p0[0] * t[3] * g0u[i];
-
-  double s1[4];  // pl0*(alxuul + alyuvl) + pr0*(arxuur + aryuvr)
-  {
-    double a[4], b[4], c[4], d[4];
-    contract(2, 0, 0, a, alx, tl);
-    contract(2, 0, 0, b, arx, tr);
-    contract(1, 1, 0, c, aly, tl);
-    contract(1, 1, 0, d, ary, tr);
-#pragma unroll
-    for (int i = 0; i < 4; i++) s1[i] = pl[0] * (a[i] + c[i]) + pr[0] * (b[i] + d[i]);
-  }
-#pragma unroll
-  for (int i = 0; i < 4; i++) F0[i] = F0[i] + t[1] * s1[i];
-
-  double Alul[4], Arur[4];
-  {
-    double xu[4], yv[4], der[4], A[4];
-    contract(1, 0, 0, xu, alx, wl);
-    contract(0, 1, 0, yv, aly, wl);
-#pragma unroll
-    for (int i = 0; i < 4; i++) der[i] = -pl[0] * (xu[i] + yv[i]);
-    micro_slope(A, der, pl, g);
-    contract(1, 0, 0, Alul, A, tl);
-    contract(1, 0, 0, xu, arx, wr);
-    contract(0, 1, 0, yv, ary, wr);
-#pragma unroll
-    for (int i = 0; i < 4; i++) der[i] = -pr[0] * (xu[i] + yv[i]);
-    micro_slope(A, der, pr, g);
-    contract(1, 0, 0, Arur, A, tr);
-  }
-#pragma unroll
-  for (int i = 0; i < 4; i++) F0[i] = F0[i] + t[2] * (pl[0] * Alul[i] + pr[0] * Arur[i]);
-
-  double s4[4], A0u[4];
+  double g0u[4], s4[4], A0u[4];
   double qs_raw = 0.0;  // Prandtl-fix heat-flux term (flux_function.cpp:630-636)
   {
+    MM m0;
+    mmdf(m0, p0[1], p0[2], p0[3], g);
+    const TabW w0{m0};
+    const double unit[4] = {1.0, 0.0, 0.0, 0.0};
+    contract(1, 0, 0, g0u, unit, w0);
     double a0x[4], a0y[4], a0xu[4], a0xuu[4], a0yv[4], a0yuv[4];
     micro_slope(a0x, c0x, p0, g);
     contract(1, 0, 0, a0xu, a0x, w0);
@@ -341,20 +326,29 @@ GKS_DI void gks_point(double* f, double* derf, double* center, const double* L, 
     if (g.prandtl_fix)
       qs_raw = a0xuu[3] + A0u[3] - p0[1] * (a0xuu[1] + A0u[1]) - p0[2] * (a0xuu[2] + A0u[2]);
   }
+
+  const double tau = tau_ns(p0[0], p0[3], g);
+  double t[6];
+  // F(dt): flux_function.cpp:406-629
+  double F0[4];
+  time_coe(t, tau, tau_num(pl[0], pr[0], p0[0], pl[3], pr[3], p0[3], dt, g), dt);
 #pragma unroll
-  for (int i = 0; i < 4; i++) F0[i] = F0[i] + p0[0] * t[4] * s4[i];
-#pragma unroll
-  for (int i = 0; i < 4; i++) F0[i] = F0[i] + p0[0] * t[5] * (A0u[i]);
+  for (int i = 0; i < 4; i++) {
+    F0[i] = pl[0] * t[0] * glu[i] + pr[0] * t[0] * gru[i];
+    F0[i] = F0[i] + p0[0] * t[3] * g0u[i];
+    F0[i] = F0[i] + t[1] * s1[i];
+    F0[i] = F0[i] + t[2] * (pl[0] * Alul[i] + pr[0] * Arur[i]);
+    F0[i] = F0[i] + p0[0] * t[4] * s4[i];
+    F0[i] = F0[i] + p0[0] * t[5] * (A0u[i]);
+  }
   if (g.prandtl_fix) {
     double qs = qs_raw;
     qs *= -p0[0] * tau * dt;
     F0[3] += (1.0 / g.pr - 1.0) * qs;
   }
-
-  // --- second evaluation over dt/2 (flux_function.cpp:648-690)
+  // F(dt/2): flux_function.cpp:648-684
   const double dt2 = 0.5 * dt;
-  tn = tau_num(pl[0], pr[0], p0[0], pl[3], pr[3], p0[3], dt2, g);
-  time_coe(t, tau, tn, dt2);
+  time_coe(t, tau, tau_num(pl[0], pr[0], p0[0], pl[3], pr[3], p0[3], dt2, g), dt2);
   double F1[4];
 #pragma unroll
   for (int i = 0; i < 4; i++) {

@@ -484,6 +484,17 @@ __global__ void __launch_bounds__(kBlock) k_flux_points(Gas g, long long n, cons
   }
 }
 
+// explicit instantiations: the launchers below are host-only, so the device
+// pass must be told which template kernels to emit
+template __global__ void k_normal<0>(Dev);
+template __global__ void k_normal<1>(Dev);
+template __global__ void k_tangent<0>(Dev);
+template __global__ void k_tangent<1>(Dev);
+template __global__ void k_flux<0>(Dev, int, double);
+template __global__ void k_flux<1>(Dev, int, double);
+template __global__ void k_update<0>(Dev);
+template __global__ void k_update<1>(Dev);
+
 // ----------------------------------------------------------------- launchers
 #if !defined(__HIP_DEVICE_COMPILE__)
 static inline unsigned blocks_for(long long n) { return (unsigned)((n + kBlock - 1) / kBlock); }

@@ -18,6 +18,8 @@ def build_hip(force=True):
     if force:
         _make(csrc, "clean")
     _make(csrc)
+    # reference-named C++ call surface + example driver, over the C ABI
+    _make(os.path.join(PKG_DIR, "host"), "-B")
 
 
 def build_oracle(with_reference=True):

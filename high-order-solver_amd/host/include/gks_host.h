@@ -1,0 +1,235 @@
+// gks_host.h — the reference solver's 2D call surface (types, globals,
+// function-pointer plugin points and sweep functions) over the gfx950 C ABI
+// (include/gks2d.h).  A 2D problem driver written against the reference
+// (src/2Dproblem.cpp style: set globals, assign the scheme pointers, call the
+// sweeps in order) compiles against these headers unchanged and runs its
+// per-step hot path on the GPU.
+//
+// Names, argument meanings and error behaviour follow the reference headers
+// (include/fluid_mesh.h, basic_function.h, boundary_condition.h,
+// reconstruction.h, flux_function.h, time_advance.h, output.h).  What
+// differs, by design:
+//  * face records (Interface2d gauss states, Flux2d_gauss fluxes) live on the
+//    device; the host arrays Setinterface_array / Setflux_gauss_array return
+//    carry geometry only;
+//  * the cell array is uploaded at the first sweep that touches it and copied
+//    back by output2d() or Download_fluid(); host writes to `fluids` between
+//    sweeps need Upload_fluid();
+//  * a blow-up found by Convar_to_Primvar is reported (message + exit(0), as
+//    the reference does) at the next Get_CFL.
+#pragma once
+#include <cmath>
+#include <ctime>
+#include <cstring>
+#include <iostream>
+#include <string>
+
+using namespace std;
+
+// ---------------------------------------------------------------- fluid_mesh
+extern int gausspoint;
+extern double* gauss_loc;
+extern double* gauss_weight;
+
+class Block2d {
+ public:
+  int index = 0;
+  bool uniform = true;
+  int ghost = 5;
+  int nodex = 0, nodey = 0;
+  int nx = 0, ny = 0;
+  int xcell_begin = 0, xcell_end = 0, ycell_begin = 0, ycell_end = 0;
+  int xinterface_begin_n = 0, xinterface_begin_t = 0, xinterface_end_n = 0, xinterface_end_t = 0;
+  int yinterface_begin_n = 0, yinterface_begin_t = 0, yinterface_end_n = 0, yinterface_end_t = 0;
+  double dx = 0, dy = 0, overdx = 0, overdy = 0;
+  double left = 0, right = 0, down = 0, up = 0;
+  int stages = 0;
+  double timecoefficient[5][5][2] = {};
+  double t = 0;
+  double CFL = 0;
+  double dt = 0;
+  int step = 0;
+  double T = 0;
+  int outputperstep = 0;
+};
+
+class Point2d {
+ public:
+  double convar[4];
+  double primvar[4];
+  double der1x[4];
+  double der1y[4];
+  double x, y;
+  double normal[2];
+  bool is_reduce_order;
+};
+
+class Fluid2d {
+ public:
+  double primvar[4];
+  double exact;
+  double convar[4];
+  double convar_old[4];
+  double res[4];
+  double xindex, yindex;
+  double coordx, coordy;
+  double dx, dy;
+  double area;
+  double node[8];
+  double alpha_x = 0.0;
+  double alpha_y = 0.0;
+};
+
+class Flux2d {
+ public:
+  double x[4];
+  double f[4];
+  double derf[4];
+  double length;
+  double normal[2];
+};
+
+class Recon2d {
+ public:
+  Point2d left, center, right;
+  double x, y;
+  double normal[2];
+};
+
+// geometry of one face; its reconstructed states live on the device
+class Interface2d {
+ public:
+  double x = 0, y = 0;
+  double normal[2] = {0, 0};
+  double length = 0;
+};
+
+// per-face flux records live on the device; the host array holds null
+// entries so drivers can keep passing it around
+class Flux2d_gauss {
+ public:
+  Flux2d* gauss = nullptr;
+};
+
+void SetGuassPoint();
+
+// ------------------------------------------------------------ basic_function
+extern int K;
+extern double Gamma;
+extern double Mu;
+extern double Nu;
+extern double T_inf;
+extern double c1_euler;
+extern double c2_euler;
+extern bool is_Prandtl_fix;
+extern double Pr;
+extern double R_gas;
+enum TAU_TYPE { Euler, NS };
+extern TAU_TYPE tau_type;
+
+Fluid2d* Setfluid(Block2d& block);
+Interface2d* Setinterface_array(Block2d block);
+Flux2d_gauss** Setflux_gauss_array(Block2d block);
+void SetUniformMesh(Block2d& block, Fluid2d* fluids, Interface2d* xinterfaces, Interface2d* yinterfaces,
+                    Flux2d_gauss** xfluxes, Flux2d_gauss** yfluxes);
+void CopyFluid_new_to_old(Fluid2d* fluids, Block2d block);
+void Convar_to_Primvar(Fluid2d* fluids, Block2d block);
+void Copy_Array(double* target, double* origin, int dim);
+void Convar_to_primvar_2D(double* primvar, double convar[4]);
+void Primvar_to_convar_2D(double* convar, double primvar[4]);
+
+// -------------------------------------------------------- boundary_condition
+typedef void (*BoundaryCondition2d)(Fluid2d* fluids, Block2d block, Fluid2d bcvalue);
+void free_boundary_left(Fluid2d* fluids, Block2d block, Fluid2d bcvalue);
+void free_boundary_right(Fluid2d* fluids, Block2d block, Fluid2d bcvalue);
+void free_boundary_down(Fluid2d* fluids, Block2d block, Fluid2d bcvalue);
+void free_boundary_up(Fluid2d* fluids, Block2d block, Fluid2d bcvalue);
+void periodic_boundary_left(Fluid2d* fluids, Block2d block, Fluid2d bcvalue);
+void periodic_boundary_right(Fluid2d* fluids, Block2d block, Fluid2d bcvalue);
+void periodic_boundary_down(Fluid2d* fluids, Block2d block, Fluid2d bcvalue);
+void periodic_boundary_up(Fluid2d* fluids, Block2d block, Fluid2d bcvalue);
+void reflection_boundary_left(Fluid2d* fluids, Block2d block, Fluid2d bcvalue);
+void reflection_boundary_right(Fluid2d* fluids, Block2d block, Fluid2d bcvalue);
+void reflection_boundary_down(Fluid2d* fluids, Block2d block, Fluid2d bcvalue);
+void reflection_boundary_up(Fluid2d* fluids, Block2d block, Fluid2d bcvalue);
+
+// ------------------------------------------------------------ reconstruction
+enum Reconstruction_variable { conservative, characteristic };
+extern Reconstruction_variable reconstruction_variable;
+enum WENOtype { linear, wenojs, wenoz };
+extern WENOtype wenotype;
+extern bool is_reduce_order_warning;
+extern bool is_using_df_factor;
+
+typedef void (*Reconstruction_within_Cell_2D_normal)(Interface2d& left, Interface2d& right, Interface2d& down,
+                                                     Interface2d& up, Fluid2d* fluids, Block2d block);
+extern Reconstruction_within_Cell_2D_normal cellreconstruction_2D_normal;
+typedef void (*Reconstruction_within_Cell_2D_tangent)(Interface2d* left, Interface2d* right, Interface2d* down,
+                                                      Interface2d* up, Fluid2d* fluids, Block2d block);
+extern Reconstruction_within_Cell_2D_tangent cellreconstruction_2D_tangent;
+typedef void (*Reconstruction_forG0_2D_normal)(Interface2d* xinterfaces, Interface2d* yinterfaces, Fluid2d* fluids,
+                                               Block2d block);
+extern Reconstruction_forG0_2D_normal g0reconstruction_2D_normal;
+typedef void (*Reconstruction_forG0_2D_tangent)(Interface2d* xinterfaces, Interface2d* yinterfaces,
+                                                Fluid2d* fluids, Block2d block);
+extern Reconstruction_forG0_2D_tangent g0reconstruction_2D_tangent;
+
+// plugin leaves: assignable to the pointers above; the sweeps dispatch on
+// their identity (a leaf called directly is an error: its work is fused
+// into the device sweep)
+void WENO5_AO_normal(Interface2d& left, Interface2d& right, Interface2d& down, Interface2d& up, Fluid2d* fluids,
+                     Block2d block);
+void WENO5_AO_with_df_normal(Interface2d& left, Interface2d& right, Interface2d& down, Interface2d& up,
+                             Fluid2d* fluids, Block2d block);
+void WENO5_AO_tangent(Interface2d* left, Interface2d* right, Interface2d* down, Interface2d* up, Fluid2d* fluids,
+                      Block2d block);
+void WENO5_AO_with_df_tangent(Interface2d* left, Interface2d* right, Interface2d* down, Interface2d* up,
+                              Fluid2d* fluids, Block2d block);
+void Center_do_nothing_normal(Interface2d* xinterfaces, Interface2d* yinterfaces, Fluid2d* fluids, Block2d block);
+void Center_all_collision_multi(Interface2d* xinterfaces, Interface2d* yinterfaces, Fluid2d* fluids,
+                                Block2d block);
+
+void Reconstruction_within_cell(Interface2d* xinterfaces, Interface2d* yinterfaces, Fluid2d* fluids,
+                                Block2d block);
+void Reconstruction_forg0(Interface2d* xinterfaces, Interface2d* yinterfaces, Fluid2d* fluids, Block2d block);
+void Update_alpha(Interface2d* xinterfaces, Interface2d* yinterfaces, Fluid2d* fluids, Block2d block);
+
+// ------------------------------------------------------------- flux_function
+enum GKS2d_type { nothing_2d, kfvs1st_2d, kfvs2nd_2d, gks1st_2d, gks2nd_2d, pp_gks_2d };
+extern GKS2d_type gks2dsolver;
+typedef void (*Flux_function_2d)(Flux2d& flux, Recon2d& re, double dt);
+extern Flux_function_2d flux_function_2d;
+// one point on the GPU: centre state + GKS2D (flux_function.cpp:345-698)
+void GKS2D(Flux2d& flux, Recon2d& interface, double dt);
+void Calculate_flux(Flux2d_gauss** xfluxes, Flux2d_gauss** yfluxes, Interface2d* xinterfaces,
+                    Interface2d* yinterfaces, Block2d block, int stage);
+
+// -------------------------------------------------------------- time_advance
+typedef void (*TimeMarchingCoefficient_2d)(Block2d& block);
+extern TimeMarchingCoefficient_2d timecoe_list_2d;
+void S2O4_2D(Block2d& block);
+void Initial_stages(Block2d& block);
+double Get_CFL(Block2d& block, Fluid2d* fluids, double tstop);
+void Update(Fluid2d* fluids, Flux2d_gauss** xfluxes, Flux2d_gauss** yfluxes, Block2d block, int stage);
+
+// -------------------------------------------------------------------- output
+struct Runtime {
+  clock_t start_initial = 0, finish_initial = 0, start_compute = 0, finish_compute = 0;
+};
+// writes convar[0] of the interior, j outer / i inner, to ../data/R-<step>.txt
+// (output.cpp:64-84), after copying the device state back
+void output2d(Fluid2d* fluids, Block2d block);
+
+// ----------------------------------------------------------- problem set-up
+void IC_for_riemann_2d(Fluid2d* fluid, double* zone1, double* zone2, double* zone3, double* zone4, Block2d block,
+                       double* discon);
+
+// --------------------------------------------- device residency (shim only)
+// copies the device state (convar, alpha) back into `fluids` and refreshes
+// primvar on the host
+void Download_fluid(Fluid2d* fluids, Block2d block);
+// pushes host-side edits of `fluids` (convar, alpha, primvar for the next
+// Get_CFL) to the device
+void Upload_fluid(Fluid2d* fluids, Block2d block);
+// releases the device state bound to `fluids`
+void Release_fluid(Fluid2d* fluids);
