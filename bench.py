@@ -89,8 +89,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--n", type=int, default=2048)
     ap.add_argument("--seed", type=int, default=0)
-    ap.add_argument("--variant", choices=["ieee", "contracted"], default="ieee")
-    ap.add_argument("--cpu-n", type=int, default=1024)
+    ap.add_argument("--variant", choices=["ieee", "fast"], default="ieee")
+    ap.add_argument("--cpu-n", type=int, default=2048)
     ap.add_argument("--cpu-steps", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     args = ap.parse_args()
@@ -108,7 +108,7 @@ def main():
     from gks_amd.problems import riemann_prim, riemann_states
 
     n = args.n
-    p = planar_shock_params(n, fast_math=1 if args.variant == "contracted" else 0)
+    p = planar_shock_params(n, fast_math=1 if args.variant == "fast" else 0)
     prim = riemann_prim(p, riemann_states(args.seed))
     s = gks_amd.Solver2D(p)
     s.init_prim(prim)

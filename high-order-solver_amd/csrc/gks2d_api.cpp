@@ -41,6 +41,7 @@ gksd::Gas gas_of(const gks2d_params& p) {
   g.c2 = p.c2_euler;
   g.tau_type = p.tau_type;
   g.prandtl_fix = p.prandtl_fix;
+  g.inv_gm1 = 1.0 / (p.gamma - 1.0);
   return g;
 }
 
@@ -174,13 +175,15 @@ int gks2d_create(const gks2d_params* p, gks2d_ctx** out) {
   d.dx = (p->right - p->left) / p->nodex;  // SetUniformMesh, basic_function.cpp:1411-1416
   d.dy = (p->up - p->down) / p->nodey;
   d.area = d.dx * d.dy;
+  d.idx = 1.0 / d.dx;
+  d.idy = 1.0 / d.dy;
   d.gw[0] = 0.5;  // SetGuassPoint, fluid_mesh.cpp:19-23
   d.gw[1] = 0.5;
   d.gas = gas_of(*p);
   d.nc = (long long)d.nx * d.ny;
   d.nfx = (long long)(d.nx + 1) * d.ny;
   d.nfy = (long long)d.nx * (d.ny + 1);
-  c->lt = p->fast_math ? &contracted::table : &ieee::table;
+  c->lt = p->fast_math ? &fast::table : &ieee::table;
   const long long nfm = d.nfx > d.nfy ? d.nfx : d.nfy;
   (void)nfm;
   hipError_t e = hipStreamCreateWithFlags(&c->s, hipStreamNonBlocking);
@@ -469,7 +472,7 @@ int gks2d_flux_points(const gks2d_params* p, int64_t n, const double* left, cons
   if (!p || !left || !right || !f || !derf) return fail(GKS_ERR_ARG, "null");
   if (n < 0) return fail(GKS_ERR_ARG, "n < 0");
   if (n == 0) return GKS_OK;
-  const LaunchTable* lt = p->fast_math ? &contracted::table : &ieee::table;
+  const LaunchTable* lt = p->fast_math ? &fast::table : &ieee::table;
   double *dl = nullptr, *dr = nullptr, *dc = nullptr, *dfo = nullptr, *dd = nullptr;
   hipError_t e = hipMalloc(&dl, n * 12 * 8);
   if (e == hipSuccess) e = hipMalloc(&dr, n * 12 * 8);

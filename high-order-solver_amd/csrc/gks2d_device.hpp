@@ -1,34 +1,61 @@
 // gks2d_device.hpp — device numerics of the 2D GKS hot path (gfx950, FP64).
 //
 // Each function restates one reference leaf (cited file:line in the
-// reference repository) with the reference's floating-point operation order,
-// so that the IEEE build (-ffp-contract=off) differs from the reference only
-// through the device erfc/exp/sqrt.  The Maxwellian moment tables of the
-// reference (MMDF2d, 3 x 7x7x3 doubles) are not materialised: an entry
-// (i,j,k) is the product uplus[i]*vwhole[j]*xi_k formed where it is used,
-// which gives the same value the table holds and keeps the working set in
-// registers (4 length-7 vectors per Maxwellian).
+// reference repository) with the reference's floating-point operation order.
+// Two builds include this header:
+//  * GKS_FAST == 0 (kernels_ieee.hip, -ffp-contract=off): every division of
+//    the reference stays a division, so the result differs from the CPU
+//    reference only through the device erfc/exp/sqrt;
+//  * GKS_FAST == 1 (kernels_fast.hip, -ffp-contract=fast-honor-pragmas): divisions by
+//    constants, by the cell size and by a denominator shared by several
+//    quotients become multiplications by a reciprocal formed once (each
+//    quotient moves by at most an ulp or two); FMAs are contracted.
+// The functions live in a namespace named after the build so the two
+// definitions never meet at link time.
+//
+// The Maxwellian moment tables of the reference (MMDF2d, 3 x 7x7x3 doubles)
+// are not materialised: an entry (i,j,k) is the product uplus[i]*vwhole[j]*xi_k
+// formed where it is used, which gives the same value the table holds and
+// keeps the working set in registers (4 length-7 vectors per Maxwellian).
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include "gks2d_types.hpp"
+
+#ifndef GKS_VARIANT
+#error "GKS_VARIANT must name the kernel build"
+#endif
+#ifndef GKS_FAST
+#error "GKS_FAST must be 0 or 1"
+#endif
+
 namespace gksd {
+inline namespace GKS_VARIANT {
 
-struct Gas {
-  int K;
-  double gamma, mu, nu, pr, c1, c2;
-  int tau_type;
-  int prandtl_fix;
-};
-
+#ifndef GKS_DI
 #define GKS_DI __device__ __forceinline__
+#endif
+// a / b where b is a compile-time constant
+#if GKS_FAST
+#define GKS_CDIV(a, b) ((a) * (1.0 / (b)))
+#else
+#define GKS_CDIV(a, b) ((a) / (b))
+#endif
 
 // ---------------------------------------------------------------- gas algebra
 // Convar_to_primvar_2D: basic_function.cpp:811-817 (+ U/V/Pressure :1137-1160)
 GKS_DI void convar_to_primvar(double* p, const double* c, const Gas& g) {
   p[0] = c[0];
+#if GKS_FAST
+  const double ir = 1.0 / c[0];
+  p[1] = c[1] * ir;
+  p[2] = c[2] * ir;
+  p[3] = (g.gamma - 1.0) * (c[3] - 0.5 * c[1] * c[1] * ir - 0.5 * c[2] * c[2] * ir);
+#else
   p[1] = c[1] / c[0];
   p[2] = c[2] / c[0];
   p[3] = (g.gamma - 1.0) * (c[3] - 0.5 * c[1] * c[1] / c[0] - 0.5 * c[2] * c[2] / c[0]);
+#endif
 }
 // Lambda (2D): basic_function.cpp:1174-1177
 GKS_DI double lambda2(double rho, double u, double v, double rhoE, const Gas& g) {
@@ -37,8 +64,14 @@ GKS_DI double lambda2(double rho, double u, double v, double rhoE, const Gas& g)
 // Convar_to_ULambda_2d: basic_function.cpp:835-842
 GKS_DI void convar_to_ulambda(double* p, const double* c, const Gas& g) {
   p[0] = c[0];
+#if GKS_FAST
+  const double ir = 1.0 / c[0];
+  p[1] = c[1] * ir;
+  p[2] = c[2] * ir;
+#else
   p[1] = c[1] / c[0];
   p[2] = c[2] / c[0];
+#endif
   p[3] = lambda2(c[0], p[1], p[2], c[3], g);
 }
 
@@ -87,7 +120,12 @@ GKS_DI void from_char(double* c, const CharBase& b, const double* ch, const Gas&
   const double s0[4] = {1., 0.0, 1.0, 1.0};
   const double s1[4] = {u, 0., u + cs, u - cs};
   const double s2[4] = {v, 1., v, v};
-  const double s3[4] = {hq, v, hq + cs * u + cs * cs / (g.gamma - 1.0), hq - cs * u + cs * cs / (g.gamma - 1.0)};
+#if GKS_FAST
+  const double h0 = cs * cs * g.inv_gm1;
+#else
+  const double h0 = cs * cs / (g.gamma - 1.0);
+#endif
+  const double s3[4] = {hq, v, hq + cs * u + h0, hq - cs * u + h0};
   double a;
   a = 0.0;
 #pragma unroll
@@ -397,15 +435,22 @@ GKS_DI void weno5_weights(double* fw, double wn2, double wn1, double w0, double 
     sum += alpha[i];
   }
   double ww[4];
+#if GKS_FAST
+  const double isum = 1.0 / sum;
+#pragma unroll
+  for (int k = 0; k < 4; k++) ww[k] = alpha[k] * isum;
+#else
 #pragma unroll
   for (int k = 0; k < 4; k++) ww[k] = alpha[k] / sum;
-  fw[3] = ww[3] / d[3];
+#endif
+  const double w3d = GKS_CDIV(ww[3], d[3]);
+  fw[3] = w3d;
 #pragma unroll
-  for (int k = 0; k < 3; k++) fw[k] = ww[k] - ww[3] / d[3] * d[k];
+  for (int k = 0; k < 3; k++) fw[k] = ww[k] - w3d * d[k];
 }
 // weno_5th_ao_with_df_left: reconstruction.cpp:480-567
 GKS_DI void weno5_left(double& var, double& der1, double wn2, double wn1, double w0, double wp1, double wp2,
-                       const double* df, double h) {
+                       const double* df, double h, double ih) {
   double fw[4];
   weno5_weights(fw, wn2, wn1, w0, wp1, wp2);
   double p[4], px[4];
@@ -413,10 +458,18 @@ GKS_DI void weno5_left(double& var, double& der1, double wn2, double wn1, double
   p[1] = w0 + df[1] * (1.0 / 3.0 * wn1 + 5.0 / 6.0 * w0 - 1.0 / 6.0 * wp1 - w0);
   p[2] = w0 + df[2] * (11.0 / 6.0 * w0 - 7.0 / 6.0 * wp1 + 1.0 / 3.0 * wp2 - w0);
   p[3] = w0 + df[3] * ((1.0 / 60.0) * (47.0 * w0 + 27.0 * wn1 - 3.0 * wn2 - 13.0 * wp1 + 2.0 * wp2) - w0);
+#if GKS_FAST
+  px[0] = df[0] * (w0 - wn1) * ih;
+  px[1] = df[1] * (w0 - wn1) * ih;
+  px[2] = -df[2] * ((2.0 * w0 - 3.0 * wp1 + wp2) * ih);
+  px[3] = df[3] * (15.0 * w0 - 15.0 * wn1 + wn2 - wp1) * (ih * (1.0 / 12.0));
+#else
+  (void)ih;
   px[0] = df[0] * (w0 - wn1) / h;
   px[1] = df[1] * (w0 - wn1) / h;
   px[2] = -df[2] * ((2.0 * w0 - 3.0 * wp1 + wp2) / h);
   px[3] = df[3] * (15.0 * w0 - 15.0 * wn1 + wn2 - wp1) / (12.0 * h);
+#endif
   var = 0.0;
   der1 = 0.0;
 #pragma unroll
@@ -427,7 +480,7 @@ GKS_DI void weno5_left(double& var, double& der1, double wn2, double wn1, double
 }
 // weno_5th_ao_with_df_right: reconstruction.cpp:569-657
 GKS_DI void weno5_right(double& var, double& der1, double wn2, double wn1, double w0, double wp1, double wp2,
-                        const double* df, double h) {
+                        const double* df, double h, double ih) {
   double fw[4];
   weno5_weights(fw, wn2, wn1, w0, wp1, wp2);
   double p[4], px[4];
@@ -435,10 +488,18 @@ GKS_DI void weno5_right(double& var, double& der1, double wn2, double wn1, doubl
   p[1] = w0 + df[1] * (-1.0 / 6.0 * wn1 + 5.0 / 6.0 * w0 + 1.0 / 3.0 * wp1 - w0);
   p[2] = w0 + df[2] * (1.0 / 3.0 * w0 + 5.0 / 6.0 * wp1 - 1.0 / 6.0 * wp2 - w0);
   p[3] = w0 + df[3] * ((1.0 / 60.0) * (47.0 * w0 - 13.0 * wn1 + 2.0 * wn2 + 27.0 * wp1 - 3.0 * wp2) - w0);
+#if GKS_FAST
+  px[0] = df[0] * (2.0 * w0 - 3.0 * wn1 + wn2) * ih;
+  px[1] = df[1] * (-w0 + wp1) * ih;
+  px[2] = df[2] * (-w0 + wp1) * ih;
+  px[3] = df[3] * (-15.0 * w0 + wn1 + 15.0 * wp1 - wp2) * (ih * (1.0 / 12.0));
+#else
+  (void)ih;
   px[0] = df[0] * (2.0 * w0 - 3.0 * wn1 + wn2) / h;
   px[1] = df[1] * (-w0 + wp1) / h;
   px[2] = df[2] * (-w0 + wp1) / h;
   px[3] = df[3] * (-15.0 * w0 + wn1 + 15.0 * wp1 - wp2) / (12.0 * h);
+#endif
   var = 0.0;
   der1 = 0.0;
 #pragma unroll
@@ -449,21 +510,36 @@ GKS_DI void weno5_right(double& var, double& der1, double wn2, double wn1, doubl
 }
 // weno_5th_ao_with_df_2gauss: reconstruction.cpp:2742-2838
 GKS_DI void weno5_2gauss(double& g1, double& g1x, double& g2, double& g2x, const double* df, double wn2,
-                         double wn1, double w0, double wp1, double wp2, double h) {
+                         double wn1, double w0, double wp1, double wp2, double h, double ih) {
   double fw[4];
   weno5_weights(fw, wn2, wn1, w0, wp1, wp2);
   const double sqrt3 = 1.732050807568877294;
+  // the reference's divisors: divisions in the IEEE build, multiplications
+  // by reciprocals formed once in the fast build
+#if GKS_FAST
+  const double r6h = ih * (1.0 / 6.0), r108h = ih * (1.0 / 108.0);
+#define GKS_D4S3 *(1.0 / (4 * sqrt3))
+#define GKS_D4320 *(1.0 / 4320)
+#define GKS_D6H *r6h
+#define GKS_D108H *r108h
+#else
+  (void)ih;
+#define GKS_D4S3 / (4 * sqrt3)
+#define GKS_D4320 / 4320
+#define GKS_D6H / (6 * h)
+#define GKS_D108H / (108 * h)
+#endif
   double p[4], px[4];
-  p[0] = w0 + df[0] * (w0 - (sqrt3 * w0) / 4 + (4 * wn1 - wn2) / (4 * sqrt3) - w0);
-  p[1] = w0 + df[1] * (w0 + (wn1 - wp1) / (4 * sqrt3) - w0);
+  p[0] = w0 + df[0] * (w0 - (sqrt3 * w0) / 4 + (4 * wn1 - wn2) GKS_D4S3 - w0);
+  p[1] = w0 + df[1] * (w0 + (wn1 - wp1) GKS_D4S3 - w0);
   p[2] = w0 + df[2] * ((1.0 / 12.0) * (3 * (4 + sqrt3) * w0 + sqrt3 * (-4 * wp1 + wp2)) - w0);
   p[3] = w0 + df[3] * ((4314 * w0 + (4 + 500 * sqrt3) * wn1 - wn2 - 70 * sqrt3 * wn2 + 4 * wp1 -
-                        500 * sqrt3 * wp1 - wp2 + 70 * sqrt3 * wp2) / 4320 - w0);
-  px[0] = -df[0] * ((-9 + sqrt3) * w0 - 2 * (-6 + sqrt3) * wn1 + (-3 + sqrt3) * wn2) / (6 * h);
-  px[1] = -df[1] * (-2 * sqrt3 * w0 + (3 + sqrt3) * wn1 + (-3 + sqrt3) * wp1) / (6 * h);
-  px[2] = -df[2] * ((9 + sqrt3) * w0 - 2 * (6 + sqrt3) * wp1 + (3 + sqrt3) * wp2) / (6 * h);
+                        500 * sqrt3 * wp1 - wp2 + 70 * sqrt3 * wp2) GKS_D4320 - w0);
+  px[0] = -df[0] * ((-9 + sqrt3) * w0 - 2 * (-6 + sqrt3) * wn1 + (-3 + sqrt3) * wn2) GKS_D6H;
+  px[1] = -df[1] * (-2 * sqrt3 * w0 + (3 + sqrt3) * wn1 + (-3 + sqrt3) * wp1) GKS_D6H;
+  px[2] = -df[2] * ((9 + sqrt3) * w0 - 2 * (6 + sqrt3) * wp1 + (3 + sqrt3) * wp2) GKS_D6H;
   px[3] = df[3] * (48 * sqrt3 * w0 - 72 * wn1 - 26 * sqrt3 * wn1 + 9 * wn2 + 2 * sqrt3 * wn2 + 72 * wp1 -
-                   26 * sqrt3 * wp1 - 9 * wp2 + 2 * sqrt3 * wp2) / (108 * h);
+                   26 * sqrt3 * wp1 - 9 * wp2 + 2 * sqrt3 * wp2) GKS_D108H;
   g1 = 0.0;
   g1x = 0.0;
 #pragma unroll
@@ -472,15 +548,15 @@ GKS_DI void weno5_2gauss(double& g1, double& g1x, double& g2, double& g2x, const
     g1x += fw[k] * px[k];
   }
   p[0] = w0 + df[0] * ((1.0 / 12.0) * (3 * (4 + sqrt3) * w0 + sqrt3 * (-4 * wn1 + wn2)) - w0);
-  p[1] = w0 + df[1] * (w0 + (-wn1 + wp1) / (4 * sqrt3) - w0);
-  p[2] = w0 + df[2] * (w0 - (sqrt3 * w0) / 4 + (4 * wp1 - wp2) / (4 * sqrt3) - w0);
+  p[1] = w0 + df[1] * (w0 + (-wn1 + wp1) GKS_D4S3 - w0);
+  p[2] = w0 + df[2] * (w0 - (sqrt3 * w0) / 4 + (4 * wp1 - wp2) GKS_D4S3 - w0);
   p[3] = w0 + df[3] * ((4314 * w0 + (4 - 500 * sqrt3) * wn1 - wn2 + 70 * sqrt3 * wn2 + 4 * wp1 +
-                        500 * sqrt3 * wp1 - wp2 - 70 * sqrt3 * wp2) / 4320 - w0);
-  px[0] = df[0] * ((9 + sqrt3) * w0 - 2 * (6 + sqrt3) * wn1 + (3 + sqrt3) * wn2) / (6 * h);
-  px[1] = df[1] * (-2 * sqrt3 * w0 + (-3 + sqrt3) * wn1 + (3 + sqrt3) * wp1) / (6 * h);
-  px[2] = df[2] * ((-9 + sqrt3) * w0 - 2 * (-6 + sqrt3) * wp1 + (-3 + sqrt3) * wp2) / (6 * h);
+                        500 * sqrt3 * wp1 - wp2 - 70 * sqrt3 * wp2) GKS_D4320 - w0);
+  px[0] = df[0] * ((9 + sqrt3) * w0 - 2 * (6 + sqrt3) * wn1 + (3 + sqrt3) * wn2) GKS_D6H;
+  px[1] = df[1] * (-2 * sqrt3 * w0 + (-3 + sqrt3) * wn1 + (3 + sqrt3) * wp1) GKS_D6H;
+  px[2] = df[2] * ((-9 + sqrt3) * w0 - 2 * (-6 + sqrt3) * wp1 + (-3 + sqrt3) * wp2) GKS_D6H;
   px[3] = -df[3] * ((48 * sqrt3 * w0 + 72 * wn1 - 26 * sqrt3 * wn1 - 9 * wn2 + 2 * sqrt3 * wn2 - 72 * wp1 -
-                     26 * sqrt3 * wp1 + 9 * wp2 + 2 * sqrt3 * wp2) / (108 * h));
+                     26 * sqrt3 * wp1 + 9 * wp2 + 2 * sqrt3 * wp2) GKS_D108H);
   g2 = 0.0;
   g2x = 0.0;
 #pragma unroll
@@ -489,6 +565,10 @@ GKS_DI void weno5_2gauss(double& g1, double& g1x, double& g2, double& g2x, const
     g2x += fw[k] * px[k];
   }
 }
+#undef GKS_D4S3
+#undef GKS_D4320
+#undef GKS_D6H
+#undef GKS_D108H
 
 // DF factors: reconstruction.cpp:1733-1742
 GKS_DI void df5(double* df, const double* a) {
@@ -514,4 +594,5 @@ GKS_DI double alpha_k(const double* pl, const double* pr, const Gas& g) {
   return A;
 }
 
+}  // namespace GKS_VARIANT
 }  // namespace gksd

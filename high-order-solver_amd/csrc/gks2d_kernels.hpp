@@ -1,7 +1,8 @@
 // gks2d_kernels.hpp — HIP kernels of the 2D GKS hot path and their launchers.
 //
-// Included by kernels_ieee.hip (built with -ffp-contract=off) and
-// kernels_contracted.hip (-ffp-contract=fast); GKS_VARIANT names the namespace.
+// Included by kernels_ieee.hip (GKS_FAST 0, -ffp-contract=off) and
+// kernels_fast.hip (GKS_FAST 1, -ffp-contract=fast-honor-pragmas); GKS_VARIANT names the
+// namespace of each build.
 //
 // Device layout (all FP64, SoA, y fastest as in the reference):
 //   U, Uold      [4][nx*ny]          conservative state, cell i*ny+j
@@ -155,7 +156,7 @@ __global__ void __launch_bounds__(kBlock) k_copy_old(Dev d) {
 // WENO5_AO_with_df (reconstruction.cpp:1731-1859) for one cell along DIR;
 // the stencil is already in the x-like frame (YchangetoX for y).
 __device__ __forceinline__ void weno5_normal(double* lo, double* ro, const double (*w)[4], const double* alpha,
-                                             double h, int recon_var, const Gas& g) {
+                                             double h, double ih, int recon_var, const Gas& g) {
   double df[4];
   df5(df, alpha);
   double pn1[4], p0[4], pp1[4];
@@ -182,8 +183,8 @@ __device__ __forceinline__ void weno5_normal(double* lo, double* ro, const doubl
     double var[4], d1[4];
 #pragma unroll
     for (int i = 0; i < 4; i++) {
-      if (side == 0) weno5_left(var[i], d1[i], c[0][i], c[1][i], c[2][i], c[3][i], c[4][i], df, h);
-      else weno5_right(var[i], d1[i], c[0][i], c[1][i], c[2][i], c[3][i], c[4][i], df, h);
+      if (side == 0) weno5_left(var[i], d1[i], c[0][i], c[1][i], c[2][i], c[3][i], c[4][i], df, h, ih);
+      else weno5_right(var[i], d1[i], c[0][i], c[1][i], c[2][i], c[3][i], c[4][i], df, h, ih);
     }
     double* out = side == 0 ? lo : ro;
     if (recon_var == 0) {
@@ -245,7 +246,7 @@ template <int DIR> __global__ void __launch_bounds__(kBlock) k_normal(Dev d) {
     }
   }
   double lo[8], ro[8];
-  weno5_normal(lo, ro, w, a, DIR == 0 ? d.dx : d.dy, d.recon_var, d.gas);
+  weno5_normal(lo, ro, w, a, DIR == 0 ? d.dx : d.dy, DIR == 0 ? d.idx : d.idy, d.recon_var, d.gas);
   double* line = DIR == 0 ? d.line_x : d.line_y;
   const long long nf = DIR == 0 ? d.nfx : d.nfy;
   const long long fa = DIR == 0 ? (long long)i * d.ny + j : (long long)i * (d.ny + 1) + j;
@@ -261,14 +262,14 @@ template <int DIR> __global__ void __launch_bounds__(kBlock) k_normal(Dev d) {
 // weno_5th_ao_with_df_tangential, gausspoint 2 (reconstruction.cpp:2578-2740)
 // for one side of one face.  line[5][8]: convar, der1x of the 5 faces.
 __device__ __forceinline__ void weno5_tangent(double (*gp)[12], const double (*line)[8], const double* alpha,
-                                              double h, int recon_var, const Gas& g) {
+                                              double h, double ih, int recon_var, const Gas& g) {
   double df[4];
   df5(df, alpha);
   if (recon_var == 0) {
 #pragma unroll
     for (int i = 0; i < 4; i++)
       weno5_2gauss(gp[0][i], gp[0][8 + i], gp[1][i], gp[1][8 + i], df, line[0][i], line[1][i], line[2][i],
-                   line[3][i], line[4][i], h);
+                   line[3][i], line[4][i], h, ih);
   } else {
     double base[4];
     convar_to_primvar(base, line[2], g);
@@ -280,7 +281,8 @@ __device__ __forceinline__ void weno5_tangent(double (*gp)[12], const double (*l
     double var[2][4], d1[2][4];
 #pragma unroll
     for (int i = 0; i < 4; i++)
-      weno5_2gauss(var[0][i], d1[0][i], var[1][i], d1[1][i], df, c[0][i], c[1][i], c[2][i], c[3][i], c[4][i], h);
+      weno5_2gauss(var[0][i], d1[0][i], var[1][i], d1[1][i], df, c[0][i], c[1][i], c[2][i], c[3][i], c[4][i], h,
+                   ih);
 #pragma unroll
     for (int q = 0; q < 2; q++) {
       from_char(gp[q], cb, var[q], g);
@@ -291,7 +293,7 @@ __device__ __forceinline__ void weno5_tangent(double (*gp)[12], const double (*l
   for (int i = 0; i < 4; i++) {
     double t0, t1;
     weno5_2gauss(gp[0][4 + i], t0, gp[1][4 + i], t1, df, line[0][4 + i], line[1][4 + i], line[2][4 + i],
-                 line[3][4 + i], line[4][4 + i], h);
+                 line[3][4 + i], line[4][4 + i], h, ih);
   }
 #pragma unroll
   for (int q = 0; q < 2; q++) {
@@ -339,7 +341,7 @@ template <int DIR> __global__ void __launch_bounds__(kBlock) k_tangent(Dev d) {
     for (int k = 0; k < 8; k++) L[s][k] = line[(side * 8 + k) * nf + f];
   }
   double gp[2][12];
-  weno5_tangent(gp, L, a, DIR == 0 ? d.dy : d.dx, d.recon_var, d.gas);
+  weno5_tangent(gp, L, a, DIR == 0 ? d.dy : d.dx, DIR == 0 ? d.idy : d.idx, d.recon_var, d.gas);
   double* gst = DIR == 0 ? d.gst_x : d.gst_y;
   const long long f0 = DIR == 0 ? (long long)i * d.ny + j : (long long)i * (d.ny + 1) + j;
 #pragma unroll

@@ -3,11 +3,12 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
-#include "gks2d_device.hpp"
+#include "gks2d_types.hpp"
 
 struct Dev {
   int NX, NY, G, nx, ny, gp, recon_var, use_df;
   double dx, dy, area;
+  double idx, idy;  // 1/dx, 1/dy (fast build)
   double gw[4];
   gksd::Gas gas;
   long long nc, nfx, nfy;
@@ -47,4 +48,4 @@ struct LaunchTable {
 };
 
 namespace ieee { extern const LaunchTable table; }
-namespace contracted { extern const LaunchTable table; }
+namespace fast { extern const LaunchTable table; }

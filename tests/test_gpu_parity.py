@@ -33,7 +33,7 @@ GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 G = 5
 FLUX_RTOL = 1e-10
 L1_TOL = 1e-9
-VARIANTS = [0, 1]  # 0: IEEE kernels (no FMA contraction), 1: contracted
+VARIANTS = [0, 1]  # 0: IEEE kernels, 1: fast kernels (FMA contraction, reciprocal multiplication)
 
 
 def say(*a):
