@@ -89,7 +89,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--n", type=int, default=2048)
     ap.add_argument("--seed", type=int, default=0)
-    ap.add_argument("--variant", choices=["ieee", "fast"], default="ieee")
+    ap.add_argument("--variant", choices=["ieee", "fast"], default="fast")
     ap.add_argument("--cpu-n", type=int, default=2048)
     ap.add_argument("--cpu-steps", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")

@@ -72,6 +72,7 @@ struct gks2d_ctx {
   std::vector<void*> allocs;
   int last_slot = -1;        // slot of the last primvar / set_cfl_primvar
   int last_flux_stage = -1;
+  int kept_stage_mask = 0;   // stages whose raw face fluxes are stored
   // per-phase HIP-event timing (gks2d_timing)
   bool timing = false;
   struct Rec {
@@ -203,8 +204,8 @@ int gks2d_create(const gks2d_params* p, gks2d_ctx** out) {
       (rc = dev_alloc(c, &d.ax, d.nc)) || (rc = dev_alloc(c, &d.ay, d.nc)) ||
       (rc = dev_alloc(c, &d.line_x, 16 * d.nfx)) || (rc = dev_alloc(c, &d.line_y, 16 * d.nfy)) ||
       (rc = dev_alloc(c, &d.gst_x, 24 * gp * d.nfx)) || (rc = dev_alloc(c, &d.gst_y, 24 * gp * d.nfy)) ||
-      (rc = dev_alloc(c, &d.flx[0], 8 * gp * d.nfx)) || (rc = dev_alloc(c, &d.flx[1], 8 * gp * d.nfx)) ||
-      (rc = dev_alloc(c, &d.fly[0], 8 * gp * d.nfy)) || (rc = dev_alloc(c, &d.fly[1], 8 * gp * d.nfy)) ||
+      (rc = dev_alloc(c, &d.xcx, 4 * gp * d.nfx)) || (rc = dev_alloc(c, &d.xcy, 4 * gp * d.nfy)) ||
+      (rc = dev_alloc(c, &d.pcx, 4 * gp * d.nfx)) || (rc = dev_alloc(c, &d.pcy, 4 * gp * d.nfy)) ||
       (rc = dev_alloc(c, &d.akx, gp * d.nfx)) || (rc = dev_alloc(c, &d.aky, gp * d.nfy)) ||
       (rc = dev_alloc(c, &d.partial, 2 * 1024)) || (rc = dev_alloc(c, &d.stat, 8)) ||
       (rc = dev_alloc(c, &c->d_prim, 4 * d.nc))) {
@@ -305,6 +306,7 @@ int gks2d_flux(gks2d_ctx* c, int stage, double dt) {
   RC(timed(c, PH_FLUX_X, [&] { return c->lt->flux(c->d, 0, stage, dt, c->s); }));
   RC(timed(c, PH_FLUX_Y, [&] { return c->lt->flux(c->d, 1, stage, dt, c->s); }));
   c->last_flux_stage = stage;
+  if (c->d.keep_face_fluxes) c->kept_stage_mask |= 1 << stage;
   return GKS_OK;
 }
 int gks2d_update(gks2d_ctx* c, int stage) {
@@ -447,9 +449,26 @@ int gks2d_run(gks2d_ctx* c, const int32_t bc[4], double tstop, int32_t max_steps
   return GKS_OK;
 }
 
+int gks2d_keep_face_fluxes(gks2d_ctx* c, int on) {
+  if (!c) return fail(GKS_ERR_ARG, "null");
+  Dev& d = c->d;
+  if (on && !d.flx[0]) {
+    const int gp = d.gp;
+    int rc;
+    if ((rc = dev_alloc(c, &d.flx[0], 8 * gp * d.nfx)) || (rc = dev_alloc(c, &d.flx[1], 8 * gp * d.nfx)) ||
+        (rc = dev_alloc(c, &d.fly[0], 8 * gp * d.nfy)) || (rc = dev_alloc(c, &d.fly[1], 8 * gp * d.nfy)))
+      return rc;
+  }
+  d.keep_face_fluxes = on ? 1 : 0;
+  c->kept_stage_mask = 0;
+  return GKS_OK;
+}
+
 int gks2d_get_face_fluxes(gks2d_ctx* c, double* xf, double* yf) {
   if (!c || !xf || !yf) return fail(GKS_ERR_ARG, "null");
   if (c->last_flux_stage < 0) return fail(GKS_ERR_STATE, "no flux computed yet");
+  if (!(c->kept_stage_mask & (1 << c->last_flux_stage)))
+    return fail(GKS_ERR_STATE, "face fluxes of the last stage were not kept: call gks2d_keep_face_fluxes(ctx, 1) first");
   const Dev& d = c->d;
   const int st = c->last_flux_stage, gp = d.gp;
   std::vector<double> h(8 * gp * d.nfx);

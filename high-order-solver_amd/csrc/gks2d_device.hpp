@@ -259,57 +259,47 @@ GKS_DI void time_coe(double* t, double tau, double tn, double dt) {
 // (flux_function.cpp:345-698).  L/R: convar[0..3], der1x[4..7], der1y[8..11]
 // in the face-local frame.  Writes f[4], derf[4]; optionally the centre.
 //
-// Register plan: every moment contraction that needs the left/right
-// Maxwellians is formed first (they die before the centre Maxwellian is
-// built); F(dt) and F(dt/2) are then summed from the stored 4-vectors, each in
-// the reference's own term order (flux_function.cpp:443-629 and 650-684).
+// Register plan: every quantity that involves the left/right Maxwellians is
+// a sum (left term) + (right term) or a single-side term, so the left side
+// is evaluated with only its moments live, then the right side; the centre
+// Maxwellian comes after both.  F(dt) and F(dt/2) are summed from the stored
+// 4-vectors, each in the reference's own term order
+// (flux_function.cpp:443-629 and 650-684).
 GKS_DI void gks_point(double* f, double* derf, double* center, const double* L, const double* Rr,
                       double dt, const Gas& g) {
   double pl[4], pr[4];
   convar_to_ulambda(pl, L, g);
   convar_to_ulambda(pr, Rr, g);
+  const double unit[4] = {1.0, 0.0, 0.0, 0.0};
   double c0[4], c0x[4], c0y[4];
   double glu[4], gru[4], s1[4], Alul[4], Arur[4];
+  // ---- left state: u > 0 half moments
   {
-    MM ml, mr;
+    MM ml;
     mmdf(ml, pl[1], pl[2], pl[3], g);
-    mmdf(mr, pr[1], pr[2], pr[3], g);
     const TabP tl{ml};
-    const TabN tr{mr};
-    const TabW wl{ml}, wr{mr};
-    // centre state (Collision, basic_function.cpp:561-569)
-    c0[0] = pl[0] * ml.up[0] + pr[0] * mr.un[0];
-    c0[1] = pl[0] * ml.up[1] + pr[0] * mr.un[1];
-    c0[2] = pl[0] * ml.vw[1] * ml.up[0] + pr[0] * mr.vw[1] * mr.un[0];
-    c0[3] = 0.5 * pl[0] * (ml.up[2] + ml.up[0] * ml.vw[2] + ml.up[0] * ml.xi2) +
-            0.5 * pr[0] * (mr.un[2] + mr.un[0] * mr.vw[2] + mr.un[0] * mr.xi2);
-    double alx[4], arx[4], aly[4], ary[4];
+    const TabW wl{ml};
+    // left terms of Collision (basic_function.cpp:561-569)
+    c0[0] = pl[0] * ml.up[0];
+    c0[1] = pl[0] * ml.up[1];
+    c0[2] = pl[0] * ml.vw[1] * ml.up[0];
+    c0[3] = 0.5 * pl[0] * (ml.up[2] + ml.up[0] * ml.vw[2] + ml.up[0] * ml.xi2);
+    double alx[4], aly[4], t4[4];
     micro_slope(alx, L + 4, pl, g);
-    micro_slope(arx, Rr + 4, pr, g);
     micro_slope(aly, L + 8, pl, g);
-    micro_slope(ary, Rr + 8, pr, g);
-    {
-      double al0[4], ar0[4];
-      contract(0, 0, 0, al0, alx, tl);
-      contract(0, 0, 0, ar0, arx, tr);
+    contract(0, 0, 0, t4, alx, tl);
 #pragma unroll
-      for (int k = 0; k < 4; k++) c0x[k] = pl[0] * al0[k] + pr[0] * ar0[k];
-      contract(0, 0, 0, al0, aly, tl);
-      contract(0, 0, 0, ar0, ary, tr);
+    for (int k = 0; k < 4; k++) c0x[k] = pl[0] * t4[k];
+    contract(0, 0, 0, t4, aly, tl);
 #pragma unroll
-      for (int k = 0; k < 4; k++) c0y[k] = pl[0] * al0[k] + pr[0] * ar0[k];
-    }
-    const double unit[4] = {1.0, 0.0, 0.0, 0.0};
+    for (int k = 0; k < 4; k++) c0y[k] = pl[0] * t4[k];
     contract(1, 0, 0, glu, unit, tl);
-    contract(1, 0, 0, gru, unit, tr);
     {
-      double a[4], b[4], c[4], d[4];
+      double a[4], c[4];
       contract(2, 0, 0, a, alx, tl);
-      contract(2, 0, 0, b, arx, tr);
       contract(1, 1, 0, c, aly, tl);
-      contract(1, 1, 0, d, ary, tr);
 #pragma unroll
-      for (int i = 0; i < 4; i++) s1[i] = pl[0] * (a[i] + c[i]) + pr[0] * (b[i] + d[i]);
+      for (int i = 0; i < 4; i++) s1[i] = pl[0] * (a[i] + c[i]);
     }
     {
       double xu[4], yv[4], der[4], A[4];
@@ -319,6 +309,40 @@ GKS_DI void gks_point(double* f, double* derf, double* center, const double* L, 
       for (int i = 0; i < 4; i++) der[i] = -pl[0] * (xu[i] + yv[i]);
       micro_slope(A, der, pl, g);
       contract(1, 0, 0, Alul, A, tl);
+    }
+  }
+  // keep the scheduler from interleaving the two sides (it would raise the
+  // number of live registers)
+  __builtin_amdgcn_sched_barrier(0);
+  // ---- right state: u < 0 half moments
+  {
+    MM mr;
+    mmdf(mr, pr[1], pr[2], pr[3], g);
+    const TabN tr{mr};
+    const TabW wr{mr};
+    c0[0] = c0[0] + pr[0] * mr.un[0];
+    c0[1] = c0[1] + pr[0] * mr.un[1];
+    c0[2] = c0[2] + pr[0] * mr.vw[1] * mr.un[0];
+    c0[3] = c0[3] + 0.5 * pr[0] * (mr.un[2] + mr.un[0] * mr.vw[2] + mr.un[0] * mr.xi2);
+    double arx[4], ary[4], t4[4];
+    micro_slope(arx, Rr + 4, pr, g);
+    micro_slope(ary, Rr + 8, pr, g);
+    contract(0, 0, 0, t4, arx, tr);
+#pragma unroll
+    for (int k = 0; k < 4; k++) c0x[k] = c0x[k] + pr[0] * t4[k];
+    contract(0, 0, 0, t4, ary, tr);
+#pragma unroll
+    for (int k = 0; k < 4; k++) c0y[k] = c0y[k] + pr[0] * t4[k];
+    contract(1, 0, 0, gru, unit, tr);
+    {
+      double b[4], d[4];
+      contract(2, 0, 0, b, arx, tr);
+      contract(1, 1, 0, d, ary, tr);
+#pragma unroll
+      for (int i = 0; i < 4; i++) s1[i] = s1[i] + pr[0] * (b[i] + d[i]);
+    }
+    {
+      double xu[4], yv[4], der[4], A[4];
       contract(1, 0, 0, xu, arx, wr);
       contract(0, 1, 0, yv, ary, wr);
 #pragma unroll
@@ -336,6 +360,7 @@ GKS_DI void gks_point(double* f, double* derf, double* center, const double* L, 
     }
   }
 
+  __builtin_amdgcn_sched_barrier(0);
   // equilibrium (centre) part
   double p0[4];
   convar_to_ulambda(p0, c0, g);

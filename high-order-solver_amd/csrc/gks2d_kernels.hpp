@@ -11,7 +11,11 @@
 //                                    x face (i,j) = i*ny+j, i in [0,nx]
 //   line_y       [2][8][nfy]         y face (i,j) = i*(ny+1)+j, j in [0,ny]
 //   gst_x/_y     [2][gp][12][nf]     Gauss-point states (convar,der1x,der1y)
-//   flx/fly[s]   [gp][8][nf]         f[4], derf[4] of stage s (face frame)
+//   xcx/xcy      [gp][4][nf]         stage-combined flux of the current stage,
+//                                    weighted and rotated to the global frame
+//   pcx/pcy      [gp][4][nf]         stage-1 share of the stage-0 flux
+//   flx/fly[s]   [gp][8][nf]         f[4], derf[4] of stage s (face frame),
+//                                    stored only when keep_face_fluxes is set
 //   akx/aky      [gp][nf]            Calculate_alpha_k per Gauss point
 // side 0 = state left of the face ("line.left"), 1 = right ("line.right").
 #pragma once
@@ -351,9 +355,18 @@ template <int DIR> __global__ void __launch_bounds__(kBlock) k_tangent(Dev d) {
 }
 
 // -------------------------------------------------------------------- flux
-// Centre state + GKS2D per (face, Gauss point), and Calculate_alpha_k of the
-// point (reconstruction.cpp:1466-1495; y faces back to the global frame with
+// Centre state + GKS2D per (face, Gauss point); the S2O4 combination of the
+// Gauss point's flux for this stage (time_advance.cpp:380-426, weighted by the
+// Gauss weight) rotated to the global frame (Local_to_Global,
+// basic_function.cpp:1226-1233), so the cell update reads one 4-vector per
+// (face, Gauss point); and Calculate_alpha_k of the point
+// (reconstruction.cpp:1466-1495; y faces back to the global frame with
 // XchangetoY first).
+//
+// S2O4_2D (time_advance.cpp:221-230): stage 0 uses 0.5 f0 + 1/8 derf0;
+// stage 1 uses (f0 + 1/6 derf0) + (0 f1 + 1/3 derf1).  Stage 0 stores the
+// first bracket (times the weight) for stage 1; the reference's running sum
+// starts from 0.0, which adds nothing.
 template <int DIR> __global__ void __launch_bounds__(kBlock) k_flux(Dev d, int stage, double dt) {
   long long t = gtid();
   const int ni = DIR == 0 ? d.NX + 1 : d.NX, nj = DIR == 0 ? d.NY : d.NY + 1;
@@ -372,12 +385,33 @@ template <int DIR> __global__ void __launch_bounds__(kBlock) k_flux(Dev d, int s
   }
   double fo[4], dfo[4];
   gks_point(fo, dfo, nullptr, L, R, dt, d.gas);
-  double* fl = DIR == 0 ? d.flx[stage] : d.fly[stage];
+  if (d.keep_face_fluxes) {
+    double* fl = DIR == 0 ? d.flx[stage] : d.fly[stage];
 #pragma unroll
-  for (int k = 0; k < 4; k++) {
-    fl[(q * 8 + k) * nf + f] = fo[k];
-    fl[(q * 8 + 4 + k) * nf + f] = dfo[k];
+    for (int k = 0; k < 4; k++) {
+      fl[(q * 8 + k) * nf + f] = fo[k];
+      fl[(q * 8 + 4 + k) * nf + f] = dfo[k];
+    }
   }
+  const double w = d.gw[q];
+  double x[4];
+  double* pc = DIR == 0 ? d.pcx : d.pcy;
+  if (stage == 0) {
+#pragma unroll
+    for (int v = 0; v < 4; v++) {
+      x[v] = w * (0.5 * fo[v] + 1.0 / 8.0 * dfo[v]);
+      pc[(q * 4 + v) * nf + f] = w * (1.0 * fo[v] + 1.0 / 6.0 * dfo[v]);
+    }
+  } else {
+#pragma unroll
+    for (int v = 0; v < 4; v++) x[v] = pc[(q * 4 + v) * nf + f] + w * (0.0 * fo[v] + 1.0 / 3.0 * dfo[v]);
+  }
+  double* xc = DIR == 0 ? d.xcx : d.xcy;
+  // normal (1,0): identity; normal (0,1): (x1, x2) -> (-x2, x1)
+  xc[(q * 4 + 0) * nf + f] = x[0];
+  xc[(q * 4 + 1) * nf + f] = DIR == 0 ? x[1] : -x[2];
+  xc[(q * 4 + 2) * nf + f] = DIR == 0 ? x[2] : x[1];
+  xc[(q * 4 + 3) * nf + f] = x[3];
   double pl[4], pr[4];
   convar_to_primvar(pl, L, d.gas);
   convar_to_primvar(pr, R, d.gas);
@@ -393,26 +427,10 @@ template <int DIR> __global__ void __launch_bounds__(kBlock) k_flux(Dev d, int s
 }
 
 // ------------------------------------------------------------------ update
-// S2O4 combination + Local_to_Global + cell update (time_advance.cpp:380-474)
-template <int STAGE>
-__device__ __forceinline__ void combined(double* x, const Dev& d, const double* const* fl, long long nf,
-                                         long long f, int q, bool yface) {
-  // S2O4_2D: c[0][0] = (0.5, 1/8); c[1][0] = (1, 1/6); c[1][1] = (0, 1/3)
-  const double c[2][2][2] = {{{0.5, 1.0 / 8.0}, {0.0, 0.0}}, {{1.0, 1.0 / 6.0}, {0.0, 1.0 / 3.0}}};
-#pragma unroll
-  for (int v = 0; v < 4; v++) {
-    double F = 0.0;
-#pragma unroll
-    for (int s = 0; s < STAGE + 1; s++)
-      F = F + d.gw[q] * (c[STAGE][s][0] * fl[s][(q * 8 + v) * nf + f] + c[STAGE][s][1] * fl[s][(q * 8 + 4 + v) * nf + f]);
-    x[v] = F;
-  }
-  const double n0 = yface ? 0.0 : 1.0, n1 = yface ? 1.0 : 0.0;
-  const double t0 = x[1], t1 = x[2];
-  x[1] = t0 * n0 - t1 * n1;
-  x[2] = t0 * n1 + t1 * n0;
-}
-template <int STAGE> __global__ void __launch_bounds__(kBlock) k_update(Dev d) {
+// cell update of Update_with_gauss (time_advance.cpp:442-474): U = U_old +
+// sum over Gauss points of len * (x_S - x_N + x_W - x_E), in the reference's
+// summation order, divided by the cell area
+__global__ void __launch_bounds__(kBlock) k_update(Dev d) {
   long long t = gtid();
   if (t >= (long long)d.NX * d.NY) return;
   const int i = (int)(t / d.NY) + d.G, j = (int)(t % d.NY) + d.G;
@@ -421,17 +439,13 @@ template <int STAGE> __global__ void __launch_bounds__(kBlock) k_update(Dev d) {
   const long long fl_ = (long long)i * d.ny + j, fr = (long long)(i + 1) * d.ny + j;
   double total[4] = {0.0, 0.0, 0.0, 0.0};
   for (int q = 0; q < d.gp; q++) {
-    double xb[4], xt[4], xl[4], xr[4];
-    combined<STAGE>(xb, d, d.fly, d.nfy, fb, q, true);
-    combined<STAGE>(xt, d, d.fly, d.nfy, ft, q, true);
-    combined<STAGE>(xl, d, d.flx, d.nfx, fl_, q, false);
-    combined<STAGE>(xr, d, d.flx, d.nfx, fr, q, false);
 #pragma unroll
     for (int v = 0; v < 4; v++) {
-      total[v] += d.dx * xb[v];
-      total[v] += -d.dx * xt[v];
-      total[v] += d.dy * xl[v];
-      total[v] += -d.dy * xr[v];
+      const long long oy = (long long)(q * 4 + v) * d.nfy, ox = (long long)(q * 4 + v) * d.nfx;
+      total[v] += d.dx * d.xcy[oy + fb];
+      total[v] += -d.dx * d.xcy[oy + ft];
+      total[v] += d.dy * d.xcx[ox + fl_];
+      total[v] += -d.dy * d.xcx[ox + fr];
     }
   }
 #pragma unroll
@@ -494,8 +508,6 @@ template __global__ void k_tangent<0>(Dev);
 template __global__ void k_tangent<1>(Dev);
 template __global__ void k_flux<0>(Dev, int, double);
 template __global__ void k_flux<1>(Dev, int, double);
-template __global__ void k_update<0>(Dev);
-template __global__ void k_update<1>(Dev);
 
 // ----------------------------------------------------------------- launchers
 #if !defined(__HIP_DEVICE_COMPILE__)
@@ -537,9 +549,8 @@ static hipError_t flux(const Dev& d, int dir, int stage, double dt, hipStream_t 
   return hipGetLastError();
 }
 static hipError_t update(const Dev& d, int stage, hipStream_t s) {
-  const unsigned nb = blocks_for((long long)d.NX * d.NY);
-  if (stage == 0) hipLaunchKernelGGL(k_update<0>, dim3(nb), dim3(kBlock), 0, s, d);
-  else hipLaunchKernelGGL(k_update<1>, dim3(nb), dim3(kBlock), 0, s, d);
+  (void)stage;  // the stage's combination is formed by k_flux
+  hipLaunchKernelGGL(k_update, dim3(blocks_for((long long)d.NX * d.NY)), dim3(kBlock), 0, s, d);
   return hipGetLastError();
 }
 static hipError_t alpha(const Dev& d, hipStream_t s) {

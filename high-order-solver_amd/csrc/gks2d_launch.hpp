@@ -7,6 +7,7 @@
 
 struct Dev {
   int NX, NY, G, nx, ny, gp, recon_var, use_df;
+  int keep_face_fluxes;  // store raw (f, derf) per face for parity checks
   double dx, dy, area;
   double idx, idy;  // 1/dx, 1/dy (fast build)
   double gw[4];
@@ -20,6 +21,10 @@ struct Dev {
   double* line_y;
   double* gst_x;
   double* gst_y;
+  double* xcx;
+  double* xcy;
+  double* pcx;
+  double* pcy;
   double* flx[2];
   double* fly[2];
   double* akx;

@@ -42,6 +42,7 @@ _SIGS = {
     "gks2d_sync": (ctypes.c_int, [_P]),
     "gks2d_flux_points": (ctypes.c_int, [ctypes.POINTER(Params), _I64, _D, _D, ctypes.c_double, _D, _D, _D]),
     "gks2d_get_face_fluxes": (ctypes.c_int, [_P, _D, _D]),
+    "gks2d_keep_face_fluxes": (ctypes.c_int, [_P, ctypes.c_int]),
     "gks2d_timing": (ctypes.c_int, [_P, ctypes.c_int]),
     "gks2d_get_timing": (ctypes.c_int, [_P, _D, ctypes.POINTER(ctypes.c_int64)]),
     "gks2d_last_error": (ctypes.c_char_p, []),
@@ -217,6 +218,9 @@ class Solver2D:
         n = (ctypes.c_int64 * len(PHASES))()
         _check(lib().gks2d_get_timing(self.h, _dp(ms), n))
         return {ph: (float(ms[k]), int(n[k])) for k, ph in enumerate(PHASES)}
+
+    def keep_face_fluxes(self, on=True):
+        _check(lib().gks2d_keep_face_fluxes(self.h, 1 if on else 0))
 
     def face_fluxes(self):
         xf = np.zeros((self.nx + 1, self.ny, self.gp, 8))

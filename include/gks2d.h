@@ -150,7 +150,14 @@ int gks2d_flux_points(const gks2d_params* p, int64_t n, const double* left,
                       const double* right, double dt, double* center, double* f,
                       double* derf);
 
-/* Per-face fluxes of the last gks2d_flux call, for parity checks:
+/* Parity support (no counterpart in the reference, whose face records are
+ * always on the host): with on=1, gks2d_flux also stores the raw f/derf of
+ * every (face, Gauss point) so gks2d_get_face_fluxes can return them.  Off by
+ * default: the update reads only the stage-combined flux. */
+int gks2d_keep_face_fluxes(gks2d_ctx* ctx, int on);
+
+/* Per-face fluxes of the last gks2d_flux call (needs gks2d_keep_face_fluxes
+ * on during that call), for parity checks:
  * xf: (nx+1) x ny faces, yf: nx x (ny+1) faces, each face gausspoint x 8
  * doubles (f[4], derf[4]) in the face-local frame.  Faces outside the range
  * the reference computes are left untouched. */

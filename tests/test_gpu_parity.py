@@ -103,7 +103,14 @@ def test_flux_points_random_large_batch(fast):
     R[n // 2:, :4] = L[::-1][n // 2:, :4]
     L[:, 4:] = rng.normal(0, 0.3, (n, 8))
     R[:, 4:] = rng.normal(0, 0.3, (n, 8))
-    say("random batch", n, "fast", fast)
+    # drop near-vacuum faces: both states leave the face supersonically and the
+    # collision state's density is rounding noise (~1e-16), so its lambda can
+    # round negative in any evaluation order (such a face is a blow-up of the
+    # scheme, not a flux to compare)
+    co, _, _ = oracle().flux_points(p, L, R, 2e-3)
+    ok = co[:, 0] > 1e-8 * np.minimum(L[:, 0], R[:, 0])
+    L, R = np.ascontiguousarray(L[ok]), np.ascontiguousarray(R[ok])
+    say("random batch", n, "kept", int(ok.sum()), "fast", fast)
     _, f, d = gks_amd.flux_points(p, L, R, 2e-3)
     _, fo, do = oracle().flux_points(p, L, R, 2e-3)
     e = vec_rel(np.concatenate([f, d], 1), np.concatenate([fo, do], 1), face_floor(L, R, 2e-3))
@@ -117,6 +124,7 @@ def one_stage_both(p, conv, ax, ay, dt, bc, stage):
     s = gks_amd.Solver2D(p)
     s.set_convar(conv)
     s.set_alpha(ax, ay)
+    s.keep_face_fluxes(True)
     s.copy_old()
     for side in range(4):
         s.boundary(side, bc[side])
