@@ -63,3 +63,67 @@ def planar_shock_params(n, weno_order=5, cfl=0.5, **kw):
     for k, v in kw.items():
         setattr(p, k, v)
     return p
+
+
+class Params1D(ctypes.Structure):
+    """ctypes mirror of ``gks1d_params`` (include/gks1d.h)."""
+    _fields_ = [
+        ("nodex", ctypes.c_int32), ("ghost", ctypes.c_int32), ("recon_variable", ctypes.c_int32),
+        ("use_df", ctypes.c_int32), ("tau_type", ctypes.c_int32), ("K", ctypes.c_int32),
+        ("fast_math", ctypes.c_int32), ("df_from_start", ctypes.c_int32),
+        ("gamma", ctypes.c_double), ("mu", ctypes.c_double), ("nu", ctypes.c_double),
+        ("pr", ctypes.c_double), ("c1_euler", ctypes.c_double), ("c2_euler", ctypes.c_double),
+        ("left", ctypes.c_double), ("right", ctypes.c_double), ("cfl", ctypes.c_double),
+    ]
+
+    @property
+    def nx(self):
+        return self.nodex + 2 * self.ghost
+
+    @property
+    def dx(self):
+        return (self.right - self.left) / self.nodex
+
+    def copy(self, **kw):
+        p = Params1D()
+        ctypes.pointer(p)[0] = self
+        for k, v in kw.items():
+            setattr(p, k, v)
+        return p
+
+
+def sod_params(n=400, **kw):
+    """Settings of SodTubeProblem (reference src/1Dproblem.cpp:3-43): K=4,
+    gamma=1.4, Euler tau c1=0.05 c2=1, WENO5-AO (wenoz) in characteristic
+    variables, Center_collision, GKS gks2nd, S2O4, CFL 0.5 on [0, 1].
+    The reference leaves is_using_df_factor at its default (false,
+    reconstruction.cpp:10); ``use_df=1`` gives the DF variant of the loop."""
+    p = Params1D(nodex=n, ghost=5, recon_variable=VAR_CHARACTERISTIC, use_df=0, tau_type=TAU_EULER, K=4,
+                 fast_math=0, df_from_start=0, gamma=1.4, mu=-1.0, nu=-1.0, pr=1.0, c1_euler=0.05,
+                 c2_euler=1.0, left=0.0, right=1.0, cfl=0.5)
+    for k, v in kw.items():
+        setattr(p, k, v)
+    return p
+
+
+def blastwave_params(n=400, **kw):
+    """Blastwave (reference src/1Dproblem.cpp:176-321): as Sod with c1=0.01,
+    c2=5 and reflective walls; run to t=0.038."""
+    kw = dict(dict(c1_euler=0.01, c2_euler=5.0), **kw)
+    return sod_params(n, **kw)
+
+
+def shu_osher_params(n=200, **kw):
+    """ShuOsher (reference src/1Dproblem.cpp:347-491): as Sod on [0, 10],
+    free boundaries; run to t=1.8."""
+    kw = dict(dict(left=0.0, right=10.0), **kw)
+    return sod_params(n, **kw)
+
+
+def sinwave1d_params(n, cfl=0.1, **kw):
+    """accuracy_sinwave_1d (reference src/1Dproblem.cpp:550-690): periodic
+    density wave on [0, 2], WENO5_AO_with_DF in conservative variables from
+    the first step (no alpha update), c1=c2=0, run to t=2."""
+    kw = dict(dict(recon_variable=VAR_CONSERVATIVE, df_from_start=1, c1_euler=0.0, c2_euler=0.0,
+                   left=0.0, right=2.0, cfl=cfl), **kw)
+    return sod_params(n, **kw)

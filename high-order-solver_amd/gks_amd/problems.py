@@ -95,3 +95,68 @@ PLANAR_SHOCK_ZONES = np.array([
     [1.5, 0.0, 0.0, 1.5],
     [0.5323, 1.206, 0.0, 0.3],
 ])
+
+
+# ------------------------------------------------------------------------ 1D
+def _fill_ghosts_1d(prim, G):
+    prim[:, :G] = prim[:, G:G + 1]
+    prim[:, -G:] = prim[:, -G - 1:-G]
+    return prim
+
+
+def sod_prim(p):
+    """ICforSodTube (reference src/1Dproblem.cpp:151-173): (1, 0, 1) where
+    (i - ghost + 1) * dx <= 0.5, else (0.125, 0, 0.1); [3, nx].  The
+    reference leaves the left ghost layer unset; here it repeats the first
+    interior cell (ghosts are overwritten by the boundary fill before use)."""
+    G, nx, dx = p.ghost, p.nx, p.dx
+    prim = np.zeros((3, nx))
+    i = np.arange(G, nx)
+    left = (i - G + 1).astype(np.float64) * dx <= 0.5
+    prim[:, G:] = np.where(left[None, :], np.array([[1.0], [0.0], [1.0]]), np.array([[0.125], [0.0], [0.1]]))
+    return _fill_ghosts_1d(prim, G)
+
+
+def blastwave_prim(p):
+    """ICforBlastwave (reference src/1Dproblem.cpp:323-345): p = 1000 where
+    (i - ghost + 1) * dx <= 0.1, p = 100 where >= 0.9, else 0.01; rho 1, u 0."""
+    G, nx, dx = p.ghost, p.nx, p.dx
+    prim = np.zeros((3, nx))
+    i = np.arange(G, nx - G)
+    x = (i - G + 1).astype(np.float64) * dx
+    prim[0, G:nx - G] = 1.0
+    prim[2, G:nx - G] = np.where(x <= 0.1, 1000.0, np.where(x >= 0.9, 100.0, 0.01))
+    return _fill_ghosts_1d(prim, G)
+
+
+def shu_osher_prim(p):
+    """ICforShuOsher (reference src/1Dproblem.cpp:493-515), cell-averaged
+    density wave right of x = 1."""
+    G, nx, dx = p.ghost, p.nx, p.dx
+    prim = np.zeros((3, nx))
+    for i in range(G, nx):
+        x = (i - G + 0.5) * dx
+        if x <= 1.0:
+            prim[:, i] = (3.85714, 2.629369, 10.33333)
+        else:
+            prim[:, i] = (1 - (0.2 / (5 * dx)) * (np.cos(5 * (x + 0.5 * dx)) - np.cos(5 * (x - 0.5 * dx))), 0.0, 1.0)
+    return _fill_ghosts_1d(prim, G)
+
+
+def sinwave_prim_1d(p):
+    """ICfor_sinwave (reference src/1Dproblem.cpp:692-706): cell averages of
+    rho = 1 + 0.2 sin(pi x), u = 1, p = 1 on the interior."""
+    G, nx, dx = p.ghost, p.nx, p.dx
+    pi = 3.14159265358979323846
+    prim = np.zeros((3, nx))
+    for i in range(G, nx - G):
+        prim[0, i] = 1.0 - 0.2 / pi / dx * (np.cos(pi * (i + 1 - G) * dx) - np.cos(pi * (i - G) * dx))
+        prim[1, i] = 1.0
+        prim[2, i] = 1.0
+    return _fill_ghosts_1d(prim, G)
+
+
+def sinwave_exact_1d(p):
+    """Exact cell-averaged density of the sine wave at any t that is a whole
+    number of periods (error_for_sinwave, reference src/1Dproblem.cpp:708-735)."""
+    return sinwave_prim_1d(p)[0, p.ghost:p.nx - p.ghost]

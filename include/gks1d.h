@@ -1,0 +1,68 @@
+/*
+ * gks1d.h — C ABI of the 1D GKS hot path on gfx950: WENO5-AO (first step) /
+ * WENO5-AO-DF reconstruction, Center_collision, GKS (gks2nd), S2O4, DF
+ * sensor, CFL and ghost fills of the reference's SodTubeProblem loop
+ * (reference src/1Dproblem.cpp:79-148).  The whole time loop runs inside one
+ * persistent workgroup on the device (a 1D grid of a few hundred cells is far
+ * too small to fill the GPU; keeping it in one workgroup removes every launch
+ * and host round trip).  Same conventions as gks2d.h.
+ */
+#ifndef GKS1D_H
+#define GKS1D_H
+
+#include <stdint.h>
+
+#include "gks2d.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* snapshot of the reference's 1D globals and Block1d (fluid_mesh.h:8-26,
+ * basic_function.h:5-16, reconstruction.h:4-11); S2O4 (time_advance.cpp:20-30) */
+typedef struct gks1d_params {
+  int32_t nodex;           /* interior cells */
+  int32_t ghost;           /* ghost layers (5) */
+  int32_t recon_variable;  /* GKS_VAR_* */
+  int32_t use_df;          /* is_using_df_factor: WENO5_AO_with_DF from step 1 on + Update_alpha */
+  int32_t tau_type;        /* GKS_TAU_* */
+  int32_t K;
+  int32_t fast_math;       /* 0: IEEE kernels; 1: fast kernels */
+  int32_t df_from_start;   /* 1: cellreconstruction = WENO5_AO_with_DF before the loop
+                              (accuracy_sinwave_1d, 1Dproblem.cpp:587); 0: WENO5_AO */
+  double gamma, mu, nu, pr, c1_euler, c2_euler;
+  double left, right;      /* domain */
+  double cfl;
+} gks1d_params;
+
+typedef struct gks1d_ctx gks1d_ctx;
+
+/* Replaces the 1D Setflux_array / SetUniformMesh (basic_function.cpp:1262-1309). */
+int gks1d_create(const gks1d_params* p, gks1d_ctx** out);
+int gks1d_destroy(gks1d_ctx* ctx);
+/* conservative state of all nx = nodex + 2*ghost cells (strided host array) */
+int gks1d_set_convar(gks1d_ctx* ctx, const double* base, int64_t cell_stride_bytes, int64_t var_stride_bytes);
+int gks1d_get_convar(gks1d_ctx* ctx, double* base, int64_t cell_stride_bytes, int64_t var_stride_bytes);
+/* DF sensor (Fluid1d::alpha, fluid_mesh.h:38) */
+int gks1d_get_alpha(gks1d_ctx* ctx, double* alpha, int64_t cell_stride_bytes);
+/* The driver's time loop (1Dproblem.cpp:79-148): while (t < tstop && steps <
+ * max_steps) { copy old; dt = Get_CFL (time_advance.cpp:75-91); two stages of
+ * boundary fills (bc[0] left, bc[1] right), Reconstruction_within_cell,
+ * Reconstruction_forg0, Calculate_flux, Update, Update_alpha }.  The
+ * reconstruction is WENO5_AO at step 0 and WENO5_AO_with_DF afterwards when
+ * use_df.  t_inout / step_inout carry block.t / block.step; dt_hist
+ * (optional, max_steps entries) receives each step's dt.  max_steps is
+ * limited to 10^7. */
+int gks1d_run(gks1d_ctx* ctx, const int32_t bc[2], double tstop, int32_t max_steps, double* t_inout,
+              int32_t* step_inout, double* dt_hist);
+/* leaf parity: Center_collision (reconstruction.cpp:1375-1407) + GKS
+ * (flux_function.cpp:16-276) for n points; left/right n x 6 (convar[3],
+ * der1[3]); center n x 6; f/derf n x 3.  Host pointers. */
+int gks1d_flux_points(const gks1d_params* p, int64_t n, const double* left, const double* right, double dt,
+                      double* center, double* f, double* derf);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* GKS1D_H */

@@ -46,6 +46,7 @@ inline CD sqrt(const CD& a) { cnt::n_arith++; cnt::n_sqrt++; return CD(std::sqrt
 inline CD exp(const CD& a) { cnt::n_exp++; return CD(std::exp(a.v)); }
 inline CD erfc(const CD& a) { cnt::n_erfc++; return CD(std::erfc(a.v)); }
 inline CD fabs(const CD& a) { return CD(a.v < 0 ? -a.v : a.v); }
+inline double as_double(const CD& a) { return a.v; }
 }  // namespace orc
 
 #include "solver2d.hpp"

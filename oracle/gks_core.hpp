@@ -15,6 +15,8 @@
 
 namespace orc {
 
+inline double as_double(double x) { return x; }
+
 using std::erfc;
 using std::exp;
 using std::fabs;

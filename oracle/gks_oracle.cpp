@@ -4,7 +4,9 @@
 #include <cstdint>
 #include <cstring>
 
+#include "solver1d.hpp"
 #include "solver2d.hpp"
+#include "../include/gks1d.h"
 
 namespace orc {
 template <> double Solver2d<double>::to_double(const double& r) { return r; }
@@ -212,6 +214,94 @@ int orc2d_weno5_tangent(const gks2d_params* p, int64_t n, const double* line5,
     orc::weno5_df_tangent_side(out, line, alpha5 + k * 5, h, p->recon_variable, g);
     store_pt(gp2 + k * 24, out[0]);
     store_pt(gp2 + k * 24 + 12, out[1]);
+  }
+  return GKS_OK;
+}
+
+}  // extern "C"
+
+// =================================================================== 1D
+using S1 = orc::Solver1d<double>;
+
+static orc::Gas gas1_from(const gks1d_params& p) {
+  orc::Gas g;
+  g.K = p.K;
+  g.gamma = p.gamma;
+  g.mu = p.mu;
+  g.nu = p.nu;
+  g.pr = p.pr;
+  g.c1 = p.c1_euler;
+  g.c2 = p.c2_euler;
+  g.tau_type = p.tau_type;
+  g.prandtl_fix = 0;
+  return g;
+}
+
+extern "C" {
+
+int64_t orc1d_params_layout(int64_t* off) {
+  const gks1d_params* z = nullptr;
+  const char* b = reinterpret_cast<const char*>(z);
+  int k = 0;
+#define OFF(f) off[k++] = reinterpret_cast<const char*>(&z->f) - b
+  OFF(nodex); OFF(ghost); OFF(recon_variable); OFF(use_df); OFF(tau_type); OFF(K); OFF(fast_math);
+  OFF(df_from_start); OFF(gamma); OFF(mu); OFF(nu); OFF(pr); OFF(c1_euler); OFF(c2_euler); OFF(left);
+  OFF(right); OFF(cfl);
+#undef OFF
+  return (int64_t)sizeof(gks1d_params);
+}
+
+void* orc1d_create(const gks1d_params* p) {
+  if (!p || p->ghost != 5 || p->nodex < 5) return nullptr;
+  return new S1(p->nodex, p->ghost, p->left, p->right, p->cfl, p->recon_variable, p->use_df,
+                p->df_from_start, gas1_from(*p));
+}
+void orc1d_destroy(void* h) { delete static_cast<S1*>(h); }
+// ICforSodTube-style set-up: primvar of all nx cells ([3][nx]), convar =
+// Primvar_to_convar_1D for every cell
+int orc1d_set_prim(void* h, const double* prim) {
+  S1* s = static_cast<S1*>(h);
+  for (int i = 0; i < s->nx; i++) {
+    double p[3] = {prim[i], prim[s->nx + i], prim[2 * s->nx + i]}, u[3];
+    orc::primvar_to_convar_1d(u, p, s->g);
+    for (int v = 0; v < 3; v++) s->U[v][i] = u[v];
+  }
+  return GKS_OK;
+}
+int orc1d_get(void* h, double* conv, double* alpha, double* t, int32_t* step) {
+  S1* s = static_cast<S1*>(h);
+  for (int v = 0; v < 3; v++)
+    if (conv) std::memcpy(conv + v * s->nx, s->U[v].data(), s->nx * 8);
+  if (alpha) std::memcpy(alpha, s->alpha.data(), s->nx * 8);
+  if (t) *t = s->t;
+  if (step) *step = s->step;
+  return GKS_OK;
+}
+int orc1d_run(void* h, const int32_t* bc, double tstop, int32_t max_steps, double* dt_hist) {
+  int b[2] = {bc[0], bc[1]};
+  return static_cast<S1*>(h)->run(b, tstop, max_steps, dt_hist);
+}
+// face fluxes of `stage`: (nx+1) x 6 (f[3], derf[3])
+int orc1d_get_face_fluxes(void* h, int32_t stage, double* out) {
+  S1* s = static_cast<S1*>(h);
+  std::memcpy(out, s->fl[stage].data(), (size_t)(s->nx + 1) * 6 * 8);
+  return GKS_OK;
+}
+int orc1d_flux_points(const gks1d_params* p, int64_t n, const double* left, const double* right, double dt,
+                      double* center, double* f, double* derf) {
+  orc::Gas g = gas1_from(*p);
+  for (int64_t k = 0; k < n; k++) {
+    orc::Pt1<double> L, R, C;
+    std::memcpy(L.convar, left + k * 6, 24);
+    std::memcpy(L.der1, left + k * 6 + 3, 24);
+    std::memcpy(R.convar, right + k * 6, 24);
+    std::memcpy(R.der1, right + k * 6 + 3, 24);
+    orc::center_collision_1d(C, L, R, g);
+    if (center) {
+      std::memcpy(center + k * 6, C.convar, 24);
+      std::memcpy(center + k * 6 + 3, C.der1, 24);
+    }
+    orc::gks1d_flux(f + k * 3, derf + k * 3, L, R, C, dt, g);
   }
   return GKS_OK;
 }

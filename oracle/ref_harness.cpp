@@ -13,6 +13,7 @@
 
 #include "2Dproblem.h"
 #include "../include/gks2d.h"
+#include "../include/gks1d.h"
 
 namespace {
 
@@ -314,6 +315,155 @@ int ref2d_weno5_tangent(const gks2d_params* p, int64_t n, const double* line5,
     weno_5th_ao_with_df_tangential(re, line[0], line[1], line[2], line[3], line[4], a1, a2, h);
     put_point(gp2 + k * 24, re[0].left);
     put_point(gp2 + k * 24 + 12, re[1].left);
+  }
+  return GKS_OK;
+}
+
+}  // extern "C"
+
+// =================================================================== 1D
+// SodTubeProblem's set-up and loop (1Dproblem.cpp:3-149) with the globals of
+// the parameters, boundary kinds as function pointers
+namespace {
+struct RefCtx1 {
+  gks1d_params p;
+  Block1d block;
+  Fluid1d* fluids = nullptr;
+  Interface1d* interfaces = nullptr;
+  Flux1d** fluxes = nullptr;
+  Fluid1d bcvalue[2];
+};
+void set_globals1(const gks1d_params* p) {
+  K = p->K;
+  Gamma = p->gamma;
+  Mu = p->mu;
+  Nu = p->nu;
+  Pr = p->pr;
+  c1_euler = p->c1_euler;
+  c2_euler = p->c2_euler;
+  tau_type = p->tau_type == GKS_TAU_NS ? NS : Euler;
+  gks1dsolver = gks2nd;
+  wenotype = wenoz;
+  reconstruction_variable = p->recon_variable == GKS_VAR_CHARACTERISTIC ? characteristic : conservative;
+  g0reconstruction = Center_collision;
+  is_reduce_order_warning = false;
+  is_using_df_factor = p->use_df != 0;
+  flux_function = GKS;
+  timecoe_list = S2O4;
+}
+BoundaryCondition bc1_fn(int side, int kind) {
+  static const BoundaryCondition tab[3][2] = {{free_boundary_left, free_boundary_right},
+                                             {periodic_boundary_left, periodic_boundary_right},
+                                             {reflection_boundary_left, reflection_boundary_right}};
+  return tab[kind][side];
+}
+}  // namespace
+
+extern "C" {
+
+void* ref1d_create(const gks1d_params* p) {
+  silence();
+  RefCtx1* c = new RefCtx1();
+  c->p = *p;
+  set_globals1(p);
+  Block1d& b = c->block;
+  b.nodex = p->nodex;
+  b.ghost = p->ghost;
+  b.CFL = p->cfl;
+  Initial_stages(b);
+  b.nx = b.nodex + 2 * b.ghost;
+  b.nodex_begin = b.ghost;
+  b.nodex_end = b.nodex + b.ghost - 1;
+  c->fluids = new Fluid1d[b.nx];
+  c->interfaces = new Interface1d[b.nx + 1];
+  c->fluxes = Setflux_array(b);
+  b.left = p->left;
+  b.right = p->right;
+  b.dx = (b.right - b.left) / b.nodex;
+  SetUniformMesh(b, c->fluids, c->interfaces, c->fluxes);
+  b.t = 0;
+  b.step = 0;
+  return c;
+}
+void ref1d_destroy(void* h) {
+  RefCtx1* c = static_cast<RefCtx1*>(h);
+  delete[] c->fluids;
+  delete c;
+}
+int ref1d_set_prim(void* h, const double* prim) {
+  RefCtx1* c = static_cast<RefCtx1*>(h);
+  int nx = c->block.nx;
+  for (int i = 0; i < nx; i++) {
+    for (int v = 0; v < 3; v++) c->fluids[i].primvar[v] = prim[v * nx + i];
+    Primvar_to_convar_1D(c->fluids[i].convar, c->fluids[i].primvar);
+  }
+  return GKS_OK;
+}
+int ref1d_get(void* h, double* conv, double* alpha, double* t, int32_t* step) {
+  RefCtx1* c = static_cast<RefCtx1*>(h);
+  int nx = c->block.nx;
+  for (int i = 0; i < nx; i++) {
+    for (int v = 0; v < 3; v++)
+      if (conv) conv[v * nx + i] = c->fluids[i].convar[v];
+    if (alpha) alpha[i] = c->fluids[i].alpha;
+  }
+  if (t) *t = c->block.t;
+  if (step) *step = c->block.step;
+  return GKS_OK;
+}
+int ref1d_run(void* h, const int32_t* bc, double tstop, int32_t max_steps, double* dt_hist) {
+  RefCtx1* c = static_cast<RefCtx1*>(h);
+  set_globals1(&c->p);
+  Block1d& b = c->block;
+  int n = 0;
+  while (b.t < tstop && n < max_steps) {
+    CopyFluid_new_to_old(c->fluids, b);
+    b.dt = Get_CFL(b, c->fluids, tstop);
+    if (dt_hist) dt_hist[n] = b.dt;
+    if (c->p.df_from_start || (b.step > 0 && is_using_df_factor)) cellreconstruction = WENO5_AO_with_DF;
+    else cellreconstruction = WENO5_AO;
+    for (int s = 0; s < b.stages; s++) {
+      bc1_fn(0, bc[0])(c->fluids, b, c->bcvalue[0]);
+      bc1_fn(1, bc[1])(c->fluids, b, c->bcvalue[1]);
+      Reconstruction_within_cell(c->interfaces, c->fluids, b);
+      Reconstruction_forg0(c->interfaces, c->fluids, b);
+      Calculate_flux(c->fluxes, c->interfaces, b, s);
+      Update(c->fluids, c->fluxes, b, s);
+      if (is_using_df_factor) Update_alpha(c->interfaces, c->fluids, b);
+    }
+    b.step++;
+    b.t = b.t + b.dt;
+    n++;
+  }
+  return GKS_OK;
+}
+int ref1d_get_face_fluxes(void* h, int32_t stage, double* out) {
+  RefCtx1* c = static_cast<RefCtx1*>(h);
+  for (int i = 0; i <= c->block.nx; i++) {
+    std::memcpy(out + i * 6, c->fluxes[i][stage].f, 24);
+    std::memcpy(out + i * 6 + 3, c->fluxes[i][stage].derf, 24);
+  }
+  return GKS_OK;
+}
+int ref1d_flux_points(const gks1d_params* p, int64_t n, const double* left, const double* right, double dt,
+                      double* center, double* f, double* derf) {
+  silence();
+  set_globals1(p);
+  for (int64_t k = 0; k < n; k++) {
+    Interface1d in;
+    std::memcpy(in.left.convar, left + k * 6, 24);
+    std::memcpy(in.left.der1, left + k * 6 + 3, 24);
+    std::memcpy(in.right.convar, right + k * 6, 24);
+    std::memcpy(in.right.der1, right + k * 6 + 3, 24);
+    Center_collision(in, nullptr);
+    Flux1d fl;
+    GKS(fl, in, dt);
+    if (center) {
+      std::memcpy(center + k * 6, in.center.convar, 24);
+      std::memcpy(center + k * 6 + 3, in.center.der1, 24);
+    }
+    std::memcpy(f + k * 3, fl.f, 24);
+    std::memcpy(derf + k * 3, fl.derf, 24);
   }
   return GKS_OK;
 }

@@ -18,9 +18,11 @@ ROOT = os.path.dirname(os.path.dirname(HERE))
 sys.path.insert(0, os.path.join(ROOT, "high-order-solver_amd"))
 sys.path.insert(0, os.path.join(ROOT, "tests"))
 
-from gks_amd.params import BC_FREE, BC_PERIODIC, BC_REFLECT, planar_shock_params  # noqa: E402
-from gks_amd.problems import PLANAR_SHOCK_ZONES, riemann_prim, riemann_states  # noqa: E402
-from oracle_lib import reference  # noqa: E402
+from gks_amd.params import (BC_FREE, BC_PERIODIC, BC_REFLECT, blastwave_params, planar_shock_params,  # noqa: E402
+                            shu_osher_params, sinwave1d_params, sod_params)
+from gks_amd.problems import (PLANAR_SHOCK_ZONES, blastwave_prim, riemann_prim, riemann_states,  # noqa: E402
+                              shu_osher_prim, sinwave_prim_1d, sod_prim)
+from oracle_lib import reference, reference1d  # noqa: E402
 
 
 def random_points(rng, n, gamma=1.4):
@@ -126,11 +128,80 @@ def gen_runs(ref):
     np.savez_compressed(os.path.join(HERE, "runs2d.npz"), **out)
 
 
+# 1D cases: name -> (params, initial condition, bc, tstop)
+RUNS1D = {
+    "sod400": (lambda: sod_params(400), sod_prim, [BC_FREE, BC_FREE], 0.2),
+    "sod400_df": (lambda: sod_params(400, use_df=1), sod_prim, [BC_FREE, BC_FREE], 0.2),
+    "sod100_cons_df": (lambda: sod_params(100, use_df=1, recon_variable=0), sod_prim, [BC_FREE, BC_FREE], 0.2),
+    "blastwave400": (lambda: blastwave_params(400), blastwave_prim, [BC_REFLECT, BC_REFLECT], 0.038),
+    "shu_osher200_df": (lambda: shu_osher_params(200, use_df=1), shu_osher_prim, [BC_FREE, BC_FREE], 1.8),
+    "sinwave40": (lambda: sinwave1d_params(40), sinwave_prim_1d, [BC_PERIODIC, BC_PERIODIC], 2.0),
+}
+
+
+def gen_runs1d(ref1):
+    out = {}
+    for name, (mk, ic, bc, tstop) in RUNS1D.items():
+        p = mk()
+        s = ref1.solver(p)
+        s.set_prim(ic(p))
+        rc, hist = s.run(bc, tstop, 100000)
+        assert rc == 0
+        st = s.get()
+        G = p.ghost
+        out.update({f"{name}_convar": st["convar"][:, G:-G], f"{name}_alpha": st["alpha"][G:-G],
+                    f"{name}_dt": hist[:st["step"]], f"{name}_t": np.array(st["t"]),
+                    f"{name}_step": np.array(st["step"])})
+        if name == "sod400_df":  # both stages' face fluxes of the last step
+            for k in range(2):
+                out[f"{name}_faces{k}"] = s.face_fluxes(k)[G:p.nodex + G + 1]
+        s.close()
+    np.savez_compressed(os.path.join(HERE, "runs1d.npz"), **out)
+
+
+def random_points_1d(rng, n, gamma=1.4):
+    """Physical left/right 1D states (convar[3], der1[3])."""
+    def state(m):
+        rho = rng.uniform(0.1, 2.0, m)
+        u = rng.uniform(-1.5, 1.5, m)
+        p = rng.uniform(0.05, 2.0, m)
+        out = np.zeros((m, 6))
+        out[:, 0], out[:, 1], out[:, 2] = rho, rho * u, p / (gamma - 1) + 0.5 * rho * u * u
+        out[:, 3:6] = rng.normal(0.0, 0.5, (m, 3)) * rng.choice([0.0, 1.0, 10.0], (m, 1))
+        return out
+    left = state(n)
+    right = left.copy()
+    k = n // 2
+    right[:k] = left[:k] * (1 + rng.normal(0, 1e-3, (k, 6)))
+    right[k:] = state(n - k)
+    return left, right
+
+
+FLUX1D_CASES = {"euler": {}, "c0": dict(c1_euler=0.0, c2_euler=0.0),
+                "ns": dict(tau_type=1, mu=1e-3, c1_euler=0.0, c2_euler=1.0)}
+
+
+def gen_flux_points_1d(ref1):
+    rng = np.random.default_rng(4321)
+    out = {}
+    for name, kw in FLUX1D_CASES.items():
+        p = sod_params(16, **kw)
+        left, right = random_points_1d(rng, 256)
+        dt = 10 ** rng.uniform(-4, -2)
+        c, f, d = ref1.flux_points(p, left, right, dt)
+        out.update({f"{name}_left": left, f"{name}_right": right, f"{name}_dt": np.array(dt),
+                    f"{name}_center": c, f"{name}_f": f, f"{name}_derf": d})
+    np.savez_compressed(os.path.join(HERE, "gks1d_flux_points.npz"), **out)
+
+
 def main():
     ref = reference()
     gen_flux_points(ref)
     gen_weno(ref)
     gen_runs(ref)
+    ref1 = reference1d()
+    gen_runs1d(ref1)
+    gen_flux_points_1d(ref1)
     for f in sorted(os.listdir(HERE)):
         if f.endswith(".npz"):
             print(f, os.path.getsize(os.path.join(HERE, f)))

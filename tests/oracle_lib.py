@@ -186,3 +186,106 @@ def reference():
     if _ref is None:
         _ref = CpuLib(REF_SO, "ref2d_")
     return _ref
+
+
+# ------------------------------------------------------------------------ 1D
+def _bind1(lib, prefix):
+    from gks_amd.params import Params1D
+    pp = ctypes.POINTER(Params1D)
+    sig = {
+        "create": (_P, [pp]),
+        "destroy": (None, [_P]),
+        "set_prim": (ctypes.c_int, [_P, _D]),
+        "get": (ctypes.c_int, [_P, _D, _D, _D, _I]),
+        "run": (ctypes.c_int, [_P, _I, ctypes.c_double, ctypes.c_int32, _D]),
+        "get_face_fluxes": (ctypes.c_int, [_P, ctypes.c_int32, _D]),
+        "flux_points": (ctypes.c_int, [pp, ctypes.c_int64, _D, _D, ctypes.c_double, _D, _D, _D]),
+    }
+    fns = {}
+    for name, (res, args) in sig.items():
+        f = getattr(lib, prefix + name)
+        f.restype = res
+        f.argtypes = args
+        fns[name] = f
+    return fns
+
+
+class CpuSolver1D:
+    """One 1D block (SodTubeProblem-style loop) in the oracle or the reference."""
+
+    def __init__(self, fns, p):
+        self.f = fns
+        self.p = p
+        self.h = fns["create"](ctypes.byref(p))
+        if not self.h:
+            raise ValueError("unsupported parameters")
+        self.nx = p.nx
+
+    def close(self):
+        if self.h:
+            self.f["destroy"](self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def set_prim(self, prim):
+        prim = np.ascontiguousarray(prim, dtype=np.float64)
+        assert prim.shape == (3, self.nx)
+        return self.f["set_prim"](self.h, dptr(prim))
+
+    def get(self):
+        u = np.zeros((3, self.nx))
+        a = np.zeros(self.nx)
+        t = ctypes.c_double()
+        s = ctypes.c_int32()
+        self.f["get"](self.h, dptr(u), dptr(a), ctypes.byref(t), ctypes.byref(s))
+        return dict(convar=u, alpha=a, t=t.value, step=s.value)
+
+    def run(self, bc, tstop, max_steps):
+        bca = (ctypes.c_int32 * 2)(*bc)
+        hist = np.zeros(max(1, min(max_steps, 10_000_000)))
+        rc = self.f["run"](self.h, bca, tstop, max_steps, dptr(hist))
+        return rc, hist
+
+    def face_fluxes(self, stage):
+        out = np.zeros((self.nx + 1, 6))
+        self.f["get_face_fluxes"](self.h, stage, dptr(out))
+        return out
+
+
+class CpuLib1D:
+    def __init__(self, path, prefix):
+        self.lib = ctypes.CDLL(path)
+        self.fns = _bind1(self.lib, prefix)
+
+    def solver(self, p):
+        return CpuSolver1D(self.fns, p)
+
+    def flux_points(self, p, left, right, dt):
+        left = np.ascontiguousarray(left, dtype=np.float64)
+        right = np.ascontiguousarray(right, dtype=np.float64)
+        n = left.shape[0]
+        c = np.zeros((n, 6))
+        f = np.zeros((n, 3))
+        d = np.zeros((n, 3))
+        self.fns["flux_points"](ctypes.byref(p), n, dptr(left), dptr(right), dt, dptr(c), dptr(f), dptr(d))
+        return c, f, d
+
+
+_oracle1 = None
+_ref1 = None
+
+
+def oracle1d():
+    global _oracle1
+    if _oracle1 is None:
+        _oracle1 = CpuLib1D(ORACLE_SO, "orc1d_")
+    return _oracle1
+
+
+def reference1d():
+    global _ref1
+    if _ref1 is None:
+        _ref1 = CpuLib1D(REF_SO, "ref1d_")
+    return _ref1
