@@ -55,6 +55,11 @@ int gks1d_get_alpha(gks1d_ctx* ctx, double* alpha, int64_t cell_stride_bytes);
  * limited to 10^7. */
 int gks1d_run(gks1d_ctx* ctx, const int32_t bc[2], double tstop, int32_t max_steps, double* t_inout,
               int32_t* step_inout, double* dt_hist);
+/* f[3], derf[3] of every face (nx+1 faces, face-major, 6 doubles each) from
+ * the last step's stage 0 or 1 (Flux1d fluxes[i][stage], fluid_mesh.h:52-58) */
+int gks1d_get_face_fluxes(gks1d_ctx* ctx, int32_t stage, double* out);
+/* message of the last failed call on this thread */
+const char* gks1d_last_error(void);
 /* leaf parity: Center_collision (reconstruction.cpp:1375-1407) + GKS
  * (flux_function.cpp:16-276) for n points; left/right n x 6 (convar[3],
  * der1[3]); center n x 6; f/derf n x 3.  Host pointers. */

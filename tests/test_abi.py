@@ -82,3 +82,61 @@ def test_riemann_states_in_range_and_seeded():
     assert np.all((a[:, 0] >= 0.1) & (a[:, 0] <= 1.5))
     assert np.all(np.abs(a[:, 1:3]) <= 0.8)
     assert np.all((a[:, 3] >= 0.02) & (a[:, 3] <= 1.5))
+
+
+# ------------------------------------------------------------------------ 1D
+HEADER1 = os.path.join(ROOT, "include", "gks1d.h")
+
+
+def declared_functions_1d():
+    text = open(HEADER1).read()
+    return sorted(set(re.findall(r"\b(gks1d_[a-z_0-9]+)\s*\(", text)))
+
+
+def test_1d_library_exports_every_declared_symbol():
+    from gks_amd import solver1d
+    names = declared_functions_1d()
+    assert "gks1d_run" in names and len(names) >= 8
+    lib = solver1d.lib1()
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
+    assert set(names) <= set(solver1d.SIGS1)
+
+
+def test_1d_params_layout_matches_c():
+    from gks_amd.params import Params1D
+    off = (ctypes.c_int64 * 32)()
+    f = oracle().lib.orc1d_params_layout
+    f.restype = ctypes.c_int64
+    f.argtypes = [ctypes.POINTER(ctypes.c_int64)]
+    size = f(off)
+    assert size == ctypes.sizeof(Params1D)
+    for k, (name, _) in enumerate(Params1D._fields_):
+        assert off[k] == getattr(Params1D, name).offset, name
+
+
+@pytest.mark.parametrize("bad,code", [
+    (dict(ghost=4), 2), (dict(nodex=3), 1), (dict(right=-1.0), 1), (dict(use_df=2), 1),
+    (dict(recon_variable=3), 1), (dict(cfl=0.0), 1), (dict(tau_type=5), 1),
+])
+def test_1d_create_rejects_bad_params_on_host(bad, code):
+    from gks_amd import solver1d
+    from gks_amd.params import sod_params
+    p = sod_params(16).copy(**bad)
+    h = ctypes.c_void_p()
+    rc = solver1d.lib1().gks1d_create(ctypes.byref(p), ctypes.byref(h))
+    assert rc == code
+    assert not h.value
+    assert solver1d.lib1().gks1d_last_error()
+
+
+def test_1d_null_arguments_are_argument_errors():
+    from gks_amd import solver1d
+    L = solver1d.lib1()
+    bc = (ctypes.c_int32 * 2)(0, 0)
+    t = ctypes.c_double(0.0)
+    s = ctypes.c_int32(0)
+    assert L.gks1d_run(None, bc, 1.0, 1, ctypes.byref(t), ctypes.byref(s), None) == 1
+    assert L.gks1d_get_face_fluxes(None, 0, None) == 1
+    assert L.gks1d_destroy(None) == 0
+    assert L.gks1d_flux_points(None, 0, None, None, 0.0, None, None, None) == 1
