@@ -17,8 +17,9 @@ using S2 = orc::Solver2d<double>;
 static bool params_ok(const gks2d_params* p) {
   if (!p) return false;
   if (p->nodex < 5 || p->nodey < 5 || p->ghost != 5) return false;
-  if (p->weno_order != 5 || p->gausspoint != 2) return false;
-  return true;
+  const bool ok = (p->weno_order == 5 && p->gausspoint == 2) || (p->weno_order == 7 && p->gausspoint == 3) ||
+                  (p->weno_order == 9 && p->gausspoint == 4);
+  return ok;
 }
 
 extern "C" {
@@ -214,6 +215,55 @@ int orc2d_weno5_tangent(const gks2d_params* p, int64_t n, const double* line5,
     orc::weno5_df_tangent_side(out, line, alpha5 + k * 5, h, p->recon_variable, g);
     store_pt(gp2 + k * 24, out[0]);
     store_pt(gp2 + k * 24 + 12, out[1]);
+  }
+  return GKS_OK;
+}
+
+// WENO7/9_AO_with_df for n cells: w (n x (order) x 4, x-like frame), alpha
+// (n x order) -> left/right line states (n x 8: convar, der1x)
+int orc2d_weno_normal(const gks2d_params* p, int64_t n, int32_t order, const double* w, const double* alpha,
+                      double h, double* left8, double* right8) {
+  if (order != 7 && order != 9) return GKS_ERR_ARG;
+  orc::Gas g = orc::gas_from(*p);
+  for (int64_t k = 0; k < n; k++) {
+    const double* wk = w + k * order * 4;
+    const double* wv[9];
+    for (int s = 0; s < order; s++) wv[s] = wk + s * 4;
+    orc::Pt<double> L, R;
+    const int r = (order - 1) / 2;
+    if (order == 7) orc::weno79_normal<7>(L, R, alpha + k * order + r, wv + r, h, p->recon_variable, g);
+    else orc::weno79_normal<9>(L, R, alpha + k * order + r, wv + r, h, p->recon_variable, g);
+    std::memcpy(left8 + k * 8, L.convar, 32);
+    std::memcpy(left8 + k * 8 + 4, L.der1x, 32);
+    std::memcpy(right8 + k * 8, R.convar, 32);
+    std::memcpy(right8 + k * 8 + 4, R.der1x, 32);
+  }
+  return GKS_OK;
+}
+// weno_7th/9th_ao_with_df_tangential for n faces: lineL/lineR (n x order x
+// 8: convar, der1x), a1/a2 (n x order) -> gpL/gpR (n x NG x 12)
+int orc2d_weno_tangent(const gks2d_params* p, int64_t n, int32_t order, const double* lineL, const double* lineR,
+                       const double* a1, const double* a2, double h, double* gpL, double* gpR) {
+  if (order != 7 && order != 9) return GKS_ERR_ARG;
+  orc::Gas g = orc::gas_from(*p);
+  const int ng = order == 7 ? 3 : 4;
+  for (int64_t k = 0; k < n; k++) {
+    orc::Pt<double> L[9], R[9], oL[4], oR[4];
+    for (int s = 0; s < order; s++) {
+      std::memcpy(L[s].convar, lineL + (k * order + s) * 8, 32);
+      std::memcpy(L[s].der1x, lineL + (k * order + s) * 8 + 4, 32);
+      std::memcpy(R[s].convar, lineR + (k * order + s) * 8, 32);
+      std::memcpy(R[s].der1x, lineR + (k * order + s) * 8 + 4, 32);
+      for (int v = 0; v < 4; v++) L[s].der1y[v] = R[s].der1y[v] = 0.0;
+    }
+    if (order == 7)
+      orc::weno79_tangent_face<7>(oL, oR, L, R, a1 + k * order, a2 + k * order, h, p->recon_variable, g);
+    else
+      orc::weno79_tangent_face<9>(oL, oR, L, R, a1 + k * order, a2 + k * order, h, p->recon_variable, g);
+    for (int q = 0; q < ng; q++) {
+      store_pt(gpL + (k * ng + q) * 12, oL[q]);
+      store_pt(gpR + (k * ng + q) * 12, oR[q]);
+    }
   }
   return GKS_OK;
 }

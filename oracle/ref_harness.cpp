@@ -319,6 +319,58 @@ int ref2d_weno5_tangent(const gks2d_params* p, int64_t n, const double* line5,
   return GKS_OK;
 }
 
+// WENO7_AO_with_df / WENO9_AO_with_df for n cells (layout as orc2d_weno_normal)
+int ref2d_weno_normal(const gks2d_params* p, int64_t n, int32_t order, const double* w, const double* alpha,
+                      double h, double* left8, double* right8) {
+  if (order != 7 && order != 9) return GKS_ERR_ARG;
+  silence();
+  set_globals(p);
+  for (int64_t k = 0; k < n; k++) {
+    Point2d L, R;
+    double wk[9][4], a[9];
+    std::memcpy(wk, w + k * order * 4, order * 32);
+    std::memcpy(a, alpha + k * order, order * 8);
+    if (order == 7) WENO7_AO_with_df(L, R, a, wk[0], wk[1], wk[2], wk[3], wk[4], wk[5], wk[6], h);
+    else WENO9_AO_with_df(L, R, a, wk[0], wk[1], wk[2], wk[3], wk[4], wk[5], wk[6], wk[7], wk[8], h);
+    std::memcpy(left8 + k * 8, L.convar, 32);
+    std::memcpy(left8 + k * 8 + 4, L.der1x, 32);
+    std::memcpy(right8 + k * 8, R.convar, 32);
+    std::memcpy(right8 + k * 8 + 4, R.der1x, 32);
+  }
+  return GKS_OK;
+}
+// weno_7th/9th_ao_with_df_tangential for n faces (layout as orc2d_weno_tangent)
+int ref2d_weno_tangent(const gks2d_params* p, int64_t n, int32_t order, const double* lineL, const double* lineR,
+                       const double* a1, const double* a2, double h, double* gpL, double* gpR) {
+  if (order != 7 && order != 9) return GKS_ERR_ARG;
+  silence();
+  set_globals(p);
+  const int ng = order == 7 ? 3 : 4;
+  gausspoint = ng;
+  for (int64_t k = 0; k < n; k++) {
+    Recon2d line[9], re[4];
+    for (int s = 0; s < order; s++) {
+      std::memcpy(line[s].left.convar, lineL + (k * order + s) * 8, 32);
+      std::memcpy(line[s].left.der1x, lineL + (k * order + s) * 8 + 4, 32);
+      std::memcpy(line[s].right.convar, lineR + (k * order + s) * 8, 32);
+      std::memcpy(line[s].right.der1x, lineR + (k * order + s) * 8 + 4, 32);
+    }
+    double b1[9], b2[9];
+    std::memcpy(b1, a1 + k * order, order * 8);
+    std::memcpy(b2, a2 + k * order, order * 8);
+    if (order == 7)
+      weno_7th_ao_with_df_tangential(re, line[0], line[1], line[2], line[3], line[4], line[5], line[6], b1, b2, h);
+    else
+      weno_9th_ao_with_df_tangential(re, line[0], line[1], line[2], line[3], line[4], line[5], line[6], line[7],
+                                     line[8], b1, b2, h);
+    for (int q = 0; q < ng; q++) {
+      put_point(gpL + (k * ng + q) * 12, re[q].left);
+      put_point(gpR + (k * ng + q) * 12, re[q].right);
+    }
+  }
+  return GKS_OK;
+}
+
 }  // extern "C"
 
 // =================================================================== 1D

@@ -8,6 +8,7 @@
 
 #include "../include/gks2d.h"
 #include "gks_core.hpp"
+#include "weno79.hpp"
 
 namespace orc {
 
@@ -180,7 +181,9 @@ template <class R> struct Solver2d {
   // Reconstruction_within_cell: reconstruction.cpp:1415-1437 with the
   // WENO5_AO_with_df_normal (:1701-1729) and _tangent (:2553-2576) leaves.
   void reconstruct() {
-    if (P.weno_order != 5 || gp != 2) return;  // other orders: weno79.hpp
+    if (P.weno_order == 7 && gp == 3) return reconstruct79<7>();
+    if (P.weno_order == 9 && gp == 4) return reconstruct79<9>();
+    if (P.weno_order != 5 || gp != 2) return;
 #pragma omp parallel for
     for (int i = 0; i < nx; i++)
       for (int j = 0; j < ny; j++) {
@@ -234,6 +237,68 @@ template <class R> struct Solver2d {
           }
           weno5_df_tangent_side(&ygL[(size_t)f * gp], lineL, a1, dx, P.recon_variable, g);
           weno5_df_tangent_side(&ygR[(size_t)f * gp], lineR, a2, dx, P.recon_variable, g);
+        }
+      }
+  }
+
+  // Reconstruction_within_cell with the WENO7/9_AO_with_df_normal and
+  // _tangent leaves (reconstruction.cpp:1861-1892, 2030-2061, 2840-2865,
+  // 3176-3207): the same sweeps as WENO5 with stencils of radius 3 / 4
+  template <int ORDER> void reconstruct79() {
+    constexpr int r = (ORDER - 1) / 2;
+#pragma omp parallel for
+    for (int i = 0; i < nx; i++)
+      for (int j = 0; j < ny; j++) {
+        if (i > G - 2 && i < nx - G + 1) {
+          R w[2 * r + 1][4], a[2 * r + 1];
+          const R* wv[2 * r + 1];
+          for (int s = 0; s < 2 * r + 1; s++) {
+            get_cell(w[s], cell(i - r + s, j));
+            a[s] = ax[cell(i - r + s, j)];
+            wv[s] = w[s];
+          }
+          weno79_normal<ORDER>(xlR[face(i, j)], xlL[face(i + 1, j)], a + r, wv + r, dx, P.recon_variable, g);
+        }
+        if (j > G - 2 && j < ny - G + 1) {
+          R w[2 * r + 1][4], a[2 * r + 1], tmp[4];
+          const R* wv[2 * r + 1];
+          for (int s = 0; s < 2 * r + 1; s++) {
+            get_cell(tmp, cell(i, j - r + s));
+            y_to_x(w[s], tmp);
+            a[s] = ay[cell(i, j - r + s)];
+            wv[s] = w[s];
+          }
+          weno79_normal<ORDER>(ylR[face(i, j)], ylL[face(i, j + 1)], a + r, wv + r, dy, P.recon_variable, g);
+        }
+      }
+#pragma omp parallel for
+    for (int i = G - 1; i < nx - G + 1; i++)
+      for (int j = G - 1; j < ny - G + 1; j++) {
+        {  // x-face (i+1, j): faces (i+1, j-r..j+r)
+          int f = face(i + 1, j);
+          R a1[2 * r + 1], a2[2 * r + 1];
+          Pt<R> lineL[2 * r + 1], lineR[2 * r + 1];
+          for (int s = 0; s < 2 * r + 1; s++) {
+            a1[s] = ay[cell(i, j - r + s)];
+            a2[s] = ay[cell(i + 1, j - r + s)];
+            lineL[s] = xlL[face(i + 1, j - r + s)];
+            lineR[s] = xlR[face(i + 1, j - r + s)];
+          }
+          weno79_tangent_face<ORDER>(&xgL[(size_t)f * gp], &xgR[(size_t)f * gp], lineL, lineR, a1, a2, dy,
+                                     P.recon_variable, g);
+        }
+        {  // y-face (i, j+1): faces (i+r..i-r, j+1), reversed
+          int f = face(i, j + 1);
+          R a1[2 * r + 1], a2[2 * r + 1];
+          Pt<R> lineL[2 * r + 1], lineR[2 * r + 1];
+          for (int s = 0; s < 2 * r + 1; s++) {
+            a1[s] = ax[cell(i + r - s, j)];
+            a2[s] = ax[cell(i + r - s, j + 1)];
+            lineL[s] = ylL[face(i + r - s, j + 1)];
+            lineR[s] = ylR[face(i + r - s, j + 1)];
+          }
+          weno79_tangent_face<ORDER>(&ygL[(size_t)f * gp], &ygR[(size_t)f * gp], lineL, lineR, a1, a2, dx,
+                                     P.recon_variable, g);
         }
       }
   }

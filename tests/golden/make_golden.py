@@ -194,11 +194,89 @@ def gen_flux_points_1d(ref1):
     np.savez_compressed(os.path.join(HERE, "gks1d_flux_points.npz"), **out)
 
 
+def weno_states(rng, n, m):
+    """n stencils of m cells (x-like frame conservative states): smooth
+    ramps, noisy, and single jumps, with positive pressure"""
+    base = np.array([1.0, 0.3, -0.2, 2.5])
+    w = np.zeros((n, m, 4))
+    for k in range(n):
+        b = base * rng.uniform(0.5, 1.5, 4)
+        b[3] = max(b[3], 1.0 + 0.5 * (b[1] ** 2 + b[2] ** 2) / b[0])
+        kind = k % 3
+        if kind == 0:
+            w[k] = b + np.outer(np.arange(m) - m // 2, rng.normal(0, 0.02, 4))
+        elif kind == 1:
+            w[k] = b * (1 + rng.normal(0, 0.1, (m, 4)))
+        else:
+            j = rng.integers(1, m)
+            w[k] = b
+            w[k, j:] = b * rng.uniform(0.3, 0.9, 4)
+        w[k, :, 3] = np.maximum(w[k, :, 3], 0.6 + 0.5 * (w[k, :, 1] ** 2 + w[k, :, 2] ** 2) / w[k, :, 0])
+    return w
+
+
+def gen_weno79(ref):
+    """WENO7/9-AO-DF leaves: normal (both faces of a cell) and tangential
+    (both sides of a face, 3 or 4 Gauss points)"""
+    rng = np.random.default_rng(79)
+    out = {}
+    n = 96
+    for order in (7, 9):
+        for name, recon in (("char", 1), ("cons", 0)):
+            p = planar_shock_params(16, weno_order=order, recon_variable=recon)
+            key = f"o{order}_{name}"
+            h = 1.0 / 64
+            w = weno_states(rng, n, order)
+            a = rng.choice([0.0, 0.0, 1.5, 3.0, 8.0], (n, order))
+            lo, ro = ref.weno_normal(p, order, w, a, h)
+            lineL = np.zeros((n, order, 8))
+            lineR = np.zeros((n, order, 8))
+            lineL[:, :, :4] = weno_states(rng, n, order)
+            lineR[:, :, :4] = weno_states(rng, n, order)
+            lineL[:, :, 4:] = rng.normal(0, 0.3, (n, order, 4))
+            lineR[:, :, 4:] = rng.normal(0, 0.3, (n, order, 4))
+            a2 = rng.choice([0.0, 0.0, 1.5, 3.0, 8.0], (n, order))
+            gl, gr = ref.weno_tangent(p, order, lineL, lineR, a, a2, h)
+            out.update({f"{key}_w": w, f"{key}_alpha": a, f"{key}_alpha2": a2, f"{key}_h": np.array(h),
+                        f"{key}_left": lo, f"{key}_right": ro, f"{key}_lineL": lineL, f"{key}_lineR": lineR,
+                        f"{key}_gpL": gl, f"{key}_gpR": gr})
+    np.savez_compressed(os.path.join(HERE, "weno79_df.npz"), **out)
+
+
+# WENO7 / WENO9 runs: name -> (order, recon, n, zones seed, bc, steps)
+RUNS79 = {
+    "w7_char_free": (7, 1, 28, 0, [BC_FREE] * 4, 20),
+    "w7_cons_periodic": (7, 0, 24, 7, [BC_PERIODIC] * 4, 15),
+    "w9_char_mixed": (9, 1, 24, 3, [BC_REFLECT, BC_FREE, BC_PERIODIC, BC_PERIODIC], 15),
+    "w9_cons_free": (9, 0, 24, 11, [BC_FREE] * 4, 15),
+}
+
+
+def gen_runs79(ref):
+    out = {}
+    G = 5
+    for name, (order, recon, n, seed, bc, steps) in RUNS79.items():
+        p = planar_shock_params(n, weno_order=order, recon_variable=recon)
+        prim = riemann_prim(p, riemann_states(seed))
+        s = ref.solver(p)
+        s.set_prim(prim)
+        rc, hist = s.run(bc, 0.25, steps)
+        assert rc == 0
+        st = s.get()
+        out.update({f"{name}_convar": st["convar"][:, G:-G, G:-G], f"{name}_alpha_x": st["alpha_x"][G:-G, G:-G],
+                    f"{name}_alpha_y": st["alpha_y"][G:-G, G:-G], f"{name}_dt": hist[:steps],
+                    f"{name}_t": np.array(st["t"])})
+        s.close()
+    np.savez_compressed(os.path.join(HERE, "runs79.npz"), **out)
+
+
 def main():
     ref = reference()
     gen_flux_points(ref)
     gen_weno(ref)
     gen_runs(ref)
+    gen_weno79(ref)
+    gen_runs79(ref)
     ref1 = reference1d()
     gen_runs1d(ref1)
     gen_flux_points_1d(ref1)

@@ -42,6 +42,9 @@ def _bind(lib, prefix):
         "flux_points": (ctypes.c_int, [pp, ctypes.c_int64, _D, _D, ctypes.c_double, _D, _D, _D]),
         "weno5_normal": (ctypes.c_int, [pp, ctypes.c_int64, _D, _D, ctypes.c_double, _D, _D]),
         "weno5_tangent": (ctypes.c_int, [pp, ctypes.c_int64, _D, _D, ctypes.c_double, _D]),
+        "weno_normal": (ctypes.c_int, [pp, ctypes.c_int64, ctypes.c_int32, _D, _D, ctypes.c_double, _D, _D]),
+        "weno_tangent": (ctypes.c_int, [pp, ctypes.c_int64, ctypes.c_int32, _D, _D, _D, _D, ctypes.c_double, _D,
+                                        _D]),
     }
     fns = {}
     for name, (res, args) in sig.items():
@@ -164,6 +167,33 @@ class CpuLib:
         out = np.zeros((n, 2, 12))
         self.fns["weno5_tangent"](ctypes.byref(p), n, dptr(line5), dptr(alpha5), h, dptr(out))
         return out
+
+
+    def weno_normal(self, p, order, w, alpha, h):
+        """WENO7/9_AO_with_df: w (n, order, 4), alpha (n, order) -> left, right (n, 8)"""
+        w = np.ascontiguousarray(w, dtype=np.float64)
+        alpha = np.ascontiguousarray(alpha, dtype=np.float64)
+        n = w.shape[0]
+        lo = np.zeros((n, 8))
+        ro = np.zeros((n, 8))
+        rc = self.fns["weno_normal"](ctypes.byref(p), n, order, dptr(w), dptr(alpha), h, dptr(lo), dptr(ro))
+        assert rc == 0
+        return lo, ro
+
+    def weno_tangent(self, p, order, lineL, lineR, a1, a2, h):
+        """weno_7th/9th tangential: lines (n, order, 8) -> gpL, gpR (n, NG, 12)"""
+        lineL = np.ascontiguousarray(lineL, dtype=np.float64)
+        lineR = np.ascontiguousarray(lineR, dtype=np.float64)
+        a1 = np.ascontiguousarray(a1, dtype=np.float64)
+        a2 = np.ascontiguousarray(a2, dtype=np.float64)
+        n = lineL.shape[0]
+        ng = 3 if order == 7 else 4
+        gl = np.zeros((n, ng, 12))
+        gr = np.zeros((n, ng, 12))
+        rc = self.fns["weno_tangent"](ctypes.byref(p), n, order, dptr(lineL), dptr(lineR), dptr(a1), dptr(a2), h,
+                                      dptr(gl), dptr(gr))
+        assert rc == 0
+        return gl, gr
 
 
 _oracle = None
