@@ -49,8 +49,9 @@ int check_params(const gks2d_params* p) {
   if (!p) return fail(GKS_ERR_ARG, "null params");
   if (p->ghost != 5) return fail(GKS_ERR_UNSUPPORTED, "ghost must be 5");
   if (p->nodex < 5 || p->nodey < 5) return fail(GKS_ERR_ARG, "nodex and nodey must be >= 5");
-  if (p->weno_order != 5 || p->gausspoint != 2)
-    return fail(GKS_ERR_UNSUPPORTED, "implemented: WENO5-AO with 2 Gauss points");
+  if (!((p->weno_order == 5 && p->gausspoint == 2) || (p->weno_order == 7 && p->gausspoint == 3) ||
+        (p->weno_order == 9 && p->gausspoint == 4)))
+    return fail(GKS_ERR_UNSUPPORTED, "implemented: WENO5/7/9-AO with 2/3/4 Gauss points");
   if (p->use_df != 1) return fail(GKS_ERR_UNSUPPORTED, "implemented: the DF (discontinuity feedback) variants");
   if (p->recon_variable != GKS_VAR_CONSERVATIVE && p->recon_variable != GKS_VAR_CHARACTERISTIC)
     return fail(GKS_ERR_ARG, "recon_variable");
@@ -178,8 +179,20 @@ int gks2d_create(const gks2d_params* p, gks2d_ctx** out) {
   d.area = d.dx * d.dy;
   d.idx = 1.0 / d.dx;
   d.idy = 1.0 / d.dy;
-  d.gw[0] = 0.5;  // SetGuassPoint, fluid_mesh.cpp:19-23
-  d.gw[1] = 0.5;
+  d.order = p->weno_order;
+  if (d.gp == 2) {  // SetGuassPoint, fluid_mesh.cpp:9-45
+    d.gw[0] = 0.5;
+    d.gw[1] = 0.5;
+  } else if (d.gp == 3) {
+    d.gw[0] = 4.0 / 9.0;
+    d.gw[1] = 0.5 - 0.5 * d.gw[0];
+    d.gw[2] = d.gw[1];
+  } else {
+    d.gw[0] = 1.0 / 12.0;
+    d.gw[1] = 1.0 / 12.0;
+    d.gw[2] = 5.0 / 12.0;
+    d.gw[3] = 5.0 / 12.0;
+  }
   d.gas = gas_of(*p);
   d.nc = (long long)d.nx * d.ny;
   d.nfx = (long long)(d.nx + 1) * d.ny;

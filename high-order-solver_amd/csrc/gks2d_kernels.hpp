@@ -23,6 +23,7 @@
 
 #include "gks2d_device.hpp"
 #include "gks2d_launch.hpp"
+#include "gks2d_weno79_device.hpp"
 
 #ifndef GKS_VARIANT
 #error "GKS_VARIANT must name the kernel namespace"
@@ -354,6 +355,237 @@ template <int DIR> __global__ void __launch_bounds__(kBlock) k_tangent(Dev d) {
     for (int k = 0; k < 12; k++) gst[((side * d.gp + q) * 12 + k) * nf + f0] = gp[q][k];
 }
 
+// ------------------------------------------------- WENO7 / WENO9 (normal)
+// WENO7_AO_with_df / WENO9_AO_with_df (reconstruction.cpp:1894-2028,
+// 2063-2211) for one cell along DIR; w[0..2r] in the x-like frame
+template <int ORDER>
+__device__ __forceinline__ void weno79_normal(double* lo, double* ro, const double (*w)[4], const double* alpha,
+                                              double h, double ih, int recon_var, const Gas& g) {
+  constexpr int r = (ORDER - 1) / 2, NS = ORDER == 7 ? 5 : 6;
+  double df[NS];
+  df79<ORDER>(df, alpha + r);
+  double pn1[4], p0[4], pp1[4];
+  convar_to_primvar(pn1, w[r - 1], g);
+  convar_to_primvar(p0, w[r], g);
+  convar_to_primvar(pp1, w[r + 1], g);
+#pragma unroll
+  for (int side = 0; side < 2; side++) {
+    double base[4];
+#pragma unroll
+    for (int i = 0; i < 4; i++) base[i] = side == 0 ? 0.5 * (pn1[i] + p0[i]) : 0.5 * (pp1[i] + p0[i]);
+    double c[2 * r + 1][4];
+    CharBase cb;
+    if (recon_var == 0) {
+#pragma unroll
+      for (int s = 0; s < 2 * r + 1; s++)
+#pragma unroll
+        for (int i = 0; i < 4; i++) c[s][i] = w[s][i];
+    } else {
+      char_base(cb, base, g);
+#pragma unroll
+      for (int s = 0; s < 2 * r + 1; s++) to_char(c[s], cb, w[s]);
+    }
+    double var[4], d1[4];
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+      double line[2 * r + 1];
+#pragma unroll
+      for (int s = 0; s < 2 * r + 1; s++) line[s] = c[s][i];
+      if (side == 0) weno79_face<ORDER, 0>(var[i], d1[i], line + r, df, h, ih);
+      else weno79_face<ORDER, 1>(var[i], d1[i], line + r, df, h, ih);
+    }
+    double* out = side == 0 ? lo : ro;
+    if (recon_var == 0) {
+#pragma unroll
+      for (int i = 0; i < 4; i++) {
+        out[i] = var[i];
+        out[4 + i] = d1[i];
+      }
+    } else {
+      from_char(out, cb, var, g);
+      from_char(out + 4, cb, d1, g);
+    }
+  }
+  const bool rr = reduce_order(ro, g);
+  const bool rl = reduce_order(lo, g);
+  if (rl || rr) {
+#pragma unroll
+    for (int m = 0; m < 4; m++) {
+      ro[m] = w[r][m];
+      lo[m] = w[r][m];
+      ro[4 + m] = 0.0;
+      lo[4 + m] = 0.0;
+    }
+  }
+}
+
+// the cells and faces of k_normal, with stencils of radius r
+template <int DIR, int ORDER> __global__ void __launch_bounds__(kBlock) k_normal79(Dev d) {
+  constexpr int r = (ORDER - 1) / 2;
+  long long t = gtid();
+  int i, j;
+  if (DIR == 0) {
+    if (t >= (long long)(d.NX + 2) * d.ny) return;
+    i = (int)(t / d.ny) + d.G - 1;
+    j = (int)(t % d.ny);
+  } else {
+    if (t >= (long long)d.nx * (d.NY + 2)) return;
+    i = (int)(t / (d.NY + 2));
+    j = (int)(t % (d.NY + 2)) + d.G - 1;
+  }
+  double w[2 * r + 1][4], a[2 * r + 1];
+#pragma unroll
+  for (int s = 0; s < 2 * r + 1; s++) {
+    const long long c = DIR == 0 ? (long long)(i - r + s) * d.ny + j : (long long)i * d.ny + (j - r + s);
+    double u[4];
+#pragma unroll
+    for (int v = 0; v < 4; v++) u[v] = d.U[v * d.nc + c];
+    if (DIR == 0) {
+#pragma unroll
+      for (int v = 0; v < 4; v++) w[s][v] = u[v];
+      a[s] = d.ax[c];
+    } else {
+      w[s][0] = u[0];  // YchangetoX
+      w[s][1] = u[2];
+      w[s][2] = -u[1];
+      w[s][3] = u[3];
+      a[s] = d.ay[c];
+    }
+  }
+  double lo[8], ro[8];
+  weno79_normal<ORDER>(lo, ro, w, a, DIR == 0 ? d.dx : d.dy, DIR == 0 ? d.idx : d.idy, d.recon_var, d.gas);
+  double* line = DIR == 0 ? d.line_x : d.line_y;
+  const long long nf = DIR == 0 ? d.nfx : d.nfy;
+  const long long fa = DIR == 0 ? (long long)i * d.ny + j : (long long)i * (d.ny + 1) + j;
+  const long long fb = DIR == 0 ? (long long)(i + 1) * d.ny + j : (long long)i * (d.ny + 1) + j + 1;
+#pragma unroll
+  for (int k = 0; k < 8; k++) {
+    line[(8 + k) * nf + fa] = lo[k];
+    line[k * nf + fb] = ro[k];
+  }
+}
+
+// --------------------------------------------- WENO7 / WENO9 (tangential)
+// weno_7th/9th_ao_with_df_tangential (reconstruction.cpp:2867-3018,
+// 3209-3360) for one side of one face: NG = 3 / 4 Gauss points.  The
+// convar components go first (values and der1y, then the order-reduction
+// test, which reads only them), der1x after, so the line's der1x values are
+// loaded only when needed.  WENO9, characteristic variables, right side:
+// the outermost two points keep the left side's projection (`quirk`: left
+// side convar of line points 0, r, 2r), as the reference leaves them
+// (:3306-3312).
+template <int DIR, int ORDER> __global__ void __launch_bounds__(kBlock) k_tangent79(Dev d) {
+  constexpr int r = (ORDER - 1) / 2, NS = ORDER == 7 ? 5 : 6, NG = ORDER == 7 ? 3 : 4;
+  long long t = gtid();
+  const int ni = DIR == 0 ? d.NX + 1 : d.NX, nj = DIR == 0 ? d.NY : d.NY + 1;
+  if (t >= 2LL * ni * nj) return;
+  const int side = (int)(t / ((long long)ni * nj));
+  const long long rr = t % ((long long)ni * nj);
+  const int i = (int)(rr / nj) + d.G, j = (int)(rr % nj) + d.G;
+  const double* line = DIR == 0 ? d.line_x : d.line_y;
+  const long long nf = DIR == 0 ? d.nfx : d.nfy;
+  long long fs[2 * r + 1];
+  double a[2 * r + 1];
+#pragma unroll
+  for (int s = 0; s < 2 * r + 1; s++) {
+    long long c;
+    if (DIR == 0) {
+      fs[s] = (long long)i * d.ny + (j - r + s);
+      c = (long long)(side == 0 ? i - 1 : i) * d.ny + (j - r + s);
+      a[s] = d.ay[c];
+    } else {
+      fs[s] = (long long)(i + r - s) * (d.ny + 1) + j;
+      c = (long long)(i + r - s) * d.ny + (side == 0 ? j - 1 : j);
+      a[s] = d.ax[c];
+    }
+  }
+  double df[NS];
+  df79<ORDER>(df, a + r);
+  const double h = DIR == 0 ? d.dy : d.dx, ih = DIR == 0 ? d.idy : d.idx;
+  double L[2 * r + 1][4];
+#pragma unroll
+  for (int s = 0; s < 2 * r + 1; s++)
+#pragma unroll
+    for (int k = 0; k < 4; k++) L[s][k] = line[(side * 8 + k) * nf + fs[s]];
+  double gv[NG][4], gy[NG][4];
+  if (d.recon_var == 0) {
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+      double w[2 * r + 1], v[NG], vx[NG];
+#pragma unroll
+      for (int s = 0; s < 2 * r + 1; s++) w[s] = L[s][k];
+      weno79_gauss<ORDER>(v, vx, w + r, df, h, ih);
+#pragma unroll
+      for (int q = 0; q < NG; q++) {
+        gv[q][k] = v[q];
+        gy[q][k] = vx[q];
+      }
+    }
+  } else {
+    double base[4];
+    convar_to_primvar(base, L[r], d.gas);
+    CharBase cb;
+    char_base(cb, base, d.gas);
+    double c[2 * r + 1][4];
+#pragma unroll
+    for (int s = 0; s < 2 * r + 1; s++) to_char(c[s], cb, L[s]);
+    if (ORDER == 9 && side == 1) {
+      double q0[4], qr[4], q2r[4], bl[4];
+#pragma unroll
+      for (int k = 0; k < 4; k++) {
+        q0[k] = line[k * nf + fs[0]];
+        qr[k] = line[k * nf + fs[r]];
+        q2r[k] = line[k * nf + fs[2 * r]];
+      }
+      convar_to_primvar(bl, qr, d.gas);
+      CharBase cl;
+      char_base(cl, bl, d.gas);
+      to_char(c[0], cl, q0);
+      to_char(c[2 * r], cl, q2r);
+    }
+    double var[NG][4], d1[NG][4];
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+      double w[2 * r + 1], v[NG], vx[NG];
+#pragma unroll
+      for (int s = 0; s < 2 * r + 1; s++) w[s] = c[s][k];
+      weno79_gauss<ORDER>(v, vx, w + r, df, h, ih);
+#pragma unroll
+      for (int q = 0; q < NG; q++) {
+        var[q][k] = v[q];
+        d1[q][k] = vx[q];
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < NG; q++) {
+      from_char(gv[q], cb, var[q], d.gas);
+      from_char(gy[q], cb, d1[q], d.gas);
+    }
+  }
+  bool red[NG];
+#pragma unroll
+  for (int q = 0; q < NG; q++) red[q] = reduce_order(gv[q], d.gas);
+  double* gst = DIR == 0 ? d.gst_x : d.gst_y;
+  const long long f0 = DIR == 0 ? (long long)i * d.ny + j : (long long)i * (d.ny + 1) + j;
+#pragma unroll
+  for (int q = 0; q < NG; q++)
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+      gst[((side * d.gp + q) * 12 + k) * nf + f0] = red[q] ? L[r][k] : gv[q][k];
+      gst[((side * d.gp + q) * 12 + 8 + k) * nf + f0] = red[q] ? 0.0 : gy[q][k];
+    }
+  // der1x along the face
+#pragma unroll
+  for (int k = 0; k < 4; k++) {
+    double w[2 * r + 1], v[NG], vx[NG];
+#pragma unroll
+    for (int s = 0; s < 2 * r + 1; s++) w[s] = line[(side * 8 + 4 + k) * nf + fs[s]];
+    weno79_gauss<ORDER>(v, vx, w + r, df, h, ih);
+#pragma unroll
+    for (int q = 0; q < NG; q++) gst[((side * d.gp + q) * 12 + 4 + k) * nf + f0] = red[q] ? w[r] : v[q];
+  }
+}
+
 // -------------------------------------------------------------------- flux
 // Centre state + GKS2D per (face, Gauss point); the S2O4 combination of the
 // Gauss point's flux for this stage (time_advance.cpp:380-426, weighted by the
@@ -502,6 +734,14 @@ __global__ void __launch_bounds__(kBlock) k_flux_points(Gas g, long long n, cons
 
 // explicit instantiations: the launchers below are host-only, so the device
 // pass must be told which template kernels to emit
+template __global__ void k_normal79<0, 7>(Dev);
+template __global__ void k_normal79<1, 7>(Dev);
+template __global__ void k_normal79<0, 9>(Dev);
+template __global__ void k_normal79<1, 9>(Dev);
+template __global__ void k_tangent79<0, 7>(Dev);
+template __global__ void k_tangent79<1, 7>(Dev);
+template __global__ void k_tangent79<0, 9>(Dev);
+template __global__ void k_tangent79<1, 9>(Dev);
 template __global__ void k_normal<0>(Dev);
 template __global__ void k_normal<1>(Dev);
 template __global__ void k_tangent<0>(Dev);
@@ -530,13 +770,31 @@ static hipError_t copy_old(const Dev& d, hipStream_t s) {
   return hipGetLastError();
 }
 static hipError_t normal(const Dev& d, int dir, hipStream_t s) {
-  if (dir == 0) hipLaunchKernelGGL(k_normal<0>, dim3(blocks_for((long long)(d.NX + 2) * d.ny)), dim3(kBlock), 0, s, d);
-  else hipLaunchKernelGGL(k_normal<1>, dim3(blocks_for((long long)d.nx * (d.NY + 2))), dim3(kBlock), 0, s, d);
+  const unsigned nb = dir == 0 ? blocks_for((long long)(d.NX + 2) * d.ny) : blocks_for((long long)d.nx * (d.NY + 2));
+  if (d.order == 7) {
+    if (dir == 0) hipLaunchKernelGGL((k_normal79<0, 7>), dim3(nb), dim3(kBlock), 0, s, d);
+    else hipLaunchKernelGGL((k_normal79<1, 7>), dim3(nb), dim3(kBlock), 0, s, d);
+  } else if (d.order == 9) {
+    if (dir == 0) hipLaunchKernelGGL((k_normal79<0, 9>), dim3(nb), dim3(kBlock), 0, s, d);
+    else hipLaunchKernelGGL((k_normal79<1, 9>), dim3(nb), dim3(kBlock), 0, s, d);
+  } else {
+    if (dir == 0) hipLaunchKernelGGL(k_normal<0>, dim3(nb), dim3(kBlock), 0, s, d);
+    else hipLaunchKernelGGL(k_normal<1>, dim3(nb), dim3(kBlock), 0, s, d);
+  }
   return hipGetLastError();
 }
 static hipError_t tangent(const Dev& d, int dir, hipStream_t s) {
-  if (dir == 0) hipLaunchKernelGGL(k_tangent<0>, dim3(blocks_for(2LL * (d.NX + 1) * d.NY)), dim3(kBlock), 0, s, d);
-  else hipLaunchKernelGGL(k_tangent<1>, dim3(blocks_for(2LL * d.NX * (d.NY + 1))), dim3(kBlock), 0, s, d);
+  const unsigned nb = dir == 0 ? blocks_for(2LL * (d.NX + 1) * d.NY) : blocks_for(2LL * d.NX * (d.NY + 1));
+  if (d.order == 7) {
+    if (dir == 0) hipLaunchKernelGGL((k_tangent79<0, 7>), dim3(nb), dim3(kBlock), 0, s, d);
+    else hipLaunchKernelGGL((k_tangent79<1, 7>), dim3(nb), dim3(kBlock), 0, s, d);
+  } else if (d.order == 9) {
+    if (dir == 0) hipLaunchKernelGGL((k_tangent79<0, 9>), dim3(nb), dim3(kBlock), 0, s, d);
+    else hipLaunchKernelGGL((k_tangent79<1, 9>), dim3(nb), dim3(kBlock), 0, s, d);
+  } else {
+    if (dir == 0) hipLaunchKernelGGL(k_tangent<0>, dim3(nb), dim3(kBlock), 0, s, d);
+    else hipLaunchKernelGGL(k_tangent<1>, dim3(nb), dim3(kBlock), 0, s, d);
+  }
   return hipGetLastError();
 }
 static hipError_t flux(const Dev& d, int dir, int stage, double dt, hipStream_t s) {
