@@ -164,6 +164,17 @@ class Solver2D:
         self.t = 0.0
         self.step_count = 0
 
+    def init_convar(self, u):
+        """Start from given conservative cell averages: the step-0 CFL reads
+        their primitive variables (Convar_to_primvar_2D)."""
+        from .problems import convar_to_prim
+        u = np.asarray(u, dtype=np.float64)
+        self.set_convar(u)
+        with np.errstate(invalid="ignore", divide="ignore"):
+            self.set_cfl_primvar(convar_to_prim(u, self.p.gamma))
+        self.t = 0.0
+        self.step_count = 0
+
     # ---------------------------------------------------------------- sweeps
     def copy_old(self):
         _check(lib().gks2d_copy_old(self.h))

@@ -1,0 +1,108 @@
+"""Problem drivers over the GPU solvers, and the reference's output formats.
+
+The drivers restate the set-up and time loop of the reference's problem
+functions (reference src/1Dproblem.cpp, src/2Dproblem.cpp) for the
+workloads of this build; every step runs on the GPU through the C ABI.
+
+* ``run_sod``      - SodTubeProblem (1Dproblem.cpp:3-149), N cells to t=0.2;
+* ``run_riemann``  - PlanarShock-style four-quadrant Riemann problem
+                     (2Dproblem.cpp:3-168 with IC_for_riemann_2d, :345-385)
+                     from seeded quadrant states;
+* ``run_vortex``   - isentropic vortex advection on the periodic [0,10]^2,
+                     with the L1 error against the exact solution (the
+                     initial field moved by t (1, 1), periodically);
+* ``write_output1d`` / ``write_output2d`` - the text formats of output1d /
+  output2d (output.cpp:3-15, 64-84): density of the interior cells, one
+  value per line, six significant digits, x fastest in 2D.
+"""
+import os
+import time
+
+import numpy as np
+
+from . import Solver2D
+from .params import BC_FREE, BC_PERIODIC, planar_shock_params, sod_params, vortex_params
+from .problems import riemann_prim, riemann_states, sod_prim, vortex_convar
+from .solver1d import Solver1D
+
+
+def _g(x):
+    return "%g" % x
+
+
+def write_output1d(path, convar, p):
+    """output1d: interior densities, one per line (`R.txt`)."""
+    G = p.ghost
+    with open(path, "w") as f:
+        f.write("\n".join(_g(v) for v in convar[0, G:G + p.nodex]) + "\n")
+
+
+def write_output2d(directory, convar, p, step):
+    """output2d: `R-<step>.txt` with the interior densities, x fastest."""
+    G = p.ghost
+    rho = convar[0, G:G + p.nodex, G:G + p.nodey]
+    path = os.path.join(directory, f"R-{step}.txt")
+    with open(path, "w") as f:
+        f.write("\n".join(_g(v) for v in rho.T.reshape(-1)) + "\n")
+    return path
+
+
+def run_sod(n=400, tstop=0.2, use_df=0, fast_math=1):
+    p = sod_params(n, use_df=use_df, fast_math=fast_math)
+    s = Solver1D(p)
+    s.init_prim(sod_prim(p))
+    t0 = time.perf_counter()
+    steps = s.run([BC_FREE, BC_FREE], tstop)
+    wall = time.perf_counter() - t0
+    u = s.get_convar()
+    s.close()
+    return dict(params=p, convar=u, steps=steps, wall_s=wall, mcells=n * steps / wall / 1e6)
+
+
+def run_riemann(n, seed=0, tstop=0.25, weno_order=5, fast_math=1, max_steps=2**31 - 1):
+    p = planar_shock_params(n, weno_order=weno_order, fast_math=fast_math)
+    s = Solver2D(p)
+    s.init_prim(riemann_prim(p, riemann_states(seed)))
+    s.sync()
+    t0 = time.perf_counter()
+    steps = s.run([BC_FREE] * 4, tstop, max_steps)
+    s.sync()
+    wall = time.perf_counter() - t0
+    u = s.get_convar()
+    s.close()
+    return dict(params=p, convar=u, steps=steps, wall_s=wall, mcells=n * n * steps / wall / 1e6)
+
+
+def vortex_l1(convar, p, t):
+    """Mean absolute error per conservative variable over the interior
+    against the exact cell averages at time t."""
+    G = p.ghost
+    exact = vortex_convar(p, shift=(t, t))
+    d = np.abs(convar[:, G:G + p.nodex, G:G + p.nodey] - exact[:, G:G + p.nodex, G:G + p.nodey])
+    return d.mean(axis=(1, 2))
+
+
+def run_vortex(n, tstop=10.0, weno_order=7, fast_math=1, max_steps=2**31 - 1, steps_per_call=None):
+    """Isentropic vortex to tstop; steps_per_call splits the run into calls
+    of that many steps (for progress output); returns the L1 errors."""
+    p = vortex_params(n, weno_order=weno_order, fast_math=fast_math)
+    s = Solver2D(p)
+    s.init_convar(vortex_convar(p))
+    s.sync()
+    t0 = time.perf_counter()
+    steps = 0
+    chunk = steps_per_call or max_steps
+    while s.t < tstop and steps < max_steps:
+        k = s.run([BC_PERIODIC] * 4, tstop, min(chunk, max_steps - steps))
+        steps += k
+        if steps_per_call:
+            print(f"vortex n={n}: step {steps} t={s.t:.4f}", flush=True)
+        if k == 0:
+            break
+    s.sync()
+    wall = time.perf_counter() - t0
+    u = s.get_convar()
+    t = s.t
+    s.close()
+    return dict(params=p, convar=u, steps=steps, t=t, wall_s=wall, mcells=n * n * steps / wall / 1e6,
+                l1=vortex_l1(u, p, t))

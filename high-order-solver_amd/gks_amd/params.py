@@ -127,3 +127,17 @@ def sinwave1d_params(n, cfl=0.1, **kw):
     kw = dict(dict(recon_variable=VAR_CONSERVATIVE, df_from_start=1, c1_euler=0.0, c2_euler=0.0,
                    left=0.0, right=2.0, cfl=cfl), **kw)
     return sod_params(n, **kw)
+
+
+def vortex_params(n, weno_order=7, cfl=0.5, **kw):
+    """Isentropic vortex advection on the periodic square [0, 10]^2 (a
+    workload of this build; the reference has no vortex driver): K=3,
+    gamma=1.4, the smooth-flow collision time of the reference's accuracy
+    drivers (Euler tau with c1=c2=0, sinwave_2d, src/2Dproblem.cpp:1358-1361),
+    WENO7-AO (3 Gauss points) with the DF factor in characteristic
+    variables, S2O4, CFL 0.5."""
+    p = planar_shock_params(n, weno_order=weno_order, cfl=cfl, c1_euler=0.0, c2_euler=0.0,
+                            left=0.0, right=10.0, down=0.0, up=10.0)
+    for k, v in kw.items():
+        setattr(p, k, v)
+    return p

@@ -160,3 +160,50 @@ def sinwave_exact_1d(p):
     """Exact cell-averaged density of the sine wave at any t that is a whole
     number of periods (error_for_sinwave, reference src/1Dproblem.cpp:708-735)."""
     return sinwave_prim_1d(p)[0, p.ghost:p.nx - p.ghost]
+
+
+# ------------------------------------------------------------ isentropic vortex
+def vortex_prim_point(x, y, gamma=1.4, eps=5.0, xc=5.0, yc=5.0, free=(1.0, 1.0, 1.0, 1.0)):
+    """Point values (rho, u, v, p) of the isentropic vortex of strength eps
+    centred at (xc, yc) in the free stream (rho, u, v, p) = free (with
+    p/rho = 1): du = -eps/(2 pi) e^((1-r^2)/2) (y-yc), dv = eps/(2 pi)
+    e^((1-r^2)/2) (x-xc), T = 1 - (gamma-1) eps^2/(8 gamma pi^2) e^(1-r^2),
+    rho = T^(1/(gamma-1)), p = rho T."""
+    r2 = (x - xc) ** 2 + (y - yc) ** 2
+    a = eps / (2 * np.pi) * np.exp(0.5 * (1.0 - r2))
+    T = 1.0 - (gamma - 1) * eps * eps / (8 * gamma * np.pi ** 2) * np.exp(1.0 - r2)
+    rho = free[0] * T ** (1.0 / (gamma - 1))
+    return np.stack([rho, free[1] - a * (y - yc), free[2] + a * (x - xc), rho * T])
+
+
+def vortex_convar(p, eps=5.0, quad=5, shift=(0.0, 0.0)):
+    """Cell averages of the conservative variables [4, nx, ny] (ghost cells
+    included) by quad x quad Gauss-Legendre points per cell; the vortex
+    centre moved by `shift` and wrapped into the periodic square (the exact
+    solution at time t is the initial one shifted by t (u, v) = (t, t))."""
+    G, nx, ny = p.ghost, p.nx, p.ny
+    dx = (p.right - p.left) / p.nodex
+    dy = (p.up - p.down) / p.nodey
+    Lx, Ly = p.right - p.left, p.up - p.down
+    xg, wg = np.polynomial.legendre.leggauss(quad)
+    xc0 = np.mod(5.0 + shift[0], Lx)
+    yc0 = np.mod(5.0 + shift[1], Ly)
+    out = np.zeros((4, nx, ny))
+    cx = p.left + (np.arange(nx) - G + 0.5) * dx
+    cy = p.down + (np.arange(ny) - G + 0.5) * dy
+    for a, wa in zip(xg, wg):
+        for b, wb in zip(xg, wg):
+            X = (cx + 0.5 * dx * a)[:, None] * np.ones((1, ny))
+            Y = np.ones((nx, 1)) * (cy + 0.5 * dy * b)[None, :]
+            # nearest periodic image of the centre
+            ddx = np.mod(X - xc0 + 0.5 * Lx, Lx) - 0.5 * Lx
+            ddy = np.mod(Y - yc0 + 0.5 * Ly, Ly) - 0.5 * Ly
+            pr = vortex_prim_point(ddx, ddy, p.gamma, eps, 0.0, 0.0)
+            out += 0.25 * wa * wb * prim_to_convar(pr, p.gamma)
+    return out
+
+
+def convar_to_prim(c, gamma):
+    """Convar_to_primvar_2D (reference src/basic_function.cpp:811-817)."""
+    r, mu, mv, E = c
+    return np.stack([r, mu / r, mv / r, (gamma - 1.0) * (E - 0.5 * mu * mu / r - 0.5 * mv * mv / r)])

@@ -68,6 +68,12 @@ int orc2d_set_convar(void* h, const double* conv) {
   for (int v = 0; v < 4; v++) std::memcpy(s->U[v].data(), conv + v * nc, nc * 8);
   return GKS_OK;
 }
+// primvar of every cell from the current convar (Convar_to_Primvar,
+// basic_function.cpp:752-773), for starts from conservative cell averages
+int orc2d_primvar(void* h) {
+  static_cast<S2*>(h)->primvar();
+  return GKS_OK;
+}
 int orc2d_set_alpha(void* h, const double* ax, const double* ay) {
   S2* s = static_cast<S2*>(h);
   size_t nc = (size_t)s->nx * s->ny;

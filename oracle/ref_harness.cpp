@@ -139,6 +139,12 @@ int ref2d_set_convar(void* h, const double* conv) {
     for (int v = 0; v < 4; v++) c->fluids[k].convar[v] = conv[v * nc + k];
   return GKS_OK;
 }
+int ref2d_primvar(void* h) {
+  RefCtx* c = static_cast<RefCtx*>(h);
+  set_globals(&c->p);
+  Convar_to_Primvar(c->fluids, c->block);
+  return GKS_OK;
+}
 int ref2d_set_alpha(void* h, const double* ax, const double* ay) {
   RefCtx* c = static_cast<RefCtx*>(h);
   size_t nc = (size_t)c->block.nx * c->block.ny;

@@ -30,6 +30,7 @@ def _bind(lib, prefix):
         "destroy": (None, [_P]),
         "set_prim": (ctypes.c_int, [_P, _D]),
         "set_convar": (ctypes.c_int, [_P, _D]),
+        "primvar": (ctypes.c_int, [_P]),
         "set_alpha": (ctypes.c_int, [_P, _D, _D]),
         "get": (ctypes.c_int, [_P, _D, _D, _D, _D, _D, _I]),
         "set_time": (ctypes.c_int, [_P, ctypes.c_double, ctypes.c_int32]),
@@ -88,6 +89,11 @@ class CpuSolver:
         ax = np.ascontiguousarray(ax, dtype=np.float64)
         ay = np.ascontiguousarray(ay, dtype=np.float64)
         return self.f["set_alpha"](self.h, dptr(ax), dptr(ay))
+
+    def init_convar(self, u):
+        """start from conservative cell averages; primvar from them"""
+        self.set_convar(u)
+        return self.f["primvar"](self.h)
 
     def get(self):
         u = np.zeros((4, self.nx, self.ny))
