@@ -33,14 +33,24 @@ FLOPS_FLUX_POINT = 3041 + 43 + 5 + 3  # centre + GKS2D + DF alpha_k per (face, G
 FLOPS_TANGENT_SIDE = 2857  # tangential WENO per (face, side)
 FLOPS_NORMAL_CELL = 2253  # normal WENO per (cell, direction)
 PEAK_FP64_TFLOPS = 78.6  # MI355X FP64 (vector and matrix)
+# HBM bytes per launch of the dominant kernels at 2048^2, fast kernels, from
+# rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE (profiles/gks2d_riemann2048_fast_hbm_traffic.md),
+# each scaled by the counter's reading for k_copy_old (4 x 2048^2 doubles read
+# and written): fetch x1.882, write x0.985
+TRAFFIC_2048_FAST = {"flux_x": 1.2409e9, "flux_y": 1.2289e9}
 PEAK_HBM_GBS = 8000.0
 
 
-def kernel_units(phase, n, gp=2):
-    """Work units one launch of `phase` processes on an n x n grid (G=5)."""
+def kernel_units(phase, n, gp=2, fused=False):
+    """Work units one launch of `phase` processes on an n x n grid (G=5).
+    With the tangential reconstruction inside the flux launch (WENO5), a
+    flux launch does both: FLOPs per face = 2 sides x tangential + gp x
+    flux point, per launch (n+1) n faces."""
     G = 5
     nx = n + 2 * G
     if phase in ("flux_x", "flux_y"):
+        if fused:
+            return (n + 1) * n, 2 * FLOPS_TANGENT_SIDE + gp * FLOPS_FLUX_POINT
         return gp * (n + 1) * n, FLOPS_FLUX_POINT
     if phase in ("tangent_x", "tangent_y"):
         return 2 * (n + 1) * n, FLOPS_TANGENT_SIDE
@@ -127,13 +137,17 @@ def main():
 
     # dominant kernel
     dom = max(tim, key=lambda ph: tim[ph][0])
-    units, fpu = kernel_units(dom, n)
+    # the tangential phases take no time when they run inside the flux launch
+    fused = tim["tangent_x"][0] < 0.05 * tim["flux_x"][0]
+    units, fpu = kernel_units(dom, n, fused=fused)
     avg_ms = tim[dom][0] / max(tim[dom][1], 1)
     roof = None
     if units:
         achieved = units * fpu / (avg_ms * 1e-3) / 1e12
+        traffic = TRAFFIC_2048_FAST.get(dom) if (n == 2048 and args.variant == "fast" and fused) else None
         roof = {"bound": "mfma", "achieved": round(achieved, 3), "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
-                "frac": round(achieved / PEAK_FP64_TFLOPS, 4), "traffic": None, "kernel": dom,
+                "frac": round(achieved / PEAK_FP64_TFLOPS, 4), "traffic": traffic,
+                "kernel": dom + (" (tangential reconstruction + flux, one launch)" if fused and dom.startswith("flux") else ""),
                 "avg_ms": round(avg_ms, 4), "flops_per_launch": units * fpu}
     total_dev_ms = sum(v[0] for v in tim.values())
     step_flops = FLOPS_PER_CELL_UPDATE * n * n

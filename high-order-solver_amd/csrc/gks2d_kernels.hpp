@@ -599,22 +599,12 @@ template <int DIR, int ORDER> __global__ void __launch_bounds__(kBlock) k_tangen
 // stage 1 uses (f0 + 1/6 derf0) + (0 f1 + 1/3 derf1).  Stage 0 stores the
 // first bracket (times the weight) for stage 1; the reference's running sum
 // starts from 0.0, which adds nothing.
-template <int DIR> __global__ void __launch_bounds__(kBlock) k_flux(Dev d, int stage, double dt) {
-  long long t = gtid();
-  const int ni = DIR == 0 ? d.NX + 1 : d.NX, nj = DIR == 0 ? d.NY : d.NY + 1;
-  if (t >= (long long)d.gp * ni * nj) return;
-  const int q = (int)(t / ((long long)ni * nj));
-  const long long r = t % ((long long)ni * nj);
-  const int i = (int)(r / nj) + d.G, j = (int)(r % nj) + d.G;
-  const long long nf = DIR == 0 ? d.nfx : d.nfy;
-  const long long f = DIR == 0 ? (long long)i * d.ny + j : (long long)i * (d.ny + 1) + j;
-  const double* gst = DIR == 0 ? d.gst_x : d.gst_y;
-  double L[12], R[12];
-#pragma unroll
-  for (int k = 0; k < 12; k++) {
-    L[k] = gst[((0 * d.gp + q) * 12 + k) * nf + f];
-    R[k] = gst[((1 * d.gp + q) * 12 + k) * nf + f];
-  }
+// the flux of one (face, Gauss point) from its two Gauss-point states, and
+// everything stored per point: the stage-combined rotated flux, the stage-1
+// share, the raw (f, derf) when kept, and Calculate_alpha_k
+template <int DIR>
+__device__ __forceinline__ void flux_point_tail(const Dev& d, int q, long long f, long long nf, const double* L,
+                                                const double* R, int stage, double dt) {
   double fo[4], dfo[4];
   gks_point(fo, dfo, nullptr, L, R, dt, d.gas);
   if (d.keep_face_fluxes) {
@@ -656,6 +646,81 @@ template <int DIR> __global__ void __launch_bounds__(kBlock) k_flux(Dev d, int s
     a = alpha_k(tl, tr, d.gas);
   }
   (DIR == 0 ? d.akx : d.aky)[q * nf + f] = a;
+}
+
+template <int DIR> __global__ void __launch_bounds__(kBlock) k_flux(Dev d, int stage, double dt) {
+  long long t = gtid();
+  const int ni = DIR == 0 ? d.NX + 1 : d.NX, nj = DIR == 0 ? d.NY : d.NY + 1;
+  if (t >= (long long)d.gp * ni * nj) return;
+  const int q = (int)(t / ((long long)ni * nj));
+  const long long r = t % ((long long)ni * nj);
+  const int i = (int)(r / nj) + d.G, j = (int)(r % nj) + d.G;
+  const long long nf = DIR == 0 ? d.nfx : d.nfy;
+  const long long f = DIR == 0 ? (long long)i * d.ny + j : (long long)i * (d.ny + 1) + j;
+  const double* gst = DIR == 0 ? d.gst_x : d.gst_y;
+  double L[12], R[12];
+#pragma unroll
+  for (int k = 0; k < 12; k++) {
+    L[k] = gst[((0 * d.gp + q) * 12 + k) * nf + f];
+    R[k] = gst[((1 * d.gp + q) * 12 + k) * nf + f];
+  }
+  flux_point_tail<DIR>(d, q, f, nf, L, R, stage, dt);
+}
+
+// ------------------------------------------- tangential + flux, one launch
+// WENO5 (2 Gauss points): a workgroup takes kTanFlux consecutive faces.
+// Phase 1, thread (side, face): the tangential reconstruction of k_tangent,
+// its Gauss-point states kept in LDS; phase 2, thread (Gauss point, face):
+// the flux of k_flux from those states.  The Gauss-point states never go to
+// HBM (48 doubles per face each way).
+constexpr int kTanFlux = 128;
+template <int DIR> __global__ void __launch_bounds__(2 * kTanFlux) k_tanflux(Dev d, int stage, double dt) {
+  __shared__ double sg[2][2][12][kTanFlux];  // [side][gauss point][component][face]
+  const int ni = DIR == 0 ? d.NX + 1 : d.NX, nj = DIR == 0 ? d.NY : d.NY + 1;
+  const long long nfaces = (long long)ni * nj;
+  const int tl = threadIdx.x % kTanFlux, hi = threadIdx.x / kTanFlux;
+  const long long r = (long long)blockIdx.x * kTanFlux + tl;
+  const bool valid = r < nfaces;
+  const int i = valid ? (int)(r / nj) + d.G : d.G, j = valid ? (int)(r % nj) + d.G : d.G;
+  const long long nf = DIR == 0 ? d.nfx : d.nfy;
+  const long long f0 = DIR == 0 ? (long long)i * d.ny + j : (long long)i * (d.ny + 1) + j;
+  if (valid) {
+    const int side = hi;
+    const double* line = DIR == 0 ? d.line_x : d.line_y;
+    double L[5][8], a[5];
+#pragma unroll
+    for (int s = 0; s < 5; s++) {
+      long long f, c;
+      if (DIR == 0) {
+        f = (long long)i * d.ny + (j - 2 + s);
+        c = (long long)(side == 0 ? i - 1 : i) * d.ny + (j - 2 + s);
+        a[s] = d.ay[c];
+      } else {
+        f = (long long)(i + 2 - s) * (d.ny + 1) + j;
+        c = (long long)(i + 2 - s) * d.ny + (side == 0 ? j - 1 : j);
+        a[s] = d.ax[c];
+      }
+#pragma unroll
+      for (int k = 0; k < 8; k++) L[s][k] = line[(side * 8 + k) * nf + f];
+    }
+    double gp[2][12];
+    weno5_tangent(gp, L, a, DIR == 0 ? d.dy : d.dx, DIR == 0 ? d.idy : d.idx, d.recon_var, d.gas);
+#pragma unroll
+    for (int q = 0; q < 2; q++)
+#pragma unroll
+      for (int k = 0; k < 12; k++) sg[side][q][k][tl] = gp[q][k];
+  }
+  __syncthreads();
+  if (valid) {
+    const int q = hi;
+    double L[12], R[12];
+#pragma unroll
+    for (int k = 0; k < 12; k++) {
+      L[k] = sg[0][q][k][tl];
+      R[k] = sg[1][q][k][tl];
+    }
+    flux_point_tail<DIR>(d, q, f0, nf, L, R, stage, dt);
+  }
 }
 
 // ------------------------------------------------------------------ update
@@ -746,6 +811,8 @@ template __global__ void k_normal<0>(Dev);
 template __global__ void k_normal<1>(Dev);
 template __global__ void k_tangent<0>(Dev);
 template __global__ void k_tangent<1>(Dev);
+template __global__ void k_tanflux<0>(Dev, int, double);
+template __global__ void k_tanflux<1>(Dev, int, double);
 template __global__ void k_flux<0>(Dev, int, double);
 template __global__ void k_flux<1>(Dev, int, double);
 
@@ -791,13 +858,20 @@ static hipError_t tangent(const Dev& d, int dir, hipStream_t s) {
   } else if (d.order == 9) {
     if (dir == 0) hipLaunchKernelGGL((k_tangent79<0, 9>), dim3(nb), dim3(kBlock), 0, s, d);
     else hipLaunchKernelGGL((k_tangent79<1, 9>), dim3(nb), dim3(kBlock), 0, s, d);
-  } else {
+  } else if (!d.fuse_tanflux) {
     if (dir == 0) hipLaunchKernelGGL(k_tangent<0>, dim3(nb), dim3(kBlock), 0, s, d);
     else hipLaunchKernelGGL(k_tangent<1>, dim3(nb), dim3(kBlock), 0, s, d);
   }
   return hipGetLastError();
 }
 static hipError_t flux(const Dev& d, int dir, int stage, double dt, hipStream_t s) {
+  if (d.order == 5 && d.fuse_tanflux) {
+    const long long nfaces = dir == 0 ? (long long)(d.NX + 1) * d.NY : (long long)d.NX * (d.NY + 1);
+    const unsigned nb = (unsigned)((nfaces + kTanFlux - 1) / kTanFlux);
+    if (dir == 0) hipLaunchKernelGGL(k_tanflux<0>, dim3(nb), dim3(2 * kTanFlux), 0, s, d, stage, dt);
+    else hipLaunchKernelGGL(k_tanflux<1>, dim3(nb), dim3(2 * kTanFlux), 0, s, d, stage, dt);
+    return hipGetLastError();
+  }
   if (dir == 0)
     hipLaunchKernelGGL(k_flux<0>, dim3(blocks_for((long long)d.gp * (d.NX + 1) * d.NY)), dim3(kBlock), 0, s, d,
                        stage, dt);

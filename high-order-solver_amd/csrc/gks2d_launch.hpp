@@ -8,6 +8,7 @@
 struct Dev {
   int NX, NY, G, nx, ny, gp, recon_var, use_df;
   int order;             // WENO order: 5, 7 or 9 (gp 2, 3, 4)
+  int fuse_tanflux;      // WENO5: tangential reconstruction inside the flux launch
   int keep_face_fluxes;  // store raw (f, derf) per face for parity checks
   double dx, dy, area;
   double idx, idy;  // 1/dx, 1/dy (fast build)
