@@ -103,6 +103,9 @@ def main():
     ap.add_argument("--cpu-n", type=int, default=2048)
     ap.add_argument("--cpu-steps", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--path", choices=["plain", "learned"], default="plain",
+                    help="learned: the trained troubled-interface classifier picks linear weights for smooth stencils")
+    ap.add_argument("--classifier", default=None, help="trained classifier (.npz); default: the packaged one")
     args = ap.parse_args()
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -122,6 +125,12 @@ def main():
     prim = riemann_prim(p, riemann_states(args.seed))
     s = gks_amd.Solver2D(p)
     s.init_prim(prim)
+    learned = None
+    if args.path == "learned":
+        from gks_amd import learn
+        P, thr, _ = learn.load(args.classifier or learn.DEFAULT_PARAMS)
+        learn.set_classifier(s, P, thr)
+        learned = (learn, thr)
     bc = [BC_FREE] * 4
     s.run(bc, 0.25, args.warmup)
     s.sync()
@@ -144,12 +153,17 @@ def main():
     roof = None
     if units:
         achieved = units * fpu / (avg_ms * 1e-3) / 1e12
-        traffic = TRAFFIC_2048_FAST.get(dom) if (n == 2048 and args.variant == "fast" and fused) else None
+        traffic = (TRAFFIC_2048_FAST.get(dom) if (n == 2048 and args.variant == "fast" and fused and args.path == "plain")
+                   else None)
         roof = {"bound": "mfma", "achieved": round(achieved, 3), "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
                 "frac": round(achieved / PEAK_FP64_TFLOPS, 4), "traffic": traffic,
                 "kernel": dom + (" (tangential reconstruction + flux, one launch)" if fused and dom.startswith("flux") else ""),
                 "avg_ms": round(avg_ms, 4), "flops_per_launch": units * fpu}
     total_dev_ms = sum(v[0] for v in tim.values())
+    learned_info = None
+    if learned:
+        lin, tot = learned[0].classifier_stats(s)
+        learned_info = {"linear_share": round(lin / max(tot, 1), 4), "logit_threshold": learned[1]}
     step_flops = FLOPS_PER_CELL_UPDATE * n * n
     out = {
         "metric": "Mcell-updates/s FP64, 2D 2048² WENO5-AO GKS: learned vs plain at equal L1",
@@ -165,13 +179,15 @@ def main():
         "dtype": "f64",
         "data": "synthetic",
         "config": {"workload": f"four-quadrant Riemann {n}x{n} seed {args.seed}, WENO5-AO-DF characteristic, "
-                               f"GKS2D gks2nd, S2O4, CFL 0.5, free BCs (plain path, {args.variant} kernels)",
-                   "n": n, "path": "plain", "kernels": args.variant},
+                               f"GKS2D gks2nd, S2O4, CFL 0.5, free BCs ({args.path} path, {args.variant} kernels)",
+                   "n": n, "path": args.path, "kernels": args.variant},
         "roofline": roof,
         "step_fp64_tflops": round(step_flops * k / wall / 1e12, 3),
         "phase_ms_per_step": {ph: round(v[0] / k, 4) for ph, v in tim.items()},
         "device_ms_per_step": round(total_dev_ms / k, 3),
     }
+    if learned_info:
+        out["learned"] = learned_info
     if not args.no_cpu_baseline:
         threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or (os.cpu_count() or 1)
         out["cpu_baseline"] = cpu_baseline(args.cpu_n, threads, args.cpu_steps)

@@ -10,6 +10,7 @@
 #include <vector>
 
 #include "../../include/gks2d.h"
+#include "../../include/gks_learn.h"
 #include "gks2d_launch.hpp"
 
 namespace {
@@ -70,6 +71,8 @@ struct gks2d_ctx {
   hipEvent_t ev = nullptr;
   double* h_stat = nullptr;  // pinned copy of d.stat
   double* d_prim = nullptr;  // staging for gks2d_set_cfl_primvar
+  float* d_mlp = nullptr;    // classifier parameters (gks2d_set_classifier)
+  unsigned long long* d_lstat = nullptr;  // its linear / total counters
   std::vector<void*> allocs;
   int last_slot = -1;        // slot of the last primvar / set_cfl_primvar
   int last_flux_stage = -1;
@@ -151,6 +154,8 @@ int gks2d_destroy(gks2d_ctx* c) {
   for (void* a : c->allocs) (void)hipFree(a);
   for (hipEvent_t e : c->pool) (void)hipEventDestroy(e);
   if (c->h_stat) (void)hipHostFree(c->h_stat);
+  if (c->d_mlp) (void)hipFree(c->d_mlp);
+  if (c->d_lstat) (void)hipFree(c->d_lstat);
   if (c->ev) (void)hipEventDestroy(c->ev);
   if (c->s) (void)hipStreamDestroy(c->s);
   delete c;
@@ -527,6 +532,68 @@ int gks2d_flux_points(const gks2d_params* p, int64_t n, const double* left, cons
   (void)hipFree(dfo);
   (void)hipFree(dd);
   if (e != hipSuccess) return hip_fail(e, "gks2d_flux_points");
+  return GKS_OK;
+}
+
+// ------------------------------------------------- learned classifier hooks
+// 2 kernels (normal, tangential) x 1024 slots x (linear, total) counters
+constexpr size_t kLstatWords = 2 * 1024 * 2;
+int gks2d_classifier_samples(gks2d_ctx* c, int32_t dir, int32_t stride, float* features, float* labels, int64_t cap,
+                             int64_t* n_out) {
+  if (!c || !features || !labels || !n_out) return fail(GKS_ERR_ARG, "null");
+  if (dir < 0 || dir > 1 || stride < 1) return fail(GKS_ERR_ARG, "dir / stride");
+  if (c->p.weno_order != 5) return fail(GKS_ERR_UNSUPPORTED, "the classifier is for WENO5-AO");
+  const Dev& d = c->d;
+  const long long total = dir == 0 ? (long long)(d.NX + 2) * d.ny : (long long)d.nx * (d.NY + 2);
+  const long long ns = (total + stride - 1) / stride;
+  if (ns > cap) return fail(GKS_ERR_ARG, "cap smaller than the number of samples");
+  float *dX = nullptr, *dY = nullptr;
+  hipError_t e = hipMalloc(&dX, ns * 16 * sizeof(float));
+  if (e == hipSuccess) e = hipMalloc(&dY, ns * sizeof(float));
+  if (e == hipSuccess) e = c->lt->label(d, dir, stride, dX, dY, c->s);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->s);
+  if (e == hipSuccess) e = hipMemcpy(features, dX, ns * 16 * sizeof(float), hipMemcpyDeviceToHost);
+  if (e == hipSuccess) e = hipMemcpy(labels, dY, ns * sizeof(float), hipMemcpyDeviceToHost);
+  (void)hipFree(dX);
+  (void)hipFree(dY);
+  if (e != hipSuccess) return hip_fail(e, "gks2d_classifier_samples");
+  *n_out = ns;
+  return GKS_OK;
+}
+
+int gks2d_set_classifier(gks2d_ctx* c, const float* params, float logit_threshold) {
+  if (!c) return fail(GKS_ERR_ARG, "null");
+  HIPCHK(hipStreamSynchronize(c->s));
+  if (!params) {
+    c->d.mlp = nullptr;
+    return GKS_OK;
+  }
+  if (c->p.weno_order != 5) return fail(GKS_ERR_UNSUPPORTED, "the classifier is for WENO5-AO");
+  if (!c->d_mlp) {
+    HIPCHK(hipMalloc(&c->d_mlp, GKS_MLP_NPARAM * sizeof(float)));
+    HIPCHK(hipMalloc(&c->d_lstat, kLstatWords * sizeof(unsigned long long)));
+  }
+  HIPCHK(hipMemcpy(c->d_mlp, params, GKS_MLP_NPARAM * sizeof(float), hipMemcpyHostToDevice));
+  HIPCHK(hipMemset(c->d_lstat, 0, kLstatWords * sizeof(unsigned long long)));
+  c->d.mlp = c->d_mlp;
+  c->d.mlp_thr = logit_threshold;
+  c->d.lstat = c->d_lstat;
+  return GKS_OK;
+}
+
+int gks2d_classifier_stats(gks2d_ctx* c, int64_t* linear, int64_t* total) {
+  if (!c || !linear || !total) return fail(GKS_ERR_ARG, "null");
+  if (!c->d_lstat) return fail(GKS_ERR_STATE, "no classifier set");
+  std::vector<unsigned long long> h(kLstatWords);
+  HIPCHK(hipStreamSynchronize(c->s));
+  HIPCHK(hipMemcpy(h.data(), c->d_lstat, h.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost));
+  unsigned long long lin = 0, tot = 0;
+  for (size_t k = 0; k < h.size(); k += 2) {
+    lin += h[k];
+    tot += h[k + 1];
+  }
+  *linear = (int64_t)lin;
+  *total = (int64_t)tot;
   return GKS_OK;
 }
 

@@ -24,6 +24,7 @@
 #include "gks2d_device.hpp"
 #include "gks2d_launch.hpp"
 #include "gks2d_weno79_device.hpp"
+#include "gks2d_learn_device.hpp"
 
 #ifndef GKS_VARIANT
 #error "GKS_VARIANT must name the kernel namespace"
@@ -355,6 +356,159 @@ template <int DIR> __global__ void __launch_bounds__(kBlock) k_tangent(Dev d) {
     for (int k = 0; k < 12; k++) gst[((side * d.gp + q) * 12 + k) * nf + f0] = gp[q][k];
 }
 
+// ---------------------------------------------------- learned WENO5 normal
+// cell t of k_normal's range along DIR, and its stencil (x-like frame)
+template <int DIR> __device__ __forceinline__ void normal_cell(const Dev& d, long long t, int& i, int& j) {
+  if (DIR == 0) {
+    i = (int)(t / d.ny) + d.G - 1;
+    j = (int)(t % d.ny);
+  } else {
+    i = (int)(t / (d.NY + 2));
+    j = (int)(t % (d.NY + 2)) + d.G - 1;
+  }
+}
+template <int DIR> __device__ __forceinline__ long long normal_count(const Dev& d) {
+  return DIR == 0 ? (long long)(d.NX + 2) * d.ny : (long long)d.nx * (d.NY + 2);
+}
+template <int DIR>
+__device__ __forceinline__ void normal_stencil(const Dev& d, int i, int j, double (*w)[4], double* a) {
+#pragma unroll
+  for (int s = 0; s < 5; s++) {
+    const long long c = DIR == 0 ? (long long)(i - 2 + s) * d.ny + j : (long long)i * d.ny + (j - 2 + s);
+    double u[4];
+#pragma unroll
+    for (int v = 0; v < 4; v++) u[v] = d.U[v * d.nc + c];
+    if (DIR == 0) {
+#pragma unroll
+      for (int v = 0; v < 4; v++) w[s][v] = u[v];
+      a[s] = d.ax[c];
+    } else {
+      w[s][0] = u[0];  // YchangetoX
+      w[s][1] = u[2];
+      w[s][2] = -u[1];
+      w[s][3] = u[3];
+      a[s] = d.ay[c];
+    }
+  }
+}
+
+// classifier training samples: features and label of every stride-th cell
+template <int DIR> __global__ void __launch_bounds__(kBlock) k_label(Dev d, int stride, float* X, float* Y) {
+  const long long t = gtid();
+  const long long ns = (normal_count<DIR>(d) + stride - 1) / stride;
+  if (t >= ns) return;
+  int i, j;
+  normal_cell<DIR>(d, t * stride, i, j);
+  double w[5][4], a[5];
+  normal_stencil<DIR>(d, i, j, w, a);
+  double q[2][5];
+  stencil_rho_p(q, w, d.gas);
+  float x[16];
+  stencil_features(x, q);
+#pragma unroll
+  for (int k = 0; k < 16; k++) X[t * 16 + k] = x[k];
+  Y[t] = normal_label(w, a, DIR == 0 ? d.dx : d.dy, DIR == 0 ? d.idx : d.idy, d.recon_var, d.gas);
+}
+
+// counts of linear / all reconstructions of a wave, added by its first
+// active lane to one of kStatSlots slot pairs (spread by block so that the
+// adds do not queue on one address); the host sums the slots
+constexpr int kStatSlots = 1024;
+__device__ __forceinline__ void count_linear(unsigned long long* stat, bool lin) {
+  const unsigned long long act = __ballot(1), bl = __ballot(lin);
+  if ((threadIdx.x & 63) == __ffsll((long long)act) - 1) {
+    unsigned long long* slot = stat + 2 * ((blockIdx.x * 4 + (threadIdx.x >> 6)) % kStatSlots);
+    atomicAdd(slot, (unsigned long long)__popcll(bl));
+    atomicAdd(slot + 1, (unsigned long long)__popcll(act));
+  }
+}
+
+// k_normal with the classifier: a stencil the network calls smooth (logit
+// below the threshold) with a zero DF sensor takes the linear weights, i.e.
+// the fifth-order candidate, formed directly on the conservative variables
+// (the characteristic projection commutes with a fixed linear combination);
+// the order-reduction test is kept.  Every other stencil runs WENO5-AO-DF.
+// the classifier's parameters copied to LDS by the block (one load each)
+__device__ __forceinline__ void stage_params(float* ps, const float* P) {
+  for (int k = threadIdx.x; k < kMlpB3 + 1; k += blockDim.x) ps[k] = P[k];
+}
+
+template <int DIR> __global__ void __launch_bounds__(kBlock) k_normal_learned(Dev d) {
+  __shared__ short xs[kBlock * 16];
+  __shared__ float ls[kBlock];
+  __shared__ float ps[kMlpB3 + 1];
+  stage_params(ps, d.mlp);
+  const long long t = gtid();
+  const bool valid = t < normal_count<DIR>(d);
+  int i, j;
+  normal_cell<DIR>(d, valid ? t : 0, i, j);
+  double w[5][4], a[5];
+  float x[16];
+  if (valid) {
+    normal_stencil<DIR>(d, i, j, w, a);
+    double q[2][5];
+    stencil_rho_p(q, w, d.gas);
+    stencil_features(x, q);
+  } else {
+#pragma unroll
+    for (int k = 0; k < 16; k++) x[k] = 0.0f;
+  }
+  const int wv = threadIdx.x >> 6;
+  __syncthreads();
+  const float z = mlp_wave_logit(x, ps, xs + wv * 64 * 16, ls + wv * 64);
+  if (!valid) return;
+  double amax = 0.0;
+#pragma unroll
+  for (int s = 0; s < 5; s++) amax = fmax(amax, a[s]);
+  const bool lin = z < d.mlp_thr && amax == 0.0;
+  count_linear(d.lstat, lin);
+  const double h = DIR == 0 ? d.dx : d.dy, ih = DIR == 0 ? d.idx : d.idy;
+  double lo[8], ro[8];
+  if (lin) {
+#pragma unroll
+    for (int v = 0; v < 4; v++) {
+      weno5_linear_face(0, lo[v], lo[4 + v], w[0][v], w[1][v], w[2][v], w[3][v], w[4][v], h, ih);
+      weno5_linear_face(1, ro[v], ro[4 + v], w[0][v], w[1][v], w[2][v], w[3][v], w[4][v], h, ih);
+    }
+    const bool rr = reduce_order(ro, d.gas);
+    const bool rl = reduce_order(lo, d.gas);
+    if (rl || rr) {
+#pragma unroll
+      for (int m = 0; m < 4; m++) {
+        ro[m] = w[2][m];
+        lo[m] = w[2][m];
+        ro[4 + m] = 0.0;
+        lo[4 + m] = 0.0;
+      }
+    }
+  } else {
+    weno5_normal(lo, ro, w, a, h, ih, d.recon_var, d.gas);
+  }
+  double* line = DIR == 0 ? d.line_x : d.line_y;
+  const long long nf = DIR == 0 ? d.nfx : d.nfy;
+  const long long fa = DIR == 0 ? (long long)i * d.ny + j : (long long)i * (d.ny + 1) + j;
+  const long long fb = DIR == 0 ? (long long)(i + 1) * d.ny + j : (long long)i * (d.ny + 1) + j + 1;
+#pragma unroll
+  for (int k = 0; k < 8; k++) {
+    line[(8 + k) * nf + fa] = lo[k];
+    line[k * nf + fb] = ro[k];
+  }
+}
+
+// logits of n feature rows (parity of the MFMA inference)
+__global__ void __launch_bounds__(kBlock) k_mlp_infer(const float* P, long long n, const float* X, float* out) {
+  __shared__ short xs[kBlock * 16];
+  __shared__ float ls[kBlock];
+  const long long t = gtid();
+  const bool valid = t < n;
+  float x[16];
+#pragma unroll
+  for (int k = 0; k < 16; k++) x[k] = valid ? X[t * 16 + k] : 0.0f;
+  const int wv = threadIdx.x >> 6;
+  const float z = mlp_wave_logit(x, P, xs + wv * 64 * 16, ls + wv * 64);
+  if (valid) out[t] = z;
+}
+
 // ------------------------------------------------- WENO7 / WENO9 (normal)
 // WENO7_AO_with_df / WENO9_AO_with_df (reconstruction.cpp:1894-2028,
 // 2063-2211) for one cell along DIR; w[0..2r] in the x-like frame
@@ -674,7 +828,8 @@ template <int DIR> __global__ void __launch_bounds__(kBlock) k_flux(Dev d, int s
 // the flux of k_flux from those states.  The Gauss-point states never go to
 // HBM (48 doubles per face each way).
 constexpr int kTanFlux = 128;
-template <int DIR> __global__ void __launch_bounds__(2 * kTanFlux) k_tanflux(Dev d, int stage, double dt) {
+template <int DIR, bool LEARNED>
+__global__ void __launch_bounds__(2 * kTanFlux) k_tanflux(Dev d, int stage, double dt) {
   __shared__ double sg[2][2][12][kTanFlux];  // [side][gauss point][component][face]
   const int ni = DIR == 0 ? d.NX + 1 : d.NX, nj = DIR == 0 ? d.NY : d.NY + 1;
   const long long nfaces = (long long)ni * nj;
@@ -684,10 +839,10 @@ template <int DIR> __global__ void __launch_bounds__(2 * kTanFlux) k_tanflux(Dev
   const int i = valid ? (int)(r / nj) + d.G : d.G, j = valid ? (int)(r % nj) + d.G : d.G;
   const long long nf = DIR == 0 ? d.nfx : d.nfy;
   const long long f0 = DIR == 0 ? (long long)i * d.ny + j : (long long)i * (d.ny + 1) + j;
+  const int side = hi;
+  double L[5][8], a[5];
   if (valid) {
-    const int side = hi;
     const double* line = DIR == 0 ? d.line_x : d.line_y;
-    double L[5][8], a[5];
 #pragma unroll
     for (int s = 0; s < 5; s++) {
       long long f, c;
@@ -703,8 +858,64 @@ template <int DIR> __global__ void __launch_bounds__(2 * kTanFlux) k_tanflux(Dev
 #pragma unroll
       for (int k = 0; k < 8; k++) L[s][k] = line[(side * 8 + k) * nf + f];
     }
+  }
+  bool lin = false;
+  if constexpr (LEARNED) {
+    // the classifier on the line values along the face (all lanes take part)
+    __shared__ short xs[2 * kTanFlux * 16];
+    __shared__ float ls[2 * kTanFlux];
+    __shared__ float ps[kMlpB3 + 1];
+    stage_params(ps, d.mlp);
+    float x[16];
+    if (valid) {
+      double w4[5][4], q[2][5];
+#pragma unroll
+      for (int s = 0; s < 5; s++)
+#pragma unroll
+        for (int k = 0; k < 4; k++) w4[s][k] = L[s][k];
+      stencil_rho_p(q, w4, d.gas);
+      stencil_features(x, q);
+    } else {
+#pragma unroll
+      for (int k = 0; k < 16; k++) x[k] = 0.0f;
+    }
+    const int wv = threadIdx.x >> 6;
+    __syncthreads();
+    const float z = mlp_wave_logit(x, ps, xs + wv * 64 * 16, ls + wv * 64);
+    if (valid) {
+      double amax = 0.0;
+#pragma unroll
+      for (int s = 0; s < 5; s++) amax = fmax(amax, a[s]);
+      lin = z < d.mlp_thr && amax == 0.0;
+      count_linear(d.lstat + 2 * kStatSlots, lin);
+    }
+  }
+  if (valid) {
+    const double h = DIR == 0 ? d.dy : d.dx, ih = DIR == 0 ? d.idy : d.idx;
     double gp[2][12];
-    weno5_tangent(gp, L, a, DIR == 0 ? d.dy : d.dx, DIR == 0 ? d.idy : d.idx, d.recon_var, d.gas);
+    if (lin) {
+#pragma unroll
+      for (int v = 0; v < 4; v++) {
+        weno5_linear_2gauss(gp[0][v], gp[0][8 + v], gp[1][v], gp[1][8 + v], L[0][v], L[1][v], L[2][v], L[3][v],
+                            L[4][v], h, ih);
+        double t0, t1;
+        weno5_linear_2gauss(gp[0][4 + v], t0, gp[1][4 + v], t1, L[0][4 + v], L[1][4 + v], L[2][4 + v],
+                            L[3][4 + v], L[4][4 + v], h, ih);
+      }
+#pragma unroll
+      for (int q = 0; q < 2; q++) {
+        if (reduce_order(gp[q], d.gas)) {
+#pragma unroll
+          for (int v = 0; v < 4; v++) {
+            gp[q][v] = L[2][v];
+            gp[q][4 + v] = L[2][4 + v];
+            gp[q][8 + v] = 0.0;
+          }
+        }
+      }
+    } else {
+      weno5_tangent(gp, L, a, h, ih, d.recon_var, d.gas);
+    }
 #pragma unroll
     for (int q = 0; q < 2; q++)
 #pragma unroll
@@ -811,8 +1022,14 @@ template __global__ void k_normal<0>(Dev);
 template __global__ void k_normal<1>(Dev);
 template __global__ void k_tangent<0>(Dev);
 template __global__ void k_tangent<1>(Dev);
-template __global__ void k_tanflux<0>(Dev, int, double);
-template __global__ void k_tanflux<1>(Dev, int, double);
+template __global__ void k_tanflux<0, false>(Dev, int, double);
+template __global__ void k_tanflux<1, false>(Dev, int, double);
+template __global__ void k_tanflux<0, true>(Dev, int, double);
+template __global__ void k_tanflux<1, true>(Dev, int, double);
+template __global__ void k_normal_learned<0>(Dev);
+template __global__ void k_normal_learned<1>(Dev);
+template __global__ void k_label<0>(Dev, int, float*, float*);
+template __global__ void k_label<1>(Dev, int, float*, float*);
 template __global__ void k_flux<0>(Dev, int, double);
 template __global__ void k_flux<1>(Dev, int, double);
 
@@ -844,6 +1061,9 @@ static hipError_t normal(const Dev& d, int dir, hipStream_t s) {
   } else if (d.order == 9) {
     if (dir == 0) hipLaunchKernelGGL((k_normal79<0, 9>), dim3(nb), dim3(kBlock), 0, s, d);
     else hipLaunchKernelGGL((k_normal79<1, 9>), dim3(nb), dim3(kBlock), 0, s, d);
+  } else if (d.mlp) {
+    if (dir == 0) hipLaunchKernelGGL(k_normal_learned<0>, dim3(nb), dim3(kBlock), 0, s, d);
+    else hipLaunchKernelGGL(k_normal_learned<1>, dim3(nb), dim3(kBlock), 0, s, d);
   } else {
     if (dir == 0) hipLaunchKernelGGL(k_normal<0>, dim3(nb), dim3(kBlock), 0, s, d);
     else hipLaunchKernelGGL(k_normal<1>, dim3(nb), dim3(kBlock), 0, s, d);
@@ -868,8 +1088,13 @@ static hipError_t flux(const Dev& d, int dir, int stage, double dt, hipStream_t 
   if (d.order == 5 && d.fuse_tanflux) {
     const long long nfaces = dir == 0 ? (long long)(d.NX + 1) * d.NY : (long long)d.NX * (d.NY + 1);
     const unsigned nb = (unsigned)((nfaces + kTanFlux - 1) / kTanFlux);
-    if (dir == 0) hipLaunchKernelGGL(k_tanflux<0>, dim3(nb), dim3(2 * kTanFlux), 0, s, d, stage, dt);
-    else hipLaunchKernelGGL(k_tanflux<1>, dim3(nb), dim3(2 * kTanFlux), 0, s, d, stage, dt);
+    if (d.mlp) {
+      if (dir == 0) hipLaunchKernelGGL((k_tanflux<0, true>), dim3(nb), dim3(2 * kTanFlux), 0, s, d, stage, dt);
+      else hipLaunchKernelGGL((k_tanflux<1, true>), dim3(nb), dim3(2 * kTanFlux), 0, s, d, stage, dt);
+    } else {
+      if (dir == 0) hipLaunchKernelGGL((k_tanflux<0, false>), dim3(nb), dim3(2 * kTanFlux), 0, s, d, stage, dt);
+      else hipLaunchKernelGGL((k_tanflux<1, false>), dim3(nb), dim3(2 * kTanFlux), 0, s, d, stage, dt);
+    }
     return hipGetLastError();
   }
   if (dir == 0)
@@ -896,7 +1121,19 @@ static hipError_t flux_points(const Gas& g, long long n, const double* L, const 
   return hipGetLastError();
 }
 
-const LaunchTable table = {bc, primvar, copy_old, normal, tangent, flux, update, alpha, flux_points};
+static hipError_t label(const Dev& d, int dir, int stride, float* X, float* Y, hipStream_t s) {
+  const long long total = dir == 0 ? (long long)(d.NX + 2) * d.ny : (long long)d.nx * (d.NY + 2);
+  const long long ns = (total + stride - 1) / stride;
+  if (dir == 0) hipLaunchKernelGGL(k_label<0>, dim3(blocks_for(ns)), dim3(kBlock), 0, s, d, stride, X, Y);
+  else hipLaunchKernelGGL(k_label<1>, dim3(blocks_for(ns)), dim3(kBlock), 0, s, d, stride, X, Y);
+  return hipGetLastError();
+}
+static hipError_t infer(const float* P, long long n, const float* X, float* out, hipStream_t s) {
+  hipLaunchKernelGGL(k_mlp_infer, dim3(blocks_for(n)), dim3(kBlock), 0, s, P, n, X, out);
+  return hipGetLastError();
+}
+
+const LaunchTable table = {bc, primvar, copy_old, normal, tangent, flux, update, alpha, flux_points, label, infer};
 #endif
 
 }  // namespace GKS_VARIANT

@@ -31,6 +31,9 @@ struct Dev {
   double* fly[2];
   double* akx;
   double* aky;
+  const float* mlp;             // classifier parameters (gks_learn.h), null: plain WENO5-AO
+  float mlp_thr;                // logit below which a smooth stencil takes the linear weights
+  unsigned long long* lstat;    // 2 x 1024 (linear, total) slot pairs: normal, then tangential
   double* partial;  // per-block (max speed, blow-up flag) pairs
   double* stat;     // [0..3]: max speed of slot 0 / 1 (+pad); [4]: sticky blow-up flag
 };
@@ -52,6 +55,10 @@ struct LaunchTable {
   hipError_t (*alpha)(const Dev&, hipStream_t);
   hipError_t (*flux_points)(const gksd::Gas&, long long n, const double* L, const double* R, double dt,
                             double* C, double* F, double* D, hipStream_t);
+  // classifier training samples of the cells reconstructed along dir
+  hipError_t (*label)(const Dev&, int dir, int stride, float* X, float* Y, hipStream_t);
+  // logits of n feature rows through the solver's MFMA inference
+  hipError_t (*infer)(const float* P, long long n, const float* X, float* out, hipStream_t);
 };
 
 namespace ieee { extern const LaunchTable table; }

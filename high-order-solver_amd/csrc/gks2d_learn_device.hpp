@@ -1,0 +1,229 @@
+// gks2d_learn_device.hpp — device side of the learned troubled-interface
+// classifier (include/gks_learn.h): stencil features, the training label
+// (WENO5-AO-DF against the linear weights, from the solver's own
+// reconstruction leaves), the linear-weight reconstructions, and the MLP on
+// the matrix cores.
+#pragma once
+#include "gks2d_device.hpp"
+
+namespace gksd {
+inline namespace GKS_VARIANT {
+
+// MLP parameter offsets in the packed float vector (gks_learn.h)
+constexpr int kMlpW1 = 0, kMlpB1 = 512, kMlpW2 = 544, kMlpB2 = 1568, kMlpW3 = 1600, kMlpB3 = 1632;
+
+// 16 features of one stencil from its five (rho, p) pairs, in FP32: for
+// each variable the values centred on their mean and scaled by their spread
+// (in [-1, 1]), then log10 of the spread relative to the mean, / 16; the
+// last four inputs are zero padding
+GKS_DI void stencil_features(float* x, const double (*q)[5]) {
+#pragma unroll
+  for (int v = 0; v < 2; v++) {
+    float f[5];
+#pragma unroll
+    for (int s = 0; s < 5; s++) f[s] = (float)q[v][s];
+    const float m = ((((f[0] + f[1]) + f[2]) + f[3]) + f[4]) * 0.2f;
+    float mx = f[0], mn = f[0];
+#pragma unroll
+    for (int s = 1; s < 5; s++) {
+      mx = fmaxf(mx, f[s]);
+      mn = fminf(mn, f[s]);
+    }
+    const float spread = mx - mn;
+    const float inv = spread > 0.0f ? 1.0f / spread : 0.0f;
+#pragma unroll
+    for (int s = 0; s < 5; s++) x[5 * v + s] = (f[s] - m) * inv;
+    x[10 + v] = log10f(spread / (fabsf(m) + 1e-30f) + 1e-16f) * 0.0625f;
+  }
+#pragma unroll
+  for (int k = 12; k < 16; k++) x[k] = 0.0f;
+}
+
+// (rho, p) of five conservative states (x-like frame)
+GKS_DI void stencil_rho_p(double (*q)[5], const double (*w)[4], const Gas& g) {
+#pragma unroll
+  for (int s = 0; s < 5; s++) {
+    double p[4];
+    convar_to_primvar(p, w[s], g);
+    q[0][s] = p[0];
+    q[1][s] = p[3];
+  }
+}
+
+// the fifth-order candidate alone (AO weights at their linear values, DF
+// factor 1): value and derivative at the cell's left (side 0) / right
+// (side 1) face, the p[3] / px[3] of weno_5th_ao_with_df_left/right
+GKS_DI void weno5_linear_face(int side, double& var, double& der1, double wn2, double wn1, double w0, double wp1,
+                              double wp2, double h, double ih) {
+  (void)h;
+  (void)ih;
+  if (side == 0) {
+    var = w0 + ((1.0 / 60.0) * (47.0 * w0 + 27.0 * wn1 - 3.0 * wn2 - 13.0 * wp1 + 2.0 * wp2) - w0);
+#if GKS_FAST
+    der1 = (15.0 * w0 - 15.0 * wn1 + wn2 - wp1) * (ih * (1.0 / 12.0));
+#else
+    der1 = (15.0 * w0 - 15.0 * wn1 + wn2 - wp1) / (12.0 * h);
+#endif
+  } else {
+    var = w0 + ((1.0 / 60.0) * (47.0 * w0 - 13.0 * wn1 + 2.0 * wn2 + 27.0 * wp1 - 3.0 * wp2) - w0);
+#if GKS_FAST
+    der1 = (-15.0 * w0 + wn1 + 15.0 * wp1 - wp2) * (ih * (1.0 / 12.0));
+#else
+    der1 = (-15.0 * w0 + wn1 + 15.0 * wp1 - wp2) / (12.0 * h);
+#endif
+  }
+}
+
+// the fifth-order candidate at the two tangential Gauss points (p[3],
+// px[3] of weno_5th_ao_with_df_2gauss with DF factor 1)
+GKS_DI void weno5_linear_2gauss(double& g1, double& g1x, double& g2, double& g2x, double wn2, double wn1, double w0,
+                                double wp1, double wp2, double h, double ih) {
+  (void)h;
+  (void)ih;
+  const double sqrt3 = 1.732050807568877294;
+  g1 = w0 + (GKS_CDIV(4314 * w0 + (4 + 500 * sqrt3) * wn1 - wn2 - 70 * sqrt3 * wn2 + 4 * wp1 - 500 * sqrt3 * wp1 - wp2 +
+                          70 * sqrt3 * wp2,
+                      4320) -
+             w0);
+  g2 = w0 + (GKS_CDIV(4314 * w0 + (4 - 500 * sqrt3) * wn1 - wn2 + 70 * sqrt3 * wn2 + 4 * wp1 + 500 * sqrt3 * wp1 - wp2 -
+                          70 * sqrt3 * wp2,
+                      4320) -
+             w0);
+  const double s1 = 48 * sqrt3 * w0 - 72 * wn1 - 26 * sqrt3 * wn1 + 9 * wn2 + 2 * sqrt3 * wn2 + 72 * wp1 -
+                    26 * sqrt3 * wp1 - 9 * wp2 + 2 * sqrt3 * wp2;
+  const double s2 = 48 * sqrt3 * w0 + 72 * wn1 - 26 * sqrt3 * wn1 - 9 * wn2 + 2 * sqrt3 * wn2 - 72 * wp1 -
+                    26 * sqrt3 * wp1 + 9 * wp2 + 2 * sqrt3 * wp2;
+#if GKS_FAST
+  g1x = s1 * (ih * (1.0 / 108.0));
+  g2x = -(s2 * (ih * (1.0 / 108.0)));
+#else
+  g1x = s1 / (108 * h);
+  g2x = -(s2 / (108 * h));
+#endif
+}
+
+// training label of one cell's normal reconstruction: 1 when the DF sensor
+// is nonzero on the stencil, or when for some characteristic (or
+// conservative) component and face the WENO5-AO-DF value or derivative
+// differs from the linear one by more than 1e-3 of that component's range
+// over the stencil
+GKS_DI float normal_label(const double (*w)[4], const double* alpha, double h, double ih, int recon_var,
+                          const Gas& g) {
+  double amax = 0.0;
+#pragma unroll
+  for (int s = 0; s < 5; s++) amax = fmax(amax, alpha[s]);
+  if (amax > 0.0) return 1.0f;
+  const double df[4] = {1.0, 1.0, 1.0, 1.0};
+  double pn1[4], p0[4], pp1[4];
+  convar_to_primvar(pn1, w[1], g);
+  convar_to_primvar(p0, w[2], g);
+  convar_to_primvar(pp1, w[3], g);
+  bool bad = false;
+#pragma unroll
+  for (int side = 0; side < 2; side++) {
+    double base[4];
+#pragma unroll
+    for (int i = 0; i < 4; i++) base[i] = side == 0 ? 0.5 * (pn1[i] + p0[i]) : 0.5 * (pp1[i] + p0[i]);
+    double c[5][4];
+    if (recon_var == 0) {
+#pragma unroll
+      for (int s = 0; s < 5; s++)
+#pragma unroll
+        for (int i = 0; i < 4; i++) c[s][i] = w[s][i];
+    } else {
+      CharBase cb;
+      char_base(cb, base, g);
+#pragma unroll
+      for (int s = 0; s < 5; s++) to_char(c[s], cb, w[s]);
+    }
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+      double v, d1, vl, dl;
+      if (side == 0) weno5_left(v, d1, c[0][i], c[1][i], c[2][i], c[3][i], c[4][i], df, h, ih);
+      else weno5_right(v, d1, c[0][i], c[1][i], c[2][i], c[3][i], c[4][i], df, h, ih);
+      weno5_linear_face(side, vl, dl, c[0][i], c[1][i], c[2][i], c[3][i], c[4][i], h, ih);
+      double mx = c[0][i], mn = c[0][i];
+#pragma unroll
+      for (int s = 1; s < 5; s++) {
+        mx = fmax(mx, c[s][i]);
+        mn = fmin(mn, c[s][i]);
+      }
+      const double tol = 1e-3 * (mx - mn) + 1e-13 * fabs(c[2][i]);
+      if (fabs(v - vl) > tol || fabs(d1 - dl) * h > tol) bad = true;
+    }
+  }
+  return bad ? 1.0f : 0.0f;
+}
+
+// ------------------------------------------------------------- inference
+typedef short bf16x4_t __attribute__((ext_vector_type(4)));
+typedef float f32x4_t __attribute__((ext_vector_type(4)));
+
+GKS_DI short to_bf16(float f) {  // round to nearest even
+  unsigned u = __float_as_uint(f);
+  u += 0x7FFFu + ((u >> 16) & 1u);
+  return (short)(u >> 16);
+}
+
+// One wave evaluates the MLP for its 64 lanes' samples (x: this lane's 16
+// features) and returns this lane's logit.  Every lane of the wave must
+// call it (an inactive lane passes zeros).  The layers run transposed,
+// H^T = W^T X^T, on 16x16x16 BF16 MFMAs (FP32 accumulation): the output
+// tile of one layer is then already laid out as the B operand of the next,
+// so only the input tile passes through LDS.  xs: this wave's 64 x 16
+// staging area (BF16 bits), ls: its 64 logits.  P may point to LDS.
+GKS_DI float mlp_wave_logit(const float* x, const float* __restrict__ P, short* xs, float* ls) {
+  const int lane = threadIdx.x & 63, col = lane & 15, kb = lane >> 4;
+#pragma unroll
+  for (int k = 0; k < 16; k++) xs[lane * 16 + k] = to_bf16(x[k]);
+  // A operands (weights, transposed): A[i][k] with i = col, k = 4 kb + r
+  bf16x4_t a1[2], a2[2][2], a3[2];
+  float b1v[2][4], b2v[2][4];
+#pragma unroll
+  for (int m = 0; m < 2; m++) {
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+      a1[m][r] = to_bf16(P[kMlpW1 + (4 * kb + r) * 32 + m * 16 + col]);
+      b1v[m][r] = P[kMlpB1 + m * 16 + 4 * kb + r];
+      b2v[m][r] = P[kMlpB2 + m * 16 + 4 * kb + r];
+#pragma unroll
+      for (int kk = 0; kk < 2; kk++) a2[m][kk][r] = to_bf16(P[kMlpW2 + (kk * 16 + 4 * kb + r) * 32 + m * 16 + col]);
+      a3[m][r] = col == 0 ? to_bf16(P[kMlpW3 + m * 16 + 4 * kb + r]) : (short)0;
+    }
+  }
+  const float b3 = P[kMlpB3];
+  __syncthreads();
+#pragma unroll
+  for (int t = 0; t < 4; t++) {
+    // B operand of layer 1: X^T[k = 4 kb + r][n = col] of sample 16 t + col
+    bf16x4_t bx;
+#pragma unroll
+    for (int r = 0; r < 4; r++) bx[r] = xs[(16 * t + col) * 16 + 4 * kb + r];
+    bf16x4_t h1[2], h2[2];
+#pragma unroll
+    for (int m = 0; m < 2; m++) {
+      f32x4_t acc = {0.f, 0.f, 0.f, 0.f};
+      acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a1[m], bx, acc, 0, 0, 0);
+#pragma unroll
+      for (int r = 0; r < 4; r++) h1[m][r] = to_bf16(fmaxf(acc[r] + b1v[m][r], 0.0f));
+    }
+#pragma unroll
+    for (int m = 0; m < 2; m++) {
+      f32x4_t acc = {0.f, 0.f, 0.f, 0.f};
+      acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a2[m][0], h1[0], acc, 0, 0, 0);
+      acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a2[m][1], h1[1], acc, 0, 0, 0);
+#pragma unroll
+      for (int r = 0; r < 4; r++) h2[m][r] = to_bf16(fmaxf(acc[r] + b2v[m][r], 0.0f));
+    }
+    f32x4_t z = {0.f, 0.f, 0.f, 0.f};
+    z = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a3[0], h2[0], z, 0, 0, 0);
+    z = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a3[1], h2[1], z, 0, 0, 0);
+    // row 0 of z^T: lanes with kb == 0 hold the logits of samples 16 t + col
+    if (kb == 0) ls[16 * t + col] = z[0] + b3;
+  }
+  __syncthreads();
+  return ls[lane];
+}
+
+}  // namespace GKS_VARIANT
+}  // namespace gksd

@@ -1,0 +1,244 @@
+"""The learned troubled-interface classifier (include/gks_learn.h).
+
+Data: the solver generates its own training set.  Seeded four-quadrant
+Riemann problems run on the GPU; at a few times the context labels every
+stride-th WENO5 reconstruction stencil by comparing WENO5-AO-DF with the
+linear weights (gks2d_classifier_samples).  Training: Adam on the GPU by the
+library's own forward/backward kernels (gks_mlp_train).  Use: the trained
+parameters go into a solver context (Solver2D.set_classifier); stencils the
+network calls smooth, with a zero DF sensor, take the linear weights.
+
+    python -m gks_amd.learn train --out gks_amd/data/classifier_riemann.npz
+    python -m gks_amd.learn evaluate --params gks_amd/data/classifier_riemann.npz
+"""
+import argparse
+import ctypes
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+from . import GksError, Solver2D, lib
+from .params import BC_FREE, planar_shock_params
+from .problems import riemann_prim, riemann_states
+
+NPARAM = 16 * 32 + 32 + 32 * 32 + 32 + 32 + 1
+DATA_DIR = os.path.join(os.path.dirname(os.path.abspath(__file__)), "data")
+DEFAULT_PARAMS = os.path.join(DATA_DIR, "classifier_riemann.npz")
+
+_F = ctypes.POINTER(ctypes.c_float)
+_D = ctypes.POINTER(ctypes.c_double)
+_P = ctypes.c_void_p
+_I64 = ctypes.c_int64
+_SIGS = {
+    "gks2d_classifier_samples": (ctypes.c_int, [_P, ctypes.c_int32, ctypes.c_int32, _F, _F, _I64,
+                                                ctypes.POINTER(_I64)]),
+    "gks_mlp_loss_grad": (ctypes.c_int, [_F, _I64, _F, _F, _F, _F, _D]),
+    "gks_mlp_train": (ctypes.c_int, [_F, _I64, _F, _F, _F, ctypes.c_int32, ctypes.c_int32, ctypes.c_float,
+                                     ctypes.c_uint64, _F]),
+    "gks_mlp_infer_bf16": (ctypes.c_int, [_F, _I64, _F, _F]),
+    "gks2d_set_classifier": (ctypes.c_int, [_P, _F, ctypes.c_float]),
+    "gks2d_classifier_stats": (ctypes.c_int, [_P, ctypes.POINTER(_I64), ctypes.POINTER(_I64)]),
+    "gks_mlp_last_error": (ctypes.c_char_p, []),
+}
+_bound = False
+
+
+def L():
+    global _bound
+    lb = lib()
+    if not _bound:
+        for name, (res, args) in _SIGS.items():
+            f = getattr(lb, name)
+            f.restype = res
+            f.argtypes = args
+        _bound = True
+    return lb
+
+
+def _fp(a):
+    return None if a is None else a.ctypes.data_as(_F)
+
+
+def _check_mlp(rc):
+    if rc != 0:
+        raise RuntimeError(f"gks_mlp error {rc}: {L().gks_mlp_last_error().decode()}")
+
+
+def _check_ctx(rc):
+    if rc != 0:
+        raise RuntimeError(f"gks2d error {rc}: {L().gks2d_last_error().decode()}")
+
+
+def init_params(seed=0):
+    """He-initialised weights, zero biases (float32, packed)."""
+    rng = np.random.default_rng(seed)
+    P = np.zeros(NPARAM, dtype=np.float32)
+    P[0:512] = rng.normal(0, np.sqrt(2.0 / 12), 512)  # 12 live inputs
+    P[544:1568] = rng.normal(0, np.sqrt(2.0 / 32), 1024)
+    P[1600:1632] = rng.normal(0, np.sqrt(1.0 / 32), 32)
+    return P
+
+
+def samples(solver, direction, stride):
+    """(features [n, 16], labels [n]) of the context's current state."""
+    d = solver.p
+    total = (d.nodex + 2) * d.ny if direction == 0 else d.nx * (d.nodey + 2)
+    cap = (total + stride - 1) // stride
+    X = np.zeros((cap, 16), dtype=np.float32)
+    y = np.zeros(cap, dtype=np.float32)
+    n = _I64()
+    _check_ctx(L().gks2d_classifier_samples(solver.h, direction, stride, _fp(X), _fp(y), cap, ctypes.byref(n)))
+    return X[:n.value], y[:n.value]
+
+
+def collect(n=512, seeds=range(32), times=(0.06, 0.14, 0.25), stride=13, fast_math=1, log=None):
+    """Training set from seeded four-quadrant Riemann runs on the GPU."""
+    Xs, ys = [], []
+    for seed in seeds:
+        p = planar_shock_params(n, fast_math=fast_math)
+        s = Solver2D(p)
+        s.init_prim(riemann_prim(p, riemann_states(seed)))
+        for t in times:
+            try:
+                s.run([BC_FREE] * 4, t)
+            except GksError as e:  # the scheme blows up on some seeds, as the reference does
+                if log:
+                    log(f"seed {seed}: stops at t={s.t:.4f} ({e})")
+                break
+            for side in range(4):
+                s.boundary(side, BC_FREE)
+            for direction in (0, 1):
+                X, y = samples(s, direction, stride)
+                Xs.append(X)
+                ys.append(y)
+        s.close()
+        if log:
+            log(f"seed {seed}: {sum(len(v) for v in ys)} samples, troubled {np.mean(np.concatenate(ys)):.3f}")
+    return np.concatenate(Xs), np.concatenate(ys)
+
+
+def balance(X, y, ratio=3.0, seed=0):
+    """All troubled samples and at most ratio x as many smooth ones."""
+    rng = np.random.default_rng(seed)
+    bad = np.flatnonzero(y > 0.5)
+    good = np.flatnonzero(y <= 0.5)
+    keep = rng.permutation(good)[: int(ratio * max(len(bad), 1))]
+    idx = rng.permutation(np.concatenate([bad, keep]))
+    return np.ascontiguousarray(X[idx]), np.ascontiguousarray(y[idx])
+
+
+def loss_grad(P, X, y, w=None):
+    """Mean weighted BCE and its gradient, on the GPU."""
+    P = np.ascontiguousarray(P, dtype=np.float32)
+    X = np.ascontiguousarray(X, dtype=np.float32)
+    y = np.ascontiguousarray(y, dtype=np.float32)
+    w = None if w is None else np.ascontiguousarray(w, dtype=np.float32)
+    g = np.zeros(NPARAM, dtype=np.float32)
+    loss = ctypes.c_double()
+    _check_mlp(L().gks_mlp_loss_grad(_fp(P), len(y), _fp(X), _fp(y), _fp(w), _fp(g), ctypes.byref(loss)))
+    return loss.value, g
+
+
+def train(X, y, w=None, P=None, epochs=20, batch=4096, lr=2e-3, seed=0):
+    """Adam on the GPU; returns (params, loss per epoch)."""
+    P = init_params(seed) if P is None else np.array(P, dtype=np.float32)
+    X = np.ascontiguousarray(X, dtype=np.float32)
+    y = np.ascontiguousarray(y, dtype=np.float32)
+    w = None if w is None else np.ascontiguousarray(w, dtype=np.float32)
+    hist = np.zeros(max(epochs, 1), dtype=np.float32)
+    _check_mlp(L().gks_mlp_train(_fp(P), len(y), _fp(X), _fp(y), _fp(w), batch, epochs, lr, seed, _fp(hist)))
+    return P, hist[:epochs]
+
+
+def infer(P, X):
+    """Logits through the solver's BF16 MFMA inference."""
+    P = np.ascontiguousarray(P, dtype=np.float32)
+    X = np.ascontiguousarray(X, dtype=np.float32)
+    out = np.zeros(len(X), dtype=np.float32)
+    _check_mlp(L().gks_mlp_infer_bf16(_fp(P), len(X), _fp(X), _fp(out)))
+    return out
+
+
+def pick_threshold(logits, y, max_miss=0.002):
+    """Largest logit threshold at which at most max_miss of the troubled
+    stencils would take the linear weights."""
+    bad = np.sort(logits[y > 0.5])
+    if len(bad) == 0:
+        return 0.0
+    k = int(np.floor(max_miss * len(bad)))
+    return float(bad[k]) if k < len(bad) else float(bad[-1])
+
+
+def save(path, P, threshold, meta):
+    os.makedirs(os.path.dirname(os.path.abspath(path)), exist_ok=True)
+    np.savez(path, params=np.asarray(P, dtype=np.float32), threshold=np.float32(threshold),
+             meta=np.array(json.dumps(meta)))
+
+
+def load(path=DEFAULT_PARAMS):
+    z = np.load(path, allow_pickle=False)
+    return z["params"].astype(np.float32), float(z["threshold"]), json.loads(str(z["meta"]))
+
+
+def set_classifier(solver, P, threshold):
+    P = None if P is None else np.ascontiguousarray(P, dtype=np.float32)
+    _check_ctx(L().gks2d_set_classifier(solver.h, _fp(P), float(threshold)))
+
+
+def classifier_stats(solver):
+    a, b = _I64(), _I64()
+    _check_ctx(L().gks2d_classifier_stats(solver.h, ctypes.byref(a), ctypes.byref(b)))
+    return a.value, b.value
+
+
+def _train_main(args):
+    log = lambda *a: print(*a, flush=True)
+    t0 = time.time()
+    X, y = collect(args.n, range(args.seeds), stride=args.stride, log=log)
+    log(f"collected {len(y)} samples, troubled fraction {y.mean():.4f}, {time.time() - t0:.1f} s")
+    rng = np.random.default_rng(1)
+    perm = rng.permutation(len(y))
+    nval = len(y) // 10
+    Xv, yv = X[perm[:nval]], y[perm[:nval]]
+    Xt, yt = balance(X[perm[nval:]], y[perm[nval:]], ratio=args.ratio)
+    log(f"training on {len(yt)} (troubled {yt.mean():.3f}), validating on {len(yv)}")
+    t1 = time.time()
+    P, hist = train(Xt, yt, epochs=args.epochs, batch=args.batch, lr=args.lr, seed=args.seed)
+    log(f"trained in {time.time() - t1:.1f} s, loss per epoch {np.round(hist, 5).tolist()}")
+    zv = infer(P, Xv)
+    thr = pick_threshold(zv, yv, args.max_miss)
+    lin = zv < thr
+    log(f"threshold {thr:.4f}: validation smooth share taking linear weights "
+        f"{lin[yv < 0.5].mean():.4f}, troubled taking them {lin[yv > 0.5].mean():.5f}")
+    meta = dict(n=args.n, seeds=args.seeds, stride=args.stride, samples=int(len(y)), train=int(len(yt)),
+                epochs=args.epochs, batch=args.batch, lr=args.lr, max_miss=args.max_miss,
+                loss=[float(v) for v in hist], val_linear_smooth=float(lin[yv < 0.5].mean()),
+                val_linear_troubled=float(lin[yv > 0.5].mean()))
+    save(args.out, P, thr, meta)
+    log(f"saved {args.out}")
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser(prog="python -m gks_amd.learn")
+    sub = ap.add_subparsers(dest="cmd", required=True)
+    t = sub.add_parser("train")
+    t.add_argument("--out", default=DEFAULT_PARAMS)
+    t.add_argument("--n", type=int, default=512)
+    t.add_argument("--seeds", type=int, default=32)
+    t.add_argument("--stride", type=int, default=13)
+    t.add_argument("--ratio", type=float, default=3.0)
+    t.add_argument("--epochs", type=int, default=12)
+    t.add_argument("--batch", type=int, default=4096)
+    t.add_argument("--lr", type=float, default=2e-3)
+    t.add_argument("--seed", type=int, default=0)
+    t.add_argument("--max-miss", type=float, default=0.002)
+    args = ap.parse_args(argv)
+    if args.cmd == "train":
+        _train_main(args)
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,75 @@
+/*
+ * gks_learn.h — the learned troubled-interface classifier of the 2D path.
+ *
+ * Not a reference interface: the reference has no learned component.  A
+ * small MLP (16 inputs: the normalised 5-point density and pressure stencil
+ * of a reconstruction and their relative spreads; two hidden layers of 32
+ * ReLU units; one logit) predicts whether WENO5-AO's nonlinear weights would
+ * move the reconstruction away from the linear (fifth-order) one.  Where the
+ * network is confident the stencil is smooth and the DF sensor is zero over
+ * the stencil, the reconstruction takes the linear weights; everywhere else
+ * it runs the full WENO5-AO-DF of the reference.
+ *
+ * Parameter vector (float32, GKS_MLP_NPARAM entries, row-major):
+ *   W1[16][32], b1[32], W2[32][32], b2[32], W3[32], b3
+ * Training runs on the GPU (gks_mlp_train); inference inside the solver runs
+ * on the matrix cores in BF16 with FP32 accumulation.
+ */
+#ifndef GKS_LEARN_H
+#define GKS_LEARN_H
+
+#include <stdint.h>
+
+#include "gks2d.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define GKS_MLP_NIN 16
+#define GKS_MLP_NH 32
+#define GKS_MLP_NPARAM (16 * 32 + 32 + 32 * 32 + 32 + 32 + 1)
+
+/* Training samples from the current device state: for every stride-th
+ * reconstruction cell along `dir` (0: x, 1: y; the cells k_normal
+ * reconstructs), the 16 features and the label (1: the WENO5-AO-DF face
+ * values or derivatives differ from the linear-weight ones by more than
+ * 1e-3 of the component's stencil range, or the DF sensor is nonzero on the
+ * stencil; 0: otherwise).  Writes at most `cap` samples; *n_out = count. */
+int gks2d_classifier_samples(gks2d_ctx* ctx, int32_t dir, int32_t stride, float* features, float* labels,
+                             int64_t cap, int64_t* n_out);
+
+/* Mean weighted binary cross-entropy of the logits and its gradient with
+ * respect to the parameters, on the GPU in FP32 (n samples, host arrays). */
+int gks_mlp_loss_grad(const float* params, int64_t n, const float* X, const float* y, const float* w, float* grad,
+                      double* loss);
+
+/* Adam on the GPU (beta1 0.9, beta2 0.999, eps 1e-8): `epochs` passes over
+ * the n samples in a seeded shuffled order, mini-batches of `batch`.
+ * params is updated in place; loss_per_epoch (optional, `epochs` entries)
+ * receives each epoch's mean loss. */
+int gks_mlp_train(float* params, int64_t n, const float* X, const float* y, const float* w, int32_t batch,
+                  int32_t epochs, float lr, uint64_t seed, float* loss_per_epoch);
+
+/* Logits of n samples through the solver's inference kernel (BF16 weights
+ * and activations on the matrix cores, FP32 accumulation). */
+int gks_mlp_infer_bf16(const float* params, int64_t n, const float* X, float* logits);
+
+/* Enables the learned reconstruction in a 2D context (WENO5 only): a
+ * stencil takes the linear weights where logit < threshold and the DF
+ * sensor is zero on it.  params == NULL disables it. */
+int gks2d_set_classifier(gks2d_ctx* ctx, const float* params, float logit_threshold);
+
+/* Reconstructions that took the linear weights / all reconstructions, over
+ * the calls since the classifier was set (normal and tangential). */
+int gks2d_classifier_stats(gks2d_ctx* ctx, int64_t* linear, int64_t* total);
+
+/* message of the last failed gks_mlp_* call on this thread (the gks2d_*
+ * functions above report through gks2d_last_error) */
+const char* gks_mlp_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* GKS_LEARN_H */
