@@ -12,6 +12,7 @@
 #include "../../include/gks2d.h"
 #include "../../include/gks_learn.h"
 #include "gks2d_launch.hpp"
+#include "mlp_pack.hpp"
 
 namespace {
 
@@ -72,6 +73,8 @@ struct gks2d_ctx {
   double* h_stat = nullptr;  // pinned copy of d.stat
   double* d_prim = nullptr;  // staging for gks2d_set_cfl_primvar
   float* d_mlp = nullptr;    // classifier parameters (gks2d_set_classifier)
+  unsigned short* d_mlpw = nullptr;  // their weights in the MFMA operand layout
+  unsigned char* d_lin = nullptr;    // per-cell linear-weight flags, x then y
   unsigned long long* d_lstat = nullptr;  // its linear / total counters
   std::vector<void*> allocs;
   int last_slot = -1;        // slot of the last primvar / set_cfl_primvar
@@ -155,6 +158,8 @@ int gks2d_destroy(gks2d_ctx* c) {
   for (hipEvent_t e : c->pool) (void)hipEventDestroy(e);
   if (c->h_stat) (void)hipHostFree(c->h_stat);
   if (c->d_mlp) (void)hipFree(c->d_mlp);
+  if (c->d_mlpw) (void)hipFree(c->d_mlpw);
+  if (c->d_lin) (void)hipFree(c->d_lin);
   if (c->d_lstat) (void)hipFree(c->d_lstat);
   if (c->ev) (void)hipEventDestroy(c->ev);
   if (c->s) (void)hipStreamDestroy(c->s);
@@ -569,13 +574,23 @@ int gks2d_set_classifier(gks2d_ctx* c, const float* params, float logit_threshol
     return GKS_OK;
   }
   if (c->p.weno_order != 5) return fail(GKS_ERR_UNSUPPORTED, "the classifier is for WENO5-AO");
+  const size_t nc = (size_t)c->d.nc;
   if (!c->d_mlp) {
     HIPCHK(hipMalloc(&c->d_mlp, GKS_MLP_NPARAM * sizeof(float)));
+    HIPCHK(hipMalloc(&c->d_mlpw, mlp::kPackHalves * sizeof(unsigned short)));
     HIPCHK(hipMalloc(&c->d_lstat, kLstatWords * sizeof(unsigned long long)));
+    HIPCHK(hipMalloc(&c->d_lin, 2 * nc));
   }
+  std::vector<uint16_t> packed(mlp::kPackHalves);
+  mlp::pack_bf16(params, packed.data());
   HIPCHK(hipMemcpy(c->d_mlp, params, GKS_MLP_NPARAM * sizeof(float), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(c->d_mlpw, packed.data(), packed.size() * sizeof(uint16_t), hipMemcpyHostToDevice));
   HIPCHK(hipMemset(c->d_lstat, 0, kLstatWords * sizeof(unsigned long long)));
+  HIPCHK(hipMemset(c->d_lin, 0, 2 * nc));
   c->d.mlp = c->d_mlp;
+  c->d.mlpw = c->d_mlpw;
+  c->d.lin[0] = c->d_lin;
+  c->d.lin[1] = c->d_lin + nc;
   c->d.mlp_thr = logit_threshold;
   c->d.lstat = c->d_lstat;
   return GKS_OK;

@@ -254,6 +254,13 @@ GKS_DI void time_coe(double* t, double tau, double tn, double dt) {
   t[5] = 0.5 * dt * dt - tau * tn * (eta - 1) - tau * dt;
 }
 
+// a strided read-only view (e.g. one face's column of an LDS tile)
+struct Strided {
+  const double* p;
+  int s;
+  GKS_DI double operator[](int k) const { return p[k * s]; }
+};
+
 // One Gauss point: collision centre state (Center_all_collision_2d_multi,
 // reconstruction.cpp:3652-3687) followed by GKS2D with gks2nd_2d
 // (flux_function.cpp:345-698).  L/R: convar[0..3], der1x[4..7], der1y[8..11]
@@ -265,11 +272,19 @@ GKS_DI void time_coe(double* t, double tau, double tn, double dt) {
 // Maxwellian comes after both.  F(dt) and F(dt/2) are summed from the stored
 // 4-vectors, each in the reference's own term order
 // (flux_function.cpp:443-629 and 650-684).
-GKS_DI void gks_point(double* f, double* derf, double* center, const double* L, const double* Rr,
-                      double dt, const Gas& g) {
+//
+// L and Rr are read through operator[] (a local array, or a strided view of
+// LDS): each side's values are read where that side is evaluated, so the
+// right side's twelve values are not held through the left side.
+template <class V>
+GKS_DI void gks_point(double* f, double* derf, double* center, const V& L, const V& Rr, double dt, const Gas& g) {
   double pl[4], pr[4];
-  convar_to_ulambda(pl, L, g);
-  convar_to_ulambda(pr, Rr, g);
+  {
+    double c[4];
+#pragma unroll
+    for (int k = 0; k < 4; k++) c[k] = L[k];
+    convar_to_ulambda(pl, c, g);
+  }
   const double unit[4] = {1.0, 0.0, 0.0, 0.0};
   double c0[4], c0x[4], c0y[4];
   double glu[4], gru[4], s1[4], Alul[4], Arur[4];
@@ -285,8 +300,16 @@ GKS_DI void gks_point(double* f, double* derf, double* center, const double* L, 
     c0[2] = pl[0] * ml.vw[1] * ml.up[0];
     c0[3] = 0.5 * pl[0] * (ml.up[2] + ml.up[0] * ml.vw[2] + ml.up[0] * ml.xi2);
     double alx[4], aly[4], t4[4];
-    micro_slope(alx, L + 4, pl, g);
-    micro_slope(aly, L + 8, pl, g);
+    {
+      double dx[4], dy[4];
+#pragma unroll
+      for (int k = 0; k < 4; k++) {
+        dx[k] = L[4 + k];
+        dy[k] = L[8 + k];
+      }
+      micro_slope(alx, dx, pl, g);
+      micro_slope(aly, dy, pl, g);
+    }
     contract(0, 0, 0, t4, alx, tl);
 #pragma unroll
     for (int k = 0; k < 4; k++) c0x[k] = pl[0] * t4[k];
@@ -316,6 +339,12 @@ GKS_DI void gks_point(double* f, double* derf, double* center, const double* L, 
   __builtin_amdgcn_sched_barrier(0);
   // ---- right state: u < 0 half moments
   {
+    double c[4];
+#pragma unroll
+    for (int k = 0; k < 4; k++) c[k] = Rr[k];
+    convar_to_ulambda(pr, c, g);
+  }
+  {
     MM mr;
     mmdf(mr, pr[1], pr[2], pr[3], g);
     const TabN tr{mr};
@@ -325,8 +354,16 @@ GKS_DI void gks_point(double* f, double* derf, double* center, const double* L, 
     c0[2] = c0[2] + pr[0] * mr.vw[1] * mr.un[0];
     c0[3] = c0[3] + 0.5 * pr[0] * (mr.un[2] + mr.un[0] * mr.vw[2] + mr.un[0] * mr.xi2);
     double arx[4], ary[4], t4[4];
-    micro_slope(arx, Rr + 4, pr, g);
-    micro_slope(ary, Rr + 8, pr, g);
+    {
+      double dx[4], dy[4];
+#pragma unroll
+      for (int k = 0; k < 4; k++) {
+        dx[k] = Rr[4 + k];
+        dy[k] = Rr[8 + k];
+      }
+      micro_slope(arx, dx, pr, g);
+      micro_slope(ary, dy, pr, g);
+    }
     contract(0, 0, 0, t4, arx, tr);
 #pragma unroll
     for (int k = 0; k < 4; k++) c0x[k] = c0x[k] + pr[0] * t4[k];

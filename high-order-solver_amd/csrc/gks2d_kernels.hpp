@@ -39,6 +39,24 @@ constexpr int kPartialBlocks = 1024;
 
 __device__ __forceinline__ long long gtid() { return (long long)blockIdx.x * blockDim.x + threadIdx.x; }
 
+// The tile a workgroup takes: workgroups go to the eight XCDs round robin
+// (workgroup b to XCD b % 8), and each XCD has its own L2.  Giving XCD x the
+// x-th contiguous eighth of the tiles keeps the stencil halos of
+// neighbouring tiles in one L2, and spreads a cost pattern that repeats
+// every 8k tiles (a troubled band across every grid column of 16 tiles, in
+// the learned reconstructions) over all eight XCDs instead of one.
+#ifndef GKS_XCD_REMAP
+#define GKS_XCD_REMAP 1
+#endif
+__device__ __forceinline__ unsigned xcd_block() {
+#if GKS_XCD_REMAP
+  const unsigned nb = gridDim.x, b = blockIdx.x, q = nb / 8, r = nb % 8, x = b % 8;
+  return x * q + (x < r ? x : r) + b / 8;
+#else
+  return blockIdx.x;
+#endif
+}
+
 // ------------------------------------------------------------------ boundary
 // free / periodic / reflective fills of one side, in the reference's
 // semantics (boundary_condition.cpp:66-186, 367-442): left/right cover all
@@ -220,8 +238,12 @@ __device__ __forceinline__ void weno5_normal(double* lo, double* ro, const doubl
 // DIR 0: cells i in [G-1, nx-G], all j; writes x faces i (side 1) and i+1
 // (side 0).  DIR 1: cells j in [G-1, ny-G], all i; y faces j and j+1.
 // (reconstruction.cpp:1415-1425 with the guards of :1703, :1715)
-template <int DIR> __global__ void __launch_bounds__(kBlock) k_normal(Dev d) {
-  long long t = gtid();
+#ifndef GKS_NORMAL_WAVES
+#define GKS_NORMAL_WAVES 1
+#endif
+template <int DIR>
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(GKS_NORMAL_WAVES))) k_normal(Dev d) {
+  long long t = (long long)xcd_block() * blockDim.x + threadIdx.x;
   int i, j;
   if (DIR == 0) {
     if (t >= (long long)(d.NX + 2) * d.ny) return;
@@ -428,17 +450,12 @@ __device__ __forceinline__ void count_linear(unsigned long long* stat, bool lin)
 // the fifth-order candidate, formed directly on the conservative variables
 // (the characteristic projection commutes with a fixed linear combination);
 // the order-reduction test is kept.  Every other stencil runs WENO5-AO-DF.
-// the classifier's parameters copied to LDS by the block (one load each)
-__device__ __forceinline__ void stage_params(float* ps, const float* P) {
-  for (int k = threadIdx.x; k < kMlpB3 + 1; k += blockDim.x) ps[k] = P[k];
-}
-
-template <int DIR> __global__ void __launch_bounds__(kBlock) k_normal_learned(Dev d) {
-  __shared__ short xs[kBlock * 16];
+template <int DIR>
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(GKS_NORMAL_WAVES)))
+k_normal_learned(Dev d) {
+  __shared__ __align__(16) short xs[kBlock * 16];
   __shared__ float ls[kBlock];
-  __shared__ float ps[kMlpB3 + 1];
-  stage_params(ps, d.mlp);
-  const long long t = gtid();
+  const long long t = (long long)xcd_block() * blockDim.x + threadIdx.x;
   const bool valid = t < normal_count<DIR>(d);
   int i, j;
   normal_cell<DIR>(d, valid ? t : 0, i, j);
@@ -454,14 +471,14 @@ template <int DIR> __global__ void __launch_bounds__(kBlock) k_normal_learned(De
     for (int k = 0; k < 16; k++) x[k] = 0.0f;
   }
   const int wv = threadIdx.x >> 6;
-  __syncthreads();
-  const float z = mlp_wave_logit(x, ps, xs + wv * 64 * 16, ls + wv * 64);
+  const float z = mlp_wave_logit(x, d.mlp, d.mlpw, xs + wv * 64 * 16, ls + wv * 64);
   if (!valid) return;
   double amax = 0.0;
 #pragma unroll
   for (int s = 0; s < 5; s++) amax = fmax(amax, a[s]);
   const bool lin = z < d.mlp_thr && amax == 0.0;
   count_linear(d.lstat, lin);
+  d.lin[DIR][(long long)i * d.ny + j] = lin ? 1 : 0;
   const double h = DIR == 0 ? d.dx : d.dy, ih = DIR == 0 ? d.idx : d.idy;
   double lo[8], ro[8];
   if (lin) {
@@ -496,8 +513,9 @@ template <int DIR> __global__ void __launch_bounds__(kBlock) k_normal_learned(De
 }
 
 // logits of n feature rows (parity of the MFMA inference)
-__global__ void __launch_bounds__(kBlock) k_mlp_infer(const float* P, long long n, const float* X, float* out) {
-  __shared__ short xs[kBlock * 16];
+__global__ void __launch_bounds__(kBlock) k_mlp_infer(const float* P, const unsigned short* W, long long n, const float* X,
+                                                   float* out) {
+  __shared__ __align__(16) short xs[kBlock * 16];
   __shared__ float ls[kBlock];
   const long long t = gtid();
   const bool valid = t < n;
@@ -505,7 +523,7 @@ __global__ void __launch_bounds__(kBlock) k_mlp_infer(const float* P, long long 
 #pragma unroll
   for (int k = 0; k < 16; k++) x[k] = valid ? X[t * 16 + k] : 0.0f;
   const int wv = threadIdx.x >> 6;
-  const float z = mlp_wave_logit(x, P, xs + wv * 64 * 16, ls + wv * 64);
+  const float z = mlp_wave_logit(x, P, W, xs + wv * 64 * 16, ls + wv * 64);
   if (valid) out[t] = z;
 }
 
@@ -756,9 +774,30 @@ template <int DIR, int ORDER> __global__ void __launch_bounds__(kBlock) k_tangen
 // the flux of one (face, Gauss point) from its two Gauss-point states, and
 // everything stored per point: the stage-combined rotated flux, the stage-1
 // share, the raw (f, derf) when kept, and Calculate_alpha_k
-template <int DIR>
-__device__ __forceinline__ void flux_point_tail(const Dev& d, int q, long long f, long long nf, const double* L,
-                                                const double* R, int stage, double dt) {
+template <int DIR, class V>
+__device__ __forceinline__ void flux_point_tail(const Dev& d, int q, long long f, long long nf, const V& L,
+                                                const V& R, int stage, double dt) {
+  // alpha_k first: only the four conservative values of each side stay
+  // live into it, and none of them into the flux below
+  {
+    double pl[4], pr[4], cl[4], cr[4];
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+      cl[k] = L[k];
+      cr[k] = R[k];
+    }
+    convar_to_primvar(pl, cl, d.gas);
+    convar_to_primvar(pr, cr, d.gas);
+    double a;
+    if (DIR == 0) {
+      a = alpha_k(pl, pr, d.gas);
+    } else {
+      const double tl[4] = {pl[0], -pl[2], pl[1], pl[3]};
+      const double tr[4] = {pr[0], -pr[2], pr[1], pr[3]};
+      a = alpha_k(tl, tr, d.gas);
+    }
+    (DIR == 0 ? d.akx : d.aky)[q * nf + f] = a;
+  }
   double fo[4], dfo[4];
   gks_point(fo, dfo, nullptr, L, R, dt, d.gas);
   if (d.keep_face_fluxes) {
@@ -788,21 +827,15 @@ __device__ __forceinline__ void flux_point_tail(const Dev& d, int q, long long f
   xc[(q * 4 + 1) * nf + f] = DIR == 0 ? x[1] : -x[2];
   xc[(q * 4 + 2) * nf + f] = DIR == 0 ? x[2] : x[1];
   xc[(q * 4 + 3) * nf + f] = x[3];
-  double pl[4], pr[4];
-  convar_to_primvar(pl, L, d.gas);
-  convar_to_primvar(pr, R, d.gas);
-  double a;
-  if (DIR == 0) {
-    a = alpha_k(pl, pr, d.gas);
-  } else {
-    const double tl[4] = {pl[0], -pl[2], pl[1], pl[3]};
-    const double tr[4] = {pr[0], -pr[2], pr[1], pr[3]};
-    a = alpha_k(tl, tr, d.gas);
-  }
-  (DIR == 0 ? d.akx : d.aky)[q * nf + f] = a;
 }
 
-template <int DIR> __global__ void __launch_bounds__(kBlock) k_flux(Dev d, int stage, double dt) {
+// waves per SIMD asked of the flux-point kernels (k_flux, k_tanflux)
+#ifndef GKS_FLUX_WAVES
+#define GKS_FLUX_WAVES 2
+#endif
+template <int DIR>
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(GKS_FLUX_WAVES)))
+k_flux(Dev d, int stage, double dt) {
   long long t = gtid();
   const int ni = DIR == 0 ? d.NX + 1 : d.NX, nj = DIR == 0 ? d.NY : d.NY + 1;
   if (t >= (long long)d.gp * ni * nj) return;
@@ -827,14 +860,20 @@ template <int DIR> __global__ void __launch_bounds__(kBlock) k_flux(Dev d, int s
 // its Gauss-point states kept in LDS; phase 2, thread (Gauss point, face):
 // the flux of k_flux from those states.  The Gauss-point states never go to
 // HBM (48 doubles per face each way).
+//
+// The flux point needs about 280 registers per lane, which would leave one
+// wave per SIMD; asking for two waves spills about 24 of them to scratch
+// (a few hundred bytes per lane, cache resident) and the second wave hides
+// the latency: 1.54 -> 1.16 ms per launch at 2048^2.
 constexpr int kTanFlux = 128;
 template <int DIR, bool LEARNED>
-__global__ void __launch_bounds__(2 * kTanFlux) k_tanflux(Dev d, int stage, double dt) {
+__global__ void __launch_bounds__(2 * kTanFlux) __attribute__((amdgpu_waves_per_eu(GKS_FLUX_WAVES)))
+k_tanflux(Dev d, int stage, double dt) {
   __shared__ double sg[2][2][12][kTanFlux];  // [side][gauss point][component][face]
   const int ni = DIR == 0 ? d.NX + 1 : d.NX, nj = DIR == 0 ? d.NY : d.NY + 1;
   const long long nfaces = (long long)ni * nj;
   const int tl = threadIdx.x % kTanFlux, hi = threadIdx.x / kTanFlux;
-  const long long r = (long long)blockIdx.x * kTanFlux + tl;
+  const long long r = (long long)xcd_block() * kTanFlux + tl;
   const bool valid = r < nfaces;
   const int i = valid ? (int)(r / nj) + d.G : d.G, j = valid ? (int)(r % nj) + d.G : d.G;
   const long long nf = DIR == 0 ? d.nfx : d.nfy;
@@ -861,32 +900,27 @@ __global__ void __launch_bounds__(2 * kTanFlux) k_tanflux(Dev d, int stage, doub
   }
   bool lin = false;
   if constexpr (LEARNED) {
-    // the classifier on the line values along the face (all lanes take part)
-    __shared__ short xs[2 * kTanFlux * 16];
-    __shared__ float ls[2 * kTanFlux];
-    __shared__ float ps[kMlpB3 + 1];
-    stage_params(ps, d.mlp);
-    float x[16];
+    // the tangential stencil inherits the classification of the normal
+    // stencils its line values come from: linear when the five cells' normal
+    // stencils and the centre cell's stencil across them all took the linear
+    // weights, and the DF sensor is zero on the tangential stencil
     if (valid) {
-      double w4[5][4], q[2][5];
-#pragma unroll
-      for (int s = 0; s < 5; s++)
-#pragma unroll
-        for (int k = 0; k < 4; k++) w4[s][k] = L[s][k];
-      stencil_rho_p(q, w4, d.gas);
-      stencil_features(x, q);
-    } else {
-#pragma unroll
-      for (int k = 0; k < 16; k++) x[k] = 0.0f;
-    }
-    const int wv = threadIdx.x >> 6;
-    __syncthreads();
-    const float z = mlp_wave_logit(x, ps, xs + wv * 64 * 16, ls + wv * 64);
-    if (valid) {
+      unsigned smooth;
       double amax = 0.0;
 #pragma unroll
       for (int s = 0; s < 5; s++) amax = fmax(amax, a[s]);
-      lin = z < d.mlp_thr && amax == 0.0;
+      if (DIR == 0) {
+        const long long c0 = (long long)(side == 0 ? i - 1 : i) * d.ny + j;
+        smooth = d.lin[1][c0];
+#pragma unroll
+        for (int s = 0; s < 5; s++) smooth &= d.lin[0][c0 - 2 + s];
+      } else {
+        const int cj = side == 0 ? j - 1 : j;
+        smooth = d.lin[0][(long long)i * d.ny + cj];
+#pragma unroll
+        for (int s = 0; s < 5; s++) smooth &= d.lin[1][(long long)(i + 2 - s) * d.ny + cj];
+      }
+      lin = smooth != 0 && amax == 0.0;
       count_linear(d.lstat + 2 * kStatSlots, lin);
     }
   }
@@ -924,12 +958,7 @@ __global__ void __launch_bounds__(2 * kTanFlux) k_tanflux(Dev d, int stage, doub
   __syncthreads();
   if (valid) {
     const int q = hi;
-    double L[12], R[12];
-#pragma unroll
-    for (int k = 0; k < 12; k++) {
-      L[k] = sg[0][q][k][tl];
-      R[k] = sg[1][q][k][tl];
-    }
+    const Strided L{&sg[0][q][0][tl], kTanFlux}, R{&sg[1][q][0][tl], kTanFlux};
     flux_point_tail<DIR>(d, q, f0, nf, L, R, stage, dt);
   }
 }
@@ -1128,8 +1157,9 @@ static hipError_t label(const Dev& d, int dir, int stride, float* X, float* Y, h
   else hipLaunchKernelGGL(k_label<1>, dim3(blocks_for(ns)), dim3(kBlock), 0, s, d, stride, X, Y);
   return hipGetLastError();
 }
-static hipError_t infer(const float* P, long long n, const float* X, float* out, hipStream_t s) {
-  hipLaunchKernelGGL(k_mlp_infer, dim3(blocks_for(n)), dim3(kBlock), 0, s, P, n, X, out);
+static hipError_t infer(const float* P, const unsigned short* W, long long n, const float* X, float* out,
+                        hipStream_t s) {
+  hipLaunchKernelGGL(k_mlp_infer, dim3(blocks_for(n)), dim3(kBlock), 0, s, P, W, n, X, out);
   return hipGetLastError();
 }
 

@@ -32,8 +32,10 @@ struct Dev {
   double* akx;
   double* aky;
   const float* mlp;             // classifier parameters (gks_learn.h), null: plain WENO5-AO
+  const unsigned short* mlpw;   // its weights in the MFMA operand layout (mlp_pack.hpp)
   float mlp_thr;                // logit below which a smooth stencil takes the linear weights
   unsigned long long* lstat;    // 2 x 1024 (linear, total) slot pairs: normal, then tangential
+  unsigned char* lin[2];        // per cell: its x / y normal stencil took the linear weights
   double* partial;  // per-block (max speed, blow-up flag) pairs
   double* stat;     // [0..3]: max speed of slot 0 / 1 (+pad); [4]: sticky blow-up flag
 };
@@ -58,7 +60,8 @@ struct LaunchTable {
   // classifier training samples of the cells reconstructed along dir
   hipError_t (*label)(const Dev&, int dir, int stride, float* X, float* Y, hipStream_t);
   // logits of n feature rows through the solver's MFMA inference
-  hipError_t (*infer)(const float* P, long long n, const float* X, float* out, hipStream_t);
+  hipError_t (*infer)(const float* P, const unsigned short* W, long long n, const float* X, float* out,
+                        hipStream_t);
 };
 
 namespace ieee { extern const LaunchTable table; }

@@ -33,7 +33,7 @@ GKS_DI void stencil_features(float* x, const double (*q)[5]) {
     const float inv = spread > 0.0f ? 1.0f / spread : 0.0f;
 #pragma unroll
     for (int s = 0; s < 5; s++) x[5 * v + s] = (f[s] - m) * inv;
-    x[10 + v] = log10f(spread / (fabsf(m) + 1e-30f) + 1e-16f) * 0.0625f;
+    x[10 + v] = __log2f(spread / (fabsf(m) + 1e-30f) + 1e-16f) * (0.30102999566398120f * 0.0625f);
   }
 #pragma unroll
   for (int k = 12; k < 16; k++) x[k] = 0.0f;
@@ -158,70 +158,89 @@ GKS_DI float normal_label(const double (*w)[4], const double* alpha, double h, d
 // ------------------------------------------------------------- inference
 typedef short bf16x4_t __attribute__((ext_vector_type(4)));
 typedef float f32x4_t __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x2_hw __attribute__((ext_vector_type(2)));
+typedef float f32x2_t __attribute__((ext_vector_type(2)));
 
-GKS_DI short to_bf16(float f) {  // round to nearest even
-  unsigned u = __float_as_uint(f);
-  u += 0x7FFFu + ((u >> 16) & 1u);
-  return (short)(u >> 16);
+// two floats to BF16 (round to nearest even, one v_cvt_pk_bf16_f32), low
+// half first
+GKS_DI unsigned pack_bf16x2(float a, float b) {
+  const f32x2_t v = {a, b};
+  return __builtin_bit_cast(unsigned, __builtin_convertvector(v, bf16x2_hw));
+}
+GKS_DI bf16x4_t relu_bf16x4(const f32x4_t& acc, const float* bias) {
+  const uint2 u = make_uint2(pack_bf16x2(fmaxf(acc[0] + bias[0], 0.0f), fmaxf(acc[1] + bias[1], 0.0f)),
+                             pack_bf16x2(fmaxf(acc[2] + bias[2], 0.0f), fmaxf(acc[3] + bias[3], 0.0f)));
+  return __builtin_bit_cast(bf16x4_t, u);
+}
+
+// orders this wave's LDS writes before its later LDS reads of other lanes'
+// data (the LDS processes one wave's operations in order, so no workgroup
+// barrier is needed for data that stays within the wave)
+GKS_DI void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
 // One wave evaluates the MLP for its 64 lanes' samples (x: this lane's 16
 // features) and returns this lane's logit.  Every lane of the wave must
-// call it (an inactive lane passes zeros).  The layers run transposed,
-// H^T = W^T X^T, on 16x16x16 BF16 MFMAs (FP32 accumulation): the output
-// tile of one layer is then already laid out as the B operand of the next,
-// so only the input tile passes through LDS.  xs: this wave's 64 x 16
-// staging area (BF16 bits), ls: its 64 logits.  P may point to LDS.
-GKS_DI float mlp_wave_logit(const float* x, const float* __restrict__ P, short* xs, float* ls) {
+// call it (an inactive lane passes zeros); other waves of the workgroup are
+// not involved.  The layers run transposed, H^T = W^T X^T, on 16x16x16 BF16
+// MFMAs (FP32 accumulation): the output tile of one layer is then already
+// laid out as the B operand of the next, so only the input tile passes
+// through LDS.  P: the float parameters (biases are read from it), W: the
+// weights packed in the operand layout of mlp_pack.hpp, xs: this wave's
+// 64 x 16 staging area (BF16, 16-byte aligned), ls: its 64 logits.
+GKS_DI float mlp_wave_logit(const float* x, const float* __restrict__ P, const unsigned short* __restrict__ W,
+                            short* xs, float* ls) {
   const int lane = threadIdx.x & 63, col = lane & 15, kb = lane >> 4;
+  {
+    uint4* dst = reinterpret_cast<uint4*>(xs + lane * 16);
+    dst[0] = make_uint4(pack_bf16x2(x[0], x[1]), pack_bf16x2(x[2], x[3]), pack_bf16x2(x[4], x[5]),
+                        pack_bf16x2(x[6], x[7]));
+    dst[1] = make_uint4(pack_bf16x2(x[8], x[9]), pack_bf16x2(x[10], x[11]), pack_bf16x2(x[12], x[13]),
+                        pack_bf16x2(x[14], x[15]));
+  }
+  // A operands (weights, transposed), one 8-byte load each
+  const uint2* Wv = reinterpret_cast<const uint2*>(W);
+  bf16x4_t a[8];
 #pragma unroll
-  for (int k = 0; k < 16; k++) xs[lane * 16 + k] = to_bf16(x[k]);
-  // A operands (weights, transposed): A[i][k] with i = col, k = 4 kb + r
-  bf16x4_t a1[2], a2[2][2], a3[2];
+  for (int v = 0; v < 8; v++) a[v] = __builtin_bit_cast(bf16x4_t, Wv[v * 64 + lane]);
   float b1v[2][4], b2v[2][4];
 #pragma unroll
-  for (int m = 0; m < 2; m++) {
+  for (int m = 0; m < 2; m++)
 #pragma unroll
     for (int r = 0; r < 4; r++) {
-      a1[m][r] = to_bf16(P[kMlpW1 + (4 * kb + r) * 32 + m * 16 + col]);
       b1v[m][r] = P[kMlpB1 + m * 16 + 4 * kb + r];
       b2v[m][r] = P[kMlpB2 + m * 16 + 4 * kb + r];
-#pragma unroll
-      for (int kk = 0; kk < 2; kk++) a2[m][kk][r] = to_bf16(P[kMlpW2 + (kk * 16 + 4 * kb + r) * 32 + m * 16 + col]);
-      a3[m][r] = col == 0 ? to_bf16(P[kMlpW3 + m * 16 + 4 * kb + r]) : (short)0;
     }
-  }
   const float b3 = P[kMlpB3];
-  __syncthreads();
+  wave_lds_sync();
 #pragma unroll
   for (int t = 0; t < 4; t++) {
     // B operand of layer 1: X^T[k = 4 kb + r][n = col] of sample 16 t + col
-    bf16x4_t bx;
-#pragma unroll
-    for (int r = 0; r < 4; r++) bx[r] = xs[(16 * t + col) * 16 + 4 * kb + r];
+    const bf16x4_t bx = *reinterpret_cast<const bf16x4_t*>(xs + (16 * t + col) * 16 + 4 * kb);
     bf16x4_t h1[2], h2[2];
 #pragma unroll
     for (int m = 0; m < 2; m++) {
       f32x4_t acc = {0.f, 0.f, 0.f, 0.f};
-      acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a1[m], bx, acc, 0, 0, 0);
-#pragma unroll
-      for (int r = 0; r < 4; r++) h1[m][r] = to_bf16(fmaxf(acc[r] + b1v[m][r], 0.0f));
+      acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[m], bx, acc, 0, 0, 0);
+      h1[m] = relu_bf16x4(acc, b1v[m]);
     }
 #pragma unroll
     for (int m = 0; m < 2; m++) {
       f32x4_t acc = {0.f, 0.f, 0.f, 0.f};
-      acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a2[m][0], h1[0], acc, 0, 0, 0);
-      acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a2[m][1], h1[1], acc, 0, 0, 0);
-#pragma unroll
-      for (int r = 0; r < 4; r++) h2[m][r] = to_bf16(fmaxf(acc[r] + b2v[m][r], 0.0f));
+      acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[2 + 2 * m], h1[0], acc, 0, 0, 0);
+      acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[3 + 2 * m], h1[1], acc, 0, 0, 0);
+      h2[m] = relu_bf16x4(acc, b2v[m]);
     }
     f32x4_t z = {0.f, 0.f, 0.f, 0.f};
-    z = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a3[0], h2[0], z, 0, 0, 0);
-    z = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a3[1], h2[1], z, 0, 0, 0);
+    z = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[6], h2[0], z, 0, 0, 0);
+    z = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[7], h2[1], z, 0, 0, 0);
     // row 0 of z^T: lanes with kb == 0 hold the logits of samples 16 t + col
     if (kb == 0) ls[16 * t + col] = z[0] + b3;
   }
-  __syncthreads();
+  wave_lds_sync();
   return ls[lane];
 }
 

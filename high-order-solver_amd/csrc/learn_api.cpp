@@ -12,6 +12,7 @@
 
 #include "../../include/gks_learn.h"
 #include "gks2d_launch.hpp"
+#include "mlp_pack.hpp"
 #include "mlp_train.hpp"
 
 namespace {
@@ -150,13 +151,17 @@ int gks_mlp_infer_bf16(const float* params, int64_t n, const float* X, float* lo
   if (!params || !X || !logits) return fail(GKS_ERR_ARG, "null");
   if (n <= 0 || n > (1 << 26)) return fail(GKS_ERR_ARG, "n must be in [1, 2^26]");
   DevBufs b;
-  float *P, *dX, *dO;
+  float *P, *dX, *dO, *dW;
+  std::vector<uint16_t> packed(mlp::kPackHalves);
+  mlp::pack_bf16(params, packed.data());
   HIPCHK(b.alloc(&P, GKS_MLP_NPARAM));
+  HIPCHK(b.alloc(&dW, mlp::kPackHalves / 2));
   HIPCHK(b.alloc(&dX, (size_t)n * 16));
   HIPCHK(b.alloc(&dO, (size_t)n));
   HIPCHK(hipMemcpy(P, params, GKS_MLP_NPARAM * 4, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(dW, packed.data(), packed.size() * sizeof(uint16_t), hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy(dX, X, (size_t)n * 64, hipMemcpyHostToDevice));
-  HIPCHK(fast::table.infer(P, n, dX, dO, nullptr));
+  HIPCHK(fast::table.infer(P, reinterpret_cast<const unsigned short*>(dW), n, dX, dO, nullptr));
   HIPCHK(hipDeviceSynchronize());
   HIPCHK(hipMemcpy(logits, dO, (size_t)n * 4, hipMemcpyDeviceToHost));
   return GKS_OK;
