@@ -3,6 +3,7 @@
 // re-entrant (the reference is single-threaded at this level too).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -609,6 +610,355 @@ int gks2d_classifier_stats(gks2d_ctx* c, int64_t* linear, int64_t* total) {
   }
   *linear = (int64_t)lin;
   *total = (int64_t)tot;
+  return GKS_OK;
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------------ batches
+#include "../../include/gks2d_batch.h"
+
+struct gks2d_batch {
+  std::vector<gks2d_ctx*> ctx;
+  int np = 0;
+  hipStream_t s = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  Dev* d_dev = nullptr;
+  BState* d_st = nullptr;
+  unsigned* d_start = nullptr;
+  int* d_pid = nullptr;
+  BState* h_st = nullptr;  // pinned
+  BatchShape shape{128, 0};
+  int largest_first = 1;
+  std::vector<unsigned> tiles = std::vector<unsigned>(KB_COUNT, 0);
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  int graph_key = -1;  // shape the graph was captured for
+};
+
+namespace {
+
+unsigned ceil_div(long long n, long long k) { return (unsigned)((n + k - 1) / k); }
+
+void drop_graph(gks2d_batch* b) {
+  if (b->exec) (void)hipGraphExecDestroy(b->exec);
+  if (b->graph) (void)hipGraphDestroy(b->graph);
+  b->exec = nullptr;
+  b->graph = nullptr;
+  b->graph_key = -1;
+}
+
+// tile prefix tables of every work class, in launch order
+int batch_tables(gks2d_batch* b) {
+  const int np = b->np;
+  std::vector<int> pid(np);
+  for (int p = 0; p < np; p++) pid[p] = p;
+  if (b->largest_first)
+    std::stable_sort(pid.begin(), pid.end(), [&](int a, int c) { return b->ctx[a]->d.nc > b->ctx[c]->d.nc; });
+  std::vector<unsigned> start((size_t)KB_COUNT * (np + 1));
+  const int tf = b->shape.tanflux_faces;
+  for (int k = 0; k < KB_COUNT; k++) {
+    unsigned acc = 0;
+    for (int q = 0; q < np; q++) {
+      start[(size_t)k * (np + 1) + q] = acc;
+      const Dev& d = b->ctx[pid[q]]->d;
+      long long n = 0;
+      switch (k) {
+        case KB_BC_LR: n = ceil_div((long long)d.G * d.ny, kSweepBlock); break;
+        case KB_BC_DU: n = ceil_div((long long)d.G * d.nx, kSweepBlock); break;
+        case KB_PRIMVAR: n = std::min<long long>(ceil_div(d.nc, kSweepBlock), kSweepPartials); break;
+        case KB_COPY:
+        case KB_UPDATE: n = ceil_div((long long)d.NX * d.NY, kSweepBlock); break;
+        case KB_NORMAL_X: n = ceil_div((long long)(d.NX + 2) * d.ny, kSweepBlock); break;
+        case KB_NORMAL_Y: n = ceil_div((long long)d.nx * (d.NY + 2), kSweepBlock); break;
+        case KB_FLUX_X: n = ceil_div((long long)(d.NX + 1) * d.NY, tf); break;
+        case KB_FLUX_Y: n = ceil_div((long long)d.NX * (d.NY + 1), tf); break;
+        case KB_ALPHA: n = ceil_div((long long)d.NX * std::min(d.NX, d.NY), kSweepBlock); break;
+      }
+      if (acc + (unsigned long long)n > 0x7fffffffULL) return fail(GKS_ERR_ARG, "batch too large for one launch");
+      acc += (unsigned)n;
+    }
+    start[(size_t)k * (np + 1) + np] = acc;
+    b->tiles[k] = acc;
+  }
+  HIPCHK(hipMemcpy(b->d_start, start.data(), start.size() * sizeof(unsigned), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(b->d_pid, pid.data(), pid.size() * sizeof(int), hipMemcpyHostToDevice));
+  return GKS_OK;
+}
+
+// device copies of the problems' descriptors (they change when a
+// classifier is set) and a consistency check of the batch
+int batch_upload_devs(gks2d_batch* b, int* learned) {
+  std::vector<Dev> h(b->np);
+  for (int p = 0; p < b->np; p++) {
+    gks2d_ctx* c = b->ctx[p];
+    HIPCHK(hipStreamSynchronize(c->s));
+    h[p] = c->d;
+    if (c->last_slot < 0) return fail(GKS_ERR_STATE, "batch problem without a primitive state: set its initial condition first");
+  }
+  const int l0 = h[0].mlp != nullptr;
+  for (int p = 1; p < b->np; p++)
+    if ((h[p].mlp != nullptr) != l0) return fail(GKS_ERR_STATE, "the classifier must be set on all problems or on none");
+  *learned = l0;
+  HIPCHK(hipMemcpy(b->d_dev, h.data(), h.size() * sizeof(Dev), hipMemcpyHostToDevice));
+  return GKS_OK;
+}
+
+int batch_init_state(gks2d_batch* b, const int32_t* bc, const double* tstop, int32_t max_steps, const double* t,
+                     const int32_t* steps) {
+  for (int p = 0; p < b->np; p++) {
+    BState& st = b->h_st[p];
+    std::memset(&st, 0, sizeof(st));
+    st.t = t[p];
+    st.tstop = tstop[p];
+    st.cfl = b->ctx[p]->p.cfl;
+    st.slot = b->ctx[p]->last_slot;
+    st.steps = steps[p];
+    st.max_steps = (int)std::min<long long>((long long)steps[p] + max_steps, 0x7fffffff);
+    for (int k = 0; k < 4; k++) {
+      if (bc[4 * p + k] < 0 || bc[4 * p + k] > 2) return fail(GKS_ERR_ARG, "bc kind");
+      st.bc[k] = bc[4 * p + k];
+    }
+  }
+  HIPCHK(hipMemcpyAsync(b->d_st, b->h_st, b->np * sizeof(BState), hipMemcpyHostToDevice, b->s));
+  return GKS_OK;
+}
+
+// the captured step for the current shape
+int batch_graph(gks2d_batch* b, int learned) {
+  const int key = b->shape.tanflux_faces * 4 + b->largest_first * 2 + learned;
+  if (b->exec && b->graph_key == key) return GKS_OK;
+  drop_graph(b);
+  RC(batch_tables(b));
+  b->shape.learned = learned;
+  const BatchArgs a{b->d_dev, b->d_st, b->d_start, b->d_pid, b->np};
+  const LaunchTable* lt = b->ctx[0]->lt;
+  HIPCHK(hipStreamBeginCapture(b->s, hipStreamCaptureModeThreadLocal));
+  hipError_t e = lt->batch_step(a, b->shape, b->tiles.data(), b->s);
+  hipGraph_t g = nullptr;
+  hipError_t e2 = hipStreamEndCapture(b->s, &g);
+  if (e != hipSuccess) {
+    if (g) (void)hipGraphDestroy(g);
+    return hip_fail(e, "batch_step capture");
+  }
+  if (e2 != hipSuccess) return hip_fail(e2, "hipStreamEndCapture");
+  b->graph = g;
+  HIPCHK(hipGraphInstantiate(&b->exec, g, nullptr, nullptr, 0));
+  b->graph_key = key;
+  return GKS_OK;
+}
+
+bool batch_all_done(const gks2d_batch* b) {
+  for (int p = 0; p < b->np; p++)
+    if (!b->h_st[p].done) return false;
+  return true;
+}
+
+}  // namespace
+
+extern "C" {
+
+int gks2d_batch_destroy(gks2d_batch* b) {
+  if (!b) return GKS_OK;
+  if (b->s) (void)hipStreamSynchronize(b->s);
+  drop_graph(b);
+  for (gks2d_ctx* c : b->ctx) gks2d_destroy(c);
+  if (b->d_dev) (void)hipFree(b->d_dev);
+  if (b->d_st) (void)hipFree(b->d_st);
+  if (b->d_start) (void)hipFree(b->d_start);
+  if (b->d_pid) (void)hipFree(b->d_pid);
+  if (b->h_st) (void)hipHostFree(b->h_st);
+  if (b->e0) (void)hipEventDestroy(b->e0);
+  if (b->e1) (void)hipEventDestroy(b->e1);
+  if (b->s) (void)hipStreamDestroy(b->s);
+  delete b;
+  return GKS_OK;
+}
+
+int gks2d_batch_create(const gks2d_params* params, int32_t n, gks2d_batch** out) {
+  if (!params || !out) return fail(GKS_ERR_ARG, "null");
+  *out = nullptr;
+  if (n < 1 || n > 1024) return fail(GKS_ERR_ARG, "a batch holds 1 to 1024 problems");
+  for (int p = 0; p < n; p++) {
+    if (params[p].weno_order != 5) return fail(GKS_ERR_UNSUPPORTED, "batches run WENO5-AO (2 Gauss points)");
+    if (params[p].fast_math != params[0].fast_math) return fail(GKS_ERR_ARG, "fast_math must agree in a batch");
+  }
+  gks2d_batch* b = new gks2d_batch();
+  b->np = n;
+  for (int p = 0; p < n; p++) {
+    gks2d_ctx* c = nullptr;
+    int rc = gks2d_create(&params[p], &c);
+    if (rc) {
+      gks2d_batch_destroy(b);
+      return rc;
+    }
+    if (!c->d.fuse_tanflux) {
+      gks2d_destroy(c);
+      gks2d_batch_destroy(b);
+      return fail(GKS_ERR_UNSUPPORTED, "batches need the fused tangential + flux launch (GKS_TANFLUX=0 is set)");
+    }
+    b->ctx.push_back(c);
+  }
+  hipError_t e = hipStreamCreateWithFlags(&b->s, hipStreamNonBlocking);
+  if (e == hipSuccess) e = hipEventCreate(&b->e0);
+  if (e == hipSuccess) e = hipEventCreate(&b->e1);
+  if (e == hipSuccess) e = hipMalloc(&b->d_dev, n * sizeof(Dev));
+  if (e == hipSuccess) e = hipMalloc(&b->d_st, n * sizeof(BState));
+  if (e == hipSuccess) e = hipMalloc(&b->d_start, (size_t)KB_COUNT * (n + 1) * sizeof(unsigned));
+  if (e == hipSuccess) e = hipMalloc(&b->d_pid, n * sizeof(int));
+  if (e == hipSuccess) e = hipHostMalloc((void**)&b->h_st, n * sizeof(BState));
+  if (e != hipSuccess) {
+    gks2d_batch_destroy(b);
+    return hip_fail(e, "gks2d_batch_create");
+  }
+  *out = b;
+  return GKS_OK;
+}
+
+int gks2d_batch_problem(gks2d_batch* b, int32_t p, gks2d_ctx** out) {
+  if (!b || !out) return fail(GKS_ERR_ARG, "null");
+  if (p < 0 || p >= b->np) return fail(GKS_ERR_ARG, "problem index");
+  *out = b->ctx[p];
+  return GKS_OK;
+}
+
+int gks2d_batch_set_shape(gks2d_batch* b, int32_t tanflux_faces, int32_t largest_first) {
+  if (!b) return fail(GKS_ERR_ARG, "null");
+  if (tanflux_faces != 64 && tanflux_faces != 128) return fail(GKS_ERR_ARG, "tanflux_faces: 64 or 128");
+  b->shape.tanflux_faces = tanflux_faces;
+  b->largest_first = largest_first ? 1 : 0;
+  return GKS_OK;
+}
+
+int gks2d_batch_run(gks2d_batch* b, const int32_t* bc, const double* tstop, int32_t max_steps,
+                    int32_t check_every, double* t, int32_t* steps) {
+  if (!b || !bc || !tstop || !t || !steps) return fail(GKS_ERR_ARG, "null");
+  if (max_steps < 0) return fail(GKS_ERR_ARG, "max_steps");
+  if (check_every < 1) check_every = 1;
+  int learned = 0;
+  RC(batch_upload_devs(b, &learned));
+  RC(batch_init_state(b, bc, tstop, max_steps, t, steps));
+  RC(batch_graph(b, learned));
+  // every replay advances each unfinished problem by one step (or ends it),
+  // so max_steps + 1 replays end every problem
+  long long left = (long long)max_steps + 1;
+  while (left > 0) {
+    const long long r = std::min<long long>(check_every, left);
+    for (long long k = 0; k < r; k++) HIPCHK(hipGraphLaunch(b->exec, b->s));
+    left -= r;
+    HIPCHK(hipMemcpyAsync(b->h_st, b->d_st, b->np * sizeof(BState), hipMemcpyDeviceToHost, b->s));
+    HIPCHK(hipStreamSynchronize(b->s));
+    if (batch_all_done(b)) break;
+  }
+  std::string bad;
+  for (int p = 0; p < b->np; p++) {
+    const BState& st = b->h_st[p];
+    t[p] = st.t;
+    steps[p] = st.steps;
+    gks2d_ctx* c = b->ctx[p];
+    c->last_slot = st.slot;
+    if (st.steps > 0) c->last_flux_stage = 1;
+    if (st.blowup) {
+      const double zero = 0.0;
+      HIPCHK(hipMemcpy(c->d.stat + 4, &zero, sizeof(double), hipMemcpyHostToDevice));
+      bad += (bad.empty() ? "" : ", ") + std::to_string(p);
+    }
+  }
+  if (!bad.empty()) return fail(GKS_ERR_BLOWUP, ("density or pressure negative or NaN in problem(s) " + bad).c_str());
+  return GKS_OK;
+}
+
+int gks2d_batch_autotune(gks2d_batch* b, const int32_t* bc, int32_t steps, int32_t* tanflux_faces,
+                         int32_t* largest_first, double* ms_per_step) {
+  if (!b || !bc) return fail(GKS_ERR_ARG, "null");
+  if (steps < 1) return fail(GKS_ERR_ARG, "steps");
+  int learned = 0;
+  RC(batch_upload_devs(b, &learned));
+  // snapshot of what a step changes and a later step reads
+  std::vector<double*> snap(b->np, nullptr);
+  std::vector<unsigned long long*> lsnap(b->np, nullptr);
+  auto release = [&]() {
+    for (double* q : snap)
+      if (q) (void)hipFree(q);
+    for (unsigned long long* q : lsnap)
+      if (q) (void)hipFree(q);
+  };
+  auto copy_state = [&](bool save) -> hipError_t {
+    for (int p = 0; p < b->np; p++) {
+      const Dev& d = b->ctx[p]->d;
+      double* q = snap[p];
+      const size_t nc = (size_t)d.nc;
+      struct {
+        double* dev;
+        size_t n;
+      } parts[4] = {{d.U, 4 * nc}, {d.ax, nc}, {d.ay, nc}, {d.stat, 8}};
+      size_t off = 0;
+      for (auto& part : parts) {
+        hipError_t e = save ? hipMemcpyAsync(q + off, part.dev, part.n * 8, hipMemcpyDeviceToDevice, b->s)
+                            : hipMemcpyAsync(part.dev, q + off, part.n * 8, hipMemcpyDeviceToDevice, b->s);
+        if (e != hipSuccess) return e;
+        off += part.n;
+      }
+      if (lsnap[p]) {
+        hipError_t e = save ? hipMemcpyAsync(lsnap[p], d.lstat, kLstatWords * 8, hipMemcpyDeviceToDevice, b->s)
+                            : hipMemcpyAsync(d.lstat, lsnap[p], kLstatWords * 8, hipMemcpyDeviceToDevice, b->s);
+        if (e != hipSuccess) return e;
+      }
+    }
+    return hipSuccess;
+  };
+  for (int p = 0; p < b->np; p++) {
+    const Dev& d = b->ctx[p]->d;
+    hipError_t e = hipMalloc(&snap[p], (6 * (size_t)d.nc + 8) * sizeof(double));
+    if (e == hipSuccess && d.lstat) e = hipMalloc(&lsnap[p], kLstatWords * sizeof(unsigned long long));
+    if (e != hipSuccess) {
+      release();
+      return hip_fail(e, "gks2d_batch_autotune snapshot");
+    }
+  }
+  std::vector<double> t0(b->np, 0.0), big(b->np, 1e300);
+  std::vector<int32_t> s0(b->np, 0);
+  hipError_t e = copy_state(true);
+  const BatchShape keep = b->shape;
+  const int keep_order = b->largest_first;
+  double best = 1e300;
+  int best_tf = keep.tanflux_faces, best_order = keep_order;
+  int rc = GKS_OK;
+  const int tfs[2] = {64, 128};
+  for (int ti = 0; ti < 2 && e == hipSuccess && rc == GKS_OK; ti++) {
+    for (int order = 0; order < 2 && e == hipSuccess && rc == GKS_OK; order++) {
+      b->shape.tanflux_faces = tfs[ti];
+      b->largest_first = order;
+      if ((rc = batch_init_state(b, bc, big.data(), steps + 1, t0.data(), s0.data()))) break;
+      if ((rc = batch_graph(b, learned))) break;
+      e = hipGraphLaunch(b->exec, b->s);  // warm-up step
+      if (e == hipSuccess) e = hipEventRecord(b->e0, b->s);
+      for (int k = 0; k < steps && e == hipSuccess; k++) e = hipGraphLaunch(b->exec, b->s);
+      if (e == hipSuccess) e = hipEventRecord(b->e1, b->s);
+      if (e == hipSuccess) e = hipEventSynchronize(b->e1);
+      float ms = 0.f;
+      if (e == hipSuccess) e = hipEventElapsedTime(&ms, b->e0, b->e1);
+      if (e == hipSuccess && ms / steps < best) {
+        best = ms / steps;
+        best_tf = tfs[ti];
+        best_order = order;
+      }
+      if (e == hipSuccess) e = copy_state(false);
+    }
+  }
+  if (e == hipSuccess) e = hipStreamSynchronize(b->s);
+  release();
+  if (rc) {
+    b->shape = keep;
+    b->largest_first = keep_order;
+    return rc;
+  }
+  if (e != hipSuccess) return hip_fail(e, "gks2d_batch_autotune");
+  b->shape.tanflux_faces = best_tf;
+  b->largest_first = best_order;
+  if (tanflux_faces) *tanflux_faces = best_tf;
+  if (largest_first) *largest_first = best_order;
+  if (ms_per_step) *ms_per_step = best;
   return GKS_OK;
 }
 

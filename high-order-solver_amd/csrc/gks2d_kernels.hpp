@@ -34,8 +34,8 @@ namespace GKS_VARIANT {
 
 using namespace gksd;
 
-constexpr int kBlock = 256;
-constexpr int kPartialBlocks = 1024;
+constexpr int kBlock = kSweepBlock;
+constexpr int kPartialBlocks = kSweepPartials;
 
 __device__ __forceinline__ long long gtid() { return (long long)blockIdx.x * blockDim.x + threadIdx.x; }
 
@@ -61,8 +61,8 @@ __device__ __forceinline__ unsigned xcd_block() {
 // free / periodic / reflective fills of one side, in the reference's
 // semantics (boundary_condition.cpp:66-186, 367-442): left/right cover all
 // rows, down/up all columns (including the ghost columns just filled).
-__global__ void __launch_bounds__(kBlock) k_bc(Dev d, int side, int kind) {
-  long long t = gtid();
+__device__ __forceinline__ void bc_body(const Dev& d, unsigned blk, int side, int kind) {
+  long long t = (long long)blk * kBlock + threadIdx.x;
   const int len = side < 2 ? d.ny : d.nx;
   if (t >= (long long)d.G * len) return;
   const int g = (int)(t / len), k = (int)(t % len);
@@ -93,18 +93,20 @@ __global__ void __launch_bounds__(kBlock) k_bc(Dev d, int side, int kind) {
     d.U[v * d.nc + dst] = val;
   }
 }
+__global__ void __launch_bounds__(kBlock) k_bc(Dev d, int side, int kind) { bc_body(d, blockIdx.x, side, kind); }
 
 // ------------------------------------------------------- primvar + CFL speed
 // Convar_to_Primvar (basic_function.cpp:752-773): blow-up test on all cells;
 // plus the Dtx speed sqrt(u^2+v^2)+c (time_advance.cpp:292-305) maxed over
 // the interior, which is what the next Get_CFL reads.  from_prim: take the
 // primitive variables from `prim` instead (step 0, initial condition).
-__global__ void __launch_bounds__(kBlock) k_primvar(Dev d, const double* prim, double* partial) {
+__device__ __forceinline__ void primvar_body(const Dev& d, unsigned blk, unsigned nblk, const double* prim,
+                                             double* partial) {
   __shared__ double smax[kBlock];
   __shared__ int sbad[kBlock];
   double tmax = 0.0;
   int bad = 0;
-  for (long long c = gtid(); c < d.nc; c += (long long)gridDim.x * blockDim.x) {
+  for (long long c = (long long)blk * kBlock + threadIdx.x; c < d.nc; c += (long long)nblk * kBlock) {
     double p[4];
     if (prim) {
 #pragma unroll
@@ -134,13 +136,16 @@ __global__ void __launch_bounds__(kBlock) k_primvar(Dev d, const double* prim, d
     __syncthreads();
   }
   if (threadIdx.x == 0) {
-    partial[2 * blockIdx.x] = smax[0];
-    partial[2 * blockIdx.x + 1] = sbad[0] ? 1.0 : 0.0;
+    partial[2 * blk] = smax[0];
+    partial[2 * blk + 1] = sbad[0] ? 1.0 : 0.0;
   }
+}
+__global__ void __launch_bounds__(kBlock) k_primvar(Dev d, const double* prim, double* partial) {
+  primvar_body(d, blockIdx.x, gridDim.x, prim, partial);
 }
 // final max / or over the partials into stat[slot*2 .. slot*2+1]; the flag
 // accumulates (cleared by the host when it reads it)
-__global__ void __launch_bounds__(kBlock) k_reduce(const double* partial, int n, double* stat, int slot) {
+__device__ __forceinline__ void reduce_body(const double* partial, int n, double* stat, int slot) {
   __shared__ double smax[kBlock];
   __shared__ double sbad[kBlock];
   double m = 0.0, b = 0.0;
@@ -165,16 +170,20 @@ __global__ void __launch_bounds__(kBlock) k_reduce(const double* partial, int n,
     if (sbad[0] != 0.0) stat[4] = 1.0;  // sticky blow-up flag
   }
 }
+__global__ void __launch_bounds__(kBlock) k_reduce(const double* partial, int n, double* stat, int slot) {
+  reduce_body(partial, n, stat, slot);
+}
 
 // CopyFluid_new_to_old (basic_function.cpp:1571-1584), interior cells
-__global__ void __launch_bounds__(kBlock) k_copy_old(Dev d) {
-  long long t = gtid();
+__device__ __forceinline__ void copy_old_body(const Dev& d, unsigned blk) {
+  long long t = (long long)blk * kBlock + threadIdx.x;
   if (t >= (long long)d.NX * d.NY) return;
   const int i = (int)(t / d.NY) + d.G, j = (int)(t % d.NY) + d.G;
   const long long c = (long long)i * d.ny + j;
 #pragma unroll
   for (int v = 0; v < 4; v++) d.Uold[v * d.nc + c] = d.U[v * d.nc + c];
 }
+__global__ void __launch_bounds__(kBlock) k_copy_old(Dev d) { copy_old_body(d, blockIdx.x); }
 
 // ------------------------------------------------------ normal reconstruction
 // WENO5_AO_with_df (reconstruction.cpp:1731-1859) for one cell along DIR;
@@ -241,9 +250,8 @@ __device__ __forceinline__ void weno5_normal(double* lo, double* ro, const doubl
 #ifndef GKS_NORMAL_WAVES
 #define GKS_NORMAL_WAVES 1
 #endif
-template <int DIR>
-__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(GKS_NORMAL_WAVES))) k_normal(Dev d) {
-  long long t = (long long)xcd_block() * blockDim.x + threadIdx.x;
+template <int DIR> __device__ __forceinline__ void normal_body(const Dev& d, unsigned blk) {
+  long long t = (long long)blk * kBlock + threadIdx.x;
   int i, j;
   if (DIR == 0) {
     if (t >= (long long)(d.NX + 2) * d.ny) return;
@@ -284,6 +292,10 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(GKS
     line[(8 + k) * nf + fa] = lo[k];  // side 1 (right state) of the face at the cell's left
     line[k * nf + fb] = ro[k];        // side 0 (left state) of the face at the cell's right
   }
+}
+template <int DIR>
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(GKS_NORMAL_WAVES))) k_normal(Dev d) {
+  normal_body<DIR>(d, xcd_block());
 }
 
 // -------------------------------------------------- tangential reconstruction
@@ -450,12 +462,10 @@ __device__ __forceinline__ void count_linear(unsigned long long* stat, bool lin)
 // the fifth-order candidate, formed directly on the conservative variables
 // (the characteristic projection commutes with a fixed linear combination);
 // the order-reduction test is kept.  Every other stencil runs WENO5-AO-DF.
-template <int DIR>
-__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(GKS_NORMAL_WAVES)))
-k_normal_learned(Dev d) {
+template <int DIR> __device__ __forceinline__ void normal_learned_body(const Dev& d, unsigned blk) {
   __shared__ __align__(16) short xs[kBlock * 16];
   __shared__ float ls[kBlock];
-  const long long t = (long long)xcd_block() * blockDim.x + threadIdx.x;
+  const long long t = (long long)blk * kBlock + threadIdx.x;
   const bool valid = t < normal_count<DIR>(d);
   int i, j;
   normal_cell<DIR>(d, valid ? t : 0, i, j);
@@ -510,6 +520,11 @@ k_normal_learned(Dev d) {
     line[(8 + k) * nf + fa] = lo[k];
     line[k * nf + fb] = ro[k];
   }
+}
+template <int DIR>
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(GKS_NORMAL_WAVES)))
+k_normal_learned(Dev d) {
+  normal_learned_body<DIR>(d, xcd_block());
 }
 
 // logits of n feature rows (parity of the MFMA inference)
@@ -866,14 +881,13 @@ k_flux(Dev d, int stage, double dt) {
 // (a few hundred bytes per lane, cache resident) and the second wave hides
 // the latency: 1.54 -> 1.16 ms per launch at 2048^2.
 constexpr int kTanFlux = 128;
-template <int DIR, bool LEARNED>
-__global__ void __launch_bounds__(2 * kTanFlux) __attribute__((amdgpu_waves_per_eu(GKS_FLUX_WAVES)))
-k_tanflux(Dev d, int stage, double dt) {
-  __shared__ double sg[2][2][12][kTanFlux];  // [side][gauss point][component][face]
+template <int DIR, bool LEARNED, int TF>
+__device__ __forceinline__ void tanflux_body(const Dev& d, unsigned blk, int stage, double dt) {
+  __shared__ double sg[2][2][12][TF];  // [side][gauss point][component][face]
   const int ni = DIR == 0 ? d.NX + 1 : d.NX, nj = DIR == 0 ? d.NY : d.NY + 1;
   const long long nfaces = (long long)ni * nj;
-  const int tl = threadIdx.x % kTanFlux, hi = threadIdx.x / kTanFlux;
-  const long long r = (long long)xcd_block() * kTanFlux + tl;
+  const int tl = threadIdx.x % TF, hi = threadIdx.x / TF;
+  const long long r = (long long)blk * TF + tl;
   const bool valid = r < nfaces;
   const int i = valid ? (int)(r / nj) + d.G : d.G, j = valid ? (int)(r % nj) + d.G : d.G;
   const long long nf = DIR == 0 ? d.nfx : d.nfy;
@@ -958,17 +972,22 @@ k_tanflux(Dev d, int stage, double dt) {
   __syncthreads();
   if (valid) {
     const int q = hi;
-    const Strided L{&sg[0][q][0][tl], kTanFlux}, R{&sg[1][q][0][tl], kTanFlux};
+    const Strided L{&sg[0][q][0][tl], TF}, R{&sg[1][q][0][tl], TF};
     flux_point_tail<DIR>(d, q, f0, nf, L, R, stage, dt);
   }
+}
+template <int DIR, bool LEARNED>
+__global__ void __launch_bounds__(2 * kTanFlux) __attribute__((amdgpu_waves_per_eu(GKS_FLUX_WAVES)))
+k_tanflux(Dev d, int stage, double dt) {
+  tanflux_body<DIR, LEARNED, kTanFlux>(d, xcd_block(), stage, dt);
 }
 
 // ------------------------------------------------------------------ update
 // cell update of Update_with_gauss (time_advance.cpp:442-474): U = U_old +
 // sum over Gauss points of len * (x_S - x_N + x_W - x_E), in the reference's
 // summation order, divided by the cell area
-__global__ void __launch_bounds__(kBlock) k_update(Dev d) {
-  long long t = gtid();
+__device__ __forceinline__ void update_body(const Dev& d, unsigned blk) {
+  long long t = (long long)blk * kBlock + threadIdx.x;
   if (t >= (long long)d.NX * d.NY) return;
   const int i = (int)(t / d.NY) + d.G, j = (int)(t % d.NY) + d.G;
   const long long c = (long long)i * d.ny + j;
@@ -992,12 +1011,13 @@ __global__ void __launch_bounds__(kBlock) k_update(Dev d) {
     d.U[v * d.nc + c] = u;
   }
 }
+__global__ void __launch_bounds__(kBlock) k_update(Dev d) { update_body(d, blockIdx.x); }
 
 // Update_alpha (reconstruction.cpp:1466-1534): the reference bounds j by
 // nodex (:1528); kept for nodey >= nodex, clipped to nodey otherwise.
-__global__ void __launch_bounds__(kBlock) k_alpha(Dev d) {
+__device__ __forceinline__ void alpha_body(const Dev& d, unsigned blk) {
   const int nj = d.NY < d.NX ? d.NY : d.NX;
-  long long t = gtid();
+  long long t = (long long)blk * kBlock + threadIdx.x;
   if (t >= (long long)d.NX * nj) return;
   const int i = (int)(t / nj) + d.G, j = (int)(t % nj) + d.G;
   auto face_alpha = [&](const double* ak, long long nf, long long f) {
@@ -1015,6 +1035,7 @@ __global__ void __launch_bounds__(kBlock) k_alpha(Dev d) {
   d.ax[c] = al + ar;
   d.ay[c] = ad + au;
 }
+__global__ void __launch_bounds__(kBlock) k_alpha(Dev d) { alpha_body(d, blockIdx.x); }
 
 // leaf parity: centre + GKS2D over n independent points
 __global__ void __launch_bounds__(kBlock) k_flux_points(Gas g, long long n, const double* L, const double* R,
@@ -1037,6 +1058,125 @@ __global__ void __launch_bounds__(kBlock) k_flux_points(Gas g, long long n, cons
   }
 }
 
+// ------------------------------------------------------------------ batches
+// A batched launch covers the tiles of every problem of a batch: workgroup
+// b (after the XCD mapping) finds its problem by a binary search of the
+// work class's tile prefix table, and runs the same body as the single-
+// problem kernel on that problem's device state.  Problems not active in
+// this step return at once.
+__device__ __forceinline__ int batch_slot(const BatchArgs& B, int kind, unsigned gb) {
+  const unsigned* st = B.start + kind * (B.np + 1);
+  int lo = 0, hi = B.np;
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (st[mid] <= gb) lo = mid;
+    else hi = mid;
+  }
+  return lo;
+}
+#define GKS_BATCH_TILE(kind)                                    \
+  const unsigned gb_ = xcd_block();                             \
+  const int q_ = batch_slot(B, kind, gb_);                      \
+  const int p_ = B.pid[q_];                                     \
+  if (!B.st[p_].active) return;                                 \
+  const Dev& d = B.dev[p_];                                     \
+  const unsigned blk = gb_ - B.start[(kind) * (B.np + 1) + q_]; \
+  (void)d;                                                      \
+  (void)blk
+
+__global__ void __launch_bounds__(kBlock) kb_bc(BatchArgs B, int side) {
+  GKS_BATCH_TILE(side < 2 ? KB_BC_LR : KB_BC_DU);
+  bc_body(d, blk, side, B.st[p_].bc[side]);
+}
+__global__ void __launch_bounds__(kBlock) kb_primvar(BatchArgs B) {
+  GKS_BATCH_TILE(KB_PRIMVAR);
+  const unsigned* st = B.start + KB_PRIMVAR * (B.np + 1);
+  primvar_body(d, blk, st[q_ + 1] - st[q_], nullptr, d.partial);
+}
+// one workgroup per problem
+__global__ void __launch_bounds__(kBlock) kb_reduce(BatchArgs B, int slot) {
+  const int q = blockIdx.x, p = B.pid[q];
+  if (!B.st[p].active) return;
+  const unsigned* st = B.start + KB_PRIMVAR * (B.np + 1);
+  reduce_body(B.dev[p].partial, (int)(st[q + 1] - st[q]), B.dev[p].stat, slot);
+}
+__global__ void __launch_bounds__(kBlock) kb_copy_old(BatchArgs B) {
+  GKS_BATCH_TILE(KB_COPY);
+  copy_old_body(d, blk);
+}
+template <int DIR, bool LEARNED>
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(GKS_NORMAL_WAVES))) kb_normal(BatchArgs B) {
+  GKS_BATCH_TILE(DIR == 0 ? KB_NORMAL_X : KB_NORMAL_Y);
+  if constexpr (LEARNED) normal_learned_body<DIR>(d, blk);
+  else normal_body<DIR>(d, blk);
+}
+template <int DIR, bool LEARNED, int TF>
+__global__ void __launch_bounds__(2 * TF) __attribute__((amdgpu_waves_per_eu(GKS_FLUX_WAVES)))
+kb_tanflux(BatchArgs B, int stage) {
+  GKS_BATCH_TILE(DIR == 0 ? KB_FLUX_X : KB_FLUX_Y);
+  tanflux_body<DIR, LEARNED, TF>(d, blk, stage, B.st[p_].dt);
+}
+__global__ void __launch_bounds__(kBlock) kb_update(BatchArgs B) {
+  GKS_BATCH_TILE(KB_UPDATE);
+  update_body(d, blk);
+}
+__global__ void __launch_bounds__(kBlock) kb_alpha(BatchArgs B) {
+  GKS_BATCH_TILE(KB_ALPHA);
+  if (!d.use_df) return;
+  alpha_body(d, blk);
+}
+
+// start of a step: which problems run, and their dt (Get_CFL / Dtx,
+// time_advance.cpp:264-305, from the stale speed of the last primvar, as
+// gks2d_run computes it on the host; same operations, not contracted)
+__global__ void __launch_bounds__(64) kb_step_begin(BatchArgs B) {
+#pragma clang fp contract(off)
+  for (int p = threadIdx.x; p < B.np; p += blockDim.x) {
+    BState& st = B.st[p];
+    const Dev& d = B.dev[p];
+    st.active = 0;
+    if (st.done) continue;
+    if (d.stat[4] != 0.0) {
+      st.blowup = 1;
+      st.done = 1;
+      continue;
+    }
+    if (!(st.t < st.tstop) || st.steps >= st.max_steps) {
+      st.done = 1;
+      continue;
+    }
+    const double tmax = d.stat[2 * st.slot];
+    double dt = d.dx;
+    const double hs[2] = {d.dx, d.dy};
+    for (int k = 0; k < 2; k++) {
+      const double h = hs[k];
+      if (tmax > st.cfl * h / dt) dt = st.cfl * h / tmax;
+      if (dt > 0.25 * st.cfl * h * h / d.gas.mu && d.gas.tau_type == 1 && d.gas.mu > 0)
+        dt = 0.25 * st.cfl * h * h / d.gas.mu;
+    }
+    if (st.t + dt > st.tstop) dt = st.tstop - st.t + 1e-16;
+    st.dt = dt;
+    st.active = 1;
+  }
+}
+// end of a step: a problem whose state blew up stops without advancing t
+__global__ void __launch_bounds__(64) kb_step_end(BatchArgs B) {
+#pragma clang fp contract(off)
+  for (int p = threadIdx.x; p < B.np; p += blockDim.x) {
+    BState& st = B.st[p];
+    if (!st.active) continue;
+    if (B.dev[p].stat[4] != 0.0) {
+      st.blowup = 1;
+      st.done = 1;
+      st.active = 0;
+      continue;
+    }
+    st.t = st.t + st.dt;
+    st.steps++;
+    st.slot = 1;
+  }
+}
+
 // explicit instantiations: the launchers below are host-only, so the device
 // pass must be told which template kernels to emit
 template __global__ void k_normal79<0, 7>(Dev);
@@ -1055,6 +1195,18 @@ template __global__ void k_tanflux<0, false>(Dev, int, double);
 template __global__ void k_tanflux<1, false>(Dev, int, double);
 template __global__ void k_tanflux<0, true>(Dev, int, double);
 template __global__ void k_tanflux<1, true>(Dev, int, double);
+template __global__ void kb_normal<0, false>(BatchArgs);
+template __global__ void kb_normal<1, false>(BatchArgs);
+template __global__ void kb_normal<0, true>(BatchArgs);
+template __global__ void kb_normal<1, true>(BatchArgs);
+template __global__ void kb_tanflux<0, false, 64>(BatchArgs, int);
+template __global__ void kb_tanflux<1, false, 64>(BatchArgs, int);
+template __global__ void kb_tanflux<0, true, 64>(BatchArgs, int);
+template __global__ void kb_tanflux<1, true, 64>(BatchArgs, int);
+template __global__ void kb_tanflux<0, false, 128>(BatchArgs, int);
+template __global__ void kb_tanflux<1, false, 128>(BatchArgs, int);
+template __global__ void kb_tanflux<0, true, 128>(BatchArgs, int);
+template __global__ void kb_tanflux<1, true, 128>(BatchArgs, int);
 template __global__ void k_normal_learned<0>(Dev);
 template __global__ void k_normal_learned<1>(Dev);
 template __global__ void k_label<0>(Dev, int, float*, float*);
@@ -1163,7 +1315,42 @@ static hipError_t infer(const float* P, const unsigned short* W, long long n, co
   return hipGetLastError();
 }
 
-const LaunchTable table = {bc, primvar, copy_old, normal, tangent, flux, update, alpha, flux_points, label, infer};
+template <bool LEARNED, int TF>
+static void batch_stage(const BatchArgs& B, const unsigned* tiles, int stage, hipStream_t s) {
+  for (int side = 0; side < 4; side++)
+    hipLaunchKernelGGL(kb_bc, dim3(tiles[side < 2 ? KB_BC_LR : KB_BC_DU]), dim3(kBlock), 0, s, B, side);
+  hipLaunchKernelGGL(kb_primvar, dim3(tiles[KB_PRIMVAR]), dim3(kBlock), 0, s, B);
+  hipLaunchKernelGGL(kb_reduce, dim3(B.np), dim3(kBlock), 0, s, B, stage);
+  hipLaunchKernelGGL((kb_normal<0, LEARNED>), dim3(tiles[KB_NORMAL_X]), dim3(kBlock), 0, s, B);
+  hipLaunchKernelGGL((kb_normal<1, LEARNED>), dim3(tiles[KB_NORMAL_Y]), dim3(kBlock), 0, s, B);
+  hipLaunchKernelGGL((kb_tanflux<0, LEARNED, TF>), dim3(tiles[KB_FLUX_X]), dim3(2 * TF), 0, s, B, stage);
+  hipLaunchKernelGGL((kb_tanflux<1, LEARNED, TF>), dim3(tiles[KB_FLUX_Y]), dim3(2 * TF), 0, s, B, stage);
+  hipLaunchKernelGGL(kb_update, dim3(tiles[KB_UPDATE]), dim3(kBlock), 0, s, B);
+  hipLaunchKernelGGL(kb_alpha, dim3(tiles[KB_ALPHA]), dim3(kBlock), 0, s, B);
+}
+template <bool LEARNED, int TF>
+static void batch_step_t(const BatchArgs& B, const unsigned* tiles, hipStream_t s) {
+  hipLaunchKernelGGL(kb_step_begin, dim3(1), dim3(64), 0, s, B);
+  hipLaunchKernelGGL(kb_copy_old, dim3(tiles[KB_COPY]), dim3(kBlock), 0, s, B);
+  batch_stage<LEARNED, TF>(B, tiles, 0, s);
+  batch_stage<LEARNED, TF>(B, tiles, 1, s);
+  hipLaunchKernelGGL(kb_step_end, dim3(1), dim3(64), 0, s, B);
+}
+// the batch path is WENO5-AO-DF with the fused tangential + flux launch and
+// the DF sensor update, as gks2d_run's stage sequence for weno_order 5
+static hipError_t batch_step(const BatchArgs& B, const BatchShape& sh, const unsigned* tiles, hipStream_t s) {
+  if (sh.learned) {
+    if (sh.tanflux_faces == 64) batch_step_t<true, 64>(B, tiles, s);
+    else batch_step_t<true, 128>(B, tiles, s);
+  } else {
+    if (sh.tanflux_faces == 64) batch_step_t<false, 64>(B, tiles, s);
+    else batch_step_t<false, 128>(B, tiles, s);
+  }
+  return hipGetLastError();
+}
+
+const LaunchTable table = {bc, primvar, copy_old, normal, tangent, flux, update, alpha, flux_points, label, infer,
+                          batch_step};
 #endif
 
 }  // namespace GKS_VARIANT

@@ -40,6 +40,45 @@ struct Dev {
   double* stat;     // [0..3]: max speed of slot 0 / 1 (+pad); [4]: sticky blow-up flag
 };
 
+// workgroup size of the sweep kernels and the number of per-block partials
+// of the CFL speed reduction (the host sizes batched launches with them)
+constexpr int kSweepBlock = 256;
+constexpr int kSweepPartials = 1024;
+
+// ------------------------------------------------------------------ batches
+// Per-problem time-loop state of a batch, on the device: the driver loop of
+// gks2d_run (dt from the stale stage-1 speed, clipped to tstop; stop at
+// tstop, at max_steps or on blow-up) kept per problem so that a captured
+// step needs no host round trip.
+struct BState {
+  double t, tstop, dt, cfl;
+  int active;  // this step runs for the problem
+  int done;    // reached tstop or max_steps, or blew up
+  int slot;    // stat slot the next dt reads (the last primvar's)
+  int steps, max_steps;
+  int blowup;
+  int bc[4];
+};
+// work classes of a batched launch; each has its own tile prefix table
+enum BatchKind {
+  KB_BC_LR = 0, KB_BC_DU, KB_PRIMVAR, KB_COPY, KB_NORMAL_X, KB_NORMAL_Y, KB_FLUX_X, KB_FLUX_Y, KB_UPDATE,
+  KB_ALPHA, KB_COUNT
+};
+// start[k * (np + 1) + q]: first tile of the q-th problem in launch order for
+// work class k (start[.. + np] = tiles in the launch); pid[q]: that problem
+struct BatchArgs {
+  const Dev* dev;
+  BState* st;
+  const unsigned* start;
+  const int* pid;
+  int np;
+};
+// tile shapes of a batch (chosen by gks2d_batch_autotune)
+struct BatchShape {
+  int tanflux_faces;  // faces per flux workgroup: 64 or 128
+  int learned;        // the classifier runs (set on every problem)
+};
+
 // device phases timed by gks2d_timing (order of GKS_PHASE_* in gks2d.h)
 enum Phase {
   PH_BC = 0, PH_PRIMVAR, PH_NORMAL_X, PH_NORMAL_Y, PH_TANGENT_X, PH_TANGENT_Y,
@@ -62,6 +101,9 @@ struct LaunchTable {
   // logits of n feature rows through the solver's MFMA inference
   hipError_t (*infer)(const float* P, const unsigned short* W, long long n, const float* X, float* out,
                         hipStream_t);
+  // one S2O4 step of every active problem of a batch (capturable: no host
+  // synchronisation, no memset); tiles[k] = start[k * (np + 1) + np]
+  hipError_t (*batch_step)(const BatchArgs&, const BatchShape&, const unsigned* tiles, hipStream_t);
 };
 
 namespace ieee { extern const LaunchTable table; }

@@ -140,3 +140,29 @@ def test_1d_null_arguments_are_argument_errors():
     assert L.gks1d_get_face_fluxes(None, 0, None) == 1
     assert L.gks1d_destroy(None) == 0
     assert L.gks1d_flux_points(None, 0, None, None, 0.0, None, None, None) == 1
+
+
+HEADER_B = os.path.join(ROOT, "include", "gks2d_batch.h")
+
+
+def test_batch_library_exports_every_declared_symbol():
+    from gks_amd import batch
+    names = sorted(set(re.findall(r"\b(gks2d_batch_[a-z_0-9]+)\s*\(", open(HEADER_B).read())))
+    assert len(names) == 6
+    lib = gks_amd.lib()
+    assert not [n for n in names if not hasattr(lib, n)]
+    assert set(names) <= set(batch._SIGS)
+
+
+def test_batch_rejects_bad_arguments_on_host():
+    from gks_amd import batch
+    L = batch.L()
+    h = ctypes.c_void_p()
+    arr = (Params * 2)(planar_shock_params(16), planar_shock_params(16).copy(weno_order=7, gausspoint=3))
+    assert L.gks2d_batch_create(arr, 2, ctypes.byref(h)) == 2 and not h.value
+    arr = (Params * 2)(planar_shock_params(16), planar_shock_params(16, fast_math=1))
+    assert L.gks2d_batch_create(arr, 2, ctypes.byref(h)) == 1 and not h.value
+    assert L.gks2d_batch_create(arr, 0, ctypes.byref(h)) == 1
+    assert L.gks2d_batch_set_shape(None, 64, 1) == 1
+    assert L.gks2d_batch_run(None, None, None, 1, 1, None, None) == 1
+    assert L.gks2d_batch_destroy(None) == 0

@@ -126,8 +126,12 @@ def test_samples_match_host_features():
 
 
 def test_classifier_that_never_fires_gives_the_plain_run():
+    """IEEE build: its kernels round every operation as written, so the
+    learned kernels' WENO-AO branch must reproduce the plain kernels bit for
+    bit (the fast build contracts multiply-adds per kernel as the compiler
+    sees fit, so two different kernels need not round alike there)."""
     n = 48
-    p = planar_shock_params(n, fast_math=1)
+    p = planar_shock_params(n, fast_math=0)
     prim = riemann_prim(p, riemann_states(1))
     P = rand_params(9)
     P[1632] = 1e9  # logit bias: nothing is smooth
