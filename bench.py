@@ -92,6 +92,85 @@ print(json.dumps(dict(kind=kind, seconds=t1 - t0, rc=rc)))
                       f"{r['seconds']:.1f} s wall, OMP_NUM_THREADS={threads}"}
 
 
+BATCH_SIDES = (128, 256, 384, 512, 768, 1024)
+
+
+def batch_sizes(n, seed):
+    """Side lengths of the batched workload: uniform over BATCH_SIDES."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    return [int(BATCH_SIDES[k]) for k in rng.integers(0, len(BATCH_SIDES), n)]
+
+
+def bench_batch(args):
+    """Batched workload: args.batch independent seeded four-quadrant Riemann
+    problems with uneven grids, WENO5-AO-DF + GKS + S2O4, free boundaries,
+    all advanced together (one launch per sweep over every problem's tiles,
+    one captured graph per step), against the same problems run one after
+    the other through the single-problem path.  Both legs time the same
+    steps: `warmup` untimed then `steps` timed steps of every problem (the
+    window ends before the smallest grid reaches t = 0.2)."""
+    import gks_amd
+    from gks_amd import learn
+    from gks_amd.batch import Batch
+    from gks_amd.params import BC_FREE, planar_shock_params
+    from gks_amd.problems import riemann_prim, riemann_states
+
+    sides = batch_sizes(args.batch, args.seed)
+    seeds = [1000 + args.seed * 1000 + k for k in range(args.batch)]
+    fast = 1 if args.variant == "fast" else 0
+    params = [planar_shock_params(n, fast_math=fast) for n in sides]
+    cls = learn.load(args.classifier or learn.DEFAULT_PARAMS) if args.path == "learned" else None
+    bc = [BC_FREE] * 4
+    cells = sum(n * n for n in sides)
+
+    b = Batch(params)
+    for k, s in enumerate(b.problems):
+        s.init_prim(riemann_prim(s.p, riemann_states(seeds[k])))
+        if cls:
+            learn.set_classifier(s, cls[0], cls[1])
+    tf, lf, tune_ms = b.autotune(bc, steps=2)
+    b.run(bc, 0.2, args.warmup)
+    t0 = time.perf_counter()
+    done = b.run(bc, 0.2, args.steps)
+    wall_b = time.perf_counter() - t0
+    upd_b = sum(n * n * int(k) for n, k in zip(sides, done))
+    b.close()
+
+    # the same problems, one after the other, through gks2d_run
+    upd_s, wall_s = 0, 0.0
+    for k, p in enumerate(params):
+        s = gks_amd.Solver2D(p)
+        s.init_prim(riemann_prim(p, riemann_states(seeds[k])))
+        if cls:
+            learn.set_classifier(s, cls[0], cls[1])
+        s.run(bc, 0.2, args.warmup)
+        s.sync()
+        t0 = time.perf_counter()
+        m = s.run(bc, 0.2, args.steps)
+        s.sync()
+        wall_s += time.perf_counter() - t0
+        upd_s += p.nodex * p.nodey * m
+        s.close()
+    value = upd_b / wall_b / 1e6
+    out = {
+        "metric": "Mcell-updates/s FP64, batched uneven 2D four-quadrant Riemann problems, WENO5-AO GKS",
+        "value": round(value, 3), "unit": "Mcell-updates/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(wall_b / args.steps * 1e3, 3), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"{args.batch} seeded four-quadrant Riemann problems, sides uniform over "
+                               f"{list(BATCH_SIDES)} (seed {args.seed}), {cells / 1e6:.2f} M cells, WENO5-AO-DF "
+                               f"characteristic, GKS2D gks2nd, S2O4, CFL 0.5, free BCs ({args.path} path, "
+                               f"{args.variant} kernels)",
+                   "problems": args.batch, "sides": sides, "path": args.path, "kernels": args.variant},
+        "autotune": {"tanflux_faces": tf, "largest_first": lf, "ms_per_step": round(tune_ms, 3)},
+        "sequential": {"value": round(upd_s / wall_s / 1e6, 3), "wall_s": round(wall_s, 3)},
+        "batch_vs_sequential": round(value / (upd_s / wall_s / 1e6), 3),
+        "roofline": None, "cpu_baseline": None,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -106,6 +185,9 @@ def main():
     ap.add_argument("--path", choices=["plain", "learned"], default="plain",
                     help="learned: the trained troubled-interface classifier picks linear weights for smooth stencils")
     ap.add_argument("--classifier", default=None, help="trained classifier (.npz); default: the packaged one")
+    ap.add_argument("--workload", choices=["single", "batch"], default="single",
+                    help="batch: --batch uneven problems stepped together, against running them one by one")
+    ap.add_argument("--batch", type=int, default=64)
     args = ap.parse_args()
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -113,6 +195,9 @@ def main():
         print("bench.py: this build runs on one GPU; multi-GPU runs are not part of it "
               f"(asked for --gpus {args.gpus}, WORLD_SIZE {world})", file=sys.stderr)
         sys.exit(3)
+    if args.workload == "batch":
+        bench_batch(args)
+        return
 
     import numpy as np  # noqa: F401
 

@@ -513,8 +513,8 @@ int gks2d_get_face_fluxes(gks2d_ctx* c, double* xf, double* yf) {
   return GKS_OK;
 }
 
-int gks2d_flux_points(const gks2d_params* p, int64_t n, const double* left, const double* right, double dt,
-                      double* center, double* f, double* derf) {
+static int flux_points_impl(const gks2d_params* p, int64_t n, const double* left, const double* right, double dt,
+                            double* center, double* f, double* derf, int smooth) {
   if (!p || !left || !right || !f || !derf) return fail(GKS_ERR_ARG, "null");
   if (n < 0) return fail(GKS_ERR_ARG, "n < 0");
   if (n == 0) return GKS_OK;
@@ -527,9 +527,9 @@ int gks2d_flux_points(const gks2d_params* p, int64_t n, const double* left, cons
   if (e == hipSuccess) e = hipMalloc(&dd, n * 4 * 8);
   if (e == hipSuccess) e = hipMemcpy(dl, left, n * 12 * 8, hipMemcpyHostToDevice);
   if (e == hipSuccess) e = hipMemcpy(dr, right, n * 12 * 8, hipMemcpyHostToDevice);
-  if (e == hipSuccess) e = lt->flux_points(gas_of(*p), n, dl, dr, dt, dc, dfo, dd, nullptr);
+  if (e == hipSuccess) e = lt->flux_points(gas_of(*p), n, dl, dr, dt, dc, dfo, dd, smooth, nullptr);
   if (e == hipSuccess) e = hipDeviceSynchronize();
-  if (e == hipSuccess && center) e = hipMemcpy(center, dc, n * 12 * 8, hipMemcpyDeviceToHost);
+  if (e == hipSuccess && center && !smooth) e = hipMemcpy(center, dc, n * 12 * 8, hipMemcpyDeviceToHost);
   if (e == hipSuccess) e = hipMemcpy(f, dfo, n * 4 * 8, hipMemcpyDeviceToHost);
   if (e == hipSuccess) e = hipMemcpy(derf, dd, n * 4 * 8, hipMemcpyDeviceToHost);
   (void)hipFree(dl);
@@ -539,6 +539,14 @@ int gks2d_flux_points(const gks2d_params* p, int64_t n, const double* left, cons
   (void)hipFree(dd);
   if (e != hipSuccess) return hip_fail(e, "gks2d_flux_points");
   return GKS_OK;
+}
+int gks2d_flux_points(const gks2d_params* p, int64_t n, const double* left, const double* right, double dt,
+                      double* center, double* f, double* derf) {
+  return flux_points_impl(p, n, left, right, dt, center, f, derf, 0);
+}
+int gks2d_flux_points_smooth(const gks2d_params* p, int64_t n, const double* left, const double* right, double dt,
+                             double* f, double* derf) {
+  return flux_points_impl(p, n, left, right, dt, nullptr, f, derf, 1);
 }
 
 // ------------------------------------------------- learned classifier hooks
@@ -590,10 +598,19 @@ int gks2d_set_classifier(gks2d_ctx* c, const float* params, float logit_threshol
   HIPCHK(hipMemset(c->d_lin, 0, 2 * nc));
   c->d.mlp = c->d_mlp;
   c->d.mlpw = c->d_mlpw;
+  c->d.smooth_flux = 1;
   c->d.lin[0] = c->d_lin;
   c->d.lin[1] = c->d_lin + nc;
   c->d.mlp_thr = logit_threshold;
   c->d.lstat = c->d_lstat;
+  return GKS_OK;
+}
+
+int gks2d_classifier_options(gks2d_ctx* c, int32_t smooth_flux) {
+  if (!c) return fail(GKS_ERR_ARG, "null");
+  if (!c->d.mlp) return fail(GKS_ERR_STATE, "no classifier set");
+  HIPCHK(hipStreamSynchronize(c->s));
+  c->d.smooth_flux = smooth_flux ? 1 : 0;
   return GKS_OK;
 }
 

@@ -472,6 +472,81 @@ GKS_DI void gks_point(double* f, double* derf, double* center, const V& L, const
 }
 
 // ------------------------------------------------------------- WENO5-AO (+DF)
+// ------------------------------------------------- smooth-flow GKS flux
+// No counterpart in the reference: the flux of a Gauss point whose two
+// states both come from linear-weight (smooth) reconstructions.  For
+// continuous data the BGK solution at the interface is the equilibrium
+// g0 [1 - tau (a u + b v + A) + t A] of the interface state (Xu's smooth
+// GKS flux); here the interface state and its slopes are the averages of
+// the two sides.  Its time integrals F(T) = rho0 [T g0u - tau T s4
+// + (T^2/2 - tau T) A0u] are what GKS2D's (dt, dt/2) combination gives when
+// the two states coincide (then the numerical collision time tau_num drops
+// out: t0 + t3 = T, t1 + t4 = -tau T, t2 + t5 = T^2/2 - tau T), so
+// f = 4 F(dt/2) - F(dt) = rho0 dt [g0u - tau (s4 + A0u)] and
+// derf = 4 (F(dt) - 2 F(dt/2)) = rho0 dt^2 A0u.  Only whole-space moments:
+// no erfc or exp.
+struct MMW {
+  double uw[7], vw[7], xi2, xi4;
+};
+GKS_DI void mmdf_whole(MMW& m, double u, double v, double lambda, const Gas& g) {
+  const double ol = 1.0 / lambda;
+  m.uw[0] = 1.0;
+  m.uw[1] = u;
+  m.vw[0] = 1.0;
+  m.vw[1] = v;
+#pragma unroll
+  for (int i = 2; i <= 6; i++) {
+    m.uw[i] = u * m.uw[i - 1] + 0.5 * (i - 1) * ol * m.uw[i - 2];
+    m.vw[i] = v * m.vw[i - 1] + 0.5 * (i - 1) * ol * m.vw[i - 2];
+  }
+  m.xi2 = 0.5 * g.K * ol;
+  m.xi4 = 0.25 * (g.K * g.K + 2 * g.K) * ol * ol;
+}
+struct TabWhole {
+  const MMW& m;
+  GKS_DI double operator()(int i, int j, int k) const {
+    return k == 0 ? m.uw[i] * m.vw[j] : (k == 1 ? m.uw[i] * m.vw[j] * m.xi2 : m.uw[i] * m.vw[j] * m.xi4);
+  }
+};
+template <class V>
+GKS_DI void gks_point_smooth(double* f, double* derf, const V& L, const V& Rr, double dt, const Gas& g) {
+  double c0[4], c0x[4], c0y[4];
+#pragma unroll
+  for (int k = 0; k < 4; k++) {
+    c0[k] = 0.5 * (L[k] + Rr[k]);
+    c0x[k] = 0.5 * (L[4 + k] + Rr[4 + k]);
+    c0y[k] = 0.5 * (L[8 + k] + Rr[8 + k]);
+  }
+  double p0[4];
+  convar_to_ulambda(p0, c0, g);
+  MMW m0;
+  mmdf_whole(m0, p0[1], p0[2], p0[3], g);
+  const TabWhole w0{m0};
+  const double unit[4] = {1.0, 0.0, 0.0, 0.0};
+  double g0u[4], a0x[4], a0y[4], a0xu[4], a0xuu[4], a0yv[4], a0yuv[4], derA0[4], A0[4], A0u[4];
+  contract(1, 0, 0, g0u, unit, w0);
+  micro_slope(a0x, c0x, p0, g);
+  contract(1, 0, 0, a0xu, a0x, w0);
+  contract(2, 0, 0, a0xuu, a0x, w0);
+  micro_slope(a0y, c0y, p0, g);
+  contract(0, 1, 0, a0yv, a0y, w0);
+  contract(1, 1, 0, a0yuv, a0y, w0);
+#pragma unroll
+  for (int i = 0; i < 4; i++) derA0[i] = -p0[0] * (a0xu[i] + a0yv[i]);
+  micro_slope(A0, derA0, p0, g);
+  contract(1, 0, 0, A0u, A0, w0);
+  const double tau = tau_ns(p0[0], p0[3], g);
+#pragma unroll
+  for (int i = 0; i < 4; i++) {
+    f[i] = p0[0] * dt * (g0u[i] - tau * ((a0xuu[i] + a0yuv[i]) + A0u[i]));
+    derf[i] = p0[0] * dt * dt * A0u[i];
+  }
+  if (g.prandtl_fix) {  // the heat-flux correction of GKS2D, integrated the same way
+    const double qs = a0xuu[3] + A0u[3] - p0[1] * (a0xuu[1] + A0u[1]) - p0[2] * (a0xuu[2] + A0u[2]);
+    f[3] += (1.0 / g.pr - 1.0) * (-p0[0] * tau * dt * qs);
+  }
+}
+
 // smoothness indicators and AO weights (reconstruction.cpp:480-511, 2742-2776)
 GKS_DI void weno5_weights(double* fw, double wn2, double wn1, double w0, double wp1, double wp2) {
   const double dhi = 0.85, dlo = 0.85;

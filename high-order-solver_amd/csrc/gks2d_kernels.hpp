@@ -791,7 +791,7 @@ template <int DIR, int ORDER> __global__ void __launch_bounds__(kBlock) k_tangen
 // share, the raw (f, derf) when kept, and Calculate_alpha_k
 template <int DIR, class V>
 __device__ __forceinline__ void flux_point_tail(const Dev& d, int q, long long f, long long nf, const V& L,
-                                                const V& R, int stage, double dt) {
+                                                const V& R, int stage, double dt, bool smooth = false) {
   // alpha_k first: only the four conservative values of each side stay
   // live into it, and none of them into the flux below
   {
@@ -814,7 +814,8 @@ __device__ __forceinline__ void flux_point_tail(const Dev& d, int q, long long f
     (DIR == 0 ? d.akx : d.aky)[q * nf + f] = a;
   }
   double fo[4], dfo[4];
-  gks_point(fo, dfo, nullptr, L, R, dt, d.gas);
+  if (smooth) gks_point_smooth(fo, dfo, L, R, dt, d.gas);
+  else gks_point(fo, dfo, nullptr, L, R, dt, d.gas);
   if (d.keep_face_fluxes) {
     double* fl = DIR == 0 ? d.flx[stage] : d.fly[stage];
 #pragma unroll
@@ -884,6 +885,7 @@ constexpr int kTanFlux = 128;
 template <int DIR, bool LEARNED, int TF>
 __device__ __forceinline__ void tanflux_body(const Dev& d, unsigned blk, int stage, double dt) {
   __shared__ double sg[2][2][12][TF];  // [side][gauss point][component][face]
+  __shared__ unsigned char slin[2][LEARNED ? TF : 1];  // side took the linear weights
   const int ni = DIR == 0 ? d.NX + 1 : d.NX, nj = DIR == 0 ? d.NY : d.NY + 1;
   const long long nfaces = (long long)ni * nj;
   const int tl = threadIdx.x % TF, hi = threadIdx.x / TF;
@@ -936,6 +938,7 @@ __device__ __forceinline__ void tanflux_body(const Dev& d, unsigned blk, int sta
       }
       lin = smooth != 0 && amax == 0.0;
       count_linear(d.lstat + 2 * kStatSlots, lin);
+      slin[side][tl] = lin ? 1 : 0;
     }
   }
   if (valid) {
@@ -973,7 +976,9 @@ __device__ __forceinline__ void tanflux_body(const Dev& d, unsigned blk, int sta
   if (valid) {
     const int q = hi;
     const Strided L{&sg[0][q][0][tl], TF}, R{&sg[1][q][0][tl], TF};
-    flux_point_tail<DIR>(d, q, f0, nf, L, R, stage, dt);
+    // both sides smooth: the smooth-flow flux (learned path)
+    const bool smooth = LEARNED && d.smooth_flux && (slin[0][tl] & slin[1][tl]);
+    flux_point_tail<DIR>(d, q, f0, nf, L, R, stage, dt, smooth);
   }
 }
 template <int DIR, bool LEARNED>
@@ -1038,8 +1043,9 @@ __device__ __forceinline__ void alpha_body(const Dev& d, unsigned blk) {
 __global__ void __launch_bounds__(kBlock) k_alpha(Dev d) { alpha_body(d, blockIdx.x); }
 
 // leaf parity: centre + GKS2D over n independent points
+// (smooth: the smooth-flow flux instead, no centre output)
 __global__ void __launch_bounds__(kBlock) k_flux_points(Gas g, long long n, const double* L, const double* R,
-                                                         double dt, double* C, double* F, double* D) {
+                                                         double dt, double* C, double* F, double* D, int smooth) {
   long long t = gtid();
   if (t >= n) return;
   double l[12], r[12], c[12], f[4], df[4];
@@ -1048,9 +1054,13 @@ __global__ void __launch_bounds__(kBlock) k_flux_points(Gas g, long long n, cons
     l[k] = L[t * 12 + k];
     r[k] = R[t * 12 + k];
   }
-  gks_point(f, df, c, l, r, dt, g);
+  if (smooth) {
+    gks_point_smooth(f, df, l, r, dt, g);
+  } else {
+    gks_point(f, df, c, l, r, dt, g);
 #pragma unroll
-  for (int k = 0; k < 12; k++) C[t * 12 + k] = c[k];
+    for (int k = 0; k < 12; k++) C[t * 12 + k] = c[k];
+  }
 #pragma unroll
   for (int k = 0; k < 4; k++) {
     F[t * 4 + k] = f[k];
@@ -1297,8 +1307,8 @@ static hipError_t alpha(const Dev& d, hipStream_t s) {
   return hipGetLastError();
 }
 static hipError_t flux_points(const Gas& g, long long n, const double* L, const double* R, double dt, double* C,
-                              double* F, double* D, hipStream_t s) {
-  hipLaunchKernelGGL(k_flux_points, dim3(blocks_for(n)), dim3(kBlock), 0, s, g, n, L, R, dt, C, F, D);
+                              double* F, double* D, int smooth, hipStream_t s) {
+  hipLaunchKernelGGL(k_flux_points, dim3(blocks_for(n)), dim3(kBlock), 0, s, g, n, L, R, dt, C, F, D, smooth);
   return hipGetLastError();
 }
 

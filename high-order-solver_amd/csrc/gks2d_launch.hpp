@@ -36,6 +36,7 @@ struct Dev {
   float mlp_thr;                // logit below which a smooth stencil takes the linear weights
   unsigned long long* lstat;    // 2 x 1024 (linear, total) slot pairs: normal, then tangential
   unsigned char* lin[2];        // per cell: its x / y normal stencil took the linear weights
+  int smooth_flux;              // learned path: faces with two linear sides take the smooth-flow flux
   double* partial;  // per-block (max speed, blow-up flag) pairs
   double* stat;     // [0..3]: max speed of slot 0 / 1 (+pad); [4]: sticky blow-up flag
 };
@@ -95,7 +96,7 @@ struct LaunchTable {
   hipError_t (*update)(const Dev&, int stage, hipStream_t);
   hipError_t (*alpha)(const Dev&, hipStream_t);
   hipError_t (*flux_points)(const gksd::Gas&, long long n, const double* L, const double* R, double dt,
-                            double* C, double* F, double* D, hipStream_t);
+                            double* C, double* F, double* D, int smooth, hipStream_t);
   // classifier training samples of the cells reconstructed along dir
   hipError_t (*label)(const Dev&, int dir, int stride, float* X, float* Y, hipStream_t);
   // logits of n feature rows through the solver's MFMA inference

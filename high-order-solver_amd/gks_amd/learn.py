@@ -42,6 +42,8 @@ _SIGS = {
     "gks2d_set_classifier": (ctypes.c_int, [_P, _F, ctypes.c_float]),
     "gks2d_classifier_stats": (ctypes.c_int, [_P, ctypes.POINTER(_I64), ctypes.POINTER(_I64)]),
     "gks_mlp_last_error": (ctypes.c_char_p, []),
+    "gks2d_classifier_options": (ctypes.c_int, [_P, ctypes.c_int32]),
+    "gks2d_flux_points_smooth": (ctypes.c_int, [ctypes.c_void_p, _I64, _D, _D, ctypes.c_double, _D, _D]),
 }
 _bound = False
 
@@ -186,6 +188,22 @@ def load(path=DEFAULT_PARAMS):
 def set_classifier(solver, P, threshold):
     P = None if P is None else np.ascontiguousarray(P, dtype=np.float32)
     _check_ctx(L().gks2d_set_classifier(solver.h, _fp(P), float(threshold)))
+
+
+def classifier_options(solver, smooth_flux=True):
+    _check_ctx(L().gks2d_classifier_options(solver.h, 1 if smooth_flux else 0))
+
+
+def flux_points_smooth(p, left, right, dt):
+    """Smooth-flow flux of n Gauss points on the GPU (n x 12 inputs)."""
+    left = np.ascontiguousarray(left, dtype=np.float64)
+    right = np.ascontiguousarray(right, dtype=np.float64)
+    n = left.shape[0]
+    f = np.zeros((n, 4))
+    d = np.zeros((n, 4))
+    _check_ctx(L().gks2d_flux_points_smooth(ctypes.byref(p), n, left.ctypes.data_as(_D), right.ctypes.data_as(_D),
+                                            dt, f.ctypes.data_as(_D), d.ctypes.data_as(_D)))
+    return f, d
 
 
 def classifier_stats(solver):

@@ -56,9 +56,26 @@ int gks_mlp_train(float* params, int64_t n, const float* X, const float* y, cons
 int gks_mlp_infer_bf16(const float* params, int64_t n, const float* X, float* logits);
 
 /* Enables the learned reconstruction in a 2D context (WENO5 only): a
- * stencil takes the linear weights where logit < threshold and the DF
- * sensor is zero on it.  params == NULL disables it. */
+ * normal stencil takes the linear weights where logit < threshold and the
+ * DF sensor is zero on it; a tangential stencil takes them where the normal
+ * stencils of the cells its line values come from (and the centre cell's
+ * stencil across them) all did and its DF sensor is zero; a Gauss point
+ * whose two sides both took them gets the smooth-flow GKS flux (see
+ * gks2d_flux_points_smooth; gks2d_classifier_options turns that off).
+ * params == NULL disables the classifier. */
 int gks2d_set_classifier(gks2d_ctx* ctx, const float* params, float logit_threshold);
+
+/* smooth_flux: 1 (default) or 0 (every Gauss point keeps GKS2D) */
+int gks2d_classifier_options(gks2d_ctx* ctx, int32_t smooth_flux);
+
+/* The smooth-flow flux of n Gauss points (layout of gks2d_flux_points):
+ * interface state and slopes = the averages of the two sides, distribution
+ * g0 [1 - tau (a u + b v + A) + t A], f = rho0 dt [<u psi g0> - tau
+ * (<u psi g0 (a u + b v)> + <u psi g0 A>)], derf = rho0 dt^2 <u psi g0 A>:
+ * what GKS2D gives when the two sides are equal, without its half-space
+ * (erfc, exp) moments. */
+int gks2d_flux_points_smooth(const gks2d_params* p, int64_t n, const double* left, const double* right, double dt,
+                             double* f, double* derf);
 
 /* Reconstructions that took the linear weights / all reconstructions, over
  * the calls since the classifier was set (normal and tangential). */

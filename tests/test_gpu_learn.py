@@ -208,3 +208,25 @@ def test_trained_classifier_keeps_the_l1_error():
     say("   L1 plain", res[False], "learned", res[True], "linear share", res["share"])
     assert np.all(res[True] <= 1.05 * res[False])
     assert res["share"] > 0.5
+
+
+@pytest.mark.parametrize("fast", [0, 1])
+def test_smooth_flux_leaf_matches_oracle(fast):
+    """GPU smooth-flow flux against the numpy restatement (pinned to GKS2D in
+    test_smooth_flux.py), and against the GPU's own GKS2D when both sides
+    agree."""
+    import smooth_flux as sf
+    from test_smooth_flux import states
+    p = planar_shock_params(16, fast_math=fast).copy(tau_type=1, mu=2e-3, prandtl_fix=1, pr=0.72)
+    L, R = states(3000, 5), states(3000, 6)
+    R = L + 0.05 * (R - L)
+    dt = 1e-3
+    f, d = learn.flux_points_smooth(p, L, R, dt)
+    fo, do = sf.smooth_flux(L, R, dt, K=p.K, tau_type=p.tau_type, mu=p.mu, nu=p.nu, pr=p.pr, prandtl_fix=1)
+    ef = np.max(np.abs(f - fo) / np.abs(fo).max(axis=1, keepdims=True))
+    ed = np.max(np.abs(d - do) / np.abs(do).max(axis=1, keepdims=True))
+    _, fg, dg = gks_amd.flux_points(p, L, L, dt)
+    fe, de = learn.flux_points_smooth(p, L, L, dt)
+    eg = np.max(np.abs(fe - fg) / np.abs(fg).max(axis=1, keepdims=True))
+    say(f"[learn] smooth flux, fast {fast}: vs numpy f {ef:.2e} derf {ed:.2e}; vs GKS2D at equal sides {eg:.2e}")
+    assert ef < 1e-12 and ed < 1e-10 and eg < 1e-12
