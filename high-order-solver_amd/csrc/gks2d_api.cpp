@@ -552,10 +552,11 @@ int gks2d_flux_points_smooth(const gks2d_params* p, int64_t n, const double* lef
 // ------------------------------------------------- learned classifier hooks
 // 2 kernels (normal, tangential) x 1024 slots x (linear, total) counters
 constexpr size_t kLstatWords = 2 * 1024 * 2;
-int gks2d_classifier_samples(gks2d_ctx* c, int32_t dir, int32_t stride, float* features, float* labels, int64_t cap,
-                             int64_t* n_out) {
+int gks2d_classifier_samples(gks2d_ctx* c, int32_t dir, int32_t stride, double label_tol, float* features,
+                             float* labels, int64_t cap, int64_t* n_out) {
   if (!c || !features || !labels || !n_out) return fail(GKS_ERR_ARG, "null");
   if (dir < 0 || dir > 1 || stride < 1) return fail(GKS_ERR_ARG, "dir / stride");
+  if (!(label_tol > 0.0)) return fail(GKS_ERR_ARG, "label_tol must be > 0");
   if (c->p.weno_order != 5) return fail(GKS_ERR_UNSUPPORTED, "the classifier is for WENO5-AO");
   const Dev& d = c->d;
   const long long total = dir == 0 ? (long long)(d.NX + 2) * d.ny : (long long)d.nx * (d.NY + 2);
@@ -564,7 +565,7 @@ int gks2d_classifier_samples(gks2d_ctx* c, int32_t dir, int32_t stride, float* f
   float *dX = nullptr, *dY = nullptr;
   hipError_t e = hipMalloc(&dX, ns * 16 * sizeof(float));
   if (e == hipSuccess) e = hipMalloc(&dY, ns * sizeof(float));
-  if (e == hipSuccess) e = c->lt->label(d, dir, stride, dX, dY, c->s);
+  if (e == hipSuccess) e = c->lt->label(d, dir, stride, label_tol, dX, dY, c->s);
   if (e == hipSuccess) e = hipStreamSynchronize(c->s);
   if (e == hipSuccess) e = hipMemcpy(features, dX, ns * 16 * sizeof(float), hipMemcpyDeviceToHost);
   if (e == hipSuccess) e = hipMemcpy(labels, dY, ns * sizeof(float), hipMemcpyDeviceToHost);

@@ -427,7 +427,8 @@ __device__ __forceinline__ void normal_stencil(const Dev& d, int i, int j, doubl
 }
 
 // classifier training samples: features and label of every stride-th cell
-template <int DIR> __global__ void __launch_bounds__(kBlock) k_label(Dev d, int stride, float* X, float* Y) {
+template <int DIR>
+__global__ void __launch_bounds__(kBlock) k_label(Dev d, int stride, double tol, float* X, float* Y) {
   const long long t = gtid();
   const long long ns = (normal_count<DIR>(d) + stride - 1) / stride;
   if (t >= ns) return;
@@ -441,7 +442,7 @@ template <int DIR> __global__ void __launch_bounds__(kBlock) k_label(Dev d, int 
   stencil_features(x, q);
 #pragma unroll
   for (int k = 0; k < 16; k++) X[t * 16 + k] = x[k];
-  Y[t] = normal_label(w, a, DIR == 0 ? d.dx : d.dy, DIR == 0 ? d.idx : d.idy, d.recon_var, d.gas);
+  Y[t] = normal_label(w, a, DIR == 0 ? d.dx : d.dy, DIR == 0 ? d.idx : d.idy, d.recon_var, d.gas, tol);
 }
 
 // counts of linear / all reconstructions of a wave, added by its first
@@ -1219,8 +1220,8 @@ template __global__ void kb_tanflux<0, true, 128>(BatchArgs, int);
 template __global__ void kb_tanflux<1, true, 128>(BatchArgs, int);
 template __global__ void k_normal_learned<0>(Dev);
 template __global__ void k_normal_learned<1>(Dev);
-template __global__ void k_label<0>(Dev, int, float*, float*);
-template __global__ void k_label<1>(Dev, int, float*, float*);
+template __global__ void k_label<0>(Dev, int, double, float*, float*);
+template __global__ void k_label<1>(Dev, int, double, float*, float*);
 template __global__ void k_flux<0>(Dev, int, double);
 template __global__ void k_flux<1>(Dev, int, double);
 
@@ -1312,11 +1313,11 @@ static hipError_t flux_points(const Gas& g, long long n, const double* L, const 
   return hipGetLastError();
 }
 
-static hipError_t label(const Dev& d, int dir, int stride, float* X, float* Y, hipStream_t s) {
+static hipError_t label(const Dev& d, int dir, int stride, double tol, float* X, float* Y, hipStream_t s) {
   const long long total = dir == 0 ? (long long)(d.NX + 2) * d.ny : (long long)d.nx * (d.NY + 2);
   const long long ns = (total + stride - 1) / stride;
-  if (dir == 0) hipLaunchKernelGGL(k_label<0>, dim3(blocks_for(ns)), dim3(kBlock), 0, s, d, stride, X, Y);
-  else hipLaunchKernelGGL(k_label<1>, dim3(blocks_for(ns)), dim3(kBlock), 0, s, d, stride, X, Y);
+  if (dir == 0) hipLaunchKernelGGL(k_label<0>, dim3(blocks_for(ns)), dim3(kBlock), 0, s, d, stride, tol, X, Y);
+  else hipLaunchKernelGGL(k_label<1>, dim3(blocks_for(ns)), dim3(kBlock), 0, s, d, stride, tol, X, Y);
   return hipGetLastError();
 }
 static hipError_t infer(const float* P, const unsigned short* W, long long n, const float* X, float* out,

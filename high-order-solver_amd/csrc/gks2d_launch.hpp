@@ -98,7 +98,7 @@ struct LaunchTable {
   hipError_t (*flux_points)(const gksd::Gas&, long long n, const double* L, const double* R, double dt,
                             double* C, double* F, double* D, int smooth, hipStream_t);
   // classifier training samples of the cells reconstructed along dir
-  hipError_t (*label)(const Dev&, int dir, int stride, float* X, float* Y, hipStream_t);
+  hipError_t (*label)(const Dev&, int dir, int stride, double tol, float* X, float* Y, hipStream_t);
   // logits of n feature rows through the solver's MFMA inference
   hipError_t (*infer)(const float* P, const unsigned short* W, long long n, const float* X, float* out,
                         hipStream_t);

@@ -105,10 +105,10 @@ GKS_DI void weno5_linear_2gauss(double& g1, double& g1x, double& g2, double& g2x
 // training label of one cell's normal reconstruction: 1 when the DF sensor
 // is nonzero on the stencil, or when for some characteristic (or
 // conservative) component and face the WENO5-AO-DF value or derivative
-// differs from the linear one by more than 1e-3 of that component's range
+// differs from the linear one by more than tol times that component's range
 // over the stencil
 GKS_DI float normal_label(const double (*w)[4], const double* alpha, double h, double ih, int recon_var,
-                          const Gas& g) {
+                          const Gas& g, double rel_tol) {
   double amax = 0.0;
 #pragma unroll
   for (int s = 0; s < 5; s++) amax = fmax(amax, alpha[s]);
@@ -148,7 +148,7 @@ GKS_DI float normal_label(const double (*w)[4], const double* alpha, double h, d
         mx = fmax(mx, c[s][i]);
         mn = fmin(mn, c[s][i]);
       }
-      const double tol = 1e-3 * (mx - mn) + 1e-13 * fabs(c[2][i]);
+      const double tol = rel_tol * (mx - mn) + 1e-13 * fabs(c[2][i]);
       if (fabs(v - vl) > tol || fabs(d1 - dl) * h > tol) bad = true;
     }
   }

@@ -106,3 +106,53 @@ def run_vortex(n, tstop=10.0, weno_order=7, fast_math=1, max_steps=2**31 - 1, st
     s.close()
     return dict(params=p, convar=u, steps=steps, t=t, wall_s=wall, mcells=n * n * steps / wall / 1e6,
                 l1=vortex_l1(u, p, t))
+
+
+def _run_chunks(s, bc, tstop, chunk, log, tag):
+    total = 0
+    t0 = time.perf_counter()
+    while s.t < tstop:
+        n = s.run(bc, tstop, chunk)
+        total += n
+        if log:
+            log(f"{tag}: t={s.t:.4f} steps={total} wall={time.perf_counter() - t0:.1f}s")
+        if n == 0:
+            break
+    s.sync()
+    return total, time.perf_counter() - t0
+
+
+def equal_l1(n, seed=0, tstop=0.25, classifier=None, smooth_flux=True, fast_math=1, chunk=500, log=None):
+    """The learned-versus-plain accuracy check of the four-quadrant Riemann
+    workload: density L1 of the plain and the learned run at n^2 against a
+    plain run at (2n)^2 averaged over 2x2 blocks, all to tstop; plus each
+    run's steps and wall time.  classifier: (params, threshold); default the
+    packaged one."""
+    from . import learn
+    P, thr = classifier if classifier is not None else learn.load()[:2]
+    bc = [BC_FREE] * 4
+    G = 5
+    out = {}
+    finals = {}
+    for tag, m, use in (("plain", n, False), ("learned", n, True), ("reference_2n", 2 * n, False)):
+        p = planar_shock_params(m, fast_math=fast_math)
+        s = Solver2D(p)
+        s.init_prim(riemann_prim(p, riemann_states(seed)))
+        if use:
+            learn.set_classifier(s, P, thr)
+            learn.classifier_options(s, smooth_flux)
+        steps, wall = _run_chunks(s, bc, tstop, chunk, log, tag)
+        finals[tag] = s.get_convar()[:, G:G + m, G:G + m]
+        out[tag] = {"steps": steps, "wall_s": round(wall, 3), "t": s.t,
+                    "mcell_updates_per_s": round(m * m * steps / wall / 1e6, 3)}
+        if use:
+            lin, tot = learn.classifier_stats(s)
+            out[tag]["linear_share"] = round(lin / max(tot, 1), 4)
+        s.close()
+    ref = finals["reference_2n"].reshape(4, n, 2, n, 2).mean(axis=(2, 4))
+    for tag in ("plain", "learned"):
+        e = np.abs(finals[tag] - ref).mean(axis=(1, 2))
+        out[tag]["l1"] = [float(x) for x in e]
+    out["l1_ratio_density"] = out["learned"]["l1"][0] / out["plain"]["l1"][0]
+    out["speedup"] = out["learned"]["mcell_updates_per_s"] / out["plain"]["mcell_updates_per_s"]
+    return out

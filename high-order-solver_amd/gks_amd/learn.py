@@ -33,7 +33,7 @@ _D = ctypes.POINTER(ctypes.c_double)
 _P = ctypes.c_void_p
 _I64 = ctypes.c_int64
 _SIGS = {
-    "gks2d_classifier_samples": (ctypes.c_int, [_P, ctypes.c_int32, ctypes.c_int32, _F, _F, _I64,
+    "gks2d_classifier_samples": (ctypes.c_int, [_P, ctypes.c_int32, ctypes.c_int32, ctypes.c_double, _F, _F, _I64,
                                                 ctypes.POINTER(_I64)]),
     "gks_mlp_loss_grad": (ctypes.c_int, [_F, _I64, _F, _F, _F, _F, _D]),
     "gks_mlp_train": (ctypes.c_int, [_F, _I64, _F, _F, _F, ctypes.c_int32, ctypes.c_int32, ctypes.c_float,
@@ -84,7 +84,7 @@ def init_params(seed=0):
     return P
 
 
-def samples(solver, direction, stride):
+def samples(solver, direction, stride, label_tol=1e-3):
     """(features [n, 16], labels [n]) of the context's current state."""
     d = solver.p
     total = (d.nodex + 2) * d.ny if direction == 0 else d.nx * (d.nodey + 2)
@@ -92,11 +92,12 @@ def samples(solver, direction, stride):
     X = np.zeros((cap, 16), dtype=np.float32)
     y = np.zeros(cap, dtype=np.float32)
     n = _I64()
-    _check_ctx(L().gks2d_classifier_samples(solver.h, direction, stride, _fp(X), _fp(y), cap, ctypes.byref(n)))
+    _check_ctx(L().gks2d_classifier_samples(solver.h, direction, stride, label_tol, _fp(X), _fp(y), cap,
+                                            ctypes.byref(n)))
     return X[:n.value], y[:n.value]
 
 
-def collect(n=512, seeds=range(32), times=(0.06, 0.14, 0.25), stride=13, fast_math=1, log=None):
+def collect(n=512, seeds=range(32), times=(0.06, 0.14, 0.25), stride=13, fast_math=1, label_tol=1e-3, log=None):
     """Training set from seeded four-quadrant Riemann runs on the GPU."""
     Xs, ys = [], []
     for seed in seeds:
@@ -113,7 +114,7 @@ def collect(n=512, seeds=range(32), times=(0.06, 0.14, 0.25), stride=13, fast_ma
             for side in range(4):
                 s.boundary(side, BC_FREE)
             for direction in (0, 1):
-                X, y = samples(s, direction, stride)
+                X, y = samples(s, direction, stride, label_tol)
                 Xs.append(X)
                 ys.append(y)
         s.close()
@@ -215,7 +216,7 @@ def classifier_stats(solver):
 def _train_main(args):
     log = lambda *a: print(*a, flush=True)
     t0 = time.time()
-    X, y = collect(args.n, range(args.seeds), stride=args.stride, log=log)
+    X, y = collect(args.n, range(100, 100 + args.seeds), stride=args.stride, label_tol=args.label_tol, log=log)
     log(f"collected {len(y)} samples, troubled fraction {y.mean():.4f}, {time.time() - t0:.1f} s")
     rng = np.random.default_rng(1)
     perm = rng.permutation(len(y))
@@ -231,7 +232,8 @@ def _train_main(args):
     lin = zv < thr
     log(f"threshold {thr:.4f}: validation smooth share taking linear weights "
         f"{lin[yv < 0.5].mean():.4f}, troubled taking them {lin[yv > 0.5].mean():.5f}")
-    meta = dict(n=args.n, seeds=args.seeds, stride=args.stride, samples=int(len(y)), train=int(len(yt)),
+    meta = dict(n=args.n, seeds=args.seeds, stride=args.stride, label_tol=args.label_tol, samples=int(len(y)),
+                train=int(len(yt)),
                 epochs=args.epochs, batch=args.batch, lr=args.lr, max_miss=args.max_miss,
                 loss=[float(v) for v in hist], val_linear_smooth=float(lin[yv < 0.5].mean()),
                 val_linear_troubled=float(lin[yv > 0.5].mean()))
@@ -253,6 +255,9 @@ def main(argv=None):
     t.add_argument("--lr", type=float, default=2e-3)
     t.add_argument("--seed", type=int, default=0)
     t.add_argument("--max-miss", type=float, default=0.002)
+    t.add_argument("--label-tol", type=float, default=1e-3,
+                   help="a stencil is troubled where WENO-AO departs from the linear weights by more than this "
+                        "fraction of the component's stencil range")
     args = ap.parse_args(argv)
     if args.cmd == "train":
         _train_main(args)

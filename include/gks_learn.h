@@ -34,10 +34,11 @@ extern "C" {
  * reconstruction cell along `dir` (0: x, 1: y; the cells k_normal
  * reconstructs), the 16 features and the label (1: the WENO5-AO-DF face
  * values or derivatives differ from the linear-weight ones by more than
- * 1e-3 of the component's stencil range, or the DF sensor is nonzero on the
- * stencil; 0: otherwise).  Writes at most `cap` samples; *n_out = count. */
-int gks2d_classifier_samples(gks2d_ctx* ctx, int32_t dir, int32_t stride, float* features, float* labels,
-                             int64_t cap, int64_t* n_out);
+ * label_tol times the component's stencil range, or the DF sensor is
+ * nonzero on the stencil; 0: otherwise).  Writes at most `cap` samples;
+ * *n_out = count. */
+int gks2d_classifier_samples(gks2d_ctx* ctx, int32_t dir, int32_t stride, double label_tol, float* features,
+                             float* labels, int64_t cap, int64_t* n_out);
 
 /* Mean weighted binary cross-entropy of the logits and its gradient with
  * respect to the parameters, on the GPU in FP32 (n samples, host arrays). */

@@ -120,10 +120,12 @@ def test_tile_shapes_give_the_same_states(shape):
 
 
 def test_batch_with_the_classifier_equals_single_learned_runs():
+    """IEEE build (the batched and the single-problem learned paths launch
+    different kernels; only the IEEE build rounds alike in any kernel)."""
     rng = np.random.default_rng(3)
     P = rng.normal(0, 0.4, learn.NPARAM).astype(np.float32)
     thr = 0.0
-    params = [planar_shock_params(n, fast_math=1) for n in SIZES]
+    params = [planar_shock_params(n, fast_math=0) for n in SIZES]
     b = Batch(params)
     for k, s in enumerate(b.problems):
         setup(s, 40 + k)

@@ -104,7 +104,7 @@ def test_samples_match_host_features():
            gm1 * (u[3] - 0.5 * u[2] * u[2] / u[0] - 0.5 * (-u[1]) * (-u[1]) / u[0])]
     say("[learn] solver samples vs host features")
     for direction in (0, 1):
-        X, y = learn.samples(s, direction, 1)
+        X, y = learn.samples(s, direction, 1, 1e-3)
         pr = prs[direction]
         if direction == 0:
             i = np.arange(G - 1, n + G + 1)[:, None] * np.ones((1, p.ny), int)
@@ -186,7 +186,7 @@ def test_trained_classifier_keeps_the_l1_error():
     density L1 error against a 1024^2 plain run (averaged to 512^2) is within
     5% of the plain run's, and most stencils take the linear weights."""
     P, thr, meta = learn.load()
-    seed, n = 101, 512
+    seed, n = 5, 512  # training used seeds 100-131
     say(f"[learn] trained classifier, seed {seed}, {n}^2 vs 1024^2 plain, threshold {thr:.3f}")
     from gks_amd.drivers import run_riemann
     ref = run_riemann(2 * n, seed)["convar"][:, G:-G, G:-G]
