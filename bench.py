@@ -37,7 +37,7 @@ PEAK_FP64_TFLOPS = 78.6  # MI355X FP64 (vector and matrix)
 # rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE (profiles/gks2d_riemann2048_fast_hbm_traffic.md),
 # each scaled by the counter's reading for k_copy_old (4 x 2048^2 doubles read
 # and written): fetch x1.882, write x0.985
-TRAFFIC_2048_FAST = {"flux_x": 1.2409e9, "flux_y": 1.2289e9}
+TRAFFIC_2048_FAST = {"flux_x": 1.9280e9, "flux_y": 1.9668e9}
 PEAK_HBM_GBS = 8000.0
 
 
