@@ -4,6 +4,7 @@ The presets restate the settings the reference's problem drivers assign to
 its globals (e.g. ``PlanarShock``, reference ``src/2Dproblem.cpp:8-53``).
 """
 import ctypes
+import math
 
 BC_FREE, BC_PERIODIC, BC_REFLECT = 0, 1, 2
 SIDE_LEFT, SIDE_RIGHT, SIDE_DOWN, SIDE_UP = 0, 1, 2, 3
@@ -138,6 +139,79 @@ def vortex_params(n, weno_order=7, cfl=0.5, **kw):
     variables, S2O4, CFL 0.5."""
     p = planar_shock_params(n, weno_order=weno_order, cfl=cfl, c1_euler=0.0, c2_euler=0.0,
                             left=0.0, right=10.0, down=0.0, up=10.0)
+    for k, v in kw.items():
+        setattr(p, k, v)
+    return p
+
+
+class Params3D(ctypes.Structure):
+    """ctypes mirror of ``gks3d_params`` (include/gks3d.h)."""
+    _fields_ = [
+        ("nodex", ctypes.c_int32), ("nodey", ctypes.c_int32), ("nodez", ctypes.c_int32),
+        ("ghost", ctypes.c_int32), ("recon_variable", ctypes.c_int32), ("tau_type", ctypes.c_int32),
+        ("K", ctypes.c_int32), ("fast_math", ctypes.c_int32),
+        ("gamma", ctypes.c_double), ("mu", ctypes.c_double), ("nu", ctypes.c_double),
+        ("c1_euler", ctypes.c_double), ("c2_euler", ctypes.c_double),
+        ("left", ctypes.c_double), ("right", ctypes.c_double), ("down", ctypes.c_double),
+        ("up", ctypes.c_double), ("back", ctypes.c_double), ("front", ctypes.c_double),
+        ("cfl", ctypes.c_double),
+    ]
+
+    @property
+    def nx(self):
+        return self.nodex + 2 * self.ghost
+
+    @property
+    def ny(self):
+        return self.nodey + 2 * self.ghost
+
+    @property
+    def nz(self):
+        return self.nodez + 2 * self.ghost
+
+    @property
+    def dx(self):
+        return (self.right - self.left) / self.nodex
+
+    @property
+    def dy(self):
+        return (self.up - self.down) / self.nodey
+
+    @property
+    def dz(self):
+        return (self.front - self.back) / self.nodez
+
+    def copy(self, **kw):
+        p = Params3D()
+        ctypes.pointer(p)[0] = self
+        for k, v in kw.items():
+            setattr(p, k, v)
+        return p
+
+
+def cubic_params(n, **kw):
+    """Settings of CubicTube (reference src/3Dproblem.cpp:3-108): K=2,
+    gamma=1.4, Euler tau c1=0.05 c2=1, WENO5-AO (wenoz) in characteristic
+    variables, 3 ghost layers, 2x2 Gauss points, GKS3D gks2nd, S2O4, CFL 0.5
+    on [0,1]^3."""
+    p = Params3D(nodex=n, nodey=n, nodez=n, ghost=3, recon_variable=VAR_CHARACTERISTIC, tau_type=TAU_EULER,
+                 K=2, fast_math=0, gamma=1.4, mu=-1.0, nu=-1.0, c1_euler=0.05, c2_euler=1.0,
+                 left=0.0, right=1.0, down=0.0, up=1.0, back=0.0, front=1.0, cfl=0.5)
+    for k, v in kw.items():
+        setattr(p, k, v)
+    return p
+
+
+def tgv_params(n, re=1600.0, **kw):
+    """Viscous Taylor-Green vortex on the periodic cube [0, 2 pi]^3 (a
+    workload of this build; the reference has no 3D viscous driver): rho0=1,
+    U0=1, L=1, gamma=1.4 (K=2), Navier-Stokes collision time with
+    mu = rho0 U0 L / Re (Get_Tau_NS, basic_function.cpp:638-661; the
+    numerical part c2 |jump| dt vanishes on smooth data, c1=0), WENO5-AO in
+    characteristic variables, 3 ghost layers, CFL 0.5."""
+    two_pi = 2.0 * math.pi
+    p = cubic_params(n, tau_type=TAU_NS, mu=1.0 / re, c1_euler=0.0, c2_euler=1.0, left=0.0, right=two_pi,
+                     down=0.0, up=two_pi, back=0.0, front=two_pi)
     for k, v in kw.items():
         setattr(p, k, v)
     return p

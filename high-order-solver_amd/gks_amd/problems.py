@@ -207,3 +207,73 @@ def convar_to_prim(c, gamma):
     """Convar_to_primvar_2D (reference src/basic_function.cpp:811-817)."""
     r, mu, mv, E = c
     return np.stack([r, mu / r, mv / r, (gamma - 1.0) * (E - 0.5 * mu * mu / r - 0.5 * mv * mv / r)])
+
+
+# --------------------------------------------------------------------- 3D
+def prim_to_convar_3d(prim, gamma):
+    """Primvar_to_convar_3D (basic_function.cpp:859-866) on [5, ...] arrays."""
+    r, u, v, w, p = prim
+    return np.stack([r, r * u, r * v, r * w, r * (p / r / (gamma - 1) + 0.5 * (u * u + v * v + w * w))])
+
+
+def convar_to_prim_3d(c, gamma):
+    r = c[0]
+    u, v, w = c[1] / r, c[2] / r, c[3] / r
+    p = (gamma - 1.0) * (c[4] - 0.5 * c[1] * c[1] / r - 0.5 * c[2] * c[2] / r - 0.5 * c[3] * c[3] / r)
+    return np.stack([r, u, v, w, p])
+
+
+def cubic_sod_prim(p, inner=(1.0, 0.0, 0.0, 0.0, 1.0), outer=(0.125, 0.0, 0.0, 0.0, 0.1)):
+    """ICfor_cubic_sod (reference src/3Dproblem.cpp:174-223): the cube of
+    cells whose centres lie within 0.2 of the domain centre in every
+    direction holds `inner`, the rest `outer`; ghost cells copy the nearest
+    interior cell (they are overwritten by the first boundary fill)."""
+    G = p.ghost
+    xs = p.left + (np.arange(p.nx) + 0.5 - G) * p.dx
+    ys = p.down + (np.arange(p.ny) + 0.5 - G) * p.dy
+    zs = p.back + (np.arange(p.nz) + 0.5 - G) * p.dz
+    X, Y, Z = np.meshgrid(xs, ys, zs, indexing="ij")
+    box = ((np.abs(X - 0.5 * (p.right + p.left)) < 0.2 * (p.right - p.left)) &
+           (np.abs(Y - 0.5 * (p.up + p.down)) < 0.2 * (p.up - p.down)) &
+           (np.abs(Z - 0.5 * (p.front + p.back)) < 0.2 * (p.front - p.back)))
+    prim = np.empty((5, p.nx, p.ny, p.nz))
+    for v in range(5):
+        prim[v] = np.where(box, inner[v], outer[v])
+    return prim
+
+
+def tgv_prim_point(x, y, z, mach=0.1, rho0=1.0, u0=1.0, gamma=1.4):
+    """Taylor-Green vortex at t=0: u = U0 sin x cos y cos z, v = -U0 cos x
+    sin y cos z, w = 0, p = p0 + rho0 U0^2 / 16 (cos 2x + cos 2y)(cos 2z + 2)
+    with p0 = rho0 U0^2 / (gamma Ma^2), rho = rho0."""
+    p0 = rho0 * u0 * u0 / (gamma * mach * mach)
+    u = u0 * np.sin(x) * np.cos(y) * np.cos(z)
+    v = -u0 * np.cos(x) * np.sin(y) * np.cos(z)
+    w = np.zeros_like(u)
+    pr = p0 + rho0 * u0 * u0 / 16.0 * (np.cos(2 * x) + np.cos(2 * y)) * (np.cos(2 * z) + 2.0)
+    return np.stack([np.full_like(u, rho0), u, v, w, pr])
+
+
+def tgv_convar(p, mach=0.1, quad=3):
+    """Cell averages of the conservative variables of the Taylor-Green vortex
+    (Gauss-Legendre quadrature with `quad` points per direction), ghost
+    cells included (periodic)."""
+    G = p.ghost
+    gx, gw = np.polynomial.legendre.leggauss(quad)
+    xs = p.left + (np.arange(p.nx) + 0.5 - G) * p.dx
+    ys = p.down + (np.arange(p.ny) + 0.5 - G) * p.dy
+    zs = p.back + (np.arange(p.nz) + 0.5 - G) * p.dz
+    out = np.zeros((5, p.nx, p.ny, p.nz))
+    for a, wa in zip(gx, gw):
+        for b, wb in zip(gx, gw):
+            for c, wc in zip(gx, gw):
+                X, Y, Z = np.meshgrid(xs + 0.5 * a * p.dx, ys + 0.5 * b * p.dy, zs + 0.5 * c * p.dz, indexing="ij")
+                out += (wa * wb * wc / 8.0) * prim_to_convar_3d(tgv_prim_point(X, Y, Z, mach, gamma=p.gamma), p.gamma)
+    return out
+
+
+def kinetic_energy(convar, p):
+    """Mean kinetic energy 0.5 rho |u|^2 over the interior cells."""
+    G = p.ghost
+    c = convar[:, G:G + p.nodex, G:G + p.nodey, G:G + p.nodez]
+    return float(np.mean(0.5 * (c[1] ** 2 + c[2] ** 2 + c[3] ** 2) / c[0]))

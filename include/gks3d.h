@@ -1,0 +1,81 @@
+/*
+ * gks3d.h — C ABI of the 3D GKS path on the GPU (libgks2d.so).
+ *
+ * Replaces the 3D driver loop of the reference (CubicTube,
+ * src/3Dproblem.cpp:109-166: CopyFluid_new_to_old, Get_CFL, and per S2O4
+ * stage the six boundary functions, Convar_to_primvar,
+ * Reconstruction_within_cell (face, line and point values,
+ * reconstruction.cpp:3698-3753), Calculate_flux with GKS3D
+ * (flux_function.cpp:760-1163) and Update_gauss (time_advance.cpp:850-999)).
+ *
+ * The reference's 3D path does not run as shipped (Interface3d::gauss is
+ * never allocated, basic_function.cpp:1896-1904, and GKS3D takes the
+ * pressure for the Maxwellian's lambda, flux_function.cpp:832-838); this
+ * path keeps its algorithm with lambda from the conservative state, decides
+ * the line / point order-reduction fallback at every reconstruction, and
+ * adds periodic boundaries.  Scheme: WENO5-AO (wenoz weights, no DF factor)
+ * in conservative or characteristic variables, 2 x 2 Gauss points per face,
+ * GKS3D gks2nd with the S2O4 (dt, dt/2) combination, Euler or
+ * Navier-Stokes collision time.
+ *
+ * State arrays: conservative variables (rho, rho u, rho v, rho w, rho E) of
+ * every cell including ghosts, cell (i, j, k) at (i*ny + j)*nz + k
+ * (z fastest), with byte strides between cells and between variables as in
+ * gks2d.h.  Status codes and boundary kinds (free, periodic) as in gks2d.h.
+ */
+#ifndef GKS3D_H
+#define GKS3D_H
+
+#include <stdint.h>
+
+#include "gks2d.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* sides in the reference's call order (3Dproblem.cpp:137-142) */
+enum { GKS_SIDE3_XLEFT = 0, GKS_SIDE3_XRIGHT = 1, GKS_SIDE3_YLEFT = 2, GKS_SIDE3_YRIGHT = 3,
+       GKS_SIDE3_ZLEFT = 4, GKS_SIDE3_ZRIGHT = 5 };
+
+typedef struct gks3d_params {
+  int32_t nodex, nodey, nodez; /* interior cells */
+  int32_t ghost;               /* ghost layers: 3 (the reference's CubicTube) or more */
+  int32_t recon_variable;      /* GKS_VAR_* */
+  int32_t tau_type;            /* GKS_TAU_* */
+  int32_t K;                   /* internal degrees of freedom (2 for gamma 1.4) */
+  int32_t fast_math;           /* 0: IEEE build; 1: contracted FMAs */
+  double gamma, mu, nu;        /* Gamma, Mu, Nu */
+  double c1_euler, c2_euler;   /* numerical collision time coefficients */
+  double left, right, down, up, back, front; /* domain */
+  double cfl;
+} gks3d_params;
+
+typedef struct gks3d_ctx gks3d_ctx;
+
+int gks3d_create(const gks3d_params* p, gks3d_ctx** out);
+int gks3d_destroy(gks3d_ctx* ctx);
+int gks3d_set_convar(gks3d_ctx* ctx, const double* base, int64_t cell_stride, int64_t var_stride);
+int gks3d_get_convar(gks3d_ctx* ctx, double* base, int64_t cell_stride, int64_t var_stride);
+/* the primitive variables (rho, u, v, w, p) the first step's CFL reads
+ * (Get_CFL reads the primvar of the last stage; at step 0 the initial
+ * condition's) */
+int gks3d_set_cfl_primvar(gks3d_ctx* ctx, const double* base, int64_t cell_stride, int64_t var_stride);
+/* The driver loop: steps until *t reaches tstop or max_steps steps are
+ * taken; bc[6] per side (GKS_BC_FREE or GKS_BC_PERIODIC).  Returns
+ * GKS_ERR_BLOWUP if rho or p turns negative or NaN (that step is not
+ * counted). */
+int gks3d_run(gks3d_ctx* ctx, const int32_t bc[6], double tstop, int32_t max_steps, double* t, int32_t* step);
+int gks3d_sync(gks3d_ctx* ctx);
+/* Leaf parity: GKS3D at n Gauss points.  left/right: n x 20 (convar[5],
+ * der1x[5], der1y[5], der1z[5] in the face frame); center (optional):
+ * n x 20 collision state and slopes; f, derf: n x 5. */
+int gks3d_flux_points(const gks3d_params* p, int64_t n, const double* left, const double* right, double dt,
+                      double* center, double* f, double* derf);
+const char* gks3d_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* GKS3D_H */

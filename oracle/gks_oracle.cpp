@@ -1,12 +1,15 @@
 // gks_oracle.cpp — C API over the CPU restatement (oracle).  TEST
 // INFRASTRUCTURE ONLY: loaded by tests/, __graft_entry__.smoke() and the
 // cpu_baseline leg of bench.py as the checker; the product never links it.
+#include <cstddef>
 #include <cstdint>
 #include <cstring>
 
 #include "solver1d.hpp"
 #include "solver2d.hpp"
+#include "solver3d.hpp"
 #include "../include/gks1d.h"
+#include "../include/gks3d.h"
 
 namespace orc {
 template <> double Solver2d<double>::to_double(const double& r) { return r; }
@@ -358,6 +361,104 @@ int orc1d_flux_points(const gks1d_params* p, int64_t n, const double* left, cons
       std::memcpy(center + k * 6 + 3, C.der1, 24);
     }
     orc::gks1d_flux(f + k * 3, derf + k * 3, L, R, C, dt, g);
+  }
+  return GKS_OK;
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------------ 3D
+namespace {
+orc3d::Gas3 gas3_from(const gks3d_params& p) {
+  orc3d::Gas3 g;
+  g.K = p.K;
+  g.gamma = p.gamma;
+  g.mu = p.mu;
+  g.nu = p.nu;
+  g.c1 = p.c1_euler;
+  g.c2 = p.c2_euler;
+  g.tau_type = p.tau_type;
+  return g;
+}
+}  // namespace
+
+extern "C" {
+
+int64_t orc3d_params_layout(int64_t* off) {
+  int k = 0;
+#define OFF3(f) off[k++] = (int64_t)offsetof(gks3d_params, f)
+  OFF3(nodex); OFF3(nodey); OFF3(nodez); OFF3(ghost); OFF3(recon_variable); OFF3(tau_type); OFF3(K);
+  OFF3(fast_math); OFF3(gamma); OFF3(mu); OFF3(nu); OFF3(c1_euler); OFF3(c2_euler); OFF3(left); OFF3(right);
+  OFF3(down); OFF3(up); OFF3(back); OFF3(front); OFF3(cfl);
+#undef OFF3
+  return (int64_t)sizeof(gks3d_params);
+}
+
+void* orc3d_create(const gks3d_params* p) {
+  if (!p || p->ghost < 3 || p->nodex < 3 || p->nodey < 3 || p->nodez < 3) return nullptr;
+  orc3d::Params3 P;
+  P.NX = p->nodex;
+  P.NY = p->nodey;
+  P.NZ = p->nodez;
+  P.G = p->ghost;
+  P.recon_var = p->recon_variable;
+  P.dx = (p->right - p->left) / p->nodex;
+  P.dy = (p->up - p->down) / p->nodey;
+  P.dz = (p->front - p->back) / p->nodez;
+  P.cfl = p->cfl;
+  P.gas = gas3_from(*p);
+  return new orc3d::Solver3d(P);
+}
+void orc3d_destroy(void* h) { delete static_cast<orc3d::Solver3d*>(h); }
+// SoA [5][nc] conservative state, all cells
+int orc3d_set_convar(void* h, const double* conv) {
+  auto* s = static_cast<orc3d::Solver3d*>(h);
+  std::memcpy(s->U.data(), conv, s->U.size() * 8);
+  return GKS_OK;
+}
+int orc3d_set_cfl_primvar(void* h, const double* prim) {
+  static_cast<orc3d::Solver3d*>(h)->cfl_from_prim(prim);
+  return GKS_OK;
+}
+int orc3d_get(void* h, double* conv, double* t, int32_t* step) {
+  auto* s = static_cast<orc3d::Solver3d*>(h);
+  if (conv) std::memcpy(conv, s->U.data(), s->U.size() * 8);
+  if (t) *t = s->t;
+  if (step) *step = s->step;
+  return GKS_OK;
+}
+int orc3d_run(void* h, const int32_t* bc, double tstop, int32_t max_steps, double* dt_hist) {
+  int b[6];
+  for (int k = 0; k < 6; k++) b[k] = bc[k];
+  return static_cast<orc3d::Solver3d*>(h)->run(b, tstop, max_steps, dt_hist);
+}
+int orc3d_flux_points(const gks3d_params* p, int64_t n, const double* left, const double* right, double dt,
+                      double* center, double* f, double* derf) {
+  const orc3d::Gas3 g = gas3_from(*p);
+  for (int64_t k = 0; k < n; k++)
+    orc3d::gks3d_point(f + 5 * k, derf + 5 * k, center ? center + 20 * k : nullptr, left + 20 * k, right + 20 * k,
+                       dt, g);
+  return GKS_OK;
+}
+// leaves: face values of n 5-cell stencils (n x 5 x 5, face frame) ->
+// lo, ro (n x 10 each); tangential side of n stencils of records with nc
+// components (10: face records, 15: line records) -> n x 2 x (nc + 5)
+int orc3d_weno_face(const gks3d_params* p, int64_t n, const double* w, double h, double* lo, double* ro) {
+  const orc3d::Gas3 g = gas3_from(*p);
+  for (int64_t k = 0; k < n; k++)
+    orc3d::weno5_face(lo + 10 * k, ro + 10 * k, reinterpret_cast<const double(*)[5]>(w + 25 * k), h,
+                      p->recon_variable, g);
+  return GKS_OK;
+}
+int orc3d_weno_tangential(const gks3d_params* p, int64_t n, int32_t nc, const double* st, double h, double* out) {
+  const orc3d::Gas3 g = gas3_from(*p);
+  for (int64_t k = 0; k < n; k++) {
+    if (nc == 10)
+      orc3d::weno5_tangential_side<10>(reinterpret_cast<double(*)[15]>(out + 30 * k),
+                                      reinterpret_cast<const double(*)[10]>(st + 50 * k), h, p->recon_variable, g);
+    else
+      orc3d::weno5_tangential_side<15>(reinterpret_cast<double(*)[20]>(out + 40 * k),
+                                      reinterpret_cast<const double(*)[15]>(st + 75 * k), h, p->recon_variable, g);
   }
   return GKS_OK;
 }

@@ -527,3 +527,265 @@ int ref1d_flux_points(const gks1d_params* p, int64_t n, const double* left, cons
 }
 
 }  // extern "C"
+
+// =================================================================== 3D
+// The reference's 3D leaves, and its 3D driver loop (CubicTube,
+// 3Dproblem.cpp:3-166) made runnable by closing the two gaps of the shipped
+// path in the harness: the Gauss-point records Interface3d::gauss are
+// allocated here (Allocate_reconstruction_variable_along_interface,
+// basic_function.cpp:1896-1904, allocates the lines only), and
+// flux_function_3d points at a GKS3D that takes lambda from the
+// conservative state (the oracle's gks3d_point; GKS3D,
+// flux_function.cpp:813-1163, hands the pressure to MMDF3d_left/right, A
+// and Get_Tau where lambda belongs).  The boundary functions,
+// Convar_to_primvar, the face / line / point reconstructions,
+// Calculate_flux, Update_gauss, Get_CFL and the S2O4 coefficients are the
+// reference's own code.
+#include "3Dproblem.h"
+#include "../include/gks3d.h"
+#include "solver3d.hpp"
+
+namespace {
+orc3d::Gas3 g_gas3;
+
+void set_globals3(const gks3d_params* p) {
+  K = p->K;
+  Gamma = p->gamma;
+  Mu = p->mu;
+  Nu = p->nu;
+  c1_euler = p->c1_euler;
+  c2_euler = p->c2_euler;
+  tau_type = p->tau_type == GKS_TAU_NS ? NS : Euler;
+  reconstruction_variable = p->recon_variable == GKS_VAR_CHARACTERISTIC ? characteristic : conservative;
+  wenotype = wenoz;
+  is_reduce_order_warning = false;
+  is_multi_dimensional = true;
+  gks3dsolver = gks2nd_3d;
+  g0type = all_collisionn;
+  timecoe_list_3d = S2O4_3D;
+  cellreconstruction_3D_of_face_value = WENO5_AO_splitting_3d;
+  cellreconstruction_3D_of_line_value = WENO5_AO_of_line_value;
+  cellreconstruction_3D_of_point_value = WENO5_AO_of_point_value;
+  g0reconstruction_3D_of_face_value = Do_nothing_splitting_3d;
+  g0reconstruction_3D_of_line_value = Do_nothing_reconstruction_of_line_value;
+  g0reconstruction_3D_of_point_value = Do_nothing_reconstruction_of_point_value;
+  g_gas3 = orc3d::Gas3{p->K, p->gamma, p->mu, p->nu, p->c1_euler, p->c2_euler, p->tau_type};
+}
+
+void pack_point(double* o, const Point3d& pt) {
+  std::memcpy(o, pt.convar, 40);
+  std::memcpy(o + 5, pt.der1x, 40);
+  std::memcpy(o + 10, pt.der1y, 40);
+  std::memcpy(o + 15, pt.der1z, 40);
+}
+void unpack_point(Point3d& pt, const double* s) {
+  std::memcpy(pt.convar, s, 40);
+  std::memcpy(pt.der1x, s + 5, 40);
+  std::memcpy(pt.der1y, s + 10, 40);
+  std::memcpy(pt.der1z, s + 15, 40);
+}
+
+// the flux the harness gives flux_function_3d
+void gks3d_lambda_fixed(Flux3d& flux, Recon3d& re, double area, double dt) {
+  double L[20], R[20];
+  pack_point(L, re.left);
+  pack_point(R, re.right);
+  orc3d::gks3d_point(flux.f, flux.derf, nullptr, L, R, dt, g_gas3);
+}
+
+struct RefCtx3 {
+  gks3d_params p;
+  Block3d block;
+  Fluid3d* fluids = nullptr;
+  Interface3d* xi = nullptr;
+  Interface3d* yi = nullptr;
+  Interface3d* zi = nullptr;
+  Flux3d_gauss** xf = nullptr;
+  Flux3d_gauss** yf = nullptr;
+  Flux3d_gauss** zf = nullptr;
+  Fluid3d bcvalue[6];
+};
+}  // namespace
+
+extern "C" {
+
+void* ref3d_create(const gks3d_params* p) {
+  silence();
+  RefCtx3* c = new RefCtx3();
+  c->p = *p;
+  set_globals3(p);
+  flux_function_3d = gks3d_lambda_fixed;
+  gausspoint = 4;
+  SetGuassPoint_3D();
+  Block3d& b = c->block;
+  b.uniform = true;
+  b.nodex = p->nodex;
+  b.nodey = p->nodey;
+  b.nodez = p->nodez;
+  b.ghost = p->ghost;
+  b.CFL = p->cfl;
+  Initial_stages(b);
+  b.nx = b.nodex + 2 * b.ghost;
+  b.ny = b.nodey + 2 * b.ghost;
+  b.nz = b.nodez + 2 * b.ghost;
+  c->fluids = Setfluid_array(b);
+  c->xi = Setinterface_array(b);
+  c->yi = Setinterface_array(b);
+  c->zi = Setinterface_array(b);
+  c->xf = Setflux_gauss_array(b);
+  c->yf = Setflux_gauss_array(b);
+  c->zf = Setflux_gauss_array(b);
+  b.left = p->left;
+  b.right = p->right;
+  b.down = p->down;
+  b.up = p->up;
+  b.back = p->back;
+  b.front = p->front;
+  b.dx = (b.right - b.left) / b.nodex;
+  b.dy = (b.up - b.down) / b.nodey;
+  b.dz = (b.front - b.back) / b.nodez;
+  b.xarea = b.dy * b.dz;
+  b.yarea = b.dx * b.dz;
+  b.zarea = b.dx * b.dy;
+  b.overdx = 1 / b.dx;
+  b.overdy = 1 / b.dy;
+  b.overdz = 1 / b.dz;
+  SetUniformMesh(b, c->fluids, c->xi, c->yi, c->zi);
+  const size_t nf = (size_t)(b.nx + 1) * (b.ny + 1) * (b.nz + 1);
+  for (Interface3d* arr : {c->xi, c->yi, c->zi})
+    for (size_t f = 0; f < nf; f++) {
+      arr[f].gauss = new Recon3d[gausspoint];
+      for (int q = 0; q < gausspoint; q++) arr[f].gauss[q].order_reduce = false;
+    }
+  b.t = 0;
+  b.step = 0;
+  return c;
+}
+// the top-level blocks only, as ref2d_destroy
+void ref3d_destroy(void* h) {
+  RefCtx3* c = static_cast<RefCtx3*>(h);
+  delete[] c->fluids;
+  delete c;
+}
+// convar and the primitive variables Get_CFL reads first, [5][nc] each
+int ref3d_set_state(void* h, const double* conv, const double* prim) {
+  RefCtx3* c = static_cast<RefCtx3*>(h);
+  const Block3d& b = c->block;
+  size_t nc = (size_t)b.nx * b.ny * b.nz;
+  for (size_t k = 0; k < nc; k++)
+    for (int v = 0; v < 5; v++) {
+      c->fluids[k].convar[v] = conv[v * nc + k];
+      c->fluids[k].primvar[v] = prim[v * nc + k];
+    }
+  return GKS_OK;
+}
+int ref3d_get(void* h, double* conv, double* t, int32_t* step) {
+  RefCtx3* c = static_cast<RefCtx3*>(h);
+  const Block3d& b = c->block;
+  size_t nc = (size_t)b.nx * b.ny * b.nz;
+  for (size_t k = 0; k < nc; k++)
+    for (int v = 0; v < 5; v++) conv[v * nc + k] = c->fluids[k].convar[v];
+  if (t) *t = b.t;
+  if (step) *step = b.step;
+  return GKS_OK;
+}
+// the driver loop (3Dproblem.cpp:109-166) with the free boundaries, which
+// are the only 3D boundary functions the reference has
+int ref3d_run(void* h, double tstop, int32_t max_steps, double* dt_hist) {
+  RefCtx3* c = static_cast<RefCtx3*>(h);
+  set_globals3(&c->p);
+  flux_function_3d = gks3d_lambda_fixed;
+  gausspoint = 4;
+  Block3d& b = c->block;
+  int n = 0;
+  while (b.t < tstop && n < max_steps) {
+    CopyFluid_new_to_old(c->fluids, b);
+    b.dt = Get_CFL(b, c->fluids, tstop);
+    if (dt_hist) dt_hist[n] = b.dt;
+    for (int s = 0; s < b.stages; s++) {
+      free_boundary_xleft(c->fluids, b, c->bcvalue[0]);
+      free_boundary_xright(c->fluids, b, c->bcvalue[1]);
+      free_boundary_yleft(c->fluids, b, c->bcvalue[2]);
+      free_boundary_yright(c->fluids, b, c->bcvalue[3]);
+      free_boundary_zleft(c->fluids, b, c->bcvalue[4]);
+      free_boundary_zright(c->fluids, b, c->bcvalue[5]);
+      Convar_to_primvar(c->fluids, b);
+      Reconstruction_within_cell(c->xi, c->yi, c->zi, c->fluids, b);
+      Reconstruction_forg0(c->xi, c->yi, c->zi, c->fluids, b);
+      Calculate_flux(c->xf, c->yf, c->zf, c->xi, c->yi, c->zi, b, s);
+      Update_gauss(c->fluids, c->xf, c->yf, c->zf, b, s);
+    }
+    b.step++;
+    b.t = b.t + b.dt;
+    n++;
+  }
+  return GKS_OK;
+}
+
+// WENO5_AO (reconstruction.cpp:3803-3923) on n stencils w[n][5][5]; lo, ro:
+// n x 10 (convar, der1x) at the cell's left / right face
+int ref3d_weno_face(const gks3d_params* p, int64_t n, const double* w, double h, double* lo, double* ro) {
+  silence();
+  set_globals3(p);
+  for (int64_t k = 0; k < n; k++) {
+    double s[5][5];
+    std::memcpy(s, w + k * 25, sizeof(s));
+    Point3d l, r;
+    WENO5_AO(l, r, s[0], s[1], s[2], s[3], s[4], h);
+    std::memcpy(lo + k * 10, l.convar, 40);
+    std::memcpy(lo + k * 10 + 5, l.der1x, 40);
+    std::memcpy(ro + k * 10, r.convar, 40);
+    std::memcpy(ro + k * 10 + 5, r.der1x, 40);
+  }
+  return GKS_OK;
+}
+// WENO5_AO_for_line_value (nc 10: convar, der1x -> + der1y) or
+// WENO5_AO_for_point_value (nc 15: convar, der1x, der1y -> + der1z)
+// (reconstruction.cpp:3955-4145, 4163-4399) on n stencils st[n][5][nc],
+// given to both sides; out[n][2][nc + 5] the left side's two Gauss points
+int ref3d_weno_tangential(const gks3d_params* p, int64_t n, int32_t nc, const double* st, double h, double* out) {
+  silence();
+  set_globals3(p);
+  gausspoint = 4;
+  if (nc != 10 && nc != 15) return GKS_ERR_ARG;
+  for (int64_t k = 0; k < n; k++) {
+    Recon3d s[5], res[2];
+    for (int q = 0; q < 5; q++) {
+      double rec[20] = {0.0};
+      std::memcpy(rec, st + (k * 5 + q) * nc, sizeof(double) * nc);
+      unpack_point(s[q].left, rec);
+      unpack_point(s[q].right, rec);
+    }
+    res[0].order_reduce = false;
+    res[1].order_reduce = false;
+    if (nc == 10) WENO5_AO_for_line_value(res, s[0], s[1], s[2], s[3], s[4], h);
+    else WENO5_AO_for_point_value(res, s[0], s[1], s[2], s[3], s[4], h);
+    for (int g = 0; g < 2; g++) {
+      double rec[20];
+      pack_point(rec, res[g].left);
+      double* o = out + (k * 2 + g) * (nc + 5);
+      std::memcpy(o, rec, sizeof(double) * (nc + 5));
+    }
+  }
+  return GKS_OK;
+}
+// the reference's own GKS3D (flux_function.cpp:813-1163) at n Gauss points;
+// it is right only where lambda equals the pressure in the left, right and
+// collision states (p = sqrt(rho (K+3) (gamma-1) / 4))
+int ref3d_gks3d(const gks3d_params* p, int64_t n, const double* left, const double* right, double dt, double* f,
+                double* derf) {
+  silence();
+  set_globals3(p);
+  for (int64_t k = 0; k < n; k++) {
+    Recon3d re;
+    unpack_point(re.left, left + k * 20);
+    unpack_point(re.right, right + k * 20);
+    Flux3d fl;
+    GKS3D(fl, re, 1.0, dt);
+    std::memcpy(f + k * 5, fl.f, 40);
+    std::memcpy(derf + k * 5, fl.derf, 40);
+  }
+  return GKS_OK;
+}
+
+}  // extern "C"

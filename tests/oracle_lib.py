@@ -325,3 +325,157 @@ def reference1d():
     if _ref1 is None:
         _ref1 = CpuLib1D(REF_SO, "ref1d_")
     return _ref1
+
+
+# ------------------------------------------------------------------------ 3D
+class Oracle3:
+    """The 3D restatement (oracle/solver3d.hpp) through orc3d_*."""
+
+    def __init__(self):
+        from gks_amd.params import Params3D
+        self.lib = ctypes.CDLL(ORACLE_SO)
+        pp = ctypes.POINTER(Params3D)
+        sig = {
+            "orc3d_params_layout": (ctypes.c_int64, [ctypes.POINTER(ctypes.c_int64)]),
+            "orc3d_create": (_P, [pp]),
+            "orc3d_destroy": (None, [_P]),
+            "orc3d_set_convar": (ctypes.c_int, [_P, _D]),
+            "orc3d_set_cfl_primvar": (ctypes.c_int, [_P, _D]),
+            "orc3d_get": (ctypes.c_int, [_P, _D, _D, _I]),
+            "orc3d_run": (ctypes.c_int, [_P, _I, ctypes.c_double, ctypes.c_int32, _D]),
+            "orc3d_flux_points": (ctypes.c_int, [pp, ctypes.c_int64, _D, _D, ctypes.c_double, _D, _D, _D]),
+            "orc3d_weno_face": (ctypes.c_int, [pp, ctypes.c_int64, _D, ctypes.c_double, _D, _D]),
+            "orc3d_weno_tangential": (ctypes.c_int, [pp, ctypes.c_int64, ctypes.c_int32, _D, ctypes.c_double, _D]),
+        }
+        for name, (res, args) in sig.items():
+            f = getattr(self.lib, name)
+            f.restype = res
+            f.argtypes = args
+        self.f = self.lib
+
+    def flux_points(self, p, left, right, dt):
+        left = np.ascontiguousarray(left, dtype=np.float64)
+        right = np.ascontiguousarray(right, dtype=np.float64)
+        n = left.shape[0]
+        c, f, d = np.zeros((n, 20)), np.zeros((n, 5)), np.zeros((n, 5))
+        self.f.orc3d_flux_points(ctypes.byref(p), n, dptr(left), dptr(right), dt, dptr(c), dptr(f), dptr(d))
+        return c, f, d
+
+    def weno_face(self, p, w, h):
+        w = np.ascontiguousarray(w, dtype=np.float64)
+        n = w.shape[0]
+        lo, ro = np.zeros((n, 10)), np.zeros((n, 10))
+        self.f.orc3d_weno_face(ctypes.byref(p), n, dptr(w), h, dptr(lo), dptr(ro))
+        return lo, ro
+
+    def weno_tangential(self, p, st, h):
+        st = np.ascontiguousarray(st, dtype=np.float64)
+        n, nc = st.shape[0], st.shape[2]
+        out = np.zeros((n, 2, nc + 5))
+        self.f.orc3d_weno_tangential(ctypes.byref(p), n, nc, dptr(st), h, dptr(out))
+        return out
+
+    def run(self, p, convar, prim, bc, tstop, max_steps):
+        """(convar, t, steps, rc, dt history) after the driver loop."""
+        h = self.f.orc3d_create(ctypes.byref(p))
+        assert h
+        try:
+            u = np.ascontiguousarray(convar, dtype=np.float64)
+            pr = np.ascontiguousarray(prim, dtype=np.float64)
+            self.f.orc3d_set_convar(h, dptr(u))
+            self.f.orc3d_set_cfl_primvar(h, dptr(pr))
+            hist = np.zeros(max(1, max_steps))
+            rc = self.f.orc3d_run(h, (ctypes.c_int32 * 6)(*bc), tstop, max_steps, dptr(hist))
+            out = np.empty_like(u)
+            t = ctypes.c_double()
+            s = ctypes.c_int32()
+            self.f.orc3d_get(h, dptr(out), ctypes.byref(t), ctypes.byref(s))
+            return out, t.value, s.value, rc, hist[:s.value]
+        finally:
+            self.f.orc3d_destroy(h)
+
+
+_oracle3 = None
+
+
+def oracle3():
+    global _oracle3
+    if _oracle3 is None:
+        _oracle3 = Oracle3()
+    return _oracle3
+
+
+class Ref3:
+    """The compiled reference's 3D leaves and driver loop through ref3d_*
+    (oracle/ref_harness.cpp: the reference's code with Interface3d::gauss
+    allocated and a GKS3D that takes lambda from the conservative state)."""
+
+    def __init__(self):
+        from gks_amd.params import Params3D
+        self.lib = ctypes.CDLL(REF_SO)
+        pp = ctypes.POINTER(Params3D)
+        sig = {
+            "ref3d_create": (_P, [pp]),
+            "ref3d_destroy": (None, [_P]),
+            "ref3d_set_state": (ctypes.c_int, [_P, _D, _D]),
+            "ref3d_get": (ctypes.c_int, [_P, _D, _D, _I]),
+            "ref3d_run": (ctypes.c_int, [_P, ctypes.c_double, ctypes.c_int32, _D]),
+            "ref3d_weno_face": (ctypes.c_int, [pp, ctypes.c_int64, _D, ctypes.c_double, _D, _D]),
+            "ref3d_weno_tangential": (ctypes.c_int, [pp, ctypes.c_int64, ctypes.c_int32, _D, ctypes.c_double, _D]),
+            "ref3d_gks3d": (ctypes.c_int, [pp, ctypes.c_int64, _D, _D, ctypes.c_double, _D, _D]),
+        }
+        for name, (res, args) in sig.items():
+            f = getattr(self.lib, name)
+            f.restype = res
+            f.argtypes = args
+        self.f = self.lib
+
+    def gks3d(self, p, left, right, dt):
+        left = np.ascontiguousarray(left, dtype=np.float64)
+        right = np.ascontiguousarray(right, dtype=np.float64)
+        n = left.shape[0]
+        f, d = np.zeros((n, 5)), np.zeros((n, 5))
+        self.f.ref3d_gks3d(ctypes.byref(p), n, dptr(left), dptr(right), dt, dptr(f), dptr(d))
+        return f, d
+
+    def weno_face(self, p, w, h):
+        w = np.ascontiguousarray(w, dtype=np.float64)
+        n = w.shape[0]
+        lo, ro = np.zeros((n, 10)), np.zeros((n, 10))
+        self.f.ref3d_weno_face(ctypes.byref(p), n, dptr(w), h, dptr(lo), dptr(ro))
+        return lo, ro
+
+    def weno_tangential(self, p, st, h):
+        st = np.ascontiguousarray(st, dtype=np.float64)
+        n, nc = st.shape[0], st.shape[2]
+        out = np.zeros((n, 2, nc + 5))
+        assert self.f.ref3d_weno_tangential(ctypes.byref(p), n, nc, dptr(st), h, dptr(out)) == 0
+        return out
+
+    def run(self, p, convar, prim, tstop, max_steps):
+        """(convar, t, steps, dt history) after the driver loop (free boundaries)."""
+        h = self.f.ref3d_create(ctypes.byref(p))
+        assert h
+        try:
+            u = np.ascontiguousarray(convar, dtype=np.float64)
+            pr = np.ascontiguousarray(prim, dtype=np.float64)
+            self.f.ref3d_set_state(h, dptr(u), dptr(pr))
+            hist = np.zeros(max(1, max_steps))
+            self.f.ref3d_run(h, tstop, max_steps, dptr(hist))
+            out = np.empty_like(u)
+            t = ctypes.c_double()
+            s = ctypes.c_int32()
+            self.f.ref3d_get(h, dptr(out), ctypes.byref(t), ctypes.byref(s))
+            return out, t.value, s.value, hist[:s.value]
+        finally:
+            self.f.ref3d_destroy(h)
+
+
+_ref3 = None
+
+
+def reference3():
+    global _ref3
+    if _ref3 is None:
+        _ref3 = Ref3()
+    return _ref3
