@@ -22,6 +22,14 @@
 // the interior (order-free) instead of the sequential Dtx_3d scan.  Periodic
 // boundaries are added (the reference has free boundaries only in 3D).
 //
+// Pinned (tests/test_oracle_3d.py) against the reference's own compiled 3D
+// code through oracle/ref_harness.cpp: the face, line and point leaves
+// (WENO5_AO, WENO5_AO_for_line_value, WENO5_AO_for_point_value), GKS3D on
+// states whose pressure equals lambda (where its bug is silent), and the
+// whole driver loop with the Gauss-point records allocated and this file's
+// flux in flux_function_3d; further against the reference's GKS2D on planar
+// data and its 2D non-DF run on z-invariant data.
+//
 // Layout: cell (i, j, k) at (i*ny + j)*nz + k (z fastest, as the
 // reference); faces of each direction at (i*(ny+1) + j)*(nz+1) + k.
 // Gauss point q = 2 * line + point (SetGuassPoint_3D, fluid_mesh.cpp:80-88;
