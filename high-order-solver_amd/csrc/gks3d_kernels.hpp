@@ -1,0 +1,569 @@
+// gks3d_kernels.hpp — the kernels of the 3D GKS path (one S2O4 step of the
+// reference's CubicTube loop, 3Dproblem.cpp:109-166, with lambda from the
+// state and periodic boundaries added).
+//
+// Per stage: one boundary launch, the primitive-variable sweep + reduction
+// (the CFL inputs and the blow-up flag), then per direction d = x, y, z
+//   face   one thread per cell: WENO5-AO across the d faces
+//          (WENO5_AO_splitting_3d, reconstruction.cpp:3774-3923) -> fv
+//   line   one thread per d face: the first tangential reconstruction
+//          (WENO5_AO_of_line_value, :3925-4145) -> lv
+//   flux   one thread per d face that gets a flux: the second tangential
+//          reconstruction (WENO5_AO_of_point_value, :4147-4399) and GKS3D
+//          at the 2 x 2 Gauss points, combined into the S2O4 stage sum ->
+//          xs[d] (and the stage-0 share pc[d])
+// and finally the cell update (Update_gauss, time_advance.cpp:937-998).
+// The face and line buffers are reused by the three directions.
+//
+// Every thread's index box is the reference's loop range for that array;
+// threads are laid out with k (z) fastest so neighbouring lanes touch
+// neighbouring cells / faces in every SoA component.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "gks3d_device.hpp"
+#include "gks3d_launch.hpp"
+
+#ifndef GKS_VARIANT
+#error "GKS_VARIANT must name the kernel namespace"
+#endif
+
+namespace GKS_VARIANT {
+
+using namespace gksd3;
+
+constexpr int kBlock3 = 256;
+
+// a box [lo, hi) of cell or face indices, k fastest
+struct Box3 {
+  int lo[3], n[3];
+  __device__ __forceinline__ long long count() const { return (long long)n[0] * n[1] * n[2]; }
+  __device__ __forceinline__ void at(long long t, int& i, int& j, int& k) const {
+    k = lo[2] + (int)(t % n[2]);
+    t /= n[2];
+    j = lo[1] + (int)(t % n[1]);
+    i = lo[0] + (int)(t / n[1]);
+  }
+};
+__host__ __device__ inline Box3 make_box(int i0, int i1, int j0, int j1, int k0, int k1) {
+  Box3 b;
+  b.lo[0] = i0;
+  b.lo[1] = j0;
+  b.lo[2] = k0;
+  b.n[0] = i1 > i0 ? i1 - i0 : 0;
+  b.n[1] = j1 > j0 ? j1 - j0 : 0;
+  b.n[2] = k1 > k0 ? k1 - k0 : 0;
+  return b;
+}
+
+__device__ __forceinline__ long long cell3(const Dev3& d, int i, int j, int k) {
+  return ((long long)i * d.ny + j) * d.nz + k;
+}
+__device__ __forceinline__ long long face3(const Dev3& d, int i, int j, int k) {
+  return ((long long)i * (d.ny + 1) + j) * (d.nz + 1) + k;
+}
+
+// ------------------------------------------------------------ boundaries
+// free_boundary_* (boundary_condition.cpp:445-570) in the reference's side
+// order x, y, z compose into one map per ghost cell: each coordinate outside
+// the interior is clamped to the nearest interior layer (free) or wrapped
+// by the interior size (periodic); sources are interior cells only, so one
+// pass over the ghost cells gives the sequential result.
+__global__ void __launch_bounds__(kBlock3) k3_bc(Dev3 d, int b0, int b1, int b2, int b3, int b4, int b5) {
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= d.nc) return;
+  const int k = (int)(t % d.nz);
+  const int j = (int)((t / d.nz) % d.ny);
+  const int i = (int)(t / ((long long)d.nz * d.ny));
+  const int G = d.G;
+  auto map = [G](int x, int N, int lo_kind, int hi_kind) {
+    if (x < G) return lo_kind == 0 ? G : x + N;
+    if (x >= N + G) return hi_kind == 0 ? N + G - 1 : x - N;
+    return x;
+  };
+  const int si = map(i, d.NX, b0, b1), sj = map(j, d.NY, b2, b3), sk = map(k, d.NZ, b4, b5);
+  if (si == i && sj == j && sk == k) return;
+  const long long src = cell3(d, si, sj, sk);
+#pragma unroll
+  for (int v = 0; v < 5; v++) d.U[v * d.nc + t] = d.U[v * d.nc + src];
+}
+
+// ------------------------------------------------------- primitive sweep
+// Convar_to_primvar over the interior (basic_function.cpp:775-804) reduced
+// to what the next step's Get_CFL / Dtx_3d (time_advance.cpp:793-848) and
+// the blow-up check need: the largest |u| + c, the smallest density, and
+// whether any density or pressure is negative or NaN.
+__global__ void __launch_bounds__(kBlock3) k3_primvar(Dev3 d) {
+  __shared__ double s_max[kBlock3], s_min[kBlock3], s_bad[kBlock3];
+  const long long n = (long long)d.NX * d.NY * d.NZ;
+  double vmax = 0.0, rmin = 1e300, bad = 0.0;
+  for (long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (long long)gridDim.x * blockDim.x) {
+    const int k = d.G + (int)(t % d.NZ);
+    const int j = d.G + (int)((t / d.NZ) % d.NY);
+    const int i = d.G + (int)(t / ((long long)d.NZ * d.NY));
+    const long long c = cell3(d, i, j, k);
+    double u[5], p[5];
+#pragma unroll
+    for (int v = 0; v < 5; v++) u[v] = d.U[v * d.nc + c];
+    prim3(p, u, d.gas);
+    if (isnan(p[0]) || isnan(p[4]) || p[0] < 0 || p[4] < 0) bad = 1.0;
+    const double overden = 1.0 / p[0];
+    const double s = sqrt(p[1] * p[1] + p[2] * p[2] + p[3] * p[3]) + sqrt(d.gas.gamma * p[4] * overden);
+    if (s > vmax) vmax = s;
+    if (p[0] < rmin) rmin = p[0];
+  }
+  s_max[threadIdx.x] = vmax;
+  s_min[threadIdx.x] = rmin;
+  s_bad[threadIdx.x] = bad;
+  __syncthreads();
+  for (int w = kBlock3 / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) {
+      s_max[threadIdx.x] = fmax(s_max[threadIdx.x], s_max[threadIdx.x + w]);
+      s_min[threadIdx.x] = fmin(s_min[threadIdx.x], s_min[threadIdx.x + w]);
+      s_bad[threadIdx.x] = fmax(s_bad[threadIdx.x], s_bad[threadIdx.x + w]);
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    d.part[blockIdx.x] = s_max[0];
+    d.part[kParts3 + blockIdx.x] = s_min[0];
+    d.part[2 * kParts3 + blockIdx.x] = s_bad[0];
+  }
+}
+// one workgroup of kParts3 lanes: the partials -> stat (the blow-up flag
+// only ever rises; the host clears it)
+__global__ void __launch_bounds__(kParts3) k3_reduce(Dev3 d, int nparts) {
+  __shared__ double s_max[kParts3], s_min[kParts3], s_bad[kParts3];
+  const int t = threadIdx.x;
+  s_max[t] = t < nparts ? d.part[t] : 0.0;
+  s_min[t] = t < nparts ? d.part[kParts3 + t] : 1e300;
+  s_bad[t] = t < nparts ? d.part[2 * kParts3 + t] : 0.0;
+  __syncthreads();
+  for (int w = kParts3 / 2; w > 0; w >>= 1) {
+    if (t < w) {
+      s_max[t] = fmax(s_max[t], s_max[t + w]);
+      s_min[t] = fmin(s_min[t], s_min[t + w]);
+      s_bad[t] = fmax(s_bad[t], s_bad[t + w]);
+    }
+    __syncthreads();
+  }
+  if (t == 0) {
+    d.stat[0] = s_max[0];
+    d.stat[1] = s_min[0];
+    d.stat[2] = fmax(d.stat[2], s_bad[0]);
+  }
+}
+
+// CopyFluid_new_to_old (basic_function.cpp:1586-1603): interior cells
+__global__ void __launch_bounds__(kBlock3) k3_copy_old(Dev3 d, Box3 b) {
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= b.count()) return;
+  int i, j, k;
+  b.at(t, i, j, k);
+  const long long c = cell3(d, i, j, k);
+#pragma unroll
+  for (int v = 0; v < 5; v++) d.Uold[v * d.nc + c] = d.U[v * d.nc + c];
+}
+
+// ------------------------------------------------------------ face values
+// WENO5_AO (reconstruction.cpp:3803-3923) of one cell across its two DIR
+// faces, in the face frame (Ydirection / Zdirection for y / z)
+template <int DIR>
+__global__ void __launch_bounds__(kBlock3) k3_face(Dev3 d, Box3 b) {
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= b.count()) return;
+  int i, j, k;
+  b.at(t, i, j, k);
+  const Gas3& g = d.gas;
+  const long long c = cell3(d, i, j, k);
+  const long long cs = DIR == 0 ? (long long)d.ny * d.nz : (DIR == 1 ? (long long)d.nz : 1LL);
+  const double h = DIR == 0 ? d.dx : (DIR == 1 ? d.dy : d.dz);
+  double w[5][5];
+#pragma unroll
+  for (int s = 0; s < 5; s++) {
+    double u[5];
+#pragma unroll
+    for (int v = 0; v < 5; v++) u[v] = d.U[v * d.nc + c + (s - 2) * cs];
+    to_frame(w[s], u, DIR);
+  }
+  double lo[10], ro[10];
+  {
+    double pn1[5], p0[5], pp1[5];
+    prim3(pn1, w[1], g);
+    prim3(p0, w[2], g);
+    prim3(pp1, w[3], g);
+#pragma unroll
+    for (int side = 0; side < 2; side++) {
+      double base[5];
+#pragma unroll
+      for (int v = 0; v < 5; v++) base[v] = 0.5 * ((side == 0 ? pn1[v] : pp1[v]) + p0[v]);
+      double r[5][5], var[5], der1[5];
+      if (d.recon_var == 0) {
+#pragma unroll
+        for (int s = 0; s < 5; s++)
+#pragma unroll
+          for (int v = 0; v < 5; v++) r[s][v] = w[s][v];
+      } else {
+        double sm[25];
+        char_base3(sm, base, g);
+#pragma unroll
+        for (int s = 0; s < 5; s++) to_char3(r[s], sm, w[s]);
+      }
+#pragma unroll
+      for (int v = 0; v < 5; v++) {
+        if (side == 0) weno5_left3(var[v], der1[v], r[0][v], r[1][v], r[2][v], r[3][v], r[4][v], h);
+        else weno5_right3(var[v], der1[v], r[0][v], r[1][v], r[2][v], r[3][v], r[4][v], h);
+      }
+      double* out = side == 0 ? lo : ro;
+      if (d.recon_var == 0) {
+#pragma unroll
+        for (int v = 0; v < 5; v++) {
+          out[v] = var[v];
+          out[5 + v] = der1[v];
+        }
+      } else {
+        const RightEig e = right_eig(base, g);
+        from_char3(out, e, var);
+        from_char3(out + 5, e, der1);
+      }
+    }
+  }
+  if (reduce3(ro, g) || reduce3(lo, g)) {
+#pragma unroll
+    for (int v = 0; v < 5; v++) {
+      lo[v] = w[2][v];
+      ro[v] = w[2][v];
+      lo[5 + v] = 0.0;
+      ro[5 + v] = 0.0;
+    }
+  }
+  // the cell's left face gets its right-side state, the right face its left
+  const long long fa = face3(d, i, j, k);
+  const long long fb = face3(d, i + (DIR == 0), j + (DIR == 1), k + (DIR == 2));
+#pragma unroll
+  for (int m = 0; m < 10; m++) {
+    d.fv[(10 + m) * d.nf + fa] = lo[m];
+    d.fv[m * d.nf + fb] = ro[m];
+  }
+}
+
+// ------------------------------------------------ tangential reconstruction
+// One side of WENO5_AO_for_line_value (NC = 10: convar, der1x -> + der1y;
+// reconstruction.cpp:3955-4145) or WENO5_AO_for_point_value (NC = 15:
+// convar, der1x, der1y -> + der1z; :4163-4399).  st(q, m) reads stencil
+// entry q (0..4) component m; out[p][NC + 5]: per Gauss point p the
+// reconstructed components, then the new tangential derivative.  The first
+// field (convar) takes the derivative, the carried derivatives only values;
+// in characteristic variables every field is projected on the base of the
+// centre entry's primitive variables.
+template <int NC, class ST>
+__device__ __forceinline__ void tangential_side(double (*out)[NC + 5], const ST& st, double h, int recon_var,
+                                                const Gas3& g) {
+  constexpr int NF = NC / 5;
+  double c2[5], base[5];
+#pragma unroll
+  for (int v = 0; v < 5; v++) c2[v] = st(2, v);
+  prim3(base, c2, g);
+  if (recon_var == 0) {
+#pragma unroll
+    for (int fld = 0; fld < NF; fld++)
+#pragma unroll
+      for (int v = 0; v < 5; v++) {
+        const int m = 5 * fld + v;
+        double d0, d1;
+        if (fld == 0)
+          weno5_2gauss3<true>(out[0][m], d0, out[1][m], d1, st(0, m), st(1, m), st(2, m), st(3, m), st(4, m), h);
+        else
+          weno5_2gauss3<false>(out[0][m], d0, out[1][m], d1, st(0, m), st(1, m), st(2, m), st(3, m), st(4, m), h);
+        if (fld == 0) {
+          out[0][NC + v] = d0;
+          out[1][NC + v] = d1;
+        }
+      }
+    return;
+  }
+  double sm[25];
+  char_base3(sm, base, g);
+  const RightEig e = right_eig(base, g);
+#pragma unroll
+  for (int fld = 0; fld < NF; fld++) {
+    double r[5][5];
+#pragma unroll
+    for (int q = 0; q < 5; q++) {
+      double c[5];
+#pragma unroll
+      for (int v = 0; v < 5; v++) c[v] = st(q, 5 * fld + v);
+      to_char3(r[q], sm, c);
+    }
+    double var[2][5], der[2][5];
+#pragma unroll
+    for (int v = 0; v < 5; v++) {
+      if (fld == 0)
+        weno5_2gauss3<true>(var[0][v], der[0][v], var[1][v], der[1][v], r[0][v], r[1][v], r[2][v], r[3][v], r[4][v],
+                            h);
+      else
+        weno5_2gauss3<false>(var[0][v], der[0][v], var[1][v], der[1][v], r[0][v], r[1][v], r[2][v], r[3][v],
+                             r[4][v], h);
+    }
+#pragma unroll
+    for (int p = 0; p < 2; p++) {
+      from_char3(out[p] + 5 * fld, e, var[p]);
+      if (fld == 0) from_char3(out[p] + NC, e, der[p]);
+    }
+  }
+}
+
+// stencil readers: entry q at face f0 + (q - 2) * step (reversed stencils
+// pass a negative step), component m of side `side` in an SoA buffer
+struct FvStencil {
+  const double* p;
+  long long nf, f0, step;
+  int side;
+  __device__ __forceinline__ double operator()(int q, int m) const {
+    return p[(side * 10 + m) * nf + f0 + (q - 2) * step];
+  }
+};
+struct LvStencil {
+  const double* p;
+  long long nf, f0, step;
+  int line, side;
+  __device__ __forceinline__ double operator()(int q, int m) const {
+    return p[((line * 2 + side) * 15 + m) * nf + f0 + (q - 2) * step];
+  }
+};
+
+// WENO5_AO_of_line_value (reconstruction.cpp:3925-3953): x faces along y
+// (h = dy), y faces along -x (h = dx, the stencil reversed), z faces along
+// y (h = dy); both lines fall back to the face value (tangential
+// derivative 0) when any of the four reconstructed states has lambda <= 0
+template <int DIR>
+__global__ void __launch_bounds__(kBlock3) k3_line(Dev3 d, Box3 b) {
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= b.count()) return;
+  int i, j, k;
+  b.at(t, i, j, k);
+  const long long f = face3(d, i, j, k);
+  const long long step = DIR == 1 ? -(long long)(d.ny + 1) * (d.nz + 1) : (long long)(d.nz + 1);
+  const double h = DIR == 1 ? d.dx : d.dy;
+  double res[2][2][15];  // [side][line]
+#pragma unroll
+  for (int side = 0; side < 2; side++)
+    tangential_side<10>(res[side], FvStencil{d.fv, d.nf, f, step, side}, h, d.recon_var, d.gas);
+  bool red = false;
+#pragma unroll
+  for (int l = 0; l < 2; l++) red = red || reduce3(res[0][l], d.gas) || reduce3(res[1][l], d.gas);
+  if (red) {
+#pragma unroll
+    for (int side = 0; side < 2; side++)
+#pragma unroll
+      for (int l = 0; l < 2; l++) {
+#pragma unroll
+        for (int m = 0; m < 10; m++) res[side][l][m] = d.fv[(side * 10 + m) * d.nf + f];
+#pragma unroll
+        for (int m = 10; m < 15; m++) res[side][l][m] = 0.0;
+      }
+  }
+#pragma unroll
+  for (int l = 0; l < 2; l++)
+#pragma unroll
+    for (int side = 0; side < 2; side++)
+#pragma unroll
+      for (int m = 0; m < 15; m++) d.lv[((l * 2 + side) * 15 + m) * d.nf + f] = res[side][l][m];
+}
+
+// WENO5_AO_of_point_value (reconstruction.cpp:4147-4161) at the faces that
+// get a flux (Calculate_flux, flux_function.cpp:760-811) — x and y faces
+// along z (h = dz), z faces along -x (h = dx, reversed) — then GKS3D at the
+// two Gauss points of each line and the S2O4 stage combination of
+// Update_gauss (time_advance.cpp:867-935): stage 0 x = 0.5 f + 1/8 derf
+// (keeping 1.0 f + 1/6 derf for stage 1), stage 1 x = that + 1/3 derf.
+template <int DIR>
+__global__ void __launch_bounds__(kBlock3) k3_flux(Dev3 d, Box3 b, int stage, double dt) {
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= b.count()) return;
+  int i, j, k;
+  b.at(t, i, j, k);
+  const long long f = face3(d, i, j, k);
+  const long long step = DIR == 2 ? -(long long)(d.ny + 1) * (d.nz + 1) : 1LL;
+  const double h = DIR == 2 ? d.dx : d.dz;
+  double* xs = d.xs[DIR];
+  double* pc = d.pc[DIR];
+  for (int l = 0; l < 2; l++) {
+    double res[2][2][20];  // [side][point]
+#pragma unroll
+    for (int side = 0; side < 2; side++)
+      tangential_side<15>(res[side], LvStencil{d.lv, d.nf, f, step, l, side}, h, d.recon_var, d.gas);
+    bool red = false;
+#pragma unroll
+    for (int p = 0; p < 2; p++) red = red || reduce3(res[0][p], d.gas) || reduce3(res[1][p], d.gas);
+    if (red) {
+#pragma unroll
+      for (int side = 0; side < 2; side++)
+#pragma unroll
+        for (int p = 0; p < 2; p++) {
+#pragma unroll
+          for (int m = 0; m < 15; m++) res[side][p][m] = d.lv[((l * 2 + side) * 15 + m) * d.nf + f];
+#pragma unroll
+          for (int m = 15; m < 20; m++) res[side][p][m] = 0.0;
+        }
+    }
+    double x[5] = {0.0, 0.0, 0.0, 0.0, 0.0}, share[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int p = 0; p < 2; p++) {
+      double fo[5], dfo[5];
+      gks3d_point_dev(fo, dfo, nullptr, res[0][p], res[1][p], dt, d.gas);
+#pragma unroll
+      for (int v = 0; v < 5; v++) {
+        if (stage == 0) {
+          x[v] += 0.0 + (0.5 * fo[v] + 1.0 / 8.0 * dfo[v]);
+          share[v] += 0.0 + (1.0 * fo[v] + 1.0 / 6.0 * dfo[v]);
+        } else {
+          x[v] += 0.0 * fo[v] + 1.0 / 3.0 * dfo[v];
+        }
+      }
+    }
+#pragma unroll
+    for (int v = 0; v < 5; v++) {
+      const long long o = (long long)(l * 5 + v) * d.nf + f;
+      if (stage == 0) {
+        xs[o] = x[v];
+        pc[o] = share[v];
+      } else {
+        xs[o] = pc[o] + x[v];
+      }
+    }
+  }
+}
+
+// the cell update of Update_gauss (time_advance.cpp:937-998): U = Uold +
+// sum over the Gauss points (weights 1/4) of the face sums, the y / z face
+// frames rotated back
+__global__ void __launch_bounds__(kBlock3) k3_update(Dev3 d, Box3 b) {
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= b.count()) return;
+  int i, j, k;
+  b.at(t, i, j, k);
+  const long long c = cell3(d, i, j, k), fc = face3(d, i, j, k);
+  const long long fx = face3(d, i + 1, j, k), fy = face3(d, i, j + 1, k), fz = face3(d, i, j, k + 1);
+  const double odx = 1 / d.dx, ody = 1 / d.dy, odz = 1 / d.dz;
+  const double w = 0.25;
+  double tot[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+  for (int l = 0; l < 2; l++) {
+    auto X = [&](int dir, long long f, int v) { return d.xs[dir][(long long)(l * 5 + v) * d.nf + f]; };
+    tot[0] += w * odz * (X(2, fc, 0) - X(2, fz, 0));
+    tot[1] += w * odz * (-X(2, fc, 3) + X(2, fz, 3));
+    tot[2] += w * odz * (X(2, fc, 2) - X(2, fz, 2));
+    tot[3] += w * odz * (X(2, fc, 1) - X(2, fz, 1));
+    tot[4] += w * odz * (X(2, fc, 4) - X(2, fz, 4));
+    tot[0] += w * ody * (X(1, fc, 0) - X(1, fy, 0));
+    tot[1] += w * ody * (-X(1, fc, 2) + X(1, fy, 2));
+    tot[2] += w * ody * (X(1, fc, 1) - X(1, fy, 1));
+    tot[3] += w * ody * (X(1, fc, 3) - X(1, fy, 3));
+    tot[4] += w * ody * (X(1, fc, 4) - X(1, fy, 4));
+#pragma unroll
+    for (int v = 0; v < 5; v++) tot[v] += w * odx * (X(0, fc, v) - X(0, fx, v));
+  }
+#pragma unroll
+  for (int v = 0; v < 5; v++) d.U[v * d.nc + c] = d.Uold[v * d.nc + c] + tot[v];
+}
+
+// leaf parity: GKS3D at n Gauss points (left / right n x 20, row major)
+struct RowSide {
+  const double* p;
+  __device__ __forceinline__ double operator[](int k) const { return p[k]; }
+};
+__global__ void __launch_bounds__(kBlock3) k3_flux_points(Gas3 g, long long n, const double* L, const double* R,
+                                                          double dt, double* C, double* F, double* D) {
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  double f[5], df[5], c[20];
+  gks3d_point_dev(f, df, C ? c : nullptr, RowSide{L + t * 20}, RowSide{R + t * 20}, dt, g);
+#pragma unroll
+  for (int v = 0; v < 5; v++) {
+    F[t * 5 + v] = f[v];
+    D[t * 5 + v] = df[v];
+  }
+  if (C)
+    for (int v = 0; v < 20; v++) C[t * 20 + v] = c[v];
+}
+
+// the device pass sees no launcher, so the direction templates are
+// instantiated here explicitly
+template __global__ void k3_face<0>(Dev3, Box3);
+template __global__ void k3_face<1>(Dev3, Box3);
+template __global__ void k3_face<2>(Dev3, Box3);
+template __global__ void k3_line<0>(Dev3, Box3);
+template __global__ void k3_line<1>(Dev3, Box3);
+template __global__ void k3_line<2>(Dev3, Box3);
+template __global__ void k3_flux<0>(Dev3, Box3, int, double);
+template __global__ void k3_flux<1>(Dev3, Box3, int, double);
+template __global__ void k3_flux<2>(Dev3, Box3, int, double);
+
+// ------------------------------------------------------------ launchers
+#if !defined(__HIP_DEVICE_COMPILE__)
+static inline unsigned blocks_for(long long n) { return (unsigned)((n + kBlock3 - 1) / kBlock3); }
+static inline long long box_count(const Box3& b) { return (long long)b.n[0] * b.n[1] * b.n[2]; }
+
+static hipError_t launch_bc(const Dev3& d, const int* bc, hipStream_t s) {
+  k3_bc<<<blocks_for(d.nc), kBlock3, 0, s>>>(d, bc[0], bc[1], bc[2], bc[3], bc[4], bc[5]);
+  return hipGetLastError();
+}
+static hipError_t launch_primvar(const Dev3& d, hipStream_t s) {
+  const long long n = (long long)d.NX * d.NY * d.NZ;
+  long long nb = (n + kBlock3 - 1) / kBlock3;
+  if (nb > kParts3) nb = kParts3;
+  k3_primvar<<<(unsigned)nb, kBlock3, 0, s>>>(d);
+  k3_reduce<<<1, kParts3, 0, s>>>(d, (int)nb);
+  return hipGetLastError();
+}
+static Box3 interior(const Dev3& d) {
+  return make_box(d.G, d.G + d.NX, d.G, d.G + d.NY, d.G, d.G + d.NZ);
+}
+static hipError_t launch_copy_old(const Dev3& d, hipStream_t s) {
+  const Box3 b = interior(d);
+  if (box_count(b) > 0) k3_copy_old<<<blocks_for(box_count(b)), kBlock3, 0, s>>>(d, b);
+  return hipGetLastError();
+}
+// the index boxes of the reference's loops (reconstruction.cpp:3698-3753
+// with the guards of :3779, 3928-3949, 4147-4161; flux_function.cpp:760-811)
+template <int DIR>
+static hipError_t launch_direction_t(const Dev3& d, int stage, double dt, hipStream_t s) {
+  const int G = d.G, nx = d.nx, ny = d.ny, nz = d.nz;
+  // face values: the DIR index in [G-1, n-G], the others in [G-2, n-G+2)
+  const Box3 bf = make_box(DIR == 0 ? G - 1 : G - 2, DIR == 0 ? nx - G + 1 : nx - G + 2,
+                           DIR == 1 ? G - 1 : G - 2, DIR == 1 ? ny - G + 1 : ny - G + 2,
+                           DIR == 2 ? G - 1 : G - 2, DIR == 2 ? nz - G + 1 : nz - G + 2);
+  // line values
+  Box3 bl;
+  if (DIR == 0) bl = make_box(G, nx - G + 1, G, ny - G, G - 2, nz - G + 2);
+  else if (DIR == 1) bl = make_box(G, nx - G, G, ny - G + 1, G - 2, nz - G + 2);
+  else bl = make_box(G - 2, nx - G + 2, G, ny - G, G, nz - G + 1);
+  // the faces that get a flux
+  const Box3 bx = make_box(G, G + d.NX + (DIR == 0), G, G + d.NY + (DIR == 1), G, G + d.NZ + (DIR == 2));
+  if (box_count(bf) > 0) k3_face<DIR><<<blocks_for(box_count(bf)), kBlock3, 0, s>>>(d, bf);
+  if (box_count(bl) > 0) k3_line<DIR><<<blocks_for(box_count(bl)), kBlock3, 0, s>>>(d, bl);
+  if (box_count(bx) > 0) k3_flux<DIR><<<blocks_for(box_count(bx)), kBlock3, 0, s>>>(d, bx, stage, dt);
+  return hipGetLastError();
+}
+static hipError_t launch_direction(const Dev3& d, int dir, int stage, double dt, hipStream_t s) {
+  if (dir == 0) return launch_direction_t<0>(d, stage, dt, s);
+  if (dir == 1) return launch_direction_t<1>(d, stage, dt, s);
+  return launch_direction_t<2>(d, stage, dt, s);
+}
+static hipError_t launch_update(const Dev3& d, hipStream_t s) {
+  const Box3 b = interior(d);
+  if (box_count(b) > 0) k3_update<<<blocks_for(box_count(b)), kBlock3, 0, s>>>(d, b);
+  return hipGetLastError();
+}
+static hipError_t launch_flux_points(const Gas3& g, long long n, const double* L, const double* R, double dt,
+                                     double* C, double* F, double* D, hipStream_t s) {
+  if (n > 0) k3_flux_points<<<blocks_for(n), kBlock3, 0, s>>>(g, n, L, R, dt, C, F, D);
+  return hipGetLastError();
+}
+
+extern const Launch3dTable table3d = {launch_bc, launch_primvar, launch_copy_old, launch_direction, launch_update,
+                                      launch_flux_points};
+#endif
+
+}  // namespace GKS_VARIANT

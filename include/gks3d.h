@@ -1,5 +1,5 @@
 /*
- * gks3d.h — C ABI of the 3D GKS path on the GPU (libgks2d.so).
+ * gks3d.h — C ABI of the 3D GKS path on the GPU (libgks3d.so).
  *
  * Replaces the 3D driver loop of the reference (CubicTube,
  * src/3Dproblem.cpp:109-166: CopyFluid_new_to_old, Get_CFL, and per S2O4

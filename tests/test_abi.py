@@ -166,3 +166,47 @@ def test_batch_rejects_bad_arguments_on_host():
     assert L.gks2d_batch_set_shape(None, 64, 1) == 1
     assert L.gks2d_batch_run(None, None, None, 1, 1, None, None) == 1
     assert L.gks2d_batch_destroy(None) == 0
+
+
+# ------------------------------------------------------------------------ 3D
+HEADER3 = os.path.join(ROOT, "include", "gks3d.h")
+
+
+def test_3d_library_exports_every_declared_symbol():
+    from gks_amd import solver3d
+    names = sorted(set(re.findall(r"\b(gks3d_[a-z_0-9]+)\s*\(", open(HEADER3).read())))
+    assert "gks3d_run" in names and len(names) >= 9
+    lib = solver3d.lib3()
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
+    assert set(names) <= set(solver3d.SIGS3)
+
+
+@pytest.mark.parametrize("bad,code", [
+    (dict(ghost=2), 2), (dict(ghost=9), 2), (dict(nodex=0), 1), (dict(right=-1.0), 1), (dict(front=-2.0), 1),
+    (dict(recon_variable=3), 1), (dict(cfl=0.0), 1), (dict(tau_type=5), 1), (dict(gamma=1.0), 1),
+])
+def test_3d_create_rejects_bad_params_on_host(bad, code):
+    from gks_amd import solver3d
+    from gks_amd.params import cubic_params
+    p = cubic_params(8).copy(**bad)
+    h = ctypes.c_void_p()
+    rc = solver3d.lib3().gks3d_create(ctypes.byref(p), ctypes.byref(h))
+    assert rc == code
+    assert not h.value
+    assert solver3d.lib3().gks3d_last_error()
+
+
+def test_3d_null_arguments_are_argument_errors():
+    from gks_amd import solver3d
+    L = solver3d.lib3()
+    bc = (ctypes.c_int32 * 6)(0, 0, 0, 0, 0, 0)
+    t = ctypes.c_double(0.0)
+    s = ctypes.c_int32(0)
+    assert L.gks3d_run(None, bc, 1.0, 1, ctypes.byref(t), ctypes.byref(s)) == 1
+    assert L.gks3d_set_convar(None, None, 8, 8) == 1
+    assert L.gks3d_get_convar(None, None, 8, 8) == 1
+    assert L.gks3d_set_cfl_primvar(None, None, 8, 8) == 1
+    assert L.gks3d_sync(None) == 1
+    assert L.gks3d_destroy(None) == 0
+    assert L.gks3d_flux_points(None, 0, None, None, 0.0, None, None, None) == 1

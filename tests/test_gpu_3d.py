@@ -1,0 +1,175 @@
+"""The 3D path on the GPU (include/gks3d.h) against the 3D oracle
+(oracle/solver3d.hpp, itself pinned to the reference's compiled 3D leaves
+and driver loop, tests/test_oracle_3d.py).
+
+Tolerances: the IEEE build (no FMA contraction, the oracle's operation
+order) differs from the oracle through the device erfc / exp / sqrt and the
+order in which the four Gauss points of a face are summed: 1e-13 relative
+on a flux point, 1e-11 relative on the state after a few steps.  The fast
+build contracts FMAs: 1e-11 on a flux point, 1e-8 on the state."""
+import numpy as np
+import pytest
+
+from gks_amd.params import BC_FREE, BC_PERIODIC, cubic_params, tgv_params
+from gks_amd.problems import convar_to_prim_3d, cubic_sod_prim, kinetic_energy, prim_to_convar_3d, tgv_convar
+from gks_amd.solver3d import Gks3dError, Solver3D, flux_points_3d
+from oracle_lib import oracle3
+
+pytestmark = pytest.mark.gpu
+TOL_STATE = {0: 1e-11, 1: 1e-8}
+TOL_FLUX = {0: 1e-13, 1: 1e-11}
+
+
+def say(*a):
+    print(*a, flush=True)
+
+
+def states3(n, seed, slope=0.3):
+    rng = np.random.default_rng(seed)
+    r = rng.uniform(0.5, 2.0, n)
+    u, v, w = rng.uniform(-1, 1, (3, n))
+    p = rng.uniform(0.5, 2.0, n)
+    c = prim_to_convar_3d(np.stack([r, u, v, w, p]), 1.4).T
+    out = np.zeros((n, 20))
+    out[:, :5] = c
+    for d in range(1, 4):
+        out[:, 5 * d:5 * d + 5] = slope * c * rng.uniform(-1, 1, (n, 5))
+    return out
+
+
+@pytest.mark.parametrize("fast", [0, 1])
+@pytest.mark.parametrize("gas", ["euler", "ns"])
+def test_flux_points_match_the_oracle(fast, gas):
+    kw = dict(tau_type=1, mu=2e-3) if gas == "ns" else {}
+    p = cubic_params(8, fast_math=fast, **kw)
+    L, R = states3(2000, 1), states3(2000, 2)
+    R = L + 0.3 * (R - L)
+    say(f"[3d flux points] fast {fast} {gas}")
+    c, f, d = flux_points_3d(p, L, R, 2e-3)
+    oc, of, od = oracle3().flux_points(p, L, R, 2e-3)
+    tol = TOL_FLUX[fast]
+    # derf = 4 (F(dt) - 2 F(dt/2)) cancels: its error is measured on the scale of f
+    sf = np.abs(of).max(axis=1, keepdims=True)
+    assert np.max(np.abs(f - of) / sf) < tol
+    assert np.max(np.abs(d - od) / sf) < tol
+    assert np.max(np.abs(c - oc) / np.abs(oc).max(axis=1, keepdims=True)) < tol
+
+
+def gpu_run(p, u0, prim, bc, tstop, steps):
+    s = Solver3D(p)
+    s.set_convar(u0)
+    s.set_cfl_primvar(prim)
+    n = s.run(bc, tstop, steps)
+    out = (s.get_convar(), s.t, n)
+    s.close()
+    return out
+
+
+def compare_runs(p, u0, prim, bc, steps, fast, tstop=10.0):
+    u, t, n = gpu_run(p, u0, prim, bc, tstop, steps)
+    ou, ot, on, rc, _ = oracle3().run(p, u0, prim, bc, tstop, steps)
+    assert rc == 0 and n == on == steps
+    assert abs(t - ot) <= 1e-13 * ot
+    G = p.ghost
+    a, b = u[:, G:-G, G:-G, G:-G], ou[:, G:-G, G:-G, G:-G]
+    err = np.max(np.abs(a - b)) / np.max(np.abs(b))
+    say(f"   {steps} steps, t {t:.6g}: max rel diff {err:.2e}")
+    assert err < TOL_STATE[fast]
+
+
+@pytest.mark.parametrize("fast", [0, 1])
+@pytest.mark.parametrize("rv", [0, 1])
+def test_cubic_sod_matches_the_oracle(fast, rv):
+    p = cubic_params(12, fast_math=fast, recon_variable=rv)
+    prim = cubic_sod_prim(p)
+    say(f"[3d cubic sod] 12^3 fast {fast} recon {rv}")
+    compare_runs(p, prim_to_convar_3d(prim, 1.4), prim, [BC_FREE] * 6, 4, fast)
+
+
+@pytest.mark.parametrize("fast", [0, 1])
+def test_taylor_green_matches_the_oracle(fast):
+    p = tgv_params(12, fast_math=fast)
+    u0 = tgv_convar(p)
+    say(f"[3d taylor-green] 12^3 fast {fast}")
+    compare_runs(p, u0, convar_to_prim_3d(u0, 1.4), [BC_PERIODIC] * 6, 4, fast)
+
+
+@pytest.mark.parametrize("ghost", [3, 4])
+def test_uneven_box_mixed_boundaries_match_the_oracle(ghost):
+    """nodex != nodey != nodez, a domain that is not a cube, free x and
+    periodic y / z boundaries, a Navier-Stokes gas with numerical jump term."""
+    p = cubic_params(10, ghost=ghost, tau_type=1, mu=5e-3, c1_euler=0.0)
+    p.nodey, p.nodez = 9, 7
+    p.up, p.front = 0.9, 1.4
+    rng = np.random.default_rng(ghost)
+    shape = (p.nx, p.ny, p.nz)
+    x = np.linspace(0, 1, p.nx)[:, None, None]
+    y = np.linspace(0, 1, p.ny)[None, :, None]
+    z = np.linspace(0, 1, p.nz)[None, None, :]
+    prim = np.zeros((5,) + shape)
+    prim[0] = 1 + 0.3 * (x > 0.5) + 0.05 * np.sin(2 * np.pi * (y + z))
+    prim[1] = 0.2 * np.cos(2 * np.pi * z)
+    prim[2] = -0.1 + 0.05 * rng.standard_normal(shape)
+    prim[3] = 0.1 * np.sin(2 * np.pi * y)
+    prim[4] = 1 + 0.5 * (x > 0.5)
+    say(f"[3d uneven box] 10x9x7 ghost {ghost}")
+    compare_runs(p, prim_to_convar_3d(prim, 1.4), prim, [BC_FREE, BC_FREE] + [BC_PERIODIC] * 4, 4, 0)
+
+
+def test_run_stops_at_tstop_and_resumes():
+    p = cubic_params(10, fast_math=1)
+    prim = cubic_sod_prim(p)
+    s = Solver3D(p)
+    s.init_prim(prim)
+    s.run([BC_FREE] * 6, 0.01)
+    assert abs(s.t - 0.01) < 1e-12
+    n1 = s.step_count
+    s.run([BC_FREE] * 6, 0.02)
+    assert abs(s.t - 0.02) < 1e-12 and s.step_count > n1
+    u = s.get_convar()
+    s.close()
+    ou, ot, on, rc, _ = oracle3().run(p, prim_to_convar_3d(prim, 1.4), prim, [BC_FREE] * 6, 0.01, 10**6)
+    assert on == n1
+
+
+def test_periodic_taylor_green_conserves_on_the_gpu():
+    p = tgv_params(24, fast_math=1)
+    u0 = tgv_convar(p)
+    s = Solver3D(p)
+    s.init_convar(u0)
+    n = s.run([BC_PERIODIC] * 6, 10.0, 6)
+    u = s.get_convar()
+    s.close()
+    G = p.ghost
+    a = u0[:, G:-G, G:-G, G:-G].sum(axis=(1, 2, 3))
+    b = u[:, G:-G, G:-G, G:-G].sum(axis=(1, 2, 3))
+    scale = np.abs(u0[:, G:-G, G:-G, G:-G]).sum(axis=(1, 2, 3))
+    scale[3] = scale[1]
+    say(f"[3d tgv 24^3] {n} steps, conservation error {np.abs(a - b) / scale}")
+    assert n == 6 and np.all(np.abs(a - b) <= 1e-12 * scale)
+    assert kinetic_energy(u, p) < kinetic_energy(u0, p)
+
+
+def test_blowup_is_reported_and_the_step_not_counted():
+    p = cubic_params(8, fast_math=1)
+    prim = cubic_sod_prim(p)
+    s = Solver3D(p)
+    s.init_prim(prim)
+    u = s.get_convar()
+    u[0, 6, 6, 6] = -1.0
+    s.set_convar(u)
+    with pytest.raises(Gks3dError) as e:
+        s.run([BC_FREE] * 6, 1.0, 3)
+    assert e.value.code == 4
+    assert s.step_count == 0 and s.t == 0.0
+    s.close()
+
+
+def test_bad_boundary_kind_is_rejected():
+    p = cubic_params(8)
+    s = Solver3D(p)
+    s.init_prim(cubic_sod_prim(p))
+    with pytest.raises(Gks3dError) as e:
+        s.run([2] * 6, 1.0, 1)
+    assert e.value.code == 2
+    s.close()
