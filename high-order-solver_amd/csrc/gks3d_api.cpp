@@ -151,12 +151,13 @@ int gks3d_create(const gks3d_params* p, gks3d_ctx** out) {
   int rc;
   if ((rc = dev_alloc(c, &d.U, 5 * nc)) || (rc = dev_alloc(c, &d.Uold, 5 * nc)) ||
       (rc = dev_alloc(c, &d.fv, 20 * nf)) || (rc = dev_alloc(c, &d.lv, 60 * nf)) ||
+      (rc = dev_alloc(c, &d.pv, 160 * nf)) ||
       (rc = dev_alloc(c, &d.part, 3 * kParts3)) || (rc = dev_alloc(c, &d.stat, 4))) {
     gks3d_destroy(c);
     return rc;
   }
   for (int dir = 0; dir < 3; dir++)
-    if ((rc = dev_alloc(c, &d.xs[dir], 10 * nf)) || (rc = dev_alloc(c, &d.pc[dir], 10 * nf))) {
+    if ((rc = dev_alloc(c, &d.xs[dir], 5 * nf)) || (rc = dev_alloc(c, &d.pc[dir], 5 * nf))) {
       gks3d_destroy(c);
       return rc;
     }

@@ -2,9 +2,10 @@
 //
 // Each function restates one leaf of the reference's 3D scheme (file:line in
 // the reference repository) in the reference's floating-point operation
-// order, so that the IEEE build (-ffp-contract=off) differs from a CPU run
-// of the same algorithm only through the device erfc / exp / sqrt.  The fast
-// build includes the same text with FMA contraction on.
+// order, so that the IEEE build (GKS_FAST 0, -ffp-contract=off) differs from
+// a CPU run of the same algorithm only through the device erfc / exp / sqrt
+// (and the order in which a face's Gauss points are summed).  The fast build
+// (GKS_FAST 1) contracts FMAs and multiplies by reciprocals (G3_* below).
 //
 // Departures from the shipped reference, all deliberate (its 3D path does
 // not run as shipped):
@@ -26,6 +27,9 @@
 #ifndef GKS_VARIANT
 #error "GKS_VARIANT must name the kernel build"
 #endif
+#ifndef GKS_FAST
+#error "GKS_FAST must be 0 or 1"
+#endif
 
 namespace gksd3 {
 inline namespace GKS_VARIANT {
@@ -34,14 +38,56 @@ inline namespace GKS_VARIANT {
 #define GKS3_DI __device__ __forceinline__
 #endif
 
+// The fast build (GKS_FAST 1) turns divisions by constants, by the cell
+// size and by a denominator shared by several quotients into
+// multiplications by a reciprocal formed once (each quotient moves by an ulp
+// or two); the IEEE build keeps every division of the reference.
+#if GKS_FAST
+#define G3_CDIV(a, b) ((a) * (1.0 / (b)))
+#else
+#define G3_CDIV(a, b) ((a) / (b))
+#endif
+// the cell size of a reconstruction and, for the fast build, its reciprocals
+struct HInv {
+  double h, r1, r6, r12, r108;
+};
+GKS3_DI HInv hinv(double h) {
+  HInv r;
+  r.h = h;
+  r.r1 = 1.0 / h;
+  r.r6 = 1.0 / (6 * h);
+  r.r12 = 1.0 / (12.0 * h);
+  r.r108 = 1.0 / (108 * h);
+  return r;
+}
+#if GKS_FAST
+#define G3_DH(a, H) ((a) * (H).r1)
+#define G3_D6H(a, H) ((a) * (H).r6)
+#define G3_D12H(a, H) ((a) * (H).r12)
+#define G3_D108H(a, H) ((a) * (H).r108)
+#else
+#define G3_DH(a, H) ((a) / (H).h)
+#define G3_D6H(a, H) ((a) / (6 * (H).h))
+#define G3_D12H(a, H) ((a) / (12.0 * (H).h))
+#define G3_D108H(a, H) ((a) / (108 * (H).h))
+#endif
+
 // ---------------------------------------------------------------- gas algebra
 // Convar_to_primvar_3D: basic_function.cpp:819-826 (Pressure :1162-1167)
 GKS3_DI void prim3(double* p, const double* c, const Gas3& g) {
   p[0] = c[0];
+#if GKS_FAST
+  const double ir = 1.0 / c[0];
+  p[1] = c[1] * ir;
+  p[2] = c[2] * ir;
+  p[3] = c[3] * ir;
+  p[4] = (g.gamma - 1.0) * (c[4] - 0.5 * c[1] * c[1] * ir - 0.5 * c[2] * c[2] * ir - 0.5 * c[3] * c[3] * ir);
+#else
   p[1] = c[1] / c[0];
   p[2] = c[2] / c[0];
   p[3] = c[3] / c[0];
   p[4] = (g.gamma - 1.0) * (c[4] - 0.5 * c[1] * c[1] / c[0] - 0.5 * c[2] * c[2] / c[0] - 0.5 * c[3] * c[3] / c[0]);
+#endif
 }
 // Lambda (3D): basic_function.cpp:1179-1182
 GKS3_DI double lambda3(double rho, double u, double v, double w, double rhoE, const Gas3& g) {
@@ -50,14 +96,26 @@ GKS3_DI double lambda3(double rho, double u, double v, double w, double rhoE, co
 // (rho, u, v, w, lambda): the state the Maxwellian moments need
 GKS3_DI void ulambda3(double* p, const double* c, const Gas3& g) {
   p[0] = c[0];
+#if GKS_FAST
+  const double ir = 1.0 / c[0];
+  p[1] = c[1] * ir;
+  p[2] = c[2] * ir;
+  p[3] = c[3] * ir;
+#else
   p[1] = c[1] / c[0];
   p[2] = c[2] / c[0];
   p[3] = c[3] / c[0];
+#endif
   p[4] = lambda3(c[0], p[1], p[2], p[3], c[4], g);
 }
 // Check_Order_Reduce_by_Lambda_3D: reconstruction.cpp:3755-3771
 GKS3_DI bool reduce3(const double* c, const Gas3& g) {
+#if GKS_FAST
+  const double ir = 1.0 / c[0];
+  const double l = lambda3(c[0], c[1] * ir, c[2] * ir, c[3] * ir, c[4], g);
+#else
   const double l = lambda3(c[0], c[1] / c[0], c[2] / c[0], c[3] / c[0], c[4], g);
+#endif
   return l <= 0.0 || isnan(l);
 }
 // Ydirection / Zdirection: basic_function.cpp:1208-1224 (global frame ->
@@ -191,25 +249,34 @@ GKS3_DI void weno5_weights3(double* fw, double wn2, double wn1, double w0, doubl
     sum_alpha += alpha[i];
   }
   double ww[4];
+#if GKS_FAST
+  const double rs = 1.0 / sum_alpha;
+#pragma unroll
+  for (int k = 0; k < 4; k++) ww[k] = alpha[k] * rs;
+  fw[3] = ww[3] * (1.0 / dhi);
+#pragma unroll
+  for (int k = 0; k < 3; k++) fw[k] = ww[k] - ww[3] * (1.0 / dhi) * d[k];
+#else
 #pragma unroll
   for (int k = 0; k < 4; k++) ww[k] = alpha[k] / sum_alpha;
   fw[3] = ww[3] / d[3];
 #pragma unroll
   for (int k = 0; k < 3; k++) fw[k] = ww[k] - ww[3] / d[3] * d[k];
+#endif
 }
 // weno_5th_ao_left: reconstruction.cpp:161-258 (value, first derivative)
 GKS3_DI void weno5_left3(double& var, double& der1, double wn2, double wn1, double w0, double wp1, double wp2,
-                         double h) {
+                         const HInv& H) {
   double fw[4], p[4], px[4];
   weno5_weights3(fw, wn2, wn1, w0, wp1, wp2, 1e-10);
   p[0] = -1.0 / 6.0 * wn2 + 5.0 / 6.0 * wn1 + 1.0 / 3.0 * w0;
   p[1] = 1.0 / 3.0 * wn1 + 5.0 / 6.0 * w0 - 1.0 / 6.0 * wp1;
   p[2] = 11.0 / 6.0 * w0 - 7.0 / 6.0 * wp1 + 1.0 / 3.0 * wp2;
   p[3] = (1.0 / 60.0) * (47.0 * w0 + 27.0 * wn1 - 3.0 * wn2 - 13.0 * wp1 + 2.0 * wp2);
-  px[0] = (w0 - wn1) / h;
-  px[1] = (w0 - wn1) / h;
-  px[2] = -((2.0 * w0 - 3.0 * wp1 + wp2) / h);
-  px[3] = (15.0 * w0 - 15.0 * wn1 + wn2 - wp1) / (12.0 * h);
+  px[0] = G3_DH(w0 - wn1, H);
+  px[1] = G3_DH(w0 - wn1, H);
+  px[2] = -(G3_DH(2.0 * w0 - 3.0 * wp1 + wp2, H));
+  px[3] = G3_D12H(15.0 * w0 - 15.0 * wn1 + wn2 - wp1, H);
   var = 0.0;
   der1 = 0.0;
 #pragma unroll
@@ -220,17 +287,17 @@ GKS3_DI void weno5_left3(double& var, double& der1, double wn2, double wn1, doub
 }
 // weno_5th_ao_right: reconstruction.cpp:260-358
 GKS3_DI void weno5_right3(double& var, double& der1, double wn2, double wn1, double w0, double wp1, double wp2,
-                          double h) {
+                          const HInv& H) {
   double fw[4], p[4], px[4];
   weno5_weights3(fw, wn2, wn1, w0, wp1, wp2, 1e-10);
   p[0] = 1.0 / 3.0 * wn2 - 7.0 / 6.0 * wn1 + 11.0 / 6.0 * w0;
   p[1] = -1.0 / 6.0 * wn1 + 5.0 / 6.0 * w0 + 1.0 / 3.0 * wp1;
   p[2] = 1.0 / 3.0 * w0 + 5.0 / 6.0 * wp1 - 1.0 / 6.0 * wp2;
   p[3] = (1.0 / 60.0) * (47.0 * w0 - 13.0 * wn1 + 2.0 * wn2 + 27.0 * wp1 - 3.0 * wp2);
-  px[0] = (2.0 * w0 - 3.0 * wn1 + wn2) / h;
-  px[1] = (-w0 + wp1) / h;
-  px[2] = (-w0 + wp1) / h;
-  px[3] = (-15.0 * w0 + wn1 + 15.0 * wp1 - wp2) / (12.0 * h);
+  px[0] = G3_DH(2.0 * w0 - 3.0 * wn1 + wn2, H);
+  px[1] = G3_DH(-w0 + wp1, H);
+  px[2] = G3_DH(-w0 + wp1, H);
+  px[3] = G3_D12H(-15.0 * w0 + wn1 + 15.0 * wp1 - wp2, H);
   var = 0.0;
   der1 = 0.0;
 #pragma unroll
@@ -243,49 +310,49 @@ GKS3_DI void weno5_right3(double& var, double& der1, double wn2, double wn1, dou
 // points -+ h / (2 sqrt 3) and, when DER, their first derivatives
 template <bool DER>
 GKS3_DI void weno5_2gauss3(double& g1, double& g1x, double& g2, double& g2x, double wn2, double wn1, double w0,
-                           double wp1, double wp2, double h) {
+                           double wp1, double wp2, const HInv& H) {
   double fw[4], p[4];
   weno5_weights3(fw, wn2, wn1, w0, wp1, wp2, 1e-6);
   const double sqrt3 = 1.732050807568877294;
-  p[0] = w0 - (sqrt3 * w0) / 4 + (4 * wn1 - wn2) / (4 * sqrt3);
-  p[1] = w0 + (wn1 - wp1) / (4 * sqrt3);
+  p[0] = w0 - G3_CDIV(sqrt3 * w0, 4) + G3_CDIV(4 * wn1 - wn2, 4 * sqrt3);
+  p[1] = w0 + G3_CDIV(wn1 - wp1, 4 * sqrt3);
   p[2] = (1.0 / 12.0) * (3 * (4 + sqrt3) * w0 + sqrt3 * (-4 * wp1 + wp2));
-  p[3] = (4314 * w0 + (4 + 500 * sqrt3) * wn1 - wn2 - 70 * sqrt3 * wn2 + 4 * wp1 - 500 * sqrt3 * wp1 - wp2 +
-          70 * sqrt3 * wp2) /
-         4320;
+  p[3] = G3_CDIV(4314 * w0 + (4 + 500 * sqrt3) * wn1 - wn2 - 70 * sqrt3 * wn2 + 4 * wp1 - 500 * sqrt3 * wp1 - wp2 +
+                     70 * sqrt3 * wp2,
+                 4320);
   g1 = 0.0;
 #pragma unroll
   for (int k = 0; k < 4; k++) g1 += fw[k] * p[k];
   g1x = 0.0;
   if (DER) {
     double px[4];
-    px[0] = -((-9 + sqrt3) * w0 - 2 * (-6 + sqrt3) * wn1 + (-3 + sqrt3) * wn2) / (6 * h);
-    px[1] = -(-2 * sqrt3 * w0 + (3 + sqrt3) * wn1 + (-3 + sqrt3) * wp1) / (6 * h);
-    px[2] = -((9 + sqrt3) * w0 - 2 * (6 + sqrt3) * wp1 + (3 + sqrt3) * wp2) / (6 * h);
-    px[3] = (48 * sqrt3 * w0 - 72 * wn1 - 26 * sqrt3 * wn1 + 9 * wn2 + 2 * sqrt3 * wn2 + 72 * wp1 - 26 * sqrt3 * wp1 -
-             9 * wp2 + 2 * sqrt3 * wp2) /
-            (108 * h);
+    px[0] = G3_D6H(-((-9 + sqrt3) * w0 - 2 * (-6 + sqrt3) * wn1 + (-3 + sqrt3) * wn2), H);
+    px[1] = G3_D6H(-(-2 * sqrt3 * w0 + (3 + sqrt3) * wn1 + (-3 + sqrt3) * wp1), H);
+    px[2] = G3_D6H(-((9 + sqrt3) * w0 - 2 * (6 + sqrt3) * wp1 + (3 + sqrt3) * wp2), H);
+    px[3] = G3_D108H(48 * sqrt3 * w0 - 72 * wn1 - 26 * sqrt3 * wn1 + 9 * wn2 + 2 * sqrt3 * wn2 + 72 * wp1 -
+                         26 * sqrt3 * wp1 - 9 * wp2 + 2 * sqrt3 * wp2,
+                     H);
 #pragma unroll
     for (int k = 0; k < 4; k++) g1x += fw[k] * px[k];
   }
   p[0] = (1.0 / 12.0) * (3 * (4 + sqrt3) * w0 + sqrt3 * (-4 * wn1 + wn2));
-  p[1] = w0 + (-wn1 + wp1) / (4 * sqrt3);
-  p[2] = w0 - (sqrt3 * w0) / 4 + (4 * wp1 - wp2) / (4 * sqrt3);
-  p[3] = (4314 * w0 + (4 - 500 * sqrt3) * wn1 - wn2 + 70 * sqrt3 * wn2 + 4 * wp1 + 500 * sqrt3 * wp1 - wp2 -
-          70 * sqrt3 * wp2) /
-         4320;
+  p[1] = w0 + G3_CDIV(-wn1 + wp1, 4 * sqrt3);
+  p[2] = w0 - G3_CDIV(sqrt3 * w0, 4) + G3_CDIV(4 * wp1 - wp2, 4 * sqrt3);
+  p[3] = G3_CDIV(4314 * w0 + (4 - 500 * sqrt3) * wn1 - wn2 + 70 * sqrt3 * wn2 + 4 * wp1 + 500 * sqrt3 * wp1 - wp2 -
+                     70 * sqrt3 * wp2,
+                 4320);
   g2 = 0.0;
 #pragma unroll
   for (int k = 0; k < 4; k++) g2 += fw[k] * p[k];
   g2x = 0.0;
   if (DER) {
     double px[4];
-    px[0] = ((9 + sqrt3) * w0 - 2 * (6 + sqrt3) * wn1 + (3 + sqrt3) * wn2) / (6 * h);
-    px[1] = (-2 * sqrt3 * w0 + (-3 + sqrt3) * wn1 + (3 + sqrt3) * wp1) / (6 * h);
-    px[2] = ((-9 + sqrt3) * w0 - 2 * (-6 + sqrt3) * wp1 + (-3 + sqrt3) * wp2) / (6 * h);
-    px[3] = -((48 * sqrt3 * w0 + 72 * wn1 - 26 * sqrt3 * wn1 - 9 * wn2 + 2 * sqrt3 * wn2 - 72 * wp1 -
-               26 * sqrt3 * wp1 + 9 * wp2 + 2 * sqrt3 * wp2) /
-              (108 * h));
+    px[0] = G3_D6H((9 + sqrt3) * w0 - 2 * (6 + sqrt3) * wn1 + (3 + sqrt3) * wn2, H);
+    px[1] = G3_D6H(-2 * sqrt3 * w0 + (-3 + sqrt3) * wn1 + (3 + sqrt3) * wp1, H);
+    px[2] = G3_D6H((-9 + sqrt3) * w0 - 2 * (-6 + sqrt3) * wp1 + (-3 + sqrt3) * wp2, H);
+    px[3] = -(G3_D108H(48 * sqrt3 * w0 + 72 * wn1 - 26 * sqrt3 * wn1 - 9 * wn2 + 2 * sqrt3 * wn2 - 72 * wp1 -
+                           26 * sqrt3 * wp1 + 9 * wp2 + 2 * sqrt3 * wp2,
+                       H));
 #pragma unroll
     for (int k = 0; k < 4; k++) g2x += fw[k] * px[k];
   }
@@ -294,57 +361,38 @@ GKS3_DI void weno5_2gauss3(double& g1, double& g1x, double& g2, double& g2x, dou
 // ------------------------------------------------------------ GKS3D
 // MMDF3d_left / _right / MMDF3d: basic_function.cpp:189-393.  uh: the
 // half-space u moments (u > 0 for the left state, u < 0 for the right), uw
-// the whole-space ones.
+// the whole-space ones (mm3_side_from01 and half01 below fill a side).
 struct MM3 {
   double uh[7], uw[7], vw[6], ww[6], xi2, xi4;
 };
-GKS3_DI void mm3_side(MM3& m, int side, double u, double v, double w, double lambda, const Gas3& g) {
-  const double pi = 3.14159265358979323846;
-  const double ol = 1.0 / lambda;
-  m.uw[0] = 1;
-  m.uw[1] = u;
-  const double beta = exp(-lambda * u * u) / sqrt(pi * lambda);
-  if (side == 0) {
-    m.uh[0] = 0.5 * erfc(sqrt(lambda) * (-u));
-    m.uh[1] = u * m.uh[0] + 0.5 * beta;
-  } else {
-    m.uh[0] = 0.5 * erfc(sqrt(lambda) * u);
-    m.uh[1] = u * m.uh[0] - 0.5 * beta;
-  }
-#pragma unroll
-  for (int i = 2; i <= 6; i++) m.uw[i] = u * m.uw[i - 1] + 0.5 * (i - 1) * ol * m.uw[i - 2];
-#pragma unroll
-  for (int i = 2; i <= 6; i++) m.uh[i] = u * m.uh[i - 1] + 0.5 * (i - 1) * ol * m.uh[i - 2];
-  m.vw[0] = 1;
-  m.vw[1] = v;
-  m.ww[0] = 1;
-  m.ww[1] = w;
-#pragma unroll
-  for (int i = 2; i <= 5; i++) {
-    m.vw[i] = v * m.vw[i - 1] + 0.5 * (i - 1) * ol * m.vw[i - 2];
-    m.ww[i] = w * m.ww[i - 1] + 0.5 * (i - 1) * ol * m.ww[i - 2];
-  }
-  m.xi2 = 0.5 * g.K * ol;
-  m.xi4 = 0.25 * (g.K * g.K + 2 * g.K) * ol * ol;
-}
 // the whole-space moments of MMDF3d (it divides by lambda where the
 // half-space classes multiply by 1/lambda)
 GKS3_DI void mm3_whole(MM3& m, double u, double v, double w, double lambda, const Gas3& g) {
+#if GKS_FAST
+  const double ol = 1.0 / lambda;
+#define G3_DL(a) ((a) * ol)
+#define G3_DLL(a) ((a) * ol * ol)
+#else
+#define G3_DL(a) ((a) / lambda)
+#define G3_DLL(a) ((a) / (lambda * lambda))
+#endif
   m.uw[0] = 1;
   m.uw[1] = u;
 #pragma unroll
-  for (int i = 2; i <= 6; i++) m.uw[i] = u * m.uw[i - 1] + 0.5 * (i - 1) / lambda * m.uw[i - 2];
+  for (int i = 2; i <= 6; i++) m.uw[i] = u * m.uw[i - 1] + G3_DL(0.5 * (i - 1)) * m.uw[i - 2];
   m.vw[0] = 1;
   m.vw[1] = v;
   m.ww[0] = 1;
   m.ww[1] = w;
 #pragma unroll
   for (int i = 2; i <= 5; i++) {
-    m.vw[i] = v * m.vw[i - 1] + 0.5 * (i - 1) / lambda * m.vw[i - 2];
-    m.ww[i] = w * m.ww[i - 1] + 0.5 * (i - 1) / lambda * m.ww[i - 2];
+    m.vw[i] = v * m.vw[i - 1] + G3_DL(0.5 * (i - 1)) * m.vw[i - 2];
+    m.ww[i] = w * m.ww[i - 1] + G3_DL(0.5 * (i - 1)) * m.ww[i - 2];
   }
-  m.xi2 = 0.5 * g.K / lambda;
-  m.xi4 = 0.25 * (g.K * g.K + 2 * g.K) / (lambda * lambda);
+  m.xi2 = G3_DL(0.5 * g.K);
+  m.xi4 = G3_DLL(0.25 * (g.K * g.K + 2 * g.K));
+#undef G3_DL
+#undef G3_DLL
 }
 // table entries upvwxi / unvwxi (half, H) and uvwxi (whole, W)
 template <bool HALF>
@@ -434,36 +482,91 @@ struct Side3 {
   GKS3_DI double operator[](int k) const { return p[k * s]; }
 };
 
+// the half-space moments of one side from its order-0 and order-1 entries
+// (the rest of mm3_side, without the erfc / exp)
+GKS3_DI void mm3_side_from01(MM3& m, double uh0, double uh1, double u, double v, double w, double lambda,
+                             const Gas3& g) {
+  const double ol = 1.0 / lambda;
+  m.uw[0] = 1;
+  m.uw[1] = u;
+  m.uh[0] = uh0;
+  m.uh[1] = uh1;
+#pragma unroll
+  for (int i = 2; i <= 6; i++) m.uw[i] = u * m.uw[i - 1] + 0.5 * (i - 1) * ol * m.uw[i - 2];
+#pragma unroll
+  for (int i = 2; i <= 6; i++) m.uh[i] = u * m.uh[i - 1] + 0.5 * (i - 1) * ol * m.uh[i - 2];
+  m.vw[0] = 1;
+  m.vw[1] = v;
+  m.ww[0] = 1;
+  m.ww[1] = w;
+#pragma unroll
+  for (int i = 2; i <= 5; i++) {
+    m.vw[i] = v * m.vw[i - 1] + 0.5 * (i - 1) * ol * m.vw[i - 2];
+    m.ww[i] = w * m.ww[i - 1] + 0.5 * (i - 1) * ol * m.ww[i - 2];
+  }
+  m.xi2 = 0.5 * g.K * ol;
+  m.xi4 = 0.25 * (g.K * g.K + 2 * g.K) * ol * ol;
+}
+// the order-0 and order-1 half-space u moments (MMDF3d_left / _right,
+// basic_function.cpp:189-290): side 0 integrates u > 0, side 1 u < 0
+GKS3_DI void half01(double& uh0, double& uh1, int side, double u, double lambda) {
+  const double pi = 3.14159265358979323846;
+  const double beta = exp(-lambda * u * u) / sqrt(pi * lambda);
+  if (side == 0) {
+    uh0 = 0.5 * erfc(sqrt(lambda) * (-u));
+    uh1 = u * uh0 + 0.5 * beta;
+  } else {
+    uh0 = 0.5 * erfc(sqrt(lambda) * u);
+    uh1 = u * uh0 - 0.5 * beta;
+  }
+}
+// one side's share of the collision state (Collision, basic_function.h:203-212)
+GKS3_DI void collision_share(double* c, double rho, const MM3& m) {
+  c[0] = rho * m.uh[0];
+  c[1] = rho * m.uh[1];
+  c[2] = rho * m.vw[1] * m.uh[0];
+  c[3] = rho * m.ww[1] * m.uh[0];
+  c[4] = 0.5 * rho * (m.uh[2] + m.uh[0] * m.vw[2] + m.uh[0] * m.ww[2] + m.uh[0] * m.xi2);
+}
+
 // GKS3D with gks2nd_3d and the S2O4 (dt, dt/2) combination for one Gauss
 // point (flux_function.cpp:813-1163), lambda from the conservative state.
-// The dt and dt/2 fluxes F0, F1 are accumulated term by term in the
-// reference's order (F0: t0, t3, t1, t2, t4, t5; F1: t0 ... t5), so each
-// moment vector dies after its use.  center (optional): the collision state
-// and its three derivatives.
+// Evaluated in three phases so that one Maxwellian's moments are live at a
+// time: (0) both sides' order-0/1 moments, the collision state, tau and
+// both time-coefficient sets, g0u; (A) the left side's moment vectors;
+// (B) the right side's, folded with the left ones into the dt and dt/2
+// fluxes F0, F1 in the reference's summation order (F0: t0, t3, t1, t2,
+// t4, t5 terms; F1: t0 ... t5); (C) the collision state's slopes.
+// center (optional): the collision state and its three derivatives.
 template <class S>
 GKS3_DI void gks3d_point_dev(double* f, double* derf, double* center, const S& L, const S& R, double dt,
                              const Gas3& g) {
-  double cl[5], cr[5], pl[5], pr[5];
+  double pl[5], pr[5];
+  {
+    double c[5];
 #pragma unroll
-  for (int v = 0; v < 5; v++) {
-    cl[v] = L[v];
-    cr[v] = R[v];
+    for (int v = 0; v < 5; v++) c[v] = L[v];
+    ulambda3(pl, c, g);
+#pragma unroll
+    for (int v = 0; v < 5; v++) c[v] = R[v];
+    ulambda3(pr, c, g);
   }
-  ulambda3(pl, cl, g);
-  ulambda3(pr, cr, g);
-  MM3 ml, mr;
-  mm3_side(ml, 0, pl[1], pl[2], pl[3], pl[4], g);
-  mm3_side(mr, 1, pr[1], pr[2], pr[3], pr[4], g);
-  const Tab3<true> tl{ml}, tr{mr};
-  const Tab3<false> wl{ml}, wr{mr};
-  // Collision (basic_function.h:203-212)
+  const double unit[5] = {1, 0.0, 0.0, 0.0, 0.0};
+  // ---- phase 0
+  double uhl0, uhl1, uhr0, uhr1;
+  half01(uhl0, uhl1, 0, pl[1], pl[4]);
+  half01(uhr0, uhr1, 1, pr[1], pr[4]);
   double c0[5];
-  c0[0] = pl[0] * ml.uh[0] + pr[0] * mr.uh[0];
-  c0[1] = pl[0] * ml.uh[1] + pr[0] * mr.uh[1];
-  c0[2] = pl[0] * ml.vw[1] * ml.uh[0] + pr[0] * mr.vw[1] * mr.uh[0];
-  c0[3] = pl[0] * ml.ww[1] * ml.uh[0] + pr[0] * mr.ww[1] * mr.uh[0];
-  c0[4] = 0.5 * pl[0] * (ml.uh[2] + ml.uh[0] * ml.vw[2] + ml.uh[0] * ml.ww[2] + ml.uh[0] * ml.xi2) +
-          0.5 * pr[0] * (mr.uh[2] + mr.uh[0] * mr.vw[2] + mr.uh[0] * mr.ww[2] + mr.uh[0] * mr.xi2);
+  {
+    MM3 m;
+    double cl[5], cr[5];
+    mm3_side_from01(m, uhl0, uhl1, pl[1], pl[2], pl[3], pl[4], g);
+    collision_share(cl, pl[0], m);
+    mm3_side_from01(m, uhr0, uhr1, pr[1], pr[2], pr[3], pr[4], g);
+    collision_share(cr, pr[0], m);
+#pragma unroll
+    for (int v = 0; v < 5; v++) c0[v] = cl[v] + cr[v];
+  }
   double p0[5];
   ulambda3(p0, c0, g);
   const double tau = tau_ns3(p0[0], p0[4], g);
@@ -471,89 +574,93 @@ GKS3_DI void gks3d_point_dev(double* f, double* derf, double* center, const S& L
   time_coe3(t0, tau, tau_num3(pl[0], pr[0], p0[0], pl[4], pr[4], p0[4], dt, g), dt);
   const double dt2 = 0.5 * dt;
   time_coe3(t1, tau, tau_num3(pl[0], pr[0], p0[0], pl[4], pr[4], p0[4], dt2, g), dt2);
-  double F0[5], F1[5];
-  const double unit[5] = {1, 0.0, 0.0, 0.0, 0.0};
-  {
-    double glu[5], gru[5];
-    contract3(1, 0, 0, 0, glu, unit, tl);
-    contract3(1, 0, 0, 0, gru, unit, tr);
-#pragma unroll
-    for (int i = 0; i < 5; i++) {
-      F0[i] = pl[0] * t0[0] * glu[i] + pr[0] * t0[0] * gru[i];
-      F1[i] = t1[0] * (pl[0] * glu[i] + pr[0] * gru[i]);
-    }
-  }
-  MM3 m0;
-  mm3_whole(m0, p0[1], p0[2], p0[3], p0[4], g);
-  const Tab3<false> w0{m0};
   double g0u[5];
-  contract3(1, 0, 0, 0, g0u, unit, w0);
-#pragma unroll
-  for (int i = 0; i < 5; i++) F0[i] = F0[i] + p0[0] * t0[3] * g0u[i];
-  // the slopes of the two sides (w_x, w_y, w_z); c0x.. the collision
-  // state's slopes (Collision of the slopes, flux_function.cpp:845-879)
-  double c0d[3][5], aud[3][5];  // aud: <u a> (x), <v a> (y), <w a> (z) of both sides, for A
   {
-    double t1sum[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
-    double t1l[5] = {0.0, 0.0, 0.0, 0.0, 0.0}, t1r[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
-    double aul[5] = {0.0, 0.0, 0.0, 0.0, 0.0}, aur[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    MM3 m0;
+    mm3_whole(m0, p0[1], p0[2], p0[3], p0[4], g);
+    contract3(1, 0, 0, 0, g0u, unit, Tab3<false>{m0});
+  }
+  // ---- phase A: the left side
+  double glu[5], T1L[5], Alul[5], c0d[3][5];
+  {
+    MM3 ml;
+    mm3_side_from01(ml, uhl0, uhl1, pl[1], pl[2], pl[3], pl[4], g);
+    const Tab3<true> tl{ml};
+    const Tab3<false> wl{ml};
+    contract3(1, 0, 0, 0, glu, unit, tl);
+    double aul[5];
 #pragma unroll
     for (int dd = 0; dd < 3; dd++) {
-      double der[5], al[5], ar[5];
+      double der[5], al[5], tmp[5];
 #pragma unroll
       for (int v = 0; v < 5; v++) der[v] = L[5 * (dd + 1) + v];
       micro_slope3(al, der, pl, g);
+      contract3(0, 0, 0, 0, tmp, al, tl);
+#pragma unroll
+      for (int v = 0; v < 5; v++) c0d[dd][v] = pl[0] * tmp[v];
+      contract3(1 + (dd == 0), dd == 1, dd == 2, 0, tmp, al, tl);
+#pragma unroll
+      for (int v = 0; v < 5; v++) T1L[v] = dd == 0 ? tmp[v] : T1L[v] + tmp[v];
+      contract3(dd == 0, dd == 1, dd == 2, 0, tmp, al, wl);
+#pragma unroll
+      for (int v = 0; v < 5; v++) aul[v] = dd == 0 ? tmp[v] : aul[v] + tmp[v];
+    }
+    double der_AL[5], Al[5];
+#pragma unroll
+    for (int v = 0; v < 5; v++) der_AL[v] = -pl[0] * aul[v];
+    micro_slope3(Al, der_AL, pl, g);
+    contract3(1, 0, 0, 0, Alul, Al, tl);
+  }
+  // ---- phase B: the right side, folded with the left
+  double F0[5], F1[5];
+  {
+    MM3 mr;
+    mm3_side_from01(mr, uhr0, uhr1, pr[1], pr[2], pr[3], pr[4], g);
+    const Tab3<true> tr{mr};
+    const Tab3<false> wr{mr};
+    {
+      double gru[5];
+      contract3(1, 0, 0, 0, gru, unit, tr);
+#pragma unroll
+      for (int i = 0; i < 5; i++) {
+        F0[i] = pl[0] * t0[0] * glu[i] + pr[0] * t0[0] * gru[i];
+        F0[i] = F0[i] + p0[0] * t0[3] * g0u[i];
+        F1[i] = t1[0] * (pl[0] * glu[i] + pr[0] * gru[i]);
+      }
+    }
+    double T1R[5], aur[5];
+#pragma unroll
+    for (int dd = 0; dd < 3; dd++) {
+      double der[5], ar[5], tmp[5];
 #pragma unroll
       for (int v = 0; v < 5; v++) der[v] = R[5 * (dd + 1) + v];
       micro_slope3(ar, der, pr, g);
-      double l0[5], r0[5];
-      contract3(0, 0, 0, 0, l0, al, tl);
-      contract3(0, 0, 0, 0, r0, ar, tr);
+      contract3(0, 0, 0, 0, tmp, ar, tr);
 #pragma unroll
-      for (int v = 0; v < 5; v++) c0d[dd][v] = pl[0] * l0[v] + pr[0] * r0[v];
-      // t1 part: <u^2 a_x>, <u v a_y>, <u w a_z> over the half spaces
-      contract3(1 + (dd == 0), dd == 1, dd == 2, 0, l0, al, tl);
-      contract3(1 + (dd == 0), dd == 1, dd == 2, 0, r0, ar, tr);
+      for (int v = 0; v < 5; v++) c0d[dd][v] = c0d[dd][v] + pr[0] * tmp[v];
+      contract3(1 + (dd == 0), dd == 1, dd == 2, 0, tmp, ar, tr);
 #pragma unroll
-      for (int v = 0; v < 5; v++) {
-        t1l[v] = dd == 0 ? l0[v] : t1l[v] + l0[v];
-        t1r[v] = dd == 0 ? r0[v] : t1r[v] + r0[v];
-      }
-      // <u a_x>, <v a_y>, <w a_z> over the whole space (for Al, Ar)
-      contract3(dd == 0, dd == 1, dd == 2, 0, l0, al, wl);
-      contract3(dd == 0, dd == 1, dd == 2, 0, r0, ar, wr);
+      for (int v = 0; v < 5; v++) T1R[v] = dd == 0 ? tmp[v] : T1R[v] + tmp[v];
+      contract3(dd == 0, dd == 1, dd == 2, 0, tmp, ar, wr);
 #pragma unroll
-      for (int v = 0; v < 5; v++) {
-        aul[v] = dd == 0 ? l0[v] : aul[v] + l0[v];
-        aur[v] = dd == 0 ? r0[v] : aur[v] + r0[v];
-      }
+      for (int v = 0; v < 5; v++) aur[v] = dd == 0 ? tmp[v] : aur[v] + tmp[v];
     }
 #pragma unroll
     for (int i = 0; i < 5; i++) {
-      t1sum[i] = pl[0] * t1l[i] + pr[0] * t1r[i];
-      F0[i] = F0[i] + t0[1] * t1sum[i];
-      F1[i] = F1[i] + t1[1] * t1sum[i];
-      aud[0][i] = aul[i];
-      aud[1][i] = aur[i];
+      const double s1 = pl[0] * T1L[i] + pr[0] * T1R[i];
+      F0[i] = F0[i] + t0[1] * s1;
+      F1[i] = F1[i] + t1[1] * s1;
     }
-  }
-  {
-    // t2 part: Al, Ar from the compatibility condition
-    double der_AL[5], der_AR[5], Al[5], Ar[5], Alul[5], Arur[5];
+    double der_AR[5], Ar[5], Arur[5];
 #pragma unroll
-    for (int i = 0; i < 5; i++) {
-      der_AL[i] = -pl[0] * aud[0][i];
-      der_AR[i] = -pr[0] * aud[1][i];
-    }
-    micro_slope3(Al, der_AL, pl, g);
+    for (int v = 0; v < 5; v++) der_AR[v] = -pr[0] * aur[v];
     micro_slope3(Ar, der_AR, pr, g);
-    contract3(1, 0, 0, 0, Alul, Al, tl);
     contract3(1, 0, 0, 0, Arur, Ar, tr);
 #pragma unroll
     for (int i = 0; i < 5; i++) {
-      const double s = pl[0] * Alul[i] + pr[0] * Arur[i];
-      F0[i] = F0[i] + t0[2] * s;
-      F1[i] = F1[i] + t1[2] * s;
+      const double s2 = pl[0] * Alul[i] + pr[0] * Arur[i];
+      F0[i] = F0[i] + t0[2] * s2;
+      F1[i] = F1[i] + t1[2] * s2;
       F1[i] = F1[i] + p0[0] * t1[3] * g0u[i];
     }
   }
@@ -566,8 +673,11 @@ GKS3_DI void gks3d_point_dev(double* f, double* derf, double* center, const S& L
       center[15 + v] = c0d[2][v];
     }
   }
+  // ---- phase C: the collision state's slopes (t4) and time derivative (t5)
   {
-    // t4 part: the collision state's slopes; t5 part: its time derivative
+    MM3 m0;
+    mm3_whole(m0, p0[1], p0[2], p0[3], p0[4], g);
+    const Tab3<false> w0{m0};
     double t4s[5], derA0[5];
 #pragma unroll
     for (int dd = 0; dd < 3; dd++) {

@@ -6,14 +6,16 @@
 // (the CFL inputs and the blow-up flag), then per direction d = x, y, z
 //   face   one thread per cell: WENO5-AO across the d faces
 //          (WENO5_AO_splitting_3d, reconstruction.cpp:3774-3923) -> fv
-//   line   one thread per d face: the first tangential reconstruction
-//          (WENO5_AO_of_line_value, :3925-4145) -> lv
-//   flux   one thread per d face that gets a flux: the second tangential
-//          reconstruction (WENO5_AO_of_point_value, :4147-4399) and GKS3D
-//          at the 2 x 2 Gauss points, combined into the S2O4 stage sum ->
-//          xs[d] (and the stage-0 share pc[d])
+//   line   one thread per (d face, side): the first tangential
+//          reconstruction (WENO5_AO_of_line_value, :3925-4145) -> lv
+//   point  one thread per (d face that gets a flux, line, side): the second
+//          tangential reconstruction (WENO5_AO_of_point_value, :4147-4399)
+//          -> pv
+//   flux   one thread per (d face that gets a flux, Gauss point): GKS3D
+//          (flux_function.cpp:813-1163), the S2O4 stage combination summed
+//          over the face's 4 points -> xs[d] (and the stage-0 share pc[d])
 // and finally the cell update (Update_gauss, time_advance.cpp:937-998).
-// The face and line buffers are reused by the three directions.
+// The face, line and point buffers are reused by the three directions.
 //
 // Every thread's index box is the reference's loop range for that array;
 // threads are laid out with k (z) fastest so neighbouring lanes touch
@@ -177,7 +179,7 @@ __global__ void __launch_bounds__(kBlock3) k3_face(Dev3 d, Box3 b) {
   const Gas3& g = d.gas;
   const long long c = cell3(d, i, j, k);
   const long long cs = DIR == 0 ? (long long)d.ny * d.nz : (DIR == 1 ? (long long)d.nz : 1LL);
-  const double h = DIR == 0 ? d.dx : (DIR == 1 ? d.dy : d.dz);
+  const HInv h = hinv(DIR == 0 ? d.dx : (DIR == 1 ? d.dy : d.dz));
   double w[5][5];
 #pragma unroll
   for (int s = 0; s < 5; s++) {
@@ -257,7 +259,7 @@ __global__ void __launch_bounds__(kBlock3) k3_face(Dev3 d, Box3 b) {
 // in characteristic variables every field is projected on the base of the
 // centre entry's primitive variables.
 template <int NC, class ST>
-__device__ __forceinline__ void tangential_side(double (*out)[NC + 5], const ST& st, double h, int recon_var,
+__device__ __forceinline__ void tangential_side(double (*out)[NC + 5], const ST& st, const HInv& h, int recon_var,
                                                 const Gas3& g) {
   constexpr int NF = NC / 5;
   double c2[5], base[5];
@@ -334,97 +336,121 @@ struct LvStencil {
 
 // WENO5_AO_of_line_value (reconstruction.cpp:3925-3953): x faces along y
 // (h = dy), y faces along -x (h = dx, the stencil reversed), z faces along
-// y (h = dy); both lines fall back to the face value (tangential
-// derivative 0) when any of the four reconstructed states has lambda <= 0
+// y (h = dy).  One thread per (face, side); the two sides of a face are
+// neighbouring lanes, which share their order-reduction verdict: both lines
+// of both sides fall back to the face value (tangential derivative 0) when
+// any of the four reconstructed states has lambda <= 0.
 template <int DIR>
 __global__ void __launch_bounds__(kBlock3) k3_line(Dev3 d, Box3 b) {
   const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= b.count()) return;
+  const long long fi = t >> 1;
+  const int side = (int)(t & 1);
+  if (fi >= b.count()) return;  // both lanes of a pair leave together
   int i, j, k;
-  b.at(t, i, j, k);
+  b.at(fi, i, j, k);
   const long long f = face3(d, i, j, k);
   const long long step = DIR == 1 ? -(long long)(d.ny + 1) * (d.nz + 1) : (long long)(d.nz + 1);
-  const double h = DIR == 1 ? d.dx : d.dy;
-  double res[2][2][15];  // [side][line]
-#pragma unroll
-  for (int side = 0; side < 2; side++)
-    tangential_side<10>(res[side], FvStencil{d.fv, d.nf, f, step, side}, h, d.recon_var, d.gas);
-  bool red = false;
-#pragma unroll
-  for (int l = 0; l < 2; l++) red = red || reduce3(res[0][l], d.gas) || reduce3(res[1][l], d.gas);
+  const HInv h = hinv(DIR == 1 ? d.dx : d.dy);
+  double res[2][15];  // [line]
+  tangential_side<10>(res, FvStencil{d.fv, d.nf, f, step, side}, h, d.recon_var, d.gas);
+  int red = (reduce3(res[0], d.gas) || reduce3(res[1], d.gas)) ? 1 : 0;
+  red |= __shfl_xor(red, 1);
   if (red) {
 #pragma unroll
-    for (int side = 0; side < 2; side++)
+    for (int l = 0; l < 2; l++) {
 #pragma unroll
-      for (int l = 0; l < 2; l++) {
+      for (int m = 0; m < 10; m++) res[l][m] = d.fv[(side * 10 + m) * d.nf + f];
 #pragma unroll
-        for (int m = 0; m < 10; m++) res[side][l][m] = d.fv[(side * 10 + m) * d.nf + f];
-#pragma unroll
-        for (int m = 10; m < 15; m++) res[side][l][m] = 0.0;
-      }
+      for (int m = 10; m < 15; m++) res[l][m] = 0.0;
+    }
   }
 #pragma unroll
   for (int l = 0; l < 2; l++)
 #pragma unroll
-    for (int side = 0; side < 2; side++)
-#pragma unroll
-      for (int m = 0; m < 15; m++) d.lv[((l * 2 + side) * 15 + m) * d.nf + f] = res[side][l][m];
+    for (int m = 0; m < 15; m++) d.lv[((l * 2 + side) * 15 + m) * d.nf + f] = res[l][m];
 }
 
 // WENO5_AO_of_point_value (reconstruction.cpp:4147-4161) at the faces that
-// get a flux (Calculate_flux, flux_function.cpp:760-811) — x and y faces
-// along z (h = dz), z faces along -x (h = dx, reversed) — then GKS3D at the
-// two Gauss points of each line and the S2O4 stage combination of
-// Update_gauss (time_advance.cpp:867-935): stage 0 x = 0.5 f + 1/8 derf
-// (keeping 1.0 f + 1/6 derf for stage 1), stage 1 x = that + 1/3 derf.
+// get a flux (Calculate_flux, flux_function.cpp:760-811): x and y faces along
+// z (h = dz), z faces along -x (h = dx, reversed).  One thread per (face,
+// line, side); the two sides are neighbouring lanes sharing the
+// order-reduction verdict (fallback: the line value, der1z 0).
+template <int DIR>
+__global__ void __launch_bounds__(kBlock3) k3_point(Dev3 d, Box3 b) {
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long fi = t >> 2;
+  const int l = (int)((t >> 1) & 1), side = (int)(t & 1);
+  if (fi >= b.count()) return;  // the four lanes of a face leave together
+  int i, j, k;
+  b.at(fi, i, j, k);
+  const long long f = face3(d, i, j, k);
+  const long long step = DIR == 2 ? -(long long)(d.ny + 1) * (d.nz + 1) : 1LL;
+  const HInv h = hinv(DIR == 2 ? d.dx : d.dz);
+  double res[2][20];  // [point]
+  tangential_side<15>(res, LvStencil{d.lv, d.nf, f, step, l, side}, h, d.recon_var, d.gas);
+  int red = (reduce3(res[0], d.gas) || reduce3(res[1], d.gas)) ? 1 : 0;
+  red |= __shfl_xor(red, 1);
+  if (red) {
+#pragma unroll
+    for (int p = 0; p < 2; p++) {
+#pragma unroll
+      for (int m = 0; m < 15; m++) res[p][m] = d.lv[((l * 2 + side) * 15 + m) * d.nf + f];
+#pragma unroll
+      for (int m = 15; m < 20; m++) res[p][m] = 0.0;
+    }
+  }
+#pragma unroll
+  for (int p = 0; p < 2; p++)
+#pragma unroll
+    for (int m = 0; m < 20; m++) d.pv[((side * 20 + m) * d.nf + f) * 4 + 2 * l + p] = res[p][m];
+}
+
+// one side of a Gauss point in pv
+struct PvSide {
+  const double* p;  // &pv[(side * 20) * nf * 4 + f * 4 + q]
+  long long s;      // nf * 4
+  __device__ __forceinline__ double operator[](int k) const { return p[k * s]; }
+};
+
+// GKS3D at every Gauss point of the faces that get a flux, one thread per
+// (face, point), the four points of a face neighbouring lanes; the S2O4
+// stage combination of Update_gauss (time_advance.cpp:867-935) — stage 0
+// x = 0.5 f + 1/8 derf (keeping 1.0 f + 1/6 derf for stage 1), stage 1
+// x = that + 1/3 derf — summed over the four points by lane shuffles.
 template <int DIR>
 __global__ void __launch_bounds__(kBlock3) k3_flux(Dev3 d, Box3 b, int stage, double dt) {
   const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= b.count()) return;
+  const long long fi = t >> 2;
+  const int q = (int)(t & 3);
+  if (fi >= b.count()) return;  // the four lanes of a face leave together
   int i, j, k;
-  b.at(t, i, j, k);
+  b.at(fi, i, j, k);
   const long long f = face3(d, i, j, k);
-  const long long step = DIR == 2 ? -(long long)(d.ny + 1) * (d.nz + 1) : 1LL;
-  const double h = DIR == 2 ? d.dx : d.dz;
-  double* xs = d.xs[DIR];
-  double* pc = d.pc[DIR];
-  for (int l = 0; l < 2; l++) {
-    double res[2][2][20];  // [side][point]
+  const long long s4 = d.nf * 4;
+  const PvSide L{d.pv + f * 4 + q, s4}, R{d.pv + 20 * s4 + f * 4 + q, s4};
+  double fo[5], dfo[5];
+  gks3d_point_dev(fo, dfo, nullptr, L, R, dt, d.gas);
+  double x[5], share[5];
 #pragma unroll
-    for (int side = 0; side < 2; side++)
-      tangential_side<15>(res[side], LvStencil{d.lv, d.nf, f, step, l, side}, h, d.recon_var, d.gas);
-    bool red = false;
-#pragma unroll
-    for (int p = 0; p < 2; p++) red = red || reduce3(res[0][p], d.gas) || reduce3(res[1][p], d.gas);
-    if (red) {
-#pragma unroll
-      for (int side = 0; side < 2; side++)
-#pragma unroll
-        for (int p = 0; p < 2; p++) {
-#pragma unroll
-          for (int m = 0; m < 15; m++) res[side][p][m] = d.lv[((l * 2 + side) * 15 + m) * d.nf + f];
-#pragma unroll
-          for (int m = 15; m < 20; m++) res[side][p][m] = 0.0;
-        }
+  for (int v = 0; v < 5; v++) {
+    if (stage == 0) {
+      x[v] = 0.0 + (0.5 * fo[v] + 1.0 / 8.0 * dfo[v]);
+      share[v] = 0.0 + (1.0 * fo[v] + 1.0 / 6.0 * dfo[v]);
+    } else {
+      x[v] = 0.0 * fo[v] + 1.0 / 3.0 * dfo[v];
+      share[v] = 0.0;
     }
-    double x[5] = {0.0, 0.0, 0.0, 0.0, 0.0}, share[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
-#pragma unroll
-    for (int p = 0; p < 2; p++) {
-      double fo[5], dfo[5];
-      gks3d_point_dev(fo, dfo, nullptr, res[0][p], res[1][p], dt, d.gas);
-#pragma unroll
-      for (int v = 0; v < 5; v++) {
-        if (stage == 0) {
-          x[v] += 0.0 + (0.5 * fo[v] + 1.0 / 8.0 * dfo[v]);
-          share[v] += 0.0 + (1.0 * fo[v] + 1.0 / 6.0 * dfo[v]);
-        } else {
-          x[v] += 0.0 * fo[v] + 1.0 / 3.0 * dfo[v];
-        }
-      }
-    }
+    x[v] += __shfl_xor(x[v], 1);
+    x[v] += __shfl_xor(x[v], 2);
+    share[v] += __shfl_xor(share[v], 1);
+    share[v] += __shfl_xor(share[v], 2);
+  }
+  if (q == 0) {
+    double* xs = d.xs[DIR];
+    double* pc = d.pc[DIR];
 #pragma unroll
     for (int v = 0; v < 5; v++) {
-      const long long o = (long long)(l * 5 + v) * d.nf + f;
+      const long long o = (long long)v * d.nf + f;
       if (stage == 0) {
         xs[o] = x[v];
         pc[o] = share[v];
@@ -436,8 +462,8 @@ __global__ void __launch_bounds__(kBlock3) k3_flux(Dev3 d, Box3 b, int stage, do
 }
 
 // the cell update of Update_gauss (time_advance.cpp:937-998): U = Uold +
-// sum over the Gauss points (weights 1/4) of the face sums, the y / z face
-// frames rotated back
+// the Gauss-point sums (weights 1/4) of the faces, the y / z face frames
+// rotated back
 __global__ void __launch_bounds__(kBlock3) k3_update(Dev3 d, Box3 b) {
   const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= b.count()) return;
@@ -447,23 +473,20 @@ __global__ void __launch_bounds__(kBlock3) k3_update(Dev3 d, Box3 b) {
   const long long fx = face3(d, i + 1, j, k), fy = face3(d, i, j + 1, k), fz = face3(d, i, j, k + 1);
   const double odx = 1 / d.dx, ody = 1 / d.dy, odz = 1 / d.dz;
   const double w = 0.25;
-  double tot[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+  auto X = [&](int dir, long long f, int v) { return d.xs[dir][(long long)v * d.nf + f]; };
+  double tot[5];
+  tot[0] = w * odz * (X(2, fc, 0) - X(2, fz, 0));
+  tot[1] = w * odz * (-X(2, fc, 3) + X(2, fz, 3));
+  tot[2] = w * odz * (X(2, fc, 2) - X(2, fz, 2));
+  tot[3] = w * odz * (X(2, fc, 1) - X(2, fz, 1));
+  tot[4] = w * odz * (X(2, fc, 4) - X(2, fz, 4));
+  tot[0] += w * ody * (X(1, fc, 0) - X(1, fy, 0));
+  tot[1] += w * ody * (-X(1, fc, 2) + X(1, fy, 2));
+  tot[2] += w * ody * (X(1, fc, 1) - X(1, fy, 1));
+  tot[3] += w * ody * (X(1, fc, 3) - X(1, fy, 3));
+  tot[4] += w * ody * (X(1, fc, 4) - X(1, fy, 4));
 #pragma unroll
-  for (int l = 0; l < 2; l++) {
-    auto X = [&](int dir, long long f, int v) { return d.xs[dir][(long long)(l * 5 + v) * d.nf + f]; };
-    tot[0] += w * odz * (X(2, fc, 0) - X(2, fz, 0));
-    tot[1] += w * odz * (-X(2, fc, 3) + X(2, fz, 3));
-    tot[2] += w * odz * (X(2, fc, 2) - X(2, fz, 2));
-    tot[3] += w * odz * (X(2, fc, 1) - X(2, fz, 1));
-    tot[4] += w * odz * (X(2, fc, 4) - X(2, fz, 4));
-    tot[0] += w * ody * (X(1, fc, 0) - X(1, fy, 0));
-    tot[1] += w * ody * (-X(1, fc, 2) + X(1, fy, 2));
-    tot[2] += w * ody * (X(1, fc, 1) - X(1, fy, 1));
-    tot[3] += w * ody * (X(1, fc, 3) - X(1, fy, 3));
-    tot[4] += w * ody * (X(1, fc, 4) - X(1, fy, 4));
-#pragma unroll
-    for (int v = 0; v < 5; v++) tot[v] += w * odx * (X(0, fc, v) - X(0, fx, v));
-  }
+  for (int v = 0; v < 5; v++) tot[v] += w * odx * (X(0, fc, v) - X(0, fx, v));
 #pragma unroll
   for (int v = 0; v < 5; v++) d.U[v * d.nc + c] = d.Uold[v * d.nc + c] + tot[v];
 }
@@ -496,6 +519,9 @@ template __global__ void k3_face<2>(Dev3, Box3);
 template __global__ void k3_line<0>(Dev3, Box3);
 template __global__ void k3_line<1>(Dev3, Box3);
 template __global__ void k3_line<2>(Dev3, Box3);
+template __global__ void k3_point<0>(Dev3, Box3);
+template __global__ void k3_point<1>(Dev3, Box3);
+template __global__ void k3_point<2>(Dev3, Box3);
 template __global__ void k3_flux<0>(Dev3, Box3, int, double);
 template __global__ void k3_flux<1>(Dev3, Box3, int, double);
 template __global__ void k3_flux<2>(Dev3, Box3, int, double);
@@ -542,8 +568,11 @@ static hipError_t launch_direction_t(const Dev3& d, int stage, double dt, hipStr
   // the faces that get a flux
   const Box3 bx = make_box(G, G + d.NX + (DIR == 0), G, G + d.NY + (DIR == 1), G, G + d.NZ + (DIR == 2));
   if (box_count(bf) > 0) k3_face<DIR><<<blocks_for(box_count(bf)), kBlock3, 0, s>>>(d, bf);
-  if (box_count(bl) > 0) k3_line<DIR><<<blocks_for(box_count(bl)), kBlock3, 0, s>>>(d, bl);
-  if (box_count(bx) > 0) k3_flux<DIR><<<blocks_for(box_count(bx)), kBlock3, 0, s>>>(d, bx, stage, dt);
+  if (box_count(bl) > 0) k3_line<DIR><<<blocks_for(2 * box_count(bl)), kBlock3, 0, s>>>(d, bl);
+  if (box_count(bx) > 0) {
+    k3_point<DIR><<<blocks_for(4 * box_count(bx)), kBlock3, 0, s>>>(d, bx);
+    k3_flux<DIR><<<blocks_for(4 * box_count(bx)), kBlock3, 0, s>>>(d, bx, stage, dt);
+  }
   return hipGetLastError();
 }
 static hipError_t launch_direction(const Dev3& d, int dir, int stage, double dt, hipStream_t s) {

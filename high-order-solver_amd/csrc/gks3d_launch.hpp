@@ -20,9 +20,11 @@ struct Dev3 {
   double* fv;     // [2 side][10][nf] face values of the direction in work:
                   //   side 0 the left cell's, side 1 the right cell's (convar, der1x)
   double* lv;     // [2 line][2 side][15][nf] line values (convar, der1x, der1y)
-  double* xs[3];  // [2 line][5][nf] per direction: the stage combination of
-                  //   the line's two Gauss points, summed
-  double* pc[3];  // [2 line][5][nf] per direction: the stage-0 share of stage 1
+  double* pv;     // [2 side][20][nf][4 point] Gauss-point values (convar, der1x,
+                  //   der1y, der1z), point q = 2 line + p fastest
+  double* xs[3];  // [5][nf] per direction: the stage combination of the
+                  //   face's four Gauss points, summed
+  double* pc[3];  // [5][nf] per direction: the stage-0 share of stage 1
   double* part;   // [3][kParts3] partials of the primitive-variable sweep
   double* stat;   // [4]: maximum speed, minimum density, blow-up flag
 };
