@@ -590,7 +590,7 @@ void gks3d_lambda_fixed(Flux3d& flux, Recon3d& re, double area, double dt) {
   double L[20], R[20];
   pack_point(L, re.left);
   pack_point(R, re.right);
-  orc3d::gks3d_point(flux.f, flux.derf, nullptr, L, R, dt, g_gas3);
+  orc3d::gks3d_point(flux.f, flux.derf, static_cast<double*>(nullptr), L, R, dt, g_gas3);
 }
 
 struct RefCtx3 {

@@ -35,6 +35,9 @@
 // Gauss point q = 2 * line + point (SetGuassPoint_3D, fluid_mesh.cpp:80-88;
 // weights 1/4).
 #pragma once
+// The leaves are templates on the number type (double here; a counting
+// scalar in oracle/flopcount3d.cpp); math calls are unqualified so that
+// argument-dependent lookup finds the counting overloads.
 #include <cmath>
 #include <cstdint>
 #include <cstring>
@@ -50,7 +53,8 @@ struct Gas3 {
 
 // ------------------------------------------------------------ gas algebra
 // Convar_to_primvar_3D / Pressure: basic_function.cpp:819-826, 1162-1167
-inline void convar_to_primvar(double* p, const double* c, const Gas3& g) {
+template <class Real>
+inline void convar_to_primvar(Real* p, const Real* c, const Gas3& g) {
   p[0] = c[0];
   p[1] = c[1] / c[0];
   p[2] = c[2] / c[0];
@@ -58,11 +62,13 @@ inline void convar_to_primvar(double* p, const double* c, const Gas3& g) {
   p[4] = (g.gamma - 1.0) * (c[4] - 0.5 * c[1] * c[1] / c[0] - 0.5 * c[2] * c[2] / c[0] - 0.5 * c[3] * c[3] / c[0]);
 }
 // Lambda (3D): basic_function.cpp:1179-1182
-inline double lambda3(double rho, double u, double v, double w, double rhoE, const Gas3& g) {
+template <class Real>
+inline Real lambda3(Real rho, Real u, Real v, Real w, Real rhoE, const Gas3& g) {
   return (g.K + 3.0) * 0.25 * (rho / (rhoE - 0.5 * rho * (u * u + v * v + w * w)));
 }
 // (rho, u, v, w, lambda), the state GKS3D's moments need
-inline void convar_to_ulambda(double* p, const double* c, const Gas3& g) {
+template <class Real>
+inline void convar_to_ulambda(Real* p, const Real* c, const Gas3& g) {
   p[0] = c[0];
   p[1] = c[1] / c[0];
   p[2] = c[2] / c[0];
@@ -70,7 +76,8 @@ inline void convar_to_ulambda(double* p, const double* c, const Gas3& g) {
   p[4] = lambda3(c[0], p[1], p[2], p[3], c[4], g);
 }
 // Primvar_to_convar_3D: basic_function.cpp:859-866, Q_densityE :1131-1135
-inline void primvar_to_convar(double* c, const double* p, const Gas3& g) {
+template <class Real>
+inline void primvar_to_convar(Real* c, const Real* p, const Gas3& g) {
   c[0] = p[0];
   c[1] = p[0] * p[1];
   c[2] = p[0] * p[2];
@@ -78,14 +85,16 @@ inline void primvar_to_convar(double* c, const double* p, const Gas3& g) {
   c[4] = p[0] * (p[4] / p[0] / (g.gamma - 1) + 0.5 * (p[1] * p[1] + p[2] * p[2] + p[3] * p[3]));
 }
 // Ydirection / Zdirection: basic_function.cpp:1208-1224
-inline void ydir(double* o, const double* f) {
+template <class Real>
+inline void ydir(Real* o, const Real* f) {
   o[0] = f[0];
   o[1] = f[2];
   o[2] = -f[1];
   o[3] = f[3];
   o[4] = f[4];
 }
-inline void zdir(double* o, const double* f) {
+template <class Real>
+inline void zdir(Real* o, const Real* f) {
   o[0] = f[0];
   o[1] = f[3];
   o[2] = f[2];
@@ -93,14 +102,15 @@ inline void zdir(double* o, const double* f) {
   o[4] = f[4];
 }
 // Char_base_3D: basic_function.cpp:994-1035
-inline void char_base(double* s, const double* prim, const Gas3& g) {
+template <class Real>
+inline void char_base(Real* s, const Real* prim, const Gas3& g) {
   const double Gamma = g.gamma;
-  double c = sqrt(Gamma * prim[4] / prim[0]);
-  double overc = 1.0 / c;
-  double alfa = (Gamma - 1.) * 0.5 * overc * overc;
-  double u = prim[1], v = prim[2], w = prim[3];
-  double r_over = 1 / (Gamma - 1);
-  double H = 0.5 * (u * u + v * v + w * w) + 0.5 / alfa;
+  Real c = sqrt(Gamma * prim[4] / prim[0]);
+  Real overc = 1.0 / c;
+  Real alfa = (Gamma - 1.) * 0.5 * overc * overc;
+  Real u = prim[1], v = prim[2], w = prim[3];
+  const double r_over = 1 / (Gamma - 1);
+  Real H = 0.5 * (u * u + v * v + w * w) + 0.5 / alfa;
   s[0] = H + c * r_over * (u - c);
   s[1] = -(u + c * r_over);
   s[2] = -v;
@@ -129,19 +139,21 @@ inline void char_base(double* s, const double* prim, const Gas3& g) {
   for (int i = 0; i < 25; i++) s[i] = s[i] * alfa;
 }
 // Convar_to_char_3D: basic_function.cpp:1037-1050
-inline void to_char(double* ch, const double* s, const double* c) {
+template <class Real>
+inline void to_char(Real* ch, const Real* s, const Real* c) {
   for (int i = 0; i < 5; i++) ch[i] = 0.0;
   for (int i = 0; i < 5; i++)
     for (int j = 0; j < 5; j++) ch[i] = ch[i] + s[i * 5 + j] * c[j];
 }
 // Char_to_convar_3D: basic_function.cpp:1052-1093
-inline void from_char(double* c, const double* prim, const double* ch, const Gas3& g) {
+template <class Real>
+inline void from_char(Real* c, const Real* prim, const Real* ch, const Gas3& g) {
   const double Gamma = g.gamma;
-  double cs = sqrt(Gamma * prim[4] / prim[0]);
-  double alfa = (Gamma - 1.) / (2.0 * cs * cs);
-  double u = prim[1], v = prim[2], w = prim[3];
-  double H = 0.5 * (u * u + v * v + w * w) + 0.5 / alfa;
-  double s[5][5] = {{1., 1.0, 0.0, 0.0, 1.0},
+  Real cs = sqrt(Gamma * prim[4] / prim[0]);
+  Real alfa = (Gamma - 1.) / (2.0 * cs * cs);
+  Real u = prim[1], v = prim[2], w = prim[3];
+  Real H = 0.5 * (u * u + v * v + w * w) + 0.5 / alfa;
+  Real s[5][5] = {{1., 1.0, 0.0, 0.0, 1.0},
                     {u - cs, u, 0, 0, u + cs},
                     {v, v, 1, 0, v},
                     {w, w, 0, 1, w},
@@ -152,33 +164,41 @@ inline void from_char(double* c, const double* prim, const double* ch, const Gas
   }
 }
 // Check_Order_Reduce_by_Lambda_3D: reconstruction.cpp:3755-3771
-inline bool reduce_order(const double* c, const Gas3& g) {
-  double l = lambda3(c[0], c[1] / c[0], c[2] / c[0], c[3] / c[0], c[4], g);
-  return l <= 0.0 || std::isnan(l);
+template <class Real>
+inline bool reduce_order(const Real* c, const Gas3& g) {
+  Real l = lambda3(c[0], c[1] / c[0], c[2] / c[0], c[3] / c[0], c[4], g);
+  return l <= 0.0 || l != l;
 }
 
 // ------------------------------------------------------------ WENO5-AO
+// pow(x, 2) of the reference: the square, correctly rounded (x * x)
+template <class Real>
+inline Real sq(const Real& x) {
+  return x * x;
+}
 // weno_5th_ao_left / right (wenotype wenoz), reconstruction.cpp:161-358:
 // value and first derivative at the cell's left / right face
-inline void weno5_weights_nodf(double* fw, double wn2, double wn1, double w0, double wp1, double wp2, double eps) {
+template <class Real>
+inline void weno5_weights_nodf(Real* fw, Real wn2, Real wn1, Real w0, Real wp1, Real wp2, double eps) {
   const double dhi = 0.85, dlo = 0.85;
-  double beta[4], d[4], ww[4], alpha[4];
+  Real beta[4], ww[4], alpha[4];
+  double d[4];
   d[0] = (1.0 - dhi) * (1.0 - dlo) / 2.0;
   d[1] = (1.0 - dhi) * dlo;
   d[2] = (1.0 - dhi) * (1.0 - dlo) / 2.0;
   d[3] = dhi;
-  beta[0] = 13.0 / 12.0 * pow((wn2 - 2.0 * wn1 + w0), 2) + 0.25 * pow((wn2 - 4.0 * wn1 + 3.0 * w0), 2);
-  beta[1] = 13.0 / 12.0 * pow((wn1 - 2.0 * w0 + wp1), 2) + 0.25 * pow((wn1 - wp1), 2);
-  beta[2] = 13.0 / 12.0 * pow((w0 - 2.0 * wp1 + wp2), 2) + 0.25 * pow((3.0 * w0 - 4.0 * wp1 + wp2), 2);
+  beta[0] = 13.0 / 12.0 * sq((wn2 - 2.0 * wn1 + w0)) + 0.25 * sq((wn2 - 4.0 * wn1 + 3.0 * w0));
+  beta[1] = 13.0 / 12.0 * sq((wn1 - 2.0 * w0 + wp1)) + 0.25 * sq((wn1 - wp1));
+  beta[2] = 13.0 / 12.0 * sq((w0 - 2.0 * wp1 + wp2)) + 0.25 * sq((3.0 * w0 - 4.0 * wp1 + wp2));
   beta[3] = (1.0 / 5040.0) * (231153.0 * w0 * w0 + 104963.0 * wn1 * wn1 + 6908.0 * wn2 * wn2 -
                               38947.0 * wn2 * wp1 + 104963.0 * wp1 * wp1 +
                               wn1 * (-51001.0 * wn2 + 179098.0 * wp1 - 38947.0 * wp2) -
                               3.0 * w0 * (99692.0 * wn1 - 22641.0 * wn2 + 99692.0 * wp1 - 22641.0 * wp2) +
                               8209.0 * wn2 * wp2 - 51001.0 * wp1 * wp2 + 6908.0 * wp2 * wp2);
-  double tau5 = 1.0 / 3.0 * (std::abs(beta[3] - beta[0]) + std::abs(beta[3] - beta[1]) + std::abs(beta[3] - beta[2]));
-  double sum_alpha = 0.0;
+  Real tau5 = 1.0 / 3.0 * (fabs(beta[3] - beta[0]) + fabs(beta[3] - beta[1]) + fabs(beta[3] - beta[2]));
+  Real sum_alpha = 0.0;
   for (int i = 0; i < 4; i++) {
-    double global_div = tau5 / (beta[i] + eps);
+    Real global_div = tau5 / (beta[i] + eps);
     alpha[i] = d[i] * (1.0 + global_div * global_div);
     sum_alpha += alpha[i];
   }
@@ -186,8 +206,9 @@ inline void weno5_weights_nodf(double* fw, double wn2, double wn1, double w0, do
   fw[3] = ww[3] / d[3];
   for (int k = 0; k < 3; k++) fw[k] = ww[k] - ww[3] / d[3] * d[k];
 }
-inline void weno5_left(double& var, double& der1, double wn2, double wn1, double w0, double wp1, double wp2, double h) {
-  double fw[4], p[4], px[4];
+template <class Real>
+inline void weno5_left(Real& var, Real& der1, Real wn2, Real wn1, Real w0, Real wp1, Real wp2, double h) {
+  Real fw[4], p[4], px[4];
   weno5_weights_nodf(fw, wn2, wn1, w0, wp1, wp2, 1e-10);
   p[0] = -1.0 / 6.0 * wn2 + 5.0 / 6.0 * wn1 + 1.0 / 3.0 * w0;
   p[1] = 1.0 / 3.0 * wn1 + 5.0 / 6.0 * w0 - 1.0 / 6.0 * wp1;
@@ -204,8 +225,9 @@ inline void weno5_left(double& var, double& der1, double wn2, double wn1, double
     der1 += fw[k] * px[k];
   }
 }
-inline void weno5_right(double& var, double& der1, double wn2, double wn1, double w0, double wp1, double wp2, double h) {
-  double fw[4], p[4], px[4];
+template <class Real>
+inline void weno5_right(Real& var, Real& der1, Real wn2, Real wn1, Real w0, Real wp1, Real wp2, double h) {
+  Real fw[4], p[4], px[4];
   weno5_weights_nodf(fw, wn2, wn1, w0, wp1, wp2, 1e-10);
   p[0] = 1.0 / 3.0 * wn2 - 7.0 / 6.0 * wn1 + 11.0 / 6.0 * w0;
   p[1] = -1.0 / 6.0 * wn1 + 5.0 / 6.0 * w0 + 1.0 / 3.0 * wp1;
@@ -224,15 +246,16 @@ inline void weno5_right(double& var, double& der1, double wn2, double wn1, doubl
 }
 // weno_5th_ao_2gauss (no DF), reconstruction.cpp:2393-2551: values (and,
 // with order 1, first derivatives) at the two Gauss points -+1/(2 sqrt 3)
-inline void weno5_2gauss(double& g1, double& g1x, double& g2, double& g2x, double wn2, double wn1, double w0,
-                         double wp1, double wp2, double h, int order) {
-  double fw[4], p[4], px[4];
+template <class Real>
+inline void weno5_2gauss(Real& g1, Real& g1x, Real& g2, Real& g2x, Real wn2, Real wn1, Real w0,
+                         Real wp1, Real wp2, double h, int order) {
+  Real fw[4], p[4], px[4];
   weno5_weights_nodf(fw, wn2, wn1, w0, wp1, wp2, 1e-6);
   g1 = 0;
   g1x = 0;
   g2 = 0;
   g2x = 0;
-  double sqrt3 = 1.732050807568877294;
+  const double sqrt3 = 1.732050807568877294;
   p[0] = w0 - (sqrt3 * w0) / 4 + (4 * wn1 - wn2) / (4 * sqrt3);
   p[1] = w0 + (wn1 - wp1) / (4 * sqrt3);
   p[2] = (1.0 / 12.0) * (3 * (4 + sqrt3) * w0 + sqrt3 * (-4 * wp1 + wp2));
@@ -270,8 +293,9 @@ inline void weno5_2gauss(double& g1, double& g1x, double& g2, double& g2x, doubl
 // WENO5_AO (3D, face values), reconstruction.cpp:3803-3923: w[5][5] the
 // stencil in the face frame; lo = value + normal derivative at the cell's
 // left face, ro at its right face (convar[5], der1x[5] each)
-inline void weno5_face(double* lo, double* ro, const double (*w)[5], double h, int recon_var, const Gas3& g) {
-  double base_left[5], base_right[5], pn1[5], p0[5], pp1[5];
+template <class Real>
+inline void weno5_face(Real* lo, Real* ro, const Real (*w)[5], double h, int recon_var, const Gas3& g) {
+  Real base_left[5], base_right[5], pn1[5], p0[5], pp1[5];
   convar_to_primvar(pn1, w[1], g);
   convar_to_primvar(p0, w[2], g);
   convar_to_primvar(pp1, w[3], g);
@@ -280,13 +304,13 @@ inline void weno5_face(double* lo, double* ro, const double (*w)[5], double h, i
     base_right[i] = 0.5 * (pp1[i] + p0[i]);
   }
   for (int side = 0; side < 2; side++) {
-    const double* base = side == 0 ? base_left : base_right;
-    double r[5][5], var[5], der1[5];
+    const Real* base = side == 0 ? base_left : base_right;
+    Real r[5][5], var[5], der1[5];
     if (recon_var == 0) {
       for (int s = 0; s < 5; s++)
         for (int i = 0; i < 5; i++) r[s][i] = w[s][i];
     } else {
-      double sm[25];
+      Real sm[25];
       char_base(sm, base, g);
       for (int s = 0; s < 5; s++) to_char(r[s], sm, w[s]);
     }
@@ -294,7 +318,7 @@ inline void weno5_face(double* lo, double* ro, const double (*w)[5], double h, i
       if (side == 0) weno5_left(var[i], der1[i], r[0][i], r[1][i], r[2][i], r[3][i], r[4][i], h);
       else weno5_right(var[i], der1[i], r[0][i], r[1][i], r[2][i], r[3][i], r[4][i], h);
     }
-    double* out = side == 0 ? lo : ro;
+    Real* out = side == 0 ? lo : ro;
     if (recon_var == 0) {
       for (int i = 0; i < 5; i++) {
         out[i] = var[i];
@@ -323,28 +347,28 @@ inline void weno5_face(double* lo, double* ro, const double (*w)[5], double h, i
 // with order 1 (values + the new tangential derivative), the others with
 // order 0.  out[2][NC + 5]: per Gauss point convar, the carried derivatives,
 // then the new derivative.
-template <int NC>
-inline void weno5_tangential_side(double (*out)[NC + 5], const double (*s)[NC], double h, int recon_var,
+template <int NC, class Real>
+inline void weno5_tangential_side(Real (*out)[NC + 5], const Real (*s)[NC], double h, int recon_var,
                                   const Gas3& g) {
   constexpr int NF = NC / 5;
-  double base[5];
+  Real base[5];
   convar_to_primvar(base, s[2], g);
-  double sm[25];
+  Real sm[25];
   if (recon_var != 0) char_base(sm, base, g);
   for (int fld = 0; fld < NF; fld++) {
-    double r[5][5];
+    Real r[5][5];
     for (int q = 0; q < 5; q++)
       for (int i = 0; i < 5; i++) {
         if (recon_var == 0) r[q][i] = s[q][5 * fld + i];
       }
     if (recon_var != 0) {
       for (int q = 0; q < 5; q++) {
-        double c[5];
+        Real c[5];
         for (int i = 0; i < 5; i++) c[i] = s[q][5 * fld + i];
         to_char(r[q], sm, c);
       }
     }
-    double var[2][5], der[2][5];
+    Real var[2][5], der[2][5];
     for (int i = 0; i < 5; i++)
       weno5_2gauss(var[0][i], der[0][i], var[1][i], der[1][i], r[0][i], r[1][i], r[2][i], r[3][i], r[4][i], h,
                    fld == 0 ? 1 : 0);
@@ -364,18 +388,22 @@ inline void weno5_tangential_side(double (*out)[NC + 5], const double (*s)[NC], 
 
 // ------------------------------------------------------------ GKS3D
 // MMDF3d_left / MMDF3d_right / MMDF3d: basic_function.cpp:189-393
+template <class Real>
 struct MM3 {
-  double uh[7], uw[7], vw[6], ww[6], xi2, xi4;  // uh: u+ (left) or u- (right)
+  Real uh[7], uw[7], vw[6], ww[6], xi2, xi4;  // uh: u+ (left) or u- (right)
 };
-inline double alpha_fn(double lambda, double u) { return erfc(sqrt(lambda) * u); }
-inline double beta_fn(double lambda, double u) {
-  double pi = 3.14159265358979323846;
+template <class Real>
+inline Real alpha_fn(Real lambda, Real u) { return erfc(sqrt(lambda) * u); }
+template <class Real>
+inline Real beta_fn(Real lambda, Real u) {
+  const double pi = 3.14159265358979323846;
   return exp(-lambda * u * u) / sqrt(pi * lambda);
 }
-inline void mmdf3_side(MM3& m, int side, double u, double v, double w, double lambda, const Gas3& g) {
+template <class Real>
+inline void mmdf3_side(MM3<Real>& m, int side, Real u, Real v, Real w, Real lambda, const Gas3& g) {
   m.uw[0] = 1;
   m.uw[1] = u;
-  double ol = 1.0 / lambda;
+  Real ol = 1.0 / lambda;
   if (side == 0) {
     m.uh[0] = 0.5 * alpha_fn(lambda, -u);
     m.uh[1] = u * m.uh[0] + 0.5 * beta_fn(lambda, u);
@@ -397,7 +425,8 @@ inline void mmdf3_side(MM3& m, int side, double u, double v, double w, double la
   m.xi4 = 0.25 * (g.K * g.K + 2 * g.K) * ol * ol;
 }
 // whole-space moments of MMDF3d (divisions by lambda, as written there)
-inline void mmdf3_whole(MM3& m, double u, double v, double w, double lambda, const Gas3& g) {
+template <class Real>
+inline void mmdf3_whole(MM3<Real>& m, Real u, Real v, Real w, Real lambda, const Gas3& g) {
   m.uw[0] = 1;
   m.uw[1] = u;
   for (int i = 2; i <= 6; i++) m.uw[i] = u * m.uw[i - 1] + 0.5 * (i - 1) / lambda * m.uw[i - 2];
@@ -414,22 +443,25 @@ inline void mmdf3_whole(MM3& m, double u, double v, double w, double lambda, con
   for (int i = 0; i < 7; i++) m.uh[i] = 0.0;
 }
 // upvwxi / unvwxi (half) and uvwxi (whole) table entries
+template <class Real>
 struct TabH {
-  const MM3& m;
-  double operator()(int i, int j, int k, int x) const {
+  const MM3<Real>& m;
+  Real operator()(int i, int j, int k, int x) const {
     return x == 0 ? m.uh[i] * m.vw[j] * m.ww[k]
                   : (x == 1 ? m.uh[i] * m.vw[j] * m.ww[k] * m.xi2 : m.uh[i] * m.vw[j] * m.ww[k] * m.xi4);
   }
 };
+template <class Real>
 struct TabW3 {
-  const MM3& m;
-  double operator()(int i, int j, int k, int x) const {
+  const MM3<Real>& m;
+  Real operator()(int i, int j, int k, int x) const {
     return x == 0 ? m.uw[i] * m.vw[j] * m.ww[k]
                   : (x == 1 ? m.uw[i] * m.vw[j] * m.ww[k] * m.xi2 : m.uw[i] * m.vw[j] * m.ww[k] * m.xi4);
   }
 };
 // GL_address / GR_address / G_address (3D): basic_function.h:225-567
-template <class T> inline void contract3(int u, int v, int w, int x, double* psi, const double* a, const T& m) {
+template <class T, class Real>
+inline void contract3(int u, int v, int w, int x, Real* psi, const Real* a, const T& m) {
   psi[0] = a[0] * m(u, v, w, x) + a[1] * m(u + 1, v, w, x) + a[2] * m(u, v + 1, w, x) + a[3] * m(u, v, w + 1, x) +
            a[4] * 0.5 * (m(u + 2, v, w, x) + m(u, v + 2, w, x) + m(u, v, w + 2, x) + m(u, v, w, x + 1));
   psi[1] = a[0] * m(u + 1, v, w, x) + a[1] * m(u + 2, v, w, x) + a[2] * m(u + 1, v + 1, w, x) +
@@ -451,14 +483,15 @@ template <class T> inline void contract3(int u, int v, int w, int x, double* psi
                        2 * m(u, v + 2, w + 2, x) + 2 * m(u, v + 2, w, x + 1) + 2 * m(u, v, w + 2, x + 1)));
 }
 // A (3D): basic_function.cpp:585-601; prim = (rho, u, v, w, lambda)
-inline void micro_slope3(double* a, const double* der, const double* prim, const Gas3& g) {
-  double density = prim[0], u = prim[1], v = prim[2], w = prim[3], lambda = prim[4];
-  double overden = 1.0 / density;
-  double overlambda = 1.0 / lambda;
-  double A = 2 * der[4] * overden - (u * u + v * v + w * w + 0.5 * (g.K + 3) * overlambda) * der[0] * overden;
-  double B = (der[1] - u * der[0]) * overden;
-  double C = (der[2] - v * der[0]) * overden;
-  double D = (der[3] - w * der[0]) * overden;
+template <class Real>
+inline void micro_slope3(Real* a, const Real* der, const Real* prim, const Gas3& g) {
+  Real density = prim[0], u = prim[1], v = prim[2], w = prim[3], lambda = prim[4];
+  Real overden = 1.0 / density;
+  Real overlambda = 1.0 / lambda;
+  Real A = 2 * der[4] * overden - (u * u + v * v + w * w + 0.5 * (g.K + 3) * overlambda) * der[0] * overden;
+  Real B = (der[1] - u * der[0]) * overden;
+  Real C = (der[2] - v * der[0]) * overden;
+  Real D = (der[3] - w * der[0]) * overden;
   a[4] = 4 * lambda * lambda / (g.K + 3) * (A - 2 * u * B - 2 * v * C - 2 * w * D);
   a[3] = 2 * lambda * D - w * a[4];
   a[2] = 2 * lambda * C - v * a[4];
@@ -466,27 +499,30 @@ inline void micro_slope3(double* a, const double* der, const double* prim, const
   a[0] = der[0] * overden - u * a[1] - v * a[2] - w * a[3] - 0.5 * a[4] * (u * u + v * v + w * w + 0.5 * (g.K + 3) * overlambda);
 }
 // Get_Tau_NS / Get_Tau: basic_function.cpp:638-708
-inline double tau_ns3(double rho0, double l0, const Gas3& g) {
+template <class Real>
+inline Real tau_ns3(Real rho0, Real l0, const Gas3& g) {
   if (g.tau_type == 0) return 0.0;
   if (g.mu > 0.0) return 2.0 * g.mu * l0 / rho0;
   if (g.nu > 0.0) return 2 * g.nu * l0;
   return 0.0;
 }
-inline double tau_num3(double rl, double rr, double r0, double ll, double lr, double l0, double dt, const Gas3& g) {
+template <class Real>
+inline Real tau_num3(Real rl, Real rr, Real r0, Real ll, Real lr, Real l0, double dt, const Gas3& g) {
   if (g.tau_type == 0) {
     if (g.c1 <= 0 && g.c2 <= 0) return 0.0;
-    double C = g.c2 * std::abs(rl / ll - rr / lr) / std::abs(rl / ll + rr / lr);
+    Real C = g.c2 * fabs(rl / ll - rr / lr) / fabs(rl / ll + rr / lr);
     return g.c1 * dt + dt * C;
   }
-  double tau_n = g.c2 * std::abs(rl / ll - rr / lr) / std::abs(rl / ll + rr / lr) * dt;
+  Real tau_n = g.c2 * fabs(rl / ll - rr / lr) / fabs(rl / ll + rr / lr) * dt;
   if (tau_n != tau_n) tau_n = 0.0;
   if ((g.mu > 0.0 && g.nu > 0.0) || (g.mu < 0.0 && g.nu < 0.0)) return 0.0;
   if (g.mu > 0.0) return tau_n + 2.0 * g.mu * l0 / r0;
   if (g.nu > 0.0) return tau_n + 2 * g.nu * l0;
   return 0.0;
 }
-inline void time_coe3(double* t, double tau, double tn, double dt) {
-  double eta = exp(-dt / tn);
+template <class Real>
+inline void time_coe3(Real* t, Real tau, Real tn, double dt) {
+  Real eta = exp(-dt / tn);
   t[0] = tn * (1 - eta);
   t[1] = tn * (eta * (dt + tn) - tn) + tau * tn * (eta - 1);
   t[2] = tau * tn * (eta - 1);
@@ -498,34 +534,35 @@ inline void time_coe3(double* t, double tau, double tn, double dt) {
 // GKS3D (gks2nd_3d, S2O4): one Gauss point.  L, R: convar[5], der1x[5],
 // der1y[5], der1z[5] in the face frame.  center (optional): the collision
 // state and its three derivatives (20 values).
-inline void gks3d_point(double* f, double* derf, double* center, const double* L, const double* R, double dt,
+template <class Real>
+inline void gks3d_point(Real* f, Real* derf, Real* center, const Real* L, const Real* R, double dt,
                         const Gas3& g) {
-  double pl[5], pr[5], p0[5];
+  Real pl[5], pr[5], p0[5];
   convar_to_ulambda(pl, L, g);
   convar_to_ulambda(pr, R, g);
-  MM3 ml, mr;
+  MM3<Real> ml, mr;
   mmdf3_side(ml, 0, pl[1], pl[2], pl[3], pl[4], g);
   mmdf3_side(mr, 1, pr[1], pr[2], pr[3], pr[4], g);
-  const TabH tl{ml}, tr{mr};
-  const TabW3 wl{ml}, wr{mr};
+  const TabH<Real> tl{ml}, tr{mr};
+  const TabW3<Real> wl{ml}, wr{mr};
   // Collision (basic_function.h:203-212)
-  double c0[5];
+  Real c0[5];
   c0[0] = pl[0] * ml.uh[0] + pr[0] * mr.uh[0];
   c0[1] = pl[0] * ml.uh[1] + pr[0] * mr.uh[1];
   c0[2] = pl[0] * ml.vw[1] * ml.uh[0] + pr[0] * mr.vw[1] * mr.uh[0];
   c0[3] = pl[0] * ml.ww[1] * ml.uh[0] + pr[0] * mr.ww[1] * mr.uh[0];
   c0[4] = 0.5 * pl[0] * (ml.uh[2] + ml.uh[0] * ml.vw[2] + ml.uh[0] * ml.ww[2] + ml.uh[0] * ml.xi2) +
           0.5 * pr[0] * (mr.uh[2] + mr.uh[0] * mr.vw[2] + mr.uh[0] * mr.ww[2] + mr.uh[0] * mr.xi2);
-  double alx[5], arx[5], aly[5], ary[5], alz[5], arz[5];
+  Real alx[5], arx[5], aly[5], ary[5], alz[5], arz[5];
   micro_slope3(alx, L + 5, pl, g);
   micro_slope3(arx, R + 5, pr, g);
   micro_slope3(aly, L + 10, pl, g);
   micro_slope3(ary, R + 10, pr, g);
   micro_slope3(alz, L + 15, pl, g);
   micro_slope3(arz, R + 15, pr, g);
-  double c0x[5], c0y[5], c0z[5];
+  Real c0x[5], c0y[5], c0z[5];
   {
-    double al0[5], ar0[5];
+    Real al0[5], ar0[5];
     contract3(0, 0, 0, 0, al0, alx, tl);
     contract3(0, 0, 0, 0, ar0, arx, tr);
     for (int v = 0; v < 5; v++) c0x[v] = pl[0] * al0[v] + pr[0] * ar0[v];
@@ -545,21 +582,21 @@ inline void gks3d_point(double* f, double* derf, double* center, const double* L
     }
   }
   convar_to_ulambda(p0, c0, g);
-  const double tau = tau_ns3(p0[0], p0[4], g);
-  double t[6], F0[5], F1[5];
+  const Real tau = tau_ns3(p0[0], p0[4], g);
+  Real t[6], F0[5], F1[5];
   time_coe3(t, tau, tau_num3(pl[0], pr[0], p0[0], pl[4], pr[4], p0[4], dt, g), dt);
-  const double unit[5] = {1, 0.0, 0.0, 0.0, 0.0};
-  double glu[5], gru[5];
+  const Real unit[5] = {1, 0.0, 0.0, 0.0, 0.0};
+  Real glu[5], gru[5];
   contract3(1, 0, 0, 0, glu, unit, tl);
   contract3(1, 0, 0, 0, gru, unit, tr);
   for (int i = 0; i < 5; i++) F0[i] = pl[0] * t[0] * glu[i] + pr[0] * t[0] * gru[i];
-  MM3 m0;
+  MM3<Real> m0;
   mmdf3_whole(m0, p0[1], p0[2], p0[3], p0[4], g);
-  const TabW3 w0{m0};
-  double g0u[5];
+  const TabW3<Real> w0{m0};
+  Real g0u[5];
   contract3(1, 0, 0, 0, g0u, unit, w0);
   for (int i = 0; i < 5; i++) F0[i] = F0[i] + p0[0] * t[3] * g0u[i];
-  double alxuul[5], arxuur[5], alyuvl[5], aryuvr[5], alzuwl[5], arzuwr[5];
+  Real alxuul[5], arxuur[5], alyuvl[5], aryuvr[5], alzuwl[5], arzuwr[5];
   contract3(2, 0, 0, 0, alxuul, alx, tl);
   contract3(2, 0, 0, 0, arxuur, arx, tr);
   contract3(1, 1, 0, 0, alyuvl, aly, tl);
@@ -568,14 +605,14 @@ inline void gks3d_point(double* f, double* derf, double* center, const double* L
   contract3(1, 0, 1, 0, arzuwr, arz, tr);
   for (int i = 0; i < 5; i++)
     F0[i] = F0[i] + t[1] * (pl[0] * (alxuul[i] + alyuvl[i] + alzuwl[i]) + pr[0] * (arxuur[i] + aryuvr[i] + arzuwr[i]));
-  double alxu[5], arxu[5], alyv[5], aryv[5], alzw[5], arzw[5];
+  Real alxu[5], arxu[5], alyv[5], aryv[5], alzw[5], arzw[5];
   contract3(1, 0, 0, 0, alxu, alx, wl);
   contract3(1, 0, 0, 0, arxu, arx, wr);
   contract3(0, 1, 0, 0, alyv, aly, wl);
   contract3(0, 1, 0, 0, aryv, ary, wr);
   contract3(0, 0, 1, 0, alzw, alz, wl);
   contract3(0, 0, 1, 0, arzw, arz, wr);
-  double der_AL[5], der_AR[5], Al[5], Ar[5], Alul[5], Arur[5];
+  Real der_AL[5], der_AR[5], Al[5], Ar[5], Alul[5], Arur[5];
   for (int i = 0; i < 5; i++) {
     der_AL[i] = -pl[0] * (alxu[i] + alyv[i] + alzw[i]);
     der_AR[i] = -pr[0] * (arxu[i] + aryv[i] + arzw[i]);
@@ -585,7 +622,7 @@ inline void gks3d_point(double* f, double* derf, double* center, const double* L
   contract3(1, 0, 0, 0, Alul, Al, tl);
   contract3(1, 0, 0, 0, Arur, Ar, tr);
   for (int i = 0; i < 5; i++) F0[i] = F0[i] + t[2] * (pl[0] * Alul[i] + pr[0] * Arur[i]);
-  double a0x[5], a0xu[5], a0xuu[5], a0y[5], a0yv[5], a0yuv[5], a0z[5], a0zw[5], a0zuw[5];
+  Real a0x[5], a0xu[5], a0xuu[5], a0y[5], a0yv[5], a0yuv[5], a0z[5], a0zw[5], a0zuw[5];
   micro_slope3(a0x, c0x, p0, g);
   contract3(1, 0, 0, 0, a0xu, a0x, w0);
   contract3(2, 0, 0, 0, a0xuu, a0x, w0);
@@ -596,7 +633,7 @@ inline void gks3d_point(double* f, double* derf, double* center, const double* L
   contract3(0, 0, 1, 0, a0zw, a0z, w0);
   contract3(1, 0, 1, 0, a0zuw, a0z, w0);
   for (int i = 0; i < 5; i++) F0[i] = F0[i] + p0[0] * t[4] * (a0xuu[i] + a0yuv[i] + a0zuw[i]);
-  double derA0[5], A0[5], A0u[5];
+  Real derA0[5], A0[5], A0u[5];
   for (int i = 0; i < 5; i++) derA0[i] = -p0[0] * (a0xu[i] + a0yv[i] + a0zw[i]);
   micro_slope3(A0, derA0, p0, g);
   contract3(1, 0, 0, 0, A0u, A0, w0);
@@ -892,7 +929,7 @@ struct Solver3d {
             const double* L = &pv[d][((f * 4 + q) * 2 + 0) * 20];
             const double* R = &pv[d][((f * 4 + q) * 2 + 1) * 20];
             double fo[5], dfo[5];
-            gks3d_point(fo, dfo, nullptr, L, R, dt, P.gas);
+            gks3d_point(fo, dfo, static_cast<double*>(nullptr), L, R, dt, P.gas);
             for (int v = 0; v < 5; v++) {
               double x = 0.0;
               if (stage == 0) {
