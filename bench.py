@@ -39,6 +39,13 @@ PEAK_FP64_TFLOPS = 78.6  # MI355X FP64 (vector and matrix)
 # and written): fetch x1.882, write x0.985
 TRAFFIC_2048_FAST = {"flux_x": 1.9280e9, "flux_y": 1.9668e9}
 PEAK_HBM_GBS = 8000.0
+# 3D units (oracle/flopcount3d, WENO5-AO characteristic, GKS3D gks2nd S2O4):
+# FP64 operations per unit of each kernel phase
+FLOPS3 = {"face": 2782,   # face values of one cell across one direction's faces
+          "line": 3424,   # line values of one face side
+          "point": 4796,  # point values of one face side along one line
+          "flux": 8115}   # GKS3D at one Gauss point (+ 4 exp, 2 erfc)
+FLOPS3_CELL_STEP = 367644  # one interior cell's S2O4 step (2 stages x 3 directions)
 
 
 def kernel_units(phase, n, gp=2, fused=False):
@@ -171,6 +178,122 @@ def bench_batch(args):
     print(json.dumps(out), flush=True)
 
 
+def units3(phase, n, G=3):
+    """Work units of one launch of a 3D phase on an n^3 cube (the three
+    directions' launches have the same count)."""
+    m = n + 2 * G
+    if phase == "flux":
+        return 4 * (n + 1) * n * n
+    if phase == "point":
+        return 4 * (n + 1) * n * n
+    if phase == "line":
+        return 2 * (n + 1) * n * (m - 2 * G + 4)
+    if phase == "face":
+        return (n + 2) * (m - 2 * G + 4) ** 2
+    return None
+
+
+def cpu_baseline3(n, threads, sample_steps):
+    """Times the reference's 3D driver loop (oracle/_ref: its own boundary,
+    reconstruction, flux loop and update code, with the harness's
+    lambda-from-state GKS3D in flux_function_3d, free boundaries) or, if it
+    is not built, the 3D oracle, on the host: `sample_steps` steps of the
+    Taylor-Green cube at n^3 in a child process."""
+    code = f"""
+import sys, time, json
+sys.path.insert(0, {os.path.join(ROOT, 'high-order-solver_amd')!r}); sys.path.insert(0, {os.path.join(ROOT, 'tests')!r})
+from oracle_lib import have_ref, reference3, oracle3
+from gks_amd.params import tgv_params
+from gks_amd.problems import tgv_convar, convar_to_prim_3d
+p = tgv_params({n})
+u0 = tgv_convar(p, quad=2)
+prim = convar_to_prim_3d(u0, 1.4)
+if have_ref():
+    import ctypes
+    import numpy as np
+    kind = 'reference'
+    R = reference3()
+    h = R.f.ref3d_create(ctypes.byref(p))
+    R.f.ref3d_set_state(h, u0.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), prim.ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
+    hist = np.zeros({sample_steps})
+    t0 = time.perf_counter()
+    R.f.ref3d_run(h, 100.0, {sample_steps}, hist.ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
+    t1 = time.perf_counter()
+    k = {sample_steps}
+else:
+    kind = 'port'
+    t0 = time.perf_counter()
+    u, t, k, rc, _ = oracle3().run(p, u0, prim, [1] * 6, 100.0, {sample_steps})
+    t1 = time.perf_counter()
+print(json.dumps(dict(kind=kind, seconds=t1 - t0, steps=k)))
+"""
+    env = dict(os.environ, OMP_NUM_THREADS=str(threads))
+    out = subprocess.run([sys.executable, "-c", code], env=env, capture_output=True, text=True, timeout=900)
+    if out.returncode != 0:
+        return None
+    r = json.loads(out.stdout.strip().splitlines()[-1])
+    v = n ** 3 * r["steps"] / r["seconds"] / 1e6
+    return {"value": round(v, 4), "unit": "Mcell-updates/s", "cores": threads if r["kind"] == "reference" else 1,
+            "kind": r["kind"],
+            "sample": f"{r['steps']} S2O4 step(s) of the Taylor-Green cube at {n}^3, {r['seconds']:.1f} s wall"
+                      + (f", OMP_NUM_THREADS={threads}; the reference's 3D loop with a lambda-from-state GKS3D "
+                         "and its free boundaries (it has no periodic 3D boundary; the fills are under 1% of a step)"
+                         if r["kind"] == "reference" else ", the 3D oracle, one thread")}
+
+
+def bench_tgv3d(args):
+    """3D workload: the viscous Taylor-Green vortex (Re 1600, Mach 0.1) on
+    the periodic cube [0, 2 pi]^3 at n^3 (3 ghost layers), WENO5-AO in
+    characteristic variables, 2 x 2 Gauss points, GKS3D (gks2nd,
+    Navier-Stokes collision time), S2O4, CFL 0.5."""
+    from gks_amd.params import BC_PERIODIC, tgv_params
+    from gks_amd.problems import tgv_convar
+    from gks_amd.solver3d import Solver3D
+
+    n = args.n3d
+    p = tgv_params(n, fast_math=1 if args.variant == "fast" else 0)
+    s = Solver3D(p)
+    s.init_convar(tgv_convar(p, quad=2))
+    bc = [BC_PERIODIC] * 6
+    s.run(bc, 100.0, args.warmup)
+    s.sync()
+    s.timing(True)
+    t0 = time.perf_counter()
+    k = s.run(bc, 100.0, args.steps)
+    s.sync()
+    wall = time.perf_counter() - t0
+    tim = s.get_timing()
+    s.close()
+    value = n ** 3 * k / wall / 1e6
+    dom = max(tim, key=lambda ph: tim[ph][0])
+    avg_ms = tim[dom][0] / max(tim[dom][1], 1)
+    u = units3(dom, n)
+    roof = None
+    if u and dom in FLOPS3:
+        achieved = u * FLOPS3[dom] / (avg_ms * 1e-3) / 1e12
+        roof = {"bound": "mfma", "achieved": round(achieved, 3), "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
+                "frac": round(achieved / PEAK_FP64_TFLOPS, 4), "traffic": None, "kernel": f"k3_{dom} (x, y, z)",
+                "avg_ms": round(avg_ms, 4), "flops_per_launch": u * FLOPS3[dom]}
+    out = {
+        "metric": "Mcell-updates/s FP64, 3D Taylor-Green WENO5-AO GKS3D S2O4",
+        "value": round(value, 3), "unit": "Mcell-updates/s", "n_gpus": 1, "steps": k, "warmup": args.warmup,
+        "ms_per_step": round(wall / k * 1e3, 3), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"Taylor-Green vortex Re 1600 Mach 0.1, periodic [0,2pi]^3 at {n}^3, WENO5-AO "
+                               f"characteristic, 2x2 Gauss points, GKS3D gks2nd Navier-Stokes, S2O4, CFL 0.5 "
+                               f"({args.variant} kernels)", "n": n, "kernels": args.variant},
+        "roofline": roof,
+        "step_fp64_tflops": round(FLOPS3_CELL_STEP * n ** 3 * k / wall / 1e12, 3),
+        "phase_ms_per_step": {ph: round(v[0] / k, 4) for ph, v in tim.items()},
+    }
+    if not args.no_cpu_baseline:
+        threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or (os.cpu_count() or 1)
+        out["cpu_baseline"] = cpu_baseline3(args.cpu_n3d, threads, args.cpu_steps)
+    else:
+        out["cpu_baseline"] = None
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -185,8 +308,11 @@ def main():
     ap.add_argument("--path", choices=["plain", "learned"], default="plain",
                     help="learned: the trained troubled-interface classifier picks linear weights for smooth stencils")
     ap.add_argument("--classifier", default=None, help="trained classifier (.npz); default: the packaged one")
-    ap.add_argument("--workload", choices=["single", "batch"], default="single",
-                    help="batch: --batch uneven problems stepped together, against running them one by one")
+    ap.add_argument("--workload", choices=["single", "batch", "tgv3d"], default="single",
+                    help="batch: --batch uneven problems stepped together, against running them one by one; "
+                         "tgv3d: the 3D Taylor-Green cube at --n3d^3")
+    ap.add_argument("--n3d", type=int, default=256)
+    ap.add_argument("--cpu-n3d", type=int, default=48)
     ap.add_argument("--batch", type=int, default=64)
     ap.add_argument("--full-run", action="store_true",
                     help="time the whole run to t=0.25 instead of a window of --steps steps (the learned path's "
@@ -202,6 +328,9 @@ def main():
         sys.exit(3)
     if args.workload == "batch":
         bench_batch(args)
+        return
+    if args.workload == "tgv3d":
+        bench_tgv3d(args)
         return
 
     import numpy as np  # noqa: F401

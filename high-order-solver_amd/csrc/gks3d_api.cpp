@@ -71,7 +71,57 @@ struct gks3d_ctx {
   bool have_speed = false;   // CFL inputs known (set_cfl_primvar or a step)
   double tmax = 0.0, rhomin = 0.0;
   std::vector<void*> allocs;
+  // per-phase HIP-event timing (gks3d_timing)
+  bool timing = false;
+  struct Rec {
+    int phase;
+    hipEvent_t a, b;
+  };
+  std::vector<Rec> pending;
+  std::vector<hipEvent_t> pool;
+  size_t pool_used = 0;
+  double t_ms[GKS3_NPHASE] = {};
+  long long t_n[GKS3_NPHASE] = {};
 };
+
+static hipEvent_t pool_event(gks3d_ctx* c) {
+  if (c->pool_used == c->pool.size()) {
+    hipEvent_t e = nullptr;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    c->pool.push_back(e);
+  }
+  return c->pool[c->pool_used++];
+}
+// launches one kernel phase; with timing on, brackets it with events on the
+// context's stream
+template <class F>
+static int timed(gks3d_ctx* c, int phase, F&& launch) {
+  hipEvent_t a = nullptr, b = nullptr;
+  if (c->timing) {
+    a = pool_event(c);
+    b = pool_event(c);
+    if (!a || !b) return fail(GKS_ERR_HIP, "hipEventCreate");
+    HIPCHK(hipEventRecord(a, c->s));
+  }
+  HIPCHK(launch());
+  if (c->timing) {
+    HIPCHK(hipEventRecord(b, c->s));
+    c->pending.push_back({phase, a, b});
+  }
+  return GKS_OK;
+}
+// folds finished event pairs into the accumulators (stream must be idle)
+static int collect_timing(gks3d_ctx* c) {
+  for (auto& r : c->pending) {
+    float ms = 0.f;
+    HIPCHK(hipEventElapsedTime(&ms, r.a, r.b));
+    c->t_ms[r.phase] += ms;
+    c->t_n[r.phase]++;
+  }
+  c->pending.clear();
+  c->pool_used = 0;
+  return GKS_OK;
+}
 
 static int dev_alloc(gks3d_ctx* c, double** out, size_t n) {
   void* ptr = nullptr;
@@ -112,6 +162,7 @@ int gks3d_destroy(gks3d_ctx* c) {
   if (c->s) (void)hipStreamSynchronize(c->s);
   for (void* a : c->allocs) (void)hipFree(a);
   if (c->h_stat) (void)hipHostFree(c->h_stat);
+  for (hipEvent_t e : c->pool) (void)hipEventDestroy(e);
   if (c->s) (void)hipStreamDestroy(c->s);
   delete c;
   return GKS_OK;
@@ -221,10 +272,16 @@ int gks3d_set_cfl_primvar(gks3d_ctx* c, const double* base, int64_t cs, int64_t 
 
 static int stage(gks3d_ctx* c, int s, double dt, const int* bc) {
   const Dev3& d = c->d;
-  HIPCHK(c->lt->bc(d, bc, c->s));
-  HIPCHK(c->lt->primvar(d, c->s));
-  for (int dir = 0; dir < 3; dir++) HIPCHK(c->lt->direction(d, dir, s, dt, c->s));
-  HIPCHK(c->lt->update(d, c->s));
+  const Launch3dTable* lt = c->lt;
+  RC(timed(c, GKS3_PHASE_BC, [&] { return lt->bc(d, bc, c->s); }));
+  RC(timed(c, GKS3_PHASE_PRIMVAR, [&] { return lt->primvar(d, c->s); }));
+  for (int dir = 0; dir < 3; dir++) {
+    RC(timed(c, GKS3_PHASE_FACE, [&] { return lt->face(d, dir, c->s); }));
+    RC(timed(c, GKS3_PHASE_LINE, [&] { return lt->line(d, dir, c->s); }));
+    RC(timed(c, GKS3_PHASE_POINT, [&] { return lt->point(d, dir, c->s); }));
+    RC(timed(c, GKS3_PHASE_FLUX, [&] { return lt->flux(d, dir, s, dt, c->s); }));
+  }
+  RC(timed(c, GKS3_PHASE_UPDATE, [&] { return lt->update(d, c->s); }));
   return GKS_OK;
 }
 
@@ -250,11 +307,12 @@ int gks3d_run(gks3d_ctx* c, const int32_t bc[6], double tstop, int32_t max_steps
   bool blew = false;
   while (t < tstop && n < max_steps) {
     const double dt = cfl_dt(c, t, tstop);
-    HIPCHK(c->lt->copy_old(d, c->s));
+    RC(timed(c, GKS3_PHASE_COPY_OLD, [&] { return c->lt->copy_old(d, c->s); }));
     RC(stage(c, 0, dt, b));
     RC(stage(c, 1, dt, b));
     HIPCHK(hipMemcpyAsync(c->h_stat, d.stat, 4 * sizeof(double), hipMemcpyDeviceToHost, c->s));
     HIPCHK(hipStreamSynchronize(c->s));
+    if (c->timing) RC(collect_timing(c));
     if (c->h_stat[2] != 0.0) {
       blew = true;
       break;
@@ -272,6 +330,28 @@ int gks3d_run(gks3d_ctx* c, const int32_t bc[6], double tstop, int32_t max_steps
     HIPCHK(hipMemcpyAsync(d.stat + 2, &zero, sizeof(double), hipMemcpyHostToDevice, c->s));
     HIPCHK(hipStreamSynchronize(c->s));
     return fail(GKS_ERR_BLOWUP, "density or pressure negative or NaN");
+  }
+  return GKS_OK;
+}
+
+int gks3d_timing(gks3d_ctx* c, int enable) {
+  if (!c) return fail(GKS_ERR_ARG, "null");
+  HIPCHK(hipStreamSynchronize(c->s));
+  RC(collect_timing(c));
+  for (int k = 0; k < GKS3_NPHASE; k++) {
+    c->t_ms[k] = 0.0;
+    c->t_n[k] = 0;
+  }
+  c->timing = enable != 0;
+  return GKS_OK;
+}
+int gks3d_get_timing(gks3d_ctx* c, double* ms, int64_t* launches) {
+  if (!c || !ms || !launches) return fail(GKS_ERR_ARG, "null");
+  HIPCHK(hipStreamSynchronize(c->s));
+  RC(collect_timing(c));
+  for (int k = 0; k < GKS3_NPHASE; k++) {
+    ms[k] = c->t_ms[k];
+    launches[k] = c->t_n[k];
   }
   return GKS_OK;
 }

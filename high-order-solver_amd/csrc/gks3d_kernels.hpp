@@ -553,32 +553,60 @@ static hipError_t launch_copy_old(const Dev3& d, hipStream_t s) {
 }
 // the index boxes of the reference's loops (reconstruction.cpp:3698-3753
 // with the guards of :3779, 3928-3949, 4147-4161; flux_function.cpp:760-811)
-template <int DIR>
-static hipError_t launch_direction_t(const Dev3& d, int stage, double dt, hipStream_t s) {
-  const int G = d.G, nx = d.nx, ny = d.ny, nz = d.nz;
-  // face values: the DIR index in [G-1, n-G], the others in [G-2, n-G+2)
-  const Box3 bf = make_box(DIR == 0 ? G - 1 : G - 2, DIR == 0 ? nx - G + 1 : nx - G + 2,
-                           DIR == 1 ? G - 1 : G - 2, DIR == 1 ? ny - G + 1 : ny - G + 2,
-                           DIR == 2 ? G - 1 : G - 2, DIR == 2 ? nz - G + 1 : nz - G + 2);
-  // line values
-  Box3 bl;
-  if (DIR == 0) bl = make_box(G, nx - G + 1, G, ny - G, G - 2, nz - G + 2);
-  else if (DIR == 1) bl = make_box(G, nx - G, G, ny - G + 1, G - 2, nz - G + 2);
-  else bl = make_box(G - 2, nx - G + 2, G, ny - G, G, nz - G + 1);
-  // the faces that get a flux
-  const Box3 bx = make_box(G, G + d.NX + (DIR == 0), G, G + d.NY + (DIR == 1), G, G + d.NZ + (DIR == 2));
-  if (box_count(bf) > 0) k3_face<DIR><<<blocks_for(box_count(bf)), kBlock3, 0, s>>>(d, bf);
-  if (box_count(bl) > 0) k3_line<DIR><<<blocks_for(2 * box_count(bl)), kBlock3, 0, s>>>(d, bl);
-  if (box_count(bx) > 0) {
-    k3_point<DIR><<<blocks_for(4 * box_count(bx)), kBlock3, 0, s>>>(d, bx);
-    k3_flux<DIR><<<blocks_for(4 * box_count(bx)), kBlock3, 0, s>>>(d, bx, stage, dt);
-  }
+// face values: the DIR index in [G-1, n-G], the others in [G-2, n-G+2)
+static Box3 face_box(const Dev3& d, int dir) {
+  const int G = d.G;
+  return make_box(dir == 0 ? G - 1 : G - 2, dir == 0 ? d.nx - G + 1 : d.nx - G + 2, dir == 1 ? G - 1 : G - 2,
+                  dir == 1 ? d.ny - G + 1 : d.ny - G + 2, dir == 2 ? G - 1 : G - 2,
+                  dir == 2 ? d.nz - G + 1 : d.nz - G + 2);
+}
+// line values
+static Box3 line_box(const Dev3& d, int dir) {
+  const int G = d.G;
+  if (dir == 0) return make_box(G, d.nx - G + 1, G, d.ny - G, G - 2, d.nz - G + 2);
+  if (dir == 1) return make_box(G, d.nx - G, G, d.ny - G + 1, G - 2, d.nz - G + 2);
+  return make_box(G - 2, d.nx - G + 2, G, d.ny - G, G, d.nz - G + 1);
+}
+// the faces that get a flux
+static Box3 flux_box(const Dev3& d, int dir) {
+  const int G = d.G;
+  return make_box(G, G + d.NX + (dir == 0), G, G + d.NY + (dir == 1), G, G + d.NZ + (dir == 2));
+}
+static hipError_t launch_face(const Dev3& d, int dir, hipStream_t s) {
+  const Box3 b = face_box(d, dir);
+  if (box_count(b) == 0) return hipSuccess;
+  const unsigned nb = blocks_for(box_count(b));
+  if (dir == 0) k3_face<0><<<nb, kBlock3, 0, s>>>(d, b);
+  else if (dir == 1) k3_face<1><<<nb, kBlock3, 0, s>>>(d, b);
+  else k3_face<2><<<nb, kBlock3, 0, s>>>(d, b);
   return hipGetLastError();
 }
-static hipError_t launch_direction(const Dev3& d, int dir, int stage, double dt, hipStream_t s) {
-  if (dir == 0) return launch_direction_t<0>(d, stage, dt, s);
-  if (dir == 1) return launch_direction_t<1>(d, stage, dt, s);
-  return launch_direction_t<2>(d, stage, dt, s);
+static hipError_t launch_line(const Dev3& d, int dir, hipStream_t s) {
+  const Box3 b = line_box(d, dir);
+  if (box_count(b) == 0) return hipSuccess;
+  const unsigned nb = blocks_for(2 * box_count(b));
+  if (dir == 0) k3_line<0><<<nb, kBlock3, 0, s>>>(d, b);
+  else if (dir == 1) k3_line<1><<<nb, kBlock3, 0, s>>>(d, b);
+  else k3_line<2><<<nb, kBlock3, 0, s>>>(d, b);
+  return hipGetLastError();
+}
+static hipError_t launch_point(const Dev3& d, int dir, hipStream_t s) {
+  const Box3 b = flux_box(d, dir);
+  if (box_count(b) == 0) return hipSuccess;
+  const unsigned nb = blocks_for(4 * box_count(b));
+  if (dir == 0) k3_point<0><<<nb, kBlock3, 0, s>>>(d, b);
+  else if (dir == 1) k3_point<1><<<nb, kBlock3, 0, s>>>(d, b);
+  else k3_point<2><<<nb, kBlock3, 0, s>>>(d, b);
+  return hipGetLastError();
+}
+static hipError_t launch_flux(const Dev3& d, int dir, int stage, double dt, hipStream_t s) {
+  const Box3 b = flux_box(d, dir);
+  if (box_count(b) == 0) return hipSuccess;
+  const unsigned nb = blocks_for(4 * box_count(b));
+  if (dir == 0) k3_flux<0><<<nb, kBlock3, 0, s>>>(d, b, stage, dt);
+  else if (dir == 1) k3_flux<1><<<nb, kBlock3, 0, s>>>(d, b, stage, dt);
+  else k3_flux<2><<<nb, kBlock3, 0, s>>>(d, b, stage, dt);
+  return hipGetLastError();
 }
 static hipError_t launch_update(const Dev3& d, hipStream_t s) {
   const Box3 b = interior(d);
@@ -591,7 +619,8 @@ static hipError_t launch_flux_points(const Gas3& g, long long n, const double* L
   return hipGetLastError();
 }
 
-extern const Launch3dTable table3d = {launch_bc, launch_primvar, launch_copy_old, launch_direction, launch_update,
+extern const Launch3dTable table3d = {launch_bc,    launch_primvar, launch_copy_old, launch_face,
+                                      launch_line,  launch_point,   launch_flux,     launch_update,
                                       launch_flux_points};
 #endif
 

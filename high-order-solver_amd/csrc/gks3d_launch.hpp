@@ -33,8 +33,11 @@ struct Launch3dTable {
   hipError_t (*bc)(const Dev3&, const int* bc6, hipStream_t);
   hipError_t (*primvar)(const Dev3&, hipStream_t);  // sweep + reduction into stat
   hipError_t (*copy_old)(const Dev3&, hipStream_t);
-  // face, line and point values plus GKS3D of one direction and stage
-  hipError_t (*direction)(const Dev3&, int dir, int stage, double dt, hipStream_t);
+  // one direction's face values, line values, point values, GKS3D + stage sum
+  hipError_t (*face)(const Dev3&, int dir, hipStream_t);
+  hipError_t (*line)(const Dev3&, int dir, hipStream_t);
+  hipError_t (*point)(const Dev3&, int dir, hipStream_t);
+  hipError_t (*flux)(const Dev3&, int dir, int stage, double dt, hipStream_t);
   hipError_t (*update)(const Dev3&, hipStream_t);
   hipError_t (*flux_points)(const gksd3::Gas3&, long long n, const double* L, const double* R, double dt,
                             double* C, double* F, double* D, hipStream_t);

@@ -30,9 +30,13 @@ SIGS3 = {
     "gks3d_set_cfl_primvar": (ctypes.c_int, [_P, _D, _I64, _I64]),
     "gks3d_run": (ctypes.c_int, [_P, _I32P, ctypes.c_double, ctypes.c_int32, _D, _I32P]),
     "gks3d_sync": (ctypes.c_int, [_P]),
+    "gks3d_timing": (ctypes.c_int, [_P, ctypes.c_int]),
+    "gks3d_get_timing": (ctypes.c_int, [_P, _D, ctypes.POINTER(ctypes.c_int64)]),
     "gks3d_flux_points": (ctypes.c_int, [ctypes.POINTER(Params3D), _I64, _D, _D, ctypes.c_double, _D, _D, _D]),
     "gks3d_last_error": (ctypes.c_char_p, []),
 }
+
+PHASES3 = ["bc", "primvar", "face", "line", "point", "flux", "update", "copy_old"]
 
 _lib3 = None
 
@@ -159,3 +163,14 @@ class Solver3D:
 
     def sync(self):
         _check(lib3().gks3d_sync(self.h))
+
+    def timing(self, on=True):
+        """Resets and (with on) starts per-phase HIP-event timing."""
+        _check(lib3().gks3d_timing(self.h, 1 if on else 0))
+
+    def get_timing(self):
+        """{phase: (summed ms, launches)} since timing(True)."""
+        ms = np.zeros(len(PHASES3))
+        n = (ctypes.c_int64 * len(PHASES3))()
+        _check(lib3().gks3d_get_timing(self.h, _dp(ms), n))
+        return {ph: (float(ms[k]), int(n[k])) for k, ph in enumerate(PHASES3)}

@@ -67,6 +67,17 @@ int gks3d_set_cfl_primvar(gks3d_ctx* ctx, const double* base, int64_t cell_strid
  * counted). */
 int gks3d_run(gks3d_ctx* ctx, const int32_t bc[6], double tstop, int32_t max_steps, double* t, int32_t* step);
 int gks3d_sync(gks3d_ctx* ctx);
+/* Device time per kernel phase (all three directions summed), measured with
+ * HIP events on the context's stream (no counterpart in the reference, whose
+ * only timer is omp_get_wtime around the loop, 3Dproblem.cpp:112-169).
+ * gks3d_timing(ctx, 1) resets and starts accumulating; gks3d_get_timing
+ * fills ms[GKS3_NPHASE] (summed milliseconds) and launches[GKS3_NPHASE]. */
+enum {
+  GKS3_PHASE_BC = 0, GKS3_PHASE_PRIMVAR, GKS3_PHASE_FACE, GKS3_PHASE_LINE, GKS3_PHASE_POINT,
+  GKS3_PHASE_FLUX, GKS3_PHASE_UPDATE, GKS3_PHASE_COPY_OLD, GKS3_NPHASE
+};
+int gks3d_timing(gks3d_ctx* ctx, int enable);
+int gks3d_get_timing(gks3d_ctx* ctx, double* ms, int64_t* launches);
 /* Leaf parity: GKS3D at n Gauss points.  left/right: n x 20 (convar[5],
  * der1x[5], der1y[5], der1z[5] in the face frame); center (optional):
  * n x 20 collision state and slopes; f, derf: n x 5. */

@@ -173,3 +173,18 @@ def test_bad_boundary_kind_is_rejected():
         s.run([2] * 6, 1.0, 1)
     assert e.value.code == 2
     s.close()
+
+
+def test_phase_timing_counts_every_launch():
+    p = tgv_params(16, fast_math=1)
+    s = Solver3D(p)
+    s.init_convar(tgv_convar(p))
+    s.timing(True)
+    n = s.run([BC_PERIODIC] * 6, 10.0, 3)
+    tim = s.get_timing()
+    s.close()
+    per_step = {"bc": 2, "primvar": 2, "face": 6, "line": 6, "point": 6, "flux": 6, "update": 2, "copy_old": 1}
+    say("[3d timing]", {k: round(v[0], 3) for k, v in tim.items()})
+    for ph, m in per_step.items():
+        assert tim[ph][1] == m * n, ph
+        assert tim[ph][0] > 0.0, ph
