@@ -565,12 +565,33 @@ GKS_DI void weno5_weights(double* fw, double wn2, double wn1, double w0, double 
   beta[3] = 1.0 / 6.0 * (beta[0] + 4.0 * beta[1] + beta[2]) + fabs(beta[0] - beta[2]);
   const double tau5 = 1.0 / 3.0 * (fabs(beta[3] - beta[0]) + fabs(beta[3] - beta[1]) + fabs(beta[3] - beta[2]));
   double alpha[4], sum = 0.0;
+#if GKS_FAST
+  // the four quotients tau5 / (beta_k + eps) from one reciprocal of the
+  // product of the denominators (each >= eps, the product stays in range)
+  double ib[4];
+  {
+    const double b0 = beta[0] + eps, b1 = beta[1] + eps, b2 = beta[2] + eps, b3 = beta[3] + eps;
+    const double p01 = b0 * b1, p23 = b2 * b3;
+    const double r = tau5 / (p01 * p23);
+    ib[0] = r * b1 * p23;
+    ib[1] = r * b0 * p23;
+    ib[2] = r * p01 * b3;
+    ib[3] = r * p01 * b2;
+  }
+#pragma unroll
+  for (int i = 0; i < 4; i++) {
+    const double gd = ib[i];
+    alpha[i] = d[i] * (1.0 + gd * gd);
+    sum += alpha[i];
+  }
+#else
 #pragma unroll
   for (int i = 0; i < 4; i++) {
     double gd = tau5 / (beta[i] + eps);
     alpha[i] = d[i] * (1.0 + gd * gd);
     sum += alpha[i];
   }
+#endif
   double ww[4];
 #if GKS_FAST
   const double isum = 1.0 / sum;

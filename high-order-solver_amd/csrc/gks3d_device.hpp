@@ -242,12 +242,33 @@ GKS3_DI void weno5_weights3(double* fw, double wn2, double wn1, double w0, doubl
                               8209.0 * wn2 * wp2 - 51001.0 * wp1 * wp2 + 6908.0 * wp2 * wp2);
   const double tau5 = 1.0 / 3.0 * (fabs(beta[3] - beta[0]) + fabs(beta[3] - beta[1]) + fabs(beta[3] - beta[2]));
   double sum_alpha = 0.0;
+#if GKS_FAST
+  // the four quotients tau5 / (beta_k + eps) from one reciprocal of the
+  // product of the denominators (each >= eps, the product stays in range)
+  double gq[4];
+  {
+    const double b0 = beta[0] + eps, b1 = beta[1] + eps, b2 = beta[2] + eps, b3 = beta[3] + eps;
+    const double p01 = b0 * b1, p23 = b2 * b3;
+    const double r = tau5 / (p01 * p23);
+    gq[0] = r * b1 * p23;
+    gq[1] = r * b0 * p23;
+    gq[2] = r * p01 * b3;
+    gq[3] = r * p01 * b2;
+  }
+#pragma unroll
+  for (int i = 0; i < 4; i++) {
+    const double gd = gq[i];
+    alpha[i] = d[i] * (1.0 + gd * gd);
+    sum_alpha += alpha[i];
+  }
+#else
 #pragma unroll
   for (int i = 0; i < 4; i++) {
     const double gd = tau5 / (beta[i] + eps);
     alpha[i] = d[i] * (1.0 + gd * gd);
     sum_alpha += alpha[i];
   }
+#endif
   double ww[4];
 #if GKS_FAST
   const double rs = 1.0 / sum_alpha;
