@@ -202,7 +202,6 @@ int gks3d_create(const gks3d_params* p, gks3d_ctx** out) {
   int rc;
   if ((rc = dev_alloc(c, &d.U, 5 * nc)) || (rc = dev_alloc(c, &d.Uold, 5 * nc)) ||
       (rc = dev_alloc(c, &d.fv, 20 * nf)) || (rc = dev_alloc(c, &d.lv, 60 * nf)) ||
-      (rc = dev_alloc(c, &d.pv, 160 * nf)) ||
       (rc = dev_alloc(c, &d.part, 3 * kParts3)) || (rc = dev_alloc(c, &d.stat, 4))) {
     gks3d_destroy(c);
     return rc;

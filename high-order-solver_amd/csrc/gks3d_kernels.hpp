@@ -461,6 +461,92 @@ __global__ void __launch_bounds__(kBlock3) k3_flux(Dev3 d, Box3 b, int stage, do
   }
 }
 
+// Point values and GKS3D fused through LDS: a workgroup of kPfBlock lanes
+// takes kPfFaces faces.  Phase 1, one lane per (face, line, side): the
+// second tangential reconstruction (as k3_point) into the workgroup's LDS
+// tile; phase 2, one lane per (face, Gauss point): GKS3D reading its left
+// and right states from the tile, the S2O4 stage sum over the face's four
+// points by lane shuffles (as k3_flux).  The tile (kPfFaces x 2 sides x 4
+// points x 20 values = 40 KB) never leaves the CU; four workgroups fill
+// the 160 KB of a CU's LDS.
+constexpr int kPfBlock = 128;
+constexpr int kPfFaces = kPfBlock / 4;
+constexpr int kPfRow = 20;
+struct LdsSide {
+  const double* p;
+  __device__ __forceinline__ double operator[](int k) const { return p[k]; }
+};
+template <int DIR>
+__global__ void __launch_bounds__(kPfBlock) k3_pflux(Dev3 d, Box3 b, int stage, double dt) {
+  __shared__ double s_pt[kPfFaces][2][4][kPfRow];
+  const int lane = threadIdx.x;
+  const int fl = lane >> 2;
+  const long long fi = (long long)blockIdx.x * kPfFaces + fl;
+  const bool active = fi < b.count();  // the four lanes of a face agree
+  int i = 0, j = 0, k = 0;
+  if (active) b.at(fi, i, j, k);
+  const long long f = active ? face3(d, i, j, k) : 0;
+  if (active) {
+    const int l = (lane >> 1) & 1, side = lane & 1;
+    const long long step = DIR == 2 ? -(long long)(d.ny + 1) * (d.nz + 1) : 1LL;
+    const HInv h = hinv(DIR == 2 ? d.dx : d.dz);
+    double res[2][20];  // [point]
+    tangential_side<15>(res, LvStencil{d.lv, d.nf, f, step, l, side}, h, d.recon_var, d.gas);
+    int red = (reduce3(res[0], d.gas) || reduce3(res[1], d.gas)) ? 1 : 0;
+    red |= __shfl_xor(red, 1);
+    if (red) {
+#pragma unroll
+      for (int p = 0; p < 2; p++) {
+#pragma unroll
+        for (int m = 0; m < 15; m++) res[p][m] = d.lv[((l * 2 + side) * 15 + m) * d.nf + f];
+#pragma unroll
+        for (int m = 15; m < 20; m++) res[p][m] = 0.0;
+      }
+    }
+#pragma unroll
+    for (int p = 0; p < 2; p++)
+#pragma unroll
+      for (int m = 0; m < 20; m++) s_pt[fl][side][2 * l + p][m] = res[p][m];
+  }
+  __syncthreads();
+  const int q = lane & 3;
+  double x[5] = {0.0, 0.0, 0.0, 0.0, 0.0}, share[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+  if (active) {
+    double fo[5], dfo[5];
+    gks3d_point_dev(fo, dfo, nullptr, LdsSide{&s_pt[fl][0][q][0]}, LdsSide{&s_pt[fl][1][q][0]}, dt, d.gas);
+#pragma unroll
+    for (int v = 0; v < 5; v++) {
+      if (stage == 0) {
+        x[v] = 0.0 + (0.5 * fo[v] + 1.0 / 8.0 * dfo[v]);
+        share[v] = 0.0 + (1.0 * fo[v] + 1.0 / 6.0 * dfo[v]);
+      } else {
+        x[v] = 0.0 * fo[v] + 1.0 / 3.0 * dfo[v];
+      }
+    }
+  }
+#pragma unroll
+  for (int v = 0; v < 5; v++) {
+    x[v] += __shfl_xor(x[v], 1);
+    x[v] += __shfl_xor(x[v], 2);
+    share[v] += __shfl_xor(share[v], 1);
+    share[v] += __shfl_xor(share[v], 2);
+  }
+  if (active && q == 0) {
+    double* xs = d.xs[DIR];
+    double* pc = d.pc[DIR];
+#pragma unroll
+    for (int v = 0; v < 5; v++) {
+      const long long o = (long long)v * d.nf + f;
+      if (stage == 0) {
+        xs[o] = x[v];
+        pc[o] = share[v];
+      } else {
+        xs[o] = pc[o] + x[v];
+      }
+    }
+  }
+}
+
 // the cell update of Update_gauss (time_advance.cpp:937-998): U = Uold +
 // the Gauss-point sums (weights 1/4) of the faces, the y / z face frames
 // rotated back
@@ -525,6 +611,9 @@ template __global__ void k3_point<2>(Dev3, Box3);
 template __global__ void k3_flux<0>(Dev3, Box3, int, double);
 template __global__ void k3_flux<1>(Dev3, Box3, int, double);
 template __global__ void k3_flux<2>(Dev3, Box3, int, double);
+template __global__ void k3_pflux<0>(Dev3, Box3, int, double);
+template __global__ void k3_pflux<1>(Dev3, Box3, int, double);
+template __global__ void k3_pflux<2>(Dev3, Box3, int, double);
 
 // ------------------------------------------------------------ launchers
 #if !defined(__HIP_DEVICE_COMPILE__)
@@ -592,7 +681,7 @@ static hipError_t launch_line(const Dev3& d, int dir, hipStream_t s) {
 }
 static hipError_t launch_point(const Dev3& d, int dir, hipStream_t s) {
   const Box3 b = flux_box(d, dir);
-  if (box_count(b) == 0) return hipSuccess;
+  if (box_count(b) == 0 || !d.pv) return hipSuccess;  // without pv the flux launch does the points
   const unsigned nb = blocks_for(4 * box_count(b));
   if (dir == 0) k3_point<0><<<nb, kBlock3, 0, s>>>(d, b);
   else if (dir == 1) k3_point<1><<<nb, kBlock3, 0, s>>>(d, b);
@@ -602,6 +691,13 @@ static hipError_t launch_point(const Dev3& d, int dir, hipStream_t s) {
 static hipError_t launch_flux(const Dev3& d, int dir, int stage, double dt, hipStream_t s) {
   const Box3 b = flux_box(d, dir);
   if (box_count(b) == 0) return hipSuccess;
+  if (!d.pv) {  // fused point values + GKS3D
+    const unsigned nbf = (unsigned)((box_count(b) + kPfFaces - 1) / kPfFaces);
+    if (dir == 0) k3_pflux<0><<<nbf, kPfBlock, 0, s>>>(d, b, stage, dt);
+    else if (dir == 1) k3_pflux<1><<<nbf, kPfBlock, 0, s>>>(d, b, stage, dt);
+    else k3_pflux<2><<<nbf, kPfBlock, 0, s>>>(d, b, stage, dt);
+    return hipGetLastError();
+  }
   const unsigned nb = blocks_for(4 * box_count(b));
   if (dir == 0) k3_flux<0><<<nb, kBlock3, 0, s>>>(d, b, stage, dt);
   else if (dir == 1) k3_flux<1><<<nb, kBlock3, 0, s>>>(d, b, stage, dt);
