@@ -43,9 +43,13 @@ PEAK_HBM_GBS = 8000.0
 # FP64 operations per unit of each kernel phase
 FLOPS3 = {"face": 2782,   # face values of one cell across one direction's faces
           "line": 3424,   # line values of one face side
-          "point": 4796,  # point values of one face side along one line
+          "point": 4796,  # point values of one face side along one line (inside the flux launch)
           "flux": 8115}   # GKS3D at one Gauss point (+ 4 exp, 2 erfc)
 FLOPS3_CELL_STEP = 367644  # one interior cell's S2O4 step (2 stages x 3 directions)
+# HBM bytes per k3_pflux launch at 256^3, fast kernels, mean of the three
+# directions, from rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE scaled by the
+# k3_copy_old calibration (profiles/gks3d_tgv256_fast_hbm_traffic.md)
+TRAFFIC3_256_FAST = {"flux": 28.55e9}
 
 
 def kernel_units(phase, n, gp=2, fused=False):
@@ -185,8 +189,6 @@ def units3(phase, n, G=3):
     m = n + 2 * G
     if phase == "flux":
         return (n + 1) * n * n
-    if phase == "point":
-        return 4 * (n + 1) * n * n
     if phase == "line":
         return 2 * (n + 1) * n * (m - 2 * G + 4)
     if phase == "face":
@@ -274,7 +276,8 @@ def bench_tgv3d(args):
     if u and fpu:
         achieved = u * fpu / (avg_ms * 1e-3) / 1e12
         roof = {"bound": "mfma", "achieved": round(achieved, 3), "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
-                "frac": round(achieved / PEAK_FP64_TFLOPS, 4), "traffic": None,
+                "frac": round(achieved / PEAK_FP64_TFLOPS, 4),
+                "traffic": TRAFFIC3_256_FAST.get(dom) if (n == 256 and args.variant == "fast") else None,
                 "kernel": "k3_pflux (x, y, z): point values + GKS3D, one launch" if dom == "flux" else f"k3_{dom} (x, y, z)",
                 "avg_ms": round(avg_ms, 4), "flops_per_launch": u * fpu}
     out = {

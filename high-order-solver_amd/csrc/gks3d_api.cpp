@@ -277,7 +277,6 @@ static int stage(gks3d_ctx* c, int s, double dt, const int* bc) {
   for (int dir = 0; dir < 3; dir++) {
     RC(timed(c, GKS3_PHASE_FACE, [&] { return lt->face(d, dir, c->s); }));
     RC(timed(c, GKS3_PHASE_LINE, [&] { return lt->line(d, dir, c->s); }));
-    RC(timed(c, GKS3_PHASE_POINT, [&] { return lt->point(d, dir, c->s); }));
     RC(timed(c, GKS3_PHASE_FLUX, [&] { return lt->flux(d, dir, s, dt, c->s); }));
   }
   RC(timed(c, GKS3_PHASE_UPDATE, [&] { return lt->update(d, c->s); }));

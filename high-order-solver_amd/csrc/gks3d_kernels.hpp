@@ -8,14 +8,14 @@
 //          (WENO5_AO_splitting_3d, reconstruction.cpp:3774-3923) -> fv
 //   line   one thread per (d face, side): the first tangential
 //          reconstruction (WENO5_AO_of_line_value, :3925-4145) -> lv
-//   point  one thread per (d face that gets a flux, line, side): the second
-//          tangential reconstruction (WENO5_AO_of_point_value, :4147-4399)
-//          -> pv
-//   flux   one thread per (d face that gets a flux, Gauss point): GKS3D
-//          (flux_function.cpp:813-1163), the S2O4 stage combination summed
-//          over the face's 4 points -> xs[d] (and the stage-0 share pc[d])
+//   flux   per d face that gets a flux, in one workgroup's LDS tile: the
+//          second tangential reconstruction (WENO5_AO_of_point_value,
+//          :4147-4399; one lane per face, line and side), then GKS3D
+//          (flux_function.cpp:813-1163; one lane per face and Gauss point),
+//          the S2O4 stage combination summed over the face's 4 points ->
+//          xs[d] (and the stage-0 share pc[d])
 // and finally the cell update (Update_gauss, time_advance.cpp:937-998).
-// The face, line and point buffers are reused by the three directions.
+// The face and line buffers are reused by the three directions.
 //
 // Every thread's index box is the reference's loop range for that array;
 // threads are laid out with k (z) fastest so neighbouring lanes touch
@@ -370,103 +370,19 @@ __global__ void __launch_bounds__(kBlock3) k3_line(Dev3 d, Box3 b) {
     for (int m = 0; m < 15; m++) d.lv[((l * 2 + side) * 15 + m) * d.nf + f] = res[l][m];
 }
 
-// WENO5_AO_of_point_value (reconstruction.cpp:4147-4161) at the faces that
-// get a flux (Calculate_flux, flux_function.cpp:760-811): x and y faces along
-// z (h = dz), z faces along -x (h = dx, reversed).  One thread per (face,
-// line, side); the two sides are neighbouring lanes sharing the
-// order-reduction verdict (fallback: the line value, der1z 0).
-template <int DIR>
-__global__ void __launch_bounds__(kBlock3) k3_point(Dev3 d, Box3 b) {
-  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  const long long fi = t >> 2;
-  const int l = (int)((t >> 1) & 1), side = (int)(t & 1);
-  if (fi >= b.count()) return;  // the four lanes of a face leave together
-  int i, j, k;
-  b.at(fi, i, j, k);
-  const long long f = face3(d, i, j, k);
-  const long long step = DIR == 2 ? -(long long)(d.ny + 1) * (d.nz + 1) : 1LL;
-  const HInv h = hinv(DIR == 2 ? d.dx : d.dz);
-  double res[2][20];  // [point]
-  tangential_side<15>(res, LvStencil{d.lv, d.nf, f, step, l, side}, h, d.recon_var, d.gas);
-  int red = (reduce3(res[0], d.gas) || reduce3(res[1], d.gas)) ? 1 : 0;
-  red |= __shfl_xor(red, 1);
-  if (red) {
-#pragma unroll
-    for (int p = 0; p < 2; p++) {
-#pragma unroll
-      for (int m = 0; m < 15; m++) res[p][m] = d.lv[((l * 2 + side) * 15 + m) * d.nf + f];
-#pragma unroll
-      for (int m = 15; m < 20; m++) res[p][m] = 0.0;
-    }
-  }
-#pragma unroll
-  for (int p = 0; p < 2; p++)
-#pragma unroll
-    for (int m = 0; m < 20; m++) d.pv[((side * 20 + m) * d.nf + f) * 4 + 2 * l + p] = res[p][m];
-}
-
-// one side of a Gauss point in pv
-struct PvSide {
-  const double* p;  // &pv[(side * 20) * nf * 4 + f * 4 + q]
-  long long s;      // nf * 4
-  __device__ __forceinline__ double operator[](int k) const { return p[k * s]; }
-};
-
-// GKS3D at every Gauss point of the faces that get a flux, one thread per
-// (face, point), the four points of a face neighbouring lanes; the S2O4
-// stage combination of Update_gauss (time_advance.cpp:867-935) — stage 0
-// x = 0.5 f + 1/8 derf (keeping 1.0 f + 1/6 derf for stage 1), stage 1
-// x = that + 1/3 derf — summed over the four points by lane shuffles.
-template <int DIR>
-__global__ void __launch_bounds__(kBlock3) k3_flux(Dev3 d, Box3 b, int stage, double dt) {
-  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  const long long fi = t >> 2;
-  const int q = (int)(t & 3);
-  if (fi >= b.count()) return;  // the four lanes of a face leave together
-  int i, j, k;
-  b.at(fi, i, j, k);
-  const long long f = face3(d, i, j, k);
-  const long long s4 = d.nf * 4;
-  const PvSide L{d.pv + f * 4 + q, s4}, R{d.pv + 20 * s4 + f * 4 + q, s4};
-  double fo[5], dfo[5];
-  gks3d_point_dev(fo, dfo, nullptr, L, R, dt, d.gas);
-  double x[5], share[5];
-#pragma unroll
-  for (int v = 0; v < 5; v++) {
-    if (stage == 0) {
-      x[v] = 0.0 + (0.5 * fo[v] + 1.0 / 8.0 * dfo[v]);
-      share[v] = 0.0 + (1.0 * fo[v] + 1.0 / 6.0 * dfo[v]);
-    } else {
-      x[v] = 0.0 * fo[v] + 1.0 / 3.0 * dfo[v];
-      share[v] = 0.0;
-    }
-    x[v] += __shfl_xor(x[v], 1);
-    x[v] += __shfl_xor(x[v], 2);
-    share[v] += __shfl_xor(share[v], 1);
-    share[v] += __shfl_xor(share[v], 2);
-  }
-  if (q == 0) {
-    double* xs = d.xs[DIR];
-    double* pc = d.pc[DIR];
-#pragma unroll
-    for (int v = 0; v < 5; v++) {
-      const long long o = (long long)v * d.nf + f;
-      if (stage == 0) {
-        xs[o] = x[v];
-        pc[o] = share[v];
-      } else {
-        xs[o] = pc[o] + x[v];
-      }
-    }
-  }
-}
-
-// Point values and GKS3D fused through LDS: a workgroup of kPfBlock lanes
-// takes kPfFaces faces.  Phase 1, one lane per (face, line, side): the
-// second tangential reconstruction (as k3_point) into the workgroup's LDS
-// tile; phase 2, one lane per (face, Gauss point): GKS3D reading its left
-// and right states from the tile, the S2O4 stage sum over the face's four
-// points by lane shuffles (as k3_flux).  The tile (kPfFaces x 2 sides x 4
+// WENO5_AO_of_point_value (reconstruction.cpp:4147-4161) and GKS3D at the
+// faces that get a flux (Calculate_flux, flux_function.cpp:760-811), fused
+// through LDS: a workgroup of kPfBlock lanes takes kPfFaces faces.
+// Phase 1, one lane per (face, line, side): the second tangential
+// reconstruction — x and y faces along z (h = dz), z faces along -x
+// (h = dx, reversed) — into the workgroup's LDS tile; the two sides are
+// neighbouring lanes sharing the order-reduction verdict (fallback: the line
+// value, der1z 0).  Phase 2, one lane per (face, Gauss point): GKS3D
+// (flux_function.cpp:813-1163) reading its left and right states from the
+// tile, and the S2O4 stage combination of Update_gauss
+// (time_advance.cpp:867-935) — stage 0 x = 0.5 f + 1/8 derf (keeping
+// 1.0 f + 1/6 derf for stage 1), stage 1 x = that + 1/3 derf — summed over
+// the face's four points by lane shuffles.  The tile (kPfFaces x 2 sides x 4
 // points x 20 values = 40 KB) never leaves the CU; four workgroups fill
 // the 160 KB of a CU's LDS.
 constexpr int kPfBlock = 128;
@@ -605,12 +521,6 @@ template __global__ void k3_face<2>(Dev3, Box3);
 template __global__ void k3_line<0>(Dev3, Box3);
 template __global__ void k3_line<1>(Dev3, Box3);
 template __global__ void k3_line<2>(Dev3, Box3);
-template __global__ void k3_point<0>(Dev3, Box3);
-template __global__ void k3_point<1>(Dev3, Box3);
-template __global__ void k3_point<2>(Dev3, Box3);
-template __global__ void k3_flux<0>(Dev3, Box3, int, double);
-template __global__ void k3_flux<1>(Dev3, Box3, int, double);
-template __global__ void k3_flux<2>(Dev3, Box3, int, double);
 template __global__ void k3_pflux<0>(Dev3, Box3, int, double);
 template __global__ void k3_pflux<1>(Dev3, Box3, int, double);
 template __global__ void k3_pflux<2>(Dev3, Box3, int, double);
@@ -679,29 +589,13 @@ static hipError_t launch_line(const Dev3& d, int dir, hipStream_t s) {
   else k3_line<2><<<nb, kBlock3, 0, s>>>(d, b);
   return hipGetLastError();
 }
-static hipError_t launch_point(const Dev3& d, int dir, hipStream_t s) {
-  const Box3 b = flux_box(d, dir);
-  if (box_count(b) == 0 || !d.pv) return hipSuccess;  // without pv the flux launch does the points
-  const unsigned nb = blocks_for(4 * box_count(b));
-  if (dir == 0) k3_point<0><<<nb, kBlock3, 0, s>>>(d, b);
-  else if (dir == 1) k3_point<1><<<nb, kBlock3, 0, s>>>(d, b);
-  else k3_point<2><<<nb, kBlock3, 0, s>>>(d, b);
-  return hipGetLastError();
-}
 static hipError_t launch_flux(const Dev3& d, int dir, int stage, double dt, hipStream_t s) {
   const Box3 b = flux_box(d, dir);
   if (box_count(b) == 0) return hipSuccess;
-  if (!d.pv) {  // fused point values + GKS3D
-    const unsigned nbf = (unsigned)((box_count(b) + kPfFaces - 1) / kPfFaces);
-    if (dir == 0) k3_pflux<0><<<nbf, kPfBlock, 0, s>>>(d, b, stage, dt);
-    else if (dir == 1) k3_pflux<1><<<nbf, kPfBlock, 0, s>>>(d, b, stage, dt);
-    else k3_pflux<2><<<nbf, kPfBlock, 0, s>>>(d, b, stage, dt);
-    return hipGetLastError();
-  }
-  const unsigned nb = blocks_for(4 * box_count(b));
-  if (dir == 0) k3_flux<0><<<nb, kBlock3, 0, s>>>(d, b, stage, dt);
-  else if (dir == 1) k3_flux<1><<<nb, kBlock3, 0, s>>>(d, b, stage, dt);
-  else k3_flux<2><<<nb, kBlock3, 0, s>>>(d, b, stage, dt);
+  const unsigned nb = (unsigned)((box_count(b) + kPfFaces - 1) / kPfFaces);
+  if (dir == 0) k3_pflux<0><<<nb, kPfBlock, 0, s>>>(d, b, stage, dt);
+  else if (dir == 1) k3_pflux<1><<<nb, kPfBlock, 0, s>>>(d, b, stage, dt);
+  else k3_pflux<2><<<nb, kPfBlock, 0, s>>>(d, b, stage, dt);
   return hipGetLastError();
 }
 static hipError_t launch_update(const Dev3& d, hipStream_t s) {
@@ -715,9 +609,8 @@ static hipError_t launch_flux_points(const Gas3& g, long long n, const double* L
   return hipGetLastError();
 }
 
-extern const Launch3dTable table3d = {launch_bc,    launch_primvar, launch_copy_old, launch_face,
-                                      launch_line,  launch_point,   launch_flux,     launch_update,
-                                      launch_flux_points};
+extern const Launch3dTable table3d = {launch_bc,   launch_primvar, launch_copy_old,   launch_face,
+                                      launch_line, launch_flux,    launch_update,     launch_flux_points};
 #endif
 
 }  // namespace GKS_VARIANT

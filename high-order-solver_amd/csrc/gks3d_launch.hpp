@@ -20,8 +20,6 @@ struct Dev3 {
   double* fv;     // [2 side][10][nf] face values of the direction in work:
                   //   side 0 the left cell's, side 1 the right cell's (convar, der1x)
   double* lv;     // [2 line][2 side][15][nf] line values (convar, der1x, der1y)
-  double* pv;     // [2 side][20][nf][4 point] Gauss-point values (convar, der1x,
-                  //   der1y, der1z), point q = 2 line + p fastest
   double* xs[3];  // [5][nf] per direction: the stage combination of the
                   //   face's four Gauss points, summed
   double* pc[3];  // [5][nf] per direction: the stage-0 share of stage 1
@@ -33,10 +31,9 @@ struct Launch3dTable {
   hipError_t (*bc)(const Dev3&, const int* bc6, hipStream_t);
   hipError_t (*primvar)(const Dev3&, hipStream_t);  // sweep + reduction into stat
   hipError_t (*copy_old)(const Dev3&, hipStream_t);
-  // one direction's face values, line values, point values, GKS3D + stage sum
+  // one direction's face values, line values, point values + GKS3D + stage sum
   hipError_t (*face)(const Dev3&, int dir, hipStream_t);
   hipError_t (*line)(const Dev3&, int dir, hipStream_t);
-  hipError_t (*point)(const Dev3&, int dir, hipStream_t);
   hipError_t (*flux)(const Dev3&, int dir, int stage, double dt, hipStream_t);
   hipError_t (*update)(const Dev3&, hipStream_t);
   hipError_t (*flux_points)(const gksd3::Gas3&, long long n, const double* L, const double* R, double dt,

@@ -36,7 +36,7 @@ SIGS3 = {
     "gks3d_last_error": (ctypes.c_char_p, []),
 }
 
-PHASES3 = ["bc", "primvar", "face", "line", "point", "flux", "update", "copy_old"]
+PHASES3 = ["bc", "primvar", "face", "line", "flux", "update", "copy_old"]
 
 _lib3 = None
 

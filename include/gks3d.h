@@ -73,8 +73,9 @@ int gks3d_sync(gks3d_ctx* ctx);
  * gks3d_timing(ctx, 1) resets and starts accumulating; gks3d_get_timing
  * fills ms[GKS3_NPHASE] (summed milliseconds) and launches[GKS3_NPHASE]. */
 enum {
-  GKS3_PHASE_BC = 0, GKS3_PHASE_PRIMVAR, GKS3_PHASE_FACE, GKS3_PHASE_LINE, GKS3_PHASE_POINT,
-  GKS3_PHASE_FLUX, GKS3_PHASE_UPDATE, GKS3_PHASE_COPY_OLD, GKS3_NPHASE
+  GKS3_PHASE_BC = 0, GKS3_PHASE_PRIMVAR, GKS3_PHASE_FACE, GKS3_PHASE_LINE,
+  GKS3_PHASE_FLUX,  /* point values + GKS3D, one launch per direction */
+  GKS3_PHASE_UPDATE, GKS3_PHASE_COPY_OLD, GKS3_NPHASE
 };
 int gks3d_timing(gks3d_ctx* ctx, int enable);
 int gks3d_get_timing(gks3d_ctx* ctx, double* ms, int64_t* launches);
