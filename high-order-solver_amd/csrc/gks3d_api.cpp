@@ -69,6 +69,7 @@ struct gks3d_ctx {
   hipStream_t s = nullptr;
   double* h_stat = nullptr;  // pinned [4]
   bool have_speed = false;   // CFL inputs known (set_cfl_primvar or a step)
+  bool stat_pending = false; // a primitive sweep's reduction waits in d.stat
   double tmax = 0.0, rhomin = 0.0;
   std::vector<void*> allocs;
   // per-phase HIP-event timing (gks3d_timing)
@@ -310,6 +311,7 @@ int gks3d_run(gks3d_ctx* c, const int32_t bc[6], double tstop, int32_t max_steps
     RC(stage(c, 1, dt, b));
     HIPCHK(hipMemcpyAsync(c->h_stat, d.stat, 4 * sizeof(double), hipMemcpyDeviceToHost, c->s));
     HIPCHK(hipStreamSynchronize(c->s));
+    c->stat_pending = false;
     if (c->timing) RC(collect_timing(c));
     if (c->h_stat[2] != 0.0) {
       blew = true;
@@ -330,6 +332,64 @@ int gks3d_run(gks3d_ctx* c, const int32_t bc[6], double tstop, int32_t max_steps
     return fail(GKS_ERR_BLOWUP, "density or pressure negative or NaN");
   }
   return GKS_OK;
+}
+
+int gks3d_boundary(gks3d_ctx* c, int32_t side, int32_t kind) {
+  if (!c) return fail(GKS_ERR_ARG, "null");
+  if (side < 0 || side > 5) return fail(GKS_ERR_ARG, "side");
+  if (kind != GKS_BC_FREE && kind != GKS_BC_PERIODIC) return fail(GKS_ERR_UNSUPPORTED, "3D boundaries are free or periodic");
+  const Dev3& d = c->d;
+  const int N = side < 2 ? d.NX : (side < 4 ? d.NY : d.NZ);
+  if (kind == GKS_BC_PERIODIC && N < d.G) return fail(GKS_ERR_ARG, "periodic boundaries need at least ghost interior cells");
+  return timed(c, GKS3_PHASE_BC, [&] { return c->lt->bc_side(d, side, kind, c->s); });
+}
+int gks3d_primvar(gks3d_ctx* c) {
+  if (!c) return fail(GKS_ERR_ARG, "null");
+  RC(timed(c, GKS3_PHASE_PRIMVAR, [&] { return c->lt->primvar(c->d, c->s); }));
+  c->stat_pending = true;
+  return GKS_OK;
+}
+int gks3d_copy_old(gks3d_ctx* c) {
+  if (!c) return fail(GKS_ERR_ARG, "null");
+  return timed(c, GKS3_PHASE_COPY_OLD, [&] { return c->lt->copy_old(c->d, c->s); });
+}
+int gks3d_cfl(gks3d_ctx* c, double t, double tstop, double* dt) {
+  if (!c || !dt) return fail(GKS_ERR_ARG, "null");
+  if (c->stat_pending) {
+    HIPCHK(hipMemcpyAsync(c->h_stat, c->d.stat, 4 * sizeof(double), hipMemcpyDeviceToHost, c->s));
+    HIPCHK(hipStreamSynchronize(c->s));
+    if (c->timing) RC(collect_timing(c));
+    c->stat_pending = false;
+    if (c->h_stat[2] != 0.0) {
+      const double zero = 0.0;
+      HIPCHK(hipMemcpyAsync(c->d.stat + 2, &zero, sizeof(double), hipMemcpyHostToDevice, c->s));
+      HIPCHK(hipStreamSynchronize(c->s));
+      return fail(GKS_ERR_BLOWUP, "density or pressure negative or NaN");
+    }
+    c->tmax = c->h_stat[0];
+    c->rhomin = c->h_stat[1];
+    c->have_speed = true;
+  }
+  if (!c->have_speed) return fail(GKS_ERR_STATE, "no primitive state yet: call gks3d_set_cfl_primvar first");
+  *dt = cfl_dt(c, t, tstop);
+  return GKS_OK;
+}
+int gks3d_flux(gks3d_ctx* c, int32_t stage, double dt) {
+  if (!c) return fail(GKS_ERR_ARG, "null");
+  if (stage < 0 || stage > 1) return fail(GKS_ERR_ARG, "stage");
+  const Dev3& d = c->d;
+  const Launch3dTable* lt = c->lt;
+  for (int dir = 0; dir < 3; dir++) {
+    RC(timed(c, GKS3_PHASE_FACE, [&] { return lt->face(d, dir, c->s); }));
+    RC(timed(c, GKS3_PHASE_LINE, [&] { return lt->line(d, dir, c->s); }));
+    RC(timed(c, GKS3_PHASE_FLUX, [&] { return lt->flux(d, dir, stage, dt, c->s); }));
+  }
+  return GKS_OK;
+}
+int gks3d_update(gks3d_ctx* c, int32_t stage) {
+  if (!c) return fail(GKS_ERR_ARG, "null");
+  if (stage < 0 || stage > 1) return fail(GKS_ERR_ARG, "stage");
+  return timed(c, GKS3_PHASE_UPDATE, [&] { return c->lt->update(c->d, c->s); });
 }
 
 int gks3d_timing(gks3d_ctx* c, int enable) {

@@ -90,6 +90,25 @@ __global__ void __launch_bounds__(kBlock3) k3_bc(Dev3 d, int b0, int b1, int b2,
   for (int v = 0; v < 5; v++) d.U[v * d.nc + t] = d.U[v * d.nc + src];
 }
 
+// one side's fill, as the reference's boundary functions apply it (the ghost
+// slab of that side over the full extent of the other two axes); sides in
+// the reference's order x-, x+, y-, y+, z-, z+
+__global__ void __launch_bounds__(kBlock3) k3_bc_side(Dev3 d, Box3 b, int side, int kind) {
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= b.count()) return;
+  int i, j, k;
+  b.at(t, i, j, k);
+  const int G = d.G, axis = side >> 1;
+  const bool lo = (side & 1) == 0;
+  const int N = axis == 0 ? d.NX : (axis == 1 ? d.NY : d.NZ);
+  const int x = axis == 0 ? i : (axis == 1 ? j : k);
+  const int sx = kind == 0 ? (lo ? G : N + G - 1) : (lo ? x + N : x - N);
+  const long long dst = cell3(d, i, j, k);
+  const long long src = cell3(d, axis == 0 ? sx : i, axis == 1 ? sx : j, axis == 2 ? sx : k);
+#pragma unroll
+  for (int v = 0; v < 5; v++) d.U[v * d.nc + dst] = d.U[v * d.nc + src];
+}
+
 // ------------------------------------------------------- primitive sweep
 // Convar_to_primvar over the interior (basic_function.cpp:775-804) reduced
 // to what the next step's Get_CFL / Dtx_3d (time_advance.cpp:793-848) and
@@ -534,6 +553,16 @@ static hipError_t launch_bc(const Dev3& d, const int* bc, hipStream_t s) {
   k3_bc<<<blocks_for(d.nc), kBlock3, 0, s>>>(d, bc[0], bc[1], bc[2], bc[3], bc[4], bc[5]);
   return hipGetLastError();
 }
+static hipError_t launch_bc_side(const Dev3& d, int side, int kind, hipStream_t s) {
+  const int G = d.G, axis = side >> 1;
+  const bool lo = (side & 1) == 0;
+  const int n = axis == 0 ? d.nx : (axis == 1 ? d.ny : d.nz);
+  const int a0 = lo ? 0 : n - G, a1 = lo ? G : n;
+  const Box3 b = make_box(axis == 0 ? a0 : 0, axis == 0 ? a1 : d.nx, axis == 1 ? a0 : 0, axis == 1 ? a1 : d.ny,
+                          axis == 2 ? a0 : 0, axis == 2 ? a1 : d.nz);
+  if (box_count(b) > 0) k3_bc_side<<<blocks_for(box_count(b)), kBlock3, 0, s>>>(d, b, side, kind);
+  return hipGetLastError();
+}
 static hipError_t launch_primvar(const Dev3& d, hipStream_t s) {
   const long long n = (long long)d.NX * d.NY * d.NZ;
   long long nb = (n + kBlock3 - 1) / kBlock3;
@@ -609,8 +638,8 @@ static hipError_t launch_flux_points(const Gas3& g, long long n, const double* L
   return hipGetLastError();
 }
 
-extern const Launch3dTable table3d = {launch_bc,   launch_primvar, launch_copy_old,   launch_face,
-                                      launch_line, launch_flux,    launch_update,     launch_flux_points};
+extern const Launch3dTable table3d = {launch_bc,   launch_bc_side, launch_primvar, launch_copy_old,   launch_face,
+                                      launch_line, launch_flux,    launch_update,  launch_flux_points};
 #endif
 
 }  // namespace GKS_VARIANT

@@ -29,6 +29,7 @@ struct Dev3 {
 
 struct Launch3dTable {
   hipError_t (*bc)(const Dev3&, const int* bc6, hipStream_t);
+  hipError_t (*bc_side)(const Dev3&, int side, int kind, hipStream_t);  // one side's fill
   hipError_t (*primvar)(const Dev3&, hipStream_t);  // sweep + reduction into stat
   hipError_t (*copy_old)(const Dev3&, hipStream_t);
   // one direction's face values, line values, point values + GKS3D + stage sum

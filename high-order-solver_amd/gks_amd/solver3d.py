@@ -30,6 +30,12 @@ SIGS3 = {
     "gks3d_set_cfl_primvar": (ctypes.c_int, [_P, _D, _I64, _I64]),
     "gks3d_run": (ctypes.c_int, [_P, _I32P, ctypes.c_double, ctypes.c_int32, _D, _I32P]),
     "gks3d_sync": (ctypes.c_int, [_P]),
+    "gks3d_boundary": (ctypes.c_int, [_P, ctypes.c_int32, ctypes.c_int32]),
+    "gks3d_primvar": (ctypes.c_int, [_P]),
+    "gks3d_copy_old": (ctypes.c_int, [_P]),
+    "gks3d_cfl": (ctypes.c_int, [_P, ctypes.c_double, ctypes.c_double, _D]),
+    "gks3d_flux": (ctypes.c_int, [_P, ctypes.c_int32, ctypes.c_double]),
+    "gks3d_update": (ctypes.c_int, [_P, ctypes.c_int32]),
     "gks3d_timing": (ctypes.c_int, [_P, ctypes.c_int]),
     "gks3d_get_timing": (ctypes.c_int, [_P, _D, ctypes.POINTER(ctypes.c_int64)]),
     "gks3d_flux_points": (ctypes.c_int, [ctypes.POINTER(Params3D), _I64, _D, _D, ctypes.c_double, _D, _D, _D]),
@@ -163,6 +169,27 @@ class Solver3D:
 
     def sync(self):
         _check(lib3().gks3d_sync(self.h))
+
+    # one sweep at a time (the reference's loop on the host)
+    def boundary(self, side, kind):
+        _check(lib3().gks3d_boundary(self.h, side, kind))
+
+    def primvar(self):
+        _check(lib3().gks3d_primvar(self.h))
+
+    def copy_old(self):
+        _check(lib3().gks3d_copy_old(self.h))
+
+    def cfl(self, tstop):
+        dt = ctypes.c_double()
+        _check(lib3().gks3d_cfl(self.h, self.t, tstop, ctypes.byref(dt)))
+        return dt.value
+
+    def flux(self, stage, dt):
+        _check(lib3().gks3d_flux(self.h, stage, dt))
+
+    def update(self, stage):
+        _check(lib3().gks3d_update(self.h, stage))
 
     def timing(self, on=True):
         """Resets and (with on) starts per-phase HIP-event timing."""

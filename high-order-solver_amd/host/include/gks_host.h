@@ -1,7 +1,8 @@
-// gks_host.h — the reference solver's 2D call surface (types, globals,
-// function-pointer plugin points and sweep functions) over the gfx950 C ABI
-// (include/gks2d.h).  A 2D problem driver written against the reference
-// (src/2Dproblem.cpp style: set globals, assign the scheme pointers, call the
+// gks_host.h — the reference solver's 2D and 3D call surface (types,
+// globals, function-pointer plugin points and sweep functions) over the
+// gfx950 C ABIs (include/gks2d.h, include/gks3d.h).  A 2D or 3D problem
+// driver written against the reference (src/2Dproblem.cpp /
+// src/3Dproblem.cpp style: set globals, assign the scheme pointers, call the
 // sweeps in order) compiles against these headers unchanged and runs its
 // per-step hot path on the GPU.
 //
@@ -233,3 +234,181 @@ void Download_fluid(Fluid2d* fluids, Block2d block);
 void Upload_fluid(Fluid2d* fluids, Block2d block);
 // releases the device state bound to `fluids`
 void Release_fluid(Fluid2d* fluids);
+
+// ======================================================================= 3D
+// (fluid_mesh.h:225-351, basic_function.h:585-710, boundary_condition.h:35-41,
+// reconstruction.h:96-136, flux_function.h:30-41, time_advance.h:47-53,
+// output.h:33).  The 3D scheme is the one of include/gks3d.h: the reference's
+// algorithm with lambda from the state (its GKS3D takes the pressure) and the
+// Gauss-point records on the device (it never allocates them).
+extern double** gauss_loc_3d;
+extern bool is_multi_dimensional;
+
+class Block3d {
+ public:
+  bool uniform = true;
+  int ghost = 3;
+  int nodex = 0, nodey = 0, nodez = 0;
+  int nx = 0, ny = 0, nz = 0;
+  double dx = 0, dy = 0, dz = 0;
+  double overdx = 0, overdy = 0, overdz = 0;
+  double xarea = 0, yarea = 0, zarea = 0;
+  double left = 0, right = 0;
+  double down = 0, up = 0;
+  double front = 0, back = 0;
+  int stages = 0;
+  double timecoefficient[5][5][3] = {};
+  double timecoefficient_hweno[5][5][3] = {};
+  double t = 0;
+  double CFL = 0;
+  double dt = 0;
+  double tstop = 0;
+  int step = 0;
+  double xcell_begin = 0, xcell_end = 0, ycell_begin = 0, ycell_end = 0, zcell_begin = 0, zcell_end = 0;
+  int index = 0;
+};
+
+class Fluid3d {
+ public:
+  double primvar[6];
+  double convar[6];
+  double convar_old[6];
+  int index[3];
+  double loc[3];
+  double node[24];
+  double vol;
+  double R_c;
+};
+
+class Point3d {
+ public:
+  double convar[6];
+  double der1x[6];
+  double der1y[6];
+  double der1z[6];
+};
+
+class Recon3d {
+ public:
+  Point3d left;
+  Point3d center;
+  Point3d right;
+  bool order_reduce;
+};
+
+// geometry of one face; its line and Gauss-point states live on the device
+class Interface3d {
+ public:
+  double loc[12] = {};
+  double weight[4] = {};
+  double normal[12] = {};
+  int index[3] = {0, 0, 0};
+  double area = 0;
+};
+
+class Flux3d {
+ public:
+  double x[6];
+  double f[6];
+  double derf[6];
+};
+
+// per-face flux records live on the device
+class Flux3d_gauss {
+ public:
+  Flux3d* gauss = nullptr;
+};
+
+void SetGuassPoint_3D();
+
+Fluid3d* Setfluid_array(Block3d& block);
+Interface3d* Setinterface_array(Block3d& block);
+Flux3d_gauss** Setflux_gauss_array(Block3d& block);
+void SetUniformMesh(Block3d block, Fluid3d* fluids, Interface3d* xinterfaces, Interface3d* yinterfaces,
+                    Interface3d* zinterfaces);
+void CopyFluid_new_to_old(Fluid3d* fluids, Block3d block);
+void Convar_to_primvar(Fluid3d* fluids, Block3d& block);
+void Convar_to_primvar_3D(double* primvar, double convar[5]);
+void Primvar_to_convar_3D(double* convar, double* primvar);
+
+typedef void (*BoundaryCondition3d)(Fluid3d* fluids, Block3d block, Fluid3d bcvalue);
+void free_boundary_xleft(Fluid3d* fluids, Block3d block, Fluid3d bcvalue);
+void free_boundary_xright(Fluid3d* fluids, Block3d block, Fluid3d bcvalue);
+void free_boundary_yleft(Fluid3d* fluids, Block3d block, Fluid3d bcvalue);
+void free_boundary_yright(Fluid3d* fluids, Block3d block, Fluid3d bcvalue);
+void free_boundary_zleft(Fluid3d* fluids, Block3d block, Fluid3d bcvalue);
+void free_boundary_zright(Fluid3d* fluids, Block3d block, Fluid3d bcvalue);
+// periodic 3D boundaries: not in the reference, same signature
+void periodic_boundary_xleft(Fluid3d* fluids, Block3d block, Fluid3d bcvalue);
+void periodic_boundary_xright(Fluid3d* fluids, Block3d block, Fluid3d bcvalue);
+void periodic_boundary_yleft(Fluid3d* fluids, Block3d block, Fluid3d bcvalue);
+void periodic_boundary_yright(Fluid3d* fluids, Block3d block, Fluid3d bcvalue);
+void periodic_boundary_zleft(Fluid3d* fluids, Block3d block, Fluid3d bcvalue);
+void periodic_boundary_zright(Fluid3d* fluids, Block3d block, Fluid3d bcvalue);
+
+enum G0_construct_type { collisionn, collisionnless, all_collisionn };
+extern G0_construct_type g0type;
+typedef void (*Reconstruction_within_Cell_3D_of_face_value)(Interface3d& left, Interface3d& right, Interface3d& down,
+                                                            Interface3d& up, Interface3d& back, Interface3d& front,
+                                                            Fluid3d* fluids, Block3d block);
+extern Reconstruction_within_Cell_3D_of_face_value cellreconstruction_3D_of_face_value;
+typedef void (*Reconstruction_within_Cell_3D_of_line_value)(Interface3d* xinterface, Interface3d* yinterface,
+                                                            Interface3d* zinterface, Block3d block);
+extern Reconstruction_within_Cell_3D_of_line_value cellreconstruction_3D_of_line_value;
+typedef void (*Reconstruction_within_Cell_3D_of_point_value)(Interface3d* xinterface, Interface3d* yinterface,
+                                                             Interface3d* zinterface, Block3d block);
+extern Reconstruction_within_Cell_3D_of_point_value cellreconstruction_3D_of_point_value;
+typedef void (*Reconstruction_forG0_3D_of_face_value)(Interface3d* xinterfaces, Interface3d* yinterfaces,
+                                                      Interface3d* zinterfaces, Fluid3d* fluids, Block3d block);
+extern Reconstruction_forG0_3D_of_face_value g0reconstruction_3D_of_face_value;
+typedef void (*Reconstruction_forG0_3D_of_line_value)(Interface3d* xinterfaces, Interface3d* yinterfaces,
+                                                      Interface3d* zinterfaces, Block3d block);
+extern Reconstruction_forG0_3D_of_line_value g0reconstruction_3D_of_line_value;
+typedef void (*Reconstruction_forG0_3D_of_point_value)(Interface3d* xinterfaces, Interface3d* yinterfaces,
+                                                       Interface3d* zinterfaces, Block3d block);
+extern Reconstruction_forG0_3D_of_point_value g0reconstruction_3D_of_point_value;
+
+// plugin leaves (their work is fused into the device sweeps)
+void WENO5_AO_splitting_3d(Interface3d& left, Interface3d& right, Interface3d& down, Interface3d& up,
+                           Interface3d& back, Interface3d& front, Fluid3d* fluids, Block3d block);
+void WENO5_AO_of_line_value(Interface3d* right, Interface3d* up, Interface3d* front, Block3d block);
+void WENO5_AO_of_point_value(Interface3d* right, Interface3d* up, Interface3d* front, Block3d block);
+void Do_nothing_splitting_3d(Interface3d* xinterfaces, Interface3d* yinterfaces, Interface3d* zinterfaces,
+                             Fluid3d* fluids, Block3d block);
+void Do_nothing_reconstruction_of_line_value(Interface3d* xinterfaces, Interface3d* yinterfaces,
+                                             Interface3d* zinterfaces, Block3d block);
+void Do_nothing_reconstruction_of_point_value(Interface3d* xinterfaces, Interface3d* yinterfaces,
+                                              Interface3d* zinterfaces, Block3d block);
+void Reconstruction_within_cell(Interface3d* xinterfaces, Interface3d* yinterfaces, Interface3d* zinterfaces,
+                                Fluid3d* fluids, Block3d block);
+void Reconstruction_forg0(Interface3d* xinterfaces, Interface3d* yinterfaces, Interface3d* zinterfaces,
+                          Fluid3d* fluids, Block3d block);
+
+enum GKS3d_type { nothing_3d, kfvs1st_3d, kfvs2nd_3d, gks1st_3d, gks2nd_3d };
+extern GKS3d_type gks3dsolver;
+typedef void (*Flux_function_3d)(Flux3d& flux, Recon3d& re, double area, double dt);
+extern Flux_function_3d flux_function_3d;
+// one Gauss point on the GPU (gks2nd_3d with the S2O4 pair)
+void GKS3D(Flux3d& flux, Recon3d& interface, double area, double dt);
+void Calculate_flux(Flux3d_gauss** xfluxes, Flux3d_gauss** yfluxes, Flux3d_gauss** zfluxes, Interface3d* xinterfaces,
+                    Interface3d* yinterfaces, Interface3d* zinterfaces, Block3d block, int stage);
+
+typedef void (*TimeMarchingCoefficient_3d)(Block3d& block);
+extern TimeMarchingCoefficient_3d timecoe_list_3d;
+void S2O4_3D(Block3d& block);
+void Initial_stages(Block3d& block);
+double Get_CFL(Block3d& block, Fluid3d* fluids, double tstop);
+void Update_gauss(Fluid3d* fluids, Flux3d_gauss** xfluxes, Flux3d_gauss** yfluxes, Flux3d_gauss** zfluxes,
+                  Block3d block, int stage);
+
+// writes convar[0] of the interior, k outer / j / i inner, to ../data/R3d.txt
+// (output.cpp:86-104), after copying the device state back
+void output3d(Fluid3d* fluids, Block3d block);
+
+// ICfor_cubic_sod (3Dproblem.cpp:174-223)
+void ICfor_cubic_sod(Fluid3d* fluids, double* zone1, double* zone2, Block3d block);
+
+// device residency (shim only), as for Fluid2d
+void Download_fluid(Fluid3d* fluids, Block3d block);
+void Upload_fluid(Fluid3d* fluids, Block3d block);
+void Release_fluid(Fluid3d* fluids);

@@ -430,3 +430,382 @@ void IC_for_riemann_2d(Fluid2d* fluid, double* zone1, double* zone2, double* zon
     for (int j = 0; j < block.ny; j++)
       Primvar_to_convar_2D(fluid[i * block.ny + j].convar, fluid[i * block.ny + j].primvar);
 }
+
+// ======================================================================= 3D
+// The 3D call surface over include/gks3d.h.  Defaults of the scheme globals
+// as the reference initialises them (fluid_mesh.cpp:6-7,
+// reconstruction.cpp:3690-3696, flux_function.cpp:757-758); the reference's
+// default time scheme (S1O1_3D) is not implemented, so timecoe_list_3d
+// starts unset.
+#include "../../../include/gks3d.h"
+
+double** gauss_loc_3d = nullptr;
+bool is_multi_dimensional = true;
+G0_construct_type g0type = all_collisionn;
+Reconstruction_within_Cell_3D_of_face_value cellreconstruction_3D_of_face_value = WENO5_AO_splitting_3d;
+Reconstruction_forG0_3D_of_face_value g0reconstruction_3D_of_face_value = Do_nothing_splitting_3d;
+Reconstruction_within_Cell_3D_of_line_value cellreconstruction_3D_of_line_value = WENO5_AO_of_line_value;
+Reconstruction_within_Cell_3D_of_point_value cellreconstruction_3D_of_point_value = WENO5_AO_of_point_value;
+Reconstruction_forG0_3D_of_line_value g0reconstruction_3D_of_line_value = Do_nothing_reconstruction_of_line_value;
+Reconstruction_forG0_3D_of_point_value g0reconstruction_3D_of_point_value = Do_nothing_reconstruction_of_point_value;
+GKS3d_type gks3dsolver = gks2nd_3d;
+Flux_function_3d flux_function_3d = GKS3D;
+TimeMarchingCoefficient_3d timecoe_list_3d = nullptr;
+
+namespace {
+
+std::map<const void*, Fluid3d*> g_owner3;  // interface / flux arrays -> cell array
+std::map<const Fluid3d*, gks3d_ctx*> g_bound3;
+
+void check3(int rc, const char* where) {
+  if (rc != GKS_OK) die(where, gks3d_last_error());
+}
+
+gks3d_params snapshot3(const Block3d& b) {
+  gks3d_params p;
+  std::memset(&p, 0, sizeof(p));
+  p.nodex = b.nodex;
+  p.nodey = b.nodey;
+  p.nodez = b.nodez;
+  p.ghost = b.ghost;
+  p.recon_variable = reconstruction_variable == characteristic ? GKS_VAR_CHARACTERISTIC : GKS_VAR_CONSERVATIVE;
+  p.tau_type = tau_type == NS ? GKS_TAU_NS : GKS_TAU_EULER;
+  p.K = K;
+  const char* fm = getenv("GKS_CONTRACTED");
+  p.fast_math = (fm && fm[0] == '1') ? 1 : 0;
+  p.gamma = Gamma;
+  p.mu = Mu;
+  p.nu = Nu;
+  p.c1_euler = c1_euler;
+  p.c2_euler = c2_euler;
+  p.left = b.left;
+  p.right = b.right;
+  p.down = b.down;
+  p.up = b.up;
+  p.back = b.back;
+  p.front = b.front;
+  p.cfl = b.CFL;
+  return p;
+}
+
+void upload3(gks3d_ctx* ctx, Fluid3d* f) {
+  const int64_t cs = sizeof(Fluid3d);
+  check3(gks3d_set_convar(ctx, &f[0].convar[0], cs, sizeof(double)), "upload convar");
+  check3(gks3d_set_cfl_primvar(ctx, &f[0].primvar[0], cs, sizeof(double)), "upload primvar");
+}
+
+gks3d_ctx* bind3(Fluid3d* fluids, const Block3d& block) {
+  auto it = g_bound3.find(fluids);
+  if (it != g_bound3.end()) return it->second;
+  if (timecoe_list_3d != S2O4_3D) die("time scheme", "the device path implements S2O4_3D");
+  if (flux_function_3d != GKS3D || gks3dsolver != gks2nd_3d)
+    die("flux", "the device path implements GKS3D with gks2nd_3d");
+  gks3d_params p = snapshot3(block);
+  gks3d_ctx* ctx = nullptr;
+  check3(gks3d_create(&p, &ctx), "gks3d_create");
+  g_bound3[fluids] = ctx;
+  upload3(ctx, fluids);
+  return ctx;
+}
+
+gks3d_ctx* bind3_via(const void* arr) {
+  auto it = g_owner3.find(arr);
+  if (it == g_owner3.end()) die("sweep", "face array not set up by SetUniformMesh");
+  auto b = g_bound3.find(it->second);
+  if (b == g_bound3.end()) die("sweep", "cell array not bound yet (call a cell sweep first)");
+  return b->second;
+}
+
+void boundary3(Fluid3d* fluids, const Block3d& block, int side, int kind) {
+  check3(gks3d_boundary(bind3(fluids, block), side, kind), "boundary");
+}
+
+}  // namespace
+
+// SetGuassPoint_3D (fluid_mesh.cpp:47-95): 1 or 2 x 2 Gauss points on a face
+void SetGuassPoint_3D() {
+  if (!is_multi_dimensional) gausspoint = 0;
+  if (gausspoint != 1 && gausspoint != 4) die("SetGuassPoint_3D", "the device path uses gausspoint = 4 (2 x 2)");
+  gauss_loc_3d = new double*[gausspoint];
+  for (int i = 0; i < gausspoint; i++) gauss_loc_3d[i] = new double[2];
+  delete[] gauss_weight;
+  gauss_weight = new double[gausspoint];
+  if (gausspoint == 1) {
+    gauss_loc_3d[0][0] = gauss_loc_3d[0][1] = 0.0;
+    gauss_weight[0] = 1.0;
+  } else {
+    const double a = -sqrt(1.0 / 3.0);
+    const double loc[4][2] = {{a, a}, {a, -a}, {-a, a}, {-a, -a}};
+    for (int q = 0; q < 4; q++) {
+      gauss_loc_3d[q][0] = loc[q][0];
+      gauss_loc_3d[q][1] = loc[q][1];
+      gauss_weight[q] = 0.25;
+    }
+  }
+}
+
+Fluid3d* Setfluid_array(Block3d& block) {
+  Fluid3d* var = new Fluid3d[(size_t)block.nx * block.ny * block.nz]();
+  for (int i = 0; i < block.nx; i++)
+    for (int j = 0; j < block.ny; j++)
+      for (int k = 0; k < block.nz; k++) {
+        Fluid3d& f = var[((size_t)i * block.ny + j) * block.nz + k];
+        f.index[0] = i;
+        f.index[1] = j;
+        f.index[2] = k;
+      }
+  cout << "fluid memory allocate done" << endl;
+  return var;
+}
+Interface3d* Setinterface_array(Block3d& block) {
+  Interface3d* var = new Interface3d[(size_t)(block.nx + 1) * (block.ny + 1) * (block.nz + 1)];
+  for (int i = 0; i <= block.nx; i++)
+    for (int j = 0; j <= block.ny; j++)
+      for (int k = 0; k <= block.nz; k++) {
+        Interface3d& f = var[((size_t)i * (block.ny + 1) + j) * (block.nz + 1) + k];
+        f.index[0] = i;
+        f.index[1] = j;
+        f.index[2] = k;
+      }
+  cout << "interface variable allocate done..." << endl;
+  return var;
+}
+Flux3d_gauss** Setflux_gauss_array(Block3d& block) {
+  Flux3d_gauss** var = new Flux3d_gauss*[(size_t)(block.nx + 1) * (block.ny + 1) * (block.nz + 1)]();
+  cout << "flux with gausspoint variable allocate done..." << endl;
+  return var;
+}
+
+// the geometry of SetUniformMesh (3D): cell centres, the eight nodes, volume
+// and R_c (the smallest cell size, Get_CFL's length); face Gauss-point
+// locations, weights, normals and areas; also records which cell array the
+// face arrays belong to
+void SetUniformMesh(Block3d block, Fluid3d* fluids, Interface3d* xinterfaces, Interface3d* yinterfaces,
+                    Interface3d* zinterfaces) {
+  if (gausspoint == 0) die("SetUniformMesh", "the number of gauss points must be > 0");
+  const size_t ny = block.ny, nz = block.nz;
+  for (int i = 0; i < block.nx; i++)
+    for (int j = 0; j < block.ny; j++)
+      for (int k = 0; k < block.nz; k++) {
+        Fluid3d& f = fluids[((size_t)i * ny + j) * nz + k];
+        f.loc[0] = block.left + (i + 0.5 - block.ghost) * block.dx;
+        f.loc[1] = block.down + (j + 0.5 - block.ghost) * block.dy;
+        f.loc[2] = block.back + (k + 0.5 - block.ghost) * block.dz;
+        for (int n = 0; n < 8; n++) {  // nodes 0-3 at z-, 4-7 at z+, counter-clockwise from (x-, y-)
+          const int sx = (n & 3) == 1 || (n & 3) == 2, sy = (n & 3) >= 2, sz = n >= 4;
+          f.node[3 * n + 0] = f.loc[0] + (sx ? 0.5 : -0.5) * block.dx;
+          f.node[3 * n + 1] = f.loc[1] + (sy ? 0.5 : -0.5) * block.dy;
+          f.node[3 * n + 2] = f.loc[2] + (sz ? 0.5 : -0.5) * block.dz;
+        }
+        f.vol = block.dx * block.dy * block.dz;
+        f.R_c = block.dx;
+        if (block.dy < f.R_c) f.R_c = block.dy;
+        if (block.dz < f.R_c) f.R_c = block.dz;
+      }
+  for (int i = 0; i <= block.nx; i++)
+    for (int j = 0; j <= block.ny; j++)
+      for (int k = 0; k <= block.nz; k++) {
+        const size_t idx = ((size_t)i * (ny + 1) + j) * (nz + 1) + k;
+        const double x0 = block.left + (i - block.ghost) * block.dx;
+        const double y0 = block.down + (j - block.ghost) * block.dy;
+        const double z0 = block.back + (k - block.ghost) * block.dz;
+        Interface3d* faces[3] = {&xinterfaces[idx], &yinterfaces[idx], &zinterfaces[idx]};
+        const double areas[3] = {block.xarea, block.yarea, block.zarea};
+        for (int d = 0; d < 3; d++) {
+          Interface3d& f = *faces[d];
+          // the face's centre and its two in-plane edge directions
+          double c[3] = {x0 + (d == 0 ? 0.0 : 0.5 * block.dx), y0 + (d == 1 ? 0.0 : 0.5 * block.dy),
+                         z0 + (d == 2 ? 0.0 : 0.5 * block.dz)};
+          double e0[3] = {0, 0, 0}, e1[3] = {0, 0, 0};
+          if (d == 0) { e0[1] = block.dy; e1[2] = block.dz; }
+          else if (d == 1) { e0[0] = block.dx; e1[2] = block.dz; }
+          else { e0[0] = block.dx; e1[1] = block.dy; }
+          for (int q = 0; q < gausspoint; q++) {
+            for (int a = 0; a < 3; a++)
+              f.loc[3 * q + a] = c[a] + 0.5 * e0[a] * gauss_loc_3d[q][0] + 0.5 * e1[a] * gauss_loc_3d[q][1];
+            for (int a = 0; a < 3; a++) f.normal[3 * q + a] = a == d ? 1.0 : 0.0;
+            f.weight[q] = gauss_weight[q];
+          }
+          f.area = areas[d];
+        }
+      }
+  g_owner3[xinterfaces] = fluids;
+  g_owner3[yinterfaces] = fluids;
+  g_owner3[zinterfaces] = fluids;
+  cout << "set uniform information done..." << endl;
+}
+
+void CopyFluid_new_to_old(Fluid3d* fluids, Block3d block) { check3(gks3d_copy_old(bind3(fluids, block)), "copy_old"); }
+void Convar_to_primvar(Fluid3d* fluids, Block3d& block) { check3(gks3d_primvar(bind3(fluids, block)), "primvar"); }
+// Convar_to_primvar_3D / Primvar_to_convar_3D (basic_function.cpp:819-826,
+// 859-866)
+void Convar_to_primvar_3D(double* primvar, double convar[5]) {
+  primvar[0] = convar[0];
+  primvar[1] = convar[1] / convar[0];
+  primvar[2] = convar[2] / convar[0];
+  primvar[3] = convar[3] / convar[0];
+  primvar[4] = (Gamma - 1.0) * (convar[4] - 0.5 * convar[1] * convar[1] / convar[0] -
+                                0.5 * convar[2] * convar[2] / convar[0] - 0.5 * convar[3] * convar[3] / convar[0]);
+}
+void Primvar_to_convar_3D(double* convar, double* primvar) {
+  convar[0] = primvar[0];
+  convar[1] = primvar[0] * primvar[1];
+  convar[2] = primvar[0] * primvar[2];
+  convar[3] = primvar[0] * primvar[3];
+  convar[4] = primvar[0] * (primvar[4] / primvar[0] / (Gamma - 1) +
+                            0.5 * (primvar[1] * primvar[1] + primvar[2] * primvar[2] + primvar[3] * primvar[3]));
+}
+
+void free_boundary_xleft(Fluid3d* f, Block3d b, Fluid3d) { boundary3(f, b, GKS_SIDE3_XLEFT, GKS_BC_FREE); }
+void free_boundary_xright(Fluid3d* f, Block3d b, Fluid3d) { boundary3(f, b, GKS_SIDE3_XRIGHT, GKS_BC_FREE); }
+void free_boundary_yleft(Fluid3d* f, Block3d b, Fluid3d) { boundary3(f, b, GKS_SIDE3_YLEFT, GKS_BC_FREE); }
+void free_boundary_yright(Fluid3d* f, Block3d b, Fluid3d) { boundary3(f, b, GKS_SIDE3_YRIGHT, GKS_BC_FREE); }
+void free_boundary_zleft(Fluid3d* f, Block3d b, Fluid3d) { boundary3(f, b, GKS_SIDE3_ZLEFT, GKS_BC_FREE); }
+void free_boundary_zright(Fluid3d* f, Block3d b, Fluid3d) { boundary3(f, b, GKS_SIDE3_ZRIGHT, GKS_BC_FREE); }
+void periodic_boundary_xleft(Fluid3d* f, Block3d b, Fluid3d) { boundary3(f, b, GKS_SIDE3_XLEFT, GKS_BC_PERIODIC); }
+void periodic_boundary_xright(Fluid3d* f, Block3d b, Fluid3d) { boundary3(f, b, GKS_SIDE3_XRIGHT, GKS_BC_PERIODIC); }
+void periodic_boundary_yleft(Fluid3d* f, Block3d b, Fluid3d) { boundary3(f, b, GKS_SIDE3_YLEFT, GKS_BC_PERIODIC); }
+void periodic_boundary_yright(Fluid3d* f, Block3d b, Fluid3d) { boundary3(f, b, GKS_SIDE3_YRIGHT, GKS_BC_PERIODIC); }
+void periodic_boundary_zleft(Fluid3d* f, Block3d b, Fluid3d) { boundary3(f, b, GKS_SIDE3_ZLEFT, GKS_BC_PERIODIC); }
+void periodic_boundary_zright(Fluid3d* f, Block3d b, Fluid3d) { boundary3(f, b, GKS_SIDE3_ZRIGHT, GKS_BC_PERIODIC); }
+
+void WENO5_AO_splitting_3d(Interface3d&, Interface3d&, Interface3d&, Interface3d&, Interface3d&, Interface3d&,
+                           Fluid3d*, Block3d) {
+  leaf_called("WENO5_AO_splitting_3d");
+}
+void WENO5_AO_of_line_value(Interface3d*, Interface3d*, Interface3d*, Block3d) { leaf_called("WENO5_AO_of_line_value"); }
+void WENO5_AO_of_point_value(Interface3d*, Interface3d*, Interface3d*, Block3d) {
+  leaf_called("WENO5_AO_of_point_value");
+}
+void Do_nothing_splitting_3d(Interface3d*, Interface3d*, Interface3d*, Fluid3d*, Block3d) {}
+void Do_nothing_reconstruction_of_line_value(Interface3d*, Interface3d*, Interface3d*, Block3d) {}
+void Do_nothing_reconstruction_of_point_value(Interface3d*, Interface3d*, Interface3d*, Block3d) {}
+
+// the face, line and point reconstructions run inside the device flux
+// sweep, one direction at a time (their buffers are reused by the next
+// direction); this checks the scheme pointers
+void Reconstruction_within_cell(Interface3d*, Interface3d*, Interface3d*, Fluid3d*, Block3d) {
+  if (cellreconstruction_3D_of_face_value != WENO5_AO_splitting_3d ||
+      cellreconstruction_3D_of_line_value != WENO5_AO_of_line_value ||
+      cellreconstruction_3D_of_point_value != WENO5_AO_of_point_value)
+    die("Reconstruction_within_cell", "the device path implements WENO5_AO_splitting_3d / _of_line_value / _of_point_value");
+  if (gausspoint != 4) die("Reconstruction_within_cell", "the 3D reconstruction needs gausspoint = 4");
+}
+void Reconstruction_forg0(Interface3d*, Interface3d*, Interface3d*, Fluid3d*, Block3d) {
+  if (g0reconstruction_3D_of_face_value != Do_nothing_splitting_3d ||
+      g0reconstruction_3D_of_line_value != Do_nothing_reconstruction_of_line_value ||
+      g0reconstruction_3D_of_point_value != Do_nothing_reconstruction_of_point_value || g0type != all_collisionn)
+    die("Reconstruction_forg0", "the device path computes the collision state inside GKS3D (all_collisionn)");
+}
+
+void Calculate_flux(Flux3d_gauss** xfluxes, Flux3d_gauss**, Flux3d_gauss**, Interface3d* xinterfaces, Interface3d*,
+                    Interface3d*, Block3d block, int stage) {
+  gks3d_ctx* ctx = xinterfaces ? bind3_via(xinterfaces) : bind3_via(xfluxes);
+  check3(gks3d_flux(ctx, stage, block.dt), "flux");
+}
+void GKS3D(Flux3d& flux, Recon3d& re, double area, double dt) {
+  (void)area;
+  Block3d b;
+  b.nodex = b.nodey = b.nodez = 8;
+  b.right = b.up = b.front = 1.0;
+  b.CFL = 0.5;
+  gks3d_params p = snapshot3(b);
+  double L[20], R[20], C[20];
+  const Point3d* sides[2] = {&re.left, &re.right};
+  double* dst[2] = {L, R};
+  for (int s = 0; s < 2; s++) {
+    std::memcpy(dst[s], sides[s]->convar, 40);
+    std::memcpy(dst[s] + 5, sides[s]->der1x, 40);
+    std::memcpy(dst[s] + 10, sides[s]->der1y, 40);
+    std::memcpy(dst[s] + 15, sides[s]->der1z, 40);
+  }
+  check3(gks3d_flux_points(&p, 1, L, R, dt, C, flux.f, flux.derf), "GKS3D");
+  std::memcpy(re.center.convar, C, 40);
+  std::memcpy(re.center.der1x, C + 5, 40);
+  std::memcpy(re.center.der1y, C + 10, 40);
+  std::memcpy(re.center.der1z, C + 15, 40);
+}
+
+// S2O4 coefficients (time_advance.cpp:732-745)
+void S2O4_3D(Block3d& block) {
+  block.stages = 2;
+  block.timecoefficient[0][0][0] = 0.5;
+  block.timecoefficient[0][0][1] = 1.0 / 8.0;
+  block.timecoefficient[1][0][0] = 1.0;
+  block.timecoefficient[1][1][0] = 0.0;
+  block.timecoefficient[1][0][1] = 1.0 / 6.0;
+  block.timecoefficient[1][1][1] = 1.0 / 3.0;
+  block.timecoefficient_hweno[0][0][0] = 0.5;
+  block.timecoefficient_hweno[1][0][0] = 0.0;
+  block.timecoefficient_hweno[1][1][0] = 1.0;
+}
+void Initial_stages(Block3d& block) {
+  std::memset(block.timecoefficient, 0, sizeof(block.timecoefficient));
+  std::memset(block.timecoefficient_hweno, 0, sizeof(block.timecoefficient_hweno));
+  if (!timecoe_list_3d) die("Initial_stages", "no time scheme assigned to timecoe_list_3d");
+  timecoe_list_3d(block);
+}
+// Get_CFL (time_advance.cpp:793-831): dt from the last primitive sweep (or
+// the initial condition's primvar), clipped to tstop; prints like the
+// reference and exits on a reported blow-up
+double Get_CFL(Block3d& block, Fluid3d* fluids, double tstop) {
+  gks3d_ctx* ctx = bind3(fluids, block);
+  double dt = 0.0;
+  const int rc = gks3d_cfl(ctx, block.t, tstop, &dt);
+  if (rc == GKS_ERR_BLOWUP) {
+    cout << "the program blows up at t=" << block.t << "!" << endl;
+    exit(0);
+  }
+  check3(rc, "Get_CFL");
+  cout << "step = " << block.step << "time size is " << dt << " time = " << block.t << endl;
+  return dt;
+}
+void Update_gauss(Fluid3d* fluids, Flux3d_gauss**, Flux3d_gauss**, Flux3d_gauss**, Block3d block, int stage) {
+  if (stage > block.stages) die("Update_gauss", "wrong middle stage ,pls check the time marching setting");
+  check3(gks3d_update(bind3(fluids, block), stage), "update");
+}
+
+void Download_fluid(Fluid3d* fluids, Block3d block) {
+  auto it = g_bound3.find(fluids);
+  if (it == g_bound3.end()) return;
+  check3(gks3d_get_convar(it->second, &fluids[0].convar[0], sizeof(Fluid3d), sizeof(double)), "download convar");
+  (void)block;
+}
+void Upload_fluid(Fluid3d* fluids, Block3d block) { upload3(bind3(fluids, block), fluids); }
+void Release_fluid(Fluid3d* fluids) {
+  auto it = g_bound3.find(fluids);
+  if (it == g_bound3.end()) return;
+  gks3d_destroy(it->second);
+  g_bound3.erase(it);
+  for (auto o = g_owner3.begin(); o != g_owner3.end();) {
+    if (o->second == fluids) o = g_owner3.erase(o);
+    else ++o;
+  }
+}
+
+void output3d(Fluid3d* fluids, Block3d block) {
+  Download_fluid(fluids, block);
+  std::ofstream ofs("../data/R3d.txt", std::ios::trunc);
+  for (int k = block.ghost; k < block.ghost + block.nodez; k++)
+    for (int j = block.ghost; j < block.ghost + block.nodey; j++)
+      for (int i = block.ghost; i < block.ghost + block.nodex; i++)
+        ofs << fluids[((size_t)i * block.ny + j) * block.nz + k].convar[0] << endl;
+}
+
+// ICfor_cubic_sod (3Dproblem.cpp:174-223): zone1 in the cells whose centres
+// lie within 0.2 of the domain centre in every direction, zone2 elsewhere;
+// interior cells only
+void ICfor_cubic_sod(Fluid3d* fluids, double* zone1, double* zone2, Block3d block) {
+  for (int i = block.ghost; i < block.nx - block.ghost; i++)
+    for (int j = block.ghost; j < block.ny - block.ghost; j++)
+      for (int k = block.ghost; k < block.nz - block.ghost; k++) {
+        Fluid3d& f = fluids[((size_t)i * block.ny + j) * block.nz + k];
+        const double xr = std::abs(f.loc[0] - 0.5 * (block.right + block.left));
+        const double yr = std::abs(f.loc[1] - 0.5 * (block.up + block.down));
+        const double zr = std::abs(f.loc[2] - 0.5 * (block.front + block.back));
+        const bool inner = xr < 0.2 * (block.right - block.left) && yr < 0.2 * (block.up - block.down) &&
+                           zr < 0.2 * (block.front - block.back);
+        Copy_Array(f.primvar, inner ? zone1 : zone2, 5);
+        Primvar_to_convar_3D(f.convar, f.primvar);
+      }
+}

@@ -67,6 +67,28 @@ int gks3d_set_cfl_primvar(gks3d_ctx* ctx, const double* base, int64_t cell_strid
  * counted). */
 int gks3d_run(gks3d_ctx* ctx, const int32_t bc[6], double tstop, int32_t max_steps, double* t, int32_t* step);
 int gks3d_sync(gks3d_ctx* ctx);
+
+/* One sweep at a time, for a driver that keeps the reference's loop
+ * (3Dproblem.cpp:109-166) on the host:
+ *  gks3d_boundary   one side's ghost fill (free_boundary_xleft ...,
+ *                   boundary_condition.cpp:445-570; periodic added), side
+ *                   GKS_SIDE3_*, kind GKS_BC_FREE or GKS_BC_PERIODIC;
+ *  gks3d_primvar    Convar_to_primvar over the interior: the next CFL's
+ *                   speed and density, the blow-up flag;
+ *  gks3d_copy_old   CopyFluid_new_to_old;
+ *  gks3d_cfl        Get_CFL / Dtx_3d (time_advance.cpp:793-848) from the last
+ *                   gks3d_primvar (or gks3d_set_cfl_primvar), clipped to
+ *                   tstop; GKS_ERR_BLOWUP (flag cleared) if the last sweeps
+ *                   saw rho or p negative or NaN;
+ *  gks3d_flux       Reconstruction_within_cell + Calculate_flux of stage s
+ *                   (face, line and point values, GKS3D, the S2O4 stage sum);
+ *  gks3d_update     Update_gauss of stage s. */
+int gks3d_boundary(gks3d_ctx* ctx, int32_t side, int32_t kind);
+int gks3d_primvar(gks3d_ctx* ctx);
+int gks3d_copy_old(gks3d_ctx* ctx);
+int gks3d_cfl(gks3d_ctx* ctx, double t, double tstop, double* dt);
+int gks3d_flux(gks3d_ctx* ctx, int32_t stage, double dt);
+int gks3d_update(gks3d_ctx* ctx, int32_t stage);
 /* Device time per kernel phase (all three directions summed), measured with
  * HIP events on the context's stream (no counterpart in the reference, whose
  * only timer is omp_get_wtime around the loop, 3Dproblem.cpp:112-169).

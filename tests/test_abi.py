@@ -208,5 +208,11 @@ def test_3d_null_arguments_are_argument_errors():
     assert L.gks3d_get_convar(None, None, 8, 8) == 1
     assert L.gks3d_set_cfl_primvar(None, None, 8, 8) == 1
     assert L.gks3d_sync(None) == 1
+    assert L.gks3d_boundary(None, 0, 0) == 1
+    assert L.gks3d_primvar(None) == 1
+    assert L.gks3d_copy_old(None) == 1
+    assert L.gks3d_cfl(None, 0.0, 1.0, None) == 1
+    assert L.gks3d_flux(None, 0, 0.1) == 1
+    assert L.gks3d_update(None, 0) == 1
     assert L.gks3d_destroy(None) == 0
     assert L.gks3d_flux_points(None, 0, None, None, 0.0, None, None, None) == 1

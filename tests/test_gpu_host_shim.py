@@ -37,3 +37,32 @@ def test_reference_style_driver_matches_oracle(tmp_path, n, seed, steps):
     l1 = np.abs(u - st["convar"][:, G:-G, G:-G]).sum(axis=(1, 2)) / (n * n)
     print("  steps", k, "L1", l1, flush=True)
     assert np.all(l1 <= 1e-9)
+
+
+EXE3 = os.path.join(ROOT, "high-order-solver_amd", "bin", "cubic_tube")
+
+
+@pytest.mark.parametrize("n,steps,periodic", [(12, 5, False), (10, 4, True)])
+def test_reference_style_3d_driver_matches_oracle(tmp_path, n, steps, periodic):
+    """cubic_tube.cpp: CubicTube's loop (3Dproblem.cpp:3-172) compiled against
+    the shim, the cubic Sod problem, against the 3D oracle."""
+    from gks_amd.params import BC_PERIODIC, cubic_params
+    from gks_amd.problems import cubic_sod_prim, prim_to_convar_3d
+    from oracle_lib import oracle3
+    out = tmp_path / "u3.bin"
+    print("3d driver", n, steps, periodic, flush=True)
+    args = [EXE3, str(n), "1.0", str(steps), str(out)] + (["periodic"] if periodic else [])
+    r = subprocess.run(args, capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stdout + r.stderr
+    raw = np.fromfile(out, dtype=np.float64)
+    t, k = raw[0], int(raw[1])
+    u = raw[2:].reshape(5, n, n, n)
+    p = cubic_params(n)
+    prim = cubic_sod_prim(p)
+    bc = [BC_PERIODIC if periodic else BC_FREE] * 6
+    ou, ot, on, rc, _ = oracle3().run(p, prim_to_convar_3d(prim, 1.4), prim, bc, 1.0, steps)
+    assert rc == 0 and k == on == steps and abs(t - ot) <= 1e-13 * ot
+    b = ou[:, 3:-3, 3:-3, 3:-3]
+    err = np.max(np.abs(u - b)) / np.max(np.abs(b))
+    print("  t", t, "max rel diff", err, flush=True)
+    assert err <= 1e-11
