@@ -208,6 +208,51 @@ int gks1d_run(gks1d_ctx* c, const int32_t bc[2], double tstop, int32_t max_steps
   return GKS_OK;
 }
 
+int gks1d_boundary(gks1d_ctx* c, int32_t side, int32_t kind) {
+  if (!c) return fail(GKS_ERR_ARG, "null");
+  if (side < 0 || side > 1) return fail(GKS_ERR_ARG, "side");
+  if (kind < 0 || kind > 2) return fail(GKS_ERR_ARG, "bc kind");
+  if (kind != GKS_BC_FREE && c->d.N < c->d.G) return fail(GKS_ERR_ARG, "periodic / reflective boundaries need nodex >= ghost");
+  HIPCHK(c->lt->phase(c->d, PH1_BC, side, kind, 0, 0.0, 0.0, 0.0, c->s));
+  return GKS_OK;
+}
+int gks1d_copy_old(gks1d_ctx* c) {
+  if (!c) return fail(GKS_ERR_ARG, "null");
+  HIPCHK(c->lt->phase(c->d, PH1_COPY_OLD, 0, 0, 0, 0.0, 0.0, 0.0, c->s));
+  return GKS_OK;
+}
+int gks1d_cfl(gks1d_ctx* c, double t, double tstop, double* dt) {
+  if (!c || !dt) return fail(GKS_ERR_ARG, "null");
+  double st[4];
+  HIPCHK(c->lt->phase(c->d, PH1_CFL, 0, 0, 0, 0.0, t, tstop, c->s));
+  HIPCHK(hipMemcpyAsync(st, c->d.st, sizeof(st), hipMemcpyDeviceToHost, c->s));
+  HIPCHK(hipStreamSynchronize(c->s));
+  *dt = st[3];
+  return GKS_OK;
+}
+int gks1d_reconstruct(gks1d_ctx* c, int32_t with_df) {
+  if (!c) return fail(GKS_ERR_ARG, "null");
+  HIPCHK(c->lt->phase(c->d, PH1_RECON, with_df ? 1 : 0, 0, 0, 0.0, 0.0, 0.0, c->s));
+  return GKS_OK;
+}
+int gks1d_flux(gks1d_ctx* c, int32_t stage, double dt) {
+  if (!c) return fail(GKS_ERR_ARG, "null");
+  if (stage < 0 || stage > 1) return fail(GKS_ERR_ARG, "stage");
+  HIPCHK(c->lt->phase(c->d, PH1_FLUX, 0, 0, stage, dt, 0.0, 0.0, c->s));
+  return GKS_OK;
+}
+int gks1d_update(gks1d_ctx* c, int32_t stage) {
+  if (!c) return fail(GKS_ERR_ARG, "null");
+  if (stage < 0 || stage > 1) return fail(GKS_ERR_ARG, "stage");
+  HIPCHK(c->lt->phase(c->d, PH1_UPDATE, 0, 0, stage, 0.0, 0.0, 0.0, c->s));
+  return GKS_OK;
+}
+int gks1d_update_alpha(gks1d_ctx* c) {
+  if (!c) return fail(GKS_ERR_ARG, "null");
+  HIPCHK(c->lt->phase(c->d, PH1_ALPHA, 0, 0, 0, 0.0, 0.0, 0.0, c->s));
+  return GKS_OK;
+}
+
 int gks1d_flux_points(const gks1d_params* p, int64_t n, const double* left, const double* right, double dt,
                       double* center, double* f, double* derf) {
   if (!p || !left || !right || !f || !derf) return fail(GKS_ERR_ARG, "null");

@@ -16,8 +16,11 @@ struct Dev1 {
   double* alpha;  // [nx], ghost cells stay 0
   double* face;   // [2 side][6][nx+1]: side 0 = interfaces[i].left, 1 = .right
   double* fl;     // [2 stage][6][nx+1]: f[3], derf[3]
-  double* st;     // [4]: t, step, steps run by the last launch
+  double* st;     // [4]: t, step, steps run by the last launch, dt of the last CFL phase
 };
+
+// the phases k_phase1d runs one at a time
+enum { PH1_BC = 0, PH1_COPY_OLD, PH1_CFL, PH1_RECON, PH1_FLUX, PH1_UPDATE, PH1_ALPHA };
 
 // steps one launch of the persistent kernel may take before it returns to
 // the host (bounds a launch to well under a second at any size)
@@ -30,6 +33,9 @@ struct Launch1dTable {
                     double* dt_out, hipStream_t);
   hipError_t (*flux_points)(const gksd::Gas&, long long n, const double* L, const double* R, double dt,
                             double* C, double* F, double* D, hipStream_t);
+  // one phase (PH1_*): a, b = side and kind (BC) or with_df (RECON)
+  hipError_t (*phase)(const Dev1&, int phase, int a, int b, int stage, double dt, double t, double tstop,
+                      hipStream_t);
 };
 
 namespace ieee { extern const Launch1dTable table1d; }

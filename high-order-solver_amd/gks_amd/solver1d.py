@@ -29,6 +29,13 @@ SIGS1 = {
     "gks1d_run": (ctypes.c_int, [_P, _I32P, ctypes.c_double, ctypes.c_int32, _D, _I32P, _D]),
     "gks1d_flux_points": (ctypes.c_int, [ctypes.POINTER(Params1D), _I64, _D, _D, ctypes.c_double, _D, _D, _D]),
     "gks1d_last_error": (ctypes.c_char_p, []),
+    "gks1d_boundary": (ctypes.c_int, [_P, ctypes.c_int32, ctypes.c_int32]),
+    "gks1d_copy_old": (ctypes.c_int, [_P]),
+    "gks1d_cfl": (ctypes.c_int, [_P, ctypes.c_double, ctypes.c_double, _D]),
+    "gks1d_reconstruct": (ctypes.c_int, [_P, ctypes.c_int32]),
+    "gks1d_flux": (ctypes.c_int, [_P, ctypes.c_int32, ctypes.c_double]),
+    "gks1d_update": (ctypes.c_int, [_P, ctypes.c_int32]),
+    "gks1d_update_alpha": (ctypes.c_int, [_P]),
 }
 
 _lib1 = None
@@ -145,3 +152,27 @@ class Solver1D:
         self.t, self.step_count = t.value, s.value
         _check(rc)
         return (n, hist[:n]) if record_dt else n
+
+    # one sweep at a time (the reference's loop on the host)
+    def boundary(self, side, kind):
+        _check(lib1().gks1d_boundary(self.h, side, kind))
+
+    def copy_old(self):
+        _check(lib1().gks1d_copy_old(self.h))
+
+    def cfl(self, tstop):
+        dt = ctypes.c_double()
+        _check(lib1().gks1d_cfl(self.h, self.t, tstop, ctypes.byref(dt)))
+        return dt.value
+
+    def reconstruct(self, with_df):
+        _check(lib1().gks1d_reconstruct(self.h, 1 if with_df else 0))
+
+    def flux(self, stage, dt):
+        _check(lib1().gks1d_flux(self.h, stage, dt))
+
+    def update(self, stage):
+        _check(lib1().gks1d_update(self.h, stage))
+
+    def update_alpha(self):
+        _check(lib1().gks1d_update_alpha(self.h))

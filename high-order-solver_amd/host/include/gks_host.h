@@ -1,8 +1,8 @@
-// gks_host.h — the reference solver's 2D and 3D call surface (types,
+// gks_host.h — the reference solver's 1D, 2D and 3D call surface (types,
 // globals, function-pointer plugin points and sweep functions) over the
-// gfx950 C ABIs (include/gks2d.h, include/gks3d.h).  A 2D or 3D problem
-// driver written against the reference (src/2Dproblem.cpp /
-// src/3Dproblem.cpp style: set globals, assign the scheme pointers, call the
+// gfx950 C ABIs (include/gks1d.h, gks2d.h, gks3d.h).  A problem driver
+// written against the reference (src/1Dproblem.cpp, 2Dproblem.cpp,
+// 3Dproblem.cpp style: set globals, assign the scheme pointers, call the
 // sweeps in order) compiles against these headers unchanged and runs its
 // per-step hot path on the GPU.
 //
@@ -412,3 +412,115 @@ void ICfor_cubic_sod(Fluid3d* fluids, double* zone1, double* zone2, Block3d bloc
 void Download_fluid(Fluid3d* fluids, Block3d block);
 void Upload_fluid(Fluid3d* fluids, Block3d block);
 void Release_fluid(Fluid3d* fluids);
+
+// ======================================================================= 1D
+// (fluid_mesh.h:8-75, basic_function.h:587-689, boundary_condition.h:5-11,
+// reconstruction.h:14-44, flux_function.h:6-13, time_advance.h:5-20,
+// output.h:27, 1Dproblem.h:10)
+class Block1d {
+ public:
+  int ghost = 5;
+  int nodex = 0;
+  int nx = 0;
+  int nodex_begin = 0;
+  int nodex_end = 0;
+  double dx = 0;
+  double left = 0;
+  double right = 0;
+  int stages = 0;
+  double timecoefficient[5][5][2] = {};
+  double t = 0;
+  double CFL = 0;
+  double dt = 0;
+  int step = 0;
+};
+
+class Fluid1d {
+ public:
+  double primvar[3];
+  double convar[3];
+  double convar_old[3];
+  double convar_stage1[3];
+  double convar_stage2[3];
+  double cx;
+  double dx;
+  double alpha = 0.0;
+};
+
+class Point1d {
+ public:
+  double convar[3];
+  double convar_old[3];
+  double der1[3];
+  double x;
+};
+
+class Flux1d {
+ public:
+  double F[3];
+  double f[3];
+  double derf[4];
+};
+
+// the face states of the device sweeps stay on the device; the host records
+// keep geometry, and GKS() fills left / center / right of one record
+class Interface1d {
+ public:
+  Point1d left;
+  Point1d center;
+  Point1d right;
+  Flux1d* flux = nullptr;
+  double x = 0;
+};
+
+Flux1d** Setflux_array(Block1d block);
+void SetUniformMesh(Block1d block, Fluid1d* fluids, Interface1d* interfaces, Flux1d** fluxes);
+void CopyFluid_new_to_old(Fluid1d* fluids, Block1d block);
+void Convar_to_primvar_1D(double* primvar, double convar[3]);
+void Primvar_to_convar_1D(double* convar, double primvar[3]);
+
+typedef void (*BoundaryCondition)(Fluid1d* fluids, Block1d block, Fluid1d bcvalue);
+void free_boundary_left(Fluid1d* fluids, Block1d block, Fluid1d bcvalue);
+void free_boundary_right(Fluid1d* fluids, Block1d block, Fluid1d bcvalue);
+void reflection_boundary_left(Fluid1d* fluids, Block1d block, Fluid1d bcvalue);
+void reflection_boundary_right(Fluid1d* fluids, Block1d block, Fluid1d bcvalue);
+void periodic_boundary_left(Fluid1d* fluids, Block1d block, Fluid1d bcvalue);
+void periodic_boundary_right(Fluid1d* fluids, Block1d block, Fluid1d bcvalue);
+
+typedef void (*Reconstruction_within_Cell)(Point1d& left, Point1d& right, Fluid1d* fluids, Block1d block);
+extern Reconstruction_within_Cell cellreconstruction;
+typedef void (*Reconstruction_forG0)(Interface1d& interfaces, Fluid1d* fluids);
+extern Reconstruction_forG0 g0reconstruction;
+// plugin leaves (their work is fused into the device sweeps)
+void WENO5_AO(Point1d& left, Point1d& right, Fluid1d* fluids, Block1d block);
+void WENO5_AO_with_DF(Point1d& left, Point1d& right, Fluid1d* fluids, Block1d block);
+void Center_collision(Interface1d& interfaces, Fluid1d* fluids);
+void Reconstruction_within_cell(Interface1d* interfaces, Fluid1d* fluids, Block1d block);
+void Reconstruction_forg0(Interface1d* interfaces, Fluid1d* fluids, Block1d block);
+void Update_alpha(Interface1d* interfaces, Fluid1d* fluids, Block1d block);
+
+enum GKS1d_type { nothing, kfvs1st, kfvs2nd, gks1st, gks2nd, pp_gks };
+extern GKS1d_type gks1dsolver;
+typedef void (*Flux_function)(Flux1d& flux, Interface1d& interface, double dt);
+extern Flux_function flux_function;
+// one interface on the GPU: Center_collision + GKS (flux_function.cpp:16-276)
+void GKS(Flux1d& flux, Interface1d& interface, double dt);
+void Calculate_flux(Flux1d** fluxes, Interface1d* interfaces, Block1d& block, int stage);
+
+typedef void (*TimeMarchingCoefficient)(Block1d& block);
+extern TimeMarchingCoefficient timecoe_list;
+void S2O4(Block1d& block);
+void Initial_stages(Block1d& block);
+double Get_CFL(Block1d& block, Fluid1d* fluids, double tstop);
+void Update(Fluid1d* fluids, Flux1d** fluxes, Block1d block, int stage);
+
+// writes convar[0] of the interior to ../data/R.txt (output.cpp:3-15), after
+// copying the device state back
+void output1d(Fluid1d* fluids, Block1d block);
+// ICforSodTube (1Dproblem.cpp:150-170)
+void ICforSodTube(Fluid1d* fluids, Block1d block);
+
+// device residency (shim only)
+void Download_fluid(Fluid1d* fluids, Block1d block);
+void Upload_fluid(Fluid1d* fluids, Block1d block);
+void Release_fluid(Fluid1d* fluids);

@@ -809,3 +809,224 @@ void ICfor_cubic_sod(Fluid3d* fluids, double* zone1, double* zone2, Block3d bloc
         Primvar_to_convar_3D(f.convar, f.primvar);
       }
 }
+
+// ======================================================================= 1D
+// The 1D call surface over include/gks1d.h.  Defaults as the reference
+// initialises them (reconstruction.cpp:12-14, flux_function.cpp:11-14,
+// time_advance.cpp:5).
+#include "../../../include/gks1d.h"
+
+Reconstruction_within_Cell cellreconstruction = WENO5_AO;
+Reconstruction_forG0 g0reconstruction = Center_collision;
+GKS1d_type gks1dsolver = nothing;
+Flux_function flux_function = GKS;
+TimeMarchingCoefficient timecoe_list = nullptr;
+
+namespace {
+
+std::map<const void*, Fluid1d*> g_owner1;  // interface / flux arrays -> cell array
+std::map<const Fluid1d*, gks1d_ctx*> g_bound1;
+
+void check1(int rc, const char* where) {
+  if (rc != GKS_OK) die(where, gks1d_last_error());
+}
+
+gks1d_params snapshot1(const Block1d& b) {
+  gks1d_params p;
+  std::memset(&p, 0, sizeof(p));
+  p.nodex = b.nodex;
+  p.ghost = b.ghost;
+  p.recon_variable = reconstruction_variable == characteristic ? GKS_VAR_CHARACTERISTIC : GKS_VAR_CONSERVATIVE;
+  p.use_df = is_using_df_factor ? 1 : 0;
+  p.df_from_start = 0;  // the driver's pointer assignment decides, per call
+  p.tau_type = tau_type == NS ? GKS_TAU_NS : GKS_TAU_EULER;
+  p.K = K;
+  const char* fm = getenv("GKS_CONTRACTED");
+  p.fast_math = (fm && fm[0] == '1') ? 1 : 0;
+  p.gamma = Gamma;
+  p.mu = Mu;
+  p.nu = Nu;
+  p.pr = Pr;
+  p.c1_euler = c1_euler;
+  p.c2_euler = c2_euler;
+  p.left = b.left;
+  p.right = b.right;
+  p.cfl = b.CFL;
+  return p;
+}
+
+gks1d_ctx* bind1(Fluid1d* fluids, const Block1d& block) {
+  auto it = g_bound1.find(fluids);
+  if (it != g_bound1.end()) return it->second;
+  if (timecoe_list != S2O4) die("time scheme", "the device path implements S2O4");
+  if (flux_function != GKS || gks1dsolver != gks2nd) die("flux", "the device path implements GKS with gks2nd");
+  gks1d_params p = snapshot1(block);
+  gks1d_ctx* ctx = nullptr;
+  check1(gks1d_create(&p, &ctx), "gks1d_create");
+  g_bound1[fluids] = ctx;
+  check1(gks1d_set_convar(ctx, &fluids[0].convar[0], sizeof(Fluid1d), sizeof(double)), "upload convar");
+  return ctx;
+}
+
+gks1d_ctx* bind1_via(const void* arr) {
+  auto it = g_owner1.find(arr);
+  if (it == g_owner1.end()) die("sweep", "interface array not set up by SetUniformMesh");
+  auto b = g_bound1.find(it->second);
+  if (b == g_bound1.end()) die("sweep", "cell array not bound yet (call a cell sweep first)");
+  return b->second;
+}
+
+void boundary1(Fluid1d* fluids, const Block1d& block, int side, int kind) {
+  check1(gks1d_boundary(bind1(fluids, block), side, kind), "boundary");
+}
+
+}  // namespace
+
+Flux1d** Setflux_array(Block1d block) {
+  Flux1d** var = new Flux1d*[block.nx + 1]();
+  return var;
+}
+// the geometry of SetUniformMesh (1D): cell centres and sizes, interface
+// coordinates; also records which cell array the face arrays belong to
+void SetUniformMesh(Block1d block, Fluid1d* fluids, Interface1d* interfaces, Flux1d** fluxes) {
+  for (int i = 0; i < block.nx; i++) {
+    fluids[i].dx = block.dx;
+    fluids[i].cx = block.left + (i + 0.5 - block.ghost) * block.dx;
+  }
+  for (int i = 0; i <= block.nx; i++) {
+    interfaces[i].x = block.left + (i - block.ghost) * block.dx;
+    interfaces[i].left.x = interfaces[i].x;
+    interfaces[i].center.x = interfaces[i].x;
+    interfaces[i].right.x = interfaces[i].x;
+  }
+  g_owner1[interfaces] = fluids;
+  g_owner1[fluxes] = fluids;
+}
+void CopyFluid_new_to_old(Fluid1d* fluids, Block1d block) { check1(gks1d_copy_old(bind1(fluids, block)), "copy_old"); }
+// Convar_to_primvar_1D / Primvar_to_convar_1D (basic_function.cpp:805-809, 853-858)
+void Convar_to_primvar_1D(double* primvar, double convar[3]) {
+  primvar[0] = convar[0];
+  primvar[1] = convar[1] / convar[0];
+  primvar[2] = (Gamma - 1.0) * (convar[2] - 0.5 * convar[1] * convar[1] / convar[0]);
+}
+void Primvar_to_convar_1D(double* convar, double primvar[3]) {
+  convar[0] = primvar[0];
+  convar[1] = primvar[0] * primvar[1];
+  convar[2] = primvar[0] * (primvar[2] / primvar[0] / (Gamma - 1) + 0.5 * (primvar[1] * primvar[1]));
+}
+
+void free_boundary_left(Fluid1d* f, Block1d b, Fluid1d) { boundary1(f, b, 0, GKS_BC_FREE); }
+void free_boundary_right(Fluid1d* f, Block1d b, Fluid1d) { boundary1(f, b, 1, GKS_BC_FREE); }
+void reflection_boundary_left(Fluid1d* f, Block1d b, Fluid1d) { boundary1(f, b, 0, GKS_BC_REFLECT); }
+void reflection_boundary_right(Fluid1d* f, Block1d b, Fluid1d) { boundary1(f, b, 1, GKS_BC_REFLECT); }
+void periodic_boundary_left(Fluid1d* f, Block1d b, Fluid1d) { boundary1(f, b, 0, GKS_BC_PERIODIC); }
+void periodic_boundary_right(Fluid1d* f, Block1d b, Fluid1d) { boundary1(f, b, 1, GKS_BC_PERIODIC); }
+
+void WENO5_AO(Point1d&, Point1d&, Fluid1d*, Block1d) { leaf_called("WENO5_AO"); }
+void WENO5_AO_with_DF(Point1d&, Point1d&, Fluid1d*, Block1d) { leaf_called("WENO5_AO_with_DF"); }
+void Center_collision(Interface1d&, Fluid1d*) { leaf_called("Center_collision"); }
+
+// Reconstruction_within_cell (reconstruction.cpp:46-53) with the leaf the
+// driver assigned: WENO5_AO, or WENO5_AO_with_DF
+void Reconstruction_within_cell(Interface1d* interfaces, Fluid1d*, Block1d) {
+  gks1d_ctx* ctx = bind1_via(interfaces);
+  if (cellreconstruction != WENO5_AO && cellreconstruction != WENO5_AO_with_DF)
+    die("Reconstruction_within_cell", "the device path implements WENO5_AO and WENO5_AO_with_DF");
+  check1(gks1d_reconstruct(ctx, cellreconstruction == WENO5_AO_with_DF ? 1 : 0), "reconstruct");
+}
+// the collision centre state is formed inside the device flux sweep
+void Reconstruction_forg0(Interface1d*, Fluid1d*, Block1d) {
+  if (g0reconstruction != Center_collision) die("Reconstruction_forg0", "the device path implements Center_collision");
+}
+void Update_alpha(Interface1d* interfaces, Fluid1d*, Block1d) {
+  check1(gks1d_update_alpha(bind1_via(interfaces)), "update_alpha");
+}
+
+void Calculate_flux(Flux1d** fluxes, Interface1d* interfaces, Block1d& block, int stage) {
+  gks1d_ctx* ctx = interfaces ? bind1_via(interfaces) : bind1_via(fluxes);
+  check1(gks1d_flux(ctx, stage, block.dt), "flux");
+}
+void GKS(Flux1d& flux, Interface1d& in, double dt) {
+  Block1d b;
+  b.nodex = 8;
+  b.right = 1.0;
+  b.CFL = 0.5;
+  gks1d_params p = snapshot1(b);
+  double L[6], R[6], C[6];
+  std::memcpy(L, in.left.convar, 24);
+  std::memcpy(L + 3, in.left.der1, 24);
+  std::memcpy(R, in.right.convar, 24);
+  std::memcpy(R + 3, in.right.der1, 24);
+  check1(gks1d_flux_points(&p, 1, L, R, dt, C, flux.f, flux.derf), "GKS");
+  std::memcpy(in.center.convar, C, 24);
+  std::memcpy(in.center.der1, C + 3, 24);
+}
+
+// S2O4 coefficients (time_advance.cpp:20-30)
+void S2O4(Block1d& block) {
+  block.stages = 2;
+  block.timecoefficient[0][0][0] = 0.5;
+  block.timecoefficient[0][0][1] = 1.0 / 8.0;
+  block.timecoefficient[1][0][0] = 1.0;
+  block.timecoefficient[1][1][0] = 0.0;
+  block.timecoefficient[1][0][1] = 1.0 / 6.0;
+  block.timecoefficient[1][1][1] = 1.0 / 3.0;
+}
+void Initial_stages(Block1d& block) {
+  std::memset(block.timecoefficient, 0, sizeof(block.timecoefficient));
+  if (!timecoe_list) die("Initial_stages", "no time scheme assigned to timecoe_list");
+  timecoe_list(block);
+}
+// Get_CFL (time_advance.cpp:75-91): the reference's sequential scan over the
+// current state, on the device
+double Get_CFL(Block1d& block, Fluid1d* fluids, double tstop) {
+  double dt = 0.0;
+  check1(gks1d_cfl(bind1(fluids, block), block.t, tstop, &dt), "Get_CFL");
+  cout << "step = " << block.step << " time size is " << dt << " time = " << block.t << endl;
+  return dt;
+}
+void Update(Fluid1d* fluids, Flux1d**, Block1d block, int stage) {
+  if (stage > block.stages) die("Update", "wrong middle stage,pls check the time marching setting");
+  check1(gks1d_update(bind1(fluids, block), stage), "update");
+}
+
+void Download_fluid(Fluid1d* fluids, Block1d block) {
+  auto it = g_bound1.find(fluids);
+  if (it == g_bound1.end()) return;
+  check1(gks1d_get_convar(it->second, &fluids[0].convar[0], sizeof(Fluid1d), sizeof(double)), "download convar");
+  check1(gks1d_get_alpha(it->second, &fluids[0].alpha, sizeof(Fluid1d)), "download alpha");
+  (void)block;
+}
+void Upload_fluid(Fluid1d* fluids, Block1d block) {
+  check1(gks1d_set_convar(bind1(fluids, block), &fluids[0].convar[0], sizeof(Fluid1d), sizeof(double)),
+         "upload convar");
+}
+void Release_fluid(Fluid1d* fluids) {
+  auto it = g_bound1.find(fluids);
+  if (it == g_bound1.end()) return;
+  gks1d_destroy(it->second);
+  g_bound1.erase(it);
+  for (auto o = g_owner1.begin(); o != g_owner1.end();) {
+    if (o->second == fluids) o = g_owner1.erase(o);
+    else ++o;
+  }
+}
+
+void output1d(Fluid1d* fluids, Block1d block) {
+  Download_fluid(fluids, block);
+  std::ofstream ofs("../data/R.txt");
+  for (int i = block.ghost; i < block.nodex + block.ghost; i++) ofs << fluids[i].convar[0] << endl;
+}
+
+// ICforSodTube (1Dproblem.cpp:150-170): (1, 0, 1) where (i - ghost + 1) dx
+// <= 0.5, (0.125, 0, 0.1) beyond, from the first interior cell to the last
+// ghost cell; convar of every cell
+void ICforSodTube(Fluid1d* fluids, Block1d block) {
+  for (int i = block.ghost; i <= block.nodex + 2 * block.ghost - 1; i++) {
+    const bool left = (double)(i - block.ghost + 1) * block.dx <= 0.5;
+    fluids[i].primvar[0] = left ? 1.0 : 0.125;
+    fluids[i].primvar[1] = 0.0;
+    fluids[i].primvar[2] = left ? 1.0 : 0.1;
+  }
+  for (int i = 0; i < block.nx; i++) Primvar_to_convar_1D(fluids[i].convar, fluids[i].primvar);
+}

@@ -58,6 +58,29 @@ int gks1d_run(gks1d_ctx* ctx, const int32_t bc[2], double tstop, int32_t max_ste
 /* f[3], derf[3] of every face (nx+1 faces, face-major, 6 doubles each) from
  * the last step's stage 0 or 1 (Flux1d fluxes[i][stage], fluid_mesh.h:52-58) */
 int gks1d_get_face_fluxes(gks1d_ctx* ctx, int32_t stage, double* out);
+
+/* One sweep at a time, for a driver that keeps the reference's loop
+ * (1Dproblem.cpp:79-148) on the host; each is one single-workgroup launch:
+ *  gks1d_boundary       one side's ghost fill (side 0 left, 1 right; kind
+ *                       GKS_BC_FREE, _PERIODIC or _REFLECT;
+ *                       boundary_condition.cpp:4-64);
+ *  gks1d_copy_old       CopyFluid_new_to_old;
+ *  gks1d_cfl            Get_CFL (time_advance.cpp:75-91): the reference's
+ *                       sequential Dtx scan over the current state, clipped
+ *                       to tstop (tstop - t + 1e-15);
+ *  gks1d_reconstruct    Reconstruction_within_cell with WENO5_AO (with_df 0)
+ *                       or WENO5_AO_with_DF (1);
+ *  gks1d_flux           Reconstruction_forg0 (Center_collision) +
+ *                       Calculate_flux of stage s with GKS;
+ *  gks1d_update         Update (S2O4) of stage s;
+ *  gks1d_update_alpha   Update_alpha (the DF sensor). */
+int gks1d_boundary(gks1d_ctx* ctx, int32_t side, int32_t kind);
+int gks1d_copy_old(gks1d_ctx* ctx);
+int gks1d_cfl(gks1d_ctx* ctx, double t, double tstop, double* dt);
+int gks1d_reconstruct(gks1d_ctx* ctx, int32_t with_df);
+int gks1d_flux(gks1d_ctx* ctx, int32_t stage, double dt);
+int gks1d_update(gks1d_ctx* ctx, int32_t stage);
+int gks1d_update_alpha(gks1d_ctx* ctx);
 /* message of the last failed call on this thread */
 const char* gks1d_last_error(void);
 /* leaf parity: Center_collision (reconstruction.cpp:1375-1407) + GKS

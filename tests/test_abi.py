@@ -140,6 +140,13 @@ def test_1d_null_arguments_are_argument_errors():
     assert L.gks1d_get_face_fluxes(None, 0, None) == 1
     assert L.gks1d_destroy(None) == 0
     assert L.gks1d_flux_points(None, 0, None, None, 0.0, None, None, None) == 1
+    assert L.gks1d_boundary(None, 0, 0) == 1
+    assert L.gks1d_copy_old(None) == 1
+    assert L.gks1d_cfl(None, 0.0, 1.0, None) == 1
+    assert L.gks1d_reconstruct(None, 0) == 1
+    assert L.gks1d_flux(None, 0, 0.1) == 1
+    assert L.gks1d_update(None, 0) == 1
+    assert L.gks1d_update_alpha(None) == 1
 
 
 HEADER_B = os.path.join(ROOT, "include", "gks2d_batch.h")

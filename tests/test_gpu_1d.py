@@ -173,3 +173,41 @@ def test_1d_sinwave_periodic_conserves():
     from gks_amd.solver1d import prim_to_convar_1d
     u0 = prim_to_convar_1d(ic, p.gamma)[:, G:-G]
     assert np.allclose(u.sum(axis=1), u0.sum(axis=1), rtol=1e-13)
+
+
+@pytest.mark.parametrize("use_df,bc", [(1, (BC_FREE, BC_FREE)), (0, (2, 2)), (1, (BC_PERIODIC, BC_PERIODIC))])
+def test_sweep_by_sweep_loop_equals_the_persistent_run(use_df, bc):
+    """SodTubeProblem's loop on the host, one ABI call per sweep (copy old,
+    Get_CFL, per stage the two side fills, reconstruction with WENO5_AO or,
+    from step 1, WENO5_AO_with_DF, flux, update, DF sensor), gives the
+    persistent kernel's state, alpha and dt sequence bit for bit."""
+    p = sod_params(200, use_df=use_df)
+    steps = 25
+    a = Solver1D(p)
+    a.init_prim(sod_prim(p))
+    _, hist = a.run(list(bc), 0.2, steps, record_dt=True)
+    b = Solver1D(p)
+    b.init_prim(sod_prim(p))
+    dts = []
+    for _ in range(steps):
+        b.copy_old()
+        dt = b.cfl(0.2)
+        with_df = bool(p.df_from_start) or (bool(use_df) and b.step_count > 0)
+        for stage in range(2):
+            b.boundary(0, bc[0])
+            b.boundary(1, bc[1])
+            b.reconstruct(with_df)
+            b.flux(stage, dt)
+            b.update(stage)
+            if use_df:
+                b.update_alpha()
+        b.t += dt
+        b.step_count += 1
+        dts.append(dt)
+    print(f"[1d sweeps] use_df {use_df} bc {bc}: t {a.t} vs {b.t}", flush=True)
+    assert np.array_equal(np.array(dts), hist)
+    assert a.t == b.t
+    assert np.array_equal(a.get_convar(), b.get_convar())
+    assert np.array_equal(a.get_alpha(), b.get_alpha())
+    a.close()
+    b.close()

@@ -66,3 +66,37 @@ def test_reference_style_3d_driver_matches_oracle(tmp_path, n, steps, periodic):
     err = np.max(np.abs(u - b)) / np.max(np.abs(b))
     print("  t", t, "max rel diff", err, flush=True)
     assert err <= 1e-11
+
+
+EXE1 = os.path.join(ROOT, "high-order-solver_amd", "bin", "sod1d")
+
+
+@pytest.mark.parametrize("n,steps,df", [(100, 1000000, False), (200, 60, True)])
+def test_reference_style_1d_driver_matches_oracle(tmp_path, n, steps, df):
+    """sod1d.cpp: SodTubeProblem's loop (1Dproblem.cpp:3-148) compiled against
+    the shim, against the 1D oracle (the DF variant switches to
+    WENO5_AO_with_DF from step 1, as the driver does)."""
+    from gks_amd.params import sod_params
+    from gks_amd.problems import sod_prim
+    from oracle_lib import oracle1d
+    out = tmp_path / "u1.bin"
+    print("1d driver", n, steps, df, flush=True)
+    args = [EXE1, str(n), "0.2", str(steps), str(out)] + (["df"] if df else [])
+    r = subprocess.run(args, capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stdout + r.stderr
+    raw = np.fromfile(out, dtype=np.float64)
+    t, k = raw[0], int(raw[1])
+    u = raw[2:2 + 3 * n].reshape(3, n)
+    alpha = raw[2 + 3 * n:]
+    p = sod_params(n, use_df=1 if df else 0)
+    o = oracle1d().solver(p)
+    o.set_prim(sod_prim(p))
+    rc, _ = o.run([BC_FREE, BC_FREE], 0.2, steps)
+    assert rc == 0
+    st = o.get()
+    assert k == st["step"] and abs(t - st["t"]) <= 1e-14
+    l1 = np.abs(u - st["convar"][:, G:-G]).sum(axis=1) / n
+    print("  steps", k, "L1", l1, flush=True)
+    assert np.all(l1 <= 1e-10)  # device erfc / exp against the CPU's over ~70 steps: ~1e-12
+    if df:
+        assert np.max(np.abs(alpha - st["alpha"][G:-G])) <= 1e-9
