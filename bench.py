@@ -37,7 +37,12 @@ PEAK_FP64_TFLOPS = 78.6  # MI355X FP64 (vector and matrix)
 # rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE (profiles/gks2d_riemann2048_fast_hbm_traffic.md),
 # each scaled by the counter's reading for k_copy_old (4 x 2048^2 doubles read
 # and written): fetch x1.882, write x0.985
-TRAFFIC_2048_FAST = {"flux_x": 1.9280e9, "flux_y": 1.9668e9}
+TRAFFIC_2048_FAST = {"flux_x": 1.131e9, "flux_y": 1.140e9}
+# FP64 operations the fast flux kernel executes per launch at 2048^2, from
+# rocprofv3 --pmc SQ_INSTS_VALU_{ADD,MUL,FMA,TRANS}_F64 (x64 lanes, FMA = 2)
+# (profiles/gks2d_riemann2048_fast_kernel_trace.md); below the algorithmic
+# count: reciprocal multiplication, FMA contraction, whole-space tables
+EXECUTED_FLOPS_2048_FAST = {"flux_x": 44.07e9, "flux_y": 44.07e9}
 PEAK_HBM_GBS = 8000.0
 # 3D units (oracle/flopcount3d, WENO5-AO characteristic, GKS3D gks2nd S2O4):
 # FP64 operations per unit of each kernel phase
@@ -384,6 +389,11 @@ def main():
                 "frac": round(achieved / PEAK_FP64_TFLOPS, 4), "traffic": traffic,
                 "kernel": dom + (" (tangential reconstruction + flux, one launch)" if fused and dom.startswith("flux") else ""),
                 "avg_ms": round(avg_ms, 4), "flops_per_launch": units * fpu}
+        ex = (EXECUTED_FLOPS_2048_FAST.get(dom) if (n == 2048 and args.variant == "fast" and fused and args.path == "plain")
+              else None)
+        if ex:
+            roof["executed_flops_per_launch"] = ex
+            roof["executed_frac"] = round(ex / (avg_ms * 1e-3) / 1e12 / PEAK_FP64_TFLOPS, 4)
     total_dev_ms = sum(v[0] for v in tim.values())
     learned_info = None
     if learned:

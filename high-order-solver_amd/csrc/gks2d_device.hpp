@@ -149,6 +149,9 @@ GKS_DI void from_char(double* c, const CharBase& b, const double* ch, const Gas&
 // MMDF2d::calcualte_MMDF: basic_function.cpp:110-187
 struct MM {
   double up[7], un[7], vw[7], xi2, xi4;
+#if GKS_FAST
+  double uw[7];  // whole-space u moments
+#endif
 };
 GKS_DI void mmdf(MM& m, double u, double v, double lambda, const Gas& g) {
   const double pi = 3.14159265358979323846;
@@ -173,6 +176,10 @@ GKS_DI void mmdf(MM& m, double u, double v, double lambda, const Gas& g) {
   for (int i = 2; i <= 6; i++) m.vw[i] = v * m.vw[i - 1] + 0.5 * (i - 1) * ol * m.vw[i - 2];
   m.xi2 = 0.5 * g.K * ol;
   m.xi4 = 0.25 * (g.K * g.K + 2 * g.K) * ol * ol;
+#if GKS_FAST
+#pragma unroll
+  for (int i = 0; i <= 6; i++) m.uw[i] = uw[i];
+#endif
 }
 // table entries upvxi / unvxi / uvxi [i][j][k]
 struct TabP {
@@ -187,9 +194,18 @@ struct TabN {
     return k == 0 ? m.un[i] * m.vw[j] : (k == 1 ? m.un[i] * m.vw[j] * m.xi2 : m.un[i] * m.vw[j] * m.xi4);
   }
 };
+// uvxi: the reference forms upvxi + unvxi (basic_function.cpp:161), which is
+// uwhole[i] vwhole[j] xi up to rounding; the fast build takes the product of
+// the whole-space moments directly (fewer operations and live registers)
 struct TabW {
   const MM& m;
-  GKS_DI double operator()(int i, int j, int k) const { return TabP{m}(i, j, k) + TabN{m}(i, j, k); }
+  GKS_DI double operator()(int i, int j, int k) const {
+#if GKS_FAST
+    return k == 0 ? m.uw[i] * m.vw[j] : (k == 1 ? m.uw[i] * m.vw[j] * m.xi2 : m.uw[i] * m.vw[j] * m.xi4);
+#else
+    return TabP{m}(i, j, k) + TabN{m}(i, j, k);
+#endif
+  }
 };
 
 // GL_address / GR_address / G_address: basic_function.cpp:603-636
@@ -254,6 +270,31 @@ GKS_DI void time_coe(double* t, double tau, double tn, double dt) {
   t[5] = 0.5 * dt * dt - tau * tn * (eta - 1) - tau * dt;
 }
 
+// whole-space moments alone (no erfc / exp): the collision state's table in
+// the fast build, and the smooth-flow flux
+struct MMW {
+  double uw[7], vw[7], xi2, xi4;
+};
+GKS_DI void mmdf_whole(MMW& m, double u, double v, double lambda, const Gas& g) {
+  const double ol = 1.0 / lambda;
+  m.uw[0] = 1.0;
+  m.uw[1] = u;
+  m.vw[0] = 1.0;
+  m.vw[1] = v;
+#pragma unroll
+  for (int i = 2; i <= 6; i++) {
+    m.uw[i] = u * m.uw[i - 1] + 0.5 * (i - 1) * ol * m.uw[i - 2];
+    m.vw[i] = v * m.vw[i - 1] + 0.5 * (i - 1) * ol * m.vw[i - 2];
+  }
+  m.xi2 = 0.5 * g.K * ol;
+  m.xi4 = 0.25 * (g.K * g.K + 2 * g.K) * ol * ol;
+}
+struct TabWhole {
+  const MMW& m;
+  GKS_DI double operator()(int i, int j, int k) const {
+    return k == 0 ? m.uw[i] * m.vw[j] : (k == 1 ? m.uw[i] * m.vw[j] * m.xi2 : m.uw[i] * m.vw[j] * m.xi4);
+  }
+};
 // a strided read-only view (e.g. one face's column of an LDS tile)
 struct Strided {
   const double* p;
@@ -404,9 +445,17 @@ GKS_DI void gks_point(double* f, double* derf, double* center, const V& L, const
   double g0u[4], s4[4], A0u[4];
   double qs_raw = 0.0;  // Prandtl-fix heat-flux term (flux_function.cpp:630-636)
   {
+    // the collision state's moments enter only through the whole-space
+    // table; the fast build forms it without the half-space integrals
+#if GKS_FAST
+    MMW m0;
+    mmdf_whole(m0, p0[1], p0[2], p0[3], g);
+    const TabWhole w0{m0};
+#else
     MM m0;
     mmdf(m0, p0[1], p0[2], p0[3], g);
     const TabW w0{m0};
+#endif
     const double unit[4] = {1.0, 0.0, 0.0, 0.0};
     contract(1, 0, 0, g0u, unit, w0);
     double a0x[4], a0y[4], a0xu[4], a0xuu[4], a0yv[4], a0yuv[4];
@@ -485,29 +534,6 @@ GKS_DI void gks_point(double* f, double* derf, double* center, const V& L, const
 // f = 4 F(dt/2) - F(dt) = rho0 dt [g0u - tau (s4 + A0u)] and
 // derf = 4 (F(dt) - 2 F(dt/2)) = rho0 dt^2 A0u.  Only whole-space moments:
 // no erfc or exp.
-struct MMW {
-  double uw[7], vw[7], xi2, xi4;
-};
-GKS_DI void mmdf_whole(MMW& m, double u, double v, double lambda, const Gas& g) {
-  const double ol = 1.0 / lambda;
-  m.uw[0] = 1.0;
-  m.uw[1] = u;
-  m.vw[0] = 1.0;
-  m.vw[1] = v;
-#pragma unroll
-  for (int i = 2; i <= 6; i++) {
-    m.uw[i] = u * m.uw[i - 1] + 0.5 * (i - 1) * ol * m.uw[i - 2];
-    m.vw[i] = v * m.vw[i - 1] + 0.5 * (i - 1) * ol * m.vw[i - 2];
-  }
-  m.xi2 = 0.5 * g.K * ol;
-  m.xi4 = 0.25 * (g.K * g.K + 2 * g.K) * ol * ol;
-}
-struct TabWhole {
-  const MMW& m;
-  GKS_DI double operator()(int i, int j, int k) const {
-    return k == 0 ? m.uw[i] * m.vw[j] : (k == 1 ? m.uw[i] * m.vw[j] * m.xi2 : m.uw[i] * m.vw[j] * m.xi4);
-  }
-};
 template <class V>
 GKS_DI void gks_point_smooth(double* f, double* derf, const V& L, const V& Rr, double dt, const Gas& g) {
   double c0[4], c0x[4], c0y[4];
