@@ -76,6 +76,7 @@ struct gks2d_ctx {
   float* d_mlp = nullptr;    // classifier parameters (gks2d_set_classifier)
   unsigned short* d_mlpw = nullptr;  // their weights in the MFMA operand layout
   unsigned char* d_lin = nullptr;    // per-cell linear-weight flags, x then y
+  unsigned* d_tlist = nullptr;       // learned flux: listed x faces, y faces, then their 2 counts
   unsigned long long* d_lstat = nullptr;  // its linear / total counters
   std::vector<void*> allocs;
   int last_slot = -1;        // slot of the last primvar / set_cfl_primvar
@@ -161,6 +162,7 @@ int gks2d_destroy(gks2d_ctx* c) {
   if (c->d_mlp) (void)hipFree(c->d_mlp);
   if (c->d_mlpw) (void)hipFree(c->d_mlpw);
   if (c->d_lin) (void)hipFree(c->d_lin);
+  if (c->d_tlist) (void)hipFree(c->d_tlist);
   if (c->d_lstat) (void)hipFree(c->d_lstat);
   if (c->ev) (void)hipEventDestroy(c->ev);
   if (c->s) (void)hipStreamDestroy(c->s);
@@ -590,6 +592,7 @@ int gks2d_set_classifier(gks2d_ctx* c, const float* params, float logit_threshol
     HIPCHK(hipMalloc(&c->d_mlpw, mlp::kPackHalves * sizeof(unsigned short)));
     HIPCHK(hipMalloc(&c->d_lstat, kLstatWords * sizeof(unsigned long long)));
     HIPCHK(hipMalloc(&c->d_lin, 2 * nc));
+    HIPCHK(hipMalloc(&c->d_tlist, (size_t)(c->d.nfx + c->d.nfy + 2) * sizeof(unsigned)));
   }
   std::vector<uint16_t> packed(mlp::kPackHalves);
   mlp::pack_bf16(params, packed.data());
@@ -602,6 +605,9 @@ int gks2d_set_classifier(gks2d_ctx* c, const float* params, float logit_threshol
   c->d.smooth_flux = 1;
   c->d.lin[0] = c->d_lin;
   c->d.lin[1] = c->d_lin + nc;
+  c->d.tlist[0] = c->d_tlist;
+  c->d.tlist[1] = c->d_tlist + c->d.nfx;
+  c->d.tcount = c->d_tlist + c->d.nfx + c->d.nfy;
   c->d.mlp_thr = logit_threshold;
   c->d.lstat = c->d_lstat;
   return GKS_OK;

@@ -883,15 +883,21 @@ k_flux(Dev d, int stage, double dt) {
 // (a few hundred bytes per lane, cache resident) and the second wave hides
 // the latency: 1.54 -> 1.16 ms per launch at 2048^2.
 constexpr int kTanFlux = 128;
-template <int DIR, bool LEARNED, int TF>
-__device__ __forceinline__ void tanflux_body(const Dev& d, unsigned blk, int stage, double dt) {
+// LISTED (learned path only): the tile's faces are list[blk * TF ...], nlist
+// entries in all, instead of the consecutive faces blk * TF ...; they are
+// the faces k_tanflux_cheap left (not both sides linear), so the smooth-flow
+// flux never applies and their linear-weight counts are already taken
+template <int DIR, bool LEARNED, int TF, bool LISTED = false>
+__device__ __forceinline__ void tanflux_body(const Dev& d, unsigned blk, int stage, double dt,
+                                             const unsigned* list = nullptr, long long nlist = 0) {
   __shared__ double sg[2][2][12][TF];  // [side][gauss point][component][face]
   __shared__ unsigned char slin[2][LEARNED ? TF : 1];  // side took the linear weights
   const int ni = DIR == 0 ? d.NX + 1 : d.NX, nj = DIR == 0 ? d.NY : d.NY + 1;
   const long long nfaces = (long long)ni * nj;
   const int tl = threadIdx.x % TF, hi = threadIdx.x / TF;
-  const long long r = (long long)blk * TF + tl;
-  const bool valid = r < nfaces;
+  const long long slot = (long long)blk * TF + tl;
+  const bool valid = LISTED ? slot < nlist : slot < nfaces;
+  const long long r = LISTED ? (valid ? (long long)list[slot] : 0) : slot;
   const int i = valid ? (int)(r / nj) + d.G : d.G, j = valid ? (int)(r % nj) + d.G : d.G;
   const long long nf = DIR == 0 ? d.nfx : d.nfy;
   const long long f0 = DIR == 0 ? (long long)i * d.ny + j : (long long)i * (d.ny + 1) + j;
@@ -938,7 +944,7 @@ __device__ __forceinline__ void tanflux_body(const Dev& d, unsigned blk, int sta
         for (int s = 0; s < 5; s++) smooth &= d.lin[1][(long long)(i + 2 - s) * d.ny + cj];
       }
       lin = smooth != 0 && amax == 0.0;
-      count_linear(d.lstat + 2 * kStatSlots, lin);
+      if (!LISTED) count_linear(d.lstat + 2 * kStatSlots, lin);
       slin[side][tl] = lin ? 1 : 0;
     }
   }
@@ -978,7 +984,7 @@ __device__ __forceinline__ void tanflux_body(const Dev& d, unsigned blk, int sta
     const int q = hi;
     const Strided L{&sg[0][q][0][tl], TF}, R{&sg[1][q][0][tl], TF};
     // both sides smooth: the smooth-flow flux (learned path)
-    const bool smooth = LEARNED && d.smooth_flux && (slin[0][tl] & slin[1][tl]);
+    const bool smooth = LEARNED && !LISTED && d.smooth_flux && (slin[0][tl] & slin[1][tl]);
     flux_point_tail<DIR>(d, q, f0, nf, L, R, stage, dt, smooth);
   }
 }
@@ -986,6 +992,130 @@ template <int DIR, bool LEARNED>
 __global__ void __launch_bounds__(2 * kTanFlux) __attribute__((amdgpu_waves_per_eu(GKS_FLUX_WAVES)))
 k_tanflux(Dev d, int stage, double dt) {
   tanflux_body<DIR, LEARNED, kTanFlux>(d, xcd_block(), stage, dt);
+}
+
+// ------------------------------------ learned path: the flux in two launches
+// With the smooth-flow flux on, the learned path's flux sweep runs in two
+// launches, so that the many smooth faces do not carry the register budget
+// (two waves per SIMD) of the full flux point.  k_tanflux_cheap takes every
+// face, one thread per (Gauss point, face), no LDS: a face whose two
+// tangential stencils are linear (the tangential stencil rule of
+// tanflux_body) gets its Gauss-point state on each side from the linear
+// weights at that Gauss point and the smooth-flow flux; every other face is
+// appended to tlist[DIR] (one wave-aggregated atomic per wave).
+// k_tanflux_list then runs tanflux_body over the listed faces (WENO5-AO-DF
+// or linear per side, full GKS2D) as a grid-stride loop over tiles of the
+// list.  Each face gets the result k_tanflux<DIR, true> gives it; the list
+// order varies from run to run, the results do not.
+// the linear-weight rule for the tangential stencil of (side, face (i, j))
+template <int DIR> __device__ __forceinline__ bool tangential_linear(const Dev& d, int side, int i, int j) {
+  double amax = 0.0;
+  unsigned smooth;
+  if (DIR == 0) {
+    const long long c0 = (long long)(side == 0 ? i - 1 : i) * d.ny + j;
+#pragma unroll
+    for (int s = 0; s < 5; s++) amax = fmax(amax, d.ay[c0 - 2 + s]);
+    smooth = d.lin[1][c0];
+#pragma unroll
+    for (int s = 0; s < 5; s++) smooth &= d.lin[0][c0 - 2 + s];
+  } else {
+    const int cj = side == 0 ? j - 1 : j;
+#pragma unroll
+    for (int s = 0; s < 5; s++) amax = fmax(amax, d.ax[(long long)(i + 2 - s) * d.ny + cj]);
+    smooth = d.lin[0][(long long)i * d.ny + cj];
+#pragma unroll
+    for (int s = 0; s < 5; s++) smooth &= d.lin[1][(long long)(i + 2 - s) * d.ny + cj];
+  }
+  return smooth != 0 && amax == 0.0;
+}
+// A thread per (Gauss point, face); it reads both sides' stencils.
+template <int DIR>
+__global__ void __launch_bounds__(2 * kTanFlux) k_tanflux_cheap(Dev d, int stage, double dt) {
+  constexpr int TF = kTanFlux;
+  const unsigned blk = xcd_block();
+  const int ni = DIR == 0 ? d.NX + 1 : d.NX, nj = DIR == 0 ? d.NY : d.NY + 1;
+  const long long nfaces = (long long)ni * nj;
+  const int tl = threadIdx.x % TF, q = threadIdx.x / TF;
+  const long long r = (long long)blk * TF + tl;
+  const bool valid = r < nfaces;
+  const int i = valid ? (int)(r / nj) + d.G : d.G, j = valid ? (int)(r % nj) + d.G : d.G;
+  bool lin0 = false, lin1 = false;
+  if (valid) {
+    lin0 = tangential_linear<DIR>(d, 0, i, j);
+    lin1 = tangential_linear<DIR>(d, 1, i, j);
+  }
+  const bool cheap = valid && lin0 && lin1;
+  if (q == 0) {  // the first TF threads: whole waves (TF is a multiple of 64)
+    if (valid) {
+      count_linear(d.lstat + 2 * kStatSlots, lin0);
+      count_linear(d.lstat + 2 * kStatSlots, lin1);
+    }
+    const bool listed = valid && !cheap;
+    const unsigned long long bl = __ballot(listed);
+    if (bl) {
+      const int lead = __ffsll((long long)bl) - 1;
+      unsigned base = 0;
+      if ((int)(threadIdx.x & 63) == lead) base = atomicAdd(d.tcount + DIR, (unsigned)__popcll(bl));
+      base = (unsigned)__shfl((int)base, lead);
+      const unsigned rank =
+          __builtin_amdgcn_mbcnt_hi((unsigned)(bl >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)bl, 0u));
+      if (listed) d.tlist[DIR][base + rank] = (unsigned)r;
+    }
+  }
+  if (!cheap) return;
+  const long long nf = DIR == 0 ? d.nfx : d.nfy;
+  const double* line = DIR == 0 ? d.line_x : d.line_y;
+  const double h = DIR == 0 ? d.dy : d.dx, ih = DIR == 0 ? d.idy : d.idx;
+  double g[2][12];
+#pragma unroll
+  for (int side = 0; side < 2; side++) {
+    double L[5][8];
+#pragma unroll
+    for (int s = 0; s < 5; s++) {
+      const long long f = DIR == 0 ? (long long)i * d.ny + (j - 2 + s) : (long long)(i + 2 - s) * (d.ny + 1) + j;
+#pragma unroll
+      for (int k = 0; k < 8; k++) L[s][k] = line[(side * 8 + k) * nf + f];
+    }
+    double* o = g[side];
+#pragma unroll
+    for (int v = 0; v < 4; v++) {
+      double a0, a1, b0, b1, c0, c1, e0, e1;
+      weno5_linear_2gauss(a0, a1, b0, b1, L[0][v], L[1][v], L[2][v], L[3][v], L[4][v], h, ih);
+      weno5_linear_2gauss(c0, c1, e0, e1, L[0][4 + v], L[1][4 + v], L[2][4 + v], L[3][4 + v], L[4][4 + v], h, ih);
+      o[v] = q == 0 ? a0 : b0;
+      o[8 + v] = q == 0 ? a1 : b1;
+      o[4 + v] = q == 0 ? c0 : e0;
+    }
+    if (reduce_order(o, d.gas)) {
+#pragma unroll
+      for (int v = 0; v < 4; v++) {
+        o[v] = L[2][v];
+        o[4 + v] = L[2][4 + v];
+        o[8 + v] = 0.0;
+      }
+    }
+  }
+  const long long f0 = DIR == 0 ? (long long)i * d.ny + j : (long long)i * (d.ny + 1) + j;
+  flux_point_tail<DIR>(d, q, f0, nf, g[0], g[1], stage, dt, true);
+}
+// one tile of tlist[DIR] per workgroup, launched for as many tiles as there
+// are faces (the count is only known on the device; workgroups past it
+// return at once).  A grid-stride loop over the tiles instead spills: the
+// body's register demand grows past the two-wave budget inside the loop.
+// The count is uniform over the workgroup, so every thread reaches the same
+// barriers.
+
+template <int DIR>
+__global__ void __launch_bounds__(2 * kTanFlux) __attribute__((amdgpu_waves_per_eu(GKS_FLUX_WAVES)))
+k_tanflux_list(Dev d, int stage, double dt) {
+  const long long n = d.tcount[DIR];
+  if ((long long)blockIdx.x * kTanFlux < n)
+    tanflux_body<DIR, true, kTanFlux, true>(d, blockIdx.x, stage, dt, d.tlist[DIR], n);
+}
+// clears a list count before the sweep that fills it (a kernel, not a
+// memset, so that the sequence can be captured in a graph)
+__global__ void __launch_bounds__(64) k_clear_count(unsigned* count) {
+  if (threadIdx.x == 0) *count = 0u;
 }
 
 // ------------------------------------------------------------------ update
@@ -1206,6 +1336,10 @@ template __global__ void k_tanflux<0, false>(Dev, int, double);
 template __global__ void k_tanflux<1, false>(Dev, int, double);
 template __global__ void k_tanflux<0, true>(Dev, int, double);
 template __global__ void k_tanflux<1, true>(Dev, int, double);
+template __global__ void k_tanflux_cheap<0>(Dev, int, double);
+template __global__ void k_tanflux_cheap<1>(Dev, int, double);
+template __global__ void k_tanflux_list<0>(Dev, int, double);
+template __global__ void k_tanflux_list<1>(Dev, int, double);
 template __global__ void kb_normal<0, false>(BatchArgs);
 template __global__ void kb_normal<1, false>(BatchArgs);
 template __global__ void kb_normal<0, true>(BatchArgs);
@@ -1280,7 +1414,17 @@ static hipError_t flux(const Dev& d, int dir, int stage, double dt, hipStream_t 
   if (d.order == 5 && d.fuse_tanflux) {
     const long long nfaces = dir == 0 ? (long long)(d.NX + 1) * d.NY : (long long)d.NX * (d.NY + 1);
     const unsigned nb = (unsigned)((nfaces + kTanFlux - 1) / kTanFlux);
-    if (d.mlp) {
+    if (d.mlp && d.smooth_flux && d.tcount) {
+      // smooth faces, then the listed rest (at most nfaces of them)
+      hipLaunchKernelGGL(k_clear_count, dim3(1), dim3(64), 0, s, d.tcount + dir);
+      if (dir == 0) {
+        hipLaunchKernelGGL(k_tanflux_cheap<0>, dim3(nb), dim3(2 * kTanFlux), 0, s, d, stage, dt);
+        hipLaunchKernelGGL(k_tanflux_list<0>, dim3(nb), dim3(2 * kTanFlux), 0, s, d, stage, dt);
+      } else {
+        hipLaunchKernelGGL(k_tanflux_cheap<1>, dim3(nb), dim3(2 * kTanFlux), 0, s, d, stage, dt);
+        hipLaunchKernelGGL(k_tanflux_list<1>, dim3(nb), dim3(2 * kTanFlux), 0, s, d, stage, dt);
+      }
+    } else if (d.mlp) {
       if (dir == 0) hipLaunchKernelGGL((k_tanflux<0, true>), dim3(nb), dim3(2 * kTanFlux), 0, s, d, stage, dt);
       else hipLaunchKernelGGL((k_tanflux<1, true>), dim3(nb), dim3(2 * kTanFlux), 0, s, d, stage, dt);
     } else {

@@ -37,6 +37,8 @@ struct Dev {
   unsigned long long* lstat;    // 2 x 1024 (linear, total) slot pairs: normal, then tangential
   unsigned char* lin[2];        // per cell: its x / y normal stencil took the linear weights
   int smooth_flux;              // learned path: faces with two linear sides take the smooth-flow flux
+  unsigned* tlist[2];           // learned path: x / y faces left to the full flux (interior face ordinals)
+  unsigned* tcount;             // their counts, x then y (cleared before each flux sweep)
   double* partial;  // per-block (max speed, blow-up flag) pairs
   double* stat;     // [0..3]: max speed of slot 0 / 1 (+pad); [4]: sticky blow-up flag
 };
