@@ -42,7 +42,7 @@ TRAFFIC_2048_FAST = {"flux_x": 1.131e9, "flux_y": 1.140e9}
 # rocprofv3 --pmc SQ_INSTS_VALU_{ADD,MUL,FMA,TRANS}_F64 (x64 lanes, FMA = 2)
 # (profiles/gks2d_riemann2048_fast_kernel_trace.md); below the algorithmic
 # count: reciprocal multiplication, FMA contraction, whole-space tables
-EXECUTED_FLOPS_2048_FAST = {"flux_x": 44.07e9, "flux_y": 44.07e9}
+EXECUTED_FLOPS_2048_FAST = {"flux_x": 41.70e9, "flux_y": 41.70e9}
 PEAK_HBM_GBS = 8000.0
 # 3D units (oracle/flopcount3d, WENO5-AO characteristic, GKS3D gks2nd S2O4):
 # FP64 operations per unit of each kernel phase
@@ -55,6 +55,10 @@ FLOPS3_CELL_STEP = 367644  # one interior cell's S2O4 step (2 stages x 3 directi
 # directions, from rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE scaled by the
 # k3_copy_old calibration (profiles/gks3d_tgv256_fast_hbm_traffic.md)
 TRAFFIC3_256_FAST = {"flux": 28.55e9}
+# FP64 operations k3_pflux executes per launch at 256^3, fast kernels
+# (SQ_INSTS_VALU_{ADD,MUL,FMA,TRANS}_F64 x64, FMA = 2;
+# profiles/gks3d_tgv256_fast_kernel_trace.md)
+EXECUTED_FLOPS3_256_FAST = {"flux": 614.46e9}
 
 
 def kernel_units(phase, n, gp=2, fused=False):
@@ -285,6 +289,10 @@ def bench_tgv3d(args):
                 "traffic": TRAFFIC3_256_FAST.get(dom) if (n == 256 and args.variant == "fast") else None,
                 "kernel": "k3_pflux (x, y, z): point values + GKS3D, one launch" if dom == "flux" else f"k3_{dom} (x, y, z)",
                 "avg_ms": round(avg_ms, 4), "flops_per_launch": u * fpu}
+        ex = EXECUTED_FLOPS3_256_FAST.get(dom) if (n == 256 and args.variant == "fast") else None
+        if ex:
+            roof["executed_flops_per_launch"] = ex
+            roof["executed_frac"] = round(ex / (avg_ms * 1e-3) / 1e12 / PEAK_FP64_TFLOPS, 4)
     out = {
         "metric": "Mcell-updates/s FP64, 3D Taylor-Green WENO5-AO GKS3D S2O4",
         "value": round(value, 3), "unit": "Mcell-updates/s", "n_gpus": 1, "steps": k, "warmup": args.warmup,
