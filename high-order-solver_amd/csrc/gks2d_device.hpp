@@ -209,17 +209,27 @@ GKS_DI void mmdf(MM& m, double u, double v, double lambda, const Gas& g) {
 #endif
 }
 // table entries upvxi / unvxi / uvxi [i][j][k]
+// Each table is separable, entry (i, j, k) = U[i] V[j] X[k] with X = (1,
+// xi2, xi4); U, V, x2, x4 expose the factors (contract_sep).
 struct TabP {
   const MM& m;
   GKS_DI double operator()(int i, int j, int k) const {
     return k == 0 ? m.up[i] * m.vw[j] : (k == 1 ? m.up[i] * m.vw[j] * m.xi2 : m.up[i] * m.vw[j] * m.xi4);
   }
+  GKS_DI double U(int i) const { return m.up[i]; }
+  GKS_DI double V(int j) const { return m.vw[j]; }
+  GKS_DI double x2() const { return m.xi2; }
+  GKS_DI double x4() const { return m.xi4; }
 };
 struct TabN {
   const MM& m;
   GKS_DI double operator()(int i, int j, int k) const {
     return k == 0 ? m.un[i] * m.vw[j] : (k == 1 ? m.un[i] * m.vw[j] * m.xi2 : m.un[i] * m.vw[j] * m.xi4);
   }
+  GKS_DI double U(int i) const { return m.un[i]; }
+  GKS_DI double V(int j) const { return m.vw[j]; }
+  GKS_DI double x2() const { return m.xi2; }
+  GKS_DI double x4() const { return m.xi4; }
 };
 // uvxi: the reference forms upvxi + unvxi (basic_function.cpp:161), which is
 // uwhole[i] vwhole[j] xi up to rounding; the fast build takes the product of
@@ -233,10 +243,44 @@ struct TabW {
     return TabP{m}(i, j, k) + TabN{m}(i, j, k);
 #endif
   }
+#if GKS_FAST
+  GKS_DI double U(int i) const { return m.uw[i]; }
+  GKS_DI double V(int j) const { return m.vw[j]; }
+  GKS_DI double x2() const { return m.xi2; }
+  GKS_DI double x4() const { return m.xi4; }
+#endif
 };
+
+#if GKS_FAST
+// contract() at x = 0 through the factors of a separable table: the 13
+// products U[u+i] V[v+j] it needs are formed once and the xi terms are
+// grouped (S_ij = U[u+i] V[v+j]; T0 = S20 + S02 + xi2 S00 is psi[0]'s a3
+// bracket, and so on); the same sums as the term-by-term form, regrouped
+template <class T> GKS_DI void contract_sep(int u, int v, double* psi, const double* a, const T& m) {
+  const double U0 = m.U(u), U1 = m.U(u + 1), U2 = m.U(u + 2), U3 = m.U(u + 3), U4 = m.U(u + 4);
+  const double V0 = m.V(v), V1 = m.V(v + 1), V2 = m.V(v + 2), V3 = m.V(v + 3), V4 = m.V(v + 4);
+  const double x2 = m.x2(), x4 = m.x4();
+  const double S00 = U0 * V0, S10 = U1 * V0, S01 = U0 * V1, S20 = U2 * V0, S02 = U0 * V2, S11 = U1 * V1;
+  const double S30 = U3 * V0, S12 = U1 * V2, S21 = U2 * V1, S03 = U0 * V3;
+  const double S40 = U4 * V0, S04 = U0 * V4, S22 = U2 * V2;
+  const double E0 = S20 + S02;
+  const double T0 = E0 + x2 * S00, T1 = (S30 + S12) + x2 * S10, T2 = (S21 + S03) + x2 * S01;
+  const double h3 = 0.5 * a[3];
+  psi[0] = a[0] * S00 + a[1] * S10 + a[2] * S01 + h3 * T0;
+  psi[1] = a[0] * S10 + a[1] * S20 + a[2] * S11 + h3 * T1;
+  psi[2] = a[0] * S01 + a[1] * S11 + a[2] * S02 + h3 * T2;
+  psi[3] = 0.5 * (a[0] * T0 + a[1] * T1 + a[2] * T2 + h3 * (S40 + S04 + 2 * S22 + x4 * S00 + 2 * x2 * E0));
+}
+#endif
 
 // GL_address / GR_address / G_address: basic_function.cpp:603-636
 template <class T> GKS_DI void contract(int u, int v, int x, double* psi, const double* a, const T& m) {
+#if GKS_FAST
+  if (x == 0) {
+    contract_sep(u, v, psi, a, m);
+    return;
+  }
+#endif
   psi[0] = a[0] * m(u, v, x) + a[1] * m(u + 1, v, x) + a[2] * m(u, v + 1, x) +
            a[3] * 0.5 * (m(u + 2, v, x) + m(u, v + 2, x) + m(u, v, x + 1));
   psi[1] = a[0] * m(u + 1, v, x) + a[1] * m(u + 2, v, x) + a[2] * m(u + 1, v + 1, x) +
@@ -321,6 +365,10 @@ struct TabWhole {
   GKS_DI double operator()(int i, int j, int k) const {
     return k == 0 ? m.uw[i] * m.vw[j] : (k == 1 ? m.uw[i] * m.vw[j] * m.xi2 : m.uw[i] * m.vw[j] * m.xi4);
   }
+  GKS_DI double U(int i) const { return m.uw[i]; }
+  GKS_DI double V(int j) const { return m.vw[j]; }
+  GKS_DI double x2() const { return m.xi2; }
+  GKS_DI double x4() const { return m.xi4; }
 };
 // a strided read-only view (e.g. one face's column of an LDS tile)
 struct Strided {
