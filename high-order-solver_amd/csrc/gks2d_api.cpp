@@ -83,7 +83,7 @@ struct gks2d_ctx {
   int last_flux_stage = -1;
   int kept_stage_mask = 0;   // stages whose raw face fluxes are stored
   // per-phase HIP-event timing (gks2d_timing)
-  bool timing = false;
+  unsigned timing = 0;  // bit mask of the timed phases
   struct Rec {
     int phase;
     hipEvent_t a, b;
@@ -105,16 +105,27 @@ static hipEvent_t pool_event(gks2d_ctx* c) {
 }
 // launches one kernel phase; with timing on, brackets it with events on the
 // context's stream
+static int collect_timing(gks2d_ctx* c);
+// Timing events are read back in batches of this many timed launches (which
+// bounds the event pool): draining the stream after every step would keep
+// the next step's launches from queueing behind the current one.
+constexpr size_t kTimingBatch = 2048;
 template <class F> static int timed(gks2d_ctx* c, int phase, F&& launch) {
   hipEvent_t a = nullptr, b = nullptr;
-  if (c->timing) {
+  const bool on = (c->timing >> phase) & 1u;
+  if (on && c->pending.size() >= kTimingBatch) {
+    HIPCHK(hipStreamSynchronize(c->s));
+    const int rc = collect_timing(c);
+    if (rc != GKS_OK) return rc;
+  }
+  if (on) {
     a = pool_event(c);
     b = pool_event(c);
     if (!a || !b) return fail(GKS_ERR_HIP, "hipEventCreate");
     HIPCHK(hipEventRecord(a, c->s));
   }
   HIPCHK(launch());
-  if (c->timing) {
+  if (on) {
     HIPCHK(hipEventRecord(b, c->s));
     c->pending.push_back({phase, a, b});
   }
@@ -347,7 +358,7 @@ int gks2d_update_alpha(gks2d_ctx* c) {
   if (!c) return fail(GKS_ERR_ARG, "null");
   return timed(c, PH_ALPHA, [&] { return c->lt->alpha(c->d, c->s); });
 }
-int gks2d_timing(gks2d_ctx* c, int enable) {
+int gks2d_timing_phases(gks2d_ctx* c, uint32_t mask) {
   if (!c) return fail(GKS_ERR_ARG, "null");
   HIPCHK(hipStreamSynchronize(c->s));
   RC(collect_timing(c));
@@ -355,9 +366,10 @@ int gks2d_timing(gks2d_ctx* c, int enable) {
     c->t_ms[k] = 0.0;
     c->t_n[k] = 0;
   }
-  c->timing = enable != 0;
+  c->timing = mask & ((1u << PH_COUNT) - 1u);
   return GKS_OK;
 }
+int gks2d_timing(gks2d_ctx* c, int enable) { return gks2d_timing_phases(c, enable ? (1u << PH_COUNT) - 1u : 0u); }
 int gks2d_get_timing(gks2d_ctx* c, double* ms, int64_t* launches) {
   if (!c || !ms || !launches) return fail(GKS_ERR_ARG, "null");
   HIPCHK(hipStreamSynchronize(c->s));
@@ -455,10 +467,6 @@ int gks2d_run(gks2d_ctx* c, const int32_t bc[4], double tstop, int32_t max_steps
     c->last_slot = 1;
     c->last_flux_stage = 1;
     HIPCHK(hipEventSynchronize(c->ev));
-    if (c->timing) {
-      HIPCHK(hipStreamSynchronize(c->s));
-      RC(collect_timing(c));
-    }
     if (c->h_stat[4] != 0.0) break;
     tmax = c->h_stat[2];
     t = t + dt;

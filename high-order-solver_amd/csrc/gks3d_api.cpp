@@ -73,7 +73,7 @@ struct gks3d_ctx {
   double tmax = 0.0, rhomin = 0.0;
   std::vector<void*> allocs;
   // per-phase HIP-event timing (gks3d_timing)
-  bool timing = false;
+  unsigned timing = 0;  // bit mask of the timed phases
   struct Rec {
     int phase;
     hipEvent_t a, b;
@@ -98,14 +98,15 @@ static hipEvent_t pool_event(gks3d_ctx* c) {
 template <class F>
 static int timed(gks3d_ctx* c, int phase, F&& launch) {
   hipEvent_t a = nullptr, b = nullptr;
-  if (c->timing) {
+  const bool on = (c->timing >> phase) & 1u;
+  if (on) {
     a = pool_event(c);
     b = pool_event(c);
     if (!a || !b) return fail(GKS_ERR_HIP, "hipEventCreate");
     HIPCHK(hipEventRecord(a, c->s));
   }
   HIPCHK(launch());
-  if (c->timing) {
+  if (on) {
     HIPCHK(hipEventRecord(b, c->s));
     c->pending.push_back({phase, a, b});
   }
@@ -392,7 +393,7 @@ int gks3d_update(gks3d_ctx* c, int32_t stage) {
   return timed(c, GKS3_PHASE_UPDATE, [&] { return c->lt->update(c->d, c->s); });
 }
 
-int gks3d_timing(gks3d_ctx* c, int enable) {
+int gks3d_timing_phases(gks3d_ctx* c, uint32_t mask) {
   if (!c) return fail(GKS_ERR_ARG, "null");
   HIPCHK(hipStreamSynchronize(c->s));
   RC(collect_timing(c));
@@ -400,9 +401,10 @@ int gks3d_timing(gks3d_ctx* c, int enable) {
     c->t_ms[k] = 0.0;
     c->t_n[k] = 0;
   }
-  c->timing = enable != 0;
+  c->timing = mask & ((1u << GKS3_NPHASE) - 1u);
   return GKS_OK;
 }
+int gks3d_timing(gks3d_ctx* c, int enable) { return gks3d_timing_phases(c, enable ? (1u << GKS3_NPHASE) - 1u : 0u); }
 int gks3d_get_timing(gks3d_ctx* c, double* ms, int64_t* launches) {
   if (!c || !ms || !launches) return fail(GKS_ERR_ARG, "null");
   HIPCHK(hipStreamSynchronize(c->s));

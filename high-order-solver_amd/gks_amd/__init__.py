@@ -44,6 +44,7 @@ _SIGS = {
     "gks2d_get_face_fluxes": (ctypes.c_int, [_P, _D, _D]),
     "gks2d_keep_face_fluxes": (ctypes.c_int, [_P, ctypes.c_int]),
     "gks2d_timing": (ctypes.c_int, [_P, ctypes.c_int]),
+    "gks2d_timing_phases": (ctypes.c_int, [_P, ctypes.c_uint32]),
     "gks2d_get_timing": (ctypes.c_int, [_P, _D, ctypes.POINTER(ctypes.c_int64)]),
     "gks2d_last_error": (ctypes.c_char_p, []),
 }
@@ -220,8 +221,17 @@ class Solver2D:
         _check(rc)
         return n
 
-    def timing(self, enable=True):
-        _check(lib().gks2d_timing(self.h, 1 if enable else 0))
+    def timing(self, enable=True, phases=None):
+        """Resets and (with enable) starts per-phase HIP-event timing, of
+        every phase or only of the named ones (each timed launch costs the
+        step two events)."""
+        if enable and phases is not None:
+            mask = 0
+            for ph in phases:
+                mask |= 1 << PHASES.index(ph)
+            _check(lib().gks2d_timing_phases(self.h, mask))
+        else:
+            _check(lib().gks2d_timing(self.h, 1 if enable else 0))
 
     def get_timing(self):
         """{phase: (total ms, launches)} accumulated since timing(True)."""

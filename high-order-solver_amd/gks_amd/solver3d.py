@@ -37,6 +37,7 @@ SIGS3 = {
     "gks3d_flux": (ctypes.c_int, [_P, ctypes.c_int32, ctypes.c_double]),
     "gks3d_update": (ctypes.c_int, [_P, ctypes.c_int32]),
     "gks3d_timing": (ctypes.c_int, [_P, ctypes.c_int]),
+    "gks3d_timing_phases": (ctypes.c_int, [_P, ctypes.c_uint32]),
     "gks3d_get_timing": (ctypes.c_int, [_P, _D, ctypes.POINTER(ctypes.c_int64)]),
     "gks3d_flux_points": (ctypes.c_int, [ctypes.POINTER(Params3D), _I64, _D, _D, ctypes.c_double, _D, _D, _D]),
     "gks3d_last_error": (ctypes.c_char_p, []),
@@ -191,9 +192,16 @@ class Solver3D:
     def update(self, stage):
         _check(lib3().gks3d_update(self.h, stage))
 
-    def timing(self, on=True):
-        """Resets and (with on) starts per-phase HIP-event timing."""
-        _check(lib3().gks3d_timing(self.h, 1 if on else 0))
+    def timing(self, on=True, phases=None):
+        """Resets and (with on) starts per-phase HIP-event timing, of every
+        phase or only of the named ones."""
+        if on and phases is not None:
+            mask = 0
+            for ph in phases:
+                mask |= 1 << PHASES3.index(ph)
+            _check(lib3().gks3d_timing_phases(self.h, mask))
+        else:
+            _check(lib3().gks3d_timing(self.h, 1 if on else 0))
 
     def get_timing(self):
         """{phase: (summed ms, launches)} since timing(True)."""

@@ -139,6 +139,12 @@ enum {
   GKS_PHASE_UPDATE, GKS_PHASE_ALPHA, GKS_PHASE_COPY_OLD, GKS_NPHASE
 };
 int gks2d_timing(gks2d_ctx* ctx, int enable);
+/* Times only the phases whose bit (1 << GKS_PHASE_x) is set in mask (0: off;
+ * gks2d_timing(ctx, 1) is the mask of every phase).  Each timed launch is
+ * bracketed by two events on the stream, which cost the step a few
+ * microseconds each, so a caller that only needs one kernel's duration
+ * times only that phase.  Resets the accumulators like gks2d_timing. */
+int gks2d_timing_phases(gks2d_ctx* ctx, uint32_t mask);
 int gks2d_get_timing(gks2d_ctx* ctx, double* ms, int64_t* launches);
 
 /* Leaf parity entry: GKS2D (flux_function.cpp:345-698) preceded by the

@@ -100,6 +100,10 @@ enum {
   GKS3_PHASE_UPDATE, GKS3_PHASE_COPY_OLD, GKS3_NPHASE
 };
 int gks3d_timing(gks3d_ctx* ctx, int enable);
+/* Times only the phases whose bit (1 << GKS3_PHASE_x) is set in mask (0:
+ * off; gks3d_timing(ctx, 1) is the mask of every phase).  Resets the
+ * accumulators like gks3d_timing. */
+int gks3d_timing_phases(gks3d_ctx* ctx, uint32_t mask);
 int gks3d_get_timing(gks3d_ctx* ctx, double* ms, int64_t* launches);
 /* Leaf parity: GKS3D at n Gauss points.  left/right: n x 20 (convar[5],
  * der1x[5], der1y[5], der1z[5] in the face frame); center (optional):

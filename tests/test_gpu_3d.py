@@ -192,6 +192,25 @@ def test_phase_timing_counts_every_launch():
         assert tim[ph][0] > 0.0, ph
 
 
+def test_phase_mask_times_only_the_named_phases():
+    p = tgv_params(16, fast_math=1)
+    s = Solver3D(p)
+    s.init_convar(tgv_convar(p))
+    s.timing(True, phases=["flux"])
+    n = s.run([BC_PERIODIC] * 6, 10.0, 2)
+    tim = s.get_timing()
+    u = s.get_convar()
+    s.close()
+    assert tim["flux"][1] == 6 * n and tim["flux"][0] > 0.0
+    assert all(v[1] == 0 for ph, v in tim.items() if ph != "flux")
+    # timing changes nothing in the state
+    r = Solver3D(p)
+    r.init_convar(tgv_convar(p))
+    assert r.run([BC_PERIODIC] * 6, 10.0, 2) == n
+    assert np.array_equal(r.get_convar(), u)
+    r.close()
+
+
 def test_sweep_by_sweep_loop_equals_the_device_run():
     """The reference's loop on the host, one ABI call per sweep (copy old,
     CFL, per stage the six side fills in the reference's order, primitive

@@ -246,6 +246,34 @@ def test_deterministic_run_to_run():
     assert outs[0][2] == outs[1][2]
 
 
+def test_timing_does_not_change_the_run_and_masks_phases():
+    """Events around the timed launches only: the state is the same with
+    every phase timed, with the flux sweeps timed and untimed, and a phase
+    mask counts exactly its own launches."""
+    p = planar_shock_params(96, fast_math=1)
+    prim = riemann_prim(p, riemann_states(5))
+    outs = []
+    for mode in (None, "flux", "all"):
+        s = gpu_solver(p, prim)
+        if mode == "flux":
+            s.timing(True, phases=["flux_x", "flux_y"])
+        elif mode == "all":
+            s.timing(True)
+        n = s.run([BC_FREE] * 4, 0.25, 12)
+        tim = s.get_timing()
+        outs.append(s.get_convar())
+        s.close()
+        if mode == "flux":
+            assert tim["flux_x"][1] == 2 * n and tim["flux_y"][1] == 2 * n
+            assert all(v[1] == 0 for ph, v in tim.items() if ph not in ("flux_x", "flux_y"))
+        elif mode == "all":
+            assert tim["flux_x"][1] == 2 * n and tim["update"][1] == 2 * n and tim["copy_old"][1] == n
+        else:
+            assert all(v[1] == 0 for v in tim.values())
+    np.testing.assert_array_equal(outs[0], outs[1])
+    np.testing.assert_array_equal(outs[0], outs[2])
+
+
 def test_granular_calls_equal_fused_step():
     """The reference-shaped per-sweep entry points and gks2d_step / gks2d_run
     produce the same bits."""
