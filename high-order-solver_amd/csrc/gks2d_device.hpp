@@ -126,12 +126,16 @@ GKS_DI void to_char(double* ch, const CharBase& b, const double* c) {
 #pragma unroll
   for (int j = 0; j < 4; j++) a = a + s0[j] * c[j];
   ch[0] = a;
+#if GKS_FAST
+  ch[1] = c[2] - v * c[0];  // the zero and unit entries dropped
+#else
   a = 0.0;
   a = a + (-v) * c[0];
   a = a + 0. * c[1];
   a = a + 1. * c[2];
   a = a + 0. * c[3];
   ch[1] = a;
+#endif
   a = 0.0;
 #pragma unroll
   for (int j = 0; j < 4; j++) a = a + s2[j] * c[j];
@@ -154,6 +158,13 @@ GKS_DI void from_char(double* c, const CharBase& b, const double* ch, const Gas&
 #endif
   const double s3[4] = {hq, v, hq + cs * u + h0, hq - cs * u + h0};
   double a;
+#if GKS_FAST
+  // the zero entries of the first two rows dropped
+  (void)s0;
+  (void)s1;
+  c[0] = ch[0] + ch[2] + ch[3];
+  c[1] = u * ch[0] + (u + cs) * ch[2] + (u - cs) * ch[3];
+#else
   a = 0.0;
 #pragma unroll
   for (int j = 0; j < 4; j++) a = a + s0[j] * ch[j];
@@ -162,6 +173,7 @@ GKS_DI void from_char(double* c, const CharBase& b, const double* ch, const Gas&
 #pragma unroll
   for (int j = 0; j < 4; j++) a = a + s1[j] * ch[j];
   c[1] = a;
+#endif
   a = 0.0;
 #pragma unroll
   for (int j = 0; j < 4; j++) a = a + s2[j] * ch[j];
@@ -789,6 +801,54 @@ GKS_DI void weno5_2gauss(double& g1, double& g1x, double& g2, double& g2x, const
 #define GKS_D108H / (108 * h)
 #endif
   double p[4], px[4];
+#if GKS_FAST
+  // the same candidate polynomials with the coefficients of each stencil
+  // value combined at compile time (the written form repeats a value in
+  // several terms, e.g. - wn2 - 70 sqrt3 wn2)
+  {
+    constexpr double s3 = 1.732050807568877294;
+    constexpr double c43 = 1.0 / (4 * s3), q = 1.0 / 4320;
+    constexpr double a0 = 1 - s3 / 4, a1 = 1 / s3, a2 = -c43;  // w0, wn1, wn2 (and mirrored)
+    constexpr double b0 = (4 + s3) / 4, b1 = -s3 / 3, b2 = s3 / 12;
+    constexpr double e0 = 4314 * q, e1 = (4 + 500 * s3) * q, e2 = (-1 - 70 * s3) * q, e3 = (4 - 500 * s3) * q,
+                     e4 = (-1 + 70 * s3) * q;
+    constexpr double k0 = -9 + s3, k1 = -2 * (-6 + s3), k2 = -3 + s3;      // px[0] of point 1
+    constexpr double m0 = -2 * s3, m1 = 3 + s3, m2 = -3 + s3;             // px[1] of point 1
+    constexpr double n0 = 9 + s3, n1 = -2 * (6 + s3), n2 = 3 + s3;        // px[2] of point 1
+    constexpr double t0 = 48 * s3, t1 = -72 - 26 * s3, t2 = 9 + 2 * s3, t3 = 72 - 26 * s3, t4 = -9 + 2 * s3;
+    p[0] = w0 + df[0] * ((a0 * w0 + a1 * wn1 + a2 * wn2) - w0);
+    p[1] = w0 + df[1] * ((w0 + c43 * wn1 - c43 * wp1) - w0);
+    p[2] = w0 + df[2] * ((b0 * w0 + b1 * wp1 + b2 * wp2) - w0);
+    p[3] = w0 + df[3] * ((e0 * w0 + e1 * wn1 + e2 * wn2 + e3 * wp1 + e4 * wp2) - w0);
+    px[0] = -df[0] * ((k0 * w0 + k1 * wn1 + k2 * wn2) * r6h);
+    px[1] = -df[1] * ((m0 * w0 + m1 * wn1 + m2 * wp1) * r6h);
+    px[2] = -df[2] * ((n0 * w0 + n1 * wp1 + n2 * wp2) * r6h);
+    px[3] = df[3] * ((t0 * w0 + t1 * wn1 + t2 * wn2 + t3 * wp1 + t4 * wp2) * r108h);
+    g1 = 0.0;
+    g1x = 0.0;
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+      g1 += fw[k] * p[k];
+      g1x += fw[k] * px[k];
+    }
+    p[0] = w0 + df[0] * ((b0 * w0 + b1 * wn1 + b2 * wn2) - w0);
+    p[1] = w0 + df[1] * ((w0 - c43 * wn1 + c43 * wp1) - w0);
+    p[2] = w0 + df[2] * ((a0 * w0 + a1 * wp1 + a2 * wp2) - w0);
+    p[3] = w0 + df[3] * ((e0 * w0 + e3 * wn1 + e4 * wn2 + e1 * wp1 + e2 * wp2) - w0);
+    px[0] = df[0] * ((n0 * w0 + n1 * wn1 + n2 * wn2) * r6h);
+    px[1] = df[1] * ((m0 * w0 + m2 * wn1 + m1 * wp1) * r6h);
+    px[2] = df[2] * ((k0 * w0 + k1 * wp1 + k2 * wp2) * r6h);
+    px[3] = -df[3] * ((t0 * w0 + t3 * wn1 + t4 * wn2 + t1 * wp1 + t2 * wp2) * r108h);
+    g2 = 0.0;
+    g2x = 0.0;
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+      g2 += fw[k] * p[k];
+      g2x += fw[k] * px[k];
+    }
+    return;
+  }
+#endif
   p[0] = w0 + df[0] * (w0 - (sqrt3 * w0) / 4 + (4 * wn1 - wn2) GKS_D4S3 - w0);
   p[1] = w0 + df[1] * (w0 + (wn1 - wp1) GKS_D4S3 - w0);
   p[2] = w0 + df[2] * ((1.0 / 12.0) * (3 * (4 + sqrt3) * w0 + sqrt3 * (-4 * wp1 + wp2)) - w0);
