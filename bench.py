@@ -42,7 +42,7 @@ TRAFFIC_2048_FAST = {"flux_x": 1.131e9, "flux_y": 1.140e9}
 # rocprofv3 --pmc SQ_INSTS_VALU_{ADD,MUL,FMA,TRANS}_F64 (x64 lanes, FMA = 2)
 # (profiles/gks2d_riemann2048_fast_kernel_trace.md); below the algorithmic
 # count: reciprocal multiplication, FMA contraction, whole-space tables
-EXECUTED_FLOPS_2048_FAST = {"flux_x": 41.70e9, "flux_y": 41.70e9}
+EXECUTED_FLOPS_2048_FAST = {"flux_x": 39.73e9, "flux_y": 39.74e9}
 PEAK_HBM_GBS = 8000.0
 # 3D units (oracle/flopcount3d, WENO5-AO characteristic, GKS3D gks2nd S2O4):
 # FP64 operations per unit of each kernel phase
