@@ -359,6 +359,58 @@ GKS3_DI void weno5_2gauss3(double& g1, double& g1x, double& g2, double& g2x, dou
   double fw[4], p[4];
   weno5_weights3(fw, wn2, wn1, w0, wp1, wp2, 1e-6);
   const double sqrt3 = 1.732050807568877294;
+#if GKS_FAST
+  // the same candidate polynomials with the coefficients of each stencil
+  // value combined at compile time
+  {
+    constexpr double s3 = 1.732050807568877294;
+    constexpr double c43 = 1.0 / (4 * s3), q = 1.0 / 4320;
+    constexpr double a0 = 1 - s3 / 4, a1 = 1 / s3, a2 = -c43;
+    constexpr double b0 = (4 + s3) / 4, b1 = -s3 / 3, b2 = s3 / 12;
+    constexpr double e0 = 4314 * q, e1 = (4 + 500 * s3) * q, e2 = (-1 - 70 * s3) * q, e3 = (4 - 500 * s3) * q,
+                     e4 = (-1 + 70 * s3) * q;
+    constexpr double k0 = -9 + s3, k1 = -2 * (-6 + s3), k2 = -3 + s3;
+    constexpr double m0 = -2 * s3, m1 = 3 + s3, m2 = -3 + s3;
+    constexpr double n0 = 9 + s3, n1 = -2 * (6 + s3), n2 = 3 + s3;
+    constexpr double t0 = 48 * s3, t1 = -72 - 26 * s3, t2 = 9 + 2 * s3, t3 = 72 - 26 * s3, t4 = -9 + 2 * s3;
+    (void)sqrt3;
+    p[0] = a0 * w0 + a1 * wn1 + a2 * wn2;
+    p[1] = w0 + c43 * wn1 - c43 * wp1;
+    p[2] = b0 * w0 + b1 * wp1 + b2 * wp2;
+    p[3] = e0 * w0 + e1 * wn1 + e2 * wn2 + e3 * wp1 + e4 * wp2;
+    g1 = 0.0;
+#pragma unroll
+    for (int k = 0; k < 4; k++) g1 += fw[k] * p[k];
+    g1x = 0.0;
+    if (DER) {
+      double px[4];
+      px[0] = -(k0 * w0 + k1 * wn1 + k2 * wn2) * H.r6;
+      px[1] = -(m0 * w0 + m1 * wn1 + m2 * wp1) * H.r6;
+      px[2] = -(n0 * w0 + n1 * wp1 + n2 * wp2) * H.r6;
+      px[3] = (t0 * w0 + t1 * wn1 + t2 * wn2 + t3 * wp1 + t4 * wp2) * H.r108;
+#pragma unroll
+      for (int k = 0; k < 4; k++) g1x += fw[k] * px[k];
+    }
+    p[0] = b0 * w0 + b1 * wn1 + b2 * wn2;
+    p[1] = w0 - c43 * wn1 + c43 * wp1;
+    p[2] = a0 * w0 + a1 * wp1 + a2 * wp2;
+    p[3] = e0 * w0 + e3 * wn1 + e4 * wn2 + e1 * wp1 + e2 * wp2;
+    g2 = 0.0;
+#pragma unroll
+    for (int k = 0; k < 4; k++) g2 += fw[k] * p[k];
+    g2x = 0.0;
+    if (DER) {
+      double px[4];
+      px[0] = (n0 * w0 + n1 * wn1 + n2 * wn2) * H.r6;
+      px[1] = (m0 * w0 + m2 * wn1 + m1 * wp1) * H.r6;
+      px[2] = (k0 * w0 + k1 * wp1 + k2 * wp2) * H.r6;
+      px[3] = -((t0 * w0 + t3 * wn1 + t4 * wn2 + t1 * wp1 + t2 * wp2) * H.r108);
+#pragma unroll
+      for (int k = 0; k < 4; k++) g2x += fw[k] * px[k];
+    }
+    return;
+  }
+#endif
   p[0] = w0 - G3_CDIV(sqrt3 * w0, 4) + G3_CDIV(4 * wn1 - wn2, 4 * sqrt3);
   p[1] = w0 + G3_CDIV(wn1 - wp1, 4 * sqrt3);
   p[2] = (1.0 / 12.0) * (3 * (4 + sqrt3) * w0 + sqrt3 * (-4 * wp1 + wp2));
