@@ -201,6 +201,25 @@ GKS3_DI void char_base3(double* s, const double* prim, const Gas3& g) {
 }
 // Convar_to_char_3D: basic_function.cpp:1037-1050
 GKS3_DI void to_char3(double* ch, const double* s, const double* c) {
+#if GKS_FAST
+  // rows 2 and 3 hold three zero entries each: dropped
+#pragma unroll
+  for (int i = 0; i < 5; i++) {
+    if (i == 2) {
+      ch[2] = s[10] * c[0] + s[12] * c[2];
+      continue;
+    }
+    if (i == 3) {
+      ch[3] = s[15] * c[0] + s[18] * c[3];
+      continue;
+    }
+    double a = 0.0;
+#pragma unroll
+    for (int j = 0; j < 5; j++) a = a + s[i * 5 + j] * c[j];
+    ch[i] = a;
+  }
+  return;
+#endif
 #pragma unroll
   for (int i = 0; i < 5; i++) {
     double a = 0.0;
@@ -226,6 +245,15 @@ GKS3_DI RightEig right_eig(const double* prim, const Gas3& g) {
 }
 GKS3_DI void from_char3(double* c, const RightEig& e, const double* ch) {
   const double cs = e.cs, u = e.u, v = e.v, w = e.w, H = e.H;
+#if GKS_FAST
+  // the zero and unit entries dropped
+  c[0] = ch[0] + ch[1] + ch[4];
+  c[1] = (u - cs) * ch[0] + u * ch[1] + (u + cs) * ch[4];
+  c[2] = v * ch[0] + v * ch[1] + ch[2] + v * ch[4];
+  c[3] = w * ch[0] + w * ch[1] + ch[3] + w * ch[4];
+  c[4] = (H - u * cs) * ch[0] + 0.5 * (u * u + v * v + w * w) * ch[1] + v * ch[2] + w * ch[3] + (H + u * cs) * ch[4];
+  return;
+#endif
   const double s[5][5] = {{1., 1.0, 0.0, 0.0, 1.0},
                           {u - cs, u, 0, 0, u + cs},
                           {v, v, 1, 0, v},
