@@ -80,6 +80,21 @@ GKS_DI void weno5_linear_2gauss(double& g1, double& g1x, double& g2, double& g2x
                                 double wp1, double wp2, double h, double ih) {
   (void)h;
   (void)ih;
+#if GKS_FAST
+  {
+    // the coefficients of each stencil value combined at compile time
+    constexpr double s3 = 1.732050807568877294, q = 1.0 / 4320;
+    constexpr double e0 = 4314 * q, e1 = (4 + 500 * s3) * q, e2 = (-1 - 70 * s3) * q, e3 = (4 - 500 * s3) * q,
+                     e4 = (-1 + 70 * s3) * q;
+    constexpr double t0 = 48 * s3, t1 = -72 - 26 * s3, t2 = 9 + 2 * s3, t3 = 72 - 26 * s3, t4 = -9 + 2 * s3;
+    const double r108 = ih * (1.0 / 108.0);
+    g1 = w0 + ((e0 * w0 + e1 * wn1 + e2 * wn2 + e3 * wp1 + e4 * wp2) - w0);
+    g2 = w0 + ((e0 * w0 + e3 * wn1 + e4 * wn2 + e1 * wp1 + e2 * wp2) - w0);
+    g1x = (t0 * w0 + t1 * wn1 + t2 * wn2 + t3 * wp1 + t4 * wp2) * r108;
+    g2x = -((t0 * w0 + t3 * wn1 + t4 * wn2 + t1 * wp1 + t2 * wp2) * r108);
+    return;
+  }
+#endif
   const double sqrt3 = 1.732050807568877294;
   g1 = w0 + (GKS_CDIV(4314 * w0 + (4 + 500 * sqrt3) * wn1 - wn2 - 70 * sqrt3 * wn2 + 4 * wp1 - 500 * sqrt3 * wp1 - wp2 +
                           70 * sqrt3 * wp2,
