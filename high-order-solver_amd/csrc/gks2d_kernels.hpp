@@ -1029,10 +1029,8 @@ template <int DIR> __device__ __forceinline__ bool tangential_linear(const Dev& 
   return smooth != 0 && amax == 0.0;
 }
 // A thread per (Gauss point, face); it reads both sides' stencils.
-template <int DIR>
-__global__ void __launch_bounds__(2 * kTanFlux) k_tanflux_cheap(Dev d, int stage, double dt) {
-  constexpr int TF = kTanFlux;
-  const unsigned blk = xcd_block();
+template <int DIR, int TF>
+__device__ __forceinline__ void tanflux_cheap_body(const Dev& d, unsigned blk, int stage, double dt) {
   const int ni = DIR == 0 ? d.NX + 1 : d.NX, nj = DIR == 0 ? d.NY : d.NY + 1;
   const long long nfaces = (long long)ni * nj;
   const int tl = threadIdx.x % TF, q = threadIdx.x / TF;
@@ -1044,7 +1042,8 @@ __global__ void __launch_bounds__(2 * kTanFlux) k_tanflux_cheap(Dev d, int stage
     lin0 = tangential_linear<DIR>(d, 0, i, j);
     lin1 = tangential_linear<DIR>(d, 1, i, j);
   }
-  const bool cheap = valid && lin0 && lin1;
+  // without the smooth-flow flux every face goes to the list
+  const bool cheap = valid && lin0 && lin1 && d.smooth_flux;
   if (q == 0) {  // the first TF threads: whole waves (TF is a multiple of 64)
     if (valid) {
       count_linear(d.lstat + 2 * kStatSlots, lin0);
@@ -1097,6 +1096,10 @@ __global__ void __launch_bounds__(2 * kTanFlux) k_tanflux_cheap(Dev d, int stage
   }
   const long long f0 = DIR == 0 ? (long long)i * d.ny + j : (long long)i * (d.ny + 1) + j;
   flux_point_tail<DIR>(d, q, f0, nf, g[0], g[1], stage, dt, true);
+}
+template <int DIR>
+__global__ void __launch_bounds__(2 * kTanFlux) k_tanflux_cheap(Dev d, int stage, double dt) {
+  tanflux_cheap_body<DIR, kTanFlux>(d, xcd_block(), stage, dt);
 }
 // one tile of tlist[DIR] per workgroup, launched for as many tiles as there
 // are faces (the count is only known on the device; workgroups past it
@@ -1257,6 +1260,31 @@ kb_tanflux(BatchArgs B, int stage) {
   GKS_BATCH_TILE(DIR == 0 ? KB_FLUX_X : KB_FLUX_Y);
   tanflux_body<DIR, LEARNED, TF>(d, blk, stage, B.st[p_].dt);
 }
+// the learned flux sweep in two launches over every problem (as
+// k_tanflux_cheap / k_tanflux_list): smooth faces, then each problem's
+// list, one list tile per workgroup of that problem's face tiles
+template <int DIR, int TF>
+__global__ void __launch_bounds__(2 * TF) kb_tanflux_cheap(BatchArgs B, int stage) {
+  GKS_BATCH_TILE(DIR == 0 ? KB_FLUX_X : KB_FLUX_Y);
+  tanflux_cheap_body<DIR, TF>(d, blk, stage, B.st[p_].dt);
+}
+template <int DIR, int TF>
+__global__ void __launch_bounds__(2 * TF) __attribute__((amdgpu_waves_per_eu(GKS_FLUX_WAVES)))
+kb_tanflux_list(BatchArgs B, int stage) {
+  GKS_BATCH_TILE(DIR == 0 ? KB_FLUX_X : KB_FLUX_Y);
+  const long long n = d.tcount[DIR];
+  if ((long long)blk * TF < n) tanflux_body<DIR, true, TF, true>(d, blk, stage, B.st[p_].dt, d.tlist[DIR], n);
+}
+// clears both list counts of every active problem (one lane per problem)
+__global__ void __launch_bounds__(64) kb_clear_counts(BatchArgs B) {
+  for (int q = threadIdx.x; q < B.np; q += 64) {
+    const int p = B.pid[q];
+    if (B.st[p].active && B.dev[p].tcount) {
+      B.dev[p].tcount[0] = 0u;
+      B.dev[p].tcount[1] = 0u;
+    }
+  }
+}
 __global__ void __launch_bounds__(kBlock) kb_update(BatchArgs B) {
   GKS_BATCH_TILE(KB_UPDATE);
   update_body(d, blk);
@@ -1352,6 +1380,14 @@ template __global__ void kb_tanflux<0, false, 128>(BatchArgs, int);
 template __global__ void kb_tanflux<1, false, 128>(BatchArgs, int);
 template __global__ void kb_tanflux<0, true, 128>(BatchArgs, int);
 template __global__ void kb_tanflux<1, true, 128>(BatchArgs, int);
+template __global__ void kb_tanflux_cheap<0, 64>(BatchArgs, int);
+template __global__ void kb_tanflux_cheap<1, 64>(BatchArgs, int);
+template __global__ void kb_tanflux_cheap<0, 128>(BatchArgs, int);
+template __global__ void kb_tanflux_cheap<1, 128>(BatchArgs, int);
+template __global__ void kb_tanflux_list<0, 64>(BatchArgs, int);
+template __global__ void kb_tanflux_list<1, 64>(BatchArgs, int);
+template __global__ void kb_tanflux_list<0, 128>(BatchArgs, int);
+template __global__ void kb_tanflux_list<1, 128>(BatchArgs, int);
 template __global__ void k_normal_learned<0>(Dev);
 template __global__ void k_normal_learned<1>(Dev);
 template __global__ void k_label<0>(Dev, int, double, float*, float*);
@@ -1478,8 +1514,16 @@ static void batch_stage(const BatchArgs& B, const unsigned* tiles, int stage, hi
   hipLaunchKernelGGL(kb_reduce, dim3(B.np), dim3(kBlock), 0, s, B, stage);
   hipLaunchKernelGGL((kb_normal<0, LEARNED>), dim3(tiles[KB_NORMAL_X]), dim3(kBlock), 0, s, B);
   hipLaunchKernelGGL((kb_normal<1, LEARNED>), dim3(tiles[KB_NORMAL_Y]), dim3(kBlock), 0, s, B);
-  hipLaunchKernelGGL((kb_tanflux<0, LEARNED, TF>), dim3(tiles[KB_FLUX_X]), dim3(2 * TF), 0, s, B, stage);
-  hipLaunchKernelGGL((kb_tanflux<1, LEARNED, TF>), dim3(tiles[KB_FLUX_Y]), dim3(2 * TF), 0, s, B, stage);
+  if (LEARNED) {
+    hipLaunchKernelGGL(kb_clear_counts, dim3(1), dim3(64), 0, s, B);
+    hipLaunchKernelGGL((kb_tanflux_cheap<0, TF>), dim3(tiles[KB_FLUX_X]), dim3(2 * TF), 0, s, B, stage);
+    hipLaunchKernelGGL((kb_tanflux_list<0, TF>), dim3(tiles[KB_FLUX_X]), dim3(2 * TF), 0, s, B, stage);
+    hipLaunchKernelGGL((kb_tanflux_cheap<1, TF>), dim3(tiles[KB_FLUX_Y]), dim3(2 * TF), 0, s, B, stage);
+    hipLaunchKernelGGL((kb_tanflux_list<1, TF>), dim3(tiles[KB_FLUX_Y]), dim3(2 * TF), 0, s, B, stage);
+  } else {
+    hipLaunchKernelGGL((kb_tanflux<0, LEARNED, TF>), dim3(tiles[KB_FLUX_X]), dim3(2 * TF), 0, s, B, stage);
+    hipLaunchKernelGGL((kb_tanflux<1, LEARNED, TF>), dim3(tiles[KB_FLUX_Y]), dim3(2 * TF), 0, s, B, stage);
+  }
   hipLaunchKernelGGL(kb_update, dim3(tiles[KB_UPDATE]), dim3(kBlock), 0, s, B);
   hipLaunchKernelGGL(kb_alpha, dim3(tiles[KB_ALPHA]), dim3(kBlock), 0, s, B);
 }
