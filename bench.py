@@ -37,7 +37,7 @@ PEAK_FP64_TFLOPS = 78.6  # MI355X FP64 (vector and matrix)
 # rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE (profiles/gks2d_riemann2048_fast_hbm_traffic.md),
 # each scaled by the counter's reading for k_copy_old (4 x 2048^2 doubles read
 # and written): fetch x1.882, write x0.985
-TRAFFIC_2048_FAST = {"flux_x": 1.131e9, "flux_y": 1.140e9}
+TRAFFIC_2048_FAST = {"flux_x": 1.131e9, "flux_y": 1.139e9}
 # FP64 operations the fast flux kernel executes per launch at 2048^2, from
 # rocprofv3 --pmc SQ_INSTS_VALU_{ADD,MUL,FMA,TRANS}_F64 (x64 lanes, FMA = 2)
 # (profiles/gks2d_riemann2048_fast_kernel_trace.md); below the algorithmic

@@ -882,7 +882,7 @@ k_flux(Dev d, int stage, double dt) {
 // wave per SIMD; asking for two waves spills about 24 of them to scratch
 // (a few hundred bytes per lane, cache resident) and the second wave hides
 // the latency: 1.54 -> 1.16 ms per launch at 2048^2.
-constexpr int kTanFlux = 128;
+constexpr int kTanFlux = 64;
 // LISTED (learned path only): the tile's faces are list[blk * TF ...], nlist
 // entries in all, instead of the consecutive faces blk * TF ...; they are
 // the faces k_tanflux_cheap left (not both sides linear), so the smooth-flow
