@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -57,6 +58,114 @@ int check_params(const gks1d_params* p) {
   return GKS_OK;
 }
 
+// geometry and gas of a tube (1D SetUniformMesh, basic_function.cpp:1262-1309);
+// the buffer pointers are left null
+void dev_of(const gks1d_params& p, Dev1& d) {
+  std::memset(&d, 0, sizeof(d));
+  d.N = p.nodex;
+  d.G = p.ghost;
+  d.nx = d.N + 2 * d.G;
+  d.recon_var = p.recon_variable;
+  d.use_df = p.use_df;
+  d.df_from_start = p.df_from_start;
+  d.dx = (p.right - p.left) / p.nodex;
+  d.idx = 1.0 / d.dx;
+  d.cfl = p.cfl;
+  d.cfl_dx = p.cfl * d.dx;                        // CFL * dx (Dtx, time_advance.cpp:93-111)
+  d.visc_dt = 0.25 * p.cfl * d.dx * d.dx / p.mu;  // the viscous limit of Dtx
+  d.gas = gas_of(p);
+}
+
+int check_run_args(const Dev1& d, const int32_t bc[2], int32_t max_steps) {
+  for (int s = 0; s < 2; s++)
+    if (bc[s] < 0 || bc[s] > 2) return fail(GKS_ERR_ARG, "bc kind");
+  if (max_steps < 0 || max_steps > 10000000) return fail(GKS_ERR_ARG, "max_steps must be in [0, 1e7]");
+  if ((bc[0] != GKS_BC_FREE || bc[1] != GKS_BC_FREE) && d.N < d.G)
+    return fail(GKS_ERR_ARG, "periodic / reflective boundaries need nodex >= ghost");
+  return GKS_OK;
+}
+
+// The driver's time loop in launches of the persistent kernel (up to
+// kStepsPerLaunch1d steps each); d_dt is the device dt record of one launch.
+int run_loop(const Dev1& d, const Launch1dTable* lt, hipStream_t s, double* d_dt, const int32_t bc[2],
+             double tstop, int32_t max_steps, double* t_inout, int32_t* step_inout, double* dt_hist) {
+  double t = *t_inout;
+  int32_t step = *step_inout;
+  int32_t n = 0;
+  double st[4];
+  while (t < tstop && n < max_steps) {
+    const int chunk = max_steps - n < kStepsPerLaunch1d ? max_steps - n : kStepsPerLaunch1d;
+    HIPCHK(lt->run(d, bc[0], bc[1], tstop, t, step, chunk, d_dt, s));
+    HIPCHK(hipMemcpyAsync(st, d.st, sizeof(st), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    const int ran = (int)st[2];
+    if (dt_hist && ran > 0) {
+      HIPCHK(hipMemcpyAsync(dt_hist + n, d_dt, (size_t)ran * sizeof(double), hipMemcpyDeviceToHost, s));
+      HIPCHK(hipStreamSynchronize(s));
+    }
+    t = st[0];
+    step = (int32_t)st[1];
+    n += ran;
+    if (ran < chunk) break;  // the loop condition ended the launch
+  }
+  *t_inout = t;
+  *step_inout = step;
+  return GKS_OK;
+}
+
+}  // namespace
+
+// ---- the allocation-free run (gks1d_run_noalloc) -------------------------
+// Every device buffer of the tube other than the caller's state (U_old, the
+// DF sensor when the caller passes none, face states, face fluxes, the loop
+// status and the dt record of one launch) lives in one file-scope array,
+// sized for tubes of up to kNoAllocMaxNodex interior cells.  Each GPU holds
+// its own copy for the life of the process; every call on a GPU shares it.
+namespace {
+constexpr size_t kNoAllocMaxNodex = 16384;
+constexpr size_t kRnd = 32;  // doubles: each part starts on a 256-byte boundary
+constexpr size_t rnd(size_t n) { return (n + kRnd - 1) / kRnd * kRnd; }
+constexpr size_t kNxMax = kNoAllocMaxNodex + 2 * 5, kNfMax = kNxMax + 1;
+constexpr size_t kOffUold = 0;
+constexpr size_t kOffAlpha = kOffUold + rnd(3 * kNxMax);
+constexpr size_t kOffFace = kOffAlpha + rnd(kNxMax);
+constexpr size_t kOffFl = kOffFace + rnd(12 * kNfMax);
+constexpr size_t kOffSt = kOffFl + rnd(12 * kNfMax);
+constexpr size_t kOffDt = kOffSt + rnd(4);
+constexpr size_t kScratchDoubles = kOffDt + rnd(kStepsPerLaunch1d);  // 3.71 MB
+}  // namespace
+
+static __device__ __align__(256) double scratch[kScratchDoubles];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double* g_scratch_ptr[kMaxDevices] = {};
+
+// The device address of the current GPU's copy of `scratch`, looked up once
+// per device with that device current.  Host code uses only this address:
+// the array's own name is a host-side stand-in, not a device pointer.
+int scratch_of_current_device(double** out) {
+  int dev = 0;
+  HIPCHK(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices) return fail(GKS_ERR_ARG, "device ordinal out of range");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* ptr = nullptr;
+    HIPCHK(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+    if (!ptr) return fail(GKS_ERR_HIP, "hipGetSymbolAddress returned null");
+    g_scratch_ptr[dev] = static_cast<double*>(ptr);
+  }
+  *out = g_scratch_ptr[dev];
+  return GKS_OK;
+}
+
+// zero fill (a kernel rather than hipMemsetAsync, so a caller's stream
+// capture records no memset node)
+__global__ void k_zero1d(double* p, int n) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < n) p[k] = 0.0;
+}
 }  // namespace
 
 struct gks1d_ctx {
@@ -100,19 +209,7 @@ int gks1d_create(const gks1d_params* p, gks1d_ctx** out) {
   gks1d_ctx* c = new gks1d_ctx();
   c->p = *p;
   Dev1& d = c->d;
-  std::memset(&d, 0, sizeof(d));
-  d.N = p->nodex;
-  d.G = p->ghost;
-  d.nx = d.N + 2 * d.G;
-  d.recon_var = p->recon_variable;
-  d.use_df = p->use_df;
-  d.df_from_start = p->df_from_start;
-  d.dx = (p->right - p->left) / p->nodex;  // 1D SetUniformMesh
-  d.idx = 1.0 / d.dx;
-  d.cfl = p->cfl;
-  d.cfl_dx = p->cfl * d.dx;                         // CFL * dx (Dtx, time_advance.cpp:93-111)
-  d.visc_dt = 0.25 * p->cfl * d.dx * d.dx / p->mu;  // the viscous limit of Dtx
-  d.gas = gas_of(*p);
+  dev_of(*p, d);
   c->lt = p->fast_math ? &fast::table1d : &ieee::table1d;
   hipError_t e = hipStreamCreateWithFlags(&c->s, hipStreamNonBlocking);
   if (e != hipSuccess) {
@@ -181,31 +278,9 @@ int gks1d_get_face_fluxes(gks1d_ctx* c, int32_t stage, double* out) {
 int gks1d_run(gks1d_ctx* c, const int32_t bc[2], double tstop, int32_t max_steps, double* t_inout,
               int32_t* step_inout, double* dt_hist) {
   if (!c || !bc || !t_inout || !step_inout) return fail(GKS_ERR_ARG, "null");
-  for (int s = 0; s < 2; s++)
-    if (bc[s] < 0 || bc[s] > 2) return fail(GKS_ERR_ARG, "bc kind");
-  if (max_steps < 0 || max_steps > 10000000) return fail(GKS_ERR_ARG, "max_steps must be in [0, 1e7]");
-  if ((bc[0] != GKS_BC_FREE || bc[1] != GKS_BC_FREE) && c->d.N < c->d.G)
-    return fail(GKS_ERR_ARG, "periodic / reflective boundaries need nodex >= ghost");
-  double t = *t_inout;
-  int32_t step = *step_inout;
-  int32_t n = 0;
-  double st[4];
-  while (t < tstop && n < max_steps) {
-    const int chunk = max_steps - n < kStepsPerLaunch1d ? max_steps - n : kStepsPerLaunch1d;
-    HIPCHK(c->lt->run(c->d, bc[0], bc[1], tstop, t, step, chunk, c->d_dt, c->s));
-    HIPCHK(hipMemcpyAsync(st, c->d.st, sizeof(st), hipMemcpyDeviceToHost, c->s));
-    HIPCHK(hipStreamSynchronize(c->s));
-    const int ran = (int)st[2];
-    if (dt_hist && ran > 0)
-      HIPCHK(hipMemcpy(dt_hist + n, c->d_dt, (size_t)ran * sizeof(double), hipMemcpyDeviceToHost));
-    t = st[0];
-    step = (int32_t)st[1];
-    n += ran;
-    if (ran < chunk) break;  // the loop condition ended the launch
-  }
-  *t_inout = t;
-  *step_inout = step;
-  return GKS_OK;
+  int rc = check_run_args(c->d, bc, max_steps);
+  if (rc) return rc;
+  return run_loop(c->d, c->lt, c->s, c->d_dt, bc, tstop, max_steps, t_inout, step_inout, dt_hist);
 }
 
 int gks1d_boundary(gks1d_ctx* c, int32_t side, int32_t kind) {
@@ -279,6 +354,36 @@ int gks1d_flux_points(const gks1d_params* p, int64_t n, const double* left, cons
   (void)hipFree(dd);
   if (e != hipSuccess) return hip_fail(e, "gks1d_flux_points");
   return GKS_OK;
+}
+
+int gks1d_run_noalloc(const gks1d_params* p, double* U, double* alpha, const int32_t bc[2], double tstop,
+                      int32_t max_steps, double* t_inout, int32_t* step_inout, double* dt_hist, void* stream) {
+  int rc = check_params(p);
+  if (rc) return rc;
+  if (!U || !bc || !t_inout || !step_inout) return fail(GKS_ERR_ARG, "null");
+  if ((size_t)p->nodex > kNoAllocMaxNodex)
+    return fail(GKS_ERR_ARG, "gks1d_run_noalloc: nodex exceeds the scratch limit of 16384 interior cells");
+  Dev1 d;
+  dev_of(*p, d);
+  if ((rc = check_run_args(d, bc, max_steps))) return rc;
+  double* scr = nullptr;
+  if ((rc = scratch_of_current_device(&scr))) return rc;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  d.U = U;
+  d.Uold = scr + kOffUold;
+  d.face = scr + kOffFace;
+  d.fl = scr + kOffFl;
+  d.st = scr + kOffSt;
+  if (alpha) {
+    d.alpha = alpha;
+  } else {
+    // a fresh DF sensor, as gks1d_create leaves it (ghost cells stay 0)
+    d.alpha = scr + kOffAlpha;
+    hipLaunchKernelGGL(k_zero1d, dim3((d.nx + 255) / 256), dim3(256), 0, s, d.alpha, d.nx);
+    HIPCHK(hipGetLastError());
+  }
+  return run_loop(d, p->fast_math ? &fast::table1d : &ieee::table1d, s, scr + kOffDt, bc, tstop, max_steps,
+                  t_inout, step_inout, dt_hist);
 }
 
 }  // extern "C"

@@ -36,6 +36,8 @@ SIGS1 = {
     "gks1d_flux": (ctypes.c_int, [_P, ctypes.c_int32, ctypes.c_double]),
     "gks1d_update": (ctypes.c_int, [_P, ctypes.c_int32]),
     "gks1d_update_alpha": (ctypes.c_int, [_P]),
+    "gks1d_run_noalloc": (ctypes.c_int, [ctypes.POINTER(Params1D), _P, _P, _I32P, ctypes.c_double, ctypes.c_int32,
+                                         _D, _I32P, _D, _P]),
 }
 
 _lib1 = None

@@ -55,6 +55,26 @@ int gks1d_get_alpha(gks1d_ctx* ctx, double* alpha, int64_t cell_stride_bytes);
  * limited to 10^7. */
 int gks1d_run(gks1d_ctx* ctx, const int32_t bc[2], double tstop, int32_t max_steps, double* t_inout,
               int32_t* step_inout, double* dt_hist);
+/* Allocation-free variant of gks1d_run.  No counterpart in the reference: it
+ * is for callers that cannot allocate device memory (code under stream
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  It runs
+ * the same kernels as gks1d_run, without a context:
+ *   U       device pointer, [3][nodex + 2*ghost] doubles (SoA, the layout of
+ *           a context's state), read and advanced in place;
+ *   alpha   device pointer, [nodex + 2*ghost] doubles, the DF sensor read and
+ *           advanced in place (ghost entries must be 0), or NULL to start
+ *           from a zero sensor as a freshly created context does;
+ *   stream  a hipStream_t (NULL: the null stream);
+ *   bc, tstop, max_steps, t_inout, step_inout, dt_hist as for gks1d_run.
+ * Every other device buffer (U_old, face states and fluxes, loop status, dt
+ * record) lives in a static per-GPU scratch array of 3.71 MB, sized for
+ * nodex <= 16384; larger tubes get GKS_ERR_ARG.  It calls no hipMalloc /
+ * hipFree.  All calls on one GPU share that GPU's scratch array, which keeps
+ * the last call's contents: calls that overlap on one GPU (other threads or
+ * streams) race and must be serialised by the caller.  The call returns
+ * after its last launch has finished, like gks1d_run. */
+int gks1d_run_noalloc(const gks1d_params* p, double* U, double* alpha, const int32_t bc[2], double tstop,
+                      int32_t max_steps, double* t_inout, int32_t* step_inout, double* dt_hist, void* stream);
 /* f[3], derf[3] of every face (nx+1 faces, face-major, 6 doubles each) from
  * the last step's stage 0 or 1 (Flux1d fluxes[i][stage], fluid_mesh.h:52-58) */
 int gks1d_get_face_fluxes(gks1d_ctx* ctx, int32_t stage, double* out);

@@ -149,6 +149,28 @@ def test_1d_null_arguments_are_argument_errors():
     assert L.gks1d_update_alpha(None) == 1
 
 
+def test_1d_noalloc_checks_arguments_on_host():
+    """The allocation-free run rejects bad parameters, null pointers and
+    tubes beyond its scratch limit (16384 interior cells) before any device
+    call (the fake state pointer is never dereferenced)."""
+    from gks_amd import solver1d
+    from gks_amd.params import sod_params
+    L = solver1d.lib1()
+    bc = (ctypes.c_int32 * 2)(0, 0)
+    t = ctypes.c_double(0.0)
+    s = ctypes.c_int32(0)
+    fake = ctypes.c_void_p(4096)
+    args = (bc, 0.2, 10, ctypes.byref(t), ctypes.byref(s), None, None)
+    assert L.gks1d_run_noalloc(None, fake, None, *args) == 1
+    p = sod_params(400)
+    assert L.gks1d_run_noalloc(ctypes.byref(p), None, None, *args) == 1
+    big = sod_params(16385)
+    assert L.gks1d_run_noalloc(ctypes.byref(big), fake, None, *args) == 1
+    assert b"16384" in L.gks1d_last_error()
+    bad = sod_params(400).copy(ghost=4)
+    assert L.gks1d_run_noalloc(ctypes.byref(bad), fake, None, *args) == 2
+
+
 HEADER_B = os.path.join(ROOT, "include", "gks2d_batch.h")
 
 
