@@ -76,7 +76,7 @@ struct gks2d_ctx {
   float* d_mlp = nullptr;    // classifier parameters (gks2d_set_classifier)
   unsigned short* d_mlpw = nullptr;  // their weights in the MFMA operand layout
   unsigned char* d_lin = nullptr;    // per-cell linear-weight flags, x then y
-  unsigned* d_tlist = nullptr;       // learned flux: listed x faces, y faces, then their 2 counts
+  unsigned* d_tlist = nullptr;       // learned path: listed x, y faces; listed x, y normal cells; 4 counts
   unsigned long long* d_lstat = nullptr;  // its linear / total counters
   std::vector<void*> allocs;
   int last_slot = -1;        // slot of the last primvar / set_cfl_primvar
@@ -586,6 +586,11 @@ int gks2d_classifier_samples(gks2d_ctx* c, int32_t dir, int32_t stride, double l
   return GKS_OK;
 }
 
+// learned-path work lists: flux faces x, y; normal-sweep cells x, y; the
+// four counts
+static size_t list_words(const Dev& d) {
+  return (size_t)(d.nfx + d.nfy + (long long)(d.NX + 2) * d.ny + (long long)d.nx * (d.NY + 2) + 4);
+}
 int gks2d_set_classifier(gks2d_ctx* c, const float* params, float logit_threshold) {
   if (!c) return fail(GKS_ERR_ARG, "null");
   HIPCHK(hipStreamSynchronize(c->s));
@@ -600,7 +605,7 @@ int gks2d_set_classifier(gks2d_ctx* c, const float* params, float logit_threshol
     HIPCHK(hipMalloc(&c->d_mlpw, mlp::kPackHalves * sizeof(unsigned short)));
     HIPCHK(hipMalloc(&c->d_lstat, kLstatWords * sizeof(unsigned long long)));
     HIPCHK(hipMalloc(&c->d_lin, 2 * nc));
-    HIPCHK(hipMalloc(&c->d_tlist, (size_t)(c->d.nfx + c->d.nfy + 2) * sizeof(unsigned)));
+    HIPCHK(hipMalloc(&c->d_tlist, list_words(c->d) * sizeof(unsigned)));
   }
   std::vector<uint16_t> packed(mlp::kPackHalves);
   mlp::pack_bf16(params, packed.data());
@@ -615,7 +620,11 @@ int gks2d_set_classifier(gks2d_ctx* c, const float* params, float logit_threshol
   c->d.lin[1] = c->d_lin + nc;
   c->d.tlist[0] = c->d_tlist;
   c->d.tlist[1] = c->d_tlist + c->d.nfx;
-  c->d.tcount = c->d_tlist + c->d.nfx + c->d.nfy;
+  const long long ncx = (long long)(c->d.NX + 2) * c->d.ny, ncy = (long long)c->d.nx * (c->d.NY + 2);
+  c->d.nlist[0] = c->d_tlist + c->d.nfx + c->d.nfy;
+  c->d.nlist[1] = c->d.nlist[0] + ncx;
+  c->d.tcount = c->d.nlist[1] + ncy;
+  c->d.ncount = c->d.tcount + 2;
   c->d.mlp_thr = logit_threshold;
   c->d.lstat = c->d_lstat;
   return GKS_OK;

@@ -458,11 +458,40 @@ __device__ __forceinline__ void count_linear(unsigned long long* stat, bool lin)
   }
 }
 
+// appends `value` of every lane with `listed` to list (one atomic per wave,
+// ranks by lane order); every active lane of the wave must call it
+__device__ __forceinline__ void list_append(unsigned* count, unsigned* list, bool listed, unsigned value) {
+  const unsigned long long bl = __ballot(listed);
+  if (!bl) return;
+  const int lead = __ffsll((long long)bl) - 1;
+  unsigned base = 0;
+  if ((int)(threadIdx.x & 63) == lead) base = atomicAdd(count, (unsigned)__popcll(bl));
+  base = (unsigned)__shfl((int)base, lead);
+  const unsigned rank = __builtin_amdgcn_mbcnt_hi((unsigned)(bl >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)bl, 0u));
+  if (listed) list[base + rank] = value;
+}
+
+// the two face records a cell's normal reconstruction writes (side 1 of
+// the face at its left, side 0 of the face at its right)
+template <int DIR>
+__device__ __forceinline__ void store_normal_faces(const Dev& d, int i, int j, const double* lo, const double* ro) {
+  double* line = DIR == 0 ? d.line_x : d.line_y;
+  const long long nf = DIR == 0 ? d.nfx : d.nfy;
+  const long long fa = DIR == 0 ? (long long)i * d.ny + j : (long long)i * (d.ny + 1) + j;
+  const long long fb = DIR == 0 ? (long long)(i + 1) * d.ny + j : (long long)i * (d.ny + 1) + j + 1;
+#pragma unroll
+  for (int k = 0; k < 8; k++) {
+    line[(8 + k) * nf + fa] = lo[k];
+    line[k * nf + fb] = ro[k];
+  }
+}
+
 // k_normal with the classifier: a stencil the network calls smooth (logit
 // below the threshold) with a zero DF sensor takes the linear weights, i.e.
 // the fifth-order candidate, formed directly on the conservative variables
 // (the characteristic projection commutes with a fixed linear combination);
-// the order-reduction test is kept.  Every other stencil runs WENO5-AO-DF.
+// the order-reduction test is kept.  Every other stencil is listed for
+// k_normal_list (WENO5-AO-DF), so this kernel carries no WENO registers.
 template <int DIR> __device__ __forceinline__ void normal_learned_body(const Dev& d, unsigned blk) {
   __shared__ __align__(16) short xs[kBlock * 16];
   __shared__ float ls[kBlock];
@@ -490,42 +519,53 @@ template <int DIR> __device__ __forceinline__ void normal_learned_body(const Dev
   const bool lin = z < d.mlp_thr && amax == 0.0;
   count_linear(d.lstat, lin);
   d.lin[DIR][(long long)i * d.ny + j] = lin ? 1 : 0;
+  // the other stencils run WENO5-AO-DF in k_normal_list, in waves of
+  // troubled cells only
+  list_append(d.ncount + DIR, d.nlist[DIR], !lin, (unsigned)t);
+  if (!lin) return;
   const double h = DIR == 0 ? d.dx : d.dy, ih = DIR == 0 ? d.idx : d.idy;
   double lo[8], ro[8];
-  if (lin) {
 #pragma unroll
-    for (int v = 0; v < 4; v++) {
-      weno5_linear_face(0, lo[v], lo[4 + v], w[0][v], w[1][v], w[2][v], w[3][v], w[4][v], h, ih);
-      weno5_linear_face(1, ro[v], ro[4 + v], w[0][v], w[1][v], w[2][v], w[3][v], w[4][v], h, ih);
-    }
-    const bool rr = reduce_order(ro, d.gas);
-    const bool rl = reduce_order(lo, d.gas);
-    if (rl || rr) {
-#pragma unroll
-      for (int m = 0; m < 4; m++) {
-        ro[m] = w[2][m];
-        lo[m] = w[2][m];
-        ro[4 + m] = 0.0;
-        lo[4 + m] = 0.0;
-      }
-    }
-  } else {
-    weno5_normal(lo, ro, w, a, h, ih, d.recon_var, d.gas);
+  for (int v = 0; v < 4; v++) {
+    weno5_linear_face(0, lo[v], lo[4 + v], w[0][v], w[1][v], w[2][v], w[3][v], w[4][v], h, ih);
+    weno5_linear_face(1, ro[v], ro[4 + v], w[0][v], w[1][v], w[2][v], w[3][v], w[4][v], h, ih);
   }
-  double* line = DIR == 0 ? d.line_x : d.line_y;
-  const long long nf = DIR == 0 ? d.nfx : d.nfy;
-  const long long fa = DIR == 0 ? (long long)i * d.ny + j : (long long)i * (d.ny + 1) + j;
-  const long long fb = DIR == 0 ? (long long)(i + 1) * d.ny + j : (long long)i * (d.ny + 1) + j + 1;
+  const bool rr = reduce_order(ro, d.gas);
+  const bool rl = reduce_order(lo, d.gas);
+  if (rl || rr) {
 #pragma unroll
-  for (int k = 0; k < 8; k++) {
-    line[(8 + k) * nf + fa] = lo[k];
-    line[k * nf + fb] = ro[k];
+    for (int m = 0; m < 4; m++) {
+      ro[m] = w[2][m];
+      lo[m] = w[2][m];
+      ro[4 + m] = 0.0;
+      lo[4 + m] = 0.0;
+    }
   }
+  store_normal_faces<DIR>(d, i, j, lo, ro);
 }
 template <int DIR>
 __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(GKS_NORMAL_WAVES)))
 k_normal_learned(Dev d) {
   normal_learned_body<DIR>(d, xcd_block());
+}
+// WENO5-AO-DF over the cells k_normal_learned listed, one thread each;
+// launched with the normal sweep's block count (the list length is known
+// only on the device; blocks past it return at once)
+template <int DIR> __device__ __forceinline__ void normal_list_body(const Dev& d, unsigned blk) {
+  const long long k = (long long)blk * kBlock + threadIdx.x;
+  if (k >= (long long)d.ncount[DIR]) return;
+  int i, j;
+  normal_cell<DIR>(d, (long long)d.nlist[DIR][k], i, j);
+  double w[5][4], a[5];
+  normal_stencil<DIR>(d, i, j, w, a);
+  double lo[8], ro[8];
+  weno5_normal(lo, ro, w, a, DIR == 0 ? d.dx : d.dy, DIR == 0 ? d.idx : d.idy, d.recon_var, d.gas);
+  store_normal_faces<DIR>(d, i, j, lo, ro);
+}
+template <int DIR>
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(GKS_NORMAL_WAVES)))
+k_normal_list(Dev d) {
+  if ((long long)blockIdx.x * kBlock < (long long)d.ncount[DIR]) normal_list_body<DIR>(d, blockIdx.x);
 }
 
 // logits of n feature rows (parity of the MFMA inference)
@@ -998,10 +1038,9 @@ k_tanflux(Dev d, int stage, double dt) {
 // With the smooth-flow flux on, the learned path's flux sweep runs in two
 // launches, so that the many smooth faces do not carry the register budget
 // (two waves per SIMD) of the full flux point.  k_tanflux_cheap takes every
-// face, one thread per (Gauss point, face), no LDS: a face whose two
-// tangential stencils are linear (the tangential stencil rule of
-// tanflux_body) gets its Gauss-point state on each side from the linear
-// weights at that Gauss point and the smooth-flow flux; every other face is
+// face: a face whose two tangential stencils are linear (the tangential
+// stencil rule of tanflux_body) gets its Gauss-point states on each side
+// from the linear weights and the smooth-flow flux; every other face is
 // appended to tlist[DIR] (one wave-aggregated atomic per wave).
 // k_tanflux_list then runs tanflux_body over the listed faces (WENO5-AO-DF
 // or linear per side, full GKS2D) as a grid-stride loop over tiles of the
@@ -1028,74 +1067,75 @@ template <int DIR> __device__ __forceinline__ bool tangential_linear(const Dev& 
   }
   return smooth != 0 && amax == 0.0;
 }
-// A thread per (Gauss point, face); it reads both sides' stencils.
+// Two phases over a tile of TF faces.  Phase 1, thread (side, face): the
+// tangential rule for its side and, when linear, that side's two
+// Gauss-point states from the linear weights into LDS (each stencil loaded
+// and reconstructed once).  Phase 2, thread (Gauss point, face): a face
+// with both sides linear gets the smooth-flow flux from the two states;
+// the first wave lists every other face.
 template <int DIR, int TF>
 __device__ __forceinline__ void tanflux_cheap_body(const Dev& d, unsigned blk, int stage, double dt) {
+  __shared__ double sg[2][2][12][TF];  // [side][gauss point][component][face]
+  __shared__ unsigned char slin[2][TF];
   const int ni = DIR == 0 ? d.NX + 1 : d.NX, nj = DIR == 0 ? d.NY : d.NY + 1;
   const long long nfaces = (long long)ni * nj;
-  const int tl = threadIdx.x % TF, q = threadIdx.x / TF;
+  const int tl = threadIdx.x % TF, hi = threadIdx.x / TF;
   const long long r = (long long)blk * TF + tl;
   const bool valid = r < nfaces;
   const int i = valid ? (int)(r / nj) + d.G : d.G, j = valid ? (int)(r % nj) + d.G : d.G;
-  bool lin0 = false, lin1 = false;
-  if (valid) {
-    lin0 = tangential_linear<DIR>(d, 0, i, j);
-    lin1 = tangential_linear<DIR>(d, 1, i, j);
-  }
-  // without the smooth-flow flux every face goes to the list
-  const bool cheap = valid && lin0 && lin1 && d.smooth_flux;
-  if (q == 0) {  // the first TF threads: whole waves (TF is a multiple of 64)
-    if (valid) {
-      count_linear(d.lstat + 2 * kStatSlots, lin0);
-      count_linear(d.lstat + 2 * kStatSlots, lin1);
-    }
-    const bool listed = valid && !cheap;
-    const unsigned long long bl = __ballot(listed);
-    if (bl) {
-      const int lead = __ffsll((long long)bl) - 1;
-      unsigned base = 0;
-      if ((int)(threadIdx.x & 63) == lead) base = atomicAdd(d.tcount + DIR, (unsigned)__popcll(bl));
-      base = (unsigned)__shfl((int)base, lead);
-      const unsigned rank =
-          __builtin_amdgcn_mbcnt_hi((unsigned)(bl >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)bl, 0u));
-      if (listed) d.tlist[DIR][base + rank] = (unsigned)r;
-    }
-  }
-  if (!cheap) return;
   const long long nf = DIR == 0 ? d.nfx : d.nfy;
-  const double* line = DIR == 0 ? d.line_x : d.line_y;
-  const double h = DIR == 0 ? d.dy : d.dx, ih = DIR == 0 ? d.idy : d.idx;
-  double g[2][12];
-#pragma unroll
-  for (int side = 0; side < 2; side++) {
-    double L[5][8];
-#pragma unroll
-    for (int s = 0; s < 5; s++) {
-      const long long f = DIR == 0 ? (long long)i * d.ny + (j - 2 + s) : (long long)(i + 2 - s) * (d.ny + 1) + j;
-#pragma unroll
-      for (int k = 0; k < 8; k++) L[s][k] = line[(side * 8 + k) * nf + f];
+  {
+    const int side = hi;
+    bool lin = false;
+    if (valid) {
+      lin = tangential_linear<DIR>(d, side, i, j);
+      count_linear(d.lstat + 2 * kStatSlots, lin);
     }
-    double* o = g[side];
+    slin[side][tl] = lin ? 1 : 0;
+    // without the smooth-flow flux every face goes to the list
+    if (lin && d.smooth_flux) {
+      const double* line = DIR == 0 ? d.line_x : d.line_y;
+      const double h = DIR == 0 ? d.dy : d.dx, ih = DIR == 0 ? d.idy : d.idx;
+      double L[5][8];
 #pragma unroll
-    for (int v = 0; v < 4; v++) {
-      double a0, a1, b0, b1, c0, c1, e0, e1;
-      weno5_linear_2gauss(a0, a1, b0, b1, L[0][v], L[1][v], L[2][v], L[3][v], L[4][v], h, ih);
-      weno5_linear_2gauss(c0, c1, e0, e1, L[0][4 + v], L[1][4 + v], L[2][4 + v], L[3][4 + v], L[4][4 + v], h, ih);
-      o[v] = q == 0 ? a0 : b0;
-      o[8 + v] = q == 0 ? a1 : b1;
-      o[4 + v] = q == 0 ? c0 : e0;
-    }
-    if (reduce_order(o, d.gas)) {
+      for (int s = 0; s < 5; s++) {
+        const long long f = DIR == 0 ? (long long)i * d.ny + (j - 2 + s) : (long long)(i + 2 - s) * (d.ny + 1) + j;
+#pragma unroll
+        for (int k = 0; k < 8; k++) L[s][k] = line[(side * 8 + k) * nf + f];
+      }
+      double gp[2][12];
 #pragma unroll
       for (int v = 0; v < 4; v++) {
-        o[v] = L[2][v];
-        o[4 + v] = L[2][4 + v];
-        o[8 + v] = 0.0;
+        weno5_linear_2gauss(gp[0][v], gp[0][8 + v], gp[1][v], gp[1][8 + v], L[0][v], L[1][v], L[2][v], L[3][v],
+                            L[4][v], h, ih);
+        double t0, t1;
+        weno5_linear_2gauss(gp[0][4 + v], t0, gp[1][4 + v], t1, L[0][4 + v], L[1][4 + v], L[2][4 + v],
+                            L[3][4 + v], L[4][4 + v], h, ih);
+      }
+#pragma unroll
+      for (int q = 0; q < 2; q++) {
+        if (reduce_order(gp[q], d.gas)) {
+#pragma unroll
+          for (int v = 0; v < 4; v++) {
+            gp[q][v] = L[2][v];
+            gp[q][4 + v] = L[2][4 + v];
+            gp[q][8 + v] = 0.0;
+          }
+        }
+#pragma unroll
+        for (int k = 0; k < 12; k++) sg[side][q][k][tl] = gp[q][k];
       }
     }
   }
+  __syncthreads();
+  const int q = hi;
+  const bool cheap = valid && d.smooth_flux && (slin[0][tl] & slin[1][tl]);
+  // the first TF threads: whole waves (TF is a multiple of 64)
+  if (q == 0) list_append(d.tcount + DIR, d.tlist[DIR], valid && !cheap, (unsigned)r);
+  if (!cheap) return;
   const long long f0 = DIR == 0 ? (long long)i * d.ny + j : (long long)i * (d.ny + 1) + j;
-  flux_point_tail<DIR>(d, q, f0, nf, g[0], g[1], stage, dt, true);
+  const Strided L{&sg[0][q][0][tl], TF}, R{&sg[1][q][0][tl], TF};
+  flux_point_tail<DIR>(d, q, f0, nf, L, R, stage, dt, true);
 }
 template <int DIR>
 __global__ void __launch_bounds__(2 * kTanFlux) k_tanflux_cheap(Dev d, int stage, double dt) {
@@ -1254,6 +1294,14 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(GKS
   if constexpr (LEARNED) normal_learned_body<DIR>(d, blk);
   else normal_body<DIR>(d, blk);
 }
+// the cells each problem's learned normal sweep listed (as k_normal_list),
+// one list tile per workgroup of that problem's normal tiles
+template <int DIR>
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(GKS_NORMAL_WAVES)))
+kb_normal_list(BatchArgs B) {
+  GKS_BATCH_TILE(DIR == 0 ? KB_NORMAL_X : KB_NORMAL_Y);
+  if (d.ncount && (long long)blk * kBlock < (long long)d.ncount[DIR]) normal_list_body<DIR>(d, blk);
+}
 template <int DIR, bool LEARNED, int TF>
 __global__ void __launch_bounds__(2 * TF) __attribute__((amdgpu_waves_per_eu(GKS_FLUX_WAVES)))
 kb_tanflux(BatchArgs B, int stage) {
@@ -1275,13 +1323,18 @@ kb_tanflux_list(BatchArgs B, int stage) {
   const long long n = d.tcount[DIR];
   if ((long long)blk * TF < n) tanflux_body<DIR, true, TF, true>(d, blk, stage, B.st[p_].dt, d.tlist[DIR], n);
 }
-// clears both list counts of every active problem (one lane per problem)
+// clears the four list counts (flux x, y; normal x, y) of every active
+// problem (one lane per problem), before the stage's normal sweeps
 __global__ void __launch_bounds__(64) kb_clear_counts(BatchArgs B) {
   for (int q = threadIdx.x; q < B.np; q += 64) {
     const int p = B.pid[q];
     if (B.st[p].active && B.dev[p].tcount) {
       B.dev[p].tcount[0] = 0u;
       B.dev[p].tcount[1] = 0u;
+    }
+    if (B.st[p].active && B.dev[p].ncount) {
+      B.dev[p].ncount[0] = 0u;
+      B.dev[p].ncount[1] = 0u;
     }
   }
 }
@@ -1390,6 +1443,10 @@ template __global__ void kb_tanflux_list<0, 128>(BatchArgs, int);
 template __global__ void kb_tanflux_list<1, 128>(BatchArgs, int);
 template __global__ void k_normal_learned<0>(Dev);
 template __global__ void k_normal_learned<1>(Dev);
+template __global__ void k_normal_list<0>(Dev);
+template __global__ void k_normal_list<1>(Dev);
+template __global__ void kb_normal_list<0>(BatchArgs);
+template __global__ void kb_normal_list<1>(BatchArgs);
 template __global__ void k_label<0>(Dev, int, double, float*, float*);
 template __global__ void k_label<1>(Dev, int, double, float*, float*);
 template __global__ void k_flux<0>(Dev, int, double);
@@ -1424,8 +1481,13 @@ static hipError_t normal(const Dev& d, int dir, hipStream_t s) {
     if (dir == 0) hipLaunchKernelGGL((k_normal79<0, 9>), dim3(nb), dim3(kBlock), 0, s, d);
     else hipLaunchKernelGGL((k_normal79<1, 9>), dim3(nb), dim3(kBlock), 0, s, d);
   } else if (d.mlp) {
+    if (d.ncount) hipLaunchKernelGGL(k_clear_count, dim3(1), dim3(64), 0, s, d.ncount + dir);
     if (dir == 0) hipLaunchKernelGGL(k_normal_learned<0>, dim3(nb), dim3(kBlock), 0, s, d);
     else hipLaunchKernelGGL(k_normal_learned<1>, dim3(nb), dim3(kBlock), 0, s, d);
+    if (d.ncount) {
+      if (dir == 0) hipLaunchKernelGGL(k_normal_list<0>, dim3(nb), dim3(kBlock), 0, s, d);
+      else hipLaunchKernelGGL(k_normal_list<1>, dim3(nb), dim3(kBlock), 0, s, d);
+    }
   } else {
     if (dir == 0) hipLaunchKernelGGL(k_normal<0>, dim3(nb), dim3(kBlock), 0, s, d);
     else hipLaunchKernelGGL(k_normal<1>, dim3(nb), dim3(kBlock), 0, s, d);
@@ -1512,10 +1574,12 @@ static void batch_stage(const BatchArgs& B, const unsigned* tiles, int stage, hi
     hipLaunchKernelGGL(kb_bc, dim3(tiles[side < 2 ? KB_BC_LR : KB_BC_DU]), dim3(kBlock), 0, s, B, side);
   hipLaunchKernelGGL(kb_primvar, dim3(tiles[KB_PRIMVAR]), dim3(kBlock), 0, s, B);
   hipLaunchKernelGGL(kb_reduce, dim3(B.np), dim3(kBlock), 0, s, B, stage);
+  if (LEARNED) hipLaunchKernelGGL(kb_clear_counts, dim3(1), dim3(64), 0, s, B);
   hipLaunchKernelGGL((kb_normal<0, LEARNED>), dim3(tiles[KB_NORMAL_X]), dim3(kBlock), 0, s, B);
+  if (LEARNED) hipLaunchKernelGGL(kb_normal_list<0>, dim3(tiles[KB_NORMAL_X]), dim3(kBlock), 0, s, B);
   hipLaunchKernelGGL((kb_normal<1, LEARNED>), dim3(tiles[KB_NORMAL_Y]), dim3(kBlock), 0, s, B);
+  if (LEARNED) hipLaunchKernelGGL(kb_normal_list<1>, dim3(tiles[KB_NORMAL_Y]), dim3(kBlock), 0, s, B);
   if (LEARNED) {
-    hipLaunchKernelGGL(kb_clear_counts, dim3(1), dim3(64), 0, s, B);
     hipLaunchKernelGGL((kb_tanflux_cheap<0, TF>), dim3(tiles[KB_FLUX_X]), dim3(2 * TF), 0, s, B, stage);
     hipLaunchKernelGGL((kb_tanflux_list<0, TF>), dim3(tiles[KB_FLUX_X]), dim3(2 * TF), 0, s, B, stage);
     hipLaunchKernelGGL((kb_tanflux_cheap<1, TF>), dim3(tiles[KB_FLUX_Y]), dim3(2 * TF), 0, s, B, stage);

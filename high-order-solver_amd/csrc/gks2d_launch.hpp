@@ -39,6 +39,9 @@ struct Dev {
   int smooth_flux;              // learned path: faces with two linear sides take the smooth-flow flux
   unsigned* tlist[2];           // learned path: x / y faces left to the full flux (interior face ordinals)
   unsigned* tcount;             // their counts, x then y (cleared before each flux sweep)
+  unsigned* nlist[2];           // learned path: cells whose x / y normal stencil runs WENO5-AO-DF (ordinals of
+                                // the normal sweep's range)
+  unsigned* ncount;             // their counts, x then y (cleared before each normal sweep)
   double* partial;  // per-block (max speed, blow-up flag) pairs
   double* stat;     // [0..3]: max speed of slot 0 / 1 (+pad); [4]: sticky blow-up flag
 };
