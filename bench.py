@@ -3,17 +3,31 @@
 Workload: the seeded four-quadrant Riemann problem on [0,1]^2 at 2048^2
 (5 ghost layers), WENO5-AO with the DF factor in characteristic variables,
 GKS2D (gks2nd_2d, Euler collision time c1=0.05, c2=1), S2O4, CFL 0.5,
-gamma 1.4, K=3, free boundaries.  A step is one full S2O4 step (two stages)
-of the whole grid; Mcell-updates/s = N^2 * steps / wall seconds / 1e6, with
-the state resident in HBM.
+gamma 1.4, K=3, free boundaries, from the initial condition to t=0.25.  A
+step is one full S2O4 step (two stages) of the whole grid;
+Mcell-updates/s = N^2 * steps / wall seconds / 1e6, with the state resident
+in HBM.
+
+The default measurement (the metric: learned vs plain at equal L1) times
+three whole runs to t=0.25 - the plain path, the learned path (trained
+troubled-interface classifier, BF16 MFMA inference) and the learned path
+through the batched executor - and compares the density of each with a
+plain run at (2N)^2 averaged over 2x2 blocks.  `value` is the learned
+path's Mcell-updates/s; `steps` is its number of S2O4 steps; --warmup
+untimed steps run on a separate solver before each leg.  --window instead
+times --steps steps of one --path from t=0.
 
 Prints one JSON line (rank 0).  Also reports:
-  roofline      - the dominant kernel's algorithmic FP64 rate (FLOPs per
-                  launch from the oracle's operation count, duration from HIP
-                  events on the solver's stream over the timed steps);
-  cpu_baseline  - the compiled reference (oracle/_ref) or, if it is not
-                  built, the oracle, timed on this host's cores on a bounded
-                  sample of the same workload.
+  roofline          - the plain path's dominant kernel (k_tanflux, y faces):
+                      algorithmic FP64 FLOPs per launch (the oracle's
+                      operation count) over its average launch duration
+                      from HIP events on the solver's stream in the timed
+                      run;
+  roofline_learned  - the learned path's full-work kernels (listed faces
+                      and cells), each priced by the units it processed;
+  cpu_baseline      - the compiled reference (oracle/_ref) or, if it is
+                      not built, the oracle, timed on this host's cores on
+                      a bounded sample of the same workload.
 """
 import argparse
 import json
@@ -32,8 +46,14 @@ FLOPS_PER_CELL_UPDATE = 57829.3 + 40.04 + 24.02  # N=1024 grid, one S2O4 step pe
 FLOPS_FLUX_POINT = 3041 + 43 + 5 + 3  # centre + GKS2D + DF alpha_k per (face, Gauss point)
 FLOPS_TANGENT_SIDE = 2857  # tangential WENO per (face, side)
 FLOPS_NORMAL_CELL = 2253  # normal WENO per (cell, direction)
+# linear-weight tangential reconstruction of one face side (learned path):
+# the fifth-order candidate at both Gauss points, counted by hand on its
+# IEEE expressions (weno5_linear_2gauss): 70 per component with the
+# tangential derivative x 4, 32 per component without it x 4 (der1x), plus
+# the two order-reduction tests (12 each)
+FLOPS_TANGENT_SIDE_LINEAR = 4 * 70 + 4 * 32 + 2 * 12
 PEAK_FP64_TFLOPS = 78.6  # MI355X FP64 (vector and matrix)
-# HBM bytes per launch of the dominant kernels at 2048^2, fast kernels, from
+# HBM bytes per launch of the dominant kernel at 2048^2, fast kernels, from
 # rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE (profiles/gks2d_riemann2048_fast_hbm_traffic.md),
 # each scaled by the counter's reading for k_copy_old (4 x 2048^2 doubles read
 # and written): fetch x1.882, write x0.985
@@ -79,9 +99,11 @@ def kernel_units(phase, n, gp=2, fused=False):
     return None, None
 
 
-def cpu_baseline(n, threads, sample_steps):
-    """Times the reference (or the oracle) on the host: `sample_steps` S2O4
-    steps of the same workload at n x n, in a child process."""
+def cpu_baseline(n, threads, sample_steps, warm_steps=1):
+    """Times the reference (or the oracle) on the host: `warm_steps` untimed
+    S2O4 steps (first touch of the ~21 GB of records at 2048^2), then
+    `sample_steps` timed steps of the same workload at n x n, in a child
+    process."""
     code = f"""
 import sys, time, json
 sys.path.insert(0, {os.path.join(ROOT, 'high-order-solver_amd')!r}); sys.path.insert(0, {os.path.join(ROOT, 'tests')!r})
@@ -94,10 +116,11 @@ p = planar_shock_params({n})
 prim = riemann_prim(p, riemann_states(0))
 s = lib.solver(p)
 s.set_prim(prim)
+rc, _ = s.run([0, 0, 0, 0], 0.25, {warm_steps}) if {warm_steps} else (0, None)
 t0 = time.perf_counter()
-rc, _ = s.run([0, 0, 0, 0], 0.25, {sample_steps})
+rc2, _ = s.run([0, 0, 0, 0], 0.25, {sample_steps})
 t1 = time.perf_counter()
-print(json.dumps(dict(kind=kind, seconds=t1 - t0, rc=rc)))
+print(json.dumps(dict(kind=kind, seconds=t1 - t0, rc=rc or rc2)))
 """
     env = dict(os.environ, OMP_NUM_THREADS=str(threads))
     out = subprocess.run([sys.executable, "-c", code], env=env, capture_output=True, text=True, timeout=900)
@@ -108,8 +131,8 @@ print(json.dumps(dict(kind=kind, seconds=t1 - t0, rc=rc)))
         return None
     v = n * n * sample_steps / r["seconds"] / 1e6
     return {"value": round(v, 4), "unit": "Mcell-updates/s", "cores": threads, "kind": r["kind"],
-            "sample": f"{sample_steps} S2O4 step(s) of the same workload at {n}x{n}, "
-                      f"{r['seconds']:.1f} s wall, OMP_NUM_THREADS={threads}"}
+            "sample": f"{sample_steps} timed S2O4 steps after {warm_steps} untimed of the same workload at {n}x{n} "
+                      f"from t=0, {r['seconds']:.1f} s wall, OMP_NUM_THREADS={threads}"}
 
 
 BATCH_SIDES = (128, 256, 384, 512, 768, 1024)
@@ -284,7 +307,7 @@ def bench_tgv3d(args):
     fpu = 4 * FLOPS3["point"] + 4 * FLOPS3["flux"] if dom == "flux" else FLOPS3.get(dom)
     if u and fpu:
         achieved = u * fpu / (avg_ms * 1e-3) / 1e12
-        roof = {"bound": "mfma", "achieved": round(achieved, 3), "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
+        roof = {"bound": "fp64-valu", "achieved": round(achieved, 3), "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
                 "frac": round(achieved / PEAK_FP64_TFLOPS, 4),
                 "traffic": TRAFFIC3_256_FAST.get(dom) if (n == 256 and args.variant == "fast") else None,
                 "kernel": "k3_pflux (x, y, z): point values + GKS3D, one launch" if dom == "flux" else f"k3_{dom} (x, y, z)",
@@ -313,45 +336,9 @@ def bench_tgv3d(args):
     print(json.dumps(out), flush=True)
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=30)
-    ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--n", type=int, default=2048)
-    ap.add_argument("--seed", type=int, default=0)
-    ap.add_argument("--variant", choices=["ieee", "fast"], default="fast")
-    ap.add_argument("--cpu-n", type=int, default=2048)
-    ap.add_argument("--cpu-steps", type=int, default=1)
-    ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--path", choices=["plain", "learned"], default="plain",
-                    help="learned: the trained troubled-interface classifier picks linear weights for smooth stencils")
-    ap.add_argument("--classifier", default=None, help="trained classifier (.npz); default: the packaged one")
-    ap.add_argument("--workload", choices=["single", "batch", "tgv3d"], default="single",
-                    help="batch: --batch uneven problems stepped together, against running them one by one; "
-                         "tgv3d: the 3D Taylor-Green cube at --n3d^3")
-    ap.add_argument("--n3d", type=int, default=256)
-    ap.add_argument("--cpu-n3d", type=int, default=48)
-    ap.add_argument("--batch", type=int, default=64)
-    ap.add_argument("--full-run", action="store_true",
-                    help="time the whole run to t=0.25 instead of a window of --steps steps (the learned path's "
-                         "cost depends on how much of the flow is smooth, which changes along the run)")
-    ap.add_argument("--equal-l1", action="store_true",
-                    help="also run plain and learned to t=0.25 and a plain (2n)^2 reference, and report density L1")
-    args = ap.parse_args()
-
-    world = int(os.environ.get("WORLD_SIZE", "1"))
-    if args.gpus > 1 or world > 1:
-        print("bench.py: this build runs on one GPU; multi-GPU runs are not part of it "
-              f"(asked for --gpus {args.gpus}, WORLD_SIZE {world})", file=sys.stderr)
-        sys.exit(3)
-    if args.workload == "batch":
-        bench_batch(args)
-        return
-    if args.workload == "tgv3d":
-        bench_tgv3d(args)
-        return
-
+def bench_window(args):
+    """The K-step window (--window): --steps timed steps after --warmup
+    untimed ones, from t=0, of one path (--path)."""
     import numpy as np  # noqa: F401
 
     import gks_amd
@@ -404,7 +391,7 @@ def main():
         achieved = units * fpu / (avg_ms * 1e-3) / 1e12
         traffic = (TRAFFIC_2048_FAST.get(dom) if (n == 2048 and args.variant == "fast" and fused and args.path == "plain")
                    else None)
-        roof = {"bound": "mfma", "achieved": round(achieved, 3), "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
+        roof = {"bound": "fp64-valu", "achieved": round(achieved, 3), "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
                 "frac": round(achieved / PEAK_FP64_TFLOPS, 4), "traffic": traffic,
                 "kernel": dom + (" (tangential reconstruction + flux, one launch)" if fused and dom.startswith("flux") else ""),
                 "avg_ms": round(avg_ms, 4), "flops_per_launch": units * fpu}
@@ -420,7 +407,8 @@ def main():
         learned_info = {"linear_share": round(lin / max(tot, 1), 4), "logit_threshold": learned[1]}
     step_flops = FLOPS_PER_CELL_UPDATE * n * n
     out = {
-        "metric": "Mcell-updates/s FP64, 2D 2048² WENO5-AO GKS: learned vs plain at equal L1",
+        "metric": f"Mcell-updates/s FP64, 2D {n}² WENO5-AO GKS, {args.path} path, "
+                  + ("whole run" if args.full_run else f"{k}-step window"),
         "value": round(value, 3),
         "unit": "Mcell-updates/s",
         "n_gpus": 1,
@@ -453,6 +441,279 @@ def main():
     else:
         out["cpu_baseline"] = None
     print(json.dumps(out), flush=True)
+
+
+def _leg(p, prim, bc, tstop, cls=None, phases=None, warmup=0, log=None, tag=""):
+    """One whole run from the initial condition to tstop on the
+    single-problem path (gks2d_run), with `warmup` untimed steps on a
+    separate solver first; the timed region is the run itself (state in
+    HBM, synchronised on both sides).  Returns steps, wall, per-phase
+    event times, classifier counters and the final interior state."""
+    import gks_amd
+    from gks_amd import learn
+    G = p.ghost
+    if warmup:
+        w = gks_amd.Solver2D(p)
+        w.init_prim(prim)
+        if cls:
+            learn.set_classifier(w, cls[0], cls[1])
+        w.run(bc, tstop, warmup)
+        w.sync()
+        w.close()
+    s = gks_amd.Solver2D(p)
+    s.init_prim(prim)
+    if cls:
+        learn.set_classifier(s, cls[0], cls[1])
+    if phases:
+        s.timing(True, phases=phases)
+    s.sync()
+    t0 = time.perf_counter()
+    k = s.run(bc, tstop)
+    s.sync()
+    wall = time.perf_counter() - t0
+    out = {"steps": k, "wall": wall, "t": s.t, "timing": s.get_timing() if phases else None,
+           "counts": learn.classifier_counts(s) if cls else None}
+    out["u"] = s.get_convar()[:, G:G + p.nodex, G:G + p.nodey]
+    s.close()
+    if log:
+        log(f"{tag}: {k} steps to t={out['t']:.4f} in {wall:.2f} s = {p.nodex * p.nodey * k / wall / 1e6:.1f} "
+            "Mcell-updates/s")
+    return out
+
+
+def _leg_batched(p, prim, bc, tstop, cls, warmup=0, log=None, tag=""):
+    """The same run through the batched executor (a batch of one problem:
+    the device-side time loop, one captured HIP graph per S2O4 step).  The
+    tile shape comes from the autotuner, run with the warm-up steps on a
+    separate batch; the timed batch starts from the initial condition."""
+    from gks_amd import learn
+    from gks_amd.batch import Batch
+    G = p.ghost
+    w = Batch([p])
+    w.problems[0].init_prim(prim)
+    learn.set_classifier(w.problems[0], cls[0], cls[1])
+    tf, lf, _ = w.autotune(bc, steps=2)
+    if warmup:
+        w.run(bc, tstop, warmup)
+    w.close()
+    b = Batch([p])
+    s = b.problems[0]
+    s.init_prim(prim)
+    learn.set_classifier(s, cls[0], cls[1])
+    b.set_shape(tf, lf)
+    s.sync()
+    t0 = time.perf_counter()
+    k = int(b.run(bc, tstop)[0])
+    s.sync()
+    wall = time.perf_counter() - t0
+    out = {"steps": k, "wall": wall, "t": s.t, "tanflux_faces": tf, "largest_first": lf,
+           "u": s.get_convar()[:, G:G + p.nodex, G:G + p.nodey]}
+    b.close()
+    if log:
+        log(f"{tag}: {k} steps to t={out['t']:.4f} in {wall:.2f} s = {p.nodex * p.nodey * k / wall / 1e6:.1f} "
+            "Mcell-updates/s")
+    return out
+
+
+def bench_headline(args):
+    """The metric: the four-quadrant Riemann problem at n^2 (seed args.seed)
+    run from the initial condition to t = 0.25 by the plain path, the
+    learned path and the learned path through the batched executor, each
+    timed over its whole run; density L1 of each against a plain FP64 run
+    at (2n)^2 averaged over 2 x 2 blocks.  `value` is the learned path's
+    Mcell-updates/s."""
+    import numpy as np
+
+    from gks_amd import learn
+    from gks_amd.params import BC_FREE, planar_shock_params
+    from gks_amd.problems import riemann_prim, riemann_states
+
+    def log(m):
+        print(m, file=sys.stderr, flush=True)
+
+    n, tstop = args.n, 0.25
+    fast = 1 if args.variant == "fast" else 0
+    p = planar_shock_params(n, fast_math=fast)
+    prim = riemann_prim(p, riemann_states(args.seed))
+    bc = [BC_FREE] * 4
+    P, thr, _ = learn.load(args.classifier or learn.DEFAULT_PARAMS)
+    legs = {}
+    # plain: HIP events on the flux launches (the dominant kernel, k_tanflux)
+    legs["plain"] = _leg(p, prim, bc, tstop, phases=["flux_x", "flux_y"], warmup=args.warmup, log=log, tag="plain")
+    # learned: HIP events on the launches over the listed cells and faces
+    legs["learned"] = _leg(p, prim, bc, tstop, cls=(P, thr),
+                           phases=["normal_list_x", "normal_list_y", "flux_list_x", "flux_list_y"],
+                           warmup=args.warmup, log=log, tag="learned")
+    if not args.no_batched:
+        legs["learned_batched"] = _leg_batched(p, prim, bc, tstop, (P, thr), warmup=args.warmup, log=log,
+                                               tag="learned+batched")
+    ref = None
+    if not args.no_l1:
+        pr = planar_shock_params(2 * n, fast_math=fast)
+        r = _leg(pr, riemann_prim(pr, riemann_states(args.seed)), bc, tstop, log=log, tag=f"plain {2 * n}^2")
+        ref = r["u"].reshape(4, n, 2, n, 2).mean(axis=(2, 4))
+        legs["reference_2n"] = r
+    res = {}
+    for tag, r in legs.items():
+        m = (2 * n if tag == "reference_2n" else n)
+        e = {"value": round(m * m * r["steps"] / r["wall"] / 1e6, 3), "steps": r["steps"],
+             "wall_s": round(r["wall"], 3), "ms_per_step": round(r["wall"] / r["steps"] * 1e3, 4),
+             "t_final": r["t"]}
+        if ref is not None and tag != "reference_2n":
+            e["l1"] = [float(x) for x in np.abs(r["u"] - ref).mean(axis=(1, 2))]
+        res[tag] = e
+    L = legs["learned"]
+    c = L["counts"]
+    nfaces = c["tangential_total"] // 2  # faces swept, x and y (one count per face side)
+    listed = c["listed_faces_x"] + c["listed_faces_y"]
+    res["learned"].update({
+        "linear_share_normal": round(c["normal_linear"] / max(c["normal_total"], 1), 4),
+        "linear_share_tangential": round(c["tangential_linear"] / max(c["tangential_total"], 1), 4),
+        "listed_face_share": round(listed / max(nfaces, 1), 4),
+        "logit_threshold": thr})
+    plain_v = res["plain"]["value"]
+    value = res["learned"]["value"]
+
+    # roofline of the plain path's dominant kernel: k_tanflux<1> (y faces:
+    # tangential WENO5-AO-DF of both sides + 2 flux points per face)
+    tim = legs["plain"]["timing"]
+    units, fpu = kernel_units("flux_y", n, fused=True)
+    avg_ms = tim["flux_y"][0] / max(tim["flux_y"][1], 1)
+    achieved = units * fpu / (avg_ms * 1e-3) / 1e12
+    exact = n == 2048 and args.variant == "fast"
+    roof = {"bound": "fp64-valu", "achieved": round(achieved, 3), "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
+            "frac": round(achieved / PEAK_FP64_TFLOPS, 4),
+            "traffic": TRAFFIC_2048_FAST["flux_y"] if exact else None,
+            "kernel": "k_tanflux<1> (plain leg: y faces, tangential reconstruction + 2 GKS2D flux points per face)",
+            "avg_ms": round(avg_ms, 4), "launches": tim["flux_y"][1], "flops_per_launch": units * fpu}
+    if exact:
+        roof["executed_flops_per_launch"] = EXECUTED_FLOPS_2048_FAST["flux_y"]
+        roof["executed_frac"] = round(EXECUTED_FLOPS_2048_FAST["flux_y"] / (avg_ms * 1e-3) / 1e12 / PEAK_FP64_TFLOPS,
+                                      4)
+    # the learned leg's full-work kernels, each priced by its own units:
+    # k_tanflux_list: per listed face 2 flux points + each side's tangential
+    # reconstruction (WENO5-AO-DF, or the linear weights); k_normal_list:
+    # per listed cell one WENO5-AO-DF normal reconstruction
+    lt = L["timing"]
+    cheap_faces = nfaces - listed
+    lin_sides_listed = c["tangential_linear"] - 2 * cheap_faces
+    weno_sides_listed = 2 * listed - lin_sides_listed
+    f_list = (listed * 2 * FLOPS_FLUX_POINT + weno_sides_listed * FLOPS_TANGENT_SIDE
+              + lin_sides_listed * FLOPS_TANGENT_SIDE_LINEAR)
+    ms_list = lt["flux_list_x"][0] + lt["flux_list_y"][0]
+    listed_cells = c["normal_total"] - c["normal_linear"]
+    ms_nlist = lt["normal_list_x"][0] + lt["normal_list_y"][0]
+    roof_learned = {
+        "k_tanflux_list": {"bound": "fp64-valu", "achieved": round(f_list / (ms_list * 1e-3) / 1e12, 3),
+                           "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
+                           "frac": round(f_list / (ms_list * 1e-3) / 1e12 / PEAK_FP64_TFLOPS, 4),
+                           "listed_faces": listed, "weno_sides": weno_sides_listed, "linear_sides": lin_sides_listed,
+                           "flops": f_list, "ms_total": round(ms_list, 3),
+                           "launches": lt["flux_list_x"][1] + lt["flux_list_y"][1]},
+        "k_normal_list": {"bound": "fp64-valu",
+                          "achieved": round(listed_cells * FLOPS_NORMAL_CELL / (ms_nlist * 1e-3) / 1e12, 3),
+                          "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
+                          "frac": round(listed_cells * FLOPS_NORMAL_CELL / (ms_nlist * 1e-3) / 1e12
+                                        / PEAK_FP64_TFLOPS, 4),
+                          "listed_cells": listed_cells, "flops": listed_cells * FLOPS_NORMAL_CELL,
+                          "ms_total": round(ms_nlist, 3),
+                          "launches": lt["normal_list_x"][1] + lt["normal_list_y"][1]},
+    }
+    out = {
+        "metric": "Mcell-updates/s FP64, 2D 2048\u00b2 WENO5-AO GKS: learned vs plain at equal L1",
+        "value": value,
+        "unit": "Mcell-updates/s",
+        "n_gpus": 1,
+        "steps": L["steps"],
+        "warmup": args.warmup,
+        "window": f"whole run from the initial condition to t={tstop} ({L['steps']} S2O4 steps of the learned path; "
+                  f"each leg also ran {args.warmup} untimed warm-up steps on a separate solver first)",
+        "ms_per_step": round(L["wall"] / L["steps"] * 1e3, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"four-quadrant Riemann {n}x{n} seed {args.seed} to t={tstop}, WENO5-AO-DF "
+                               f"characteristic, GKS2D gks2nd, S2O4, CFL 0.5, free BCs; learned path (packaged "
+                               f"troubled-interface classifier, BF16 MFMA inference) against the plain path "
+                               f"({args.variant} kernels)",
+                   "n": n, "path": "learned", "kernels": args.variant},
+        "plain": res["plain"],
+        "learned": res["learned"],
+        "learned_batched": res.get("learned_batched"),
+        "speedup_learned_vs_plain": round(value / plain_v, 4),
+        "speedup_learned_batched_vs_plain": (round(res["learned_batched"]["value"] / plain_v, 4)
+                                             if "learned_batched" in res else None),
+    }
+    if ref is not None:
+        out["l1_ratio_density"] = round(res["learned"]["l1"][0] / res["plain"]["l1"][0], 5)
+        if "learned_batched" in res:
+            out["l1_ratio_density_batched"] = round(res["learned_batched"]["l1"][0] / res["plain"]["l1"][0], 5)
+        out["equal_l1"] = bool(out["l1_ratio_density"] <= 1.05)
+        out["l1_reference"] = (f"plain FP64 run at {2 * n}^2 to t={tstop} ({res['reference_2n']['steps']} steps, "
+                               f"{res['reference_2n']['value']} Mcell-updates/s), averaged over 2x2 blocks; "
+                               "l1 = mean |difference| per conservative variable over the interior")
+    out["roofline"] = roof
+    out["roofline_learned"] = roof_learned
+    if not args.no_cpu_baseline:
+        threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or (os.cpu_count() or 1)
+        out["cpu_baseline"] = cpu_baseline(args.cpu_n, threads, args.cpu_steps, args.cpu_warm)
+    else:
+        out["cpu_baseline"] = None
+    print(json.dumps(out), flush=True)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=30,
+                    help="timed steps of the --window measurement (the default measurement times whole runs)")
+    ap.add_argument("--warmup", type=int, default=3, help="untimed warm-up steps before each timed leg")
+    ap.add_argument("--n", type=int, default=2048)
+    ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--variant", choices=["ieee", "fast"], default="fast")
+    ap.add_argument("--cpu-n", type=int, default=2048)
+    ap.add_argument("--cpu-steps", type=int, default=3)
+    ap.add_argument("--cpu-warm", type=int, default=1)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--path", choices=["plain", "learned"], default="plain",
+                    help="learned: the trained troubled-interface classifier picks linear weights for smooth stencils")
+    ap.add_argument("--classifier", default=None, help="trained classifier (.npz); default: the packaged one")
+    ap.add_argument("--workload", choices=["single", "batch", "tgv3d"], default="single",
+                    help="batch: --batch uneven problems stepped together, against running them one by one; "
+                         "tgv3d: the 3D Taylor-Green cube at --n3d^3")
+    ap.add_argument("--n3d", type=int, default=256)
+    ap.add_argument("--cpu-n3d", type=int, default=48)
+    ap.add_argument("--batch", type=int, default=64)
+    ap.add_argument("--window", action="store_true",
+                    help="time --steps steps of one --path after --warmup steps from t=0, instead of the whole-run "
+                         "learned-versus-plain measurement")
+    ap.add_argument("--no-l1", action="store_true", help="skip the (2n)^2 reference run and the L1 comparison")
+    ap.add_argument("--no-batched", action="store_true", help="skip the learned+batched leg")
+    ap.add_argument("--full-run", action="store_true",
+                    help="time the whole run to t=0.25 instead of a window of --steps steps (the learned path's "
+                         "cost depends on how much of the flow is smooth, which changes along the run)")
+    ap.add_argument("--equal-l1", action="store_true",
+                    help="also run plain and learned to t=0.25 and a plain (2n)^2 reference, and report density L1")
+    args = ap.parse_args()
+
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus > 1 or world > 1:
+        print("bench.py: this build runs on one GPU; multi-GPU runs are not part of it "
+              f"(asked for --gpus {args.gpus}, WORLD_SIZE {world})", file=sys.stderr)
+        sys.exit(3)
+    if args.workload == "batch":
+        bench_batch(args)
+        return
+    if args.workload == "tgv3d":
+        bench_tgv3d(args)
+        return
+    if args.window:
+        bench_window(args)
+        return
+    bench_headline(args)
+
 
 
 if __name__ == "__main__":

@@ -336,15 +336,20 @@ int gks2d_primvar(gks2d_ctx* c) {
 int gks2d_reconstruct(gks2d_ctx* c) {
   if (!c) return fail(GKS_ERR_ARG, "null");
   RC(timed(c, PH_NORMAL_X, [&] { return c->lt->normal(c->d, 0, c->s); }));
+  if (c->d.mlp) RC(timed(c, PH_NORMAL_LIST_X, [&] { return c->lt->normal_list(c->d, 0, c->s); }));
   RC(timed(c, PH_NORMAL_Y, [&] { return c->lt->normal(c->d, 1, c->s); }));
+  if (c->d.mlp) RC(timed(c, PH_NORMAL_LIST_Y, [&] { return c->lt->normal_list(c->d, 1, c->s); }));
   RC(timed(c, PH_TANGENT_X, [&] { return c->lt->tangent(c->d, 0, c->s); }));
   return timed(c, PH_TANGENT_Y, [&] { return c->lt->tangent(c->d, 1, c->s); });
 }
 int gks2d_flux(gks2d_ctx* c, int stage, double dt) {
   if (!c) return fail(GKS_ERR_ARG, "null");
   if (stage < 0 || stage > 1) return fail(GKS_ERR_ARG, "stage");
+  const bool listed = c->d.mlp && c->d.smooth_flux;
   RC(timed(c, PH_FLUX_X, [&] { return c->lt->flux(c->d, 0, stage, dt, c->s); }));
+  if (listed) RC(timed(c, PH_FLUX_LIST_X, [&] { return c->lt->flux_list(c->d, 0, stage, dt, c->s); }));
   RC(timed(c, PH_FLUX_Y, [&] { return c->lt->flux(c->d, 1, stage, dt, c->s); }));
+  if (listed) RC(timed(c, PH_FLUX_LIST_Y, [&] { return c->lt->flux_list(c->d, 1, stage, dt, c->s); }));
   c->last_flux_stage = stage;
   if (c->d.keep_face_fluxes) c->kept_stage_mask |= 1 << stage;
   return GKS_OK;
@@ -560,8 +565,10 @@ int gks2d_flux_points_smooth(const gks2d_params* p, int64_t n, const double* lef
 }
 
 // ------------------------------------------------- learned classifier hooks
-// 2 kernels (normal, tangential) x 1024 slots x (linear, total) counters
-constexpr size_t kLstatWords = 2 * 1024 * 2;
+// 2 kernels (normal, tangential) x 1024 slots x (linear, total) counters,
+// then the listed flux faces of x and y sweeps
+constexpr size_t kLstatPairs = 2 * 1024 * 2;
+constexpr size_t kLstatWords = kLstatPairs + 2;
 int gks2d_classifier_samples(gks2d_ctx* c, int32_t dir, int32_t stride, double label_tol, float* features,
                              float* labels, int64_t cap, int64_t* n_out) {
   if (!c || !features || !labels || !n_out) return fail(GKS_ERR_ARG, "null");
@@ -645,12 +652,29 @@ int gks2d_classifier_stats(gks2d_ctx* c, int64_t* linear, int64_t* total) {
   HIPCHK(hipStreamSynchronize(c->s));
   HIPCHK(hipMemcpy(h.data(), c->d_lstat, h.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost));
   unsigned long long lin = 0, tot = 0;
-  for (size_t k = 0; k < h.size(); k += 2) {
+  for (size_t k = 0; k < kLstatPairs; k += 2) {
     lin += h[k];
     tot += h[k + 1];
   }
   *linear = (int64_t)lin;
   *total = (int64_t)tot;
+  return GKS_OK;
+}
+
+int gks2d_classifier_counts(gks2d_ctx* c, int64_t counts[6]) {
+  if (!c || !counts) return fail(GKS_ERR_ARG, "null");
+  if (!c->d_lstat) return fail(GKS_ERR_STATE, "no classifier set");
+  std::vector<unsigned long long> h(kLstatWords);
+  HIPCHK(hipStreamSynchronize(c->s));
+  HIPCHK(hipMemcpy(h.data(), c->d_lstat, h.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost));
+  for (int k = 0; k < 6; k++) counts[k] = 0;
+  for (size_t k = 0; k < kLstatPairs; k += 2) {
+    const int part = k < kLstatPairs / 2 ? 0 : 2;  // normal, tangential
+    counts[part] += (int64_t)h[k];
+    counts[part + 1] += (int64_t)h[k + 1];
+  }
+  counts[4] = (int64_t)h[kLstatPairs];
+  counts[5] = (int64_t)h[kLstatPairs + 1];
   return GKS_OK;
 }
 
@@ -668,6 +692,7 @@ struct gks2d_batch {
   BState* d_st = nullptr;
   unsigned* d_start = nullptr;
   int* d_pid = nullptr;
+  unsigned* d_lstart = nullptr;  // list tile prefixes (BatchArgs::lstart)
   BState* h_st = nullptr;  // pinned
   BatchShape shape{128, 0};
   int largest_first = 1;
@@ -772,7 +797,7 @@ int batch_graph(gks2d_batch* b, int learned) {
   drop_graph(b);
   RC(batch_tables(b));
   b->shape.learned = learned;
-  const BatchArgs a{b->d_dev, b->d_st, b->d_start, b->d_pid, b->np};
+  const BatchArgs a{b->d_dev, b->d_st, b->d_start, b->d_pid, b->np, b->d_lstart};
   const LaunchTable* lt = b->ctx[0]->lt;
   HIPCHK(hipStreamBeginCapture(b->s, hipStreamCaptureModeThreadLocal));
   hipError_t e = lt->batch_step(a, b->shape, b->tiles.data(), b->s);
@@ -808,6 +833,7 @@ int gks2d_batch_destroy(gks2d_batch* b) {
   if (b->d_st) (void)hipFree(b->d_st);
   if (b->d_start) (void)hipFree(b->d_start);
   if (b->d_pid) (void)hipFree(b->d_pid);
+  if (b->d_lstart) (void)hipFree(b->d_lstart);
   if (b->h_st) (void)hipHostFree(b->h_st);
   if (b->e0) (void)hipEventDestroy(b->e0);
   if (b->e1) (void)hipEventDestroy(b->e1);
@@ -847,6 +873,7 @@ int gks2d_batch_create(const gks2d_params* params, int32_t n, gks2d_batch** out)
   if (e == hipSuccess) e = hipMalloc(&b->d_st, n * sizeof(BState));
   if (e == hipSuccess) e = hipMalloc(&b->d_start, (size_t)KB_COUNT * (n + 1) * sizeof(unsigned));
   if (e == hipSuccess) e = hipMalloc(&b->d_pid, n * sizeof(int));
+  if (e == hipSuccess) e = hipMalloc(&b->d_lstart, (size_t)KL_COUNT * (n + 1) * sizeof(unsigned));
   if (e == hipSuccess) e = hipHostMalloc((void**)&b->h_st, n * sizeof(BState));
   if (e != hipSuccess) {
     gks2d_batch_destroy(b);

@@ -449,6 +449,8 @@ __global__ void __launch_bounds__(kBlock) k_label(Dev d, int stride, double tol,
 // active lane to one of kStatSlots slot pairs (spread by block so that the
 // adds do not queue on one address); the host sums the slots
 constexpr int kStatSlots = 1024;
+// lstat word that accumulates the x (+0) / y (+1) flux faces the lists held
+constexpr int kListedWord = 4 * kStatSlots;
 __device__ __forceinline__ void count_linear(unsigned long long* stat, bool lin) {
   const unsigned long long act = __ballot(1), bl = __ballot(lin);
   if ((threadIdx.x & 63) == __ffsll((long long)act) - 1) {
@@ -858,14 +860,16 @@ __device__ __forceinline__ void flux_point_tail(const Dev& d, int q, long long f
   if (smooth) gks_point_smooth(fo, dfo, L, R, dt, d.gas);
   else gks_point(fo, dfo, nullptr, L, R, dt, d.gas);
   if (d.keep_face_fluxes) {
-    double* fl = DIR == 0 ? d.flx[stage] : d.fly[stage];
+    double* fl = DIR == 0 ? (stage == 0 ? d.flx[0] : d.flx[1]) : (stage == 0 ? d.fly[0] : d.fly[1]);
 #pragma unroll
     for (int k = 0; k < 4; k++) {
       fl[(q * 8 + k) * nf + f] = fo[k];
       fl[(q * 8 + 4 + k) * nf + f] = dfo[k];
     }
   }
-  const double w = d.gw[q];
+  // (selects, not d.gw[q]: a runtime index into the descriptor would force
+  // a batched kernel's copy of it into scratch)
+  const double w = q == 0 ? d.gw[0] : q == 1 ? d.gw[1] : q == 2 ? d.gw[2] : d.gw[3];
   double x[4];
   double* pc = DIR == 0 ? d.pcx : d.pcy;
   if (stage == 0) {
@@ -1152,6 +1156,7 @@ template <int DIR>
 __global__ void __launch_bounds__(2 * kTanFlux) __attribute__((amdgpu_waves_per_eu(GKS_FLUX_WAVES)))
 k_tanflux_list(Dev d, int stage, double dt) {
   const long long n = d.tcount[DIR];
+  if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(d.lstat + kListedWord + DIR, (unsigned long long)n);
   if ((long long)blockIdx.x * kTanFlux < n)
     tanflux_body<DIR, true, kTanFlux, true>(d, blockIdx.x, stage, dt, d.tlist[DIR], n);
 }
@@ -1258,12 +1263,15 @@ __device__ __forceinline__ int batch_slot(const BatchArgs& B, int kind, unsigned
   }
   return lo;
 }
+// The problem's descriptor is copied out of global memory once: read
+// through a reference, every field would be reloaded after each store the
+// body makes (the compiler cannot rule out that a store aliases it).
 #define GKS_BATCH_TILE(kind)                                    \
   const unsigned gb_ = xcd_block();                             \
   const int q_ = batch_slot(B, kind, gb_);                      \
   const int p_ = B.pid[q_];                                     \
   if (!B.st[p_].active) return;                                 \
-  const Dev& d = B.dev[p_];                                     \
+  const Dev d = B.dev[p_];                                      \
   const unsigned blk = gb_ - B.start[(kind) * (B.np + 1) + q_]; \
   (void)d;                                                      \
   (void)blk
@@ -1294,13 +1302,47 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(GKS
   if constexpr (LEARNED) normal_learned_body<DIR>(d, blk);
   else normal_body<DIR>(d, blk);
 }
-// the cells each problem's learned normal sweep listed (as k_normal_list),
-// one list tile per workgroup of that problem's normal tiles
+// Learned path, work lists: after the sweep that fills every problem's
+// list of one kind, kb_list_prefix packs the lists' tiles at the front of
+// the next launch (problem q's tiles start at lstart[q]); the list kernel
+// is launched with as many workgroups as the sweep had (a bound), and the
+// ones past lstart[np] return after one load.  Runs on one lane: np is a
+// batch's problem count (tens).
+__global__ void __launch_bounds__(64) kb_list_prefix(BatchArgs B, int kind, int per_tile) {
+  if (threadIdx.x != 0) return;
+  unsigned* ls = B.lstart + kind * (B.np + 1);
+  unsigned acc = 0;
+  for (int q = 0; q < B.np; q++) {
+    ls[q] = acc;
+    const int p = B.pid[q];
+    const unsigned* cnt = kind < KL_FLUX_X ? B.dev[p].ncount : B.dev[p].tcount;
+    const unsigned n = (B.st[p].active && cnt) ? cnt[kind & 1] : 0u;
+    acc += (n + per_tile - 1) / per_tile;
+  }
+  ls[B.np] = acc;
+}
+#define GKS_BATCH_LIST_TILE(kind)                           \
+  const unsigned* ls_ = B.lstart + (kind) * (B.np + 1);     \
+  const unsigned gb_ = blockIdx.x;                          \
+  if (gb_ >= ls_[B.np]) return;                             \
+  int q_ = 0;                                               \
+  {                                                         \
+    int hi_ = B.np;                                         \
+    while (hi_ - q_ > 1) {                                  \
+      const int mid_ = (q_ + hi_) >> 1;                     \
+      if (ls_[mid_] <= gb_) q_ = mid_;                      \
+      else hi_ = mid_;                                      \
+    }                                                       \
+  }                                                         \
+  const int p_ = B.pid[q_];                                 \
+  const Dev d = B.dev[p_];                                  \
+  const unsigned blk = gb_ - ls_[q_]
+// the cells each problem's learned normal sweep listed (as k_normal_list)
 template <int DIR>
 __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(GKS_NORMAL_WAVES)))
 kb_normal_list(BatchArgs B) {
-  GKS_BATCH_TILE(DIR == 0 ? KB_NORMAL_X : KB_NORMAL_Y);
-  if (d.ncount && (long long)blk * kBlock < (long long)d.ncount[DIR]) normal_list_body<DIR>(d, blk);
+  GKS_BATCH_LIST_TILE(DIR == 0 ? KL_NORMAL_X : KL_NORMAL_Y);
+  normal_list_body<DIR>(d, blk);
 }
 template <int DIR, bool LEARNED, int TF>
 __global__ void __launch_bounds__(2 * TF) __attribute__((amdgpu_waves_per_eu(GKS_FLUX_WAVES)))
@@ -1319,9 +1361,10 @@ __global__ void __launch_bounds__(2 * TF) kb_tanflux_cheap(BatchArgs B, int stag
 template <int DIR, int TF>
 __global__ void __launch_bounds__(2 * TF) __attribute__((amdgpu_waves_per_eu(GKS_FLUX_WAVES)))
 kb_tanflux_list(BatchArgs B, int stage) {
-  GKS_BATCH_TILE(DIR == 0 ? KB_FLUX_X : KB_FLUX_Y);
+  GKS_BATCH_LIST_TILE(DIR == 0 ? KL_FLUX_X : KL_FLUX_Y);
   const long long n = d.tcount[DIR];
-  if ((long long)blk * TF < n) tanflux_body<DIR, true, TF, true>(d, blk, stage, B.st[p_].dt, d.tlist[DIR], n);
+  if (blk == 0 && threadIdx.x == 0) atomicAdd(d.lstat + kListedWord + DIR, (unsigned long long)n);
+  tanflux_body<DIR, true, TF, true>(d, blk, stage, B.st[p_].dt, d.tlist[DIR], n);
 }
 // clears the four list counts (flux x, y; normal x, y) of every active
 // problem (one lane per problem), before the stage's normal sweeps
@@ -1481,13 +1524,10 @@ static hipError_t normal(const Dev& d, int dir, hipStream_t s) {
     if (dir == 0) hipLaunchKernelGGL((k_normal79<0, 9>), dim3(nb), dim3(kBlock), 0, s, d);
     else hipLaunchKernelGGL((k_normal79<1, 9>), dim3(nb), dim3(kBlock), 0, s, d);
   } else if (d.mlp) {
-    if (d.ncount) hipLaunchKernelGGL(k_clear_count, dim3(1), dim3(64), 0, s, d.ncount + dir);
+    // the listed cells follow in normal_list
+    hipLaunchKernelGGL(k_clear_count, dim3(1), dim3(64), 0, s, d.ncount + dir);
     if (dir == 0) hipLaunchKernelGGL(k_normal_learned<0>, dim3(nb), dim3(kBlock), 0, s, d);
     else hipLaunchKernelGGL(k_normal_learned<1>, dim3(nb), dim3(kBlock), 0, s, d);
-    if (d.ncount) {
-      if (dir == 0) hipLaunchKernelGGL(k_normal_list<0>, dim3(nb), dim3(kBlock), 0, s, d);
-      else hipLaunchKernelGGL(k_normal_list<1>, dim3(nb), dim3(kBlock), 0, s, d);
-    }
   } else {
     if (dir == 0) hipLaunchKernelGGL(k_normal<0>, dim3(nb), dim3(kBlock), 0, s, d);
     else hipLaunchKernelGGL(k_normal<1>, dim3(nb), dim3(kBlock), 0, s, d);
@@ -1514,14 +1554,10 @@ static hipError_t flux(const Dev& d, int dir, int stage, double dt, hipStream_t 
     const unsigned nb = (unsigned)((nfaces + kTanFlux - 1) / kTanFlux);
     if (d.mlp && d.smooth_flux && d.tcount) {
       // smooth faces, then the listed rest (at most nfaces of them)
+      // (the listed faces follow in flux_list)
       hipLaunchKernelGGL(k_clear_count, dim3(1), dim3(64), 0, s, d.tcount + dir);
-      if (dir == 0) {
-        hipLaunchKernelGGL(k_tanflux_cheap<0>, dim3(nb), dim3(2 * kTanFlux), 0, s, d, stage, dt);
-        hipLaunchKernelGGL(k_tanflux_list<0>, dim3(nb), dim3(2 * kTanFlux), 0, s, d, stage, dt);
-      } else {
-        hipLaunchKernelGGL(k_tanflux_cheap<1>, dim3(nb), dim3(2 * kTanFlux), 0, s, d, stage, dt);
-        hipLaunchKernelGGL(k_tanflux_list<1>, dim3(nb), dim3(2 * kTanFlux), 0, s, d, stage, dt);
-      }
+      if (dir == 0) hipLaunchKernelGGL(k_tanflux_cheap<0>, dim3(nb), dim3(2 * kTanFlux), 0, s, d, stage, dt);
+      else hipLaunchKernelGGL(k_tanflux_cheap<1>, dim3(nb), dim3(2 * kTanFlux), 0, s, d, stage, dt);
     } else if (d.mlp) {
       if (dir == 0) hipLaunchKernelGGL((k_tanflux<0, true>), dim3(nb), dim3(2 * kTanFlux), 0, s, d, stage, dt);
       else hipLaunchKernelGGL((k_tanflux<1, true>), dim3(nb), dim3(2 * kTanFlux), 0, s, d, stage, dt);
@@ -1537,6 +1573,21 @@ static hipError_t flux(const Dev& d, int dir, int stage, double dt, hipStream_t 
   else
     hipLaunchKernelGGL(k_flux<1>, dim3(blocks_for((long long)d.gp * d.NX * (d.NY + 1))), dim3(kBlock), 0, s, d,
                        stage, dt);
+  return hipGetLastError();
+}
+static hipError_t normal_list(const Dev& d, int dir, hipStream_t s) {
+  if (d.order != 5 || !d.mlp || !d.ncount) return hipSuccess;
+  const unsigned nb = dir == 0 ? blocks_for((long long)(d.NX + 2) * d.ny) : blocks_for((long long)d.nx * (d.NY + 2));
+  if (dir == 0) hipLaunchKernelGGL(k_normal_list<0>, dim3(nb), dim3(kBlock), 0, s, d);
+  else hipLaunchKernelGGL(k_normal_list<1>, dim3(nb), dim3(kBlock), 0, s, d);
+  return hipGetLastError();
+}
+static hipError_t flux_list(const Dev& d, int dir, int stage, double dt, hipStream_t s) {
+  if (d.order != 5 || !d.fuse_tanflux || !d.mlp || !d.smooth_flux || !d.tcount) return hipSuccess;
+  const long long nfaces = dir == 0 ? (long long)(d.NX + 1) * d.NY : (long long)d.NX * (d.NY + 1);
+  const unsigned nb = (unsigned)((nfaces + kTanFlux - 1) / kTanFlux);
+  if (dir == 0) hipLaunchKernelGGL(k_tanflux_list<0>, dim3(nb), dim3(2 * kTanFlux), 0, s, d, stage, dt);
+  else hipLaunchKernelGGL(k_tanflux_list<1>, dim3(nb), dim3(2 * kTanFlux), 0, s, d, stage, dt);
   return hipGetLastError();
 }
 static hipError_t update(const Dev& d, int stage, hipStream_t s) {
@@ -1576,13 +1627,21 @@ static void batch_stage(const BatchArgs& B, const unsigned* tiles, int stage, hi
   hipLaunchKernelGGL(kb_reduce, dim3(B.np), dim3(kBlock), 0, s, B, stage);
   if (LEARNED) hipLaunchKernelGGL(kb_clear_counts, dim3(1), dim3(64), 0, s, B);
   hipLaunchKernelGGL((kb_normal<0, LEARNED>), dim3(tiles[KB_NORMAL_X]), dim3(kBlock), 0, s, B);
-  if (LEARNED) hipLaunchKernelGGL(kb_normal_list<0>, dim3(tiles[KB_NORMAL_X]), dim3(kBlock), 0, s, B);
+  if (LEARNED) {
+    hipLaunchKernelGGL(kb_list_prefix, dim3(1), dim3(64), 0, s, B, (int)KL_NORMAL_X, kBlock);
+    hipLaunchKernelGGL(kb_normal_list<0>, dim3(tiles[KB_NORMAL_X]), dim3(kBlock), 0, s, B);
+  }
   hipLaunchKernelGGL((kb_normal<1, LEARNED>), dim3(tiles[KB_NORMAL_Y]), dim3(kBlock), 0, s, B);
-  if (LEARNED) hipLaunchKernelGGL(kb_normal_list<1>, dim3(tiles[KB_NORMAL_Y]), dim3(kBlock), 0, s, B);
+  if (LEARNED) {
+    hipLaunchKernelGGL(kb_list_prefix, dim3(1), dim3(64), 0, s, B, (int)KL_NORMAL_Y, kBlock);
+    hipLaunchKernelGGL(kb_normal_list<1>, dim3(tiles[KB_NORMAL_Y]), dim3(kBlock), 0, s, B);
+  }
   if (LEARNED) {
     hipLaunchKernelGGL((kb_tanflux_cheap<0, TF>), dim3(tiles[KB_FLUX_X]), dim3(2 * TF), 0, s, B, stage);
+    hipLaunchKernelGGL(kb_list_prefix, dim3(1), dim3(64), 0, s, B, (int)KL_FLUX_X, TF);
     hipLaunchKernelGGL((kb_tanflux_list<0, TF>), dim3(tiles[KB_FLUX_X]), dim3(2 * TF), 0, s, B, stage);
     hipLaunchKernelGGL((kb_tanflux_cheap<1, TF>), dim3(tiles[KB_FLUX_Y]), dim3(2 * TF), 0, s, B, stage);
+    hipLaunchKernelGGL(kb_list_prefix, dim3(1), dim3(64), 0, s, B, (int)KL_FLUX_Y, TF);
     hipLaunchKernelGGL((kb_tanflux_list<1, TF>), dim3(tiles[KB_FLUX_Y]), dim3(2 * TF), 0, s, B, stage);
   } else {
     hipLaunchKernelGGL((kb_tanflux<0, LEARNED, TF>), dim3(tiles[KB_FLUX_X]), dim3(2 * TF), 0, s, B, stage);
@@ -1612,8 +1671,8 @@ static hipError_t batch_step(const BatchArgs& B, const BatchShape& sh, const uns
   return hipGetLastError();
 }
 
-const LaunchTable table = {bc, primvar, copy_old, normal, tangent, flux, update, alpha, flux_points, label, infer,
-                          batch_step};
+const LaunchTable table = {bc,     primvar,     copy_old, normal, tangent, flux, normal_list, flux_list,
+                          update, alpha,       flux_points, label,  infer,   batch_step};
 #endif
 
 }  // namespace GKS_VARIANT

@@ -34,7 +34,8 @@ struct Dev {
   const float* mlp;             // classifier parameters (gks_learn.h), null: plain WENO5-AO
   const unsigned short* mlpw;   // its weights in the MFMA operand layout (mlp_pack.hpp)
   float mlp_thr;                // logit below which a smooth stencil takes the linear weights
-  unsigned long long* lstat;    // 2 x 1024 (linear, total) slot pairs: normal, then tangential
+  unsigned long long* lstat;    // 2 x 1024 (linear, total) slot pairs: normal, then tangential; then the
+                                // faces the flux lists held (x, y)
   unsigned char* lin[2];        // per cell: its x / y normal stencil took the linear weights
   int smooth_flux;              // learned path: faces with two linear sides take the smooth-flow flux
   unsigned* tlist[2];           // learned path: x / y faces left to the full flux (interior face ordinals)
@@ -78,7 +79,12 @@ struct BatchArgs {
   const unsigned* start;
   const int* pid;
   int np;
+  // learned path: 4 x (np + 1) tile prefixes of the work lists (normal x,
+  // normal y, flux x, flux y), in launch order, filled on the device after
+  // the sweep that builds each list
+  unsigned* lstart;
 };
+enum BatchList { KL_NORMAL_X = 0, KL_NORMAL_Y, KL_FLUX_X, KL_FLUX_Y, KL_COUNT };
 // tile shapes of a batch (chosen by gks2d_batch_autotune)
 struct BatchShape {
   int tanflux_faces;  // faces per flux workgroup: 64 or 128
@@ -88,7 +94,8 @@ struct BatchShape {
 // device phases timed by gks2d_timing (order of GKS_PHASE_* in gks2d.h)
 enum Phase {
   PH_BC = 0, PH_PRIMVAR, PH_NORMAL_X, PH_NORMAL_Y, PH_TANGENT_X, PH_TANGENT_Y,
-  PH_FLUX_X, PH_FLUX_Y, PH_UPDATE, PH_ALPHA, PH_COPY_OLD, PH_COUNT
+  PH_FLUX_X, PH_FLUX_Y, PH_UPDATE, PH_ALPHA, PH_COPY_OLD,
+  PH_NORMAL_LIST_X, PH_NORMAL_LIST_Y, PH_FLUX_LIST_X, PH_FLUX_LIST_Y, PH_COUNT
 };
 
 struct LaunchTable {
@@ -98,6 +105,10 @@ struct LaunchTable {
   hipError_t (*normal)(const Dev&, int dir, hipStream_t);
   hipError_t (*tangent)(const Dev&, int dir, hipStream_t);
   hipError_t (*flux)(const Dev&, int dir, int stage, double dt, hipStream_t);
+  // learned path: the listed normal cells (after normal) and flux faces
+  // (after flux) of direction dir; nothing otherwise
+  hipError_t (*normal_list)(const Dev&, int dir, hipStream_t);
+  hipError_t (*flux_list)(const Dev&, int dir, int stage, double dt, hipStream_t);
   hipError_t (*update)(const Dev&, int stage, hipStream_t);
   hipError_t (*alpha)(const Dev&, hipStream_t);
   hipError_t (*flux_points)(const gksd::Gas&, long long n, const double* L, const double* R, double dt,

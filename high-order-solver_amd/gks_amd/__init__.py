@@ -50,7 +50,7 @@ _SIGS = {
 }
 
 PHASES = ["bc", "primvar", "normal_x", "normal_y", "tangent_x", "tangent_y", "flux_x", "flux_y", "update",
-          "alpha", "copy_old"]
+          "alpha", "copy_old", "normal_list_x", "normal_list_y", "flux_list_x", "flux_list_y"]
 
 _lib = None
 

@@ -41,6 +41,7 @@ _SIGS = {
     "gks_mlp_infer_bf16": (ctypes.c_int, [_F, _I64, _F, _F]),
     "gks2d_set_classifier": (ctypes.c_int, [_P, _F, ctypes.c_float]),
     "gks2d_classifier_stats": (ctypes.c_int, [_P, ctypes.POINTER(_I64), ctypes.POINTER(_I64)]),
+    "gks2d_classifier_counts": (ctypes.c_int, [_P, ctypes.POINTER(_I64)]),
     "gks_mlp_last_error": (ctypes.c_char_p, []),
     "gks2d_classifier_options": (ctypes.c_int, [_P, ctypes.c_int32]),
     "gks2d_flux_points_smooth": (ctypes.c_int, [ctypes.c_void_p, _I64, _D, _D, ctypes.c_double, _D, _D]),
@@ -211,6 +212,17 @@ def classifier_stats(solver):
     a, b = _I64(), _I64()
     _check_ctx(L().gks2d_classifier_stats(solver.h, ctypes.byref(a), ctypes.byref(b)))
     return a.value, b.value
+
+
+def classifier_counts(solver):
+    """Counters since the classifier was set (gks2d_classifier_counts):
+    normal linear / total, tangential linear / total (per face side), listed
+    x / y flux faces."""
+    c = (_I64 * 6)()
+    _check_ctx(L().gks2d_classifier_counts(solver.h, c))
+    keys = ("normal_linear", "normal_total", "tangential_linear", "tangential_total", "listed_faces_x",
+            "listed_faces_y")
+    return {k: int(v) for k, v in zip(keys, c)}
 
 
 def _train_main(args):

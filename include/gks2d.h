@@ -136,7 +136,11 @@ int gks2d_sync(gks2d_ctx* ctx);
 enum {
   GKS_PHASE_BC = 0, GKS_PHASE_PRIMVAR, GKS_PHASE_NORMAL_X, GKS_PHASE_NORMAL_Y,
   GKS_PHASE_TANGENT_X, GKS_PHASE_TANGENT_Y, GKS_PHASE_FLUX_X, GKS_PHASE_FLUX_Y,
-  GKS_PHASE_UPDATE, GKS_PHASE_ALPHA, GKS_PHASE_COPY_OLD, GKS_NPHASE
+  GKS_PHASE_UPDATE, GKS_PHASE_ALPHA, GKS_PHASE_COPY_OLD,
+  /* learned path (gks_learn.h): the second launch of a normal / flux sweep,
+   * over the cells / faces the first one listed for full WENO5-AO-DF */
+  GKS_PHASE_NORMAL_LIST_X, GKS_PHASE_NORMAL_LIST_Y, GKS_PHASE_FLUX_LIST_X, GKS_PHASE_FLUX_LIST_Y,
+  GKS_NPHASE
 };
 int gks2d_timing(gks2d_ctx* ctx, int enable);
 /* Times only the phases whose bit (1 << GKS_PHASE_x) is set in mask (0: off;

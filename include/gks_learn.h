@@ -82,6 +82,13 @@ int gks2d_flux_points_smooth(const gks2d_params* p, int64_t n, const double* lef
  * the calls since the classifier was set (normal and tangential). */
 int gks2d_classifier_stats(gks2d_ctx* ctx, int64_t* linear, int64_t* total);
 
+/* Counters since the classifier was set: counts[0] / [1] normal
+ * reconstructions that took the linear weights / all of them; [2] / [3]
+ * the same for tangential stencils (one per face side); [4] / [5] the x /
+ * y flux faces that went to the full-flux list (the others took the
+ * smooth-flow flux). */
+int gks2d_classifier_counts(gks2d_ctx* ctx, int64_t counts[6]);
+
 /* message of the last failed gks_mlp_* call on this thread (the gks2d_*
  * functions above report through gks2d_last_error) */
 const char* gks_mlp_last_error(void);
