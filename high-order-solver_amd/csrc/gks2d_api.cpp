@@ -206,6 +206,7 @@ int gks2d_create(const gks2d_params* p, gks2d_ctx** out) {
   d.order = p->weno_order;
   // WENO5: the tangential reconstruction runs inside the flux launch (its
   // Gauss-point states stay in LDS); GKS_TANFLUX=0 keeps the two launches
+  d.df_frozen = p->df_frozen ? 1 : 0;
   d.fuse_tanflux = p->weno_order == 5 && !(std::getenv("GKS_TANFLUX") && std::getenv("GKS_TANFLUX")[0] == '0');
   if (d.gp == 2) {  // SetGuassPoint, fluid_mesh.cpp:9-45
     d.gw[0] = 0.5;
@@ -432,7 +433,7 @@ static int stage_launch(gks2d_ctx* c, int stage, double dt, const int32_t* bc, b
   RC(gks2d_reconstruct(c));
   RC(gks2d_flux(c, stage, dt));
   RC(gks2d_update(c, stage));
-  if (c->p.use_df) RC(gks2d_update_alpha(c));
+  if (c->p.use_df && !c->p.df_frozen) RC(gks2d_update_alpha(c));
   return GKS_OK;
 }
 
@@ -700,6 +701,12 @@ struct gks2d_batch {
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   int graph_key = -1;  // shape the graph was captured for
+  // problems the tile tables (and so the captured step) cover: every
+  // problem when a run starts; a problem that finishes is dropped at the
+  // next host check, so that the launches stop carrying its tiles
+  std::vector<char> in_tables;
+  int np_tab = 0;
+  bool tables_dirty = true;
 };
 
 namespace {
@@ -716,9 +723,11 @@ void drop_graph(gks2d_batch* b) {
 
 // tile prefix tables of every work class, in launch order
 int batch_tables(gks2d_batch* b) {
-  const int np = b->np;
-  std::vector<int> pid(np);
-  for (int p = 0; p < np; p++) pid[p] = p;
+  std::vector<int> pid;
+  for (int p = 0; p < b->np; p++)
+    if (b->in_tables[p]) pid.push_back(p);
+  const int np = (int)pid.size();
+  b->np_tab = np;
   if (b->largest_first)
     std::stable_sort(pid.begin(), pid.end(), [&](int a, int c) { return b->ctx[a]->d.nc > b->ctx[c]->d.nc; });
   std::vector<unsigned> start((size_t)KB_COUNT * (np + 1));
@@ -793,11 +802,12 @@ int batch_init_state(gks2d_batch* b, const int32_t* bc, const double* tstop, int
 // the captured step for the current shape
 int batch_graph(gks2d_batch* b, int learned) {
   const int key = b->shape.tanflux_faces * 4 + b->largest_first * 2 + learned;
-  if (b->exec && b->graph_key == key) return GKS_OK;
+  if (b->exec && b->graph_key == key && !b->tables_dirty) return GKS_OK;
   drop_graph(b);
   RC(batch_tables(b));
+  b->tables_dirty = false;
   b->shape.learned = learned;
-  const BatchArgs a{b->d_dev, b->d_st, b->d_start, b->d_pid, b->np, b->d_lstart};
+  const BatchArgs a{b->d_dev, b->d_st, b->d_start, b->d_pid, b->np_tab, b->d_lstart};
   const LaunchTable* lt = b->ctx[0]->lt;
   HIPCHK(hipStreamBeginCapture(b->s, hipStreamCaptureModeThreadLocal));
   hipError_t e = lt->batch_step(a, b->shape, b->tiles.data(), b->s);
@@ -812,6 +822,15 @@ int batch_graph(gks2d_batch* b, int learned) {
   HIPCHK(hipGraphInstantiate(&b->exec, g, nullptr, nullptr, 0));
   b->graph_key = key;
   return GKS_OK;
+}
+
+// the tile tables cover every problem again (start of a run, autotuning)
+void batch_cover_all(gks2d_batch* b) {
+  for (int p = 0; p < b->np; p++)
+    if (!b->in_tables[p]) {
+      b->in_tables[p] = 1;
+      b->tables_dirty = true;
+    }
 }
 
 bool batch_all_done(const gks2d_batch* b) {
@@ -852,6 +871,7 @@ int gks2d_batch_create(const gks2d_params* params, int32_t n, gks2d_batch** out)
   }
   gks2d_batch* b = new gks2d_batch();
   b->np = n;
+  b->in_tables.assign(n, 1);
   for (int p = 0; p < n; p++) {
     gks2d_ctx* c = nullptr;
     int rc = gks2d_create(&params[p], &c);
@@ -906,6 +926,7 @@ int gks2d_batch_run(gks2d_batch* b, const int32_t* bc, const double* tstop, int3
   int learned = 0;
   RC(batch_upload_devs(b, &learned));
   RC(batch_init_state(b, bc, tstop, max_steps, t, steps));
+  batch_cover_all(b);
   RC(batch_graph(b, learned));
   // every replay advances each unfinished problem by one step (or ends it),
   // so max_steps + 1 replays end every problem
@@ -917,6 +938,19 @@ int gks2d_batch_run(gks2d_batch* b, const int32_t* bc, const double* tstop, int3
     HIPCHK(hipMemcpyAsync(b->h_st, b->d_st, b->np * sizeof(BState), hipMemcpyDeviceToHost, b->s));
     HIPCHK(hipStreamSynchronize(b->s));
     if (batch_all_done(b)) break;
+    // drop the problems that finished from the launches (new tables, one
+    // more capture; a finished problem's tiles would otherwise still be
+    // dispatched every replay only to return)
+    bool shrink = false;
+    for (int p = 0; p < b->np; p++)
+      if (b->in_tables[p] && b->h_st[p].done) {
+        b->in_tables[p] = 0;
+        shrink = true;
+      }
+    if (shrink) {
+      b->tables_dirty = true;
+      RC(batch_graph(b, learned));
+    }
   }
   std::string bad;
   for (int p = 0; p < b->np; p++) {
@@ -942,6 +976,7 @@ int gks2d_batch_autotune(gks2d_batch* b, const int32_t* bc, int32_t steps, int32
   if (steps < 1) return fail(GKS_ERR_ARG, "steps");
   int learned = 0;
   RC(batch_upload_devs(b, &learned));
+  batch_cover_all(b);
   // snapshot of what a step changes and a later step reads
   std::vector<double*> snap(b->np, nullptr);
   std::vector<unsigned long long*> lsnap(b->np, nullptr);

@@ -1387,7 +1387,7 @@ __global__ void __launch_bounds__(kBlock) kb_update(BatchArgs B) {
 }
 __global__ void __launch_bounds__(kBlock) kb_alpha(BatchArgs B) {
   GKS_BATCH_TILE(KB_ALPHA);
-  if (!d.use_df) return;
+  if (!d.use_df || d.df_frozen) return;
   alpha_body(d, blk);
 }
 
@@ -1396,7 +1396,8 @@ __global__ void __launch_bounds__(kBlock) kb_alpha(BatchArgs B) {
 // gks2d_run computes it on the host; same operations, not contracted)
 __global__ void __launch_bounds__(64) kb_step_begin(BatchArgs B) {
 #pragma clang fp contract(off)
-  for (int p = threadIdx.x; p < B.np; p += blockDim.x) {
+  for (int q = threadIdx.x; q < B.np; q += blockDim.x) {
+    const int p = B.pid[q];
     BState& st = B.st[p];
     const Dev& d = B.dev[p];
     st.active = 0;
@@ -1427,7 +1428,8 @@ __global__ void __launch_bounds__(64) kb_step_begin(BatchArgs B) {
 // end of a step: a problem whose state blew up stops without advancing t
 __global__ void __launch_bounds__(64) kb_step_end(BatchArgs B) {
 #pragma clang fp contract(off)
-  for (int p = threadIdx.x; p < B.np; p += blockDim.x) {
+  for (int q = threadIdx.x; q < B.np; q += blockDim.x) {
+    const int p = B.pid[q];
     BState& st = B.st[p];
     if (!st.active) continue;
     if (B.dev[p].stat[4] != 0.0) {

@@ -10,6 +10,7 @@ struct Dev {
   int order;             // WENO order: 5, 7 or 9 (gp 2, 3, 4)
   int fuse_tanflux;      // WENO5: tangential reconstruction inside the flux launch
   int keep_face_fluxes;  // store raw (f, derf) per face for parity checks
+  int df_frozen;         // the DF sensor is never updated (gks2d_params.df_frozen)
   double dx, dy, area;
   double idx, idy;  // 1/dx, 1/dy (fast build)
   double gw[4];
@@ -72,7 +73,9 @@ enum BatchKind {
   KB_ALPHA, KB_COUNT
 };
 // start[k * (np + 1) + q]: first tile of the q-th problem in launch order for
-// work class k (start[.. + np] = tiles in the launch); pid[q]: that problem
+// work class k (start[.. + np] = tiles in the launch); pid[q]: that problem.
+// np counts the problems the launch covers (the unfinished ones when the
+// tables were built); dev and st are indexed by problem id.
 struct BatchArgs {
   const Dev* dev;
   BState* st;

@@ -11,9 +11,16 @@ workloads of this build; every step runs on the GPU through the C ABI.
 * ``run_vortex``   - isentropic vortex advection on the periodic [0,10]^2,
                      with the L1 error against the exact solution (the
                      initial field moved by t (1, 1), periodically);
+* ``sinwave_2d`` / ``accuracy_sinwave_2d`` - the 2D accuracy test
+                     (2Dproblem.cpp:1323-1603): the periodic density wave
+                     to t=2 with WENO9-AO-DF (4 Gauss points) on meshes
+                     10 * 2^k, L1 / L2 / Linf density errors against the
+                     exact cell averages and the observed orders;
 * ``write_output1d`` / ``write_output2d`` - the text formats of output1d /
   output2d (output.cpp:3-15, 64-84): density of the interior cells, one
-  value per line, six significant digits, x fastest in 2D.
+  value per line, six significant digits, x fastest in 2D;
+* ``write_error_form`` - output_error_form (output.cpp:17-61), the error /
+  order table of the accuracy tests.
 """
 import os
 import time
@@ -21,8 +28,8 @@ import time
 import numpy as np
 
 from . import Solver2D
-from .params import BC_FREE, BC_PERIODIC, planar_shock_params, sod_params, vortex_params
-from .problems import riemann_prim, riemann_states, sod_prim, vortex_convar
+from .params import BC_FREE, BC_PERIODIC, planar_shock_params, sinwave2d_params, sod_params, vortex_params
+from .problems import riemann_prim, riemann_states, sinwave2d_prim, sinwave2d_rho, sod_prim, vortex_convar
 from .solver1d import Solver1D
 
 
@@ -106,6 +113,74 @@ def run_vortex(n, tstop=10.0, weno_order=7, fast_math=1, max_steps=2**31 - 1, st
     s.close()
     return dict(params=p, convar=u, steps=steps, t=t, wall_s=wall, mcells=n * n * steps / wall / 1e6,
                 l1=vortex_l1(u, p, t))
+
+
+def sinwave2d_errors(convar, p, t, tstop):
+    """error_for_sinwave_2d (2Dproblem.cpp:1563-1601): L1 (mean), L2 (root
+    mean square) and Linf of the interior density against the exact cell
+    averages, computed only when t is within 1e-10 of tstop (else None)."""
+    if abs(t - tstop) > 1e-10:
+        return None
+    G = p.ghost
+    d = convar[0, G:G + p.nodex, G:G + p.nodey] - sinwave2d_rho(p)[G:G + p.nodex, G:G + p.nodey]
+    return [float(np.abs(d).mean()), float(np.sqrt((d * d).mean())), float(np.abs(d).max())]
+
+
+def sinwave_2d(n, cfl=0.05, tstop=2.0, fast_math=0):
+    """One mesh of the 2D accuracy test on the GPU: n x n cells of the
+    periodic [0, 2]^2 to tstop; returns the errors, steps and timing."""
+    p = sinwave2d_params(n, cfl=cfl, fast_math=fast_math)
+    s = Solver2D(p)
+    s.init_prim(sinwave2d_prim(p))
+    s.sync()
+    t0 = time.perf_counter()
+    steps = s.run([BC_PERIODIC] * 4, tstop)
+    s.sync()
+    wall = time.perf_counter() - t0
+    u = s.get_convar()
+    t = s.t
+    s.close()
+    return dict(params=p, n=n, steps=steps, t=t, wall_s=wall, errors=sinwave2d_errors(u, p, t, tstop))
+
+
+def convergence_orders(errors):
+    """log2 of the ratio of consecutive meshes' errors (output_error_form)."""
+    e = np.asarray(errors, dtype=np.float64)
+    return (np.log(e[:-1] / e[1:]) / np.log(2.0)).tolist()
+
+
+def accuracy_sinwave_2d(mesh_set=4, mesh_number_start=10, cfl=0.05, fast_math=0, path=None):
+    """accuracy_sinwave_2d (2Dproblem.cpp:1323-1346): meshes 10 * 2^k,
+    k < mesh_set; returns meshes, errors (per mesh [L1, L2, Linf]) and the
+    orders between consecutive meshes; with path, writes the table in
+    output_error_form's format."""
+    meshes = [mesh_number_start * 2 ** k for k in range(mesh_set)]
+    runs = [sinwave_2d(n, cfl=cfl, fast_math=fast_math) for n in meshes]
+    errors = [r["errors"] for r in runs]
+    out = dict(meshes=meshes, errors=errors, orders=convergence_orders(errors),
+               steps=[r["steps"] for r in runs], wall_s=[r["wall_s"] for r in runs])
+    if path:
+        write_error_form(path, cfl, 1.0, meshes, errors)
+    return out
+
+
+def write_error_form(path, cfl, dt_ratio, meshes, errors):
+    """output_error_form (output.cpp:17-61): the CFL number and dt ratio,
+    then one row per mesh '1/N & L1 & order & L2 & order & Linf & order \\\\'
+    (the first mesh has '~' for its orders); errors in scientific notation
+    with 6 digits, orders fixed with 2."""
+    orders = np.log(np.asarray(errors[:-1]) / np.asarray(errors[1:])) / np.log(2.0)
+    lines = [f"the CFL number is {cfl:g}", f"the dt ratio over dx is {dt_ratio:g}"]
+    for i, n in enumerate(meshes):
+        e = errors[i]
+        if i == 0:
+            lines.append(f"1/{n} & {e[0]:.6e} & ~ & {e[1]:.6e} & ~ & {e[2]:.6e} & ~ \\\\ ")
+        else:
+            o = orders[i - 1]
+            lines.append(f"1/{n} & {e[0]:.6e} & {o[0]:.2f} & {e[1]:.6e} & {o[1]:.2f} & {e[2]:.6e} & {o[2]:.2f} \\\\ ")
+    with open(path, "w") as f:
+        f.write("\n".join(lines) + "\n")
+    return path
 
 
 def _run_chunks(s, bc, tstop, chunk, log, tag):

@@ -28,7 +28,7 @@ class Params(ctypes.Structure):
         ("pr", ctypes.c_double), ("c1_euler", ctypes.c_double), ("c2_euler", ctypes.c_double),
         ("left", ctypes.c_double), ("right", ctypes.c_double), ("down", ctypes.c_double),
         ("up", ctypes.c_double), ("cfl", ctypes.c_double),
-        ("fast_math", ctypes.c_int32), ("reserved", ctypes.c_int32),
+        ("fast_math", ctypes.c_int32), ("df_frozen", ctypes.c_int32),
     ]
 
     @property
@@ -59,7 +59,7 @@ def planar_shock_params(n, weno_order=5, cfl=0.5, **kw):
         weno_order=weno_order, recon_variable=VAR_CHARACTERISTIC, use_df=1,
         tau_type=TAU_EULER, K=3, prandtl_fix=0, gamma=1.4, mu=-1.0, nu=-1.0, pr=1.0,
         c1_euler=0.05, c2_euler=1.0, left=0.0, right=1.0, down=0.0, up=1.0, cfl=cfl,
-        fast_math=0, reserved=0,
+        fast_math=0, df_frozen=0,
     )
     for k, v in kw.items():
         setattr(p, k, v)
@@ -139,6 +139,21 @@ def vortex_params(n, weno_order=7, cfl=0.5, **kw):
     variables, S2O4, CFL 0.5."""
     p = planar_shock_params(n, weno_order=weno_order, cfl=cfl, c1_euler=0.0, c2_euler=0.0,
                             left=0.0, right=10.0, down=0.0, up=10.0)
+    for k, v in kw.items():
+        setattr(p, k, v)
+    return p
+
+
+def sinwave2d_params(n, cfl=0.05, **kw):
+    """sinwave_2d (reference src/2Dproblem.cpp:1348-1527): the density wave
+    rho = 1 + 0.2 sin(pi (x + y)), u = v = p = 1, on the periodic [0, 2]^2
+    at n x n, K=3, gamma=1.4, Euler tau with c1=c2=0 (tau = 0), Mu=0,
+    Pr=0.73, WENO9-AO with the DF factor (WENO9_AO_with_df_normal /
+    _tangent) in conservative variables, 4 Gauss points (Lobatto), GKS2D
+    gks2nd_2d, S2O4, CFL 0.05; Update_alpha is commented out there, so the
+    DF sensor stays zero (df_frozen)."""
+    p = planar_shock_params(n, weno_order=9, cfl=cfl, recon_variable=VAR_CONSERVATIVE, c1_euler=0.0,
+                            c2_euler=0.0, mu=0.0, pr=0.73, left=0.0, right=2.0, down=0.0, up=2.0, df_frozen=1)
     for k, v in kw.items():
         setattr(p, k, v)
     return p

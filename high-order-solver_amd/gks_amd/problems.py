@@ -162,6 +162,36 @@ def sinwave_exact_1d(p):
     return sinwave_prim_1d(p)[0, p.ghost:p.nx - p.ghost]
 
 
+# ------------------------------------------------------------- 2D sine wave
+def sinwave2d_rho(p):
+    """Cell averages of rho = 1 + 0.2 sin(pi (x + y)) on every cell of an
+    [nx, ny] block (ICfor_sinwave_2d / error_for_sinwave_2d, reference
+    src/2Dproblem.cpp:1529-1601: the four corner sines of each cell in the
+    reference's operation order; the ghost layers get the periodic
+    continuation, which the boundary fills overwrite anyway)."""
+    G = p.ghost
+    dx = (p.right - p.left) / p.nodex
+    dy = (p.up - p.down) / p.nodey
+    pi = 3.14159265358979323846
+    i = np.arange(p.nx, dtype=np.float64)[:, None]
+    j = np.arange(p.ny, dtype=np.float64)[None, :]
+    xl, xr = (i - G) * dx, (i + 1 - G) * dx
+    yl, yr = (j - G) * dy, (j + 1 - G) * dy
+    k1 = np.sin(pi * (xr + yr))
+    k2 = np.sin(pi * (xr + yl))
+    k3 = np.sin(pi * (xl + yr))
+    k4 = np.sin(pi * (xl + yl))
+    return 1.0 - 0.2 / pi / pi / dx / dy * ((k1 - k2) - (k3 - k4))
+
+
+def sinwave2d_prim(p):
+    """Primitive field [4, nx, ny] of ICfor_sinwave_2d: the density wave,
+    u = v = 1, p = 1."""
+    prim = np.ones((4, p.nx, p.ny))
+    prim[0] = sinwave2d_rho(p)
+    return prim
+
+
 # ------------------------------------------------------------ isentropic vortex
 def vortex_prim_point(x, y, gamma=1.4, eps=5.0, xc=5.0, yc=5.0, free=(1.0, 1.0, 1.0, 1.0)):
     """Point values (rho, u, v, p) of the isentropic vortex of strength eps

@@ -66,7 +66,9 @@ typedef struct gks2d_params {
   double left, right, down, up; /* domain */
   double cfl;
   int32_t fast_math;         /* 0: IEEE parity build; 1: contracted FMAs */
-  int32_t reserved;
+  int32_t df_frozen;         /* 1: the DF sensor is never updated (no Update_alpha in
+                                gks2d_step / gks2d_run / batches), as in the reference's
+                                smooth accuracy driver sinwave_2d, 2Dproblem.cpp:1514-1519 */
 } gks2d_params;
 
 typedef struct gks2d_ctx gks2d_ctx;

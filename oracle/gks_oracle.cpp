@@ -37,7 +37,7 @@ int64_t orc2d_params_layout(int64_t* off) {
   OFF(nodex); OFF(nodey); OFF(ghost); OFF(gausspoint); OFF(weno_order); OFF(recon_variable);
   OFF(use_df); OFF(tau_type); OFF(K); OFF(prandtl_fix); OFF(gamma); OFF(mu); OFF(nu); OFF(pr);
   OFF(c1_euler); OFF(c2_euler); OFF(left); OFF(right); OFF(down); OFF(up); OFF(cfl);
-  OFF(fast_math); OFF(reserved);
+  OFF(fast_math); OFF(df_frozen);
 #undef OFF
   return (int64_t)sizeof(gks2d_params);
 }

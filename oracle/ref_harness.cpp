@@ -198,7 +198,9 @@ int ref2d_stage(void* h, int32_t stage, double dt, const int32_t* bc) {
   Reconstruction_forg0(c->xi, c->yi, c->fluids, b);
   Calculate_flux(c->xf, c->yf, c->xi, c->yi, b, stage);
   Update(c->fluids, c->xf, c->yf, b, stage);
-  if (is_using_df_factor) Update_alpha(c->xi, c->yi, c->fluids, b);
+  // df_frozen: the sensor is never updated (sinwave_2d comments the call
+  // out, 2Dproblem.cpp:1514-1519)
+  if (is_using_df_factor && !c->p.df_frozen) Update_alpha(c->xi, c->yi, c->fluids, b);
   return GKS_OK;
 }
 int ref2d_run(void* h, const int32_t* bc, double tstop, int32_t max_steps, double* dt_hist) {

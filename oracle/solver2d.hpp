@@ -424,7 +424,7 @@ template <class R> struct Solver2d {
     reconstruct();
     flux(stage, dt);
     update(stage);
-    if (P.use_df) update_alpha();
+    if (P.use_df && !P.df_frozen) update_alpha();
     return GKS_OK;
   }
 

@@ -19,9 +19,9 @@ sys.path.insert(0, os.path.join(ROOT, "high-order-solver_amd"))
 sys.path.insert(0, os.path.join(ROOT, "tests"))
 
 from gks_amd.params import (BC_FREE, BC_PERIODIC, BC_REFLECT, blastwave_params, planar_shock_params,  # noqa: E402
-                            shu_osher_params, sinwave1d_params, sod_params)
+                            shu_osher_params, sinwave1d_params, sinwave2d_params, sod_params)
 from gks_amd.problems import (PLANAR_SHOCK_ZONES, blastwave_prim, riemann_prim, riemann_states,  # noqa: E402
-                              shu_osher_prim, sinwave_prim_1d, sod_prim)
+                              shu_osher_prim, sinwave2d_prim, sinwave2d_rho, sinwave_prim_1d, sod_prim)
 from oracle_lib import reference, reference1d  # noqa: E402
 
 
@@ -270,13 +270,42 @@ def gen_runs79(ref):
     np.savez_compressed(os.path.join(HERE, "runs79.npz"), **out)
 
 
-def main():
+SINWAVE2D_MESHES = (10, 20, 40)
+
+
+def gen_sinwave2d(ref):
+    """The reference's 2D accuracy test (sinwave_2d, 2Dproblem.cpp:1348-1527,
+    WENO9-AO-DF with 4 Gauss points, DF sensor never updated) on meshes 10,
+    20, 40 to t = 2: per mesh the steps, final t and the interior density;
+    the errors against the exact cell averages are formed from these by the
+    tests (error_for_sinwave_2d's L1 / L2 / Linf)."""
+    out = {}
+    G = 5
+    for n in SINWAVE2D_MESHES:
+        p = sinwave2d_params(n)
+        s = ref.solver(p)
+        s.set_prim(sinwave2d_prim(p))
+        rc, hist = s.run([BC_PERIODIC] * 4, 2.0, 10**7)
+        assert rc == 0
+        st = s.get()
+        out.update({f"n{n}_rho": st["convar"][0, G:-G, G:-G], f"n{n}_steps": np.array(st["step"]),
+                    f"n{n}_t": np.array(st["t"]), f"n{n}_dt": hist[:st["step"]]})
+        s.close()
+        print("sinwave2d", n, st["step"], st["t"], flush=True)
+    np.savez_compressed(os.path.join(HERE, "sinwave2d.npz"), **out)
+
+
+def main(which=None):
     ref = reference()
+    if which == "sinwave2d":
+        gen_sinwave2d(ref)
+        return
     gen_flux_points(ref)
     gen_weno(ref)
     gen_runs(ref)
     gen_weno79(ref)
     gen_runs79(ref)
+    gen_sinwave2d(ref)
     ref1 = reference1d()
     gen_runs1d(ref1)
     gen_flux_points_1d(ref1)
@@ -286,4 +315,4 @@ def main():
 
 
 if __name__ == "__main__":
-    main()
+    main(sys.argv[1] if len(sys.argv) > 1 else None)
