@@ -17,7 +17,13 @@
 //    back by output2d() or Download_fluid(); host writes to `fluids` between
 //    sweeps need Upload_fluid();
 //  * a blow-up found by Convar_to_Primvar is reported (message + exit(0), as
-//    the reference does) at the next Get_CFL.
+//    the reference does) at the next Get_CFL;
+//  * the record types carry every field of the reference's; the fields the
+//    device sweeps keep on the device (Interface2d::line / gauss,
+//    Flux2d_gauss::gauss, Fluid2d's stage and HWENO work arrays) exist on
+//    the host with storage but are not filled by the sweeps;
+//  * which kernel build a context uses is the shim global gks_fast_kernels
+//    (false: the IEEE parity build; true: the contracted fast build).
 #pragma once
 #include <cmath>
 #include <ctime>
@@ -65,20 +71,35 @@ class Point2d {
   bool is_reduce_order;
 };
 
+// fluid_mesh.h:141-171, every field; the sweeps read and write convar,
+// alpha_x / alpha_y and (for Get_CFL at step 0) primvar
 class Fluid2d {
  public:
   double primvar[4];
   double exact;
   double convar[4];
   double convar_old[4];
+  double convar_stage1[4];
+  double convar_stage2[2];
   double res[4];
-  double xindex, yindex;
-  double coordx, coordy;
-  double dx, dy;
+  double xindex;
+  double yindex;
+  double coordx;
+  double coordy;
+  double dx;
+  double dy;
   double area;
+  bool is_hweno;
   double node[8];
+  bool boundary;
   double alpha_x = 0.0;
   double alpha_y = 0.0;
+  string direction;
+  bool is_left = false;
+  double Sw1[2][4];
+  double Lw1[2][4];
+  double derSw1[2][4];
+  double derLw1[2][4];
 };
 
 class Flux2d {
@@ -97,16 +118,21 @@ class Recon2d {
   double normal[2];
 };
 
-// geometry of one face; its reconstructed states live on the device
+// fluid_mesh.h:194-206: geometry of one face; its reconstructed states
+// (line, gauss) are computed and kept on the device: the host records
+// (gausspoint of them per face, allocated by Setinterface_array) stay zero
 class Interface2d {
  public:
+  Recon2d line;
+  Recon2d* gauss = nullptr;
   double x = 0, y = 0;
   double normal[2] = {0, 0};
   double length = 0;
 };
 
-// per-face flux records live on the device; the host array holds null
-// entries so drivers can keep passing it around
+// fluid_mesh.h:208-216: per-face flux records (gausspoint per face,
+// allocated by Setflux_gauss_array); the device keeps the fluxes, the host
+// records stay zero
 class Flux2d_gauss {
  public:
   Flux2d* gauss = nullptr;
@@ -186,6 +212,16 @@ void WENO5_AO_tangent(Interface2d* left, Interface2d* right, Interface2d* down, 
                       Block2d block);
 void WENO5_AO_with_df_tangent(Interface2d* left, Interface2d* right, Interface2d* down, Interface2d* up,
                               Fluid2d* fluids, Block2d block);
+// WENO7 / WENO9 with the DF factor (reconstruction.h:62-80); need
+// gausspoint 3 / 4
+void WENO7_AO_with_df_normal(Interface2d& left, Interface2d& right, Interface2d& down, Interface2d& up,
+                             Fluid2d* fluids, Block2d block);
+void WENO7_AO_with_df_tangent(Interface2d* left, Interface2d* right, Interface2d* down, Interface2d* up,
+                              Fluid2d* fluids, Block2d block);
+void WENO9_AO_with_df_normal(Interface2d& left, Interface2d& right, Interface2d& down, Interface2d& up,
+                             Fluid2d* fluids, Block2d block);
+void WENO9_AO_with_df_tangent(Interface2d* left, Interface2d* right, Interface2d* down, Interface2d* up,
+                              Fluid2d* fluids, Block2d block);
 void Center_do_nothing_normal(Interface2d* xinterfaces, Interface2d* yinterfaces, Fluid2d* fluids, Block2d block);
 void Center_all_collision_multi(Interface2d* xinterfaces, Interface2d* yinterfaces, Fluid2d* fluids,
                                 Block2d block);
@@ -226,6 +262,11 @@ void IC_for_riemann_2d(Fluid2d* fluid, double* zone1, double* zone2, double* zon
                        double* discon);
 
 // --------------------------------------------- device residency (shim only)
+// kernel build of the device contexts created from now on (1D, 2D, 3D):
+// false (default) the IEEE build, whose results differ from the reference
+// only through the device erfc / exp / sqrt; true the fast build (FMA
+// contraction, reciprocal multiplication)
+extern bool gks_fast_kernels;
 // copies the device state (convar, alpha) back into `fluids` and refreshes
 // primvar on the host
 void Download_fluid(Fluid2d* fluids, Block2d block);

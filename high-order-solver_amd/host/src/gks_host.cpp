@@ -44,6 +44,7 @@ Reconstruction_forG0_2D_normal g0reconstruction_2D_normal = Center_do_nothing_no
 Reconstruction_forG0_2D_tangent g0reconstruction_2D_tangent = Center_all_collision_multi;
 
 GKS2d_type gks2dsolver = kfvs1st_2d;
+bool gks_fast_kernels = false;
 Flux_function_2d flux_function_2d = GKS2D;
 TimeMarchingCoefficient_2d timecoe_list_2d = nullptr;
 
@@ -70,7 +71,10 @@ gks2d_params snapshot(const Block2d& b) {
   p.nodey = b.nodey;
   p.ghost = b.ghost;
   p.gausspoint = gausspoint;
-  p.weno_order = 5;
+  // the scheme from the identity of the reconstruction pointers
+  p.weno_order = cellreconstruction_2D_normal == WENO9_AO_with_df_normal   ? 9
+                 : cellreconstruction_2D_normal == WENO7_AO_with_df_normal ? 7
+                                                                           : 5;
   p.recon_variable = reconstruction_variable == characteristic ? GKS_VAR_CHARACTERISTIC : GKS_VAR_CONSERVATIVE;
   p.use_df = is_using_df_factor ? 1 : 0;
   p.tau_type = tau_type == NS ? GKS_TAU_NS : GKS_TAU_EULER;
@@ -87,8 +91,7 @@ gks2d_params snapshot(const Block2d& b) {
   p.down = b.down;
   p.up = b.up;
   p.cfl = b.CFL;
-  const char* fm = getenv("GKS_CONTRACTED");
-  p.fast_math = (fm && fm[0] == '1') ? 1 : 0;
+  p.fast_math = gks_fast_kernels ? 1 : 0;
   return p;
 }
 
@@ -175,13 +178,28 @@ Fluid2d* Setfluid(Block2d& block) {
   cout << "fluid variable allocate done..." << endl;
   return var;
 }
+// the face arrays with their per-Gauss-point records, as the reference
+// allocates them (basic_function.cpp:1322-1345); the records stay zero
 Interface2d* Setinterface_array(Block2d block) {
-  Interface2d* var = new Interface2d[(size_t)(block.nx + 1) * (block.ny + 1)];
+  const size_t nf = (size_t)(block.nx + 1) * (block.ny + 1);
+  Interface2d* var = new Interface2d[nf];
+  const int gp = gausspoint > 0 ? gausspoint : 1;
+  Recon2d* rec = new Recon2d[nf * gp]();
+  for (size_t f = 0; f < nf; f++) var[f].gauss = rec + f * gp;
   cout << "interface variable allocate done..." << endl;
   return var;
 }
 Flux2d_gauss** Setflux_gauss_array(Block2d block) {
-  Flux2d_gauss** var = new Flux2d_gauss*[(size_t)(block.nx + 1) * (block.ny + 1)]();
+  const size_t nf = (size_t)(block.nx + 1) * (block.ny + 1);
+  const int stages = block.stages > 0 ? block.stages : 1;  // set by Initial_stages
+  const int gp = gausspoint > 0 ? gausspoint : 1;
+  Flux2d_gauss** var = new Flux2d_gauss*[nf];
+  Flux2d_gauss* rows = new Flux2d_gauss[nf * stages];
+  Flux2d* rec = new Flux2d[nf * stages * gp]();
+  for (size_t f = 0; f < nf; f++) {
+    var[f] = rows + f * stages;
+    for (int s = 0; s < stages; s++) var[f][s].gauss = rec + (f * stages + s) * gp;
+  }
   cout << "flux with gausspoint variable allocate done..." << endl;
   return var;
 }
@@ -298,13 +316,41 @@ void Center_all_collision_multi(Interface2d*, Interface2d*, Fluid2d*, Block2d) {
   leaf_called("Center_all_collision_multi");
 }
 
+void WENO7_AO_with_df_normal(Interface2d&, Interface2d&, Interface2d&, Interface2d&, Fluid2d*, Block2d) {
+  leaf_called("WENO7_AO_with_df_normal");
+}
+void WENO7_AO_with_df_tangent(Interface2d*, Interface2d*, Interface2d*, Interface2d*, Fluid2d*, Block2d) {
+  leaf_called("WENO7_AO_with_df_tangent");
+}
+void WENO9_AO_with_df_normal(Interface2d&, Interface2d&, Interface2d&, Interface2d&, Fluid2d*, Block2d) {
+  leaf_called("WENO9_AO_with_df_normal");
+}
+void WENO9_AO_with_df_tangent(Interface2d*, Interface2d*, Interface2d*, Interface2d*, Fluid2d*, Block2d) {
+  leaf_called("WENO9_AO_with_df_tangent");
+}
+
 void Reconstruction_within_cell(Interface2d* xinterfaces, Interface2d* yinterfaces, Fluid2d* fluids,
                                 Block2d block) {
-  if (cellreconstruction_2D_normal != WENO5_AO_with_df_normal ||
-      cellreconstruction_2D_tangent != WENO5_AO_with_df_tangent)
-    die("Reconstruction_within_cell", "the device path implements WENO5_AO_with_df_normal/_tangent");
+  // the normal / tangential pair of one order, with its Gauss points
+  struct Pair {
+    Reconstruction_within_Cell_2D_normal n;
+    Reconstruction_within_Cell_2D_tangent t;
+    int gp;
+    const char* name;
+  };
+  static const Pair pairs[3] = {{WENO5_AO_with_df_normal, WENO5_AO_with_df_tangent, 2, "WENO5"},
+                                {WENO7_AO_with_df_normal, WENO7_AO_with_df_tangent, 3, "WENO7"},
+                                {WENO9_AO_with_df_normal, WENO9_AO_with_df_tangent, 4, "WENO9"}};
+  const Pair* pr = nullptr;
+  for (const Pair& q : pairs)
+    if (cellreconstruction_2D_normal == q.n && cellreconstruction_2D_tangent == q.t) pr = &q;
+  if (!pr)
+    die("Reconstruction_within_cell",
+        "the device path implements WENO5/7/9_AO_with_df_normal with the _tangent of the same order");
   if (!is_using_df_factor) die("Reconstruction_within_cell", "the DF variants need is_using_df_factor = true");
-  if (gausspoint != 2) die("Reconstruction_within_cell", "WENO5 tangential reconstruction needs gausspoint = 2");
+  if (gausspoint != pr->gp)
+    die("Reconstruction_within_cell", std::string(pr->name) + " tangential reconstruction needs gausspoint = " +
+                                          std::to_string(pr->gp));
   (void)xinterfaces;
   (void)yinterfaces;
   check(gks2d_reconstruct(bind(fluids, block)), "reconstruct");
@@ -471,8 +517,7 @@ gks3d_params snapshot3(const Block3d& b) {
   p.recon_variable = reconstruction_variable == characteristic ? GKS_VAR_CHARACTERISTIC : GKS_VAR_CONSERVATIVE;
   p.tau_type = tau_type == NS ? GKS_TAU_NS : GKS_TAU_EULER;
   p.K = K;
-  const char* fm = getenv("GKS_CONTRACTED");
-  p.fast_math = (fm && fm[0] == '1') ? 1 : 0;
+  p.fast_math = gks_fast_kernels ? 1 : 0;
   p.gamma = Gamma;
   p.mu = Mu;
   p.nu = Nu;
@@ -841,8 +886,7 @@ gks1d_params snapshot1(const Block1d& b) {
   p.df_from_start = 0;  // the driver's pointer assignment decides, per call
   p.tau_type = tau_type == NS ? GKS_TAU_NS : GKS_TAU_EULER;
   p.K = K;
-  const char* fm = getenv("GKS_CONTRACTED");
-  p.fast_math = (fm && fm[0] == '1') ? 1 : 0;
+  p.fast_math = gks_fast_kernels ? 1 : 0;
   p.gamma = Gamma;
   p.mu = Mu;
   p.nu = Nu;

@@ -100,3 +100,39 @@ def test_reference_style_1d_driver_matches_oracle(tmp_path, n, steps, df):
     assert np.all(l1 <= 1e-10)  # device erfc / exp against the CPU's over ~70 steps: ~1e-12
     if df:
         assert np.max(np.abs(alpha - st["alpha"][G:-G])) <= 1e-9
+
+
+EXE_V = os.path.join(ROOT, "high-order-solver_amd", "bin", "vortex2d")
+
+
+@pytest.mark.parametrize("n,order,steps,fast", [(48, 7, 60, False), (32, 9, 40, True)])
+def test_reference_style_vortex_driver_matches_oracle(tmp_path, n, order, steps, fast):
+    """vortex2d.cpp: the smooth-flow loop written the reference's way with
+    WENO7_AO_with_df_normal/_tangent (gausspoint 3) or WENO9 (gausspoint 4)
+    assigned to the scheme pointers, periodic boundaries, the kernel build
+    chosen through gks_fast_kernels; the isentropic vortex's cell averages,
+    against the oracle on the same inputs (L1 1e-9, the same steps and t)."""
+    from gks_amd.params import BC_PERIODIC, vortex_params
+    from gks_amd.problems import vortex_convar
+    p = vortex_params(n, weno_order=order)
+    u0 = vortex_convar(p)
+    src = tmp_path / "u0.bin"
+    np.ascontiguousarray(u0[:, G:-G, G:-G]).tofile(src)
+    out = tmp_path / "uv.bin"
+    print("vortex driver", n, order, steps, fast, flush=True)
+    args = [EXE_V, str(n), str(order), "10.0", str(steps), str(src), str(out)] + (["fast"] if fast else [])
+    r = subprocess.run(args, capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stdout + r.stderr
+    raw = np.fromfile(out, dtype=np.float64)
+    t, k = raw[0], int(raw[1])
+    u = raw[2:].reshape(4, n, n)
+    o = oracle().solver(p)
+    o.init_convar(u0)
+    rc, _ = o.run([BC_PERIODIC] * 4, 10.0, steps)
+    assert rc == 0
+    st = o.get()
+    assert k == st["step"] == steps and abs(t - st["t"]) <= 1e-13
+    dx = 10.0 / n
+    l1 = np.abs(u - st["convar"][:, G:-G, G:-G]).sum(axis=(1, 2)) * dx * dx
+    print("  steps", k, "L1", l1, flush=True)
+    assert np.all(l1 <= 1e-9)
