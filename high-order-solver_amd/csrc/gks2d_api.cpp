@@ -564,6 +564,23 @@ int gks2d_flux_points_smooth(const gks2d_params* p, int64_t n, const double* lef
                              double* f, double* derf) {
   return flux_points_impl(p, n, left, right, dt, nullptr, f, derf, 1);
 }
+int gks2d_linear_weights_points(const gks2d_params* p, int64_t n, const double* stencils, double h, double* out) {
+  if (!p || !stencils || !out) return fail(GKS_ERR_ARG, "null");
+  if (n < 0 || !(h > 0.0)) return fail(GKS_ERR_ARG, "n < 0 or h <= 0");
+  if (n == 0) return GKS_OK;
+  const LaunchTable* lt = p->fast_math ? &fast::table : &ieee::table;
+  double *dw = nullptr, *dout = nullptr;
+  hipError_t e = hipMalloc(&dw, n * 5 * 8);
+  if (e == hipSuccess) e = hipMalloc(&dout, n * 8 * 8);
+  if (e == hipSuccess) e = hipMemcpy(dw, stencils, n * 5 * 8, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = lt->linear_points(n, dw, h, dout, nullptr);
+  if (e == hipSuccess) e = hipDeviceSynchronize();
+  if (e == hipSuccess) e = hipMemcpy(out, dout, n * 8 * 8, hipMemcpyDeviceToHost);
+  (void)hipFree(dw);
+  (void)hipFree(dout);
+  if (e != hipSuccess) return hip_fail(e, "gks2d_linear_weights_points");
+  return GKS_OK;
+}
 
 // ------------------------------------------------- learned classifier hooks
 // 2 kernels (normal, tangential) x 1024 slots x (linear, total) counters,

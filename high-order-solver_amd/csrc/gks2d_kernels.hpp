@@ -1221,6 +1221,24 @@ __device__ __forceinline__ void alpha_body(const Dev& d, unsigned blk) {
 }
 __global__ void __launch_bounds__(kBlock) k_alpha(Dev d) { alpha_body(d, blockIdx.x); }
 
+// leaf check of the linear-weight reconstructions (learned path): per
+// 5-cell stencil the fifth-order candidate's values and derivatives at the
+// centre cell's left and right faces and at its two tangential Gauss points
+__global__ void __launch_bounds__(kBlock) k_linear_points(long long n, const double* W, double h, double* out) {
+  long long t = gtid();
+  if (t >= n) return;
+  double w[5];
+#pragma unroll
+  for (int s = 0; s < 5; s++) w[s] = W[t * 5 + s];
+  const double ih = 1.0 / h;
+  double o[8];
+  weno5_linear_face(0, o[0], o[1], w[0], w[1], w[2], w[3], w[4], h, ih);
+  weno5_linear_face(1, o[2], o[3], w[0], w[1], w[2], w[3], w[4], h, ih);
+  weno5_linear_2gauss(o[4], o[5], o[6], o[7], w[0], w[1], w[2], w[3], w[4], h, ih);
+#pragma unroll
+  for (int k = 0; k < 8; k++) out[t * 8 + k] = o[k];
+}
+
 // leaf parity: centre + GKS2D over n independent points
 // (smooth: the smooth-flow flux instead, no centre output)
 __global__ void __launch_bounds__(kBlock) k_flux_points(Gas g, long long n, const double* L, const double* R,
@@ -1608,6 +1626,11 @@ static hipError_t flux_points(const Gas& g, long long n, const double* L, const 
   return hipGetLastError();
 }
 
+static hipError_t linear_points(long long n, const double* W, double h, double* out, hipStream_t s) {
+  hipLaunchKernelGGL(k_linear_points, dim3(blocks_for(n)), dim3(kBlock), 0, s, n, W, h, out);
+  return hipGetLastError();
+}
+
 static hipError_t label(const Dev& d, int dir, int stride, double tol, float* X, float* Y, hipStream_t s) {
   const long long total = dir == 0 ? (long long)(d.NX + 2) * d.ny : (long long)d.nx * (d.NY + 2);
   const long long ns = (total + stride - 1) / stride;
@@ -1673,8 +1696,8 @@ static hipError_t batch_step(const BatchArgs& B, const BatchShape& sh, const uns
   return hipGetLastError();
 }
 
-const LaunchTable table = {bc,     primvar,     copy_old, normal, tangent, flux, normal_list, flux_list,
-                          update, alpha,       flux_points, label,  infer,   batch_step};
+const LaunchTable table = {bc,     primvar, copy_old,    normal, tangent, flux,       normal_list, flux_list,
+                          update, alpha,   flux_points, label,  infer,   batch_step, linear_points};
 #endif
 
 }  // namespace GKS_VARIANT

@@ -124,6 +124,8 @@ struct LaunchTable {
   // one S2O4 step of every active problem of a batch (capturable: no host
   // synchronisation, no memset); tiles[k] = start[k * (np + 1) + np]
   hipError_t (*batch_step)(const BatchArgs&, const BatchShape&, const unsigned* tiles, hipStream_t);
+  // the linear-weight reconstructions of n 5-cell stencils (leaf check)
+  hipError_t (*linear_points)(long long n, const double* W, double h, double* out, hipStream_t);
 };
 
 namespace ieee { extern const LaunchTable table; }

@@ -8,7 +8,9 @@
 #include <vector>
 
 #include "../../include/gks3d.h"
+#include "../../include/gks_learn.h"
 #include "gks3d_launch.hpp"
+#include "mlp_pack.hpp"
 
 namespace {
 
@@ -72,6 +74,11 @@ struct gks3d_ctx {
   bool stat_pending = false; // a primitive sweep's reduction waits in d.stat
   double tmax = 0.0, rhomin = 0.0;
   std::vector<void*> allocs;
+  // learned path (gks3d_set_classifier)
+  float* d_mlp = nullptr;
+  unsigned short* d_mlpw = nullptr;
+  unsigned char* d_lin = nullptr;         // lin3 [3][nc], then lin_line [2][nf]
+  unsigned long long* d_lstat = nullptr;  // [3][kStat3Slots][2]
   // per-phase HIP-event timing (gks3d_timing)
   unsigned timing = 0;  // bit mask of the timed phases
   struct Rec {
@@ -163,6 +170,10 @@ int gks3d_destroy(gks3d_ctx* c) {
   if (!c) return GKS_OK;
   if (c->s) (void)hipStreamSynchronize(c->s);
   for (void* a : c->allocs) (void)hipFree(a);
+  if (c->d_mlp) (void)hipFree(c->d_mlp);
+  if (c->d_mlpw) (void)hipFree(c->d_mlpw);
+  if (c->d_lin) (void)hipFree(c->d_lin);
+  if (c->d_lstat) (void)hipFree(c->d_lstat);
   if (c->h_stat) (void)hipHostFree(c->h_stat);
   for (hipEvent_t e : c->pool) (void)hipEventDestroy(e);
   if (c->s) (void)hipStreamDestroy(c->s);
@@ -276,6 +287,7 @@ static int stage(gks3d_ctx* c, int s, double dt, const int* bc) {
   const Launch3dTable* lt = c->lt;
   RC(timed(c, GKS3_PHASE_BC, [&] { return lt->bc(d, bc, c->s); }));
   RC(timed(c, GKS3_PHASE_PRIMVAR, [&] { return lt->primvar(d, c->s); }));
+  if (d.mlp) RC(timed(c, GKS3_PHASE_CLASSIFY, [&] { return lt->classify(d, c->s); }));
   for (int dir = 0; dir < 3; dir++) {
     RC(timed(c, GKS3_PHASE_FACE, [&] { return lt->face(d, dir, c->s); }));
     RC(timed(c, GKS3_PHASE_LINE, [&] { return lt->line(d, dir, c->s); }));
@@ -380,6 +392,7 @@ int gks3d_flux(gks3d_ctx* c, int32_t stage, double dt) {
   if (stage < 0 || stage > 1) return fail(GKS_ERR_ARG, "stage");
   const Dev3& d = c->d;
   const Launch3dTable* lt = c->lt;
+  if (d.mlp) RC(timed(c, GKS3_PHASE_CLASSIFY, [&] { return lt->classify(d, c->s); }));
   for (int dir = 0; dir < 3; dir++) {
     RC(timed(c, GKS3_PHASE_FACE, [&] { return lt->face(d, dir, c->s); }));
     RC(timed(c, GKS3_PHASE_LINE, [&] { return lt->line(d, dir, c->s); }));
@@ -419,6 +432,51 @@ int gks3d_get_timing(gks3d_ctx* c, double* ms, int64_t* launches) {
 int gks3d_sync(gks3d_ctx* c) {
   if (!c) return fail(GKS_ERR_ARG, "null");
   HIPCHK(hipStreamSynchronize(c->s));
+  return GKS_OK;
+}
+
+constexpr size_t kLstat3Words = 3 * kStat3Slots * 2;
+int gks3d_set_classifier(gks3d_ctx* c, const float* params, float logit_threshold) {
+  if (!c) return fail(GKS_ERR_ARG, "null");
+  HIPCHK(hipStreamSynchronize(c->s));
+  if (!params) {
+    c->d.mlp = nullptr;
+    return GKS_OK;
+  }
+  Dev3& d = c->d;
+  if (!c->d_mlp) {
+    HIPCHK(hipMalloc(&c->d_mlp, GKS_MLP_NPARAM * sizeof(float)));
+    HIPCHK(hipMalloc(&c->d_mlpw, mlp::kPackHalves * sizeof(unsigned short)));
+    HIPCHK(hipMalloc(&c->d_lin, 3 * (size_t)d.nc + 2 * (size_t)d.nf));
+    HIPCHK(hipMalloc(&c->d_lstat, kLstat3Words * sizeof(unsigned long long)));
+  }
+  std::vector<uint16_t> packed(mlp::kPackHalves);
+  mlp::pack_bf16(params, packed.data());
+  HIPCHK(hipMemcpy(c->d_mlp, params, GKS_MLP_NPARAM * sizeof(float), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(c->d_mlpw, packed.data(), packed.size() * sizeof(uint16_t), hipMemcpyHostToDevice));
+  HIPCHK(hipMemset(c->d_lin, 0, 3 * (size_t)d.nc + 2 * (size_t)d.nf));
+  HIPCHK(hipMemset(c->d_lstat, 0, kLstat3Words * sizeof(unsigned long long)));
+  d.mlp = c->d_mlp;
+  d.mlpw = c->d_mlpw;
+  d.mlp_thr = logit_threshold;
+  d.lin3 = c->d_lin;
+  d.lin_line = c->d_lin + 3 * (size_t)d.nc;
+  d.lstat3 = c->d_lstat;
+  return GKS_OK;
+}
+
+int gks3d_classifier_counts(gks3d_ctx* c, int64_t counts[6]) {
+  if (!c || !counts) return fail(GKS_ERR_ARG, "null");
+  if (!c->d_lstat) return fail(GKS_ERR_STATE, "no classifier set");
+  std::vector<unsigned long long> h(kLstat3Words);
+  HIPCHK(hipStreamSynchronize(c->s));
+  HIPCHK(hipMemcpy(h.data(), c->d_lstat, h.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost));
+  for (int k = 0; k < 6; k++) counts[k] = 0;
+  for (int kind = 0; kind < 3; kind++)
+    for (int q = 0; q < kStat3Slots; q++) {
+      counts[2 * kind] += (int64_t)h[2 * (kind * kStat3Slots + q)];
+      counts[2 * kind + 1] += (int64_t)h[2 * (kind * kStat3Slots + q) + 1];
+    }
   return GKS_OK;
 }
 

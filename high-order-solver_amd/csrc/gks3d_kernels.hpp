@@ -25,6 +25,9 @@
 
 #include "gks3d_device.hpp"
 #include "gks3d_launch.hpp"
+// the troubled-interface classifier of the 2D path (stencil features, the
+// MLP on the matrix cores) and its linear-weight reconstructions
+#include "gks2d_learn_device.hpp"
 
 #ifndef GKS_VARIANT
 #error "GKS_VARIANT must name the kernel namespace"
@@ -186,10 +189,104 @@ __global__ void __launch_bounds__(kBlock3) k3_copy_old(Dev3 d, Box3 b) {
   for (int v = 0; v < 5; v++) d.Uold[v * d.nc + c] = d.U[v * d.nc + c];
 }
 
+// ------------------------------------------------------ learned classifier
+// linear / all counts of one wave's reconstructions of a kind (0 face, 1
+// line, 2 point), added by its first active lane to one of kStat3Slots
+// slot pairs
+__device__ __forceinline__ void count3(unsigned long long* stat, int kind, bool lin) {
+  const unsigned long long act = __ballot(1), bl = __ballot(lin);
+  if ((threadIdx.x & 63) == __ffsll((long long)act) - 1) {
+    unsigned long long* slot =
+        stat + 2 * (kind * kStat3Slots + (int)((blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)) % kStat3Slots));
+    atomicAdd(slot, (unsigned long long)__popcll(bl));
+    atomicAdd(slot + 1, (unsigned long long)__popcll(act));
+  }
+}
+// Every cell of the box, along x, y and z: the 16 features of the 5-point
+// (rho, p) stencil and the MLP (one wave evaluates its 64 cells per
+// direction on the matrix cores); lin3[dir][cell] = 1 when the logit is below
+// the threshold (the 3D scheme has no DF sensor).  Along dir only cells
+// whose dir index is in [G-1, n-G+1) are classified (their stencils stay in
+// the block; every read of lin3[dir] is at such a cell).  All lanes of a
+// wave run the MLP; the other lanes carry zero features and store nothing.
+__global__ void __launch_bounds__(kBlock3) k3_classify(Dev3 d, Box3 b) {
+  __shared__ __align__(16) short xs[kBlock3 * 16];
+  __shared__ float ls[kBlock3];
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const bool valid = t < b.count();
+  int i, j, k;
+  b.at(valid ? t : 0, i, j, k);
+  const long long c = cell3(d, i, j, k);
+  const int wv = threadIdx.x >> 6;
+#pragma unroll
+  for (int dir = 0; dir < 3; dir++) {
+    const long long cs = dir == 0 ? (long long)d.ny * d.nz : (dir == 1 ? (long long)d.nz : 1LL);
+    const int cd = dir == 0 ? i : (dir == 1 ? j : k), nd = dir == 0 ? d.nx : (dir == 1 ? d.ny : d.nz);
+    const bool ok = valid && cd >= d.G - 1 && cd < nd - d.G + 1;
+    float x[16];
+    if (ok) {
+      double q[2][5];
+#pragma unroll
+      for (int s = 0; s < 5; s++) {
+        double u[5], p[5];
+#pragma unroll
+        for (int v = 0; v < 5; v++) u[v] = d.U[v * d.nc + c + (s - 2) * cs];
+        prim3(p, u, d.gas);
+        q[0][s] = p[0];
+        q[1][s] = p[4];
+      }
+      gksd::stencil_features(x, q);
+    } else {
+#pragma unroll
+      for (int m = 0; m < 16; m++) x[m] = 0.0f;
+    }
+    const float z = gksd::mlp_wave_logit(x, d.mlp, d.mlpw, xs + wv * 64 * 16, ls + wv * 64);
+    if (ok) d.lin3[dir * d.nc + c] = z < d.mlp_thr ? 1 : 0;
+  }
+}
+// the five-entry tangential rule: entry q's source cell (the left cell of
+// its face for side 0, the right one for side 1) took the linear weights
+// across the face, and the centre entry's source cell took them along the
+// tangential direction TDIR
+template <int DIR, int TDIR>
+__device__ __forceinline__ bool line_sources_linear(const Dev3& d, int i, int j, int k, int side, int sign) {
+  const int si = i - (side == 0 && DIR == 0), sj = j - (side == 0 && DIR == 1), sk = k - (side == 0 && DIR == 2);
+  unsigned ok = 1;
+#pragma unroll
+  for (int q = 0; q < 5; q++) {
+    const int o = sign * (q - 2);
+    ok &= d.lin3[DIR * d.nc + cell3(d, si + (TDIR == 0) * o, sj + (TDIR == 1) * o, sk + (TDIR == 2) * o)];
+  }
+  return ok && d.lin3[TDIR * d.nc + cell3(d, si, sj, sk)];
+}
+// tangential_side with the linear weights (the fifth-order candidate at the
+// two Gauss points), on the conservative values: the characteristic
+// projection commutes with a fixed linear combination
+template <int NC, class ST>
+__device__ __forceinline__ void tangential_side_linear(double (*out)[NC + 5], const ST& st, const HInv& h) {
+  constexpr int NF = NC / 5;
+#pragma unroll
+  for (int fld = 0; fld < NF; fld++)
+#pragma unroll
+    for (int v = 0; v < 5; v++) {
+      const int m = 5 * fld + v;
+      double g1, g1x, g2, g2x;
+      gksd::weno5_linear_2gauss(g1, g1x, g2, g2x, st(0, m), st(1, m), st(2, m), st(3, m), st(4, m), h.h, h.r1);
+      out[0][m] = g1;
+      out[1][m] = g2;
+      if (fld == 0) {
+        out[0][NC + v] = g1x;
+        out[1][NC + v] = g2x;
+      }
+    }
+}
+
 // ------------------------------------------------------------ face values
 // WENO5_AO (reconstruction.cpp:3803-3923) of one cell across its two DIR
-// faces, in the face frame (Ydirection / Zdirection for y / z)
-template <int DIR>
+// faces, in the face frame (Ydirection / Zdirection for y / z).  LEARNED: a
+// cell whose DIR stencil the classifier called smooth takes the linear
+// weights (the fifth-order candidate, on the conservative values).
+template <int DIR, bool LEARNED>
 __global__ void __launch_bounds__(kBlock3) k3_face(Dev3 d, Box3 b) {
   const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= b.count()) return;
@@ -208,7 +305,18 @@ __global__ void __launch_bounds__(kBlock3) k3_face(Dev3 d, Box3 b) {
     to_frame(w[s], u, DIR);
   }
   double lo[10], ro[10];
-  {
+  bool lin = false;
+  if constexpr (LEARNED) {
+    lin = d.lin3[DIR * d.nc + c] != 0;
+    count3(d.lstat3, 0, lin);
+  }
+  if (lin) {
+#pragma unroll
+    for (int v = 0; v < 5; v++) {
+      gksd::weno5_linear_face(0, lo[v], lo[5 + v], w[0][v], w[1][v], w[2][v], w[3][v], w[4][v], h.h, h.r1);
+      gksd::weno5_linear_face(1, ro[v], ro[5 + v], w[0][v], w[1][v], w[2][v], w[3][v], w[4][v], h.h, h.r1);
+    }
+  } else {
     double pn1[5], p0[5], pp1[5];
     prim3(pn1, w[1], g);
     prim3(p0, w[2], g);
@@ -359,7 +467,7 @@ struct LvStencil {
 // neighbouring lanes, which share their order-reduction verdict: both lines
 // of both sides fall back to the face value (tangential derivative 0) when
 // any of the four reconstructed states has lambda <= 0.
-template <int DIR>
+template <int DIR, bool LEARNED>
 __global__ void __launch_bounds__(kBlock3) k3_line(Dev3 d, Box3 b) {
   const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   const long long fi = t >> 1;
@@ -371,7 +479,15 @@ __global__ void __launch_bounds__(kBlock3) k3_line(Dev3 d, Box3 b) {
   const long long step = DIR == 1 ? -(long long)(d.ny + 1) * (d.nz + 1) : (long long)(d.nz + 1);
   const HInv h = hinv(DIR == 1 ? d.dx : d.dy);
   double res[2][15];  // [line]
-  tangential_side<10>(res, FvStencil{d.fv, d.nf, f, step, side}, h, d.recon_var, d.gas);
+  bool lin = false;
+  if constexpr (LEARNED) {
+    // line direction: y for x and z faces, -x for y faces
+    lin = DIR == 1 ? line_sources_linear<1, 0>(d, i, j, k, side, -1) : line_sources_linear<DIR, 1>(d, i, j, k, side, 1);
+    count3(d.lstat3, 1, lin);
+    d.lin_line[(long long)side * d.nf + f] = lin ? 1 : 0;
+  }
+  if (lin) tangential_side_linear<10>(res, FvStencil{d.fv, d.nf, f, step, side}, h);
+  else tangential_side<10>(res, FvStencil{d.fv, d.nf, f, step, side}, h, d.recon_var, d.gas);
   int red = (reduce3(res[0], d.gas) || reduce3(res[1], d.gas)) ? 1 : 0;
   red |= __shfl_xor(red, 1);
   if (red) {
@@ -411,7 +527,7 @@ struct LdsSide {
   const double* p;
   __device__ __forceinline__ double operator[](int k) const { return p[k]; }
 };
-template <int DIR>
+template <int DIR, bool LEARNED>
 __global__ void __launch_bounds__(kPfBlock) k3_pflux(Dev3 d, Box3 b, int stage, double dt) {
   __shared__ double s_pt[kPfFaces][2][4][kPfRow];
   const int lane = threadIdx.x;
@@ -426,7 +542,20 @@ __global__ void __launch_bounds__(kPfBlock) k3_pflux(Dev3 d, Box3 b, int stage, 
     const long long step = DIR == 2 ? -(long long)(d.ny + 1) * (d.nz + 1) : 1LL;
     const HInv h = hinv(DIR == 2 ? d.dx : d.dz);
     double res[2][20];  // [point]
-    tangential_side<15>(res, LvStencil{d.lv, d.nf, f, step, l, side}, h, d.recon_var, d.gas);
+    bool lin = false;
+    if constexpr (LEARNED) {
+      // point direction: z for x and y faces, -x for z faces; the five line
+      // values took the linear weights and the centre source cell's stencil
+      // along the point direction did
+      const int si = i - (side == 0 && DIR == 0), sj = j - (side == 0 && DIR == 1), sk = k - (side == 0 && DIR == 2);
+      unsigned ok = d.lin3[(DIR == 2 ? 0 : 2) * d.nc + cell3(d, si, sj, sk)];
+#pragma unroll
+      for (int q = 0; q < 5; q++) ok &= d.lin_line[(long long)side * d.nf + f + (q - 2) * step];
+      lin = ok != 0;
+    }
+    if constexpr (LEARNED) count3(d.lstat3, 2, lin);
+    if (lin) tangential_side_linear<15>(res, LvStencil{d.lv, d.nf, f, step, l, side}, h);
+    else tangential_side<15>(res, LvStencil{d.lv, d.nf, f, step, l, side}, h, d.recon_var, d.gas);
     int red = (reduce3(res[0], d.gas) || reduce3(res[1], d.gas)) ? 1 : 0;
     red |= __shfl_xor(red, 1);
     if (red) {
@@ -534,15 +663,19 @@ __global__ void __launch_bounds__(kBlock3) k3_flux_points(Gas3 g, long long n, c
 
 // the device pass sees no launcher, so the direction templates are
 // instantiated here explicitly
-template __global__ void k3_face<0>(Dev3, Box3);
-template __global__ void k3_face<1>(Dev3, Box3);
-template __global__ void k3_face<2>(Dev3, Box3);
-template __global__ void k3_line<0>(Dev3, Box3);
-template __global__ void k3_line<1>(Dev3, Box3);
-template __global__ void k3_line<2>(Dev3, Box3);
-template __global__ void k3_pflux<0>(Dev3, Box3, int, double);
-template __global__ void k3_pflux<1>(Dev3, Box3, int, double);
-template __global__ void k3_pflux<2>(Dev3, Box3, int, double);
+#define GKS3_INST(L)                                             \
+  template __global__ void k3_face<0, L>(Dev3, Box3);            \
+  template __global__ void k3_face<1, L>(Dev3, Box3);            \
+  template __global__ void k3_face<2, L>(Dev3, Box3);            \
+  template __global__ void k3_line<0, L>(Dev3, Box3);            \
+  template __global__ void k3_line<1, L>(Dev3, Box3);            \
+  template __global__ void k3_line<2, L>(Dev3, Box3);            \
+  template __global__ void k3_pflux<0, L>(Dev3, Box3, int, double); \
+  template __global__ void k3_pflux<1, L>(Dev3, Box3, int, double); \
+  template __global__ void k3_pflux<2, L>(Dev3, Box3, int, double);
+GKS3_INST(false)
+GKS3_INST(true)
+#undef GKS3_INST
 
 // ------------------------------------------------------------ launchers
 #if !defined(__HIP_DEVICE_COMPILE__)
@@ -604,27 +737,45 @@ static hipError_t launch_face(const Dev3& d, int dir, hipStream_t s) {
   const Box3 b = face_box(d, dir);
   if (box_count(b) == 0) return hipSuccess;
   const unsigned nb = blocks_for(box_count(b));
-  if (dir == 0) k3_face<0><<<nb, kBlock3, 0, s>>>(d, b);
-  else if (dir == 1) k3_face<1><<<nb, kBlock3, 0, s>>>(d, b);
-  else k3_face<2><<<nb, kBlock3, 0, s>>>(d, b);
+  if (d.mlp) {
+    if (dir == 0) k3_face<0, true><<<nb, kBlock3, 0, s>>>(d, b);
+    else if (dir == 1) k3_face<1, true><<<nb, kBlock3, 0, s>>>(d, b);
+    else k3_face<2, true><<<nb, kBlock3, 0, s>>>(d, b);
+  } else {
+    if (dir == 0) k3_face<0, false><<<nb, kBlock3, 0, s>>>(d, b);
+    else if (dir == 1) k3_face<1, false><<<nb, kBlock3, 0, s>>>(d, b);
+    else k3_face<2, false><<<nb, kBlock3, 0, s>>>(d, b);
+  }
   return hipGetLastError();
 }
 static hipError_t launch_line(const Dev3& d, int dir, hipStream_t s) {
   const Box3 b = line_box(d, dir);
   if (box_count(b) == 0) return hipSuccess;
   const unsigned nb = blocks_for(2 * box_count(b));
-  if (dir == 0) k3_line<0><<<nb, kBlock3, 0, s>>>(d, b);
-  else if (dir == 1) k3_line<1><<<nb, kBlock3, 0, s>>>(d, b);
-  else k3_line<2><<<nb, kBlock3, 0, s>>>(d, b);
+  if (d.mlp) {
+    if (dir == 0) k3_line<0, true><<<nb, kBlock3, 0, s>>>(d, b);
+    else if (dir == 1) k3_line<1, true><<<nb, kBlock3, 0, s>>>(d, b);
+    else k3_line<2, true><<<nb, kBlock3, 0, s>>>(d, b);
+  } else {
+    if (dir == 0) k3_line<0, false><<<nb, kBlock3, 0, s>>>(d, b);
+    else if (dir == 1) k3_line<1, false><<<nb, kBlock3, 0, s>>>(d, b);
+    else k3_line<2, false><<<nb, kBlock3, 0, s>>>(d, b);
+  }
   return hipGetLastError();
 }
 static hipError_t launch_flux(const Dev3& d, int dir, int stage, double dt, hipStream_t s) {
   const Box3 b = flux_box(d, dir);
   if (box_count(b) == 0) return hipSuccess;
   const unsigned nb = (unsigned)((box_count(b) + kPfFaces - 1) / kPfFaces);
-  if (dir == 0) k3_pflux<0><<<nb, kPfBlock, 0, s>>>(d, b, stage, dt);
-  else if (dir == 1) k3_pflux<1><<<nb, kPfBlock, 0, s>>>(d, b, stage, dt);
-  else k3_pflux<2><<<nb, kPfBlock, 0, s>>>(d, b, stage, dt);
+  if (d.mlp) {
+    if (dir == 0) k3_pflux<0, true><<<nb, kPfBlock, 0, s>>>(d, b, stage, dt);
+    else if (dir == 1) k3_pflux<1, true><<<nb, kPfBlock, 0, s>>>(d, b, stage, dt);
+    else k3_pflux<2, true><<<nb, kPfBlock, 0, s>>>(d, b, stage, dt);
+  } else {
+    if (dir == 0) k3_pflux<0, false><<<nb, kPfBlock, 0, s>>>(d, b, stage, dt);
+    else if (dir == 1) k3_pflux<1, false><<<nb, kPfBlock, 0, s>>>(d, b, stage, dt);
+    else k3_pflux<2, false><<<nb, kPfBlock, 0, s>>>(d, b, stage, dt);
+  }
   return hipGetLastError();
 }
 static hipError_t launch_update(const Dev3& d, hipStream_t s) {
@@ -638,8 +789,17 @@ static hipError_t launch_flux_points(const Gas3& g, long long n, const double* L
   return hipGetLastError();
 }
 
-extern const Launch3dTable table3d = {launch_bc,   launch_bc_side, launch_primvar, launch_copy_old,   launch_face,
-                                      launch_line, launch_flux,    launch_update,  launch_flux_points};
+// every cell any direction's face values read: [G-2, n-G+2) on each axis
+static hipError_t launch_classify(const Dev3& d, hipStream_t s) {
+  const int G = d.G;
+  const Box3 b = make_box(G - 2, d.nx - G + 2, G - 2, d.ny - G + 2, G - 2, d.nz - G + 2);
+  if (d.mlp && box_count(b) > 0) k3_classify<<<blocks_for(box_count(b)), kBlock3, 0, s>>>(d, b);
+  return hipGetLastError();
+}
+
+extern const Launch3dTable table3d = {launch_bc,     launch_bc_side,     launch_primvar, launch_copy_old,
+                                      launch_face,   launch_line,        launch_flux,    launch_update,
+                                      launch_flux_points, launch_classify};
 #endif
 
 }  // namespace GKS_VARIANT

@@ -25,7 +25,16 @@ struct Dev3 {
   double* pc[3];  // [5][nf] per direction: the stage-0 share of stage 1
   double* part;   // [3][kParts3] partials of the primitive-variable sweep
   double* stat;   // [4]: maximum speed, minimum density, blow-up flag
+  // learned path (gks3d_set_classifier; null mlp: plain WENO5-AO)
+  const float* mlp;             // classifier parameters (gks_learn.h layout)
+  const unsigned short* mlpw;   // its weights in the MFMA operand layout
+  float mlp_thr;                // logit below which a stencil takes the linear weights
+  unsigned char* lin3;          // [3][nc]: the cell's x / y / z stencil took them (k3_classify)
+  unsigned char* lin_line;      // [2 side][nf]: a face side's line values took them (direction in work)
+  unsigned long long* lstat3;   // [3 kinds][256 slots][linear, total]: face, line, point reconstructions
 };
+// slots of lstat3 per kind (the adds of different waves spread over them)
+constexpr int kStat3Slots = 256;
 
 struct Launch3dTable {
   hipError_t (*bc)(const Dev3&, const int* bc6, hipStream_t);
@@ -39,6 +48,8 @@ struct Launch3dTable {
   hipError_t (*update)(const Dev3&, hipStream_t);
   hipError_t (*flux_points)(const gksd3::Gas3&, long long n, const double* L, const double* R, double dt,
                             double* C, double* F, double* D, hipStream_t);
+  // learned path: every cell's three stencils through the classifier
+  hipError_t (*classify)(const Dev3&, hipStream_t);
 };
 
 namespace ieee { extern const Launch3dTable table3d; }

@@ -21,7 +21,7 @@ import time
 import numpy as np
 
 from . import GksError, Solver2D, lib
-from .params import BC_FREE, planar_shock_params
+from .params import BC_FREE, Params, planar_shock_params
 from .problems import riemann_prim, riemann_states
 
 NPARAM = 16 * 32 + 32 + 32 * 32 + 32 + 32 + 1
@@ -42,6 +42,7 @@ _SIGS = {
     "gks2d_set_classifier": (ctypes.c_int, [_P, _F, ctypes.c_float]),
     "gks2d_classifier_stats": (ctypes.c_int, [_P, ctypes.POINTER(_I64), ctypes.POINTER(_I64)]),
     "gks2d_classifier_counts": (ctypes.c_int, [_P, ctypes.POINTER(_I64)]),
+    "gks2d_linear_weights_points": (ctypes.c_int, [ctypes.POINTER(Params), _I64, _D, ctypes.c_double, _D]),
     "gks_mlp_last_error": (ctypes.c_char_p, []),
     "gks2d_classifier_options": (ctypes.c_int, [_P, ctypes.c_int32]),
     "gks2d_flux_points_smooth": (ctypes.c_int, [ctypes.c_void_p, _I64, _D, _D, ctypes.c_double, _D, _D]),
@@ -206,6 +207,17 @@ def flux_points_smooth(p, left, right, dt):
     _check_ctx(L().gks2d_flux_points_smooth(ctypes.byref(p), n, left.ctypes.data_as(_D), right.ctypes.data_as(_D),
                                             dt, f.ctypes.data_as(_D), d.ctypes.data_as(_D)))
     return f, d
+
+
+def linear_points(p, stencils, h):
+    """gks2d_linear_weights_points: the linear-weight reconstructions of n
+    5-cell stencils on the GPU -> n x 8 (left face value, derivative; right
+    face value, derivative; Gauss point 1 value, derivative; point 2)."""
+    W = np.ascontiguousarray(stencils, dtype=np.float64)
+    out = np.zeros((W.shape[0], 8))
+    _check_ctx(L().gks2d_linear_weights_points(ctypes.byref(p), W.shape[0], W.ctypes.data_as(_D), float(h),
+                                               out.ctypes.data_as(_D)))
+    return out
 
 
 def classifier_stats(solver):

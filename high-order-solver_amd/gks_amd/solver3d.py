@@ -41,9 +41,11 @@ SIGS3 = {
     "gks3d_get_timing": (ctypes.c_int, [_P, _D, ctypes.POINTER(ctypes.c_int64)]),
     "gks3d_flux_points": (ctypes.c_int, [ctypes.POINTER(Params3D), _I64, _D, _D, ctypes.c_double, _D, _D, _D]),
     "gks3d_last_error": (ctypes.c_char_p, []),
+    "gks3d_set_classifier": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_float), ctypes.c_float]),
+    "gks3d_classifier_counts": (ctypes.c_int, [_P, ctypes.POINTER(_I64)]),
 }
 
-PHASES3 = ["bc", "primvar", "face", "line", "flux", "update", "copy_old"]
+PHASES3 = ["bc", "primvar", "face", "line", "flux", "update", "copy_old", "classify"]
 
 _lib3 = None
 
@@ -167,6 +169,22 @@ class Solver3D:
         self.t, self.step_count = t.value, s.value
         _check(rc)
         return n
+
+    def set_classifier(self, params, threshold):
+        """The learned reconstruction (gks3d_set_classifier): the classifier
+        of gks_learn.h (params: its float vector, None disables it)."""
+        if params is None:
+            _check(lib3().gks3d_set_classifier(self.h, None, 0.0))
+            return
+        P = np.ascontiguousarray(params, dtype=np.float32)
+        _check(lib3().gks3d_set_classifier(self.h, P.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), float(threshold)))
+
+    def classifier_counts(self):
+        """(linear, total) reconstructions since the classifier was set, per
+        kind: face values, line values, point values."""
+        c = (_I64 * 6)()
+        _check(lib3().gks3d_classifier_counts(self.h, c))
+        return {"face": (int(c[0]), int(c[1])), "line": (int(c[2]), int(c[3])), "point": (int(c[4]), int(c[5]))}
 
     def sync(self):
         _check(lib3().gks3d_sync(self.h))

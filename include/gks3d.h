@@ -97,7 +97,9 @@ int gks3d_update(gks3d_ctx* ctx, int32_t stage);
 enum {
   GKS3_PHASE_BC = 0, GKS3_PHASE_PRIMVAR, GKS3_PHASE_FACE, GKS3_PHASE_LINE,
   GKS3_PHASE_FLUX,  /* point values + GKS3D, one launch per direction */
-  GKS3_PHASE_UPDATE, GKS3_PHASE_COPY_OLD, GKS3_NPHASE
+  GKS3_PHASE_UPDATE, GKS3_PHASE_COPY_OLD,
+  GKS3_PHASE_CLASSIFY,  /* learned path: the classifier over every cell, once per stage */
+  GKS3_NPHASE
 };
 int gks3d_timing(gks3d_ctx* ctx, int enable);
 /* Times only the phases whose bit (1 << GKS3_PHASE_x) is set in mask (0:
@@ -110,6 +112,22 @@ int gks3d_get_timing(gks3d_ctx* ctx, double* ms, int64_t* launches);
  * n x 20 collision state and slopes; f, derf: n x 5. */
 int gks3d_flux_points(const gks3d_params* p, int64_t n, const double* left, const double* right, double dt,
                       double* center, double* f, double* derf);
+/* The learned reconstruction in 3D (no counterpart in the reference): the
+ * troubled-interface classifier of gks_learn.h (params: GKS_MLP_NPARAM
+ * floats; the same network and stencil features as the 2D path) runs once
+ * per stage on every cell's x, y and z stencils (BF16 MFMA inference, FP32
+ * accumulation).  A face-value reconstruction takes the linear weights (the
+ * fifth-order candidate) where its cell's stencil across the face has
+ * logit < logit_threshold; a line (point) reconstruction takes them where
+ * its five source values did and the centre source cell's stencil along
+ * the line (point) direction did too; the order-reduction test is kept and
+ * GKS3D is unchanged.  params == NULL disables it. */
+int gks3d_set_classifier(gks3d_ctx* ctx, const float* params, float logit_threshold);
+/* Counters since the classifier was set: counts[0] / [1] face-value
+ * reconstructions that took the linear weights / all of them; [2] / [3]
+ * line values (per face side); [4] / [5] point values (per face side and
+ * line). */
+int gks3d_classifier_counts(gks3d_ctx* ctx, int64_t counts[6]);
 const char* gks3d_last_error(void);
 
 #ifdef __cplusplus

@@ -78,6 +78,16 @@ int gks2d_classifier_options(gks2d_ctx* ctx, int32_t smooth_flux);
 int gks2d_flux_points_smooth(const gks2d_params* p, int64_t n, const double* left, const double* right, double dt,
                              double* f, double* derf);
 
+/* The linear-weight reconstructions the learned paths use (2D and 3D), for
+ * n 5-cell stencils (n x 5 cell averages, host): per stencil 8 doubles, the
+ * fifth-order candidate's value and derivative at the centre cell's left
+ * face, at its right face, then value and derivative at its two Gauss
+ * points (-1/(2 sqrt 3) and +1/(2 sqrt 3) cell widths from the centre; the
+ * reference's weno_5th_ao_with_df_left/right / _2gauss with the AO weights
+ * at their linear values, reconstruction.cpp:480-657, 2742-2838), in the
+ * build p->fast_math selects.  h: the cell width. */
+int gks2d_linear_weights_points(const gks2d_params* p, int64_t n, const double* stencils, double h, double* out);
+
 /* Reconstructions that took the linear weights / all reconstructions, over
  * the calls since the classifier was set (normal and tangential). */
 int gks2d_classifier_stats(gks2d_ctx* ctx, int64_t* linear, int64_t* total);

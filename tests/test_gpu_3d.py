@@ -190,12 +190,13 @@ def test_phase_timing_counts_every_launch():
     tim = s.get_timing()
     s.close()
     # the point values run inside the flux launch (one LDS tile per workgroup)
-    per_step = {"bc": 2, "primvar": 2, "face": 6, "line": 6, "flux": 6, "update": 2, "copy_old": 1}
+    # (the classifier launch runs only on the learned path)
+    per_step = {"bc": 2, "primvar": 2, "face": 6, "line": 6, "flux": 6, "update": 2, "copy_old": 1, "classify": 0}
     say("[3d timing]", {k: round(v[0], 3) for k, v in tim.items()})
     assert set(tim) == set(per_step)
     for ph, m in per_step.items():
         assert tim[ph][1] == m * n, ph
-        assert tim[ph][0] > 0.0, ph
+        assert (tim[ph][0] > 0.0) == (m > 0), ph
 
 
 def test_phase_mask_times_only_the_named_phases():
@@ -249,3 +250,80 @@ def test_sweep_by_sweep_loop_equals_the_device_run():
     assert np.array_equal(ua[:, G:-G, G:-G, G:-G], ub[:, G:-G, G:-G, G:-G])
     a.close()
     b.close()
+
+
+def _tgv_run(n, steps, fast, classifier=None, tstop=100.0):
+    from gks_amd.solver3d import Solver3D
+    p = tgv_params(n, fast_math=fast)
+    s = Solver3D(p)
+    s.init_convar(tgv_convar(p))
+    if classifier is not None:
+        s.set_classifier(*classifier)
+    s.run([BC_PERIODIC] * 6, tstop, steps)
+    out = (s.get_convar(), s.t, s.step_count, s.classifier_counts() if classifier is not None else None)
+    s.close()
+    return out
+
+
+def test_learned_3d_with_a_silent_classifier_equals_plain():
+    """The learned 3D path (classify launch, flags, the LEARNED kernels) with
+    a threshold no logit reaches: every reconstruction keeps WENO5-AO and
+    the run is the plain run bit for bit (IEEE build)."""
+    from gks_amd import learn
+    P, _, _ = learn.load()
+    u0, t0, k0, _ = _tgv_run(12, 6, 0)
+    u1, t1, k1, cnt = _tgv_run(12, 6, 0, classifier=(P, -1e30))
+    say(f"[3d learned, silent] counts {cnt}")
+    assert k0 == k1 and t0 == t1 and np.array_equal(u0, u1)
+    assert all(c[0] == 0 and c[1] > 0 for c in cnt.values())
+
+
+@pytest.mark.parametrize("fast", [0, 1])
+def test_learned_3d_all_linear_stays_close_to_plain(fast):
+    """A threshold every logit passes: every face, line and point
+    reconstruction of the smooth Taylor-Green field takes the linear
+    weights; the state after 10 steps differs from the WENO5-AO run by the
+    nonlinear weights' small departure from linear, and not by more."""
+    from gks_amd import learn
+    P, _, _ = learn.load()
+    u0, t0, k0, _ = _tgv_run(16, 10, fast)
+    u1, t1, k1, cnt = _tgv_run(16, 10, fast, classifier=(P, 1e30))
+    err = np.max(np.abs(u1 - u0)) / np.max(np.abs(u0))
+    say(f"[3d learned, all linear] fast {fast}: counts {cnt}, max rel diff from plain {err:.2e}")
+    assert all(c[0] == c[1] > 0 for c in cnt.values())
+    assert k0 == k1 and 0.0 < err < 1e-3
+
+
+def test_learned_3d_dissipation_rate_follows_plain():
+    """The packaged classifier on the Taylor-Green cube (32^3, to t = 2, fast
+    build): the kinetic-energy dissipation rate -dE_k/dt (central
+    differences of E_k sampled every 10 steps) of the learned run stays
+    within 5% of the plain run's peak rate."""
+    from gks_amd import learn
+    from gks_amd.solver3d import Solver3D
+    P, thr, _ = learn.load()
+    series = {}
+    for tag, cls in (("plain", None), ("learned", (P, thr))):
+        p = tgv_params(32, fast_math=1)
+        s = Solver3D(p)
+        s.init_convar(tgv_convar(p))
+        if cls:
+            s.set_classifier(*cls)
+        ts, es = [0.0], [kinetic_energy(s.get_convar(), p)]
+        while s.t < 2.0:
+            if s.run([BC_PERIODIC] * 6, 2.0, 10) == 0:
+                break
+            ts.append(s.t)
+            es.append(kinetic_energy(s.get_convar(), p))
+        if cls:
+            say(f"[3d learned] counts {s.classifier_counts()}")
+        s.close()
+        series[tag] = (np.array(ts), np.array(es))
+    (tp, ep), (tl, el) = series["plain"], series["learned"]
+    # each run's dt follows its own state: the sample times differ slightly
+    assert len(tp) == len(tl) and np.allclose(tp, tl, rtol=1e-4)
+    dp = -np.gradient(ep, tp)
+    dl = np.interp(tp, tl, -np.gradient(el, tl))
+    rel = np.max(np.abs(dl - dp)) / np.max(dp)
+    say(f"[3d learned] E_k plain {ep[-1]:.6e} learned {el[-1]:.6e}; max |d eps| / max eps {rel:.3e}")
+    assert rel < 0.05

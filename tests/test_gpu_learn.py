@@ -230,3 +230,34 @@ def test_smooth_flux_leaf_matches_oracle(fast):
     eg = np.max(np.abs(fe - fg) / np.abs(fg).max(axis=1, keepdims=True))
     say(f"[learn] smooth flux, fast {fast}: vs numpy f {ef:.2e} derf {ed:.2e}; vs GKS2D at equal sides {eg:.2e}")
     assert ef < 1e-12 and ed < 1e-10 and eg < 1e-12
+
+
+@pytest.mark.parametrize("fast", [0, 1])
+def test_linear_weight_reconstructions_are_exact_for_quartics(fast):
+    """gks2d_linear_weights_points (the linear-weight reconstructions of the
+    learned 2D and 3D paths): on cell averages of random quartics the
+    fifth-order candidate is exact, so the values and derivatives at the
+    centre cell's faces (-h/2, +h/2) and Gauss points (-+h/(2 sqrt 3))
+    equal the polynomial's; the fast build (compile-time combined
+    coefficients, reciprocal multiplication) agrees with the IEEE one to
+    1e-13 of the values' scale."""
+    from gks_amd.learn import linear_points
+    rng = np.random.default_rng(5)
+    n, h = 4000, 0.37
+    c = rng.normal(0, 1, (n, 5))  # P(x) = sum c_k x^k
+    Pv = lambda x: sum(c[:, k] * x ** k for k in range(5))
+    dP = lambda x: sum(k * c[:, k] * x ** (k - 1) for k in range(1, 5))
+    Q = lambda x: sum(c[:, k] * x ** (k + 1) / (k + 1) for k in range(5))
+    edges = (np.arange(6) - 2.5) * h
+    W = np.stack([(Q(edges[s + 1]) - Q(edges[s])) / h for s in range(5)], axis=1)
+    p = planar_shock_params(16, fast_math=fast)
+    out = linear_points(p, W, h)
+    g = h / (2 * np.sqrt(3.0))
+    exact = np.stack([Pv(-h / 2), dP(-h / 2), Pv(h / 2), dP(h / 2), Pv(-g), dP(-g), Pv(g), dP(g)], axis=1)
+    scale = np.max(np.abs(W), axis=1, keepdims=True) / h
+    err = np.max(np.abs(out - exact) / scale)
+    print(f"[linear weights] fast {fast}: max error / scale {err:.2e}", flush=True)
+    assert err < 1e-12
+    if fast:
+        ref = linear_points(planar_shock_params(16, fast_math=0), W, h)
+        assert np.max(np.abs(out - ref) / scale) < 1e-13
