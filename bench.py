@@ -422,6 +422,88 @@ def bench_tgv3d(args):
     print(json.dumps(out), flush=True)
 
 
+def bench_tgv_dissipation(args):
+    """The Taylor-Green vortex (Re 1600, Mach 0.1, periodic [0, 2 pi]^3) at
+    n^3 to t_end by the plain path and by the learned path (the packaged
+    troubled-interface classifier on every cell's x / y / z stencils):
+    throughput of each (whole run, state in HBM; the kinetic-energy samples
+    are taken between timed stretches), the kinetic energy E_k every
+    --ek-every steps, the dissipation rate -dE_k/dt (central differences)
+    and its largest difference from the plain run's relative to the plain
+    peak rate."""
+    import numpy as np
+
+    from gks_amd import learn
+    from gks_amd.params import BC_PERIODIC, tgv_params
+    from gks_amd.problems import kinetic_energy, tgv_convar
+    from gks_amd.solver3d import Solver3D
+
+    def log(m):
+        print(m, file=sys.stderr, flush=True)
+
+    n, tend = args.n3d, args.t3d
+    fast = 1 if args.variant == "fast" else 0
+    P, thr, _ = learn.load(args.classifier or learn.DEFAULT_PARAMS)
+    bc = [BC_PERIODIC] * 6
+    legs = {}
+    for tag, cls in (("plain", None), ("learned", (P, thr))):
+        p = tgv_params(n, fast_math=fast)
+        s = Solver3D(p)
+        s.init_convar(tgv_convar(p, quad=2))
+        if cls:
+            s.set_classifier(*cls)
+        ts, es = [0.0], [kinetic_energy(s.get_convar(), p)]
+        wall, steps = 0.0, 0
+        while s.t < tend:
+            s.sync()
+            t0 = time.perf_counter()
+            k = s.run(bc, tend, args.ek_every)
+            s.sync()
+            wall += time.perf_counter() - t0
+            steps += k
+            if k == 0:
+                break
+            ts.append(s.t)
+            es.append(kinetic_energy(s.get_convar(), p))
+            if len(ts) % 20 == 0:
+                log(f"tgv {tag} {n}^3: t={s.t:.3f} steps={steps} E_k={es[-1]:.6e}")
+        legs[tag] = {"t": np.array(ts), "ek": np.array(es), "wall": wall, "steps": steps,
+                     "counts": s.classifier_counts() if cls else None}
+        s.close()
+        log(f"tgv {tag}: {steps} steps in {wall:.1f} s, {n ** 3 * steps / wall / 1e6:.1f} Mcell-updates/s")
+    lp, ll = legs["plain"], legs["learned"]
+    eps_p = -np.gradient(lp["ek"], lp["t"])
+    eps_l = np.interp(lp["t"], ll["t"], -np.gradient(ll["ek"], ll["t"]))
+    rel = float(np.max(np.abs(eps_l - eps_p)) / np.max(eps_p))
+    vp = n ** 3 * lp["steps"] / lp["wall"] / 1e6
+    vl = n ** 3 * ll["steps"] / ll["wall"] / 1e6
+    c = ll["counts"]
+    out = {
+        "metric": "Mcell-updates/s FP64, 3D Taylor-Green WENO5-AO GKS3D S2O4: learned vs plain, kinetic-energy "
+                  "dissipation rate",
+        "value": round(vl, 3), "unit": "Mcell-updates/s", "n_gpus": 1, "steps": ll["steps"], "warmup": 0,
+        "window": f"whole runs from t=0 to t={tend} (kinetic energy sampled every {args.ek_every} steps between "
+                  "timed stretches)",
+        "ms_per_step": round(ll["wall"] / ll["steps"] * 1e3, 3), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"Taylor-Green vortex Re 1600 Mach 0.1, periodic [0,2pi]^3 at {n}^3 to t={tend}, "
+                               f"WENO5-AO characteristic, 2x2 Gauss points, GKS3D gks2nd Navier-Stokes, S2O4, CFL "
+                               f"0.5; learned path (packaged classifier, BF16 MFMA inference) against plain "
+                               f"({args.variant} kernels)", "n": n, "kernels": args.variant},
+        "plain": {"value": round(vp, 3), "steps": lp["steps"], "wall_s": round(lp["wall"], 3)},
+        "speedup_learned_vs_plain": round(vl / vp, 4),
+        "linear_share": {k: round(v[0] / max(v[1], 1), 4) for k, v in c.items()},
+        "dissipation": {"max_rel_diff": round(rel, 6),
+                        "peak_plain": {"t": round(float(lp["t"][np.argmax(eps_p)]), 4), "eps": float(np.max(eps_p))},
+                        "peak_learned": {"t": round(float(lp["t"][np.argmax(eps_l)]), 4),
+                                         "eps": float(np.max(eps_l))},
+                        "t": [round(float(x), 5) for x in lp["t"]],
+                        "eps_plain": [float(x) for x in eps_p], "eps_learned": [float(x) for x in eps_l]},
+        "roofline": None, "cpu_baseline": None,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def bench_window(args):
     """The K-step window (--window): --steps timed steps after --warmup
     untimed ones, from t=0, of one path (--path)."""
@@ -766,10 +848,13 @@ def main():
     ap.add_argument("--path", choices=["plain", "learned"], default="plain",
                     help="learned: the trained troubled-interface classifier picks linear weights for smooth stencils")
     ap.add_argument("--classifier", default=None, help="trained classifier (.npz); default: the packaged one")
-    ap.add_argument("--workload", choices=["single", "batch", "tgv3d"], default="single",
+    ap.add_argument("--workload", choices=["single", "batch", "tgv3d", "tgv3d-dissipation"], default="single",
                     help="batch: --batch uneven problems stepped together, against running them one by one; "
-                         "tgv3d: the 3D Taylor-Green cube at --n3d^3")
+                         "tgv3d: the 3D Taylor-Green cube at --n3d^3; tgv3d-dissipation: plain vs learned to --t3d "
+                         "with the kinetic-energy dissipation rate")
     ap.add_argument("--n3d", type=int, default=256)
+    ap.add_argument("--t3d", type=float, default=10.0, help="end time of --workload tgv3d-dissipation")
+    ap.add_argument("--ek-every", type=int, default=50, help="steps between kinetic-energy samples")
     ap.add_argument("--cpu-n3d", type=int, default=48)
     ap.add_argument("--batch", type=int, default=64)
     ap.add_argument("--window", action="store_true",
@@ -794,6 +879,9 @@ def main():
         return
     if args.workload == "tgv3d":
         bench_tgv3d(args)
+        return
+    if args.workload == "tgv3d-dissipation":
+        bench_tgv_dissipation(args)
         return
     if args.window:
         bench_window(args)
