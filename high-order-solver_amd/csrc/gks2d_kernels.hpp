@@ -922,10 +922,10 @@ k_flux(Dev d, int stage, double dt) {
 // the flux of k_flux from those states.  The Gauss-point states never go to
 // HBM (48 doubles per face each way).
 //
-// The flux point needs about 280 registers per lane, which would leave one
-// wave per SIMD; asking for two waves spills about 24 of them to scratch
-// (a few hundred bytes per lane, cache resident) and the second wave hides
-// the latency: 1.54 -> 1.16 ms per launch at 2048^2.
+// A workgroup of 128 lanes takes 64 faces and asks for two waves per SIMD
+// (at most 256 VGPRs): the fast build fits in about 250 VGPRs without
+// scratch, about 0.88-0.95 ms per launch at 2048^2 (one wave per SIMD took
+// 1.54 ms); the IEEE build spills a few hundred bytes per lane.
 constexpr int kTanFlux = 64;
 // LISTED (learned path only): the tile's faces are list[blk * TF ...], nlist
 // entries in all, instead of the consecutive faces blk * TF ...; they are

@@ -66,44 +66,62 @@ __global__ void __launch_bounds__(kB) k_fwd_bwd(const float* __restrict__ P, int
   }
 }
 
-// gradient of parameter p: a fixed-order sum over the batch (so training is
-// reproducible), accumulated in double
+// the fixed tree over a workgroup's kB partial sums (lane 0 ends with the
+// total); the pairing depends only on kB
+__device__ __forceinline__ double block_sum(double acc, double* s_acc) {
+  const int t = threadIdx.x;
+  s_acc[t] = acc;
+  __syncthreads();
+  for (int w = kB / 2; w > 0; w >>= 1) {
+    if (t < w) s_acc[t] += s_acc[t + w];
+    __syncthreads();
+  }
+  return s_acc[0];
+}
+
+// gradient of parameter p = blockIdx.x over the mini-batch: lane t sums the
+// samples t, t + kB, ... in order, then the fixed tree over the lanes, in
+// double; the summation order depends only on nb, so training is
+// reproducible run to run
 __global__ void __launch_bounds__(kB) k_reduce(int nb, const int* __restrict__ idx, const float* __restrict__ X,
                                                const float* __restrict__ H1, const float* __restrict__ H2,
                                                const float* __restrict__ G1, const float* __restrict__ G2,
                                                const float* __restrict__ GZ, float* __restrict__ grad) {
-  const int p = blockIdx.x * kB + threadIdx.x;
-  if (p >= NP) return;
+  __shared__ double s_acc[kB];
+  const int p = blockIdx.x;
   double acc = 0.0;
   if (p < OB1) {  // W1[k][j] = sum x_k g1_j
     const int k = p / NH, j = p % NH;
-    for (int b = 0; b < nb; b++) {
+    for (int b = threadIdx.x; b < nb; b += kB) {
       const long long s = idx ? idx[b] : b;
       acc += (double)X[s * NIN + k] * G1[(long long)j * nb + b];
     }
   } else if (p < OW2) {
     const int j = p - OB1;
-    for (int b = 0; b < nb; b++) acc += G1[(long long)j * nb + b];
+    for (int b = threadIdx.x; b < nb; b += kB) acc += G1[(long long)j * nb + b];
   } else if (p < OB2) {  // W2[k][j] = sum h1_k g2_j
     const int k = (p - OW2) / NH, j = (p - OW2) % NH;
-    for (int b = 0; b < nb; b++) acc += (double)H1[(long long)k * nb + b] * G2[(long long)j * nb + b];
+    for (int b = threadIdx.x; b < nb; b += kB) acc += (double)H1[(long long)k * nb + b] * G2[(long long)j * nb + b];
   } else if (p < OW3) {
     const int j = p - OB2;
-    for (int b = 0; b < nb; b++) acc += G2[(long long)j * nb + b];
+    for (int b = threadIdx.x; b < nb; b += kB) acc += G2[(long long)j * nb + b];
   } else if (p < OB3) {  // W3[k] = sum h2_k gz
     const int k = p - OW3;
-    for (int b = 0; b < nb; b++) acc += (double)H2[(long long)k * nb + b] * GZ[b];
+    for (int b = threadIdx.x; b < nb; b += kB) acc += (double)H2[(long long)k * nb + b] * GZ[b];
   } else {
-    for (int b = 0; b < nb; b++) acc += GZ[b];
+    for (int b = threadIdx.x; b < nb; b += kB) acc += GZ[b];
   }
-  grad[p] = (float)acc;
+  const double tot = block_sum(acc, s_acc);
+  if (threadIdx.x == 0) grad[p] = (float)tot;
 }
 
+// the batch loss: one workgroup, the same fixed-order scheme
 __global__ void __launch_bounds__(kB) k_sum(int n, const float* __restrict__ v, double* __restrict__ out) {
-  if (blockIdx.x != 0 || threadIdx.x != 0) return;
+  __shared__ double s_acc[kB];
   double acc = 0.0;
-  for (int i = 0; i < n; i++) acc += v[i];
-  *out = acc;
+  for (int i = threadIdx.x; i < n; i += kB) acc += v[i];
+  const double tot = block_sum(acc, s_acc);
+  if (threadIdx.x == 0) *out = tot;
 }
 
 __global__ void __launch_bounds__(kB) k_adam(float* __restrict__ P, const float* __restrict__ g,
@@ -124,8 +142,7 @@ hipError_t fwd_bwd_reduce(const Work& w, const float* P, int nb, const int* idx,
   const float inv_b = 1.0f / (float)nb;
   hipLaunchKernelGGL(k_fwd_bwd, dim3((nb + kB - 1) / kB), dim3(kB), 0, s, P, nb, idx, X, Y, W, inv_b, w.h1, w.h2,
                      w.g1, w.g2, w.gz, w.loss);
-  hipLaunchKernelGGL(k_reduce, dim3((NP + kB - 1) / kB), dim3(kB), 0, s, nb, idx, X, w.h1, w.h2, w.g1, w.g2, w.gz,
-                     grad);
+  hipLaunchKernelGGL(k_reduce, dim3(NP), dim3(kB), 0, s, nb, idx, X, w.h1, w.h2, w.g1, w.g2, w.gz, grad);
   return hipGetLastError();
 }
 hipError_t loss_sum(const Work& w, int nb, double* out, hipStream_t s) {
