@@ -497,6 +497,9 @@ __device__ __forceinline__ void store_normal_faces(const Dev& d, int i, int j, c
 template <int DIR> __device__ __forceinline__ void normal_learned_body(const Dev& d, unsigned blk) {
   __shared__ __align__(16) short xs[kBlock * 16];
   __shared__ float ls[kBlock];
+  __shared__ MlpLds s_mlp;
+  // the network's operands load while the stencils do
+  mlp_stage(s_mlp, d.mlp, d.mlpw);
   const long long t = (long long)blk * kBlock + threadIdx.x;
   const bool valid = t < normal_count<DIR>(d);
   int i, j;
@@ -513,7 +516,8 @@ template <int DIR> __device__ __forceinline__ void normal_learned_body(const Dev
     for (int k = 0; k < 16; k++) x[k] = 0.0f;
   }
   const int wv = threadIdx.x >> 6;
-  const float z = mlp_wave_logit(x, d.mlp, d.mlpw, xs + wv * 64 * 16, ls + wv * 64);
+  __syncthreads();
+  const float z = mlp_wave_logit(x, MlpShared{&s_mlp}, xs + wv * 64 * 16, ls + wv * 64);
   if (!valid) return;
   double amax = 0.0;
 #pragma unroll
@@ -1090,23 +1094,28 @@ __device__ __forceinline__ void tanflux_cheap_body(const Dev& d, unsigned blk, i
   const long long nf = DIR == 0 ? d.nfx : d.nfy;
   {
     const int side = hi;
-    bool lin = false;
-    if (valid) {
-      lin = tangential_linear<DIR>(d, side, i, j);
-      count_linear(d.lstat + 2 * kStatSlots, lin);
-    }
-    slin[side][tl] = lin ? 1 : 0;
-    // without the smooth-flow flux every face goes to the list
-    if (lin && d.smooth_flux) {
+    // the line values load before the rule is known, so that their latency
+    // overlaps the rule's loads (most faces pass it); without the
+    // smooth-flow flux every face goes to the list
+    const bool want = valid && d.smooth_flux;
+    double L[5][8];
+    if (want) {
       const double* line = DIR == 0 ? d.line_x : d.line_y;
-      const double h = DIR == 0 ? d.dy : d.dx, ih = DIR == 0 ? d.idy : d.idx;
-      double L[5][8];
 #pragma unroll
       for (int s = 0; s < 5; s++) {
         const long long f = DIR == 0 ? (long long)i * d.ny + (j - 2 + s) : (long long)(i + 2 - s) * (d.ny + 1) + j;
 #pragma unroll
         for (int k = 0; k < 8; k++) L[s][k] = line[(side * 8 + k) * nf + f];
       }
+    }
+    bool lin = false;
+    if (valid) {
+      lin = tangential_linear<DIR>(d, side, i, j);
+      count_linear(d.lstat + 2 * kStatSlots, lin);
+    }
+    slin[side][tl] = lin ? 1 : 0;
+    if (lin && want) {
+      const double h = DIR == 0 ? d.dy : d.dx, ih = DIR == 0 ? d.idy : d.idx;
       double gp[2][12];
 #pragma unroll
       for (int v = 0; v < 4; v++) {

@@ -197,17 +197,51 @@ GKS_DI void wave_lds_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// Where one wave reads the MLP's operands: straight from global memory
+// (P: the float parameters, biases read from it; W: the weights packed in
+// the operand layout of mlp_pack.hpp), or from a copy a workgroup staged in
+// LDS at its start (mlp_stage), so that their global latency overlaps the
+// kernel's own loads instead of following them.
+struct MlpGlobal {
+  const float* __restrict__ P;
+  const unsigned short* __restrict__ W;
+  GKS_DI bf16x4_t w(int v, int lane) const {
+    return __builtin_bit_cast(bf16x4_t, reinterpret_cast<const uint2*>(W)[v * 64 + lane]);
+  }
+  GKS_DI float b1(int i) const { return P[kMlpB1 + i]; }
+  GKS_DI float b2(int i) const { return P[kMlpB2 + i]; }
+  GKS_DI float b3() const { return P[kMlpB3]; }
+};
+struct MlpLds {
+  uint2 w[8 * 64];
+  float b[65];  // b1[32], b2[32], b3
+};
+// every thread of the workgroup calls it; a __syncthreads() must separate
+// it from the first read
+GKS_DI void mlp_stage(MlpLds& s, const float* __restrict__ P, const unsigned short* __restrict__ W) {
+  const uint2* Wv = reinterpret_cast<const uint2*>(W);
+  for (int t = threadIdx.x; t < 8 * 64; t += blockDim.x) s.w[t] = Wv[t];
+  for (int t = threadIdx.x; t < 65; t += blockDim.x)
+    s.b[t] = t < 32 ? P[kMlpB1 + t] : (t < 64 ? P[kMlpB2 + t - 32] : P[kMlpB3]);
+}
+struct MlpShared {
+  const MlpLds* s;
+  GKS_DI bf16x4_t w(int v, int lane) const { return __builtin_bit_cast(bf16x4_t, s->w[v * 64 + lane]); }
+  GKS_DI float b1(int i) const { return s->b[i]; }
+  GKS_DI float b2(int i) const { return s->b[32 + i]; }
+  GKS_DI float b3() const { return s->b[64]; }
+};
+
 // One wave evaluates the MLP for its 64 lanes' samples (x: this lane's 16
 // features) and returns this lane's logit.  Every lane of the wave must
 // call it (an inactive lane passes zeros); other waves of the workgroup are
 // not involved.  The layers run transposed, H^T = W^T X^T, on 16x16x16 BF16
 // MFMAs (FP32 accumulation): the output tile of one layer is then already
 // laid out as the B operand of the next, so only the input tile passes
-// through LDS.  P: the float parameters (biases are read from it), W: the
-// weights packed in the operand layout of mlp_pack.hpp, xs: this wave's
+// through LDS.  src: the operands (MlpGlobal or MlpShared); xs: this wave's
 // 64 x 16 staging area (BF16, 16-byte aligned), ls: its 64 logits.
-GKS_DI float mlp_wave_logit(const float* x, const float* __restrict__ P, const unsigned short* __restrict__ W,
-                            short* xs, float* ls) {
+template <class S>
+GKS_DI float mlp_wave_logit(const float* x, const S& src, short* xs, float* ls) {
   const int lane = threadIdx.x & 63, col = lane & 15, kb = lane >> 4;
   {
     uint4* dst = reinterpret_cast<uint4*>(xs + lane * 16);
@@ -217,19 +251,18 @@ GKS_DI float mlp_wave_logit(const float* x, const float* __restrict__ P, const u
                         pack_bf16x2(x[14], x[15]));
   }
   // A operands (weights, transposed), one 8-byte load each
-  const uint2* Wv = reinterpret_cast<const uint2*>(W);
   bf16x4_t a[8];
 #pragma unroll
-  for (int v = 0; v < 8; v++) a[v] = __builtin_bit_cast(bf16x4_t, Wv[v * 64 + lane]);
+  for (int v = 0; v < 8; v++) a[v] = src.w(v, lane);
   float b1v[2][4], b2v[2][4];
 #pragma unroll
   for (int m = 0; m < 2; m++)
 #pragma unroll
     for (int r = 0; r < 4; r++) {
-      b1v[m][r] = P[kMlpB1 + m * 16 + 4 * kb + r];
-      b2v[m][r] = P[kMlpB2 + m * 16 + 4 * kb + r];
+      b1v[m][r] = src.b1(m * 16 + 4 * kb + r);
+      b2v[m][r] = src.b2(m * 16 + 4 * kb + r);
     }
-  const float b3 = P[kMlpB3];
+  const float b3 = src.b3();
   wave_lds_sync();
 #pragma unroll
   for (int t = 0; t < 4; t++) {
@@ -257,6 +290,10 @@ GKS_DI float mlp_wave_logit(const float* x, const float* __restrict__ P, const u
   }
   wave_lds_sync();
   return ls[lane];
+}
+GKS_DI float mlp_wave_logit(const float* x, const float* __restrict__ P, const unsigned short* __restrict__ W,
+                            short* xs, float* ls) {
+  return mlp_wave_logit(x, MlpGlobal{P, W}, xs, ls);
 }
 
 }  // namespace GKS_VARIANT

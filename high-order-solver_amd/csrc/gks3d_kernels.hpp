@@ -212,6 +212,8 @@ __device__ __forceinline__ void count3(unsigned long long* stat, int kind, bool 
 __global__ void __launch_bounds__(kBlock3) k3_classify(Dev3 d, Box3 b) {
   __shared__ __align__(16) short xs[kBlock3 * 16];
   __shared__ float ls[kBlock3];
+  __shared__ gksd::MlpLds s_mlp;
+  gksd::mlp_stage(s_mlp, d.mlp, d.mlpw);  // loads while the stencils do
   const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   const bool valid = t < b.count();
   int i, j, k;
@@ -240,7 +242,8 @@ __global__ void __launch_bounds__(kBlock3) k3_classify(Dev3 d, Box3 b) {
 #pragma unroll
       for (int m = 0; m < 16; m++) x[m] = 0.0f;
     }
-    const float z = gksd::mlp_wave_logit(x, d.mlp, d.mlpw, xs + wv * 64 * 16, ls + wv * 64);
+    if (dir == 0) __syncthreads();
+    const float z = gksd::mlp_wave_logit(x, gksd::MlpShared{&s_mlp}, xs + wv * 64 * 16, ls + wv * 64);
     if (ok) d.lin3[dir * d.nc + c] = z < d.mlp_thr ? 1 : 0;
   }
 }
