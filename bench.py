@@ -60,7 +60,7 @@ PEAK_FP64_TFLOPS = 78.6  # MI355X FP64 (vector and matrix)
 TRAFFIC_2048_FAST = {"flux_x": 1.131e9, "flux_y": 1.139e9}
 # FP64 operations the fast flux kernel executes per launch at 2048^2, from
 # rocprofv3 --pmc SQ_INSTS_VALU_{ADD,MUL,FMA,TRANS}_F64 (x64 lanes, FMA = 2)
-# (profiles/gks2d_riemann2048_fast_kernel_trace.md); below the algorithmic
+# (profiles/gks2d_riemann2048_fast_hbm_traffic.md); below the algorithmic
 # count: reciprocal multiplication, FMA contraction, whole-space tables
 EXECUTED_FLOPS_2048_FAST = {"flux_x": 39.73e9, "flux_y": 39.74e9}
 PEAK_HBM_GBS = 8000.0
@@ -527,14 +527,17 @@ def bench_window(args):
     bc = [BC_FREE] * 4
     s.run(bc, 0.25, args.warmup)
     s.sync()
-    # In the timed steps only the flux sweeps (the dominant kernels) carry
-    # timing events: events around every launch cost the step about 4%.
-    s.timing(True, phases=["flux_x", "flux_y"])
+    # In the timed steps only the flux sweeps (the dominant kernels; on the
+    # learned path the launches over listed cells and faces) carry timing
+    # events: events around every launch cost the step about 4%.
+    s.timing(True, phases=LEARNED_PHASES if learned else ["flux_x", "flux_y"])
+    c0 = learned[0].classifier_counts(s) if learned else None
     t0 = time.perf_counter()
     k = s.run(bc, 0.25, 2**31 - 1 if args.full_run else args.steps)
     s.sync()
     t1 = time.perf_counter()
     tim = s.get_timing()
+    c1 = learned[0].classifier_counts(s) if learned else None
     assert args.full_run or k == args.steps, (k, args.steps)
     wall = t1 - t0
     value = n * n * k / wall / 1e6
@@ -547,13 +550,17 @@ def bench_window(args):
     tim_all = s.get_timing()
     s.timing(False)
 
+    roof_learned = None
+    if learned:
+        roof_learned = learned_rooflines({key: c1[key] - c0[key] for key in c1}, tim)
+        tim = {}
     # dominant kernel (of the flux sweeps; the breakdown confirms it)
-    dom = max(tim, key=lambda ph: tim[ph][0])
+    dom = max(tim, key=lambda ph: tim[ph][0]) if tim else None
     # the WENO5 tangential reconstruction runs inside the flux launch
     # (unless GKS_TANFLUX=0 asks the library for separate launches)
     fused = p.weno_order == 5 and os.environ.get("GKS_TANFLUX", "1")[:1] != "0"
-    units, fpu = kernel_units(dom, n, fused=fused)
-    avg_ms = tim[dom][0] / max(tim[dom][1], 1)
+    units, fpu = kernel_units(dom, n, fused=fused) if dom else (None, None)
+    avg_ms = tim[dom][0] / max(tim[dom][1], 1) if dom else None
     roof = None
     if units:
         achieved = units * fpu / (avg_ms * 1e-3) / 1e12
@@ -600,6 +607,11 @@ def bench_window(args):
     }
     if learned_info:
         out["learned"] = learned_info
+    if roof_learned:
+        # the learned path's step has no single dominant kernel of fixed
+        # work: its roofline is that of the listed-face flux kernel
+        out["roofline"] = dict(roof_learned["k_tanflux_list"], kernel="k_tanflux_list (listed faces, x and y)")
+        out["roofline_learned"] = roof_learned
     if args.equal_l1:
         from gks_amd.drivers import equal_l1
         out["equal_l1"] = equal_l1(n, args.seed, log=lambda m: print(m, file=sys.stderr, flush=True))
@@ -609,6 +621,42 @@ def bench_window(args):
     else:
         out["cpu_baseline"] = None
     print(json.dumps(out), flush=True)
+
+
+LEARNED_PHASES = ["normal_list_x", "normal_list_y", "flux_list_x", "flux_list_y"]
+
+
+def learned_rooflines(c, lt):
+    """The learned path's full-work kernels, each priced by the units it
+    processed: k_tanflux_list per listed face 2 flux points + each side's
+    tangential reconstruction (WENO5-AO-DF, or the linear weights);
+    k_normal_list per listed cell one WENO5-AO-DF normal reconstruction.
+    c: classifier counters over the timed steps, lt: HIP-event times of
+    LEARNED_PHASES over the same steps."""
+    nfaces = c["tangential_total"] // 2  # faces swept, x and y (one count per face side)
+    listed = c["listed_faces_x"] + c["listed_faces_y"]
+    cheap_faces = nfaces - listed
+    lin_sides_listed = c["tangential_linear"] - 2 * cheap_faces
+    weno_sides_listed = 2 * listed - lin_sides_listed
+    f_list = (listed * 2 * FLOPS_FLUX_POINT + weno_sides_listed * FLOPS_TANGENT_SIDE
+              + lin_sides_listed * FLOPS_TANGENT_SIDE_LINEAR)
+    ms_list = lt["flux_list_x"][0] + lt["flux_list_y"][0]
+    listed_cells = c["normal_total"] - c["normal_linear"]
+    ms_nlist = lt["normal_list_x"][0] + lt["normal_list_y"][0]
+    a_list = f_list / (ms_list * 1e-3) / 1e12
+    a_nlist = listed_cells * FLOPS_NORMAL_CELL / (ms_nlist * 1e-3) / 1e12
+    return {
+        "k_tanflux_list": {"bound": "fp64-valu", "achieved": round(a_list, 3), "peak": PEAK_FP64_TFLOPS,
+                           "unit": "TFLOP/s", "frac": round(a_list / PEAK_FP64_TFLOPS, 4), "traffic": None,
+                           "listed_faces": listed, "weno_sides": weno_sides_listed, "linear_sides": lin_sides_listed,
+                           "flops": f_list, "ms_total": round(ms_list, 3),
+                           "launches": lt["flux_list_x"][1] + lt["flux_list_y"][1]},
+        "k_normal_list": {"bound": "fp64-valu", "achieved": round(a_nlist, 3), "peak": PEAK_FP64_TFLOPS,
+                          "unit": "TFLOP/s", "frac": round(a_nlist / PEAK_FP64_TFLOPS, 4), "traffic": None,
+                          "listed_cells": listed_cells, "flops": listed_cells * FLOPS_NORMAL_CELL,
+                          "ms_total": round(ms_nlist, 3),
+                          "launches": lt["normal_list_x"][1] + lt["normal_list_y"][1]},
+    }
 
 
 def _leg(p, prim, bc, tstop, cls=None, phases=None, warmup=0, log=None, tag=""):
@@ -709,9 +757,8 @@ def bench_headline(args):
     # plain: HIP events on the flux launches (the dominant kernel, k_tanflux)
     legs["plain"] = _leg(p, prim, bc, tstop, phases=["flux_x", "flux_y"], warmup=args.warmup, log=log, tag="plain")
     # learned: HIP events on the launches over the listed cells and faces
-    legs["learned"] = _leg(p, prim, bc, tstop, cls=(P, thr),
-                           phases=["normal_list_x", "normal_list_y", "flux_list_x", "flux_list_y"],
-                           warmup=args.warmup, log=log, tag="learned")
+    legs["learned"] = _leg(p, prim, bc, tstop, cls=(P, thr), phases=LEARNED_PHASES, warmup=args.warmup, log=log,
+                           tag="learned")
     if not args.no_batched:
         legs["learned_batched"] = _leg_batched(p, prim, bc, tstop, (P, thr), warmup=args.warmup, log=log,
                                                tag="learned+batched")
@@ -758,35 +805,8 @@ def bench_headline(args):
         roof["executed_flops_per_launch"] = EXECUTED_FLOPS_2048_FAST["flux_y"]
         roof["executed_frac"] = round(EXECUTED_FLOPS_2048_FAST["flux_y"] / (avg_ms * 1e-3) / 1e12 / PEAK_FP64_TFLOPS,
                                       4)
-    # the learned leg's full-work kernels, each priced by its own units:
-    # k_tanflux_list: per listed face 2 flux points + each side's tangential
-    # reconstruction (WENO5-AO-DF, or the linear weights); k_normal_list:
-    # per listed cell one WENO5-AO-DF normal reconstruction
-    lt = L["timing"]
-    cheap_faces = nfaces - listed
-    lin_sides_listed = c["tangential_linear"] - 2 * cheap_faces
-    weno_sides_listed = 2 * listed - lin_sides_listed
-    f_list = (listed * 2 * FLOPS_FLUX_POINT + weno_sides_listed * FLOPS_TANGENT_SIDE
-              + lin_sides_listed * FLOPS_TANGENT_SIDE_LINEAR)
-    ms_list = lt["flux_list_x"][0] + lt["flux_list_y"][0]
-    listed_cells = c["normal_total"] - c["normal_linear"]
-    ms_nlist = lt["normal_list_x"][0] + lt["normal_list_y"][0]
-    roof_learned = {
-        "k_tanflux_list": {"bound": "fp64-valu", "achieved": round(f_list / (ms_list * 1e-3) / 1e12, 3),
-                           "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
-                           "frac": round(f_list / (ms_list * 1e-3) / 1e12 / PEAK_FP64_TFLOPS, 4),
-                           "listed_faces": listed, "weno_sides": weno_sides_listed, "linear_sides": lin_sides_listed,
-                           "flops": f_list, "ms_total": round(ms_list, 3),
-                           "launches": lt["flux_list_x"][1] + lt["flux_list_y"][1]},
-        "k_normal_list": {"bound": "fp64-valu",
-                          "achieved": round(listed_cells * FLOPS_NORMAL_CELL / (ms_nlist * 1e-3) / 1e12, 3),
-                          "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
-                          "frac": round(listed_cells * FLOPS_NORMAL_CELL / (ms_nlist * 1e-3) / 1e12
-                                        / PEAK_FP64_TFLOPS, 4),
-                          "listed_cells": listed_cells, "flops": listed_cells * FLOPS_NORMAL_CELL,
-                          "ms_total": round(ms_nlist, 3),
-                          "launches": lt["normal_list_x"][1] + lt["normal_list_y"][1]},
-    }
+    # the learned leg's full-work kernels, each priced by its own units
+    roof_learned = learned_rooflines(c, L["timing"])
     out = {
         "metric": "Mcell-updates/s FP64, 2D 2048\u00b2 WENO5-AO GKS: learned vs plain at equal L1",
         "value": value,
