@@ -611,10 +611,27 @@ int gks2d_classifier_samples(gks2d_ctx* c, int32_t dir, int32_t stride, double l
   return GKS_OK;
 }
 
-// learned-path work lists: flux faces x, y; normal-sweep cells x, y; the
-// four counts
+// learned-path work lists (each in kListShards shards): the shard counts
+// of the flux lists x, y and of the normal lists x, y (one 128-byte line
+// each), the four lists' tile prefixes (32 words each), then the flux
+// faces x, y and the normal-sweep cells x, y.  A
+// shard takes the appends of a run of at most ceil(tiles / kListShards)
+// tiles of the sweep that fills the list (flux tiles: 64 or 128 faces).
+static unsigned shard_cap(long long items, int per_tile) {
+  const long long tiles = (items + per_tile - 1) / per_tile;
+  return (unsigned)((tiles + kListShards - 1) / kListShards * per_tile);
+}
+static void list_caps(Dev& d) {
+  const long long nf[2] = {(long long)(d.NX + 1) * d.NY, (long long)d.NX * (d.NY + 1)};
+  const long long ncl[2] = {(long long)(d.NX + 2) * d.ny, (long long)d.nx * (d.NY + 2)};
+  for (int dir = 0; dir < 2; dir++) {
+    d.tcap[dir] = std::max(shard_cap(nf[dir], 64), shard_cap(nf[dir], 128));
+    d.ncap[dir] = shard_cap(ncl[dir], kSweepBlock);
+  }
+}
 static size_t list_words(const Dev& d) {
-  return (size_t)(d.nfx + d.nfy + (long long)(d.NX + 2) * d.ny + (long long)d.nx * (d.NY + 2) + 4);
+  return (size_t)4 * kListShards * kCountStride + 4 * 32 +
+         (size_t)kListShards * ((size_t)d.tcap[0] + d.tcap[1] + d.ncap[0] + d.ncap[1]);
 }
 int gks2d_set_classifier(gks2d_ctx* c, const float* params, float logit_threshold) {
   if (!c) return fail(GKS_ERR_ARG, "null");
@@ -625,6 +642,7 @@ int gks2d_set_classifier(gks2d_ctx* c, const float* params, float logit_threshol
   }
   if (c->p.weno_order != 5) return fail(GKS_ERR_UNSUPPORTED, "the classifier is for WENO5-AO");
   const size_t nc = (size_t)c->d.nc;
+  list_caps(c->d);
   if (!c->d_mlp) {
     HIPCHK(hipMalloc(&c->d_mlp, GKS_MLP_NPARAM * sizeof(float)));
     HIPCHK(hipMalloc(&c->d_mlpw, mlp::kPackHalves * sizeof(unsigned short)));
@@ -643,13 +661,13 @@ int gks2d_set_classifier(gks2d_ctx* c, const float* params, float logit_threshol
   c->d.smooth_flux = 1;
   c->d.lin[0] = c->d_lin;
   c->d.lin[1] = c->d_lin + nc;
-  c->d.tlist[0] = c->d_tlist;
-  c->d.tlist[1] = c->d_tlist + c->d.nfx;
-  const long long ncx = (long long)(c->d.NX + 2) * c->d.ny, ncy = (long long)c->d.nx * (c->d.NY + 2);
-  c->d.nlist[0] = c->d_tlist + c->d.nfx + c->d.nfy;
-  c->d.nlist[1] = c->d.nlist[0] + ncx;
-  c->d.tcount = c->d.nlist[1] + ncy;
-  c->d.ncount = c->d.tcount + 2;
+  c->d.tcount = c->d_tlist;
+  c->d.ncount = c->d.tcount + 2 * kListShards * kCountStride;
+  c->d.lpre = c->d.tcount + 4 * kListShards * kCountStride;
+  c->d.tlist[0] = c->d.lpre + 4 * 32;
+  c->d.tlist[1] = c->d.tlist[0] + (size_t)kListShards * c->d.tcap[0];
+  c->d.nlist[0] = c->d.tlist[1] + (size_t)kListShards * c->d.tcap[1];
+  c->d.nlist[1] = c->d.nlist[0] + (size_t)kListShards * c->d.ncap[0];
   c->d.mlp_thr = logit_threshold;
   c->d.lstat = c->d_lstat;
   return GKS_OK;

@@ -472,6 +472,58 @@ __device__ __forceinline__ void list_append(unsigned* count, unsigned* list, boo
   const unsigned rank = __builtin_amdgcn_mbcnt_hi((unsigned)(bl >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)bl, 0u));
   if (listed) list[base + rank] = value;
 }
+// the counter of shard s of the dir list whose shard counts start at cnt
+__device__ __forceinline__ unsigned* shard_count(unsigned* cnt, int dir, int s) {
+  return cnt + (dir * kListShards + s) * kCountStride;
+}
+// the shard tile blk of a sweep of ntiles tiles appends to: contiguous runs
+// of tiles share a shard (at most ceil(ntiles / kListShards) tiles each),
+// so that a shard's entries stay close in the grid and, with xcd_block's
+// mapping, each counter takes the appends of one XCD's workgroups
+__device__ __forceinline__ int list_shard(unsigned blk, unsigned ntiles) {
+  return (int)((unsigned long long)blk * kListShards / ntiles);
+}
+// the tile prefix of the dir list (per_tile entries per tile, the shards'
+// tiles one after the other) into pre (Dev::lpre layout); one lane
+__device__ __forceinline__ unsigned list_prefix(const unsigned* cnt, int dir, int per_tile, unsigned* pre) {
+  unsigned c[kListShards];
+#pragma unroll
+  for (int k = 0; k < kListShards; k++) c[k] = cnt[(dir * kListShards + k) * kCountStride];
+  unsigned acc = 0;
+#pragma unroll
+  for (int k = 0; k < kListShards; k++) {
+    pre[k] = acc;
+    acc += (c[k] + per_tile - 1) / per_tile;
+  }
+  pre[kListShards] = acc;
+  return acc;
+}
+// tile b of the dir list: its shard s, the tile tb within the shard and the
+// shard's count n; false past the last tile (one load).  pre: the list's
+// tile prefix.  Uniform over a workgroup.
+__device__ __forceinline__ bool list_tile(const unsigned* cnt, const unsigned* pre, int dir, unsigned b, int& s,
+                                          unsigned& tb, unsigned& n) {
+  if (b >= pre[kListShards]) return false;
+  // the last shard starting at or before b (an empty shard starts where
+  // the next one does)
+  s = 0;
+#pragma unroll
+  for (int k = 1; k < kListShards; k++)
+    if (pre[k] <= b) s = k;
+  tb = b - pre[s];
+  n = cnt[(dir * kListShards + s) * kCountStride];
+  return true;
+}
+// entries of a sharded list
+__device__ __forceinline__ unsigned list_total(const unsigned* cnt, int dir) {
+  unsigned n = 0;
+  for (int k = 0; k < kListShards; k++) n += cnt[(dir * kListShards + k) * kCountStride];
+  return n;
+}
+// forms one list's tile prefix (single-problem path)
+__global__ void __launch_bounds__(64) k_list_prefix(Dev d, int kind, int per_tile) {
+  if (threadIdx.x == 0) list_prefix(kind < KL_FLUX_X ? d.ncount : d.tcount, kind & 1, per_tile, d.lpre + kind * 32);
+}
 
 // the two face records a cell's normal reconstruction writes (side 1 of
 // the face at its left, side 0 of the face at its right)
@@ -494,7 +546,8 @@ __device__ __forceinline__ void store_normal_faces(const Dev& d, int i, int j, c
 // (the characteristic projection commutes with a fixed linear combination);
 // the order-reduction test is kept.  Every other stencil is listed for
 // k_normal_list (WENO5-AO-DF), so this kernel carries no WENO registers.
-template <int DIR> __device__ __forceinline__ void normal_learned_body(const Dev& d, unsigned blk) {
+template <int DIR>
+__device__ __forceinline__ void normal_learned_body(const Dev& d, unsigned blk, unsigned ntiles) {
   __shared__ __align__(16) short xs[kBlock * 16];
   __shared__ float ls[kBlock];
   __shared__ MlpLds s_mlp;
@@ -527,7 +580,8 @@ template <int DIR> __device__ __forceinline__ void normal_learned_body(const Dev
   d.lin[DIR][(long long)i * d.ny + j] = lin ? 1 : 0;
   // the other stencils run WENO5-AO-DF in k_normal_list, in waves of
   // troubled cells only
-  list_append(d.ncount + DIR, d.nlist[DIR], !lin, (unsigned)t);
+  const int sh = list_shard(blk, ntiles);
+  list_append(shard_count(d.ncount, DIR, sh), d.nlist[DIR] + (size_t)sh * d.ncap[DIR], !lin, (unsigned)t);
   if (!lin) return;
   const double h = DIR == 0 ? d.dx : d.dy, ih = DIR == 0 ? d.idx : d.idy;
   double lo[8], ro[8];
@@ -552,16 +606,20 @@ template <int DIR> __device__ __forceinline__ void normal_learned_body(const Dev
 template <int DIR>
 __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(GKS_NORMAL_WAVES)))
 k_normal_learned(Dev d) {
-  normal_learned_body<DIR>(d, xcd_block());
+  normal_learned_body<DIR>(d, xcd_block(), gridDim.x);
 }
-// WENO5-AO-DF over the cells k_normal_learned listed, one thread each;
-// launched with the normal sweep's block count (the list length is known
-// only on the device; blocks past it return at once)
+// WENO5-AO-DF over the cells k_normal_learned listed, one thread each, a
+// list tile per workgroup; launched with the normal sweep's block count +
+// kListShards (a bound on the list's tiles, whose number is known only on
+// the device: k_list_prefix forms it; blocks past it return at once)
 template <int DIR> __device__ __forceinline__ void normal_list_body(const Dev& d, unsigned blk) {
-  const long long k = (long long)blk * kBlock + threadIdx.x;
-  if (k >= (long long)d.ncount[DIR]) return;
+  int s;
+  unsigned tb, n;
+  if (!list_tile(d.ncount, d.lpre + (DIR == 0 ? KL_NORMAL_X : KL_NORMAL_Y) * 32, DIR, blk, s, tb, n)) return;
+  const long long k = (long long)tb * kBlock + threadIdx.x;
+  if (k >= (long long)n) return;
   int i, j;
-  normal_cell<DIR>(d, (long long)d.nlist[DIR][k], i, j);
+  normal_cell<DIR>(d, (long long)d.nlist[DIR][(size_t)s * d.ncap[DIR] + k], i, j);
   double w[5][4], a[5];
   normal_stencil<DIR>(d, i, j, w, a);
   double lo[8], ro[8];
@@ -571,7 +629,7 @@ template <int DIR> __device__ __forceinline__ void normal_list_body(const Dev& d
 template <int DIR>
 __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(GKS_NORMAL_WAVES)))
 k_normal_list(Dev d) {
-  if ((long long)blockIdx.x * kBlock < (long long)d.ncount[DIR]) normal_list_body<DIR>(d, blockIdx.x);
+  normal_list_body<DIR>(d, blockIdx.x);
 }
 
 // logits of n feature rows (parity of the MFMA inference)
@@ -1082,7 +1140,8 @@ template <int DIR> __device__ __forceinline__ bool tangential_linear(const Dev& 
 // with both sides linear gets the smooth-flow flux from the two states;
 // the first wave lists every other face.
 template <int DIR, int TF>
-__device__ __forceinline__ void tanflux_cheap_body(const Dev& d, unsigned blk, int stage, double dt) {
+__device__ __forceinline__ void tanflux_cheap_body(const Dev& d, unsigned blk, unsigned ntiles, int stage,
+                                                   double dt) {
   __shared__ double sg[2][2][12][TF];  // [side][gauss point][component][face]
   __shared__ unsigned char slin[2][TF];
   const int ni = DIR == 0 ? d.NX + 1 : d.NX, nj = DIR == 0 ? d.NY : d.NY + 1;
@@ -1144,7 +1203,10 @@ __device__ __forceinline__ void tanflux_cheap_body(const Dev& d, unsigned blk, i
   const int q = hi;
   const bool cheap = valid && d.smooth_flux && (slin[0][tl] & slin[1][tl]);
   // the first TF threads: whole waves (TF is a multiple of 64)
-  if (q == 0) list_append(d.tcount + DIR, d.tlist[DIR], valid && !cheap, (unsigned)r);
+  const int sh = list_shard(blk, ntiles);
+  if (q == 0)
+    list_append(shard_count(d.tcount, DIR, sh), d.tlist[DIR] + (size_t)sh * d.tcap[DIR], valid && !cheap,
+                (unsigned)r);
   if (!cheap) return;
   const long long f0 = DIR == 0 ? (long long)i * d.ny + j : (long long)i * (d.ny + 1) + j;
   const Strided L{&sg[0][q][0][tl], TF}, R{&sg[1][q][0][tl], TF};
@@ -1152,7 +1214,7 @@ __device__ __forceinline__ void tanflux_cheap_body(const Dev& d, unsigned blk, i
 }
 template <int DIR>
 __global__ void __launch_bounds__(2 * kTanFlux) k_tanflux_cheap(Dev d, int stage, double dt) {
-  tanflux_cheap_body<DIR, kTanFlux>(d, xcd_block(), stage, dt);
+  tanflux_cheap_body<DIR, kTanFlux>(d, xcd_block(), gridDim.x, stage, dt);
 }
 // one tile of tlist[DIR] per workgroup, launched for as many tiles as there
 // are faces (the count is only known on the device; workgroups past it
@@ -1164,15 +1226,18 @@ __global__ void __launch_bounds__(2 * kTanFlux) k_tanflux_cheap(Dev d, int stage
 template <int DIR>
 __global__ void __launch_bounds__(2 * kTanFlux) __attribute__((amdgpu_waves_per_eu(GKS_FLUX_WAVES)))
 k_tanflux_list(Dev d, int stage, double dt) {
-  const long long n = d.tcount[DIR];
-  if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(d.lstat + kListedWord + DIR, (unsigned long long)n);
-  if ((long long)blockIdx.x * kTanFlux < n)
-    tanflux_body<DIR, true, kTanFlux, true>(d, blockIdx.x, stage, dt, d.tlist[DIR], n);
+  if (blockIdx.x == 0 && threadIdx.x == 0)
+    atomicAdd(d.lstat + kListedWord + DIR, (unsigned long long)list_total(d.tcount, DIR));
+  int s;
+  unsigned tb, n;
+  if (list_tile(d.tcount, d.lpre + (KL_FLUX_X + DIR) * 32, DIR, blockIdx.x, s, tb, n))
+    tanflux_body<DIR, true, kTanFlux, true>(d, tb, stage, dt, d.tlist[DIR] + (size_t)s * d.tcap[DIR], n);
 }
-// clears a list count before the sweep that fills it (a kernel, not a
-// memset, so that the sequence can be captured in a graph)
+// clears a list's shard counts (count: its first) before the sweep that
+// fills it (a kernel, not a memset, so that the sequence can be captured in
+// a graph)
 __global__ void __launch_bounds__(64) k_clear_count(unsigned* count) {
-  if (threadIdx.x == 0) *count = 0u;
+  if (threadIdx.x < kListShards) count[threadIdx.x * kCountStride] = 0u;
 }
 
 // ------------------------------------------------------------------ update
@@ -1326,25 +1391,38 @@ __global__ void __launch_bounds__(kBlock) kb_copy_old(BatchArgs B) {
 template <int DIR, bool LEARNED>
 __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(GKS_NORMAL_WAVES))) kb_normal(BatchArgs B) {
   GKS_BATCH_TILE(DIR == 0 ? KB_NORMAL_X : KB_NORMAL_Y);
-  if constexpr (LEARNED) normal_learned_body<DIR>(d, blk);
-  else normal_body<DIR>(d, blk);
+  if constexpr (LEARNED) {
+    const unsigned* st = B.start + (DIR == 0 ? KB_NORMAL_X : KB_NORMAL_Y) * (B.np + 1);
+    normal_learned_body<DIR>(d, blk, st[q_ + 1] - st[q_]);
+  } else {
+    normal_body<DIR>(d, blk);
+  }
 }
 // Learned path, work lists: after the sweep that fills every problem's
-// list of one kind, kb_list_prefix packs the lists' tiles at the front of
-// the next launch (problem q's tiles start at lstart[q]); the list kernel
-// is launched with as many workgroups as the sweep had (a bound), and the
-// ones past lstart[np] return after one load.  Runs on one lane: np is a
-// batch's problem count (tens).
+// list of one kind, kb_list_prefix forms each problem's tile prefix of
+// that list and packs the lists' tiles at the front of the next launch
+// (problem q's tiles start at lstart[q]); the list kernel is launched with
+// as many workgroups as the sweep had + kListShards per problem (a bound),
+// and the ones past lstart[np] return after one load.  One workgroup: a
+// lane per problem, then lane 0 scans the problems' tile counts (np is a
+// batch's problem count, tens).
 __global__ void __launch_bounds__(64) kb_list_prefix(BatchArgs B, int kind, int per_tile) {
-  if (threadIdx.x != 0) return;
   unsigned* ls = B.lstart + kind * (B.np + 1);
+  // each problem's tile prefix (one lane per problem), its tiles into ls
+  for (int q = threadIdx.x; q < B.np; q += 64) {
+    const int p = B.pid[q];
+    const Dev& d = B.dev[p];
+    const unsigned* cnt = kind < KL_FLUX_X ? d.ncount : d.tcount;
+    ls[q] = (B.st[p].active && cnt) ? list_prefix(cnt, kind & 1, per_tile, d.lpre + kind * 32) : 0u;
+  }
+  __threadfence_block();
+  __syncthreads();
+  if (threadIdx.x != 0) return;
   unsigned acc = 0;
   for (int q = 0; q < B.np; q++) {
+    const unsigned t = ls[q];
     ls[q] = acc;
-    const int p = B.pid[q];
-    const unsigned* cnt = kind < KL_FLUX_X ? B.dev[p].ncount : B.dev[p].tcount;
-    const unsigned n = (B.st[p].active && cnt) ? cnt[kind & 1] : 0u;
-    acc += (n + per_tile - 1) / per_tile;
+    acc += t;
   }
   ls[B.np] = acc;
 }
@@ -1383,29 +1461,31 @@ kb_tanflux(BatchArgs B, int stage) {
 template <int DIR, int TF>
 __global__ void __launch_bounds__(2 * TF) kb_tanflux_cheap(BatchArgs B, int stage) {
   GKS_BATCH_TILE(DIR == 0 ? KB_FLUX_X : KB_FLUX_Y);
-  tanflux_cheap_body<DIR, TF>(d, blk, stage, B.st[p_].dt);
+  const unsigned* st = B.start + (DIR == 0 ? KB_FLUX_X : KB_FLUX_Y) * (B.np + 1);
+  tanflux_cheap_body<DIR, TF>(d, blk, st[q_ + 1] - st[q_], stage, B.st[p_].dt);
 }
 template <int DIR, int TF>
 __global__ void __launch_bounds__(2 * TF) __attribute__((amdgpu_waves_per_eu(GKS_FLUX_WAVES)))
 kb_tanflux_list(BatchArgs B, int stage) {
   GKS_BATCH_LIST_TILE(DIR == 0 ? KL_FLUX_X : KL_FLUX_Y);
-  const long long n = d.tcount[DIR];
-  if (blk == 0 && threadIdx.x == 0) atomicAdd(d.lstat + kListedWord + DIR, (unsigned long long)n);
-  tanflux_body<DIR, true, TF, true>(d, blk, stage, B.st[p_].dt, d.tlist[DIR], n);
+  if (blk == 0 && threadIdx.x == 0)
+    atomicAdd(d.lstat + kListedWord + DIR, (unsigned long long)list_total(d.tcount, DIR));
+  int s;
+  unsigned tb, n;
+  if (list_tile(d.tcount, d.lpre + (KL_FLUX_X + DIR) * 32, DIR, blk, s, tb, n))
+    tanflux_body<DIR, true, TF, true>(d, tb, stage, B.st[p_].dt, d.tlist[DIR] + (size_t)s * d.tcap[DIR], n);
 }
-// clears the four list counts (flux x, y; normal x, y) of every active
-// problem (one lane per problem), before the stage's normal sweeps
+// clears the shard counts of the four lists (flux x, y; normal x, y) of
+// every active problem (one lane per problem), before the stage's normal
+// sweeps
 __global__ void __launch_bounds__(64) kb_clear_counts(BatchArgs B) {
   for (int q = threadIdx.x; q < B.np; q += 64) {
     const int p = B.pid[q];
-    if (B.st[p].active && B.dev[p].tcount) {
-      B.dev[p].tcount[0] = 0u;
-      B.dev[p].tcount[1] = 0u;
-    }
-    if (B.st[p].active && B.dev[p].ncount) {
-      B.dev[p].ncount[0] = 0u;
-      B.dev[p].ncount[1] = 0u;
-    }
+    if (!B.st[p].active) continue;
+    unsigned* cnt[2] = {B.dev[p].tcount, B.dev[p].ncount};
+    for (int l = 0; l < 2; l++)
+      if (cnt[l])
+        for (int k = 0; k < 2 * kListShards; k++) cnt[l][k * kCountStride] = 0u;
   }
 }
 __global__ void __launch_bounds__(kBlock) kb_update(BatchArgs B) {
@@ -1554,7 +1634,7 @@ static hipError_t normal(const Dev& d, int dir, hipStream_t s) {
     else hipLaunchKernelGGL((k_normal79<1, 9>), dim3(nb), dim3(kBlock), 0, s, d);
   } else if (d.mlp) {
     // the listed cells follow in normal_list
-    hipLaunchKernelGGL(k_clear_count, dim3(1), dim3(64), 0, s, d.ncount + dir);
+    hipLaunchKernelGGL(k_clear_count, dim3(1), dim3(64), 0, s, d.ncount + dir * kListShards * kCountStride);
     if (dir == 0) hipLaunchKernelGGL(k_normal_learned<0>, dim3(nb), dim3(kBlock), 0, s, d);
     else hipLaunchKernelGGL(k_normal_learned<1>, dim3(nb), dim3(kBlock), 0, s, d);
   } else {
@@ -1584,7 +1664,7 @@ static hipError_t flux(const Dev& d, int dir, int stage, double dt, hipStream_t 
     if (d.mlp && d.smooth_flux && d.tcount) {
       // smooth faces, then the listed rest (at most nfaces of them)
       // (the listed faces follow in flux_list)
-      hipLaunchKernelGGL(k_clear_count, dim3(1), dim3(64), 0, s, d.tcount + dir);
+      hipLaunchKernelGGL(k_clear_count, dim3(1), dim3(64), 0, s, d.tcount + dir * kListShards * kCountStride);
       if (dir == 0) hipLaunchKernelGGL(k_tanflux_cheap<0>, dim3(nb), dim3(2 * kTanFlux), 0, s, d, stage, dt);
       else hipLaunchKernelGGL(k_tanflux_cheap<1>, dim3(nb), dim3(2 * kTanFlux), 0, s, d, stage, dt);
     } else if (d.mlp) {
@@ -1606,7 +1686,11 @@ static hipError_t flux(const Dev& d, int dir, int stage, double dt, hipStream_t 
 }
 static hipError_t normal_list(const Dev& d, int dir, hipStream_t s) {
   if (d.order != 5 || !d.mlp || !d.ncount) return hipSuccess;
-  const unsigned nb = dir == 0 ? blocks_for((long long)(d.NX + 2) * d.ny) : blocks_for((long long)d.nx * (d.NY + 2));
+  // a bound on the list's tiles: each shard adds at most one partial tile
+  const unsigned nb = kListShards + (dir == 0 ? blocks_for((long long)(d.NX + 2) * d.ny)
+                                              : blocks_for((long long)d.nx * (d.NY + 2)));
+  hipLaunchKernelGGL(k_list_prefix, dim3(1), dim3(64), 0, s, d, dir == 0 ? (int)KL_NORMAL_X : (int)KL_NORMAL_Y,
+                     kBlock);
   if (dir == 0) hipLaunchKernelGGL(k_normal_list<0>, dim3(nb), dim3(kBlock), 0, s, d);
   else hipLaunchKernelGGL(k_normal_list<1>, dim3(nb), dim3(kBlock), 0, s, d);
   return hipGetLastError();
@@ -1614,7 +1698,8 @@ static hipError_t normal_list(const Dev& d, int dir, hipStream_t s) {
 static hipError_t flux_list(const Dev& d, int dir, int stage, double dt, hipStream_t s) {
   if (d.order != 5 || !d.fuse_tanflux || !d.mlp || !d.smooth_flux || !d.tcount) return hipSuccess;
   const long long nfaces = dir == 0 ? (long long)(d.NX + 1) * d.NY : (long long)d.NX * (d.NY + 1);
-  const unsigned nb = (unsigned)((nfaces + kTanFlux - 1) / kTanFlux);
+  const unsigned nb = kListShards + (unsigned)((nfaces + kTanFlux - 1) / kTanFlux);
+  hipLaunchKernelGGL(k_list_prefix, dim3(1), dim3(64), 0, s, d, (int)KL_FLUX_X + dir, kTanFlux);
   if (dir == 0) hipLaunchKernelGGL(k_tanflux_list<0>, dim3(nb), dim3(2 * kTanFlux), 0, s, d, stage, dt);
   else hipLaunchKernelGGL(k_tanflux_list<1>, dim3(nb), dim3(2 * kTanFlux), 0, s, d, stage, dt);
   return hipGetLastError();
@@ -1655,6 +1740,10 @@ static hipError_t infer(const float* P, const unsigned short* W, long long n, co
 
 template <bool LEARNED, int TF>
 static void batch_stage(const BatchArgs& B, const unsigned* tiles, int stage, hipStream_t s) {
+  // the list launches' bound: each problem's shards add at most one partial
+  // tile each
+  const unsigned lb = (unsigned)kListShards * (unsigned)B.np;
+  (void)lb;
   for (int side = 0; side < 4; side++)
     hipLaunchKernelGGL(kb_bc, dim3(tiles[side < 2 ? KB_BC_LR : KB_BC_DU]), dim3(kBlock), 0, s, B, side);
   hipLaunchKernelGGL(kb_primvar, dim3(tiles[KB_PRIMVAR]), dim3(kBlock), 0, s, B);
@@ -1663,20 +1752,20 @@ static void batch_stage(const BatchArgs& B, const unsigned* tiles, int stage, hi
   hipLaunchKernelGGL((kb_normal<0, LEARNED>), dim3(tiles[KB_NORMAL_X]), dim3(kBlock), 0, s, B);
   if (LEARNED) {
     hipLaunchKernelGGL(kb_list_prefix, dim3(1), dim3(64), 0, s, B, (int)KL_NORMAL_X, kBlock);
-    hipLaunchKernelGGL(kb_normal_list<0>, dim3(tiles[KB_NORMAL_X]), dim3(kBlock), 0, s, B);
+    hipLaunchKernelGGL(kb_normal_list<0>, dim3(tiles[KB_NORMAL_X] + lb), dim3(kBlock), 0, s, B);
   }
   hipLaunchKernelGGL((kb_normal<1, LEARNED>), dim3(tiles[KB_NORMAL_Y]), dim3(kBlock), 0, s, B);
   if (LEARNED) {
     hipLaunchKernelGGL(kb_list_prefix, dim3(1), dim3(64), 0, s, B, (int)KL_NORMAL_Y, kBlock);
-    hipLaunchKernelGGL(kb_normal_list<1>, dim3(tiles[KB_NORMAL_Y]), dim3(kBlock), 0, s, B);
+    hipLaunchKernelGGL(kb_normal_list<1>, dim3(tiles[KB_NORMAL_Y] + lb), dim3(kBlock), 0, s, B);
   }
   if (LEARNED) {
     hipLaunchKernelGGL((kb_tanflux_cheap<0, TF>), dim3(tiles[KB_FLUX_X]), dim3(2 * TF), 0, s, B, stage);
     hipLaunchKernelGGL(kb_list_prefix, dim3(1), dim3(64), 0, s, B, (int)KL_FLUX_X, TF);
-    hipLaunchKernelGGL((kb_tanflux_list<0, TF>), dim3(tiles[KB_FLUX_X]), dim3(2 * TF), 0, s, B, stage);
+    hipLaunchKernelGGL((kb_tanflux_list<0, TF>), dim3(tiles[KB_FLUX_X] + lb), dim3(2 * TF), 0, s, B, stage);
     hipLaunchKernelGGL((kb_tanflux_cheap<1, TF>), dim3(tiles[KB_FLUX_Y]), dim3(2 * TF), 0, s, B, stage);
     hipLaunchKernelGGL(kb_list_prefix, dim3(1), dim3(64), 0, s, B, (int)KL_FLUX_Y, TF);
-    hipLaunchKernelGGL((kb_tanflux_list<1, TF>), dim3(tiles[KB_FLUX_Y]), dim3(2 * TF), 0, s, B, stage);
+    hipLaunchKernelGGL((kb_tanflux_list<1, TF>), dim3(tiles[KB_FLUX_Y] + lb), dim3(2 * TF), 0, s, B, stage);
   } else {
     hipLaunchKernelGGL((kb_tanflux<0, LEARNED, TF>), dim3(tiles[KB_FLUX_X]), dim3(2 * TF), 0, s, B, stage);
     hipLaunchKernelGGL((kb_tanflux<1, LEARNED, TF>), dim3(tiles[KB_FLUX_Y]), dim3(2 * TF), 0, s, B, stage);

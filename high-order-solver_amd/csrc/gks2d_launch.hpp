@@ -39,11 +39,20 @@ struct Dev {
                                 // faces the flux lists held (x, y)
   unsigned char* lin[2];        // per cell: its x / y normal stencil took the linear weights
   int smooth_flux;              // learned path: faces with two linear sides take the smooth-flow flux
-  unsigned* tlist[2];           // learned path: x / y faces left to the full flux (interior face ordinals)
-  unsigned* tcount;             // their counts, x then y (cleared before each flux sweep)
-  unsigned* nlist[2];           // learned path: cells whose x / y normal stencil runs WENO5-AO-DF (ordinals of
-                                // the normal sweep's range)
-  unsigned* ncount;             // their counts, x then y (cleared before each normal sweep)
+  // learned path: the work lists, each in kListShards shards (a run of
+  // tiles of the sweep that fills a list appends to one shard, so that the
+  // appends of a sweep spread over kListShards counters)
+  unsigned* tlist[2];           // x / y faces left to the full flux (interior face ordinals); shard s at
+                                // tlist[dir] + s * tcap[dir]
+  unsigned* tcount;             // their shard counts: shard s of dir at tcount[(dir * kListShards + s) *
+                                // kCountStride] (cleared before each flux sweep)
+  unsigned* nlist[2];           // cells whose x / y normal stencil runs WENO5-AO-DF (ordinals of the normal
+                                // sweep's range); shard s at nlist[dir] + s * ncap[dir]
+  unsigned* ncount;             // their shard counts, laid out as tcount's (cleared before each normal sweep)
+  unsigned tcap[2], ncap[2];    // entries one shard holds
+  unsigned* lpre;               // per list (BatchList order) 32 words: the first tile of each shard, and at
+                                // [kListShards] the list's tiles (formed between the sweep that fills the
+                                // list and the list's launch)
   double* partial;  // per-block (max speed, blow-up flag) pairs
   double* stat;     // [0..3]: max speed of slot 0 / 1 (+pad); [4]: sticky blow-up flag
 };
@@ -52,6 +61,11 @@ struct Dev {
 // of the CFL speed reduction (the host sizes batched launches with them)
 constexpr int kSweepBlock = 256;
 constexpr int kSweepPartials = 1024;
+// learned-path work lists: shards per list, and the distance of two shard
+// counters in 32-bit words (one 128-byte line each: atomics on one line
+// queue behind each other)
+constexpr int kListShards = 16;
+constexpr int kCountStride = 32;
 
 // ------------------------------------------------------------------ batches
 // Per-problem time-loop state of a batch, on the device: the driver loop of
