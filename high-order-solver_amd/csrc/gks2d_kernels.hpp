@@ -952,6 +952,123 @@ __device__ __forceinline__ void flux_point_tail(const Dev& d, int q, long long f
   xc[(q * 4 + 3) * nf + f] = x[3];
 }
 
+// flux_point_tail for the fused WENO5 launches, whose two Gauss points of a
+// face are the lanes 2m (q = 0) and 2m + 1 (q = 1) of one wave.  The pair
+// forms the face's DF value 0.5 (alpha_0 + alpha_1), thresholded as
+// Calculate_alpha (reconstruction.cpp:1466-1521) does, so that k_alpha reads
+// one value per face (the same sum in the same order: bit for bit the
+// per-point form).  In the fast build the pair also sums the stage-combined
+// fluxes of its two points (each lane stores two of the four rotated
+// components) and stage 0's stage-1 share, halving what k_update reads;
+// the IEEE build keeps one record per point and Update_with_gauss's
+// summation order.  Both lanes of a pair must be active together.
+template <int DIR, class V>
+__device__ __forceinline__ void flux_pair_tail(const Dev& d, int q, long long f, long long nf, const V& L,
+                                               const V& R, int stage, double dt, bool smooth) {
+  {
+    double pl[4], pr[4], cl[4], cr[4];
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+      cl[k] = L[k];
+      cr[k] = R[k];
+    }
+    convar_to_primvar(pl, cl, d.gas);
+    convar_to_primvar(pr, cr, d.gas);
+    double a;
+    if (DIR == 0) {
+      a = alpha_k(pl, pr, d.gas);
+    } else {
+      const double tl[4] = {pl[0], -pl[2], pl[1], pl[3]};
+      const double tr[4] = {pr[0], -pr[2], pr[1], pr[3]};
+      a = alpha_k(tl, tr, d.gas);
+    }
+    const double ap = __shfl_xor(a, 1);
+    if (q == 0) {
+      // (not fused with the last product of alpha_k: see below)
+#pragma clang fp contract(off)
+      const double s = 0.5 * (a + ap);
+      (DIR == 0 ? d.akx : d.aky)[f] = s < 2.0 ? 0.0 : s;
+    }
+  }
+  double fo[4], dfo[4];
+  if (smooth) gks_point_smooth(fo, dfo, L, R, dt, d.gas);
+  else gks_point(fo, dfo, nullptr, L, R, dt, d.gas);
+  if (d.keep_face_fluxes) {
+    double* fl = DIR == 0 ? (stage == 0 ? d.flx[0] : d.flx[1]) : (stage == 0 ? d.fly[0] : d.fly[1]);
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+      fl[(q * 8 + k) * nf + f] = fo[k];
+      fl[(q * 8 + 4 + k) * nf + f] = dfo[k];
+    }
+  }
+  const double w = q == 0 ? d.gw[0] : d.gw[1];
+  double* pc = DIR == 0 ? d.pcx : d.pcy;
+  double* xc = DIR == 0 ? d.xcx : d.xcy;
+#if GKS_FAST
+  // rotated (normal (0,1): (x1, x2) -> (-x2, x1)); lane q stores the
+  // rotated components 2q, 2q + 1 of the pair's sums.  No contraction here:
+  // whether a product and the pair's sum would fuse depends on how the
+  // kernel around it compiled, and the batched and single launches must
+  // round alike.
+  {
+#pragma clang fp contract(off)
+  auto rot = [](const double* x, double* y) {
+    y[0] = x[0];
+    y[1] = DIR == 0 ? x[1] : -x[2];
+    y[2] = DIR == 0 ? x[2] : x[1];
+    y[3] = x[3];
+  };
+  auto pair_sum = [&](const double* y, double* out) {
+#pragma unroll
+    for (int m = 0; m < 2; m++) {
+      const double mine = q == 0 ? y[m] : y[2 + m], theirs = q == 0 ? y[2 + m] : y[m];
+      out[m] = mine + __shfl_xor(theirs, 1);
+    }
+  };
+  double xl[4], xr[4], xs[2];
+  if (stage == 0) {
+    double pl[4], pr[4], ps[2];
+#pragma unroll
+    for (int v = 0; v < 4; v++) {
+      xl[v] = w * (0.5 * fo[v] + 1.0 / 8.0 * dfo[v]);
+      pl[v] = w * (1.0 * fo[v] + 1.0 / 6.0 * dfo[v]);
+    }
+    rot(pl, pr);
+    pair_sum(pr, ps);
+#pragma unroll
+    for (int m = 0; m < 2; m++) pc[(2 * q + m) * nf + f] = ps[m];
+    rot(xl, xr);
+    pair_sum(xr, xs);
+  } else {
+#pragma unroll
+    for (int v = 0; v < 4; v++) xl[v] = w * (0.0 * fo[v] + 1.0 / 3.0 * dfo[v]);
+    rot(xl, xr);
+    pair_sum(xr, xs);
+#pragma unroll
+    for (int m = 0; m < 2; m++) xs[m] += pc[(2 * q + m) * nf + f];
+  }
+#pragma unroll
+  for (int m = 0; m < 2; m++) xc[(2 * q + m) * nf + f] = xs[m];
+  }
+#else
+  double x[4];
+  if (stage == 0) {
+#pragma unroll
+    for (int v = 0; v < 4; v++) {
+      x[v] = w * (0.5 * fo[v] + 1.0 / 8.0 * dfo[v]);
+      pc[(q * 4 + v) * nf + f] = w * (1.0 * fo[v] + 1.0 / 6.0 * dfo[v]);
+    }
+  } else {
+#pragma unroll
+    for (int v = 0; v < 4; v++) x[v] = pc[(q * 4 + v) * nf + f] + w * (0.0 * fo[v] + 1.0 / 3.0 * dfo[v]);
+  }
+  xc[(q * 4 + 0) * nf + f] = x[0];
+  xc[(q * 4 + 1) * nf + f] = DIR == 0 ? x[1] : -x[2];
+  xc[(q * 4 + 2) * nf + f] = DIR == 0 ? x[2] : x[1];
+  xc[(q * 4 + 3) * nf + f] = x[3];
+#endif
+}
+
 // waves per SIMD asked of the flux-point kernels (k_flux, k_tanflux)
 #ifndef GKS_FLUX_WAVES
 #define GKS_FLUX_WAVES 2
@@ -996,7 +1113,7 @@ constexpr int kTanFlux = 64;
 template <int DIR, bool LEARNED, int TF, bool LISTED = false>
 __device__ __forceinline__ void tanflux_body(const Dev& d, unsigned blk, int stage, double dt,
                                              const unsigned* list = nullptr, long long nlist = 0) {
-  __shared__ double sg[2][2][12][TF];  // [side][gauss point][component][face]
+  __shared__ double sg[2][12][TF][2];  // [side][component][face][gauss point]
   __shared__ unsigned char slin[2][LEARNED ? TF : 1];  // side took the linear weights
   const int ni = DIR == 0 ? d.NX + 1 : d.NX, nj = DIR == 0 ? d.NY : d.NY + 1;
   const long long nfaces = (long long)ni * nj;
@@ -1006,7 +1123,6 @@ __device__ __forceinline__ void tanflux_body(const Dev& d, unsigned blk, int sta
   const long long r = LISTED ? (valid ? (long long)list[slot] : 0) : slot;
   const int i = valid ? (int)(r / nj) + d.G : d.G, j = valid ? (int)(r % nj) + d.G : d.G;
   const long long nf = DIR == 0 ? d.nfx : d.nfy;
-  const long long f0 = DIR == 0 ? (long long)i * d.ny + j : (long long)i * (d.ny + 1) + j;
   const int side = hi;
   double L[5][8], a[5];
   if (valid) {
@@ -1081,17 +1197,23 @@ __device__ __forceinline__ void tanflux_body(const Dev& d, unsigned blk, int sta
       weno5_tangent(gp, L, a, h, ih, d.recon_var, d.gas);
     }
 #pragma unroll
-    for (int q = 0; q < 2; q++)
+    for (int k = 0; k < 12; k++)
 #pragma unroll
-      for (int k = 0; k < 12; k++) sg[side][q][k][tl] = gp[q][k];
+      for (int q = 0; q < 2; q++) sg[side][k][tl][q] = gp[q][k];
   }
   __syncthreads();
-  if (valid) {
-    const int q = hi;
-    const Strided L{&sg[0][q][0][tl], TF}, R{&sg[1][q][0][tl], TF};
+  // phase 2: lane pair (2 fp, 2 fp + 1) takes the two Gauss points of face fp
+  const int fp = threadIdx.x >> 1, q = threadIdx.x & 1;
+  const long long slot2 = (long long)blk * TF + fp;
+  const bool valid2 = LISTED ? slot2 < nlist : slot2 < nfaces;
+  if (valid2) {
+    const long long r2 = LISTED ? (long long)list[slot2] : slot2;
+    const int i2 = (int)(r2 / nj) + d.G, j2 = (int)(r2 % nj) + d.G;
+    const long long f2 = DIR == 0 ? (long long)i2 * d.ny + j2 : (long long)i2 * (d.ny + 1) + j2;
+    const Strided L{&sg[0][0][fp][q], 2 * TF}, R{&sg[1][0][fp][q], 2 * TF};
     // both sides smooth: the smooth-flow flux (learned path)
-    const bool smooth = LEARNED && !LISTED && d.smooth_flux && (slin[0][tl] & slin[1][tl]);
-    flux_point_tail<DIR>(d, q, f0, nf, L, R, stage, dt, smooth);
+    const bool smooth = LEARNED && !LISTED && d.smooth_flux && (slin[0][fp] & slin[1][fp]);
+    flux_pair_tail<DIR>(d, q, f2, nf, L, R, stage, dt, smooth);
   }
 }
 template <int DIR, bool LEARNED>
@@ -1142,7 +1264,7 @@ template <int DIR> __device__ __forceinline__ bool tangential_linear(const Dev& 
 template <int DIR, int TF>
 __device__ __forceinline__ void tanflux_cheap_body(const Dev& d, unsigned blk, unsigned ntiles, int stage,
                                                    double dt) {
-  __shared__ double sg[2][2][12][TF];  // [side][gauss point][component][face]
+  __shared__ double sg[2][12][TF][2];  // [side][component][face][gauss point]
   __shared__ unsigned char slin[2][TF];
   const int ni = DIR == 0 ? d.NX + 1 : d.NX, nj = DIR == 0 ? d.NY : d.NY + 1;
   const long long nfaces = (long long)ni * nj;
@@ -1194,23 +1316,31 @@ __device__ __forceinline__ void tanflux_cheap_body(const Dev& d, unsigned blk, u
             gp[q][8 + v] = 0.0;
           }
         }
-#pragma unroll
-        for (int k = 0; k < 12; k++) sg[side][q][k][tl] = gp[q][k];
       }
+#pragma unroll
+      for (int k = 0; k < 12; k++)
+#pragma unroll
+        for (int q = 0; q < 2; q++) sg[side][k][tl][q] = gp[q][k];
     }
   }
   __syncthreads();
-  const int q = hi;
-  const bool cheap = valid && d.smooth_flux && (slin[0][tl] & slin[1][tl]);
-  // the first TF threads: whole waves (TF is a multiple of 64)
-  const int sh = list_shard(blk, ntiles);
-  if (q == 0)
-    list_append(shard_count(d.tcount, DIR, sh), d.tlist[DIR] + (size_t)sh * d.tcap[DIR], valid && !cheap,
-                (unsigned)r);
-  if (!cheap) return;
-  const long long f0 = DIR == 0 ? (long long)i * d.ny + j : (long long)i * (d.ny + 1) + j;
-  const Strided L{&sg[0][q][0][tl], TF}, R{&sg[1][q][0][tl], TF};
-  flux_point_tail<DIR>(d, q, f0, nf, L, R, stage, dt, true);
+  // the first TF threads (whole waves: TF is a multiple of 64) list the
+  // faces of the tile that are not both sides linear
+  {
+    const bool cheap = valid && d.smooth_flux && (slin[0][tl] & slin[1][tl]);
+    const int sh = list_shard(blk, ntiles);
+    if (hi == 0)
+      list_append(shard_count(d.tcount, DIR, sh), d.tlist[DIR] + (size_t)sh * d.tcap[DIR], valid && !cheap,
+                  (unsigned)r);
+  }
+  // phase 2: lane pair (2 fp, 2 fp + 1) takes the two Gauss points of face fp
+  const int fp = threadIdx.x >> 1, q = threadIdx.x & 1;
+  const long long r2 = (long long)blk * TF + fp;
+  if (!(r2 < nfaces && d.smooth_flux && (slin[0][fp] & slin[1][fp]))) return;
+  const int i2 = (int)(r2 / nj) + d.G, j2 = (int)(r2 % nj) + d.G;
+  const long long f2 = DIR == 0 ? (long long)i2 * d.ny + j2 : (long long)i2 * (d.ny + 1) + j2;
+  const Strided L{&sg[0][0][fp][q], 2 * TF}, R{&sg[1][0][fp][q], 2 * TF};
+  flux_pair_tail<DIR>(d, q, f2, nf, L, R, stage, dt, true);
 }
 template <int DIR>
 __global__ void __launch_bounds__(2 * kTanFlux) k_tanflux_cheap(Dev d, int stage, double dt) {
@@ -1252,7 +1382,9 @@ __device__ __forceinline__ void update_body(const Dev& d, unsigned blk) {
   const long long fb = (long long)i * (d.ny + 1) + j, ft = fb + 1;
   const long long fl_ = (long long)i * d.ny + j, fr = (long long)(i + 1) * d.ny + j;
   double total[4] = {0.0, 0.0, 0.0, 0.0};
-  for (int q = 0; q < d.gp; q++) {
+  // the fast build's fused flux stores one Gauss-point sum per face
+  const int ng = GKS_FAST && d.presum ? 1 : d.gp;
+  for (int q = 0; q < ng; q++) {
 #pragma unroll
     for (int v = 0; v < 4; v++) {
       const long long oy = (long long)(q * 4 + v) * d.nfy, ox = (long long)(q * 4 + v) * d.nfx;
@@ -1279,6 +1411,7 @@ __device__ __forceinline__ void alpha_body(const Dev& d, unsigned blk) {
   if (t >= (long long)d.NX * nj) return;
   const int i = (int)(t / nj) + d.G, j = (int)(t % nj) + d.G;
   auto face_alpha = [&](const double* ak, long long nf, long long f) {
+    if (d.presum) return ak[f];  // formed by the fused flux launch
     double s;
     if (d.gp == 2) s = 0.5 * (ak[f] + ak[nf + f]);
     else if (d.gp == 3) s = (ak[f] + ak[nf + f] + ak[2 * nf + f]) / 3.0;

@@ -11,6 +11,8 @@ struct Dev {
   int fuse_tanflux;      // WENO5: tangential reconstruction inside the flux launch
   int keep_face_fluxes;  // store raw (f, derf) per face for parity checks
   int df_frozen;         // the DF sensor is never updated (gks2d_params.df_frozen)
+  int presum;            // fused WENO5 flux: ak holds the face's DF value (Gauss-point mean, thresholded)
+                         // and, in the fast build, xc / pc the face's Gauss-point sums (plane 0)
   double dx, dy, area;
   double idx, idy;  // 1/dx, 1/dy (fast build)
   double gw[4];
