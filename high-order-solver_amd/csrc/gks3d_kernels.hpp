@@ -530,13 +530,41 @@ struct LdsSide {
   const double* p;
   __device__ __forceinline__ double operator[](int k) const { return p[k]; }
 };
+// Workgroups go to the eight XCDs round robin, each with its own L2; the
+// tile of workgroup b is the (b % 8)-th contiguous eighth's (b / 8)-th, so
+// that tiles next to each other in the order below run on one XCD at about
+// the same time.
+__device__ __forceinline__ unsigned xcd_tile() {
+  const unsigned nb = gridDim.x, b = blockIdx.x, q = nb / 8, r = nb % 8, x = b % 8;
+  return x * q + (x < r ? x : r) + b / 8;
+}
+// tiles of k3_pflux<DIR>: x and y faces, kPfFaces consecutive faces of the
+// box (k fastest: the point stencil runs along k, inside the tile); z faces,
+// whose point stencil runs along i, (ny + 1)(nz + 1) faces apart, kPfFaces
+// consecutive faces of one (j, k) plane with i fastest from tile to tile, so
+// that the five tiles of a stencil read their line values through one L2
+__host__ __device__ inline long long pflux_tiles(const Box3& b, int dir) {
+  const long long plane = (long long)b.n[1] * b.n[2];
+  return dir == 2 ? (long long)b.n[0] * ((plane + kPfFaces - 1) / kPfFaces)
+                  : ((long long)b.n[0] * plane + kPfFaces - 1) / kPfFaces;
+}
 template <int DIR, bool LEARNED>
 __global__ void __launch_bounds__(kPfBlock) k3_pflux(Dev3 d, Box3 b, int stage, double dt) {
   __shared__ double s_pt[kPfFaces][2][4][kPfRow];
   const int lane = threadIdx.x;
   const int fl = lane >> 2;
-  const long long fi = (long long)blockIdx.x * kPfFaces + fl;
-  const bool active = fi < b.count();  // the four lanes of a face agree
+  long long fi;  // the face's index in the box (k fastest)
+  bool active;   // the four lanes of a face agree
+  if (DIR == 2) {
+    const long long plane = (long long)b.n[1] * b.n[2];
+    const unsigned t = xcd_tile(), ti = t % (unsigned)b.n[0], tp = t / (unsigned)b.n[0];
+    const long long p = (long long)tp * kPfFaces + fl;
+    active = p < plane;
+    fi = (long long)ti * plane + p;
+  } else {
+    fi = (long long)blockIdx.x * kPfFaces + fl;
+    active = fi < b.count();
+  }
   int i = 0, j = 0, k = 0;
   if (active) b.at(fi, i, j, k);
   const long long f = active ? face3(d, i, j, k) : 0;
@@ -769,7 +797,7 @@ static hipError_t launch_line(const Dev3& d, int dir, hipStream_t s) {
 static hipError_t launch_flux(const Dev3& d, int dir, int stage, double dt, hipStream_t s) {
   const Box3 b = flux_box(d, dir);
   if (box_count(b) == 0) return hipSuccess;
-  const unsigned nb = (unsigned)((box_count(b) + kPfFaces - 1) / kPfFaces);
+  const unsigned nb = (unsigned)pflux_tiles(b, dir);
   if (d.mlp) {
     if (dir == 0) k3_pflux<0, true><<<nb, kPfBlock, 0, s>>>(d, b, stage, dt);
     else if (dir == 1) k3_pflux<1, true><<<nb, kPfBlock, 0, s>>>(d, b, stage, dt);
