@@ -443,7 +443,10 @@ def bench_tgv_dissipation(args):
 
     n, tend = args.n3d, args.t3d
     fast = 1 if args.variant == "fast" else 0
-    P, thr, _ = learn.load(args.classifier or learn.DEFAULT_PARAMS)
+    P, thr, meta = learn.load(args.classifier or learn.DEFAULT_PARAMS)
+    # the threshold calibration covers the 2D four-quadrant Riemann problems:
+    # the 3D path takes the network's trained (validation) threshold
+    thr = learn.trained_threshold(meta, thr)
     bc = [BC_PERIODIC] * 6
     legs = {}
     for tag, cls in (("plain", None), ("learned", (P, thr))):

@@ -36,6 +36,9 @@ using namespace gksd;
 
 constexpr int kBlock = kSweepBlock;
 constexpr int kPartialBlocks = kSweepPartials;
+// ghost layers of every 2D context (gks2d_create accepts 5 only); sizes the
+// kernels' LDS staging of stencil halos
+constexpr int kGhost2 = 5;
 
 __device__ __forceinline__ long long gtid() { return (long long)blockIdx.x * blockDim.x + threadIdx.x; }
 
@@ -1275,24 +1278,108 @@ __device__ __forceinline__ void tanflux_cheap_body(const Dev& d, unsigned blk, u
   const long long nf = DIR == 0 ? d.nfx : d.nfy;
   {
     const int side = hi;
-    // the line values load before the rule is known, so that their latency
-    // overlaps the rule's loads (most faces pass it); without the
-    // smooth-flow flux every face goes to the list
+    // without the smooth-flow flux every face goes to the list
     const bool want = valid && d.smooth_flux;
     double L[5][8];
-    if (want) {
-      const double* line = DIR == 0 ? d.line_x : d.line_y;
-#pragma unroll
-      for (int s = 0; s < 5; s++) {
-        const long long f = DIR == 0 ? (long long)i * d.ny + (j - 2 + s) : (long long)(i + 2 - s) * (d.ny + 1) + j;
-#pragma unroll
-        for (int k = 0; k < 8; k++) L[s][k] = line[(side * 8 + k) * nf + f];
-      }
-    }
     bool lin = false;
-    if (valid) {
-      lin = tangential_linear<DIR>(d, side, i, j);
-      count_linear(d.lstat + 2 * kStatSlots, lin);
+    if constexpr (DIR == 0) {
+      // x faces: the tile's faces r0 .. r0 + TF - 1 have consecutive line
+      // records i * ny + j, apart from a jump of 2G where the tile wraps to
+      // the next row, so their 5-point stencils cover the one index range
+      // [f0 - 2, f0 - 2 + XS) (f0: the tile's first face).  The workgroup
+      // stages that range of the 16 line planes, and of the alpha_y and
+      // lin[0] planes of the two cell rows the sides take them from (side 0:
+      // row i - 1, index - ny; side 1: row i), in LDS (over the Gauss-point
+      // tile, which is written only after everyone has read the stencils),
+      // instead of every thread loading its own stencils: each value once
+      // instead of five times.
+      // (rounded up to a multiple of 8, so that 16 XS is a multiple of the
+      // workgroup size: every thread issues the same number of loads)
+      constexpr int XS = (TF + 4 + 2 * kGhost2 + 7) / 8 * 8;
+      static_assert(16 * XS % (2 * TF) == 0, "whole staging rounds");
+      static_assert(18 * XS * sizeof(double) + 2 * XS <= sizeof(sg), "staging fits the Gauss-point tile");
+      double* st = &sg[0][0][0][0];  // [16][XS] line values, [2][XS] alpha_y
+      unsigned char* sl = reinterpret_cast<unsigned char*>(st + 18 * XS);  // [2][XS] lin[0]
+      const long long r0 = (long long)blk * TF;
+      const long long f0 = (long long)((int)(r0 / nj) + d.G) * d.ny + (int)(r0 % nj) + d.G;
+      const long long base = f0 - 2;
+      // every load of a thread is issued before the first LDS store, so
+      // that their latencies overlap (indices past the planes are clamped:
+      // no valid face's stencil reaches them)
+      constexpr int NL = (16 * XS + 2 * TF - 1) / (2 * TF), NA = (2 * XS + 2 * TF - 1) / (2 * TF);
+      double va[NA];
+      unsigned char vl[NA];
+#pragma unroll
+      for (int u = 0; u < NA; u++) {
+        const int e = threadIdx.x + u * 2 * TF;
+        const int sd = e / XS, p = e - sd * XS;
+        long long g = base + p - (sd == 0 ? d.ny : 0);
+        g = g < d.nc ? g : d.nc - 1;
+        if (e < 2 * XS) {
+          va[u] = d.ay[g];
+          vl[u] = d.lin[0][g];
+        }
+      }
+      if (d.smooth_flux) {
+        const double* line = d.line_x;
+        double vv[NL];
+#pragma unroll
+        for (int u = 0; u < NL; u++) {
+          const int e = threadIdx.x + u * 2 * TF;
+          const int c = e / XS, p = e - c * XS;
+          const long long g = base + p < nf ? base + p : nf - 1;
+          vv[u] = line[c * nf + g];
+        }
+#pragma unroll
+        for (int u = 0; u < NL; u++) st[threadIdx.x + u * 2 * TF] = vv[u];
+      }
+#pragma unroll
+      for (int u = 0; u < NA; u++) {
+        const int e = threadIdx.x + u * 2 * TF;
+        if (e < 2 * XS) {
+          st[16 * XS + e] = va[u];
+          sl[e] = vl[u];
+        }
+      }
+      const long long c0 = (long long)(side == 0 ? i - 1 : i) * d.ny + j;
+      const unsigned char across = valid ? d.lin[1][c0] : 0;
+      __syncthreads();
+      const int pos = (int)((long long)i * d.ny + j - f0);
+      if (want) {
+#pragma unroll
+        for (int s = 0; s < 5; s++)
+#pragma unroll
+          for (int k = 0; k < 8; k++) L[s][k] = st[(side * 8 + k) * XS + pos + s];
+      }
+      if (valid) {
+        // tangential_linear<0> on the staged values
+        double amax = 0.0;
+        unsigned smooth = across;
+#pragma unroll
+        for (int s = 0; s < 5; s++) amax = fmax(amax, st[(16 + side) * XS + pos + s]);
+#pragma unroll
+        for (int s = 0; s < 5; s++) smooth &= sl[side * XS + pos + s];
+        lin = smooth != 0 && amax == 0.0;
+      }
+      // the Gauss-point tile below overwrites the staged values
+      __syncthreads();
+      if (valid) count_linear(d.lstat + 2 * kStatSlots, lin);
+    } else {
+      // the line values load before the rule is known, so that their
+      // latency overlaps the rule's loads (most faces pass it)
+      if (want) {
+        const double* line = d.line_y;
+#pragma unroll
+        for (int s = 0; s < 5; s++) {
+          const long long f = (long long)(i + 2 - s) * (d.ny + 1) + j;
+#pragma unroll
+          for (int k = 0; k < 8; k++) L[s][k] = line[(side * 8 + k) * nf + f];
+        }
+      }
+      if (valid) {
+        lin = tangential_linear<DIR>(d, side, i, j);
+        count_linear(d.lstat + 2 * kStatSlots, lin);
+      }
     }
     slin[side][tl] = lin ? 1 : 0;
     if (lin && want) {

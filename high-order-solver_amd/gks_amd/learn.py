@@ -9,7 +9,11 @@ parameters go into a solver context (Solver2D.set_classifier); stencils the
 network calls smooth, with a zero DF sensor, take the linear weights.
 
     python -m gks_amd.learn train --out gks_amd/data/classifier_riemann.npz
-    python -m gks_amd.learn evaluate --params gks_amd/data/classifier_riemann.npz
+    python -m gks_amd.learn calibrate --params gks_amd/data/classifier_riemann.npz
+
+`train` sets the logit threshold from the validation stencils (at most
+--max-miss of the troubled ones below it); `calibrate` then replaces it by
+the largest threshold that keeps the equal-L1 condition on held-out runs.
 """
 import argparse
 import ctypes
@@ -177,6 +181,74 @@ def pick_threshold(logits, y, max_miss=0.002):
     return float(bad[k]) if k < len(bad) else float(bad[-1])
 
 
+def _run_convar(p, prim, tstop, cls=None):
+    """Interior conservative state of a run to tstop (None if the scheme
+    blew up) and, with a classifier, its (linear, total) reconstruction
+    counts."""
+    s = Solver2D(p)
+    try:
+        s.init_prim(prim)
+        if cls is not None:
+            set_classifier(s, cls[0], cls[1])
+        try:
+            s.run([BC_FREE] * 4, tstop)
+        except GksError as e:
+            if e.code != 4:  # GKS_ERR_BLOWUP
+                raise
+            return None, None
+        G = p.ghost
+        u = s.get_convar()[:, G:G + p.nodex, G:G + p.nodey]
+        return u, (classifier_stats(s) if cls is not None else None)
+    finally:
+        s.close()
+
+
+def calibrate(P, thresholds, n=512, seeds=range(132, 142), tstop=0.25, margin=1.02, fast_math=1, log=None):
+    """The logit threshold, chosen on held-out seeded four-quadrant Riemann
+    runs at n^2 (seeds neither trained on nor benchmarked) by the equal-L1
+    condition itself: the largest of `thresholds` at which, on every
+    calibration problem the plain path finishes, the learned run also
+    finishes and the L1 error of each conservative variable (against a plain
+    run at (2n)^2 averaged over 2 x 2 blocks) is at most `margin` times the
+    plain run's.  Returns (threshold, table of per-threshold worst ratios,
+    blow-ups and linear shares)."""
+    thresholds = sorted(float(t) for t in thresholds)
+    worst = {t: 0.0 for t in thresholds}
+    blown = {t: 0 for t in thresholds}
+    share = {t: [] for t in thresholds}
+    for seed in seeds:
+        p = planar_shock_params(n, fast_math=fast_math)
+        prim = riemann_prim(p, riemann_states(seed))
+        up, _ = _run_convar(p, prim, tstop)
+        if up is None:
+            if log:
+                log(f"calibration seed {seed}: the plain run blows up; left out")
+            continue
+        pr = planar_shock_params(2 * n, fast_math=fast_math)
+        ur, _ = _run_convar(pr, riemann_prim(pr, riemann_states(seed)), tstop)
+        if ur is None:
+            if log:
+                log(f"calibration seed {seed}: the plain run at {2 * n}^2 blows up; left out")
+            continue
+        ref = ur.reshape(4, n, 2, n, 2).mean(axis=(2, 4))
+        ep = np.abs(up - ref).mean(axis=(1, 2))
+        for t in thresholds:
+            ul, st = _run_convar(p, prim, tstop, cls=(P, t))
+            if ul is None:
+                blown[t] += 1
+                continue
+            r = float(np.max(np.abs(ul - ref).mean(axis=(1, 2)) / ep))
+            worst[t] = max(worst[t], r)
+            share[t].append(st[0] / max(st[1], 1))
+        if log:
+            log(f"calibration seed {seed}: worst L1 ratio so far per threshold "
+                + ", ".join(f"{t:g}: {worst[t]:.4f}" for t in thresholds))
+    ok = [t for t in thresholds if blown[t] == 0 and worst[t] <= margin]
+    table = [{"threshold": t, "worst_l1_ratio": round(worst[t], 5), "blowups": blown[t],
+              "linear_share": round(float(np.mean(share[t])), 4) if share[t] else None} for t in thresholds]
+    return (max(ok) if ok else None), table
+
+
 def save(path, P, threshold, meta):
     os.makedirs(os.path.dirname(os.path.abspath(path)), exist_ok=True)
     np.savez(path, params=np.asarray(P, dtype=np.float32), threshold=np.float32(threshold),
@@ -184,8 +256,19 @@ def save(path, P, threshold, meta):
 
 
 def load(path=DEFAULT_PARAMS):
+    """(parameters, logit threshold, meta).  The threshold is the one the
+    2D four-quadrant Riemann problems were calibrated for (`calibrate`), or
+    the trained one if the parameters were never calibrated."""
     z = np.load(path, allow_pickle=False)
     return z["params"].astype(np.float32), float(z["threshold"]), json.loads(str(z["meta"]))
+
+
+def trained_threshold(meta, threshold):
+    """The threshold `train` chose from the validation stencils (at most
+    max_miss of the troubled ones below it): the conservative choice for
+    flows no calibration covers, such as the 3D Taylor-Green vortex."""
+    rule = meta.get("threshold_rule")
+    return float(rule["trained_threshold"]) if rule else float(threshold)
 
 
 def set_classifier(solver, P, threshold):
@@ -265,9 +348,38 @@ def _train_main(args):
     log(f"saved {args.out}")
 
 
+def _calibrate_main(args):
+    log = lambda *a: print(*a, flush=True)
+    P, thr0, meta = load(args.params)
+    grid = [float(v) for v in args.grid.split(",")]
+    seeds = range(args.first_seed, args.first_seed + args.seeds)
+    t0 = time.time()
+    thr, table = calibrate(P, grid, n=args.n, seeds=seeds, margin=args.margin, log=log)
+    log(f"calibrated in {time.time() - t0:.1f} s: threshold {thr} (was {thr0:.4f})")
+    for row in table:
+        log(f"  {row}")
+    if thr is None:
+        log("no threshold of the grid meets the condition; the parameters are left unchanged")
+        return 1
+    meta = dict(meta, threshold_rule=dict(
+        kind="equal-L1 calibration", n=args.n, seeds=[int(s) for s in seeds], margin=args.margin,
+        grid=grid, table=table, trained_threshold=float(meta.get("trained_threshold", thr0))))
+    save(args.out or args.params, P, thr, meta)
+    log(f"saved {args.out or args.params}")
+    return 0
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser(prog="python -m gks_amd.learn")
     sub = ap.add_subparsers(dest="cmd", required=True)
+    c = sub.add_parser("calibrate", help="choose the logit threshold on held-out runs by the equal-L1 condition")
+    c.add_argument("--params", default=DEFAULT_PARAMS)
+    c.add_argument("--out", default=None, help="where to save (default: over --params)")
+    c.add_argument("--n", type=int, default=512)
+    c.add_argument("--first-seed", type=int, default=132)
+    c.add_argument("--seeds", type=int, default=10)
+    c.add_argument("--margin", type=float, default=1.02)
+    c.add_argument("--grid", default="-3.5,-3,-2.5,-2,-1.5,-1,-0.5,0,0.5,1,1.5,2")
     t = sub.add_parser("train")
     t.add_argument("--out", default=DEFAULT_PARAMS)
     t.add_argument("--n", type=int, default=512)
@@ -285,6 +397,8 @@ def main(argv=None):
     args = ap.parse_args(argv)
     if args.cmd == "train":
         _train_main(args)
+    elif args.cmd == "calibrate":
+        return _calibrate_main(args)
 
 
 if __name__ == "__main__":
