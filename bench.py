@@ -57,12 +57,12 @@ PEAK_FP64_TFLOPS = 78.6  # MI355X FP64 (vector and matrix)
 # rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE (profiles/gks2d_riemann2048_fast_hbm_traffic.md),
 # each scaled by the counter's reading for k_copy_old (4 x 2048^2 doubles read
 # and written): fetch x1.882, write x0.985
-TRAFFIC_2048_FAST = {"flux_x": 1.131e9, "flux_y": 1.139e9}
+TRAFFIC_2048_FAST = {"flux_x": 0.836e9, "flux_y": 0.844e9}
 # FP64 operations the fast flux kernel executes per launch at 2048^2, from
 # rocprofv3 --pmc SQ_INSTS_VALU_{ADD,MUL,FMA,TRANS}_F64 (x64 lanes, FMA = 2)
 # (profiles/gks2d_riemann2048_fast_hbm_traffic.md); below the algorithmic
 # count: reciprocal multiplication, FMA contraction, whole-space tables
-EXECUTED_FLOPS_2048_FAST = {"flux_x": 39.73e9, "flux_y": 39.74e9}
+EXECUTED_FLOPS_2048_FAST = {"flux_x": 39.77e9, "flux_y": 39.77e9}
 PEAK_HBM_GBS = 8000.0
 # 3D units (oracle/flopcount3d, WENO5-AO characteristic, GKS3D gks2nd S2O4):
 # FP64 operations per unit of each kernel phase
