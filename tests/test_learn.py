@@ -65,3 +65,46 @@ def test_learn_abi_exports():
     assert not [n for n in names if not hasattr(lib, n)]
     assert set(names) <= set(learn._SIGS)
     assert learn.NPARAM == mo.NPARAM
+
+
+def test_threshold_calibration_rule(monkeypatch):
+    """gks_amd.learn.calibrate on the host logic alone (the solver runs
+    replaced by a stand-in whose learned error grows with the threshold):
+    the largest threshold whose worst per-variable L1 ratio over the
+    calibration problems stays within the margin and that blows up on none
+    of them; problems whose plain run or reference blows up are left out."""
+    sys.path.insert(0, os.path.join(ROOT, "high-order-solver_amd"))
+    from gks_amd import learn
+
+    base = np.ones((4, 8, 8))
+
+    def fake_run(p, prim, tstop, cls=None):
+        seed = int(prim)
+        if p.nodex == 8 and seed == 3:  # the 2n reference of seed 3 blows up
+            return None, None
+        if p.nodex == 16:  # the reference: zero error against itself
+            return np.zeros((4, 16, 16)), None
+        if cls is None:
+            return base, None
+        t = cls[1]
+        if seed == 1 and t >= 1.0:  # blows up only on the learned path
+            return None, None
+        # error ratio 1 + 0.01 * (t + 2) on seed 0, 1 + 0.005 * (t + 2) on the others
+        k = 0.01 if seed == 0 else 0.005
+        return base * (1 + k * (t + 2)), (9, 10)
+
+    monkeypatch.setattr(learn, "_run_convar", fake_run)
+    monkeypatch.setattr(learn, "riemann_prim", lambda p, st: st)
+    monkeypatch.setattr(learn, "riemann_states", lambda seed: seed)
+    grid = [-2.0, -1.5, -1.0, -0.5, 0.0, 0.5, 1.0]
+    # with n = 8 the calibration runs at 8 and its references at 16
+    thr, table = learn.calibrate(None, grid, n=8, seeds=range(4), margin=1.02)
+    rows = {r["threshold"]: r for r in table}
+    assert thr == 0.0  # seed 0: 1 + 0.01 * 2 = 1.02 at t = 0, above it at 0.5
+    assert rows[0.5]["worst_l1_ratio"] > 1.02 and rows[0.0]["worst_l1_ratio"] <= 1.02
+    assert rows[1.0]["blowups"] == 1 and rows[0.0]["blowups"] == 0
+    # the reference blow-up of seed 3 removed it: three problems contribute
+    assert rows[-2.0]["linear_share"] == 0.9
+    # no threshold meets a margin below every ratio
+    thr2, _ = learn.calibrate(None, grid, n=8, seeds=range(4), margin=0.999)
+    assert thr2 is None
