@@ -637,18 +637,29 @@ GKS_DI void gks_point_smooth(double* f, double* derf, const V& L, const V& Rr, d
   const TabWhole w0{m0};
   const double unit[4] = {1.0, 0.0, 0.0, 0.0};
   double g0u[4], a0x[4], a0y[4], a0xu[4], a0xuu[4], a0yv[4], a0yuv[4], derA0[4], A0[4], A0u[4];
+  const double tau = tau_ns(p0[0], p0[3], g);
   contract(1, 0, 0, g0u, unit, w0);
   micro_slope(a0x, c0x, p0, g);
   contract(1, 0, 0, a0xu, a0x, w0);
-  contract(2, 0, 0, a0xuu, a0x, w0);
   micro_slope(a0y, c0y, p0, g);
   contract(0, 1, 0, a0yv, a0y, w0);
-  contract(1, 1, 0, a0yuv, a0y, w0);
+  // the second-order slope terms s4 = a0xuu + a0yuv enter f only through
+  // tau * s4 (and the heat-flux correction): with tau = 0 (the Euler
+  // collision time) they are not formed
+  if (tau != 0.0 || g.prandtl_fix) {
+    contract(2, 0, 0, a0xuu, a0x, w0);
+    contract(1, 1, 0, a0yuv, a0y, w0);
+  } else {
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+      a0xuu[i] = 0.0;
+      a0yuv[i] = 0.0;
+    }
+  }
 #pragma unroll
   for (int i = 0; i < 4; i++) derA0[i] = -p0[0] * (a0xu[i] + a0yv[i]);
   micro_slope(A0, derA0, p0, g);
   contract(1, 0, 0, A0u, A0, w0);
-  const double tau = tau_ns(p0[0], p0[3], g);
 #pragma unroll
   for (int i = 0; i < 4; i++) {
     f[i] = p0[0] * dt * (g0u[i] - tau * ((a0xuu[i] + a0yuv[i]) + A0u[i]));

@@ -543,6 +543,30 @@ __device__ __forceinline__ void store_normal_faces(const Dev& d, int i, int j, c
   }
 }
 
+// the linear-weight normal reconstruction of one cell (the fifth-order
+// candidate on the conservative values, the order-reduction test kept):
+// lo the state at its left face (side 1 of that face), ro at its right
+// face (side 0); w[5][4] its stencil in the x-like frame
+__device__ __forceinline__ void linear_normal(double* lo, double* ro, const double (*w)[4], double h, double ih,
+                                              const Gas& g) {
+#pragma unroll
+  for (int v = 0; v < 4; v++) {
+    weno5_linear_face(0, lo[v], lo[4 + v], w[0][v], w[1][v], w[2][v], w[3][v], w[4][v], h, ih);
+    weno5_linear_face(1, ro[v], ro[4 + v], w[0][v], w[1][v], w[2][v], w[3][v], w[4][v], h, ih);
+  }
+  const bool rr = reduce_order(ro, g);
+  const bool rl = reduce_order(lo, g);
+  if (rl || rr) {
+#pragma unroll
+    for (int m = 0; m < 4; m++) {
+      ro[m] = w[2][m];
+      lo[m] = w[2][m];
+      ro[4 + m] = 0.0;
+      lo[4 + m] = 0.0;
+    }
+  }
+}
+
 // k_normal with the classifier: a stencil the network calls smooth (logit
 // below the threshold) with a zero DF sensor takes the linear weights, i.e.
 // the fifth-order candidate, formed directly on the conservative variables
@@ -586,24 +610,8 @@ __device__ __forceinline__ void normal_learned_body(const Dev& d, unsigned blk, 
   const int sh = list_shard(blk, ntiles);
   list_append(shard_count(d.ncount, DIR, sh), d.nlist[DIR] + (size_t)sh * d.ncap[DIR], !lin, (unsigned)t);
   if (!lin) return;
-  const double h = DIR == 0 ? d.dx : d.dy, ih = DIR == 0 ? d.idx : d.idy;
   double lo[8], ro[8];
-#pragma unroll
-  for (int v = 0; v < 4; v++) {
-    weno5_linear_face(0, lo[v], lo[4 + v], w[0][v], w[1][v], w[2][v], w[3][v], w[4][v], h, ih);
-    weno5_linear_face(1, ro[v], ro[4 + v], w[0][v], w[1][v], w[2][v], w[3][v], w[4][v], h, ih);
-  }
-  const bool rr = reduce_order(ro, d.gas);
-  const bool rl = reduce_order(lo, d.gas);
-  if (rl || rr) {
-#pragma unroll
-    for (int m = 0; m < 4; m++) {
-      ro[m] = w[2][m];
-      lo[m] = w[2][m];
-      ro[4 + m] = 0.0;
-      lo[4 + m] = 0.0;
-    }
-  }
+  linear_normal(lo, ro, w, DIR == 0 ? d.dx : d.dy, DIR == 0 ? d.idx : d.idy, d.gas);
   store_normal_faces<DIR>(d, i, j, lo, ro);
 }
 template <int DIR>
@@ -1267,8 +1275,16 @@ template <int DIR> __device__ __forceinline__ bool tangential_linear(const Dev& 
 template <int DIR, int TF>
 __device__ __forceinline__ void tanflux_cheap_body(const Dev& d, unsigned blk, unsigned ntiles, int stage,
                                                    double dt) {
-  __shared__ double sg[2][12][TF][2];  // [side][component][face][gauss point]
+  // x faces: the stencils of a tile's faces cover one index range of XS
+  // line records (below); rounded up to a multiple of 8, so that 16 XS is a
+  // multiple of the workgroup size (every thread issues the same number of
+  // loads when the line states are staged)
+  constexpr int XS = (TF + 4 + 2 * kGhost2 + 7) / 8 * 8;
+  constexpr int kSg = 2 * 12 * TF * 2, kStage = DIR == 0 ? 18 * XS : 0;
+  __shared__ double smem[kSg > kStage ? kSg : kStage];
+  double(&sg)[2][12][TF][2] = *reinterpret_cast<double(*)[2][12][TF][2]>(smem);  // [side][component][face][point]
   __shared__ unsigned char slin[2][TF];
+  __shared__ unsigned char sflag[DIR == 0 ? 2 * XS : 1];
   const int ni = DIR == 0 ? d.NX + 1 : d.NX, nj = DIR == 0 ? d.NY : d.NY + 1;
   const long long nfaces = (long long)ni * nj;
   const int tl = threadIdx.x % TF, hi = threadIdx.x / TF;
@@ -1288,78 +1304,95 @@ __device__ __forceinline__ void tanflux_cheap_body(const Dev& d, unsigned blk, u
       // the next row, so their 5-point stencils cover the one index range
       // [f0 - 2, f0 - 2 + XS) (f0: the tile's first face).  The workgroup
       // stages that range of the 16 line planes, and of the alpha_y and
-      // lin[0] planes of the two cell rows the sides take them from (side 0:
-      // row i - 1, index - ny; side 1: row i), in LDS (over the Gauss-point
-      // tile, which is written only after everyone has read the stencils),
-      // instead of every thread loading its own stencils: each value once
-      // instead of five times.
-      // (rounded up to a multiple of 8, so that 16 XS is a multiple of the
-      // workgroup size: every thread issues the same number of loads)
-      constexpr int XS = (TF + 4 + 2 * kGhost2 + 7) / 8 * 8;
+      // lin[0] planes of the two cell rows the sides take them from (side
+      // 0: row i - 1, index - ny; side 1: row i), in LDS (over the
+      // Gauss-point tile, which is written only after everyone has read the
+      // stencils), instead of every thread loading its own stencils: each
+      // value once instead of five times.
+      static_assert(18 * XS <= (int)(sizeof(smem) / sizeof(double)), "staging fits");
       static_assert(16 * XS % (2 * TF) == 0, "whole staging rounds");
-      static_assert(18 * XS * sizeof(double) + 2 * XS <= sizeof(sg), "staging fits the Gauss-point tile");
-      double* st = &sg[0][0][0][0];  // [16][XS] line values, [2][XS] alpha_y
-      unsigned char* sl = reinterpret_cast<unsigned char*>(st + 18 * XS);  // [2][XS] lin[0]
       const long long r0 = (long long)blk * TF;
       const long long f0 = (long long)((int)(r0 / nj) + d.G) * d.ny + (int)(r0 % nj) + d.G;
       const long long base = f0 - 2;
-      // every load of a thread is issued before the first LDS store, so
-      // that their latencies overlap (indices past the planes are clamped:
-      // no valid face's stencil reaches them)
-      constexpr int NL = (16 * XS + 2 * TF - 1) / (2 * TF), NA = (2 * XS + 2 * TF - 1) / (2 * TF);
-      double va[NA];
-      unsigned char vl[NA];
-#pragma unroll
-      for (int u = 0; u < NA; u++) {
-        const int e = threadIdx.x + u * 2 * TF;
-        const int sd = e / XS, p = e - sd * XS;
-        long long g = base + p - (sd == 0 ? d.ny : 0);
-        g = g < d.nc ? g : d.nc - 1;
-        if (e < 2 * XS) {
-          va[u] = d.ay[g];
-          vl[u] = d.lin[0][g];
-        }
-      }
-      if (d.smooth_flux) {
-        const double* line = d.line_x;
-        double vv[NL];
-#pragma unroll
-        for (int u = 0; u < NL; u++) {
-          const int e = threadIdx.x + u * 2 * TF;
-          const int c = e / XS, p = e - c * XS;
-          const long long g = base + p < nf ? base + p : nf - 1;
-          vv[u] = line[c * nf + g];
-        }
-#pragma unroll
-        for (int u = 0; u < NL; u++) st[threadIdx.x + u * 2 * TF] = vv[u];
-      }
-#pragma unroll
-      for (int u = 0; u < NA; u++) {
-        const int e = threadIdx.x + u * 2 * TF;
-        if (e < 2 * XS) {
-          st[16 * XS + e] = va[u];
-          sl[e] = vl[u];
-        }
-      }
-      const long long c0 = (long long)(side == 0 ? i - 1 : i) * d.ny + j;
-      const unsigned char across = valid ? d.lin[1][c0] : 0;
-      __syncthreads();
       const int pos = (int)((long long)i * d.ny + j - f0);
-      if (want) {
+      unsigned char* slin0 = sflag;  // [2][XS]: lin[0] of side 0's and side 1's source cells
+      // the tile's index range fits XS when it wraps at most once (always
+      // for NY >= TF; a narrower grid's tile can cover three rows)
+      const long long rl = (r0 + TF < nfaces ? r0 + TF : nfaces) - 1;
+      const long long fl = (long long)((int)(rl / nj) + d.G) * d.ny + (int)(rl % nj) + d.G;
+      const bool staged = fl - f0 + 5 <= XS;
+      if (!staged) {
+        // each thread loads its own stencils
+        if (want) {
 #pragma unroll
-        for (int s = 0; s < 5; s++)
+          for (int s = 0; s < 5; s++) {
+            const long long f = (long long)i * d.ny + (j - 2 + s);
 #pragma unroll
-          for (int k = 0; k < 8; k++) L[s][k] = st[(side * 8 + k) * XS + pos + s];
-      }
-      if (valid) {
-        // tangential_linear<0> on the staged values
-        double amax = 0.0;
-        unsigned smooth = across;
+            for (int k = 0; k < 8; k++) L[s][k] = d.line_x[(side * 8 + k) * nf + f];
+          }
+        }
+        if (valid) lin = tangential_linear<0>(d, side, i, j);
+      } else {
+        // the line states, and the alpha_y and lin[0] planes of the two
+        // source cell rows, staged from HBM
+        double* st = smem;  // [16][XS] line values, [2][XS] alpha_y
+        // every load of a thread is issued before the first LDS store, so
+        // that their latencies overlap (indices past the planes are
+        // clamped: no valid face's stencil reaches them)
+        constexpr int NL = (16 * XS + 2 * TF - 1) / (2 * TF), NA = (2 * XS + 2 * TF - 1) / (2 * TF);
+        double va[NA];
+        unsigned char vl[NA];
 #pragma unroll
-        for (int s = 0; s < 5; s++) amax = fmax(amax, st[(16 + side) * XS + pos + s]);
+        for (int u = 0; u < NA; u++) {
+          const int e = threadIdx.x + u * 2 * TF;
+          const int sd = e / XS, p = e - sd * XS;
+          long long g = base + p - (sd == 0 ? d.ny : 0);
+          g = g < d.nc ? g : d.nc - 1;
+          if (e < 2 * XS) {
+            va[u] = d.ay[g];
+            vl[u] = d.lin[0][g];
+          }
+        }
+        if (d.smooth_flux) {
+          const double* line = d.line_x;
+          double vv[NL];
 #pragma unroll
-        for (int s = 0; s < 5; s++) smooth &= sl[side * XS + pos + s];
-        lin = smooth != 0 && amax == 0.0;
+          for (int u = 0; u < NL; u++) {
+            const int e = threadIdx.x + u * 2 * TF;
+            const int c = e / XS, p = e - c * XS;
+            const long long g = base + p < nf ? base + p : nf - 1;
+            vv[u] = line[c * nf + g];
+          }
+#pragma unroll
+          for (int u = 0; u < NL; u++) st[threadIdx.x + u * 2 * TF] = vv[u];
+        }
+#pragma unroll
+        for (int u = 0; u < NA; u++) {
+          const int e = threadIdx.x + u * 2 * TF;
+          if (e < 2 * XS) {
+            st[16 * XS + e] = va[u];
+            slin0[e] = vl[u];
+          }
+        }
+        const long long c0 = (long long)(side == 0 ? i - 1 : i) * d.ny + j;
+        const unsigned char across = valid ? d.lin[1][c0] : 0;
+        __syncthreads();
+        if (want) {
+#pragma unroll
+          for (int s = 0; s < 5; s++)
+#pragma unroll
+            for (int k = 0; k < 8; k++) L[s][k] = st[(side * 8 + k) * XS + pos + s];
+        }
+        if (valid) {
+          // tangential_linear<0> on the staged values
+          double amax = 0.0;
+          unsigned smooth = across;
+#pragma unroll
+          for (int s = 0; s < 5; s++) amax = fmax(amax, st[(16 + side) * XS + pos + s]);
+#pragma unroll
+          for (int s = 0; s < 5; s++) smooth &= slin0[side * XS + pos + s];
+          lin = smooth != 0 && amax == 0.0;
+        }
       }
       // the Gauss-point tile below overwrites the staged values
       __syncthreads();
