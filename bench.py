@@ -74,7 +74,7 @@ FLOPS3_CELL_STEP = 367644  # one interior cell's S2O4 step (2 stages x 3 directi
 # HBM bytes per k3_pflux launch at 256^3, fast kernels, mean of the three
 # directions, from rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE scaled by the
 # k3_copy_old calibration (profiles/gks3d_tgv256_fast_hbm_traffic.md)
-TRAFFIC3_256_FAST = {"flux": 28.55e9}
+TRAFFIC3_256_FAST = {"flux": 13.47e9}
 # FP64 operations k3_pflux executes per launch at 256^3, fast kernels
 # (SQ_INSTS_VALU_{ADD,MUL,FMA,TRANS}_F64 x64, FMA = 2;
 # profiles/gks3d_tgv256_fast_kernel_trace.md)
