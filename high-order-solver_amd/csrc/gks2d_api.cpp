@@ -204,13 +204,13 @@ int gks2d_create(const gks2d_params* p, gks2d_ctx** out) {
   d.idx = 1.0 / d.dx;
   d.idy = 1.0 / d.dy;
   d.order = p->weno_order;
-  // WENO5: the tangential reconstruction runs inside the flux launch (its
+  // the tangential reconstruction runs inside the flux launch (its
   // Gauss-point states stay in LDS); GKS_TANFLUX=0 keeps the two launches
   d.df_frozen = p->df_frozen ? 1 : 0;
-  d.fuse_tanflux = p->weno_order == 5 && !(std::getenv("GKS_TANFLUX") && std::getenv("GKS_TANFLUX")[0] == '0');
+  d.fuse_tanflux = !(std::getenv("GKS_TANFLUX") && std::getenv("GKS_TANFLUX")[0] == '0');
   // the two Gauss points of a face meet in one lane pair of the fused
   // launch, which stores per-face values for k_alpha and k_update
-  d.presum = d.fuse_tanflux;
+  d.presum = d.fuse_tanflux && p->weno_order == 5;
   if (d.gp == 2) {  // SetGuassPoint, fluid_mesh.cpp:9-45
     d.gw[0] = 0.5;
     d.gw[1] = 0.5;

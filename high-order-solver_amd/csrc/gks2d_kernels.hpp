@@ -777,14 +777,11 @@ template <int DIR, int ORDER> __global__ void __launch_bounds__(kBlock) k_normal
 // the outermost two points keep the left side's projection (`quirk`: left
 // side convar of line points 0, r, 2r), as the reference leaves them
 // (:3306-3312).
-template <int DIR, int ORDER> __global__ void __launch_bounds__(kBlock) k_tangent79(Dev d) {
+// the tangential reconstruction of one side of x / y face (i, j): its NG
+// Gauss-point states (convar, der1x, der1y) go to put(q, component, value)
+template <int DIR, int ORDER, class Put>
+__device__ __forceinline__ void tangent79_side(const Dev& d, int side, int i, int j, const Put& put) {
   constexpr int r = (ORDER - 1) / 2, NS = ORDER == 7 ? 5 : 6, NG = ORDER == 7 ? 3 : 4;
-  long long t = gtid();
-  const int ni = DIR == 0 ? d.NX + 1 : d.NX, nj = DIR == 0 ? d.NY : d.NY + 1;
-  if (t >= 2LL * ni * nj) return;
-  const int side = (int)(t / ((long long)ni * nj));
-  const long long rr = t % ((long long)ni * nj);
-  const int i = (int)(rr / nj) + d.G, j = (int)(rr % nj) + d.G;
   const double* line = DIR == 0 ? d.line_x : d.line_y;
   const long long nf = DIR == 0 ? d.nfx : d.nfy;
   long long fs[2 * r + 1];
@@ -868,14 +865,12 @@ template <int DIR, int ORDER> __global__ void __launch_bounds__(kBlock) k_tangen
   bool red[NG];
 #pragma unroll
   for (int q = 0; q < NG; q++) red[q] = reduce_order(gv[q], d.gas);
-  double* gst = DIR == 0 ? d.gst_x : d.gst_y;
-  const long long f0 = DIR == 0 ? (long long)i * d.ny + j : (long long)i * (d.ny + 1) + j;
 #pragma unroll
   for (int q = 0; q < NG; q++)
 #pragma unroll
     for (int k = 0; k < 4; k++) {
-      gst[((side * d.gp + q) * 12 + k) * nf + f0] = red[q] ? L[r][k] : gv[q][k];
-      gst[((side * d.gp + q) * 12 + 8 + k) * nf + f0] = red[q] ? 0.0 : gy[q][k];
+      put(q, k, red[q] ? L[r][k] : gv[q][k]);
+      put(q, 8 + k, red[q] ? 0.0 : gy[q][k]);
     }
   // der1x along the face
 #pragma unroll
@@ -885,8 +880,23 @@ template <int DIR, int ORDER> __global__ void __launch_bounds__(kBlock) k_tangen
     for (int s = 0; s < 2 * r + 1; s++) w[s] = line[(side * 8 + 4 + k) * nf + fs[s]];
     weno79_gauss<ORDER>(v, vx, w + r, df, h, ih);
 #pragma unroll
-    for (int q = 0; q < NG; q++) gst[((side * d.gp + q) * 12 + 4 + k) * nf + f0] = red[q] ? w[r] : v[q];
+    for (int q = 0; q < NG; q++) put(q, 4 + k, red[q] ? w[r] : v[q]);
   }
+}
+
+template <int DIR, int ORDER> __global__ void __launch_bounds__(kBlock) k_tangent79(Dev d) {
+  long long t = gtid();
+  const int ni = DIR == 0 ? d.NX + 1 : d.NX, nj = DIR == 0 ? d.NY : d.NY + 1;
+  if (t >= 2LL * ni * nj) return;
+  const int side = (int)(t / ((long long)ni * nj));
+  const long long rr = t % ((long long)ni * nj);
+  const int i = (int)(rr / nj) + d.G, j = (int)(rr % nj) + d.G;
+  double* gst = DIR == 0 ? d.gst_x : d.gst_y;
+  const long long nf = DIR == 0 ? d.nfx : d.nfy;
+  const long long f0 = DIR == 0 ? (long long)i * d.ny + j : (long long)i * (d.ny + 1) + j;
+  tangent79_side<DIR, ORDER>(d, side, i, j, [&](int q, int k, double v) {
+    gst[((side * d.gp + q) * 12 + k) * nf + f0] = v;
+  });
 }
 
 // -------------------------------------------------------------------- flux
@@ -1231,6 +1241,48 @@ template <int DIR, bool LEARNED>
 __global__ void __launch_bounds__(2 * kTanFlux) __attribute__((amdgpu_waves_per_eu(GKS_FLUX_WAVES)))
 k_tanflux(Dev d, int stage, double dt) {
   tanflux_body<DIR, LEARNED, kTanFlux>(d, xcd_block(), stage, dt);
+}
+
+// WENO7 / WENO9 tangential reconstruction + flux in one launch, as
+// k_tanflux does for WENO5: a workgroup of 128 lanes takes 64 consecutive
+// faces.  Phase 1, lane (side, face): tangent79_side into an LDS tile of the
+// NG Gauss-point states of both sides (2 x 12 x 64 x NG doubles: 36 KB for
+// WENO7, 48 KB for WENO9); phase 2: k_flux's flux point for every (Gauss
+// point, face) of the tile, in two rounds of 128 (WENO7: 192 points, the
+// second round half full; WENO9: 256).  The Gauss-point states never reach
+// HBM.  WENO7 runs two waves per SIMD, WENO9 one (as its separate
+// tangential launch: at two its registers spill).
+template <int DIR, int ORDER>
+__global__ void __launch_bounds__(2 * kTanFlux) __attribute__((amdgpu_waves_per_eu(ORDER == 9 ? 1 : GKS_FLUX_WAVES)))
+k_tanflux79(Dev d, int stage, double dt) {
+  constexpr int NG = ORDER == 7 ? 3 : 4, TF = kTanFlux;
+  __shared__ double sg[2][12][TF][NG];  // [side][component][face][gauss point]
+  const unsigned blk = xcd_block();
+  const int ni = DIR == 0 ? d.NX + 1 : d.NX, nj = DIR == 0 ? d.NY : d.NY + 1;
+  const long long nfaces = (long long)ni * nj;
+  const long long nf = DIR == 0 ? d.nfx : d.nfy;
+  {
+    const int tl = threadIdx.x % TF, side = threadIdx.x / TF;
+    const long long r = (long long)blk * TF + tl;
+    if (r < nfaces) {
+      const int i = (int)(r / nj) + d.G, j = (int)(r % nj) + d.G;
+      tangent79_side<DIR, ORDER>(d, side, i, j, [&](int q, int k, double v) { sg[side][k][tl][q] = v; });
+    }
+  }
+  __syncthreads();
+  // two rounds written out (a loop around the flux point spills)
+  auto point = [&](int it) {
+    const int q = it / TF, fp = it - q * TF;
+    const long long r2 = (long long)blk * TF + fp;
+    if (it >= NG * TF || r2 >= nfaces) return;
+    const int i2 = (int)(r2 / nj) + d.G, j2 = (int)(r2 % nj) + d.G;
+    const long long f2 = DIR == 0 ? (long long)i2 * d.ny + j2 : (long long)i2 * (d.ny + 1) + j2;
+    const Strided L{&sg[0][0][fp][q], TF * NG}, R{&sg[1][0][fp][q], TF * NG};
+    flux_point_tail<DIR>(d, q, f2, nf, L, R, stage, dt);
+  };
+  static_assert(NG * TF <= 4 * TF, "at most two rounds");
+  point(threadIdx.x);
+  point(threadIdx.x + 2 * TF);
 }
 
 // ------------------------------------ learned path: the flux in two launches
@@ -1854,6 +1906,10 @@ template __global__ void kb_normal_list<0>(BatchArgs);
 template __global__ void kb_normal_list<1>(BatchArgs);
 template __global__ void k_label<0>(Dev, int, double, float*, float*);
 template __global__ void k_label<1>(Dev, int, double, float*, float*);
+template __global__ void k_tanflux79<0, 7>(Dev, int, double);
+template __global__ void k_tanflux79<1, 7>(Dev, int, double);
+template __global__ void k_tanflux79<0, 9>(Dev, int, double);
+template __global__ void k_tanflux79<1, 9>(Dev, int, double);
 template __global__ void k_flux<0>(Dev, int, double);
 template __global__ void k_flux<1>(Dev, int, double);
 
@@ -1898,13 +1954,15 @@ static hipError_t normal(const Dev& d, int dir, hipStream_t s) {
 }
 static hipError_t tangent(const Dev& d, int dir, hipStream_t s) {
   const unsigned nb = dir == 0 ? blocks_for(2LL * (d.NX + 1) * d.NY) : blocks_for(2LL * d.NX * (d.NY + 1));
+  // with the fused launch the tangential reconstruction runs inside flux()
+  if (d.fuse_tanflux) return hipSuccess;
   if (d.order == 7) {
     if (dir == 0) hipLaunchKernelGGL((k_tangent79<0, 7>), dim3(nb), dim3(kBlock), 0, s, d);
     else hipLaunchKernelGGL((k_tangent79<1, 7>), dim3(nb), dim3(kBlock), 0, s, d);
   } else if (d.order == 9) {
     if (dir == 0) hipLaunchKernelGGL((k_tangent79<0, 9>), dim3(nb), dim3(kBlock), 0, s, d);
     else hipLaunchKernelGGL((k_tangent79<1, 9>), dim3(nb), dim3(kBlock), 0, s, d);
-  } else if (!d.fuse_tanflux) {
+  } else {
     if (dir == 0) hipLaunchKernelGGL(k_tangent<0>, dim3(nb), dim3(kBlock), 0, s, d);
     else hipLaunchKernelGGL(k_tangent<1>, dim3(nb), dim3(kBlock), 0, s, d);
   }
@@ -1926,6 +1984,18 @@ static hipError_t flux(const Dev& d, int dir, int stage, double dt, hipStream_t 
     } else {
       if (dir == 0) hipLaunchKernelGGL((k_tanflux<0, false>), dim3(nb), dim3(2 * kTanFlux), 0, s, d, stage, dt);
       else hipLaunchKernelGGL((k_tanflux<1, false>), dim3(nb), dim3(2 * kTanFlux), 0, s, d, stage, dt);
+    }
+    return hipGetLastError();
+  }
+  if (d.fuse_tanflux) {
+    const long long nfaces = dir == 0 ? (long long)(d.NX + 1) * d.NY : (long long)d.NX * (d.NY + 1);
+    const unsigned nb = (unsigned)((nfaces + kTanFlux - 1) / kTanFlux);
+    if (d.order == 7) {
+      if (dir == 0) hipLaunchKernelGGL((k_tanflux79<0, 7>), dim3(nb), dim3(2 * kTanFlux), 0, s, d, stage, dt);
+      else hipLaunchKernelGGL((k_tanflux79<1, 7>), dim3(nb), dim3(2 * kTanFlux), 0, s, d, stage, dt);
+    } else {
+      if (dir == 0) hipLaunchKernelGGL((k_tanflux79<0, 9>), dim3(nb), dim3(2 * kTanFlux), 0, s, d, stage, dt);
+      else hipLaunchKernelGGL((k_tanflux79<1, 9>), dim3(nb), dim3(2 * kTanFlux), 0, s, d, stage, dt);
     }
     return hipGetLastError();
   }
