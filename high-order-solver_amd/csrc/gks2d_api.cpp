@@ -247,7 +247,9 @@ int gks2d_create(const gks2d_params* p, gks2d_ctx** out) {
   if ((rc = dev_alloc(c, &d.U, 4 * d.nc)) || (rc = dev_alloc(c, &d.Uold, 4 * d.nc)) ||
       (rc = dev_alloc(c, &d.ax, d.nc)) || (rc = dev_alloc(c, &d.ay, d.nc)) ||
       (rc = dev_alloc(c, &d.line_x, 16 * d.nfx)) || (rc = dev_alloc(c, &d.line_y, 16 * d.nfy)) ||
-      (rc = dev_alloc(c, &d.gst_x, 24 * gp * d.nfx)) || (rc = dev_alloc(c, &d.gst_y, 24 * gp * d.nfy)) ||
+      // Gauss-point states reach HBM only when the tangential launch is separate
+      (!d.fuse_tanflux && ((rc = dev_alloc(c, &d.gst_x, 24 * gp * d.nfx)) ||
+                           (rc = dev_alloc(c, &d.gst_y, 24 * gp * d.nfy)))) ||
       (rc = dev_alloc(c, &d.xcx, 4 * gp * d.nfx)) || (rc = dev_alloc(c, &d.xcy, 4 * gp * d.nfy)) ||
       (rc = dev_alloc(c, &d.pcx, 4 * gp * d.nfx)) || (rc = dev_alloc(c, &d.pcy, 4 * gp * d.nfy)) ||
       (rc = dev_alloc(c, &d.akx, gp * d.nfx)) || (rc = dev_alloc(c, &d.aky, gp * d.nfy)) ||
