@@ -634,6 +634,12 @@ static void list_caps(Dev& d) {
     d.ncap[dir] = shard_cap(ncl[dir], kSweepBlock);
   }
 }
+// the directions whose linear line values the smooth-face flux launch forms
+// itself: it runs when the smooth-flow flux is on (and the tangential
+// reconstruction is fused into the flux launch)
+static int fused_line_dirs(const Dev& d) {
+  return d.smooth_flux && d.fuse_tanflux && d.order == 5 ? kFusedLineDirs : 0;
+}
 static size_t list_words(const Dev& d) {
   return (size_t)4 * kListShards * kCountStride + 4 * 32 +
          (size_t)kListShards * ((size_t)d.tcap[0] + d.tcap[1] + d.ncap[0] + d.ncap[1]);
@@ -675,6 +681,7 @@ int gks2d_set_classifier(gks2d_ctx* c, const float* params, float logit_threshol
   c->d.nlist[1] = c->d.nlist[0] + (size_t)kListShards * c->d.ncap[0];
   c->d.mlp_thr = logit_threshold;
   c->d.lstat = c->d_lstat;
+  c->d.fused_lines = fused_line_dirs(c->d);
   return GKS_OK;
 }
 
@@ -683,6 +690,7 @@ int gks2d_classifier_options(gks2d_ctx* c, int32_t smooth_flux) {
   if (!c->d.mlp) return fail(GKS_ERR_STATE, "no classifier set");
   HIPCHK(hipStreamSynchronize(c->s));
   c->d.smooth_flux = smooth_flux ? 1 : 0;
+  c->d.fused_lines = fused_line_dirs(c->d);
   return GKS_OK;
 }
 
@@ -809,6 +817,8 @@ int batch_upload_devs(gks2d_batch* b, int* learned) {
     gks2d_ctx* c = b->ctx[p];
     HIPCHK(hipStreamSynchronize(c->s));
     h[p] = c->d;
+    // the batch's learned sweep always runs the smooth-face flux launch
+    if (h[p].mlp && h[p].order == 5) h[p].fused_lines = kFusedLineDirs;
     if (c->last_slot < 0) return fail(GKS_ERR_STATE, "batch problem without a primitive state: set its initial condition first");
   }
   const int l0 = h[0].mlp != nullptr;

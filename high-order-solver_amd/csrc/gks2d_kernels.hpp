@@ -609,7 +609,9 @@ __device__ __forceinline__ void normal_learned_body(const Dev& d, unsigned blk, 
   // troubled cells only
   const int sh = list_shard(blk, ntiles);
   list_append(shard_count(d.ncount, DIR, sh), d.nlist[DIR] + (size_t)sh * d.ncap[DIR], !lin, (unsigned)t);
-  if (!lin) return;
+  // linear cells' line values: formed by the smooth-face flux launch when
+  // it is set to (fused_lines)
+  if (!lin || (d.fused_lines >> DIR & 1)) return;
   double lo[8], ro[8];
   linear_normal(lo, ro, w, DIR == 0 ? d.dx : d.dy, DIR == 0 ? d.idx : d.idy, d.gas);
   store_normal_faces<DIR>(d, i, j, lo, ro);
@@ -1344,6 +1346,10 @@ __device__ __forceinline__ void tanflux_cheap_body(const Dev& d, unsigned blk, u
   const bool valid = r < nfaces;
   const int i = valid ? (int)(r / nj) + d.G : d.G, j = valid ? (int)(r % nj) + d.G : d.G;
   const long long nf = DIR == 0 ? d.nfx : d.nfy;
+  // fused_lines (x faces): the line values are the linear reconstructions
+  // of the cell averages, formed here; the ones the listed faces' stencils
+  // read are stored for the list launch
+  const bool fused = DIR == 0 && (d.fused_lines & 1);
   {
     const int side = hi;
     // without the smooth-flow flux every face goes to the list
@@ -1360,7 +1366,9 @@ __device__ __forceinline__ void tanflux_cheap_body(const Dev& d, unsigned blk, u
       // 0: row i - 1, index - ny; side 1: row i), in LDS (over the
       // Gauss-point tile, which is written only after everyone has read the
       // stencils), instead of every thread loading its own stencils: each
-      // value once instead of five times.
+      // value once instead of five times.  With fused_lines the workgroup
+      // forms that range of line values from the cell averages instead
+      // (a thread per column: its two source cells).
       static_assert(18 * XS <= (int)(sizeof(smem) / sizeof(double)), "staging fits");
       static_assert(16 * XS % (2 * TF) == 0, "whole staging rounds");
       const long long r0 = (long long)blk * TF;
@@ -1374,13 +1382,23 @@ __device__ __forceinline__ void tanflux_cheap_body(const Dev& d, unsigned blk, u
       const long long fl = (long long)((int)(rl / nj) + d.G) * d.ny + (int)(rl % nj) + d.G;
       const bool staged = fl - f0 + 5 <= XS;
       if (!staged) {
-        // each thread loads its own stencils
-        if (want) {
+        // each thread loads (or forms) its own stencils
+        if (valid && (want || fused)) {
 #pragma unroll
           for (int s = 0; s < 5; s++) {
-            const long long f = (long long)i * d.ny + (j - 2 + s);
+            if (fused) {
+              // side 0: the right face of cell (i - 1, j'), side 1: the
+              // left face of cell (i, j')
+              double w[5][4], a[5], lo[8], ro[8];
+              normal_stencil<0>(d, side == 0 ? i - 1 : i, j - 2 + s, w, a);
+              linear_normal(lo, ro, w, d.dx, d.idx, d.gas);
 #pragma unroll
-            for (int k = 0; k < 8; k++) L[s][k] = d.line_x[(side * 8 + k) * nf + f];
+              for (int k = 0; k < 8; k++) L[s][k] = side == 0 ? ro[k] : lo[k];
+            } else {
+              const long long f = (long long)i * d.ny + (j - 2 + s);
+#pragma unroll
+              for (int k = 0; k < 8; k++) L[s][k] = d.line_x[(side * 8 + k) * nf + f];
+            }
           }
         }
         if (valid) lin = tangential_linear<0>(d, side, i, j);
@@ -1388,6 +1406,7 @@ __device__ __forceinline__ void tanflux_cheap_body(const Dev& d, unsigned blk, u
         // the line states, and the alpha_y and lin[0] planes of the two
         // source cell rows, staged from HBM
         double* st = smem;  // [16][XS] line values, [2][XS] alpha_y
+        double* sa = smem + 16 * XS;
         // every load of a thread is issued before the first LDS store, so
         // that their latencies overlap (indices past the planes are
         // clamped: no valid face's stencil reaches them)
@@ -1405,7 +1424,36 @@ __device__ __forceinline__ void tanflux_cheap_body(const Dev& d, unsigned blk, u
             vl[u] = d.lin[0][g];
           }
         }
-        if (d.smooth_flux) {
+        if (fused) {
+          // thread p forms column p: side 0, the right face of the cell at
+          // index offset -ny (rows -3 .. 1 of its stencil), and side 1, the
+          // left face of the cell at offset 0 (rows -2 .. 2), from the six
+          // cell rows' averages (loaded directly: L2 holds the rows the
+          // neighbouring tiles read)
+          static_assert(XS <= 2 * TF, "a column per thread");
+          const int p = threadIdx.x;
+          if (p < XS && p < fl - f0 + 5) {
+            double w6[6][4];
+#pragma unroll
+            for (int row = 0; row < 6; row++) {
+              long long g = base + p + (long long)(row - 3) * d.ny;
+              g = g < 0 ? 0 : (g < d.nc ? g : d.nc - 1);
+#pragma unroll
+              for (int v = 0; v < 4; v++) w6[row][v] = d.U[v * d.nc + g];
+            }
+#pragma unroll
+            for (int sd = 0; sd < 2; sd++) {
+              double w[5][4], lo[8], ro[8];
+#pragma unroll
+              for (int s = 0; s < 5; s++)
+#pragma unroll
+                for (int v = 0; v < 4; v++) w[s][v] = w6[s + sd][v];
+              linear_normal(lo, ro, w, d.dx, d.idx, d.gas);
+#pragma unroll
+              for (int k = 0; k < 8; k++) st[(sd * 8 + k) * XS + p] = sd == 0 ? ro[k] : lo[k];
+            }
+          }
+        } else if (d.smooth_flux) {
           const double* line = d.line_x;
           double vv[NL];
 #pragma unroll
@@ -1422,14 +1470,14 @@ __device__ __forceinline__ void tanflux_cheap_body(const Dev& d, unsigned blk, u
         for (int u = 0; u < NA; u++) {
           const int e = threadIdx.x + u * 2 * TF;
           if (e < 2 * XS) {
-            st[16 * XS + e] = va[u];
+            sa[e] = va[u];
             slin0[e] = vl[u];
           }
         }
         const long long c0 = (long long)(side == 0 ? i - 1 : i) * d.ny + j;
         const unsigned char across = valid ? d.lin[1][c0] : 0;
         __syncthreads();
-        if (want) {
+        if (valid && (want || fused)) {
 #pragma unroll
           for (int s = 0; s < 5; s++)
 #pragma unroll
@@ -1440,15 +1488,42 @@ __device__ __forceinline__ void tanflux_cheap_body(const Dev& d, unsigned blk, u
           double amax = 0.0;
           unsigned smooth = across;
 #pragma unroll
-          for (int s = 0; s < 5; s++) amax = fmax(amax, st[(16 + side) * XS + pos + s]);
+          for (int s = 0; s < 5; s++) amax = fmax(amax, sa[side * XS + pos + s]);
 #pragma unroll
           for (int s = 0; s < 5; s++) smooth &= slin0[side * XS + pos + s];
           lin = smooth != 0 && amax == 0.0;
         }
       }
+      slin[side][tl] = lin ? 1 : 0;
       // the Gauss-point tile below overwrites the staged values
       __syncthreads();
       if (valid) count_linear(d.lstat + 2 * kStatSlots, lin);
+      if (fused && valid && !(d.smooth_flux && slin[0][tl] && slin[1][tl])) {
+        // a listed face: the list launch reads the line values of its
+        // stencil, so those of linear source cells are stored (entry s
+        // by the first listed face of the row that reads it: faces
+        // tl - 1 .. tl - (4 - s) are not listed; the previous tile's
+        // faces may store it too, with the same values)
+        const int jr = j - d.G;  // faces of this row before this one
+        bool seen = false;       // a listed face among tl - 1 .. tl - k
+        bool store[5];
+#pragma unroll
+        for (int k = 0; k <= 4; k++) {
+          if (k > 0 && tl - k >= 0 && jr - k >= 0 && !(d.smooth_flux && slin[0][tl - k] && slin[1][tl - k]))
+            seen = true;
+          store[4 - k] = !seen;
+        }
+        const long long f = (long long)i * d.ny + j;
+#pragma unroll
+        for (int s = 0; s < 5; s++) {
+          const bool src_lin = staged ? slin0[side * XS + pos + s] != 0
+                                      : d.lin[0][f - 2 + s - (side == 0 ? d.ny : 0)] != 0;
+          if (store[s] && src_lin) {
+#pragma unroll
+            for (int k = 0; k < 8; k++) d.line_x[(side * 8 + k) * nf + f - 2 + s] = L[s][k];
+          }
+        }
+      }
     } else {
       // the line values load before the rule is known, so that their
       // latency overlaps the rule's loads (most faces pass it)

@@ -41,6 +41,9 @@ struct Dev {
                                 // faces the flux lists held (x, y)
   unsigned char* lin[2];        // per cell: its x / y normal stencil took the linear weights
   int smooth_flux;              // learned path: faces with two linear sides take the smooth-flow flux
+  int fused_lines;              // learned path, bit DIR: the linear line values of that direction are
+                                // formed by its smooth-face flux launch from the cell averages (which also
+                                // stores the ones the listed faces read); the normal sweep stores none
   // learned path: the work lists, each in kListShards shards (a run of
   // tiles of the sweep that fills a list appends to one shard, so that the
   // appends of a sweep spread over kListShards counters)
@@ -68,6 +71,9 @@ constexpr int kSweepPartials = 1024;
 // queue behind each other)
 constexpr int kListShards = 16;
 constexpr int kCountStride = 32;
+// learned path: directions (bit 0 x, bit 1 y) whose smooth-face flux
+// launch forms the linear line values from the cell averages
+constexpr int kFusedLineDirs = 1;
 
 // ------------------------------------------------------------------ batches
 // Per-problem time-loop state of a batch, on the device: the driver loop of
