@@ -622,17 +622,24 @@ int gks2d_classifier_samples(gks2d_ctx* c, int32_t dir, int32_t stride, double l
 // faces x, y and the normal-sweep cells x, y.  A
 // shard takes the appends of a run of at most ceil(tiles / kListShards)
 // tiles of the sweep that fills the list (flux tiles: 64 or 128 faces).
-static unsigned shard_cap(long long items, int per_tile) {
-  const long long tiles = (items + per_tile - 1) / per_tile;
+// entries one shard can receive from a sweep of `tiles` tiles of per_tile
+// items (a shard takes a run of at most ceil(tiles / kListShards) tiles)
+static unsigned shard_cap_tiles(long long tiles, int per_tile) {
   return (unsigned)((tiles + kListShards - 1) / kListShards * per_tile);
+}
+static unsigned shard_cap(long long items, int per_tile) {
+  return shard_cap_tiles((items + per_tile - 1) / per_tile, per_tile);
 }
 static void list_caps(Dev& d) {
   const long long nf[2] = {(long long)(d.NX + 1) * d.NY, (long long)d.NX * (d.NY + 1)};
   const long long ncl[2] = {(long long)(d.NX + 2) * d.ny, (long long)d.nx * (d.NY + 2)};
-  for (int dir = 0; dir < 2; dir++) {
-    d.tcap[dir] = std::max(shard_cap(nf[dir], 64), shard_cap(nf[dir], 128));
-    d.ncap[dir] = shard_cap(ncl[dir], kSweepBlock);
-  }
+  for (int dir = 0; dir < 2; dir++) d.ncap[dir] = shard_cap(ncl[dir], kSweepBlock);
+  // the flux lists are filled by sweeps of 64 or 128 faces per tile: x
+  // faces in runs along the rows, y faces in 2D tiles (cheap_y_tiles)
+  d.tcap[0] = std::max(shard_cap(nf[0], 64), shard_cap(nf[0], 128));
+  d.tcap[1] = 0;
+  for (int tf : {64, 128})
+    d.tcap[1] = std::max({d.tcap[1], shard_cap(nf[1], tf), shard_cap_tiles(cheap_y_tiles(d, tf), tf)});
 }
 // the directions whose linear line values the smooth-face flux launch forms
 // itself: it runs when the smooth-flow flux is on (and the tangential
@@ -770,7 +777,7 @@ void drop_graph(gks2d_batch* b) {
 }
 
 // tile prefix tables of every work class, in launch order
-int batch_tables(gks2d_batch* b) {
+int batch_tables(gks2d_batch* b, int learned) {
   std::vector<int> pid;
   for (int p = 0; p < b->np; p++)
     if (b->in_tables[p]) pid.push_back(p);
@@ -795,7 +802,9 @@ int batch_tables(gks2d_batch* b) {
         case KB_NORMAL_X: n = ceil_div((long long)(d.NX + 2) * d.ny, kSweepBlock); break;
         case KB_NORMAL_Y: n = ceil_div((long long)d.nx * (d.NY + 2), kSweepBlock); break;
         case KB_FLUX_X: n = ceil_div((long long)(d.NX + 1) * d.NY, tf); break;
-        case KB_FLUX_Y: n = ceil_div((long long)d.NX * (d.NY + 1), tf); break;
+        case KB_FLUX_Y:  // the learned path's smooth-face launch takes 2D tiles
+          n = learned ? cheap_y_tiles(d, tf) : ceil_div((long long)d.NX * (d.NY + 1), tf);
+          break;
         case KB_ALPHA: n = ceil_div((long long)d.NX * std::min(d.NX, d.NY), kSweepBlock); break;
       }
       if (acc + (unsigned long long)n > 0x7fffffffULL) return fail(GKS_ERR_ARG, "batch too large for one launch");
@@ -854,7 +863,7 @@ int batch_graph(gks2d_batch* b, int learned) {
   const int key = b->shape.tanflux_faces * 4 + b->largest_first * 2 + learned;
   if (b->exec && b->graph_key == key && !b->tables_dirty) return GKS_OK;
   drop_graph(b);
-  RC(batch_tables(b));
+  RC(batch_tables(b, learned));
   b->tables_dirty = false;
   b->shape.learned = learned;
   const BatchArgs a{b->d_dev, b->d_st, b->d_start, b->d_pid, b->np_tab, b->d_lstart};

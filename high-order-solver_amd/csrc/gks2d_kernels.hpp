@@ -1334,22 +1334,48 @@ __device__ __forceinline__ void tanflux_cheap_body(const Dev& d, unsigned blk, u
   // multiple of the workgroup size (every thread issues the same number of
   // loads when the line states are staged)
   constexpr int XS = (TF + 4 + 2 * kGhost2 + 7) / 8 * 8;
-  constexpr int kSg = 2 * 12 * TF * 2, kStage = DIR == 0 ? 18 * XS : 0;
+  // y faces: a tile is YR rows of YC faces; its stencils cover YR + 4 rows
+  // of line records and their source cells YC + 1 columns (alpha, lin[1])
+  // whose stencils cover YC + 5 columns of cells
+  constexpr int YC = kYTileCols, YR = TF / YC, YRR = YR + 4, YAC = YC + 1, YUC = YC + 5;
+  static_assert(YR * YC == TF, "whole rows per y tile");
+  constexpr int kSg = 2 * 12 * TF * 2;
+  constexpr int kStage = DIR == 0 ? 18 * XS : YRR * 16 * YC + YRR * YAC + YRR * 4 * YUC;
   __shared__ double smem[kSg > kStage ? kSg : kStage];
   double(&sg)[2][12][TF][2] = *reinterpret_cast<double(*)[2][12][TF][2]>(smem);  // [side][component][face][point]
   __shared__ unsigned char slin[2][TF];
-  __shared__ unsigned char sflag[DIR == 0 ? 2 * XS : 1];
+  __shared__ unsigned char sflag[DIR == 0 ? 2 * XS : YRR * YAC];
   const int ni = DIR == 0 ? d.NX + 1 : d.NX, nj = DIR == 0 ? d.NY : d.NY + 1;
   const long long nfaces = (long long)ni * nj;
+  // tile-local face t: its ordinal (row-major over the interior faces),
+  // whether it exists, and its indices
+  const long long ytc = DIR == 1 ? cheap_y_tile_cols(d) : 1;
+  const int ybi = DIR == 1 ? (int)(blk / ytc) : 0, ybj = DIR == 1 ? (int)(blk % ytc) : 0;
+  auto face_of = [&](int t, bool& ok, int& fi, int& fj) -> long long {
+    long long fr;
+    if constexpr (DIR == 0) {
+      fr = (long long)blk * TF + t;
+      ok = fr < nfaces;
+      fi = ok ? (int)(fr / nj) + d.G : d.G;
+      fj = ok ? (int)(fr % nj) + d.G : d.G;
+    } else {
+      const int ri = ybi * YR + t / YC, cj = ybj * YC + t % YC;
+      ok = ri < ni && cj < nj;
+      fr = (long long)ri * nj + cj;
+      fi = (ok ? ri : 0) + d.G;
+      fj = (ok ? cj : 0) + d.G;
+    }
+    return fr;
+  };
   const int tl = threadIdx.x % TF, hi = threadIdx.x / TF;
-  const long long r = (long long)blk * TF + tl;
-  const bool valid = r < nfaces;
-  const int i = valid ? (int)(r / nj) + d.G : d.G, j = valid ? (int)(r % nj) + d.G : d.G;
+  bool valid;
+  int i, j;
+  const long long r = face_of(tl, valid, i, j);
   const long long nf = DIR == 0 ? d.nfx : d.nfy;
-  // fused_lines (x faces): the line values are the linear reconstructions
-  // of the cell averages, formed here; the ones the listed faces' stencils
-  // read are stored for the list launch
-  const bool fused = DIR == 0 && (d.fused_lines & 1);
+  // fused_lines: the line values are the linear reconstructions of the
+  // cell averages, formed here; the ones the listed faces' stencils read
+  // are stored for the list launch
+  const bool fused = (d.fused_lines >> DIR) & 1;
   {
     const int side = hi;
     // without the smooth-flow flux every face goes to the list
@@ -1525,23 +1551,154 @@ __device__ __forceinline__ void tanflux_cheap_body(const Dev& d, unsigned blk, u
         }
       }
     } else {
-      // the line values load before the rule is known, so that their
-      // latency overlaps the rule's loads (most faces pass it)
-      if (want) {
-        const double* line = d.line_y;
+      // y faces: the tile's faces are rows i0 .. i0 + YR - 1 by columns
+      // j0 .. j0 + YC - 1; their stencils read the line records of rows
+      // i0 - 2 .. i0 + YR + 1 at the same columns, staged in LDS (without
+      // fused_lines loaded, with it formed from the cell averages of those
+      // rows, columns j0 - 3 .. j0 + YC + 1: a thread per (row, column)
+      // forms both sides), and the tangential rule's alpha_x and lin[1]
+      // of the source cells, columns j0 - 1 .. j0 + YC - 1
+      const int i0 = d.G + ybi * YR, j0 = d.G + ybj * YC;
+      const int rl = tl / YC, cl = tl % YC;
+      double* st = smem;                  // [YRR][16][YC] line values
+      double* sa = smem + YRR * 16 * YC;  // [YRR][YAC] alpha_x
+      double* su = sa + YRR * YAC;        // [YRR][4][YUC] cell averages (fused)
+      unsigned char* sl1 = sflag;         // [YRR][YAC] lin[1]
+      auto clamp_row = [&](int row) { return row < 0 ? 0 : (row < d.nx ? row : d.nx - 1); };
+      // every load of a thread is issued before the first LDS store
+      constexpr int NA = (YRR * YAC + 2 * TF - 1) / (2 * TF);
+      constexpr int NU = (YRR * 4 * YUC + 2 * TF - 1) / (2 * TF), NL = (YRR * 16 * YC + 2 * TF - 1) / (2 * TF);
+      double va[NA];
+      unsigned char vl[NA];
 #pragma unroll
-        for (int s = 0; s < 5; s++) {
-          const long long f = (long long)(i + 2 - s) * (d.ny + 1) + j;
-#pragma unroll
-          for (int k = 0; k < 8; k++) L[s][k] = line[(side * 8 + k) * nf + f];
+      for (int u = 0; u < NA; u++) {
+        const int e = threadIdx.x + u * 2 * TF;
+        if (e < YRR * YAC) {
+          const int rr = e / YAC, cc = e - rr * YAC;
+          int col = j0 - 1 + cc;
+          col = col < d.ny ? col : d.ny - 1;
+          const long long g = (long long)clamp_row(i0 - 2 + rr) * d.ny + col;
+          va[u] = d.ax[g];
+          vl[u] = d.lin[1][g];
         }
       }
+      const unsigned char across = valid ? d.lin[0][(long long)i * d.ny + j - 1 + side] : 0;
+      if (fused) {
+        double vu[NU];
+#pragma unroll
+        for (int u = 0; u < NU; u++) {
+          const int e = threadIdx.x + u * 2 * TF;
+          if (e < YRR * 4 * YUC) {
+            const int rr = e / (4 * YUC), v = (e / YUC) & 3, uc = e % YUC;
+            int col = j0 - 3 + uc;
+            col = col < d.ny ? col : d.ny - 1;
+            vu[u] = d.U[v * d.nc + (long long)clamp_row(i0 - 2 + rr) * d.ny + col];
+          }
+        }
+#pragma unroll
+        for (int u = 0; u < NU; u++) {
+          const int e = threadIdx.x + u * 2 * TF;
+          if (e < YRR * 4 * YUC) su[e] = vu[u];
+        }
+      } else if (d.smooth_flux) {
+        double vv[NL];
+#pragma unroll
+        for (int u = 0; u < NL; u++) {
+          const int e = threadIdx.x + u * 2 * TF;
+          if (e < YRR * 16 * YC) {
+            const int rr = e / (16 * YC), k = (e / YC) & 15, c = e % YC;
+            int col = j0 + c;
+            col = col <= d.ny ? col : d.ny;
+            vv[u] = d.line_y[k * nf + (long long)clamp_row(i0 - 2 + rr) * (d.ny + 1) + col];
+          }
+        }
+#pragma unroll
+        for (int u = 0; u < NL; u++) {
+          const int e = threadIdx.x + u * 2 * TF;
+          if (e < YRR * 16 * YC) st[e] = vv[u];
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < NA; u++) {
+        const int e = threadIdx.x + u * 2 * TF;
+        if (e < YRR * YAC) {
+          sa[e] = va[u];
+          sl1[e] = vl[u];
+        }
+      }
+      __syncthreads();
+      if (fused) {
+        // (row rr, column c): side 0 of face column c is the upper face of
+        // cell j0 + c - 1 (averages c .. c + 4), side 1 the lower face of
+        // cell j0 + c (c + 1 .. c + 5), in the x-like frame of the y sweep
+#pragma unroll
+        for (int u = 0; u < (YRR * YC + 2 * TF - 1) / (2 * TF); u++) {
+          const int e = threadIdx.x + u * 2 * TF;
+          if (e < YRR * YC) {
+            const int rr = e / YC, c = e - rr * YC;
+#pragma unroll
+            for (int sd = 0; sd < 2; sd++) {
+              double w[5][4], lo[8], ro[8];
+#pragma unroll
+              for (int s = 0; s < 5; s++) {
+                const double* a = su + rr * 4 * YUC + c + sd + s;
+                w[s][0] = a[0];  // YchangetoX
+                w[s][1] = a[2 * YUC];
+                w[s][2] = -a[YUC];
+                w[s][3] = a[3 * YUC];
+              }
+              linear_normal(lo, ro, w, d.dy, d.idy, d.gas);
+#pragma unroll
+              for (int k = 0; k < 8; k++) st[(rr * 16 + sd * 8 + k) * YC + c] = sd == 0 ? ro[k] : lo[k];
+            }
+          }
+        }
+        __syncthreads();
+      }
+      // stencil s of this face: row i + 2 - s, staged row rl + 4 - s
+      if (valid && (want || fused)) {
+#pragma unroll
+        for (int s = 0; s < 5; s++)
+#pragma unroll
+          for (int k = 0; k < 8; k++) L[s][k] = st[((rl + 4 - s) * 16 + side * 8 + k) * YC + cl];
+      }
       if (valid) {
-        lin = tangential_linear<DIR>(d, side, i, j);
-        count_linear(d.lstat + 2 * kStatSlots, lin);
+        // tangential_linear<1> on the staged values
+        double amax = 0.0;
+        unsigned smooth = across;
+#pragma unroll
+        for (int s = 0; s < 5; s++) {
+          amax = fmax(amax, sa[(rl + 4 - s) * YAC + cl + side]);
+          smooth &= sl1[(rl + 4 - s) * YAC + cl + side];
+        }
+        lin = smooth != 0 && amax == 0.0;
+      }
+      slin[side][tl] = lin ? 1 : 0;
+      // the Gauss-point tile below overwrites the staged values
+      __syncthreads();
+      if (valid) count_linear(d.lstat + 2 * kStatSlots, lin);
+      if (fused && valid && !(d.smooth_flux && slin[0][tl] && slin[1][tl])) {
+        // a listed face: the line values of its stencil's linear source
+        // cells are stored for the list launch, entry s (row i + 2 - s) by
+        // the first listed face of the column that reads it (faces of rows
+        // i - 1 .. i - s of the tile are not listed)
+        bool seen = false, store[5];
+#pragma unroll
+        for (int s = 0; s <= 4; s++) {
+          if (s > 0 && rl - s >= 0 && !(d.smooth_flux && slin[0][tl - s * YC] && slin[1][tl - s * YC]))
+            seen = true;
+          store[s] = !seen;
+        }
+#pragma unroll
+        for (int s = 0; s < 5; s++) {
+          if (store[s] && sl1[(rl + 4 - s) * YAC + cl + side]) {
+            const long long f = (long long)(i + 2 - s) * (d.ny + 1) + j;
+#pragma unroll
+            for (int k = 0; k < 8; k++) d.line_y[(side * 8 + k) * nf + f] = L[s][k];
+          }
+        }
       }
     }
-    slin[side][tl] = lin ? 1 : 0;
     if (lin && want) {
       const double h = DIR == 0 ? d.dy : d.dx, ih = DIR == 0 ? d.idy : d.idx;
       double gp[2][12];
@@ -1582,9 +1739,10 @@ __device__ __forceinline__ void tanflux_cheap_body(const Dev& d, unsigned blk, u
   }
   // phase 2: lane pair (2 fp, 2 fp + 1) takes the two Gauss points of face fp
   const int fp = threadIdx.x >> 1, q = threadIdx.x & 1;
-  const long long r2 = (long long)blk * TF + fp;
-  if (!(r2 < nfaces && d.smooth_flux && (slin[0][fp] & slin[1][fp]))) return;
-  const int i2 = (int)(r2 / nj) + d.G, j2 = (int)(r2 % nj) + d.G;
+  bool ok2;
+  int i2, j2;
+  (void)face_of(fp, ok2, i2, j2);
+  if (!(ok2 && d.smooth_flux && (slin[0][fp] & slin[1][fp]))) return;
   const long long f2 = DIR == 0 ? (long long)i2 * d.ny + j2 : (long long)i2 * (d.ny + 1) + j2;
   const Strided L{&sg[0][0][fp][q], 2 * TF}, R{&sg[1][0][fp][q], 2 * TF};
   flux_pair_tail<DIR>(d, q, f2, nf, L, R, stage, dt, true);
@@ -2052,7 +2210,9 @@ static hipError_t flux(const Dev& d, int dir, int stage, double dt, hipStream_t 
       // (the listed faces follow in flux_list)
       hipLaunchKernelGGL(k_clear_count, dim3(1), dim3(64), 0, s, d.tcount + dir * kListShards * kCountStride);
       if (dir == 0) hipLaunchKernelGGL(k_tanflux_cheap<0>, dim3(nb), dim3(2 * kTanFlux), 0, s, d, stage, dt);
-      else hipLaunchKernelGGL(k_tanflux_cheap<1>, dim3(nb), dim3(2 * kTanFlux), 0, s, d, stage, dt);
+      else
+        hipLaunchKernelGGL(k_tanflux_cheap<1>, dim3((unsigned)cheap_y_tiles(d, kTanFlux)), dim3(2 * kTanFlux), 0, s,
+                           d, stage, dt);
     } else if (d.mlp) {
       if (dir == 0) hipLaunchKernelGGL((k_tanflux<0, true>), dim3(nb), dim3(2 * kTanFlux), 0, s, d, stage, dt);
       else hipLaunchKernelGGL((k_tanflux<1, true>), dim3(nb), dim3(2 * kTanFlux), 0, s, d, stage, dt);

@@ -73,7 +73,17 @@ constexpr int kListShards = 16;
 constexpr int kCountStride = 32;
 // learned path: directions (bit 0 x, bit 1 y) whose smooth-face flux
 // launch forms the linear line values from the cell averages
-constexpr int kFusedLineDirs = 1;
+constexpr int kFusedLineDirs = 3;
+// learned path, y faces: the smooth-face flux launch takes tiles of
+// kYTileCols faces along y by TF / kYTileCols rows (TF faces per tile)
+constexpr int kYTileCols = 16;
+__host__ __device__ inline long long cheap_y_tile_cols(const Dev& d) {
+  return (d.NY + 1 + kYTileCols - 1) / kYTileCols;
+}
+__host__ __device__ inline long long cheap_y_tiles(const Dev& d, int tf) {
+  const int rows = tf / kYTileCols;
+  return (long long)((d.NX + rows - 1) / rows) * cheap_y_tile_cols(d);
+}
 
 // ------------------------------------------------------------------ batches
 // Per-problem time-loop state of a batch, on the device: the driver loop of
