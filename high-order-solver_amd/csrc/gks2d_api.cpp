@@ -634,12 +634,14 @@ static void list_caps(Dev& d) {
   const long long nf[2] = {(long long)(d.NX + 1) * d.NY, (long long)d.NX * (d.NY + 1)};
   const long long ncl[2] = {(long long)(d.NX + 2) * d.ny, (long long)d.nx * (d.NY + 2)};
   for (int dir = 0; dir < 2; dir++) d.ncap[dir] = shard_cap(ncl[dir], kSweepBlock);
-  // the flux lists are filled by sweeps of 64 or 128 faces per tile: x
-  // faces in runs along the rows, y faces in 2D tiles (cheap_y_tiles)
-  d.tcap[0] = std::max(shard_cap(nf[0], 64), shard_cap(nf[0], 128));
-  d.tcap[1] = 0;
-  for (int tf : {64, 128})
-    d.tcap[1] = std::max({d.tcap[1], shard_cap(nf[1], tf), shard_cap_tiles(cheap_y_tiles(d, tf), tf)});
+  // the flux lists are filled by sweeps of 64 or 128 faces per tile, in
+  // 2D tiles (cheap_tiles); runs along the rows for the single learned
+  // launch
+  for (int dir = 0; dir < 2; dir++) {
+    d.tcap[dir] = 0;
+    for (int tf : {64, 128})
+      d.tcap[dir] = std::max({d.tcap[dir], shard_cap(nf[dir], tf), shard_cap_tiles(cheap_tiles(d, dir, tf), tf)});
+  }
 }
 // the directions whose linear line values the smooth-face flux launch forms
 // itself: it runs when the smooth-flow flux is on (and the tangential
@@ -801,10 +803,9 @@ int batch_tables(gks2d_batch* b, int learned) {
         case KB_UPDATE: n = ceil_div((long long)d.NX * d.NY, kSweepBlock); break;
         case KB_NORMAL_X: n = ceil_div((long long)(d.NX + 2) * d.ny, kSweepBlock); break;
         case KB_NORMAL_Y: n = ceil_div((long long)d.nx * (d.NY + 2), kSweepBlock); break;
-        case KB_FLUX_X: n = ceil_div((long long)(d.NX + 1) * d.NY, tf); break;
-        case KB_FLUX_Y:  // the learned path's smooth-face launch takes 2D tiles
-          n = learned ? cheap_y_tiles(d, tf) : ceil_div((long long)d.NX * (d.NY + 1), tf);
-          break;
+        // the learned path's smooth-face launch takes 2D tiles
+        case KB_FLUX_X: n = learned ? cheap_tiles(d, 0, tf) : ceil_div((long long)(d.NX + 1) * d.NY, tf); break;
+        case KB_FLUX_Y: n = learned ? cheap_tiles(d, 1, tf) : ceil_div((long long)d.NX * (d.NY + 1), tf); break;
         case KB_ALPHA: n = ceil_div((long long)d.NX * std::min(d.NX, d.NY), kSweepBlock); break;
       }
       if (acc + (unsigned long long)n > 0x7fffffffULL) return fail(GKS_ERR_ARG, "batch too large for one launch");

@@ -36,9 +36,6 @@ using namespace gksd;
 
 constexpr int kBlock = kSweepBlock;
 constexpr int kPartialBlocks = kSweepPartials;
-// ghost layers of every 2D context (gks2d_create accepts 5 only); sizes the
-// kernels' LDS staging of stencil halos
-constexpr int kGhost2 = 5;
 
 __device__ __forceinline__ long long gtid() { return (long long)blockIdx.x * blockDim.x + threadIdx.x; }
 
@@ -1320,382 +1317,251 @@ template <int DIR> __device__ __forceinline__ bool tangential_linear(const Dev& 
   }
   return smooth != 0 && amax == 0.0;
 }
-// Two phases over a tile of TF faces.  Phase 1, thread (side, face): the
-// tangential rule for its side and, when linear, that side's two
-// Gauss-point states from the linear weights into LDS (each stencil loaded
-// and reconstructed once).  Phase 2, thread (Gauss point, face): a face
+// Two phases over a tile of TF faces, R = TF / kCheapCols rows (i) by
+// kCheapCols faces (j).  Phase 1, thread (side, face): the tangential rule
+// for its side and, when linear, that side's two Gauss-point states from
+// the linear weights into LDS; the stencils' line values, and the DF
+// sensor and linear flags of their source cells, are staged in LDS first
+// (each value loaded once per tile).  Phase 2, lane pair per face: a face
 // with both sides linear gets the smooth-flow flux from the two states;
-// the first wave lists every other face.
+// the first TF threads list every other face.
+//
+// With fused_lines the line values are not loaded: the tile forms them
+// from the cell averages (the linear reconstruction of the normal sweep,
+// a thread per source cell, from averages staged in LDS), and a listed
+// face stores the ones of its stencil's linear source cells for the list
+// launch (the WENO cells' values come from k_normal_list).
 template <int DIR, int TF>
 __device__ __forceinline__ void tanflux_cheap_body(const Dev& d, unsigned blk, unsigned ntiles, int stage,
                                                    double dt) {
-  // x faces: the stencils of a tile's faces cover one index range of XS
-  // line records (below); rounded up to a multiple of 8, so that 16 XS is a
-  // multiple of the workgroup size (every thread issues the same number of
-  // loads when the line states are staged)
-  constexpr int XS = (TF + 4 + 2 * kGhost2 + 7) / 8 * 8;
-  // y faces: a tile is YR rows of YC faces; its stencils cover YR + 4 rows
-  // of line records and their source cells YC + 1 columns (alpha, lin[1])
-  // whose stencils cover YC + 5 columns of cells
-  constexpr int YC = kYTileCols, YR = TF / YC, YRR = YR + 4, YAC = YC + 1, YUC = YC + 5;
-  static_assert(YR * YC == TF, "whole rows per y tile");
-  constexpr int kSg = 2 * 12 * TF * 2;
-  constexpr int kStage = DIR == 0 ? 18 * XS : YRR * 16 * YC + YRR * YAC + YRR * 4 * YUC;
+  constexpr int C = kCheapCols, R = TF / C;
+  static_assert(R * C == TF, "whole rows per tile");
+  // staged extents.  x faces (stencils along j): line values of the R face
+  // rows at columns j0 - 2 .. j0 + C + 1 (XW), their source cells' rows
+  // i0 - 1 .. i0 + R - 1, and those cells' normal stencils, rows i0 - 3 ..
+  // i0 + R + 1.  y faces (stencils across rows): line values of rows i0 - 2
+  // .. i0 + R + 1 at columns j0 .. j0 + C - 1, their source cells' columns
+  // j0 - 1 .. j0 + C - 1, and those cells' stencils, columns j0 - 3 .. j0 +
+  // C + 1.
+  constexpr int XW = C + 4;
+  constexpr int LR = DIR == 0 ? R : R + 4, LW = DIR == 0 ? XW : C;      // line values [LR][16][LW]
+  constexpr int AR = DIR == 0 ? R + 1 : R + 4, AW = DIR == 0 ? XW : C + 1;  // source flags [AR][AW]
+  constexpr int UR = DIR == 0 ? R + 5 : R + 4, UW = DIR == 0 ? XW : C + 5;  // cell averages [UR][4][UW]
+  constexpr int kSg = 2 * 12 * TF * 2, kStage = LR * 16 * LW + AR * AW + UR * 4 * UW;
   __shared__ double smem[kSg > kStage ? kSg : kStage];
   double(&sg)[2][12][TF][2] = *reinterpret_cast<double(*)[2][12][TF][2]>(smem);  // [side][component][face][point]
+  double* st = smem;                // line values
+  double* sa = st + LR * 16 * LW;   // DF sensor of the source cells (alpha_y for x faces, alpha_x for y)
+  double* su = sa + AR * AW;        // cell averages (fused_lines)
   __shared__ unsigned char slin[2][TF];
-  __shared__ unsigned char sflag[DIR == 0 ? 2 * XS : YRR * YAC];
+  __shared__ unsigned char sfl[AR * AW];  // linear flag of the source cells' normal stencils
   const int ni = DIR == 0 ? d.NX + 1 : d.NX, nj = DIR == 0 ? d.NY : d.NY + 1;
-  const long long nfaces = (long long)ni * nj;
-  // tile-local face t: its ordinal (row-major over the interior faces),
-  // whether it exists, and its indices
-  const long long ytc = DIR == 1 ? cheap_y_tile_cols(d) : 1;
-  const int ybi = DIR == 1 ? (int)(blk / ytc) : 0, ybj = DIR == 1 ? (int)(blk % ytc) : 0;
+  const long long tcols = cheap_tile_cols(d, DIR);
+  const int bi = (int)(blk / tcols), bj = (int)(blk % tcols);
+  const int i0 = d.G + bi * R, j0 = d.G + bj * C;
+  // tile-local face t: whether it exists, its indices and its ordinal
+  // (row-major over the interior faces)
   auto face_of = [&](int t, bool& ok, int& fi, int& fj) -> long long {
-    long long fr;
-    if constexpr (DIR == 0) {
-      fr = (long long)blk * TF + t;
-      ok = fr < nfaces;
-      fi = ok ? (int)(fr / nj) + d.G : d.G;
-      fj = ok ? (int)(fr % nj) + d.G : d.G;
-    } else {
-      const int ri = ybi * YR + t / YC, cj = ybj * YC + t % YC;
-      ok = ri < ni && cj < nj;
-      fr = (long long)ri * nj + cj;
-      fi = (ok ? ri : 0) + d.G;
-      fj = (ok ? cj : 0) + d.G;
-    }
-    return fr;
+    const int ri = bi * R + t / C, cj = bj * C + t % C;
+    ok = ri < ni && cj < nj;
+    fi = (ok ? ri : 0) + d.G;
+    fj = (ok ? cj : 0) + d.G;
+    return (long long)ri * nj + cj;
   };
   const int tl = threadIdx.x % TF, hi = threadIdx.x / TF;
+  const int rl = tl / C, cl = tl % C;
   bool valid;
   int i, j;
   const long long r = face_of(tl, valid, i, j);
   const long long nf = DIR == 0 ? d.nfx : d.nfy;
-  // fused_lines: the line values are the linear reconstructions of the
-  // cell averages, formed here; the ones the listed faces' stencils read
-  // are stored for the list launch
   const bool fused = (d.fused_lines >> DIR) & 1;
+  auto clamp = [](int v, int hi_) { return v < 0 ? 0 : (v < hi_ ? v : hi_ - 1); };
   {
     const int side = hi;
     // without the smooth-flow flux every face goes to the list
     const bool want = valid && d.smooth_flux;
-    double L[5][8];
-    bool lin = false;
-    if constexpr (DIR == 0) {
-      // x faces: the tile's faces r0 .. r0 + TF - 1 have consecutive line
-      // records i * ny + j, apart from a jump of 2G where the tile wraps to
-      // the next row, so their 5-point stencils cover the one index range
-      // [f0 - 2, f0 - 2 + XS) (f0: the tile's first face).  The workgroup
-      // stages that range of the 16 line planes, and of the alpha_y and
-      // lin[0] planes of the two cell rows the sides take them from (side
-      // 0: row i - 1, index - ny; side 1: row i), in LDS (over the
-      // Gauss-point tile, which is written only after everyone has read the
-      // stencils), instead of every thread loading its own stencils: each
-      // value once instead of five times.  With fused_lines the workgroup
-      // forms that range of line values from the cell averages instead
-      // (a thread per column: its two source cells).
-      static_assert(18 * XS <= (int)(sizeof(smem) / sizeof(double)), "staging fits");
-      static_assert(16 * XS % (2 * TF) == 0, "whole staging rounds");
-      const long long r0 = (long long)blk * TF;
-      const long long f0 = (long long)((int)(r0 / nj) + d.G) * d.ny + (int)(r0 % nj) + d.G;
-      const long long base = f0 - 2;
-      const int pos = (int)((long long)i * d.ny + j - f0);
-      unsigned char* slin0 = sflag;  // [2][XS]: lin[0] of side 0's and side 1's source cells
-      // the tile's index range fits XS when it wraps at most once (always
-      // for NY >= TF; a narrower grid's tile can cover three rows)
-      const long long rl = (r0 + TF < nfaces ? r0 + TF : nfaces) - 1;
-      const long long fl = (long long)((int)(rl / nj) + d.G) * d.ny + (int)(rl % nj) + d.G;
-      const bool staged = fl - f0 + 5 <= XS;
-      if (!staged) {
-        // each thread loads (or forms) its own stencils
-        if (valid && (want || fused)) {
+    // every load of a thread is issued before the first LDS store
+    constexpr int NA = (AR * AW + 2 * TF - 1) / (2 * TF);
+    constexpr int NU = (UR * 4 * UW + 2 * TF - 1) / (2 * TF), NL = (LR * 16 * LW + 2 * TF - 1) / (2 * TF);
+    double va[NA];
+    unsigned char vl[NA];
 #pragma unroll
-          for (int s = 0; s < 5; s++) {
-            if (fused) {
-              // side 0: the right face of cell (i - 1, j'), side 1: the
-              // left face of cell (i, j')
-              double w[5][4], a[5], lo[8], ro[8];
-              normal_stencil<0>(d, side == 0 ? i - 1 : i, j - 2 + s, w, a);
-              linear_normal(lo, ro, w, d.dx, d.idx, d.gas);
-#pragma unroll
-              for (int k = 0; k < 8; k++) L[s][k] = side == 0 ? ro[k] : lo[k];
-            } else {
-              const long long f = (long long)i * d.ny + (j - 2 + s);
-#pragma unroll
-              for (int k = 0; k < 8; k++) L[s][k] = d.line_x[(side * 8 + k) * nf + f];
-            }
-          }
-        }
-        if (valid) lin = tangential_linear<0>(d, side, i, j);
-      } else {
-        // the line states, and the alpha_y and lin[0] planes of the two
-        // source cell rows, staged from HBM
-        double* st = smem;  // [16][XS] line values, [2][XS] alpha_y
-        double* sa = smem + 16 * XS;
-        // every load of a thread is issued before the first LDS store, so
-        // that their latencies overlap (indices past the planes are
-        // clamped: no valid face's stencil reaches them)
-        constexpr int NL = (16 * XS + 2 * TF - 1) / (2 * TF), NA = (2 * XS + 2 * TF - 1) / (2 * TF);
-        double va[NA];
-        unsigned char vl[NA];
-#pragma unroll
-        for (int u = 0; u < NA; u++) {
-          const int e = threadIdx.x + u * 2 * TF;
-          const int sd = e / XS, p = e - sd * XS;
-          long long g = base + p - (sd == 0 ? d.ny : 0);
-          g = g < d.nc ? g : d.nc - 1;
-          if (e < 2 * XS) {
-            va[u] = d.ay[g];
-            vl[u] = d.lin[0][g];
-          }
-        }
-        if (fused) {
-          // thread p forms column p: side 0, the right face of the cell at
-          // index offset -ny (rows -3 .. 1 of its stencil), and side 1, the
-          // left face of the cell at offset 0 (rows -2 .. 2), from the six
-          // cell rows' averages (loaded directly: L2 holds the rows the
-          // neighbouring tiles read)
-          static_assert(XS <= 2 * TF, "a column per thread");
-          const int p = threadIdx.x;
-          if (p < XS && p < fl - f0 + 5) {
-            double w6[6][4];
-#pragma unroll
-            for (int row = 0; row < 6; row++) {
-              long long g = base + p + (long long)(row - 3) * d.ny;
-              g = g < 0 ? 0 : (g < d.nc ? g : d.nc - 1);
-#pragma unroll
-              for (int v = 0; v < 4; v++) w6[row][v] = d.U[v * d.nc + g];
-            }
-#pragma unroll
-            for (int sd = 0; sd < 2; sd++) {
-              double w[5][4], lo[8], ro[8];
-#pragma unroll
-              for (int s = 0; s < 5; s++)
-#pragma unroll
-                for (int v = 0; v < 4; v++) w[s][v] = w6[s + sd][v];
-              linear_normal(lo, ro, w, d.dx, d.idx, d.gas);
-#pragma unroll
-              for (int k = 0; k < 8; k++) st[(sd * 8 + k) * XS + p] = sd == 0 ? ro[k] : lo[k];
-            }
-          }
-        } else if (d.smooth_flux) {
-          const double* line = d.line_x;
-          double vv[NL];
-#pragma unroll
-          for (int u = 0; u < NL; u++) {
-            const int e = threadIdx.x + u * 2 * TF;
-            const int c = e / XS, p = e - c * XS;
-            const long long g = base + p < nf ? base + p : nf - 1;
-            vv[u] = line[c * nf + g];
-          }
-#pragma unroll
-          for (int u = 0; u < NL; u++) st[threadIdx.x + u * 2 * TF] = vv[u];
-        }
-#pragma unroll
-        for (int u = 0; u < NA; u++) {
-          const int e = threadIdx.x + u * 2 * TF;
-          if (e < 2 * XS) {
-            sa[e] = va[u];
-            slin0[e] = vl[u];
-          }
-        }
-        const long long c0 = (long long)(side == 0 ? i - 1 : i) * d.ny + j;
-        const unsigned char across = valid ? d.lin[1][c0] : 0;
-        __syncthreads();
-        if (valid && (want || fused)) {
-#pragma unroll
-          for (int s = 0; s < 5; s++)
-#pragma unroll
-            for (int k = 0; k < 8; k++) L[s][k] = st[(side * 8 + k) * XS + pos + s];
-        }
-        if (valid) {
-          // tangential_linear<0> on the staged values
-          double amax = 0.0;
-          unsigned smooth = across;
-#pragma unroll
-          for (int s = 0; s < 5; s++) amax = fmax(amax, sa[side * XS + pos + s]);
-#pragma unroll
-          for (int s = 0; s < 5; s++) smooth &= slin0[side * XS + pos + s];
-          lin = smooth != 0 && amax == 0.0;
-        }
+    for (int u = 0; u < NA; u++) {
+      const int e = threadIdx.x + u * 2 * TF;
+      if (e < AR * AW) {
+        const int ar = e / AW, ac = e - ar * AW;
+        const long long g = DIR == 0 ? (long long)clamp(i0 - 1 + ar, d.nx) * d.ny + clamp(j0 - 2 + ac, d.ny)
+                                     : (long long)clamp(i0 - 2 + ar, d.nx) * d.ny + clamp(j0 - 1 + ac, d.ny);
+        va[u] = DIR == 0 ? d.ay[g] : d.ax[g];
+        vl[u] = d.lin[DIR][g];
       }
-      slin[side][tl] = lin ? 1 : 0;
-      // the Gauss-point tile below overwrites the staged values
-      __syncthreads();
-      if (valid) count_linear(d.lstat + 2 * kStatSlots, lin);
-      if (fused && valid && !(d.smooth_flux && slin[0][tl] && slin[1][tl])) {
-        // a listed face: the list launch reads the line values of its
-        // stencil, so those of linear source cells are stored (entry s
-        // by the first listed face of the row that reads it: faces
-        // tl - 1 .. tl - (4 - s) are not listed; the previous tile's
-        // faces may store it too, with the same values)
-        const int jr = j - d.G;  // faces of this row before this one
-        bool seen = false;       // a listed face among tl - 1 .. tl - k
-        bool store[5];
+    }
+    // the tangential rule's flag of the centre cell across (x faces: the
+    // source cell's y stencil; y faces: its x stencil)
+    const unsigned char across =
+        !valid ? 0
+               : (DIR == 0 ? d.lin[1][(long long)(i - 1 + side) * d.ny + j]
+                           : d.lin[0][(long long)i * d.ny + j - 1 + side]);
+    if (fused) {
+      double vu[NU];
 #pragma unroll
-        for (int k = 0; k <= 4; k++) {
-          if (k > 0 && tl - k >= 0 && jr - k >= 0 && !(d.smooth_flux && slin[0][tl - k] && slin[1][tl - k]))
-            seen = true;
-          store[4 - k] = !seen;
-        }
-        const long long f = (long long)i * d.ny + j;
-#pragma unroll
-        for (int s = 0; s < 5; s++) {
-          const bool src_lin = staged ? slin0[side * XS + pos + s] != 0
-                                      : d.lin[0][f - 2 + s - (side == 0 ? d.ny : 0)] != 0;
-          if (store[s] && src_lin) {
-#pragma unroll
-            for (int k = 0; k < 8; k++) d.line_x[(side * 8 + k) * nf + f - 2 + s] = L[s][k];
-          }
-        }
-      }
-    } else {
-      // y faces: the tile's faces are rows i0 .. i0 + YR - 1 by columns
-      // j0 .. j0 + YC - 1; their stencils read the line records of rows
-      // i0 - 2 .. i0 + YR + 1 at the same columns, staged in LDS (without
-      // fused_lines loaded, with it formed from the cell averages of those
-      // rows, columns j0 - 3 .. j0 + YC + 1: a thread per (row, column)
-      // forms both sides), and the tangential rule's alpha_x and lin[1]
-      // of the source cells, columns j0 - 1 .. j0 + YC - 1
-      const int i0 = d.G + ybi * YR, j0 = d.G + ybj * YC;
-      const int rl = tl / YC, cl = tl % YC;
-      double* st = smem;                  // [YRR][16][YC] line values
-      double* sa = smem + YRR * 16 * YC;  // [YRR][YAC] alpha_x
-      double* su = sa + YRR * YAC;        // [YRR][4][YUC] cell averages (fused)
-      unsigned char* sl1 = sflag;         // [YRR][YAC] lin[1]
-      auto clamp_row = [&](int row) { return row < 0 ? 0 : (row < d.nx ? row : d.nx - 1); };
-      // every load of a thread is issued before the first LDS store
-      constexpr int NA = (YRR * YAC + 2 * TF - 1) / (2 * TF);
-      constexpr int NU = (YRR * 4 * YUC + 2 * TF - 1) / (2 * TF), NL = (YRR * 16 * YC + 2 * TF - 1) / (2 * TF);
-      double va[NA];
-      unsigned char vl[NA];
-#pragma unroll
-      for (int u = 0; u < NA; u++) {
+      for (int u = 0; u < NU; u++) {
         const int e = threadIdx.x + u * 2 * TF;
-        if (e < YRR * YAC) {
-          const int rr = e / YAC, cc = e - rr * YAC;
-          int col = j0 - 1 + cc;
-          col = col < d.ny ? col : d.ny - 1;
-          const long long g = (long long)clamp_row(i0 - 2 + rr) * d.ny + col;
-          va[u] = d.ax[g];
-          vl[u] = d.lin[1][g];
-        }
-      }
-      const unsigned char across = valid ? d.lin[0][(long long)i * d.ny + j - 1 + side] : 0;
-      if (fused) {
-        double vu[NU];
-#pragma unroll
-        for (int u = 0; u < NU; u++) {
-          const int e = threadIdx.x + u * 2 * TF;
-          if (e < YRR * 4 * YUC) {
-            const int rr = e / (4 * YUC), v = (e / YUC) & 3, uc = e % YUC;
-            int col = j0 - 3 + uc;
-            col = col < d.ny ? col : d.ny - 1;
-            vu[u] = d.U[v * d.nc + (long long)clamp_row(i0 - 2 + rr) * d.ny + col];
-          }
-        }
-#pragma unroll
-        for (int u = 0; u < NU; u++) {
-          const int e = threadIdx.x + u * 2 * TF;
-          if (e < YRR * 4 * YUC) su[e] = vu[u];
-        }
-      } else if (d.smooth_flux) {
-        double vv[NL];
-#pragma unroll
-        for (int u = 0; u < NL; u++) {
-          const int e = threadIdx.x + u * 2 * TF;
-          if (e < YRR * 16 * YC) {
-            const int rr = e / (16 * YC), k = (e / YC) & 15, c = e % YC;
-            int col = j0 + c;
-            col = col <= d.ny ? col : d.ny;
-            vv[u] = d.line_y[k * nf + (long long)clamp_row(i0 - 2 + rr) * (d.ny + 1) + col];
-          }
-        }
-#pragma unroll
-        for (int u = 0; u < NL; u++) {
-          const int e = threadIdx.x + u * 2 * TF;
-          if (e < YRR * 16 * YC) st[e] = vv[u];
+        if (e < UR * 4 * UW) {
+          const int ur = e / (4 * UW), v = (e / UW) & 3, uc = e % UW;
+          const long long g = DIR == 0 ? (long long)clamp(i0 - 3 + ur, d.nx) * d.ny + clamp(j0 - 2 + uc, d.ny)
+                                       : (long long)clamp(i0 - 2 + ur, d.nx) * d.ny + clamp(j0 - 3 + uc, d.ny);
+          vu[u] = d.U[v * d.nc + g];
         }
       }
 #pragma unroll
-      for (int u = 0; u < NA; u++) {
+      for (int u = 0; u < NU; u++) {
         const int e = threadIdx.x + u * 2 * TF;
-        if (e < YRR * YAC) {
-          sa[e] = va[u];
-          sl1[e] = vl[u];
+        if (e < UR * 4 * UW) su[e] = vu[u];
+      }
+    } else if (d.smooth_flux) {
+      const double* line = DIR == 0 ? d.line_x : d.line_y;
+      double vv[NL];
+#pragma unroll
+      for (int u = 0; u < NL; u++) {
+        const int e = threadIdx.x + u * 2 * TF;
+        if (e < LR * 16 * LW) {
+          const int lr = e / (16 * LW), k = (e / LW) & 15, lc = e % LW;
+          // x faces: rows 0 .. nx, columns 0 .. ny - 1; y faces: rows 0 ..
+          // nx - 1, columns 0 .. ny
+          const long long g = DIR == 0
+                                  ? (long long)clamp(i0 + lr, d.nx + 1) * d.ny + clamp(j0 - 2 + lc, d.ny)
+                                  : (long long)clamp(i0 - 2 + lr, d.nx) * (d.ny + 1) + clamp(j0 + lc, d.ny + 1);
+          vv[u] = line[k * nf + g];
         }
       }
-      __syncthreads();
-      if (fused) {
-        // (row rr, column c): side 0 of face column c is the upper face of
-        // cell j0 + c - 1 (averages c .. c + 4), side 1 the lower face of
-        // cell j0 + c (c + 1 .. c + 5), in the x-like frame of the y sweep
 #pragma unroll
-        for (int u = 0; u < (YRR * YC + 2 * TF - 1) / (2 * TF); u++) {
-          const int e = threadIdx.x + u * 2 * TF;
-          if (e < YRR * YC) {
-            const int rr = e / YC, c = e - rr * YC;
+      for (int u = 0; u < NL; u++) {
+        const int e = threadIdx.x + u * 2 * TF;
+        if (e < LR * 16 * LW) st[e] = vv[u];
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < NA; u++) {
+      const int e = threadIdx.x + u * 2 * TF;
+      if (e < AR * AW) {
+        sa[e] = va[u];
+        sfl[e] = vl[u];
+      }
+    }
+    __syncthreads();
+    if (fused) {
+      // the linear reconstruction of every source cell of the tile: x
+      // faces, cell (i0 - 1 + cr, j0 - 2 + cc) with its stencil rows cr ..
+      // cr + 4, its lower face side 1 of face row cr - 1 and its upper face
+      // side 0 of face row cr; y faces, (row lr, column c) forms both sides
+      // of line record (lr, c): side 0 the upper face of cell j0 + c - 1
+      // (averages c .. c + 4), side 1 the lower face of cell j0 + c (c + 1
+      // .. c + 5), in the x-like frame of the y sweep
+      constexpr int NI = DIR == 0 ? AR * XW : LR * C;
+#pragma unroll
+      for (int u = 0; u < (NI + 2 * TF - 1) / (2 * TF); u++) {
+        const int e = threadIdx.x + u * 2 * TF;
+        if (e < NI) {
+          if constexpr (DIR == 0) {
+            const int cr = e / XW, cc = e - cr * XW;
+            double w[5][4], lo[8], ro[8];
+#pragma unroll
+            for (int s = 0; s < 5; s++)
+#pragma unroll
+              for (int v = 0; v < 4; v++) w[s][v] = su[((cr + s) * 4 + v) * UW + cc];
+            linear_normal(lo, ro, w, d.dx, d.idx, d.gas);
+            if (cr > 0) {
+#pragma unroll
+              for (int k = 0; k < 8; k++) st[((cr - 1) * 16 + 8 + k) * LW + cc] = lo[k];
+            }
+            if (cr < R) {
+#pragma unroll
+              for (int k = 0; k < 8; k++) st[(cr * 16 + k) * LW + cc] = ro[k];
+            }
+          } else {
+            const int lr = e / C, c = e - lr * C;
 #pragma unroll
             for (int sd = 0; sd < 2; sd++) {
               double w[5][4], lo[8], ro[8];
 #pragma unroll
               for (int s = 0; s < 5; s++) {
-                const double* a = su + rr * 4 * YUC + c + sd + s;
+                const double* a = su + lr * 4 * UW + c + sd + s;
                 w[s][0] = a[0];  // YchangetoX
-                w[s][1] = a[2 * YUC];
-                w[s][2] = -a[YUC];
-                w[s][3] = a[3 * YUC];
+                w[s][1] = a[2 * UW];
+                w[s][2] = -a[UW];
+                w[s][3] = a[3 * UW];
               }
               linear_normal(lo, ro, w, d.dy, d.idy, d.gas);
 #pragma unroll
-              for (int k = 0; k < 8; k++) st[(rr * 16 + sd * 8 + k) * YC + c] = sd == 0 ? ro[k] : lo[k];
+              for (int k = 0; k < 8; k++) st[(lr * 16 + sd * 8 + k) * LW + c] = sd == 0 ? ro[k] : lo[k];
             }
           }
         }
-        __syncthreads();
       }
-      // stencil s of this face: row i + 2 - s, staged row rl + 4 - s
-      if (valid && (want || fused)) {
-#pragma unroll
-        for (int s = 0; s < 5; s++)
-#pragma unroll
-          for (int k = 0; k < 8; k++) L[s][k] = st[((rl + 4 - s) * 16 + side * 8 + k) * YC + cl];
-      }
-      if (valid) {
-        // tangential_linear<1> on the staged values
-        double amax = 0.0;
-        unsigned smooth = across;
-#pragma unroll
-        for (int s = 0; s < 5; s++) {
-          amax = fmax(amax, sa[(rl + 4 - s) * YAC + cl + side]);
-          smooth &= sl1[(rl + 4 - s) * YAC + cl + side];
-        }
-        lin = smooth != 0 && amax == 0.0;
-      }
-      slin[side][tl] = lin ? 1 : 0;
-      // the Gauss-point tile below overwrites the staged values
       __syncthreads();
-      if (valid) count_linear(d.lstat + 2 * kStatSlots, lin);
-      if (fused && valid && !(d.smooth_flux && slin[0][tl] && slin[1][tl])) {
-        // a listed face: the line values of its stencil's linear source
-        // cells are stored for the list launch, entry s (row i + 2 - s) by
-        // the first listed face of the column that reads it (faces of rows
-        // i - 1 .. i - s of the tile are not listed)
-        bool seen = false, store[5];
+    }
+    // stencil entry s of this face and side: x faces, line record (i, j - 2
+    // + s) at staged (rl, cl + s), source cell row rl + side; y faces, line
+    // record (i + 2 - s, j) at staged (rl + 4 - s, cl), source cell (rl + 4
+    // - s, cl + side)
+    auto lidx = [&](int s, int k) {
+      return DIR == 0 ? (rl * 16 + side * 8 + k) * LW + cl + s : ((rl + 4 - s) * 16 + side * 8 + k) * LW + cl;
+    };
+    auto aidx = [&](int s) { return DIR == 0 ? (rl + side) * AW + cl + s : (rl + 4 - s) * AW + cl + side; };
+    double L[5][8];
+    if (valid && (want || fused)) {
+#pragma unroll
+      for (int s = 0; s < 5; s++)
+#pragma unroll
+        for (int k = 0; k < 8; k++) L[s][k] = st[lidx(s, k)];
+    }
+    bool lin = false;
+    if (valid) {
+      // tangential_linear<DIR> on the staged values
+      double amax = 0.0;
+      unsigned smooth = across;
+#pragma unroll
+      for (int s = 0; s < 5; s++) {
+        amax = fmax(amax, sa[aidx(s)]);
+        smooth &= sfl[aidx(s)];
+      }
+      lin = smooth != 0 && amax == 0.0;
+    }
+    slin[side][tl] = lin ? 1 : 0;
+    // the Gauss-point tile below overwrites the staged values
+    __syncthreads();
+    if (valid) count_linear(d.lstat + 2 * kStatSlots, lin);
+    if (fused && valid && !(d.smooth_flux && slin[0][tl] && slin[1][tl])) {
+      // a listed face: the line values of its stencil's linear source
+      // cells are stored for the list launch, each by the first listed face
+      // of the tile that reads it (x faces: entry s is read by faces j - 4
+      // + s .. j of the row, so it is stored unless one of faces tl - 1 ..
+      // tl - (4 - s) is listed; y faces: entry s by rows i - s .. i + 4 -
+      // s of the column, stored unless one of rows i - 1 .. i - s is)
+      bool store[5];
+      if constexpr (DIR == 0) {
+        bool seen = false;
+#pragma unroll
+        for (int k = 0; k <= 4; k++) {
+          if (k > 0 && cl - k >= 0 && !(d.smooth_flux && slin[0][tl - k] && slin[1][tl - k])) seen = true;
+          store[4 - k] = !seen;
+        }
+      } else {
+        bool seen = false;
 #pragma unroll
         for (int s = 0; s <= 4; s++) {
-          if (s > 0 && rl - s >= 0 && !(d.smooth_flux && slin[0][tl - s * YC] && slin[1][tl - s * YC]))
-            seen = true;
+          if (s > 0 && rl - s >= 0 && !(d.smooth_flux && slin[0][tl - s * C] && slin[1][tl - s * C])) seen = true;
           store[s] = !seen;
         }
+      }
+      double* line = DIR == 0 ? d.line_x : d.line_y;
 #pragma unroll
-        for (int s = 0; s < 5; s++) {
-          if (store[s] && sl1[(rl + 4 - s) * YAC + cl + side]) {
-            const long long f = (long long)(i + 2 - s) * (d.ny + 1) + j;
+      for (int s = 0; s < 5; s++) {
+        if (store[s] && sfl[aidx(s)]) {
+          const long long f = DIR == 0 ? (long long)i * d.ny + j - 2 + s : (long long)(i + 2 - s) * (d.ny + 1) + j;
 #pragma unroll
-            for (int k = 0; k < 8; k++) d.line_y[(side * 8 + k) * nf + f] = L[s][k];
-          }
+          for (int k = 0; k < 8; k++) line[(side * 8 + k) * nf + f] = L[s][k];
         }
       }
     }
@@ -1744,8 +1610,8 @@ __device__ __forceinline__ void tanflux_cheap_body(const Dev& d, unsigned blk, u
   (void)face_of(fp, ok2, i2, j2);
   if (!(ok2 && d.smooth_flux && (slin[0][fp] & slin[1][fp]))) return;
   const long long f2 = DIR == 0 ? (long long)i2 * d.ny + j2 : (long long)i2 * (d.ny + 1) + j2;
-  const Strided L{&sg[0][0][fp][q], 2 * TF}, R{&sg[1][0][fp][q], 2 * TF};
-  flux_pair_tail<DIR>(d, q, f2, nf, L, R, stage, dt, true);
+  const Strided L{&sg[0][0][fp][q], 2 * TF}, R2{&sg[1][0][fp][q], 2 * TF};
+  flux_pair_tail<DIR>(d, q, f2, nf, L, R2, stage, dt, true);
 }
 template <int DIR>
 __global__ void __launch_bounds__(2 * kTanFlux) k_tanflux_cheap(Dev d, int stage, double dt) {
@@ -2209,10 +2075,9 @@ static hipError_t flux(const Dev& d, int dir, int stage, double dt, hipStream_t 
       // smooth faces, then the listed rest (at most nfaces of them)
       // (the listed faces follow in flux_list)
       hipLaunchKernelGGL(k_clear_count, dim3(1), dim3(64), 0, s, d.tcount + dir * kListShards * kCountStride);
-      if (dir == 0) hipLaunchKernelGGL(k_tanflux_cheap<0>, dim3(nb), dim3(2 * kTanFlux), 0, s, d, stage, dt);
-      else
-        hipLaunchKernelGGL(k_tanflux_cheap<1>, dim3((unsigned)cheap_y_tiles(d, kTanFlux)), dim3(2 * kTanFlux), 0, s,
-                           d, stage, dt);
+      const dim3 grid((unsigned)cheap_tiles(d, dir, kTanFlux));
+      if (dir == 0) hipLaunchKernelGGL(k_tanflux_cheap<0>, grid, dim3(2 * kTanFlux), 0, s, d, stage, dt);
+      else hipLaunchKernelGGL(k_tanflux_cheap<1>, grid, dim3(2 * kTanFlux), 0, s, d, stage, dt);
     } else if (d.mlp) {
       if (dir == 0) hipLaunchKernelGGL((k_tanflux<0, true>), dim3(nb), dim3(2 * kTanFlux), 0, s, d, stage, dt);
       else hipLaunchKernelGGL((k_tanflux<1, true>), dim3(nb), dim3(2 * kTanFlux), 0, s, d, stage, dt);

@@ -74,15 +74,16 @@ constexpr int kCountStride = 32;
 // learned path: directions (bit 0 x, bit 1 y) whose smooth-face flux
 // launch forms the linear line values from the cell averages
 constexpr int kFusedLineDirs = 3;
-// learned path, y faces: the smooth-face flux launch takes tiles of
-// kYTileCols faces along y by TF / kYTileCols rows (TF faces per tile)
-constexpr int kYTileCols = 16;
-__host__ __device__ inline long long cheap_y_tile_cols(const Dev& d) {
-  return (d.NY + 1 + kYTileCols - 1) / kYTileCols;
+// learned path: the smooth-face flux launch takes tiles of TF faces,
+// TF / kCheapCols rows (i) by kCheapCols faces (j)
+constexpr int kCheapCols = 16;
+__host__ __device__ inline long long cheap_tile_cols(const Dev& d, int dir) {
+  const int nj = dir == 0 ? d.NY : d.NY + 1;  // faces per row
+  return (nj + kCheapCols - 1) / kCheapCols;
 }
-__host__ __device__ inline long long cheap_y_tiles(const Dev& d, int tf) {
-  const int rows = tf / kYTileCols;
-  return (long long)((d.NX + rows - 1) / rows) * cheap_y_tile_cols(d);
+__host__ __device__ inline long long cheap_tiles(const Dev& d, int dir, int tf) {
+  const int rows = tf / kCheapCols, ni = dir == 0 ? d.NX + 1 : d.NX;
+  return (long long)((ni + rows - 1) / rows) * cheap_tile_cols(d, dir);
 }
 
 // ------------------------------------------------------------------ batches
