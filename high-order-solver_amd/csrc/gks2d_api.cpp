@@ -427,8 +427,10 @@ int gks2d_cfl(gks2d_ctx* c, double t, double tstop, double* dt_out) {
   return GKS_OK;
 }
 
-// one stage of the driver loop (2Dproblem.cpp:137-158)
-static int stage_launch(gks2d_ctx* c, int stage, double dt, const int32_t* bc, bool readback) {
+// one stage of the driver loop (2Dproblem.cpp:137-158).  upd: the
+// descriptor the update launches with (its U the state written, its Uold
+// the step's initial state)
+static int stage_launch(gks2d_ctx* c, int stage, double dt, const int32_t* bc, bool readback, const Dev& upd) {
   for (int s = 0; s < 4; s++) RC(timed(c, PH_BC, [&] { return c->lt->bc(c->d, s, bc[s], c->s); }));
   RC(timed(c, PH_PRIMVAR, [&] { return c->lt->primvar(c->d, nullptr, stage, c->s); }));
   if (readback) {
@@ -437,9 +439,39 @@ static int stage_launch(gks2d_ctx* c, int stage, double dt, const int32_t* bc, b
   }
   RC(gks2d_reconstruct(c));
   RC(gks2d_flux(c, stage, dt));
-  RC(gks2d_update(c, stage));
+  RC(timed(c, PH_UPDATE, [&] { return c->lt->update(upd, stage, c->s); }));
   if (c->p.use_df && !c->p.df_frozen) RC(gks2d_update_alpha(c));
   return GKS_OK;
+}
+static int stage_launch(gks2d_ctx* c, int stage, double dt, const int32_t* bc, bool readback) {
+  return stage_launch(c, stage, dt, bc, readback, c->d);
+}
+
+// One S2O4 step of gks2d_run without CopyFluid_new_to_old: with A the
+// state (d.U) and B the second buffer (d.Uold), stage 0 reconstructs A and
+// writes its state A + dt F0 to B; stage 1 reconstructs B and updates A in
+// place (each cell reads its A value before writing it).  The arithmetic
+// is that of copy + two stages; the step ends with the state in A, whose
+// ghost frame still holds stage 0's fill (stage 1 filled B's): the run
+// copies B's frame into A when it ends, as the state of copy + two stages
+// has it.
+static int run_step(gks2d_ctx* c, double dt, const int32_t* bc) {
+  double* const A = c->d.U;
+  double* const B = c->d.Uold;
+  Dev upd = c->d;
+  upd.U = B;
+  upd.Uold = A;
+  RC(stage_launch(c, 0, dt, bc, false, upd));
+  c->d.U = B;
+  c->d.Uold = A;
+  upd = c->d;
+  upd.U = A;
+  // the stage-1 speed is the next step's (stale) CFL input: read it back
+  // while the rest of stage 1 runs
+  const int rc = stage_launch(c, 1, dt, bc, true, upd);
+  c->d.U = A;
+  c->d.Uold = B;
+  return rc;
 }
 
 int gks2d_step(gks2d_ctx* c, double dt, const int32_t bc[4]) {
@@ -467,14 +499,19 @@ int gks2d_run(gks2d_ctx* c, const int32_t bc[4], double tstop, int32_t max_steps
   HIPCHK(hipStreamSynchronize(c->s));
   double tmax = c->h_stat[2 * c->last_slot];
   int n = 0;
+  // the last step's stage-1 ghost fill, into the state (run_step) on every
+  // way out of the loop below once a step has run
+  struct GhostsBack {
+    gks2d_ctx* c;
+    int* n;
+    ~GhostsBack() {
+      if (*n > 0) (void)c->lt->copy_ghosts(c->d, c->d.Uold, c->s);
+    }
+  } ghosts_back{c, &n};
   while (t < tstop && n < max_steps) {
     if (c->h_stat[4] != 0.0) break;
     const double dt = dt_from_speed(c, tmax, t, tstop);
-    RC(gks2d_copy_old(c));
-    RC(stage_launch(c, 0, dt, bc, false));
-    // the stage-1 speed is the next step's (stale) CFL input: read it back
-    // while the rest of stage 1 runs
-    RC(stage_launch(c, 1, dt, bc, true));
+    RC(run_step(c, dt, bc));
     c->last_slot = 1;
     c->last_flux_stage = 1;
     HIPCHK(hipEventSynchronize(c->ev));

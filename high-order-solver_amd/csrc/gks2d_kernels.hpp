@@ -184,6 +184,15 @@ __device__ __forceinline__ void copy_old_body(const Dev& d, unsigned blk) {
   for (int v = 0; v < 4; v++) d.Uold[v * d.nc + c] = d.U[v * d.nc + c];
 }
 __global__ void __launch_bounds__(kBlock) k_copy_old(Dev d) { copy_old_body(d, blockIdx.x); }
+// the ghost frame (every cell outside the interior) of `from` into d.U
+__global__ void __launch_bounds__(kBlock) k_copy_ghosts(Dev d, const double* from) {
+  const long long c = gtid();
+  if (c >= d.nc) return;
+  const int i = (int)(c / d.ny), j = (int)(c % d.ny);
+  if (i >= d.G && i < d.nx - d.G && j >= d.G && j < d.ny - d.G) return;
+#pragma unroll
+  for (int v = 0; v < 4; v++) d.U[v * d.nc + c] = from[v * d.nc + c];
+}
 
 // ------------------------------------------------------ normal reconstruction
 // WENO5_AO_with_df (reconstruction.cpp:1731-1859) for one cell along DIR;
@@ -2032,6 +2041,10 @@ static hipError_t copy_old(const Dev& d, hipStream_t s) {
   hipLaunchKernelGGL(k_copy_old, dim3(blocks_for((long long)d.NX * d.NY)), dim3(kBlock), 0, s, d);
   return hipGetLastError();
 }
+static hipError_t copy_ghosts(const Dev& d, const double* from, hipStream_t s) {
+  hipLaunchKernelGGL(k_copy_ghosts, dim3(blocks_for(d.nc)), dim3(kBlock), 0, s, d, from);
+  return hipGetLastError();
+}
 static hipError_t normal(const Dev& d, int dir, hipStream_t s) {
   const unsigned nb = dir == 0 ? blocks_for((long long)(d.NX + 2) * d.ny) : blocks_for((long long)d.nx * (d.NY + 2));
   if (d.order == 7) {
@@ -2217,8 +2230,9 @@ static hipError_t batch_step(const BatchArgs& B, const BatchShape& sh, const uns
   return hipGetLastError();
 }
 
-const LaunchTable table = {bc,     primvar, copy_old,    normal, tangent, flux,       normal_list, flux_list,
-                          update, alpha,   flux_points, label,  infer,   batch_step, linear_points};
+const LaunchTable table = {bc,     primvar, copy_old,    copy_ghosts, normal,     tangent,      flux,
+                          normal_list, flux_list, update, alpha, flux_points, label, infer, batch_step,
+                          linear_points};
 #endif
 
 }  // namespace GKS_VARIANT

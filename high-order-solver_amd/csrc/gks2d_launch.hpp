@@ -138,6 +138,8 @@ struct LaunchTable {
   hipError_t (*bc)(const Dev&, int side, int kind, hipStream_t);
   hipError_t (*primvar)(const Dev&, const double* prim, int slot, hipStream_t);
   hipError_t (*copy_old)(const Dev&, hipStream_t);
+  // the ghost frame of `from` into the state (U)
+  hipError_t (*copy_ghosts)(const Dev&, const double* from, hipStream_t);
   hipError_t (*normal)(const Dev&, int dir, hipStream_t);
   hipError_t (*tangent)(const Dev&, int dir, hipStream_t);
   hipError_t (*flux)(const Dev&, int dir, int stage, double dt, hipStream_t);

@@ -155,9 +155,9 @@ def test_classifier_that_never_fires_gives_the_plain_run():
 def test_linear_everywhere_on_a_smooth_vortex(n):
     """A classifier that always fires: every stencil of the smooth vortex
     (DF sensor zero) takes the linear weights; the run stays conservative
-    and at least as accurate as WENO5-AO there.  At 40^2 a smooth-face
-    flux tile of 64 x faces covers three grid rows, too many for its LDS
-    stencil range (the per-thread stencils instead)."""
+    and at least as accurate as WENO5-AO there.  At 40^2 the smooth-face
+    flux tiles (16 faces wide) leave partly filled tiles at the grid's
+    edge in both directions."""
     from gks_amd.drivers import vortex_l1
     p = vortex_params(n, weno_order=5, fast_math=1)
     u0 = vortex_convar(p)
