@@ -253,7 +253,8 @@ int gks2d_create(const gks2d_params* p, gks2d_ctx** out) {
       (rc = dev_alloc(c, &d.xcx, 4 * gp * d.nfx)) || (rc = dev_alloc(c, &d.xcy, 4 * gp * d.nfy)) ||
       (rc = dev_alloc(c, &d.pcx, 4 * gp * d.nfx)) || (rc = dev_alloc(c, &d.pcy, 4 * gp * d.nfy)) ||
       (rc = dev_alloc(c, &d.akx, gp * d.nfx)) || (rc = dev_alloc(c, &d.aky, gp * d.nfy)) ||
-      (rc = dev_alloc(c, &d.partial, 2 * 1024)) || (rc = dev_alloc(c, &d.stat, 8)) ||
+      (rc = dev_alloc(c, &d.partial, 2 * std::max<long long>(kSweepPartials, (d.nc + kSweepBlock - 1) / kSweepBlock))) ||
+      (rc = dev_alloc(c, &d.stat, 8)) ||
       (rc = dev_alloc(c, &c->d_prim, 4 * d.nc))) {
     gks2d_destroy(c);
     return rc;
@@ -427,24 +428,15 @@ int gks2d_cfl(gks2d_ctx* c, double t, double tstop, double* dt_out) {
   return GKS_OK;
 }
 
-// one stage of the driver loop (2Dproblem.cpp:137-158).  upd: the
-// descriptor the update launches with (its U the state written, its Uold
-// the step's initial state)
-static int stage_launch(gks2d_ctx* c, int stage, double dt, const int32_t* bc, bool readback, const Dev& upd) {
+// one stage of the driver loop (2Dproblem.cpp:137-158)
+static int stage_launch(gks2d_ctx* c, int stage, double dt, const int32_t* bc) {
   for (int s = 0; s < 4; s++) RC(timed(c, PH_BC, [&] { return c->lt->bc(c->d, s, bc[s], c->s); }));
   RC(timed(c, PH_PRIMVAR, [&] { return c->lt->primvar(c->d, nullptr, stage, c->s); }));
-  if (readback) {
-    HIPCHK(hipMemcpyAsync(c->h_stat, c->d.stat, 8 * sizeof(double), hipMemcpyDeviceToHost, c->s));
-    HIPCHK(hipEventRecord(c->ev, c->s));
-  }
   RC(gks2d_reconstruct(c));
   RC(gks2d_flux(c, stage, dt));
-  RC(timed(c, PH_UPDATE, [&] { return c->lt->update(upd, stage, c->s); }));
+  RC(gks2d_update(c, stage));
   if (c->p.use_df && !c->p.df_frozen) RC(gks2d_update_alpha(c));
   return GKS_OK;
-}
-static int stage_launch(gks2d_ctx* c, int stage, double dt, const int32_t* bc, bool readback) {
-  return stage_launch(c, stage, dt, bc, readback, c->d);
 }
 
 // One S2O4 step of gks2d_run without CopyFluid_new_to_old: with A the
@@ -455,20 +447,37 @@ static int stage_launch(gks2d_ctx* c, int stage, double dt, const int32_t* bc, b
 // ghost frame still holds stage 0's fill (stage 1 filled B's): the run
 // copies B's frame into A when it ends, as the state of copy + two stages
 // has it.
-static int run_step(gks2d_ctx* c, double dt, const int32_t* bc) {
+//
+// Convar_to_Primvar's blow-up test and CFL speed come from the updates
+// (update_stat): stage 0's, on the state stage 1 starts from, into slot 1
+// (the next step's stale CFL input, read back right away), stage 1's, on
+// the state the next step starts from, into slot 0.  Only the run's first
+// stage runs the primvar sweep itself (first_step).
+static int run_stage(gks2d_ctx* c, int stage, double dt, const int32_t* bc, const Dev& upd, bool primvar) {
+  for (int s = 0; s < 4; s++) RC(timed(c, PH_BC, [&] { return c->lt->bc(c->d, s, bc[s], c->s); }));
+  if (primvar) RC(timed(c, PH_PRIMVAR, [&] { return c->lt->primvar(c->d, nullptr, stage, c->s); }));
+  RC(gks2d_reconstruct(c));
+  RC(gks2d_flux(c, stage, dt));
+  RC(timed(c, PH_UPDATE, [&] { return c->lt->update_stat(upd, 1 - stage, c->s); }));
+  if (stage == 0) {
+    HIPCHK(hipMemcpyAsync(c->h_stat, c->d.stat, 8 * sizeof(double), hipMemcpyDeviceToHost, c->s));
+    HIPCHK(hipEventRecord(c->ev, c->s));
+  }
+  if (c->p.use_df && !c->p.df_frozen) RC(gks2d_update_alpha(c));
+  return GKS_OK;
+}
+static int run_step(gks2d_ctx* c, double dt, const int32_t* bc, bool first_step) {
   double* const A = c->d.U;
   double* const B = c->d.Uold;
   Dev upd = c->d;
   upd.U = B;
   upd.Uold = A;
-  RC(stage_launch(c, 0, dt, bc, false, upd));
+  RC(run_stage(c, 0, dt, bc, upd, first_step));
   c->d.U = B;
   c->d.Uold = A;
   upd = c->d;
   upd.U = A;
-  // the stage-1 speed is the next step's (stale) CFL input: read it back
-  // while the rest of stage 1 runs
-  const int rc = stage_launch(c, 1, dt, bc, true, upd);
+  const int rc = run_stage(c, 1, dt, bc, upd, false);
   c->d.U = A;
   c->d.Uold = B;
   return rc;
@@ -479,8 +488,8 @@ int gks2d_step(gks2d_ctx* c, double dt, const int32_t bc[4]) {
   for (int s = 0; s < 4; s++)
     if (bc[s] < 0 || bc[s] > 2) return fail(GKS_ERR_ARG, "bc kind");
   RC(gks2d_copy_old(c));
-  RC(stage_launch(c, 0, dt, bc, false));
-  RC(stage_launch(c, 1, dt, bc, false));
+  RC(stage_launch(c, 0, dt, bc));
+  RC(stage_launch(c, 1, dt, bc));
   c->last_slot = 1;
   c->last_flux_stage = 1;
   return GKS_OK;
@@ -498,20 +507,21 @@ int gks2d_run(gks2d_ctx* c, const int32_t bc[4], double tstop, int32_t max_steps
   HIPCHK(hipMemcpyAsync(c->h_stat, c->d.stat, 8 * sizeof(double), hipMemcpyDeviceToHost, c->s));
   HIPCHK(hipStreamSynchronize(c->s));
   double tmax = c->h_stat[2 * c->last_slot];
-  int n = 0;
+  int n = 0, launched = 0;
   // the last step's stage-1 ghost fill, into the state (run_step) on every
-  // way out of the loop below once a step has run
+  // way out of the loop below once a step has been launched
   struct GhostsBack {
     gks2d_ctx* c;
-    int* n;
+    int* launched;
     ~GhostsBack() {
-      if (*n > 0) (void)c->lt->copy_ghosts(c->d, c->d.Uold, c->s);
+      if (*launched > 0) (void)c->lt->copy_ghosts(c->d, c->d.Uold, c->s);
     }
-  } ghosts_back{c, &n};
+  } ghosts_back{c, &launched};
   while (t < tstop && n < max_steps) {
     if (c->h_stat[4] != 0.0) break;
     const double dt = dt_from_speed(c, tmax, t, tstop);
-    RC(run_step(c, dt, bc));
+    RC(run_step(c, dt, bc, n == 0));
+    launched++;
     c->last_slot = 1;
     c->last_flux_stage = 1;
     HIPCHK(hipEventSynchronize(c->ev));

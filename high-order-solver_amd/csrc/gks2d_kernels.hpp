@@ -148,11 +148,36 @@ __global__ void __launch_bounds__(kBlock) k_primvar(Dev d, const double* prim, d
 __device__ __forceinline__ void reduce_body(const double* partial, int n, double* stat, int slot) {
   __shared__ double smax[kBlock];
   __shared__ double sbad[kBlock];
-  double m = 0.0, b = 0.0;
-  for (int k = threadIdx.x; k < n; k += kBlock) {
-    double o = partial[2 * k];
-    if (o > m) m = o;
-    if (partial[2 * k + 1] != 0.0) b = 1.0;
+  // eight partials per thread and round, their loads issued together (the
+  // update's reduction has a partial per 256 cells: 16384 at 2048^2)
+  constexpr int U = 8;
+  double mu[U], bu[U];
+#pragma unroll
+  for (int u = 0; u < U; u++) mu[u] = bu[u] = 0.0;
+  int k = threadIdx.x;
+  for (; k + (U - 1) * kBlock < n; k += U * kBlock) {
+    double o[U], f[U];
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      o[u] = partial[2 * (k + u * kBlock)];
+      f[u] = partial[2 * (k + u * kBlock) + 1];
+    }
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      if (o[u] > mu[u]) mu[u] = o[u];
+      if (f[u] != 0.0) bu[u] = 1.0;
+    }
+  }
+  for (; k < n; k += kBlock) {
+    const double o = partial[2 * k];
+    if (o > mu[0]) mu[0] = o;
+    if (partial[2 * k + 1] != 0.0) bu[0] = 1.0;
+  }
+  double m = mu[0], b = bu[0];
+#pragma unroll
+  for (int u = 1; u < U; u++) {
+    if (mu[u] > m) m = mu[u];
+    if (bu[u] != 0.0) b = 1.0;
   }
   smax[threadIdx.x] = m;
   sbad[threadIdx.x] = b;
@@ -1654,9 +1679,8 @@ __global__ void __launch_bounds__(64) k_clear_count(unsigned* count) {
 // cell update of Update_with_gauss (time_advance.cpp:442-474): U = U_old +
 // sum over Gauss points of len * (x_S - x_N + x_W - x_E), in the reference's
 // summation order, divided by the cell area
-__device__ __forceinline__ void update_body(const Dev& d, unsigned blk) {
-  long long t = (long long)blk * kBlock + threadIdx.x;
-  if (t >= (long long)d.NX * d.NY) return;
+// the update of cell t of the interior; u: its new state
+__device__ __forceinline__ void update_cell(const Dev& d, long long t, double* u_out) {
   const int i = (int)(t / d.NY) + d.G, j = (int)(t % d.NY) + d.G;
   const long long c = (long long)i * d.ny + j;
   const long long fb = (long long)i * (d.ny + 1) + j, ft = fb + 1;
@@ -1679,9 +1703,50 @@ __device__ __forceinline__ void update_body(const Dev& d, unsigned blk) {
     double u = d.Uold[v * d.nc + c];
     u += total[v] / d.area;
     d.U[v * d.nc + c] = u;
+    u_out[v] = u;
   }
 }
+__device__ __forceinline__ void update_body(const Dev& d, unsigned blk) {
+  const long long t = (long long)blk * kBlock + threadIdx.x;
+  double u[4];
+  if (t < (long long)d.NX * d.NY) update_cell(d, t, u);
+}
 __global__ void __launch_bounds__(kBlock) k_update(Dev d) { update_body(d, blockIdx.x); }
+// the update and, over the cells it wrote, what Convar_to_Primvar would
+// find on the new state: the blow-up test and the largest Dtx speed, into
+// the block's partial pair (as k_primvar: its test also covers the ghost
+// cells, which the boundary fill makes copies or mirror images of interior
+// cells, so it finds what this one finds)
+__global__ void __launch_bounds__(kBlock) k_update_stat(Dev d, double* partial) {
+  __shared__ double smax[kBlock];
+  __shared__ int sbad[kBlock];
+  const long long t = (long long)blockIdx.x * kBlock + threadIdx.x;
+  double tmax = 0.0;
+  int bad = 0;
+  if (t < (long long)d.NX * d.NY) {
+    double u[4], p[4];
+    update_cell(d, t, u);
+    convar_to_primvar(p, u, d.gas);
+    if (p[0] < 0 || p[3] < 0 || p[0] != p[0] || p[3] != p[3]) bad = 1;
+    const double tmp = sqrt(p[1] * p[1] + p[2] * p[2]) + sqrt(d.gas.gamma * p[3] / p[0]);
+    if (tmp > tmax) tmax = tmp;  // a NaN speed is skipped, as Dtx's comparison does
+  }
+  smax[threadIdx.x] = tmax;
+  sbad[threadIdx.x] = bad;
+  __syncthreads();
+  for (int s = kBlock / 2; s > 0; s >>= 1) {
+    if (threadIdx.x < s) {
+      const double o = smax[threadIdx.x + s];
+      if (o > smax[threadIdx.x]) smax[threadIdx.x] = o;
+      sbad[threadIdx.x] |= sbad[threadIdx.x + s];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    partial[2 * blockIdx.x] = smax[0];
+    partial[2 * blockIdx.x + 1] = sbad[0] ? 1.0 : 0.0;
+  }
+}
 
 // Update_alpha (reconstruction.cpp:1466-1534): the reference bounds j by
 // nodex (:1528); kept for nodey >= nodex, clipped to nodey otherwise.
@@ -2145,6 +2210,12 @@ static hipError_t update(const Dev& d, int stage, hipStream_t s) {
   hipLaunchKernelGGL(k_update, dim3(blocks_for((long long)d.NX * d.NY)), dim3(kBlock), 0, s, d);
   return hipGetLastError();
 }
+static hipError_t update_stat(const Dev& d, int slot, hipStream_t s) {
+  const unsigned nb = blocks_for((long long)d.NX * d.NY);
+  hipLaunchKernelGGL(k_update_stat, dim3(nb), dim3(kBlock), 0, s, d, d.partial);
+  hipLaunchKernelGGL(k_reduce, dim3(1), dim3(kBlock), 0, s, (const double*)d.partial, (int)nb, d.stat, slot);
+  return hipGetLastError();
+}
 static hipError_t alpha(const Dev& d, hipStream_t s) {
   const int nj = d.NY < d.NX ? d.NY : d.NX;
   hipLaunchKernelGGL(k_alpha, dim3(blocks_for((long long)d.NX * nj)), dim3(kBlock), 0, s, d);
@@ -2231,8 +2302,8 @@ static hipError_t batch_step(const BatchArgs& B, const BatchShape& sh, const uns
 }
 
 const LaunchTable table = {bc,     primvar, copy_old,    copy_ghosts, normal,     tangent,      flux,
-                          normal_list, flux_list, update, alpha, flux_points, label, infer, batch_step,
-                          linear_points};
+                          normal_list, flux_list, update, update_stat, alpha, flux_points, label, infer,
+                          batch_step, linear_points};
 #endif
 
 }  // namespace GKS_VARIANT

@@ -58,7 +58,7 @@ struct Dev {
   unsigned* lpre;               // per list (BatchList order) 32 words: the first tile of each shard, and at
                                 // [kListShards] the list's tiles (formed between the sweep that fills the
                                 // list and the list's launch)
-  double* partial;  // per-block (max speed, blow-up flag) pairs
+  double* partial;  // per-block (max speed, blow-up flag) pairs: max(kSweepPartials, update blocks) of them
   double* stat;     // [0..3]: max speed of slot 0 / 1 (+pad); [4]: sticky blow-up flag
 };
 
@@ -148,6 +148,9 @@ struct LaunchTable {
   hipError_t (*normal_list)(const Dev&, int dir, hipStream_t);
   hipError_t (*flux_list)(const Dev&, int dir, int stage, double dt, hipStream_t);
   hipError_t (*update)(const Dev&, int stage, hipStream_t);
+  // the update, and the new state's blow-up test and CFL speed into stat
+  // slot `slot` (as primvar's)
+  hipError_t (*update_stat)(const Dev&, int slot, hipStream_t);
   hipError_t (*alpha)(const Dev&, hipStream_t);
   hipError_t (*flux_points)(const gksd::Gas&, long long n, const double* L, const double* R, double dt,
                             double* C, double* F, double* D, int smooth, hipStream_t);
