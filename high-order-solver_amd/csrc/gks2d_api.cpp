@@ -697,7 +697,7 @@ static int fused_line_dirs(const Dev& d) {
   return d.smooth_flux && d.fuse_tanflux && d.order == 5 ? kFusedLineDirs : 0;
 }
 static size_t list_words(const Dev& d) {
-  return (size_t)4 * kListShards * kCountStride + 4 * 32 +
+  return (size_t)4 * kListShards * kCountStride + 4 * kListRecord +
          (size_t)kListShards * ((size_t)d.tcap[0] + d.tcap[1] + d.ncap[0] + d.ncap[1]);
 }
 int gks2d_set_classifier(gks2d_ctx* c, const float* params, float logit_threshold) {
@@ -723,6 +723,9 @@ int gks2d_set_classifier(gks2d_ctx* c, const float* params, float logit_threshol
   HIPCHK(hipMemcpy(c->d_mlpw, packed.data(), packed.size() * sizeof(uint16_t), hipMemcpyHostToDevice));
   HIPCHK(hipMemset(c->d_lstat, 0, kLstatWords * sizeof(unsigned long long)));
   HIPCHK(hipMemset(c->d_lin, 0, 2 * nc));
+  // the shard counts start cleared; each list's prefix kernel clears them
+  // again for the next sweep
+  HIPCHK(hipMemset(c->d_tlist, 0, (size_t)4 * kListShards * kCountStride * sizeof(unsigned)));
   c->d.mlp = c->d_mlp;
   c->d.mlpw = c->d_mlpw;
   c->d.smooth_flux = 1;
@@ -731,7 +734,7 @@ int gks2d_set_classifier(gks2d_ctx* c, const float* params, float logit_threshol
   c->d.tcount = c->d_tlist;
   c->d.ncount = c->d.tcount + 2 * kListShards * kCountStride;
   c->d.lpre = c->d.tcount + 4 * kListShards * kCountStride;
-  c->d.tlist[0] = c->d.lpre + 4 * 32;
+  c->d.tlist[0] = c->d.lpre + 4 * kListRecord;
   c->d.tlist[1] = c->d.tlist[0] + (size_t)kListShards * c->d.tcap[0];
   c->d.nlist[0] = c->d.tlist[1] + (size_t)kListShards * c->d.tcap[1];
   c->d.nlist[1] = c->d.nlist[0] + (size_t)kListShards * c->d.ncap[0];

@@ -517,9 +517,12 @@ __device__ __forceinline__ unsigned* shard_count(unsigned* cnt, int dir, int s) 
 __device__ __forceinline__ int list_shard(unsigned blk, unsigned ntiles) {
   return (int)((unsigned long long)blk * kListShards / ntiles);
 }
-// the tile prefix of the dir list (per_tile entries per tile, the shards'
-// tiles one after the other) into pre (Dev::lpre layout); one lane
-__device__ __forceinline__ unsigned list_prefix(const unsigned* cnt, int dir, int per_tile, unsigned* pre) {
+// the prefix record of the dir list (per_tile entries per tile, the
+// shards' tiles one after the other) into pre (Dev::lpre layout): the
+// shards' first tiles, the tile count, the shards' entry counts; the shard
+// counters are cleared for the list's next sweep (the list's launch reads
+// the record).  One lane; returns the tile count.
+__device__ __forceinline__ unsigned list_prefix(unsigned* cnt, int dir, int per_tile, unsigned* pre) {
   unsigned c[kListShards];
 #pragma unroll
   for (int k = 0; k < kListShards; k++) c[k] = cnt[(dir * kListShards + k) * kCountStride];
@@ -527,16 +530,17 @@ __device__ __forceinline__ unsigned list_prefix(const unsigned* cnt, int dir, in
 #pragma unroll
   for (int k = 0; k < kListShards; k++) {
     pre[k] = acc;
+    pre[kListShards + 1 + k] = c[k];
+    cnt[(dir * kListShards + k) * kCountStride] = 0u;
     acc += (c[k] + per_tile - 1) / per_tile;
   }
   pre[kListShards] = acc;
   return acc;
 }
-// tile b of the dir list: its shard s, the tile tb within the shard and the
-// shard's count n; false past the last tile (one load).  pre: the list's
-// tile prefix.  Uniform over a workgroup.
-__device__ __forceinline__ bool list_tile(const unsigned* cnt, const unsigned* pre, int dir, unsigned b, int& s,
-                                          unsigned& tb, unsigned& n) {
+// tile b of a list: its shard s, the tile tb within the shard and the
+// shard's count n; false past the last tile.  pre: the list's prefix
+// record.  Uniform over a workgroup.
+__device__ __forceinline__ bool list_tile(const unsigned* pre, unsigned b, int& s, unsigned& tb, unsigned& n) {
   if (b >= pre[kListShards]) return false;
   // the last shard starting at or before b (an empty shard starts where
   // the next one does)
@@ -545,18 +549,18 @@ __device__ __forceinline__ bool list_tile(const unsigned* cnt, const unsigned* p
   for (int k = 1; k < kListShards; k++)
     if (pre[k] <= b) s = k;
   tb = b - pre[s];
-  n = cnt[(dir * kListShards + s) * kCountStride];
+  n = pre[kListShards + 1 + s];
   return true;
 }
-// entries of a sharded list
-__device__ __forceinline__ unsigned list_total(const unsigned* cnt, int dir) {
+// entries of a list (pre: its prefix record)
+__device__ __forceinline__ unsigned list_total(const unsigned* pre) {
   unsigned n = 0;
-  for (int k = 0; k < kListShards; k++) n += cnt[(dir * kListShards + k) * kCountStride];
+  for (int k = 0; k < kListShards; k++) n += pre[kListShards + 1 + k];
   return n;
 }
 // forms one list's tile prefix (single-problem path)
 __global__ void __launch_bounds__(64) k_list_prefix(Dev d, int kind, int per_tile) {
-  if (threadIdx.x == 0) list_prefix(kind < KL_FLUX_X ? d.ncount : d.tcount, kind & 1, per_tile, d.lpre + kind * 32);
+  if (threadIdx.x == 0) list_prefix(kind < KL_FLUX_X ? d.ncount : d.tcount, kind & 1, per_tile, d.lpre + kind * kListRecord);
 }
 
 // the two face records a cell's normal reconstruction writes (side 1 of
@@ -659,7 +663,7 @@ k_normal_learned(Dev d) {
 template <int DIR> __device__ __forceinline__ void normal_list_body(const Dev& d, unsigned blk) {
   int s;
   unsigned tb, n;
-  if (!list_tile(d.ncount, d.lpre + (DIR == 0 ? KL_NORMAL_X : KL_NORMAL_Y) * 32, DIR, blk, s, tb, n)) return;
+  if (!list_tile(d.lpre + (DIR == 0 ? KL_NORMAL_X : KL_NORMAL_Y) * kListRecord, blk, s, tb, n)) return;
   const long long k = (long long)tb * kBlock + threadIdx.x;
   if (k >= (long long)n) return;
   int i, j;
@@ -1662,17 +1666,11 @@ template <int DIR>
 __global__ void __launch_bounds__(2 * kTanFlux) __attribute__((amdgpu_waves_per_eu(GKS_FLUX_WAVES)))
 k_tanflux_list(Dev d, int stage, double dt) {
   if (blockIdx.x == 0 && threadIdx.x == 0)
-    atomicAdd(d.lstat + kListedWord + DIR, (unsigned long long)list_total(d.tcount, DIR));
+    atomicAdd(d.lstat + kListedWord + DIR, (unsigned long long)list_total(d.lpre + (KL_FLUX_X + DIR) * kListRecord));
   int s;
   unsigned tb, n;
-  if (list_tile(d.tcount, d.lpre + (KL_FLUX_X + DIR) * 32, DIR, blockIdx.x, s, tb, n))
+  if (list_tile(d.lpre + (KL_FLUX_X + DIR) * kListRecord, blockIdx.x, s, tb, n))
     tanflux_body<DIR, true, kTanFlux, true>(d, tb, stage, dt, d.tlist[DIR] + (size_t)s * d.tcap[DIR], n);
-}
-// clears a list's shard counts (count: its first) before the sweep that
-// fills it (a kernel, not a memset, so that the sequence can be captured in
-// a graph)
-__global__ void __launch_bounds__(64) k_clear_count(unsigned* count) {
-  if (threadIdx.x < kListShards) count[threadIdx.x * kCountStride] = 0u;
 }
 
 // ------------------------------------------------------------------ update
@@ -1890,8 +1888,8 @@ __global__ void __launch_bounds__(64) kb_list_prefix(BatchArgs B, int kind, int 
   for (int q = threadIdx.x; q < B.np; q += 64) {
     const int p = B.pid[q];
     const Dev& d = B.dev[p];
-    const unsigned* cnt = kind < KL_FLUX_X ? d.ncount : d.tcount;
-    ls[q] = (B.st[p].active && cnt) ? list_prefix(cnt, kind & 1, per_tile, d.lpre + kind * 32) : 0u;
+    unsigned* cnt = kind < KL_FLUX_X ? d.ncount : d.tcount;
+    ls[q] = (B.st[p].active && cnt) ? list_prefix(cnt, kind & 1, per_tile, d.lpre + kind * kListRecord) : 0u;
   }
   __threadfence_block();
   __syncthreads();
@@ -1947,24 +1945,11 @@ __global__ void __launch_bounds__(2 * TF) __attribute__((amdgpu_waves_per_eu(GKS
 kb_tanflux_list(BatchArgs B, int stage) {
   GKS_BATCH_LIST_TILE(DIR == 0 ? KL_FLUX_X : KL_FLUX_Y);
   if (blk == 0 && threadIdx.x == 0)
-    atomicAdd(d.lstat + kListedWord + DIR, (unsigned long long)list_total(d.tcount, DIR));
+    atomicAdd(d.lstat + kListedWord + DIR, (unsigned long long)list_total(d.lpre + (KL_FLUX_X + DIR) * kListRecord));
   int s;
   unsigned tb, n;
-  if (list_tile(d.tcount, d.lpre + (KL_FLUX_X + DIR) * 32, DIR, blk, s, tb, n))
+  if (list_tile(d.lpre + (KL_FLUX_X + DIR) * kListRecord, blk, s, tb, n))
     tanflux_body<DIR, true, TF, true>(d, tb, stage, B.st[p_].dt, d.tlist[DIR] + (size_t)s * d.tcap[DIR], n);
-}
-// clears the shard counts of the four lists (flux x, y; normal x, y) of
-// every active problem (one lane per problem), before the stage's normal
-// sweeps
-__global__ void __launch_bounds__(64) kb_clear_counts(BatchArgs B) {
-  for (int q = threadIdx.x; q < B.np; q += 64) {
-    const int p = B.pid[q];
-    if (!B.st[p].active) continue;
-    unsigned* cnt[2] = {B.dev[p].tcount, B.dev[p].ncount};
-    for (int l = 0; l < 2; l++)
-      if (cnt[l])
-        for (int k = 0; k < 2 * kListShards; k++) cnt[l][k * kCountStride] = 0u;
-  }
 }
 __global__ void __launch_bounds__(kBlock) kb_update(BatchArgs B) {
   GKS_BATCH_TILE(KB_UPDATE);
@@ -2119,8 +2104,8 @@ static hipError_t normal(const Dev& d, int dir, hipStream_t s) {
     if (dir == 0) hipLaunchKernelGGL((k_normal79<0, 9>), dim3(nb), dim3(kBlock), 0, s, d);
     else hipLaunchKernelGGL((k_normal79<1, 9>), dim3(nb), dim3(kBlock), 0, s, d);
   } else if (d.mlp) {
-    // the listed cells follow in normal_list
-    hipLaunchKernelGGL(k_clear_count, dim3(1), dim3(64), 0, s, d.ncount + dir * kListShards * kCountStride);
+    // the listed cells follow in normal_list (whose prefix kernel leaves the
+    // shard counts cleared for the next sweep)
     if (dir == 0) hipLaunchKernelGGL(k_normal_learned<0>, dim3(nb), dim3(kBlock), 0, s, d);
     else hipLaunchKernelGGL(k_normal_learned<1>, dim3(nb), dim3(kBlock), 0, s, d);
   } else {
@@ -2152,7 +2137,6 @@ static hipError_t flux(const Dev& d, int dir, int stage, double dt, hipStream_t 
     if (d.mlp && d.smooth_flux && d.tcount) {
       // smooth faces, then the listed rest (at most nfaces of them)
       // (the listed faces follow in flux_list)
-      hipLaunchKernelGGL(k_clear_count, dim3(1), dim3(64), 0, s, d.tcount + dir * kListShards * kCountStride);
       const dim3 grid((unsigned)cheap_tiles(d, dir, kTanFlux));
       if (dir == 0) hipLaunchKernelGGL(k_tanflux_cheap<0>, grid, dim3(2 * kTanFlux), 0, s, d, stage, dt);
       else hipLaunchKernelGGL(k_tanflux_cheap<1>, grid, dim3(2 * kTanFlux), 0, s, d, stage, dt);
@@ -2255,7 +2239,6 @@ static void batch_stage(const BatchArgs& B, const unsigned* tiles, int stage, hi
     hipLaunchKernelGGL(kb_bc, dim3(tiles[side < 2 ? KB_BC_LR : KB_BC_DU]), dim3(kBlock), 0, s, B, side);
   hipLaunchKernelGGL(kb_primvar, dim3(tiles[KB_PRIMVAR]), dim3(kBlock), 0, s, B);
   hipLaunchKernelGGL(kb_reduce, dim3(B.np), dim3(kBlock), 0, s, B, stage);
-  if (LEARNED) hipLaunchKernelGGL(kb_clear_counts, dim3(1), dim3(64), 0, s, B);
   hipLaunchKernelGGL((kb_normal<0, LEARNED>), dim3(tiles[KB_NORMAL_X]), dim3(kBlock), 0, s, B);
   if (LEARNED) {
     hipLaunchKernelGGL(kb_list_prefix, dim3(1), dim3(64), 0, s, B, (int)KL_NORMAL_X, kBlock);

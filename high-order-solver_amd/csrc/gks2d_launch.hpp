@@ -55,9 +55,10 @@ struct Dev {
                                 // sweep's range); shard s at nlist[dir] + s * ncap[dir]
   unsigned* ncount;             // their shard counts, laid out as tcount's (cleared before each normal sweep)
   unsigned tcap[2], ncap[2];    // entries one shard holds
-  unsigned* lpre;               // per list (BatchList order) 32 words: the first tile of each shard, and at
-                                // [kListShards] the list's tiles (formed between the sweep that fills the
-                                // list and the list's launch)
+  unsigned* lpre;               // per list (BatchList order) kListRecord words: the first tile of each shard,
+                                // at [kListShards] the list's tiles, then the shards' entries (formed between
+                                // the sweep that fills the list and the list's launch, which clears the shard
+                                // counts for the list's next sweep)
   double* partial;  // per-block (max speed, blow-up flag) pairs: max(kSweepPartials, update blocks) of them
   double* stat;     // [0..3]: max speed of slot 0 / 1 (+pad); [4]: sticky blow-up flag
 };
@@ -71,6 +72,9 @@ constexpr int kSweepPartials = 1024;
 // queue behind each other)
 constexpr int kListShards = 16;
 constexpr int kCountStride = 32;
+// words of a list's prefix record (Dev::lpre): kListShards first tiles, the
+// tile count, kListShards entry counts
+constexpr int kListRecord = 48;
 // learned path: directions (bit 0 x, bit 1 y) whose smooth-face flux
 // launch forms the linear line values from the cell averages
 constexpr int kFusedLineDirs = 3;
