@@ -430,7 +430,8 @@ int gks2d_cfl(gks2d_ctx* c, double t, double tstop, double* dt_out) {
 
 // one stage of the driver loop (2Dproblem.cpp:137-158)
 static int stage_launch(gks2d_ctx* c, int stage, double dt, const int32_t* bc) {
-  for (int s = 0; s < 4; s++) RC(timed(c, PH_BC, [&] { return c->lt->bc(c->d, s, bc[s], c->s); }));
+  const int kinds[4] = {bc[0], bc[1], bc[2], bc[3]};
+  RC(timed(c, PH_BC, [&] { return c->lt->bc_all(c->d, kinds, c->s); }));
   RC(timed(c, PH_PRIMVAR, [&] { return c->lt->primvar(c->d, nullptr, stage, c->s); }));
   RC(gks2d_reconstruct(c));
   RC(gks2d_flux(c, stage, dt));
@@ -454,7 +455,8 @@ static int stage_launch(gks2d_ctx* c, int stage, double dt, const int32_t* bc) {
 // the state the next step starts from, into slot 0.  Only the run's first
 // stage runs the primvar sweep itself (first_step).
 static int run_stage(gks2d_ctx* c, int stage, double dt, const int32_t* bc, const Dev& upd, bool primvar) {
-  for (int s = 0; s < 4; s++) RC(timed(c, PH_BC, [&] { return c->lt->bc(c->d, s, bc[s], c->s); }));
+  const int kinds[4] = {bc[0], bc[1], bc[2], bc[3]};
+  RC(timed(c, PH_BC, [&] { return c->lt->bc_all(c->d, kinds, c->s); }));
   if (primvar) RC(timed(c, PH_PRIMVAR, [&] { return c->lt->primvar(c->d, nullptr, stage, c->s); }));
   RC(gks2d_reconstruct(c));
   RC(gks2d_flux(c, stage, dt));

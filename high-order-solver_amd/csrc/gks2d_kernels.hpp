@@ -94,6 +94,56 @@ __device__ __forceinline__ void bc_body(const Dev& d, unsigned blk, int side, in
   }
 }
 __global__ void __launch_bounds__(kBlock) k_bc(Dev d, int side, int kind) { bc_body(d, blockIdx.x, side, kind); }
+// the four fills above in one launch (kinds: left, right, down, up).  The
+// left/right fills write the ghost rows from interior rows (all columns),
+// the down/up fills then the ghost columns of every row from interior
+// columns (the ghost rows' included), so every ghost cell is a copy of one
+// interior cell with component 1 negated by a reflective row fill and
+// component 2 by a reflective column fill: each cell is formed from its
+// interior source directly (the same values; needs NX, NY >= G, so that
+// the periodic sources are interior cells).  A thread per ghost cell: the
+// 2G ghost rows, then the 2G ghost columns of the interior rows.
+__global__ void __launch_bounds__(kBlock) k_bc_all(Dev d, int k0, int k1, int k2, int k3) {
+  const long long t = gtid();
+  const long long nrow = 2LL * d.G * d.ny;
+  if (t >= nrow + 2LL * d.G * d.NX) return;
+  int i, j;
+  if (t < nrow) {
+    const int g = (int)(t / d.ny);
+    i = g < d.G ? g : d.nx - 2 * d.G + g;
+    j = (int)(t % d.ny);
+  } else {
+    const long long u = t - nrow;
+    const int g = (int)(u % (2 * d.G));
+    i = d.G + (int)(u / (2 * d.G));
+    j = g < d.G ? g : d.ny - 2 * d.G + g;
+  }
+  int si = i, sj = j;
+  bool neg1 = false, neg2 = false;
+  if (i < d.G || i >= d.nx - d.G) {
+    const bool lo = i < d.G;
+    const int kind = lo ? k0 : k1;
+    if (kind == 0) si = lo ? d.G : d.nx - d.G - 1;
+    else if (kind == 1) si = lo ? i + d.NX : i - d.NX;
+    else si = lo ? 2 * d.G - 1 - i : 2 * d.nx - 2 * d.G - 1 - i;
+    neg1 = kind == 2;
+  }
+  if (j < d.G || j >= d.ny - d.G) {
+    const bool lo = j < d.G;
+    const int kind = lo ? k2 : k3;
+    if (kind == 0) sj = lo ? d.G : d.ny - d.G - 1;
+    else if (kind == 1) sj = lo ? j + d.NY : j - d.NY;
+    else sj = lo ? 2 * d.G - 1 - j : 2 * d.ny - 2 * d.G - 1 - j;
+    neg2 = kind == 2;
+  }
+  const long long dst = (long long)i * d.ny + j, src = (long long)si * d.ny + sj;
+#pragma unroll
+  for (int v = 0; v < 4; v++) {
+    double val = d.U[v * d.nc + src];
+    if ((v == 1 && neg1) || (v == 2 && neg2)) val = -val;
+    d.U[v * d.nc + dst] = val;
+  }
+}
 
 // ------------------------------------------------------- primvar + CFL speed
 // Convar_to_Primvar (basic_function.cpp:752-773): blow-up test on all cells;
@@ -2080,6 +2130,19 @@ static hipError_t bc(const Dev& d, int side, int kind, hipStream_t s) {
   hipLaunchKernelGGL(k_bc, dim3(blocks_for(n)), dim3(kBlock), 0, s, d, side, kind);
   return hipGetLastError();
 }
+static hipError_t bc_all(const Dev& d, const int* kind, hipStream_t s) {
+  if (d.NX < d.G || d.NY < d.G) {
+    // a periodic source may be a ghost cell: the fills one after the other
+    for (int side = 0; side < 4; side++) {
+      const hipError_t e = bc(d, side, kind[side], s);
+      if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
+  }
+  const long long n = 2LL * d.G * d.ny + 2LL * d.G * d.NX;
+  hipLaunchKernelGGL(k_bc_all, dim3(blocks_for(n)), dim3(kBlock), 0, s, d, kind[0], kind[1], kind[2], kind[3]);
+  return hipGetLastError();
+}
 static hipError_t primvar(const Dev& d, const double* prim, int slot, hipStream_t s) {
   long long nb = blocks_for(d.nc);
   if (nb > kPartialBlocks) nb = kPartialBlocks;
@@ -2284,7 +2347,7 @@ static hipError_t batch_step(const BatchArgs& B, const BatchShape& sh, const uns
   return hipGetLastError();
 }
 
-const LaunchTable table = {bc,     primvar, copy_old,    copy_ghosts, normal,     tangent,      flux,
+const LaunchTable table = {bc,     bc_all,  primvar, copy_old,    copy_ghosts, normal,     tangent,      flux,
                           normal_list, flux_list, update, update_stat, alpha, flux_points, label, infer,
                           batch_step, linear_points};
 #endif

@@ -140,6 +140,9 @@ enum Phase {
 
 struct LaunchTable {
   hipError_t (*bc)(const Dev&, int side, int kind, hipStream_t);
+  // the four sides' fills (kind[4]: left, right, down, up) as bc one after
+  // the other gives them
+  hipError_t (*bc_all)(const Dev&, const int* kind, hipStream_t);
   hipError_t (*primvar)(const Dev&, const double* prim, int slot, hipStream_t);
   hipError_t (*copy_old)(const Dev&, hipStream_t);
   // the ghost frame of `from` into the state (U)
