@@ -318,7 +318,7 @@ int gks2d_set_cfl_primvar(gks2d_ctx* c, const double* base, int64_t cs, int64_t 
   std::vector<double> h(4 * c->d.nc);
   gather(h, base, c->d.nc, 4, cs, vs);
   HIPCHK(hipMemcpyAsync(c->d_prim, h.data(), h.size() * 8, hipMemcpyHostToDevice, c->s));
-  HIPCHK(c->lt->primvar(c->d, c->d_prim, 0, c->s));
+  HIPCHK(c->lt->primvar(c->d, c->d_prim, 0, 4, c->s));
   HIPCHK(hipStreamSynchronize(c->s));
   c->last_slot = 0;
   return GKS_OK;
@@ -336,7 +336,7 @@ int gks2d_boundary(gks2d_ctx* c, int side, int kind) {
 }
 int gks2d_primvar(gks2d_ctx* c) {
   if (!c) return fail(GKS_ERR_ARG, "null");
-  RC(timed(c, PH_PRIMVAR, [&] { return c->lt->primvar(c->d, nullptr, 0, c->s); }));
+  RC(timed(c, PH_PRIMVAR, [&] { return c->lt->primvar(c->d, nullptr, 0, 4, c->s); }));
   c->last_slot = 0;
   return GKS_OK;
 }
@@ -432,7 +432,7 @@ int gks2d_cfl(gks2d_ctx* c, double t, double tstop, double* dt_out) {
 static int stage_launch(gks2d_ctx* c, int stage, double dt, const int32_t* bc) {
   const int kinds[4] = {bc[0], bc[1], bc[2], bc[3]};
   RC(timed(c, PH_BC, [&] { return c->lt->bc_all(c->d, kinds, c->s); }));
-  RC(timed(c, PH_PRIMVAR, [&] { return c->lt->primvar(c->d, nullptr, stage, c->s); }));
+  RC(timed(c, PH_PRIMVAR, [&] { return c->lt->primvar(c->d, nullptr, stage, 4, c->s); }));
   RC(gks2d_reconstruct(c));
   RC(gks2d_flux(c, stage, dt));
   RC(gks2d_update(c, stage));
@@ -457,7 +457,7 @@ static int stage_launch(gks2d_ctx* c, int stage, double dt, const int32_t* bc) {
 static int run_stage(gks2d_ctx* c, int stage, double dt, const int32_t* bc, const Dev& upd, bool primvar) {
   const int kinds[4] = {bc[0], bc[1], bc[2], bc[3]};
   RC(timed(c, PH_BC, [&] { return c->lt->bc_all(c->d, kinds, c->s); }));
-  if (primvar) RC(timed(c, PH_PRIMVAR, [&] { return c->lt->primvar(c->d, nullptr, stage, c->s); }));
+  if (primvar) RC(timed(c, PH_PRIMVAR, [&] { return c->lt->primvar(c->d, nullptr, stage, 4, c->s); }));
   RC(gks2d_reconstruct(c));
   RC(gks2d_flux(c, stage, dt));
   RC(timed(c, PH_UPDATE, [&] { return c->lt->update_stat(upd, 1 - stage, c->s); }));
@@ -850,8 +850,6 @@ int batch_tables(gks2d_batch* b, int learned) {
       switch (k) {
         case KB_BC_LR: n = ceil_div((long long)d.G * d.ny, kSweepBlock); break;
         case KB_BC_DU: n = ceil_div((long long)d.G * d.nx, kSweepBlock); break;
-        case KB_PRIMVAR: n = std::min<long long>(ceil_div(d.nc, kSweepBlock), kSweepPartials); break;
-        case KB_COPY:
         case KB_UPDATE: n = ceil_div((long long)d.NX * d.NY, kSweepBlock); break;
         case KB_NORMAL_X: n = ceil_div((long long)(d.NX + 2) * d.ny, kSweepBlock); break;
         case KB_NORMAL_Y: n = ceil_div((long long)d.nx * (d.NY + 2), kSweepBlock); break;
@@ -908,6 +906,19 @@ int batch_init_state(gks2d_batch* b, const int32_t* bc, const double* tstop, int
     }
   }
   HIPCHK(hipMemcpyAsync(b->d_st, b->h_st, b->np * sizeof(BState), hipMemcpyHostToDevice, b->s));
+  // the initial states' blow-up test (in the batched step the updates test
+  // the states they write), after the boundary fill its ghost cells need
+  // (the first stage fills them the same way): raised into stat[5], which
+  // the first step's end takes, as that step's stage-0 primvar sweep would
+  // have; no speed slot is written (slot -1: the first dt reads the speed
+  // the problem's last primvar formed)
+  for (int p = 0; p < b->np; p++) {
+    const Dev& d = b->ctx[p]->d;
+    const int kinds[4] = {bc[4 * p], bc[4 * p + 1], bc[4 * p + 2], bc[4 * p + 3]};
+    HIPCHK(hipMemsetAsync(d.stat + 5, 0, 2 * sizeof(double), b->s));
+    HIPCHK(b->ctx[p]->lt->bc_all(d, kinds, b->s));
+    HIPCHK(b->ctx[p]->lt->primvar(d, nullptr, -1, 5, b->s));
+  }
   return GKS_OK;
 }
 
@@ -1038,6 +1049,7 @@ int gks2d_batch_run(gks2d_batch* b, const int32_t* bc, const double* tstop, int3
   int learned = 0;
   RC(batch_upload_devs(b, &learned));
   RC(batch_init_state(b, bc, tstop, max_steps, t, steps));
+  const std::vector<int32_t> steps0(steps, steps + b->np);
   batch_cover_all(b);
   RC(batch_graph(b, learned));
   // every replay advances each unfinished problem by one step (or ends it),
@@ -1070,6 +1082,9 @@ int gks2d_batch_run(gks2d_batch* b, const int32_t* bc, const double* tstop, int3
     t[p] = st.t;
     steps[p] = st.steps;
     gks2d_ctx* c = b->ctx[p];
+    // a problem that ran a step: its last stage-1 ghost fill, into the state
+    // (batch_stage: stage 1 filled the second buffer's ghost frame)
+    if (st.steps > steps0[p] || st.blowup) HIPCHK(c->lt->copy_ghosts(c->d, c->d.Uold, b->s));
     c->last_slot = st.slot;
     if (st.steps > 0) c->last_flux_stage = 1;
     if (st.blowup) {
@@ -1078,6 +1093,8 @@ int gks2d_batch_run(gks2d_batch* b, const int32_t* bc, const double* tstop, int3
       bad += (bad.empty() ? "" : ", ") + std::to_string(p);
     }
   }
+  // the ghost copies above ran on the batch's stream
+  HIPCHK(hipStreamSynchronize(b->s));
   if (!bad.empty()) return fail(GKS_ERR_BLOWUP, ("density or pressure negative or NaN in problem(s) " + bad).c_str());
   return GKS_OK;
 }

@@ -195,7 +195,7 @@ __global__ void __launch_bounds__(kBlock) k_primvar(Dev d, const double* prim, d
 }
 // final max / or over the partials into stat[slot*2 .. slot*2+1]; the flag
 // accumulates (cleared by the host when it reads it)
-__device__ __forceinline__ void reduce_body(const double* partial, int n, double* stat, int slot) {
+__device__ __forceinline__ void reduce_body(const double* partial, int n, double* stat, int slot, int flag = 4) {
   __shared__ double smax[kBlock];
   __shared__ double sbad[kBlock];
   // eight partials per thread and round, their loads issued together (the
@@ -241,12 +241,12 @@ __device__ __forceinline__ void reduce_body(const double* partial, int n, double
     __syncthreads();
   }
   if (threadIdx.x == 0) {
-    stat[2 * slot] = smax[0];
-    if (sbad[0] != 0.0) stat[4] = 1.0;  // sticky blow-up flag
+    if (slot >= 0) stat[2 * slot] = smax[0];  // (slot < 0: the blow-up test alone)
+    if (sbad[0] != 0.0) stat[flag] = 1.0;     // sticky blow-up flag
   }
 }
-__global__ void __launch_bounds__(kBlock) k_reduce(const double* partial, int n, double* stat, int slot) {
-  reduce_body(partial, n, stat, slot);
+__global__ void __launch_bounds__(kBlock) k_reduce(const double* partial, int n, double* stat, int slot, int flag) {
+  reduce_body(partial, n, stat, slot, flag);
 }
 
 // CopyFluid_new_to_old (basic_function.cpp:1571-1584), interior cells
@@ -1765,10 +1765,10 @@ __global__ void __launch_bounds__(kBlock) k_update(Dev d) { update_body(d, block
 // the block's partial pair (as k_primvar: its test also covers the ghost
 // cells, which the boundary fill makes copies or mirror images of interior
 // cells, so it finds what this one finds)
-__global__ void __launch_bounds__(kBlock) k_update_stat(Dev d, double* partial) {
+__device__ __forceinline__ void update_stat_body(const Dev& d, unsigned blk, double* partial) {
   __shared__ double smax[kBlock];
   __shared__ int sbad[kBlock];
-  const long long t = (long long)blockIdx.x * kBlock + threadIdx.x;
+  const long long t = (long long)blk * kBlock + threadIdx.x;
   double tmax = 0.0;
   int bad = 0;
   if (t < (long long)d.NX * d.NY) {
@@ -1791,9 +1791,12 @@ __global__ void __launch_bounds__(kBlock) k_update_stat(Dev d, double* partial) 
     __syncthreads();
   }
   if (threadIdx.x == 0) {
-    partial[2 * blockIdx.x] = smax[0];
-    partial[2 * blockIdx.x + 1] = sbad[0] ? 1.0 : 0.0;
+    partial[2 * blk] = smax[0];
+    partial[2 * blk + 1] = sbad[0] ? 1.0 : 0.0;
   }
+}
+__global__ void __launch_bounds__(kBlock) k_update_stat(Dev d, double* partial) {
+  update_stat_body(d, blockIdx.x, partial);
 }
 
 // Update_alpha (reconstruction.cpp:1466-1534): the reference bounds j by
@@ -1884,12 +1887,23 @@ __device__ __forceinline__ int batch_slot(const BatchArgs& B, int kind, unsigned
 // The problem's descriptor is copied out of global memory once: read
 // through a reference, every field would be reloaded after each store the
 // body makes (the compiler cannot rule out that a store aliases it).
+// problem p's descriptor as a launch of B sees it (stage 1: U and Uold
+// exchanged)
+__device__ __forceinline__ Dev batch_dev(const BatchArgs& B, int p) {
+  Dev d = B.dev[p];
+  if (B.swap) {
+    double* u = d.U;
+    d.U = d.Uold;
+    d.Uold = u;
+  }
+  return d;
+}
 #define GKS_BATCH_TILE(kind)                                    \
   const unsigned gb_ = xcd_block();                             \
   const int q_ = batch_slot(B, kind, gb_);                      \
   const int p_ = B.pid[q_];                                     \
   if (!B.st[p_].active) return;                                 \
-  const Dev d = B.dev[p_];                                      \
+  const Dev d = batch_dev(B, p_);                               \
   const unsigned blk = gb_ - B.start[(kind) * (B.np + 1) + q_]; \
   (void)d;                                                      \
   (void)blk
@@ -1898,21 +1912,12 @@ __global__ void __launch_bounds__(kBlock) kb_bc(BatchArgs B, int side) {
   GKS_BATCH_TILE(side < 2 ? KB_BC_LR : KB_BC_DU);
   bc_body(d, blk, side, B.st[p_].bc[side]);
 }
-__global__ void __launch_bounds__(kBlock) kb_primvar(BatchArgs B) {
-  GKS_BATCH_TILE(KB_PRIMVAR);
-  const unsigned* st = B.start + KB_PRIMVAR * (B.np + 1);
-  primvar_body(d, blk, st[q_ + 1] - st[q_], nullptr, d.partial);
-}
-// one workgroup per problem
-__global__ void __launch_bounds__(kBlock) kb_reduce(BatchArgs B, int slot) {
+// one workgroup per problem: the partials of its `kind` tiles
+__global__ void __launch_bounds__(kBlock) kb_reduce(BatchArgs B, int slot, int kind, int flag) {
   const int q = blockIdx.x, p = B.pid[q];
   if (!B.st[p].active) return;
-  const unsigned* st = B.start + KB_PRIMVAR * (B.np + 1);
-  reduce_body(B.dev[p].partial, (int)(st[q + 1] - st[q]), B.dev[p].stat, slot);
-}
-__global__ void __launch_bounds__(kBlock) kb_copy_old(BatchArgs B) {
-  GKS_BATCH_TILE(KB_COPY);
-  copy_old_body(d, blk);
+  const unsigned* st = B.start + kind * (B.np + 1);
+  reduce_body(B.dev[p].partial, (int)(st[q + 1] - st[q]), B.dev[p].stat, slot, flag);
 }
 template <int DIR, bool LEARNED>
 __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(GKS_NORMAL_WAVES))) kb_normal(BatchArgs B) {
@@ -1966,7 +1971,7 @@ __global__ void __launch_bounds__(64) kb_list_prefix(BatchArgs B, int kind, int 
     }                                                       \
   }                                                         \
   const int p_ = B.pid[q_];                                 \
-  const Dev d = B.dev[p_];                                  \
+  const Dev d = batch_dev(B, p_);                           \
   const unsigned blk = gb_ - ls_[q_]
 // the cells each problem's learned normal sweep listed (as k_normal_list)
 template <int DIR>
@@ -2001,9 +2006,15 @@ kb_tanflux_list(BatchArgs B, int stage) {
   if (list_tile(d.lpre + (KL_FLUX_X + DIR) * kListRecord, blk, s, tb, n))
     tanflux_body<DIR, true, TF, true>(d, tb, stage, B.st[p_].dt, d.tlist[DIR] + (size_t)s * d.tcap[DIR], n);
 }
-__global__ void __launch_bounds__(kBlock) kb_update(BatchArgs B) {
+// the update of stage 0 (B.swap 0) writes the second buffer from the
+// state, stage 1's (B.swap 1) updates the state in place; with the new
+// state's blow-up test and CFL speed into the problem's partials
+__global__ void __launch_bounds__(kBlock) kb_update_stat(BatchArgs B) {
   GKS_BATCH_TILE(KB_UPDATE);
-  update_body(d, blk);
+  Dev du = d;
+  du.U = d.Uold;
+  du.Uold = B.swap ? d.Uold : d.U;
+  update_stat_body(du, blk, d.partial);
 }
 __global__ void __launch_bounds__(kBlock) kb_alpha(BatchArgs B) {
   GKS_BATCH_TILE(KB_ALPHA);
@@ -2022,6 +2033,11 @@ __global__ void __launch_bounds__(64) kb_step_begin(BatchArgs B) {
     const Dev& d = B.dev[p];
     st.active = 0;
     if (st.done) continue;
+    // the last stage-1 update's blow-up flag (or the initial state's) is
+    // taken at this step's end, as the stage-0 primvar sweep of this step
+    // would have raised it
+    d.stat[6] = d.stat[5];
+    d.stat[5] = 0.0;
     if (d.stat[4] != 0.0) {
       st.blowup = 1;
       st.done = 1;
@@ -2052,7 +2068,7 @@ __global__ void __launch_bounds__(64) kb_step_end(BatchArgs B) {
     const int p = B.pid[q];
     BState& st = B.st[p];
     if (!st.active) continue;
-    if (B.dev[p].stat[4] != 0.0) {
+    if (B.dev[p].stat[4] != 0.0 || B.dev[p].stat[6] != 0.0) {
       st.blowup = 1;
       st.done = 1;
       st.active = 0;
@@ -2143,11 +2159,11 @@ static hipError_t bc_all(const Dev& d, const int* kind, hipStream_t s) {
   hipLaunchKernelGGL(k_bc_all, dim3(blocks_for(n)), dim3(kBlock), 0, s, d, kind[0], kind[1], kind[2], kind[3]);
   return hipGetLastError();
 }
-static hipError_t primvar(const Dev& d, const double* prim, int slot, hipStream_t s) {
+static hipError_t primvar(const Dev& d, const double* prim, int slot, int flag, hipStream_t s) {
   long long nb = blocks_for(d.nc);
   if (nb > kPartialBlocks) nb = kPartialBlocks;
   hipLaunchKernelGGL(k_primvar, dim3((unsigned)nb), dim3(kBlock), 0, s, d, prim, d.partial);
-  hipLaunchKernelGGL(k_reduce, dim3(1), dim3(kBlock), 0, s, (const double*)d.partial, (int)nb, d.stat, slot);
+  hipLaunchKernelGGL(k_reduce, dim3(1), dim3(kBlock), 0, s, (const double*)d.partial, (int)nb, d.stat, slot, flag);
   return hipGetLastError();
 }
 static hipError_t copy_old(const Dev& d, hipStream_t s) {
@@ -2260,7 +2276,7 @@ static hipError_t update(const Dev& d, int stage, hipStream_t s) {
 static hipError_t update_stat(const Dev& d, int slot, hipStream_t s) {
   const unsigned nb = blocks_for((long long)d.NX * d.NY);
   hipLaunchKernelGGL(k_update_stat, dim3(nb), dim3(kBlock), 0, s, d, d.partial);
-  hipLaunchKernelGGL(k_reduce, dim3(1), dim3(kBlock), 0, s, (const double*)d.partial, (int)nb, d.stat, slot);
+  hipLaunchKernelGGL(k_reduce, dim3(1), dim3(kBlock), 0, s, (const double*)d.partial, (int)nb, d.stat, slot, 4);
   return hipGetLastError();
 }
 static hipError_t alpha(const Dev& d, hipStream_t s) {
@@ -2292,16 +2308,23 @@ static hipError_t infer(const float* P, const unsigned short* W, long long n, co
   return hipGetLastError();
 }
 
+// One stage of the batched step.  As gks2d_run's step: stage 0 writes its
+// state to the second buffer, stage 1 reconstructs that buffer (B.swap) and
+// updates the state in place (no CopyFluid_new_to_old); each update forms
+// the blow-up test and CFL speed of the state it writes (no primvar sweep):
+// stage 0's into slot 1 (the next step's CFL speed) with its verdict taken
+// at this step's end, stage 1's into slot 0 with its verdict taken at the
+// next step's end (stat[5], kb_step_begin).
 template <bool LEARNED, int TF>
-static void batch_stage(const BatchArgs& B, const unsigned* tiles, int stage, hipStream_t s) {
+static void batch_stage(const BatchArgs& B0, const unsigned* tiles, int stage, hipStream_t s) {
+  BatchArgs B = B0;
+  B.swap = stage;
   // the list launches' bound: each problem's shards add at most one partial
   // tile each
   const unsigned lb = (unsigned)kListShards * (unsigned)B.np;
   (void)lb;
   for (int side = 0; side < 4; side++)
     hipLaunchKernelGGL(kb_bc, dim3(tiles[side < 2 ? KB_BC_LR : KB_BC_DU]), dim3(kBlock), 0, s, B, side);
-  hipLaunchKernelGGL(kb_primvar, dim3(tiles[KB_PRIMVAR]), dim3(kBlock), 0, s, B);
-  hipLaunchKernelGGL(kb_reduce, dim3(B.np), dim3(kBlock), 0, s, B, stage);
   hipLaunchKernelGGL((kb_normal<0, LEARNED>), dim3(tiles[KB_NORMAL_X]), dim3(kBlock), 0, s, B);
   if (LEARNED) {
     hipLaunchKernelGGL(kb_list_prefix, dim3(1), dim3(64), 0, s, B, (int)KL_NORMAL_X, kBlock);
@@ -2323,13 +2346,13 @@ static void batch_stage(const BatchArgs& B, const unsigned* tiles, int stage, hi
     hipLaunchKernelGGL((kb_tanflux<0, LEARNED, TF>), dim3(tiles[KB_FLUX_X]), dim3(2 * TF), 0, s, B, stage);
     hipLaunchKernelGGL((kb_tanflux<1, LEARNED, TF>), dim3(tiles[KB_FLUX_Y]), dim3(2 * TF), 0, s, B, stage);
   }
-  hipLaunchKernelGGL(kb_update, dim3(tiles[KB_UPDATE]), dim3(kBlock), 0, s, B);
+  hipLaunchKernelGGL(kb_update_stat, dim3(tiles[KB_UPDATE]), dim3(kBlock), 0, s, B);
+  hipLaunchKernelGGL(kb_reduce, dim3(B.np), dim3(kBlock), 0, s, B, 1 - stage, (int)KB_UPDATE, stage == 0 ? 4 : 5);
   hipLaunchKernelGGL(kb_alpha, dim3(tiles[KB_ALPHA]), dim3(kBlock), 0, s, B);
 }
 template <bool LEARNED, int TF>
 static void batch_step_t(const BatchArgs& B, const unsigned* tiles, hipStream_t s) {
   hipLaunchKernelGGL(kb_step_begin, dim3(1), dim3(64), 0, s, B);
-  hipLaunchKernelGGL(kb_copy_old, dim3(tiles[KB_COPY]), dim3(kBlock), 0, s, B);
   batch_stage<LEARNED, TF>(B, tiles, 0, s);
   batch_stage<LEARNED, TF>(B, tiles, 1, s);
   hipLaunchKernelGGL(kb_step_end, dim3(1), dim3(64), 0, s, B);

@@ -60,7 +60,9 @@ struct Dev {
                                 // the sweep that fills the list and the list's launch, which clears the shard
                                 // counts for the list's next sweep)
   double* partial;  // per-block (max speed, blow-up flag) pairs: max(kSweepPartials, update blocks) of them
-  double* stat;     // [0..3]: max speed of slot 0 / 1 (+pad); [4]: sticky blow-up flag
+  double* stat;     // [0..3]: max speed of slot 0 / 1 (+pad); [4]: sticky blow-up flag; batched steps:
+                    // [5] the blow-up flag of the last stage-1 update (or of the initial state), [6] that
+                    // flag of the step before, which the current step's end takes
 };
 
 // workgroup size of the sweep kernels and the number of per-block partials
@@ -106,8 +108,7 @@ struct BState {
 };
 // work classes of a batched launch; each has its own tile prefix table
 enum BatchKind {
-  KB_BC_LR = 0, KB_BC_DU, KB_PRIMVAR, KB_COPY, KB_NORMAL_X, KB_NORMAL_Y, KB_FLUX_X, KB_FLUX_Y, KB_UPDATE,
-  KB_ALPHA, KB_COUNT
+  KB_BC_LR = 0, KB_BC_DU, KB_NORMAL_X, KB_NORMAL_Y, KB_FLUX_X, KB_FLUX_Y, KB_UPDATE, KB_ALPHA, KB_COUNT
 };
 // start[k * (np + 1) + q]: first tile of the q-th problem in launch order for
 // work class k (start[.. + np] = tiles in the launch); pid[q]: that problem.
@@ -123,6 +124,9 @@ struct BatchArgs {
   // normal y, flux x, flux y), in launch order, filled on the device after
   // the sweep that builds each list
   unsigned* lstart;
+  // stage-1 launches of the batched step: every problem's U and Uold
+  // exchanged (stage 0 writes its state to the second buffer)
+  int swap;
 };
 enum BatchList { KL_NORMAL_X = 0, KL_NORMAL_Y, KL_FLUX_X, KL_FLUX_Y, KL_COUNT };
 // tile shapes of a batch (chosen by gks2d_batch_autotune)
@@ -143,7 +147,9 @@ struct LaunchTable {
   // the four sides' fills (kind[4]: left, right, down, up) as bc one after
   // the other gives them
   hipError_t (*bc_all)(const Dev&, const int* kind, hipStream_t);
-  hipError_t (*primvar)(const Dev&, const double* prim, int slot, hipStream_t);
+  // Convar_to_Primvar's blow-up test (flag: the stat word it raises, 4) and
+  // the CFL speed into stat slot `slot` (none for slot < 0)
+  hipError_t (*primvar)(const Dev&, const double* prim, int slot, int flag, hipStream_t);
   hipError_t (*copy_old)(const Dev&, hipStream_t);
   // the ghost frame of `from` into the state (U)
   hipError_t (*copy_ghosts)(const Dev&, const double* from, hipStream_t);
