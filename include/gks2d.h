@@ -123,7 +123,12 @@ int gks2d_update_alpha(gks2d_ctx* ctx);
 int gks2d_step(gks2d_ctx* ctx, double dt, const int32_t bc[4]);
 /* The driver's time loop: while (t < tstop && steps < max_steps) {dt = cfl;
  * step}.  Keeps the device busy by reading dt back asynchronously.
- * t_inout / step_inout carry block.t / block.step. */
+ * t_inout / step_inout carry block.t / block.step.  The states, dt and
+ * step counts equal those of gks2d_step in the loop; its steps launch no
+ * copy_old (stage 0 writes its state to the second buffer, which afterwards
+ * holds that state rather than a copy of the step's initial one) and form
+ * Convar_to_Primvar's blow-up test and CFL speed in the updates (the
+ * primvar sweep runs at the run's first stage only). */
 int gks2d_run(gks2d_ctx* ctx, const int32_t bc[4], double tstop, int32_t max_steps,
               double* t_inout, int32_t* step_inout);
 
