@@ -68,6 +68,16 @@ __device__ __forceinline__ long long face3(const Dev3& d, int i, int j, int k) {
   return ((long long)i * (d.ny + 1) + j) * (d.nz + 1) + k;
 }
 
+// Workgroups go to the eight XCDs round robin, each with its own L2; the
+// tile of workgroup b is the (b % 8)-th contiguous eighth's (b / 8)-th, so
+// that tiles next to each other in the sweep's order run on one XCD at
+// about the same time (a stencil's neighbouring rows then come through one
+// L2).
+__device__ __forceinline__ unsigned xcd_tile() {
+  const unsigned nb = gridDim.x, b = blockIdx.x, q = nb / 8, r = nb % 8, x = b % 8;
+  return x * q + (x < r ? x : r) + b / 8;
+}
+
 // ------------------------------------------------------------ boundaries
 // free_boundary_* (boundary_condition.cpp:445-570) in the reference's side
 // order x, y, z compose into one map per ghost cell: each coordinate outside
@@ -291,7 +301,7 @@ __device__ __forceinline__ void tangential_side_linear(double (*out)[NC + 5], co
 // weights (the fifth-order candidate, on the conservative values).
 template <int DIR, bool LEARNED>
 __global__ void __launch_bounds__(kBlock3) k3_face(Dev3 d, Box3 b) {
-  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long t = (long long)xcd_tile() * blockDim.x + threadIdx.x;
   if (t >= b.count()) return;
   int i, j, k;
   b.at(t, i, j, k);
@@ -464,6 +474,19 @@ struct LvStencil {
   }
 };
 
+// tiles of k3_line<DIR>, kLineFaces faces each: x and z faces, whose line
+// stencil runs along j (nz + 1 faces apart), consecutive faces of the box
+// (k fastest), consecutive tiles on one XCD so that a stencil's rows come
+// through one L2; y faces, whose stencil runs along i ((ny + 1)(nz + 1)
+// faces apart), kLineFaces consecutive faces of one (j, k) plane with i
+// fastest from tile to tile
+constexpr int kLineFaces = kBlock3 / 2;
+__host__ __device__ inline long long line_tiles(const Box3& b, int dir) {
+  const long long plane = (long long)b.n[1] * b.n[2];
+  return dir == 1 ? (long long)b.n[0] * ((plane + kLineFaces - 1) / kLineFaces)
+                  : ((long long)b.n[0] * plane + kLineFaces - 1) / kLineFaces;
+}
+
 // WENO5_AO_of_line_value (reconstruction.cpp:3925-3953): x faces along y
 // (h = dy), y faces along -x (h = dx, the stencil reversed), z faces along
 // y (h = dy).  One thread per (face, side); the two sides of a face are
@@ -472,10 +495,19 @@ struct LvStencil {
 // any of the four reconstructed states has lambda <= 0.
 template <int DIR, bool LEARNED>
 __global__ void __launch_bounds__(kBlock3) k3_line(Dev3 d, Box3 b) {
-  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  const long long fi = t >> 1;
-  const int side = (int)(t & 1);
-  if (fi >= b.count()) return;  // both lanes of a pair leave together
+  const int fl = threadIdx.x >> 1, side = threadIdx.x & 1;
+  const unsigned tile = xcd_tile();
+  long long fi;  // the face's index in the box (k fastest)
+  if (DIR == 1) {
+    const long long plane = (long long)b.n[1] * b.n[2];
+    const unsigned ti = tile % (unsigned)b.n[0], tp = tile / (unsigned)b.n[0];
+    const long long p = (long long)tp * kLineFaces + fl;
+    if (p >= plane) return;  // both lanes of a pair leave together
+    fi = (long long)ti * plane + p;
+  } else {
+    fi = (long long)tile * kLineFaces + fl;
+    if (fi >= b.count()) return;
+  }
   int i, j, k;
   b.at(fi, i, j, k);
   const long long f = face3(d, i, j, k);
@@ -530,14 +562,6 @@ struct LdsSide {
   const double* p;
   __device__ __forceinline__ double operator[](int k) const { return p[k]; }
 };
-// Workgroups go to the eight XCDs round robin, each with its own L2; the
-// tile of workgroup b is the (b % 8)-th contiguous eighth's (b / 8)-th, so
-// that tiles next to each other in the order below run on one XCD at about
-// the same time.
-__device__ __forceinline__ unsigned xcd_tile() {
-  const unsigned nb = gridDim.x, b = blockIdx.x, q = nb / 8, r = nb % 8, x = b % 8;
-  return x * q + (x < r ? x : r) + b / 8;
-}
 // tiles of k3_pflux<DIR>: x and y faces, kPfFaces consecutive faces of the
 // box (k fastest: the point stencil runs along k, inside the tile); z faces,
 // whose point stencil runs along i, (ny + 1)(nz + 1) faces apart, kPfFaces
@@ -782,7 +806,7 @@ static hipError_t launch_face(const Dev3& d, int dir, hipStream_t s) {
 static hipError_t launch_line(const Dev3& d, int dir, hipStream_t s) {
   const Box3 b = line_box(d, dir);
   if (box_count(b) == 0) return hipSuccess;
-  const unsigned nb = blocks_for(2 * box_count(b));
+  const unsigned nb = (unsigned)line_tiles(b, dir);
   if (d.mlp) {
     if (dir == 0) k3_line<0, true><<<nb, kBlock3, 0, s>>>(d, b);
     else if (dir == 1) k3_line<1, true><<<nb, kBlock3, 0, s>>>(d, b);
