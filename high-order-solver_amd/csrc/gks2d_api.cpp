@@ -667,10 +667,10 @@ int gks2d_classifier_samples(gks2d_ctx* c, int32_t dir, int32_t stride, double l
 
 // learned-path work lists (each in kListShards shards): the shard counts
 // of the flux lists x, y and of the normal lists x, y (one 128-byte line
-// each), the four lists' tile prefixes (32 words each), then the flux
-// faces x, y and the normal-sweep cells x, y.  A
-// shard takes the appends of a run of at most ceil(tiles / kListShards)
-// tiles of the sweep that fills the list (flux tiles: 64 or 128 faces).
+// each), the four lists' prefix records (kListRecord words each), then the
+// flux faces x, y and the normal-sweep cells x, y.  A shard takes the
+// appends of a run of at most ceil(tiles / kListShards) tiles of the sweep
+// that fills the list (flux tiles: 64 or 128 faces).
 // entries one shard can receive from a sweep of `tiles` tiles of per_tile
 // items (a shard takes a run of at most ceil(tiles / kListShards) tiles)
 static unsigned shard_cap_tiles(long long tiles, int per_tile) {
@@ -682,7 +682,10 @@ static unsigned shard_cap(long long items, int per_tile) {
 static void list_caps(Dev& d) {
   const long long nf[2] = {(long long)(d.NX + 1) * d.NY, (long long)d.NX * (d.NY + 1)};
   const long long ncl[2] = {(long long)(d.NX + 2) * d.ny, (long long)d.nx * (d.NY + 2)};
-  for (int dir = 0; dir < 2; dir++) d.ncap[dir] = shard_cap(ncl[dir], kSweepBlock);
+  // the normal lists: filled by one sweep per direction, or by both
+  // directions' classifier launch in tiles of kSweepBlock cells of the frame
+  for (int dir = 0; dir < 2; dir++)
+    d.ncap[dir] = std::max(shard_cap(ncl[dir], kSweepBlock), shard_cap_tiles(cls_tiles(d), kSweepBlock));
   // the flux lists are filled by sweeps of 64 or 128 faces per tile, in
   // 2D tiles (cheap_tiles); runs along the rows for the single learned
   // launch
@@ -851,7 +854,8 @@ int batch_tables(gks2d_batch* b, int learned) {
         case KB_BC_LR: n = ceil_div((long long)d.G * d.ny, kSweepBlock); break;
         case KB_BC_DU: n = ceil_div((long long)d.G * d.nx, kSweepBlock); break;
         case KB_UPDATE: n = ceil_div((long long)d.NX * d.NY, kSweepBlock); break;
-        case KB_NORMAL_X: n = ceil_div((long long)(d.NX + 2) * d.ny, kSweepBlock); break;
+        // the learned path's classifier launch takes both directions
+        case KB_NORMAL_X: n = learned ? cls_tiles(d) : ceil_div((long long)(d.NX + 2) * d.ny, kSweepBlock); break;
         case KB_NORMAL_Y: n = ceil_div((long long)d.nx * (d.NY + 2), kSweepBlock); break;
         // the learned path's smooth-face launch takes 2D tiles
         case KB_FLUX_X: n = learned ? cheap_tiles(d, 0, tf) : ceil_div((long long)(d.NX + 1) * d.NY, tf); break;

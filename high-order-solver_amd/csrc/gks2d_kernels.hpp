@@ -706,6 +706,97 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(GKS
 k_normal_learned(Dev d) {
   normal_learned_body<DIR>(d, xcd_block(), gridDim.x);
 }
+// Both classifier sweeps in one launch, for a learned path whose smooth-face
+// flux launch forms every linear line value (so the sweeps store none): a
+// workgroup takes kClsRows x kClsCols cells of the whole frame (ghost cells
+// included), forms rho and p of each cell its stencils reach once, into LDS
+// (in FP32, as the features take them; p once in each sweep's x-like frame,
+// whose operation order differs), and each thread runs the network on its
+// cell's x stencil and on its y stencil where the cell is in that sweep's
+// range (normal_cell), setting the flag and listing the cell as
+// normal_learned_body does.  A stencil cell outside the frame only feeds a
+// cell outside both ranges.
+__device__ __forceinline__ void classify_body(const Dev& d, unsigned blk, unsigned ntiles) {
+  constexpr int SR = kClsRows + 4, SC = kClsCols + 4, NS = SR * SC, NU = (NS + kBlock - 1) / kBlock;
+  static_assert(kClsRows * kClsCols == kBlock, "a thread per cell");
+  __shared__ __align__(16) short xs[kBlock * 16];
+  __shared__ float ls[kBlock];
+  __shared__ MlpLds s_mlp;
+  __shared__ float srho[NS], spx[NS], spy[NS];
+  __shared__ unsigned char sz[NS];  // bit 0: alpha_x not above zero, bit 1: alpha_y
+  mlp_stage(s_mlp, d.mlp, d.mlpw);
+  const long long tcols = cls_tile_cols(d);
+  const int i0 = (int)(blk / tcols) * kClsRows, j0 = (int)(blk % tcols) * kClsCols;
+  {
+    // staged cell e: (i0 - 2 + e / SC, j0 - 2 + e % SC); every load is
+    // issued before the first conversion
+    double u[NU][4], ax[NU], ay[NU];
+#pragma unroll
+    for (int r = 0; r < NU; r++) {
+      const int e = threadIdx.x + r * kBlock, gi = i0 - 2 + e / SC, gj = j0 - 2 + e % SC;
+      if (e < NS && gi >= 0 && gi < d.nx && gj >= 0 && gj < d.ny) {
+        const long long c = (long long)gi * d.ny + gj;
+#pragma unroll
+        for (int v = 0; v < 4; v++) u[r][v] = d.U[v * d.nc + c];
+        ax[r] = d.ax[c];
+        ay[r] = d.ay[c];
+      } else {
+        u[r][0] = 1.0;
+        u[r][1] = u[r][2] = 0.0;
+        u[r][3] = 1.0;
+        ax[r] = ay[r] = 0.0;
+      }
+    }
+#pragma unroll
+    for (int r = 0; r < NU; r++) {
+      const int e = threadIdx.x + r * kBlock;
+      if (e < NS) {
+        double px[4], py[4];
+        convar_to_primvar(px, u[r], d.gas);
+        const double wy[4] = {u[r][0], u[r][2], -u[r][1], u[r][3]};  // YchangetoX
+        convar_to_primvar(py, wy, d.gas);
+        srho[e] = (float)u[r][0];
+        spx[e] = (float)px[3];
+        spy[e] = (float)py[3];
+        sz[e] = (unsigned char)((ax[r] > 0.0 ? 0 : 1) | (ay[r] > 0.0 ? 0 : 2));
+      }
+    }
+  }
+  __syncthreads();
+  const int ti = threadIdx.x / kClsCols, tj = threadIdx.x % kClsCols, i = i0 + ti, j = j0 + tj;
+  const bool inside = i < d.nx && j < d.ny;
+  const bool vx = inside && i >= d.G - 1 && i <= d.NX + d.G, vy = inside && j >= d.G - 1 && j <= d.NY + d.G;
+  const int wv = threadIdx.x >> 6, sh = list_shard(blk, ntiles);
+#pragma unroll
+  for (int dir = 0; dir < 2; dir++) {
+    const bool valid = dir == 0 ? vx : vy;
+    float x[16];
+    unsigned zero = 1u << dir;
+    if (valid) {
+      float q[2][5];
+#pragma unroll
+      for (int s = 0; s < 5; s++) {
+        const int e = dir == 0 ? (ti + s) * SC + tj + 2 : (ti + 2) * SC + tj + s;
+        q[0][s] = srho[e];
+        q[1][s] = dir == 0 ? spx[e] : spy[e];
+        zero &= sz[e];
+      }
+      stencil_features_f(x, q);
+    } else {
+#pragma unroll
+      for (int k = 0; k < 16; k++) x[k] = 0.0f;
+    }
+    const float z = mlp_wave_logit(x, MlpShared{&s_mlp}, xs + wv * 64 * 16, ls + wv * 64);
+    if (valid) {
+      const bool lin = z < d.mlp_thr && zero != 0;
+      count_linear(d.lstat, lin);
+      d.lin[dir][(long long)i * d.ny + j] = lin ? 1 : 0;
+      const long long t = dir == 0 ? (long long)(i - d.G + 1) * d.ny + j : (long long)i * (d.NY + 2) + j - d.G + 1;
+      list_append(shard_count(d.ncount, dir, sh), d.nlist[dir] + (size_t)sh * d.ncap[dir], !lin, (unsigned)t);
+    }
+  }
+}
+__global__ void __launch_bounds__(kBlock) k_classify(Dev d) { classify_body(d, xcd_block(), gridDim.x); }
 // WENO5-AO-DF over the cells k_normal_learned listed, one thread each, a
 // list tile per workgroup; launched with the normal sweep's block count +
 // kListShards (a bound on the list's tiles, whose number is known only on
@@ -1919,15 +2010,18 @@ __global__ void __launch_bounds__(kBlock) kb_reduce(BatchArgs B, int slot, int k
   const unsigned* st = B.start + kind * (B.np + 1);
   reduce_body(B.dev[p].partial, (int)(st[q + 1] - st[q]), B.dev[p].stat, slot, flag);
 }
-template <int DIR, bool LEARNED>
+template <int DIR>
 __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(GKS_NORMAL_WAVES))) kb_normal(BatchArgs B) {
   GKS_BATCH_TILE(DIR == 0 ? KB_NORMAL_X : KB_NORMAL_Y);
-  if constexpr (LEARNED) {
-    const unsigned* st = B.start + (DIR == 0 ? KB_NORMAL_X : KB_NORMAL_Y) * (B.np + 1);
-    normal_learned_body<DIR>(d, blk, st[q_ + 1] - st[q_]);
-  } else {
-    normal_body<DIR>(d, blk);
-  }
+  normal_body<DIR>(d, blk);
+}
+// the learned batch: both directions' classifiers of every problem (as
+// k_classify; the batch forms every linear line value in the smooth-face
+// flux launch)
+__global__ void __launch_bounds__(kBlock) kb_classify(BatchArgs B) {
+  GKS_BATCH_TILE(KB_NORMAL_X);
+  const unsigned* st = B.start + KB_NORMAL_X * (B.np + 1);
+  classify_body(d, blk, st[q_ + 1] - st[q_]);
 }
 // Learned path, work lists: after the sweep that fills every problem's
 // list of one kind, kb_list_prefix forms each problem's tile prefix of
@@ -2102,10 +2196,8 @@ template __global__ void k_tanflux_cheap<0>(Dev, int, double);
 template __global__ void k_tanflux_cheap<1>(Dev, int, double);
 template __global__ void k_tanflux_list<0>(Dev, int, double);
 template __global__ void k_tanflux_list<1>(Dev, int, double);
-template __global__ void kb_normal<0, false>(BatchArgs);
-template __global__ void kb_normal<1, false>(BatchArgs);
-template __global__ void kb_normal<0, true>(BatchArgs);
-template __global__ void kb_normal<1, true>(BatchArgs);
+template __global__ void kb_normal<0>(BatchArgs);
+template __global__ void kb_normal<1>(BatchArgs);
 template __global__ void kb_tanflux<0, false, 64>(BatchArgs, int);
 template __global__ void kb_tanflux<1, false, 64>(BatchArgs, int);
 template __global__ void kb_tanflux<0, true, 64>(BatchArgs, int);
@@ -2182,6 +2274,9 @@ static hipError_t normal(const Dev& d, int dir, hipStream_t s) {
   } else if (d.order == 9) {
     if (dir == 0) hipLaunchKernelGGL((k_normal79<0, 9>), dim3(nb), dim3(kBlock), 0, s, d);
     else hipLaunchKernelGGL((k_normal79<1, 9>), dim3(nb), dim3(kBlock), 0, s, d);
+  } else if (d.mlp && d.fused_lines == kFusedLineDirs) {
+    // both sweeps' classifiers in the x call
+    if (dir == 0) hipLaunchKernelGGL(k_classify, dim3((unsigned)cls_tiles(d)), dim3(kBlock), 0, s, d);
   } else if (d.mlp) {
     // the listed cells follow in normal_list (whose prefix kernel leaves the
     // shard counts cleared for the next sweep)
@@ -2325,12 +2420,15 @@ static void batch_stage(const BatchArgs& B0, const unsigned* tiles, int stage, h
   (void)lb;
   for (int side = 0; side < 4; side++)
     hipLaunchKernelGGL(kb_bc, dim3(tiles[side < 2 ? KB_BC_LR : KB_BC_DU]), dim3(kBlock), 0, s, B, side);
-  hipLaunchKernelGGL((kb_normal<0, LEARNED>), dim3(tiles[KB_NORMAL_X]), dim3(kBlock), 0, s, B);
   if (LEARNED) {
+    // both directions' classifiers (KB_NORMAL_X counts their tiles)
+    hipLaunchKernelGGL(kb_classify, dim3(tiles[KB_NORMAL_X]), dim3(kBlock), 0, s, B);
     hipLaunchKernelGGL(kb_list_prefix, dim3(1), dim3(64), 0, s, B, (int)KL_NORMAL_X, kBlock);
     hipLaunchKernelGGL(kb_normal_list<0>, dim3(tiles[KB_NORMAL_X] + lb), dim3(kBlock), 0, s, B);
+  } else {
+    hipLaunchKernelGGL(kb_normal<0>, dim3(tiles[KB_NORMAL_X]), dim3(kBlock), 0, s, B);
+    hipLaunchKernelGGL(kb_normal<1>, dim3(tiles[KB_NORMAL_Y]), dim3(kBlock), 0, s, B);
   }
-  hipLaunchKernelGGL((kb_normal<1, LEARNED>), dim3(tiles[KB_NORMAL_Y]), dim3(kBlock), 0, s, B);
   if (LEARNED) {
     hipLaunchKernelGGL(kb_list_prefix, dim3(1), dim3(64), 0, s, B, (int)KL_NORMAL_Y, kBlock);
     hipLaunchKernelGGL(kb_normal_list<1>, dim3(tiles[KB_NORMAL_Y] + lb), dim3(kBlock), 0, s, B);

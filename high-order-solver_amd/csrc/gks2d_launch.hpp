@@ -91,6 +91,14 @@ __host__ __device__ inline long long cheap_tiles(const Dev& d, int dir, int tf) 
   const int rows = tf / kCheapCols, ni = dir == 0 ? d.NX + 1 : d.NX;
   return (long long)((ni + rows - 1) / rows) * cheap_tile_cols(d, dir);
 }
+// learned path with both directions' line values formed by the smooth-face
+// flux launch: one classifier launch takes both sweeps, in tiles of
+// kClsRows x kClsCols cells of the whole frame (ghost cells included)
+constexpr int kClsCols = 16, kClsRows = kSweepBlock / kClsCols;
+__host__ __device__ inline long long cls_tile_cols(const Dev& d) { return (d.ny + kClsCols - 1) / kClsCols; }
+__host__ __device__ inline long long cls_tiles(const Dev& d) {
+  return (long long)((d.nx + kClsRows - 1) / kClsRows) * cls_tile_cols(d);
+}
 
 // ------------------------------------------------------------------ batches
 // Per-problem time-loop state of a batch, on the device: the driver loop of

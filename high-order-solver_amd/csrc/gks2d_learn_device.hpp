@@ -16,12 +16,10 @@ constexpr int kMlpW1 = 0, kMlpB1 = 512, kMlpW2 = 544, kMlpB2 = 1568, kMlpW3 = 16
 // each variable the values centred on their mean and scaled by their spread
 // (in [-1, 1]), then log10 of the spread relative to the mean, / 16; the
 // last four inputs are zero padding
-GKS_DI void stencil_features(float* x, const double (*q)[5]) {
+GKS_DI void stencil_features_f(float* x, const float (*q)[5]) {
 #pragma unroll
   for (int v = 0; v < 2; v++) {
-    float f[5];
-#pragma unroll
-    for (int s = 0; s < 5; s++) f[s] = (float)q[v][s];
+    const float* f = q[v];
     const float m = ((((f[0] + f[1]) + f[2]) + f[3]) + f[4]) * 0.2f;
     float mx = f[0], mn = f[0];
 #pragma unroll
@@ -37,6 +35,14 @@ GKS_DI void stencil_features(float* x, const double (*q)[5]) {
   }
 #pragma unroll
   for (int k = 12; k < 16; k++) x[k] = 0.0f;
+}
+GKS_DI void stencil_features(float* x, const double (*q)[5]) {
+  float f[2][5];
+#pragma unroll
+  for (int v = 0; v < 2; v++)
+#pragma unroll
+    for (int s = 0; s < 5; s++) f[v][s] = (float)q[v][s];
+  stencil_features_f(x, f);
 }
 
 // (rho, p) of five conservative states (x-like frame)
