@@ -1792,9 +1792,12 @@ __device__ __forceinline__ void tanflux_cheap_body(const Dev& d, unsigned blk, u
   const Strided L{&sg[0][0][fp][q], 2 * TF}, R2{&sg[1][0][fp][q], 2 * TF};
   flux_pair_tail<DIR>(d, q, f2, nf, L, R2, stage, dt, true);
 }
+// faces per tile of the single-problem smooth-face launch: 8 x 16 for x
+// faces, 4 x 16 for y faces (each direction's faster shape)
+template <int DIR> constexpr int kCheapFaces = DIR == 0 ? 128 : 64;
 template <int DIR>
-__global__ void __launch_bounds__(2 * kTanFlux) k_tanflux_cheap(Dev d, int stage, double dt) {
-  tanflux_cheap_body<DIR, kTanFlux>(d, xcd_block(), gridDim.x, stage, dt);
+__global__ void __launch_bounds__(2 * kCheapFaces<DIR>) k_tanflux_cheap(Dev d, int stage, double dt) {
+  tanflux_cheap_body<DIR, kCheapFaces<DIR>>(d, xcd_block(), gridDim.x, stage, dt);
 }
 // one tile of tlist[DIR] per workgroup, launched for as many tiles as there
 // are faces (the count is only known on the device; workgroups past it
@@ -2311,9 +2314,12 @@ static hipError_t flux(const Dev& d, int dir, int stage, double dt, hipStream_t 
     if (d.mlp && d.smooth_flux && d.tcount) {
       // smooth faces, then the listed rest (at most nfaces of them)
       // (the listed faces follow in flux_list)
-      const dim3 grid((unsigned)cheap_tiles(d, dir, kTanFlux));
-      if (dir == 0) hipLaunchKernelGGL(k_tanflux_cheap<0>, grid, dim3(2 * kTanFlux), 0, s, d, stage, dt);
-      else hipLaunchKernelGGL(k_tanflux_cheap<1>, grid, dim3(2 * kTanFlux), 0, s, d, stage, dt);
+      if (dir == 0)
+        hipLaunchKernelGGL(k_tanflux_cheap<0>, dim3((unsigned)cheap_tiles(d, 0, kCheapFaces<0>)),
+                           dim3(2 * kCheapFaces<0>), 0, s, d, stage, dt);
+      else
+        hipLaunchKernelGGL(k_tanflux_cheap<1>, dim3((unsigned)cheap_tiles(d, 1, kCheapFaces<1>)),
+                           dim3(2 * kCheapFaces<1>), 0, s, d, stage, dt);
     } else if (d.mlp) {
       if (dir == 0) hipLaunchKernelGGL((k_tanflux<0, true>), dim3(nb), dim3(2 * kTanFlux), 0, s, d, stage, dt);
       else hipLaunchKernelGGL((k_tanflux<1, true>), dim3(nb), dim3(2 * kTanFlux), 0, s, d, stage, dt);
