@@ -151,6 +151,37 @@ def test_classifier_that_never_fires_gives_the_plain_run():
     assert np.array_equal(out[0], out[1])
 
 
+@pytest.mark.skipif(not os.path.exists(learn.DEFAULT_PARAMS), reason="trained classifier not present")
+@pytest.mark.parametrize("n", [48, 37])
+def test_one_classifier_launch_matches_the_per_direction_sweeps(n):
+    """IEEE build (every operation rounded as written): the classifier
+    launch that takes both directions at once (rho and p of each staged
+    cell formed once, 16 x 16-cell tiles of the whole frame) calls exactly
+    as many stencils linear as the two per-direction sweeps do (the path
+    without the smooth-flow flux), over each sweep's whole range: (n + 2)
+    rows of the x sweep, (n + 2) columns of the y sweep, ghost cells
+    included.  37^2 leaves partly filled tiles at the frame's edges."""
+    P, thr, _ = learn.load()
+    p = planar_shock_params(n, fast_math=0)
+    counts = []
+    for smooth in (True, False):
+        s = gks_amd.Solver2D(p)
+        s.init_prim(riemann_prim(p, riemann_states(0)))
+        s.run([BC_FREE] * 4, 0.25, 12)
+        learn.set_classifier(s, P, thr)
+        learn.classifier_options(s, smooth)
+        for side in range(4):
+            s.boundary(side, BC_FREE)
+        s.reconstruct()
+        c = learn.classifier_counts(s)
+        counts.append((c["normal_linear"], c["normal_total"]))
+        s.close()
+    say(f"[learn] {n}^2 classifier counts (linear, total): one launch {counts[0]}, per direction {counts[1]}")
+    assert counts[0] == counts[1]
+    assert counts[0][1] == (n + 2) * p.ny + p.nx * (n + 2)
+    assert 0 < counts[0][0] < counts[0][1]
+
+
 @pytest.mark.parametrize("n", [64, 40])
 def test_linear_everywhere_on_a_smooth_vortex(n):
     """A classifier that always fires: every stencil of the smooth vortex
