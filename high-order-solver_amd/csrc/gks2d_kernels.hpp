@@ -2021,6 +2021,7 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(GKS
 // the learned batch: both directions' classifiers of every problem (as
 // k_classify; the batch forms every linear line value in the smooth-face
 // flux launch)
+static_assert((kFusedLineDirs & 3) == 3, "the learned batch forms every line value in the smooth-face launch");
 __global__ void __launch_bounds__(kBlock) kb_classify(BatchArgs B) {
   GKS_BATCH_TILE(KB_NORMAL_X);
   const unsigned* st = B.start + KB_NORMAL_X * (B.np + 1);
@@ -2277,8 +2278,9 @@ static hipError_t normal(const Dev& d, int dir, hipStream_t s) {
   } else if (d.order == 9) {
     if (dir == 0) hipLaunchKernelGGL((k_normal79<0, 9>), dim3(nb), dim3(kBlock), 0, s, d);
     else hipLaunchKernelGGL((k_normal79<1, 9>), dim3(nb), dim3(kBlock), 0, s, d);
-  } else if (d.mlp && d.fused_lines == kFusedLineDirs) {
-    // both sweeps' classifiers in the x call
+  } else if (d.mlp && (d.fused_lines & 3) == 3) {
+    // both sweeps' classifiers in the x call (k_classify stores no line
+    // values: both directions form them in the smooth-face launch)
     if (dir == 0) hipLaunchKernelGGL(k_classify, dim3((unsigned)cls_tiles(d)), dim3(kBlock), 0, s, d);
   } else if (d.mlp) {
     // the listed cells follow in normal_list (whose prefix kernel leaves the
