@@ -1805,16 +1805,17 @@ __global__ void __launch_bounds__(2 * kCheapFaces<DIR>) k_tanflux_cheap(Dev d, i
 // body's register demand grows past the two-wave budget inside the loop.
 // The count is uniform over the workgroup, so every thread reaches the same
 // barriers.
-
+// faces per tile of the single-problem list launch
+constexpr int kListFaces = 128;
 template <int DIR>
-__global__ void __launch_bounds__(2 * kTanFlux) __attribute__((amdgpu_waves_per_eu(GKS_FLUX_WAVES)))
+__global__ void __launch_bounds__(2 * kListFaces) __attribute__((amdgpu_waves_per_eu(GKS_FLUX_WAVES)))
 k_tanflux_list(Dev d, int stage, double dt) {
   if (blockIdx.x == 0 && threadIdx.x == 0)
     atomicAdd(d.lstat + kListedWord + DIR, (unsigned long long)list_total(d.lpre + (KL_FLUX_X + DIR) * kListRecord));
   int s;
   unsigned tb, n;
   if (list_tile(d.lpre + (KL_FLUX_X + DIR) * kListRecord, blockIdx.x, s, tb, n))
-    tanflux_body<DIR, true, kTanFlux, true>(d, tb, stage, dt, d.tlist[DIR] + (size_t)s * d.tcap[DIR], n);
+    tanflux_body<DIR, true, kListFaces, true>(d, tb, stage, dt, d.tlist[DIR] + (size_t)s * d.tcap[DIR], n);
 }
 
 // ------------------------------------------------------------------ update
@@ -2365,10 +2366,10 @@ static hipError_t normal_list(const Dev& d, int dir, hipStream_t s) {
 static hipError_t flux_list(const Dev& d, int dir, int stage, double dt, hipStream_t s) {
   if (d.order != 5 || !d.fuse_tanflux || !d.mlp || !d.smooth_flux || !d.tcount) return hipSuccess;
   const long long nfaces = dir == 0 ? (long long)(d.NX + 1) * d.NY : (long long)d.NX * (d.NY + 1);
-  const unsigned nb = kListShards + (unsigned)((nfaces + kTanFlux - 1) / kTanFlux);
-  hipLaunchKernelGGL(k_list_prefix, dim3(1), dim3(64), 0, s, d, (int)KL_FLUX_X + dir, kTanFlux);
-  if (dir == 0) hipLaunchKernelGGL(k_tanflux_list<0>, dim3(nb), dim3(2 * kTanFlux), 0, s, d, stage, dt);
-  else hipLaunchKernelGGL(k_tanflux_list<1>, dim3(nb), dim3(2 * kTanFlux), 0, s, d, stage, dt);
+  const unsigned nb = kListShards + (unsigned)((nfaces + kListFaces - 1) / kListFaces);
+  hipLaunchKernelGGL(k_list_prefix, dim3(1), dim3(64), 0, s, d, (int)KL_FLUX_X + dir, kListFaces);
+  if (dir == 0) hipLaunchKernelGGL(k_tanflux_list<0>, dim3(nb), dim3(2 * kListFaces), 0, s, d, stage, dt);
+  else hipLaunchKernelGGL(k_tanflux_list<1>, dim3(nb), dim3(2 * kListFaces), 0, s, d, stage, dt);
   return hipGetLastError();
 }
 static hipError_t update(const Dev& d, int stage, hipStream_t s) {
