@@ -340,12 +340,24 @@ int gks2d_primvar(gks2d_ctx* c) {
   c->last_slot = 0;
   return GKS_OK;
 }
+// learned path whose smooth-face launches form every line value (one
+// classifier launch for both directions): each kind's two lists run in one
+// launch
+static bool both_lists(const Dev& d) {
+  return d.mlp && d.smooth_flux && d.order == 5 && d.fuse_tanflux && (d.fused_lines & 3) == 3;
+}
 int gks2d_reconstruct(gks2d_ctx* c) {
   if (!c) return fail(GKS_ERR_ARG, "null");
   RC(timed(c, PH_NORMAL_X, [&] { return c->lt->normal(c->d, 0, c->s); }));
-  if (c->d.mlp) RC(timed(c, PH_NORMAL_LIST_X, [&] { return c->lt->normal_list(c->d, 0, c->s); }));
-  RC(timed(c, PH_NORMAL_Y, [&] { return c->lt->normal(c->d, 1, c->s); }));
-  if (c->d.mlp) RC(timed(c, PH_NORMAL_LIST_Y, [&] { return c->lt->normal_list(c->d, 1, c->s); }));
+  if (both_lists(c->d)) {
+    // both directions' classifiers ran in the x call; their listed cells
+    // follow in one launch (timed as the x phase)
+    RC(timed(c, PH_NORMAL_LIST_X, [&] { return c->lt->normal_list(c->d, kBothDirs, c->s); }));
+  } else {
+    if (c->d.mlp) RC(timed(c, PH_NORMAL_LIST_X, [&] { return c->lt->normal_list(c->d, 0, c->s); }));
+    RC(timed(c, PH_NORMAL_Y, [&] { return c->lt->normal(c->d, 1, c->s); }));
+    if (c->d.mlp) RC(timed(c, PH_NORMAL_LIST_Y, [&] { return c->lt->normal_list(c->d, 1, c->s); }));
+  }
   RC(timed(c, PH_TANGENT_X, [&] { return c->lt->tangent(c->d, 0, c->s); }));
   return timed(c, PH_TANGENT_Y, [&] { return c->lt->tangent(c->d, 1, c->s); });
 }
@@ -353,10 +365,19 @@ int gks2d_flux(gks2d_ctx* c, int stage, double dt) {
   if (!c) return fail(GKS_ERR_ARG, "null");
   if (stage < 0 || stage > 1) return fail(GKS_ERR_ARG, "stage");
   const bool listed = c->d.mlp && c->d.smooth_flux;
-  RC(timed(c, PH_FLUX_X, [&] { return c->lt->flux(c->d, 0, stage, dt, c->s); }));
-  if (listed) RC(timed(c, PH_FLUX_LIST_X, [&] { return c->lt->flux_list(c->d, 0, stage, dt, c->s); }));
-  RC(timed(c, PH_FLUX_Y, [&] { return c->lt->flux(c->d, 1, stage, dt, c->s); }));
-  if (listed) RC(timed(c, PH_FLUX_LIST_Y, [&] { return c->lt->flux_list(c->d, 1, stage, dt, c->s); }));
+  if (both_lists(c->d)) {
+    // both smooth-face launches, then both directions' listed faces (one
+    // prefix launch; timed as the x phase): no list reads what the other
+    // direction's smooth-face launch writes
+    RC(timed(c, PH_FLUX_X, [&] { return c->lt->flux(c->d, 0, stage, dt, c->s); }));
+    RC(timed(c, PH_FLUX_Y, [&] { return c->lt->flux(c->d, 1, stage, dt, c->s); }));
+    RC(timed(c, PH_FLUX_LIST_X, [&] { return c->lt->flux_list(c->d, kBothDirs, stage, dt, c->s); }));
+  } else {
+    RC(timed(c, PH_FLUX_X, [&] { return c->lt->flux(c->d, 0, stage, dt, c->s); }));
+    if (listed) RC(timed(c, PH_FLUX_LIST_X, [&] { return c->lt->flux_list(c->d, 0, stage, dt, c->s); }));
+    RC(timed(c, PH_FLUX_Y, [&] { return c->lt->flux(c->d, 1, stage, dt, c->s); }));
+    if (listed) RC(timed(c, PH_FLUX_LIST_Y, [&] { return c->lt->flux_list(c->d, 1, stage, dt, c->s); }));
+  }
   c->last_flux_stage = stage;
   if (c->d.keep_face_fluxes) c->kept_stage_mask |= 1 << stage;
   return GKS_OK;

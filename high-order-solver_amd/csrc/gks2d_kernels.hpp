@@ -608,9 +608,13 @@ __device__ __forceinline__ unsigned list_total(const unsigned* pre) {
   for (int k = 0; k < kListShards; k++) n += pre[kListShards + 1 + k];
   return n;
 }
-// forms one list's tile prefix (single-problem path)
-__global__ void __launch_bounds__(64) k_list_prefix(Dev d, int kind, int per_tile) {
-  if (threadIdx.x == 0) list_prefix(kind < KL_FLUX_X ? d.ncount : d.tcount, kind & 1, per_tile, d.lpre + kind * kListRecord);
+// forms the tile prefixes of nkinds consecutive lists from `kind` (x and y
+// of one kind: nkinds 2), one lane each (single-problem path)
+__global__ void __launch_bounds__(64) k_list_prefix(Dev d, int kind, int nkinds, int per_tile) {
+  if ((int)threadIdx.x < nkinds) {
+    const int k = kind + (int)threadIdx.x;
+    list_prefix(k < KL_FLUX_X ? d.ncount : d.tcount, k & 1, per_tile, d.lpre + k * kListRecord);
+  }
 }
 
 // the two face records a cell's normal reconstruction writes (side 1 of
@@ -819,6 +823,13 @@ template <int DIR>
 __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(GKS_NORMAL_WAVES)))
 k_normal_list(Dev d) {
   normal_list_body<DIR>(d, blockIdx.x);
+}
+// both directions' listed cells in one launch: workgroups [0, nbx) take the
+// x list's tiles, the rest the y list's (one launch tail instead of two)
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(GKS_NORMAL_WAVES)))
+k_normal_list_xy(Dev d, unsigned nbx) {
+  if (blockIdx.x < nbx) normal_list_body<0>(d, blockIdx.x);
+  else normal_list_body<1>(d, blockIdx.x - nbx);
 }
 
 // logits of n feature rows (parity of the MFMA inference)
@@ -1818,6 +1829,7 @@ k_tanflux_list(Dev d, int stage, double dt) {
     tanflux_body<DIR, true, kListFaces, true>(d, tb, stage, dt, d.tlist[DIR] + (size_t)s * d.tcap[DIR], n);
 }
 
+
 // ------------------------------------------------------------------ update
 // cell update of Update_with_gauss (time_advance.cpp:442-474): U = U_old +
 // sum over Gauss points of len * (x_S - x_N + x_W - x_E), in the reference's
@@ -2352,24 +2364,31 @@ static hipError_t flux(const Dev& d, int dir, int stage, double dt, hipStream_t 
                        stage, dt);
   return hipGetLastError();
 }
+// dir 0 / 1: one direction's list; kBothDirs: both lists (one prefix
+// launch; the listed cells in one launch)
 static hipError_t normal_list(const Dev& d, int dir, hipStream_t s) {
   if (d.order != 5 || !d.mlp || !d.ncount) return hipSuccess;
-  // a bound on the list's tiles: each shard adds at most one partial tile
-  const unsigned nb = kListShards + (dir == 0 ? blocks_for((long long)(d.NX + 2) * d.ny)
-                                              : blocks_for((long long)d.nx * (d.NY + 2)));
-  hipLaunchKernelGGL(k_list_prefix, dim3(1), dim3(64), 0, s, d, dir == 0 ? (int)KL_NORMAL_X : (int)KL_NORMAL_Y,
-                     kBlock);
-  if (dir == 0) hipLaunchKernelGGL(k_normal_list<0>, dim3(nb), dim3(kBlock), 0, s, d);
-  else hipLaunchKernelGGL(k_normal_list<1>, dim3(nb), dim3(kBlock), 0, s, d);
+  // a bound on each list's tiles: each shard adds at most one partial tile
+  const unsigned nbx = kListShards + blocks_for((long long)(d.NX + 2) * d.ny);
+  const unsigned nby = kListShards + blocks_for((long long)d.nx * (d.NY + 2));
+  hipLaunchKernelGGL(k_list_prefix, dim3(1), dim3(64), 0, s, d, dir == 1 ? (int)KL_NORMAL_Y : (int)KL_NORMAL_X,
+                     dir == kBothDirs ? 2 : 1, kBlock);
+  if (dir == 0) hipLaunchKernelGGL(k_normal_list<0>, dim3(nbx), dim3(kBlock), 0, s, d);
+  else if (dir == 1) hipLaunchKernelGGL(k_normal_list<1>, dim3(nby), dim3(kBlock), 0, s, d);
+  else hipLaunchKernelGGL(k_normal_list_xy, dim3(nbx + nby), dim3(kBlock), 0, s, d, nbx);
   return hipGetLastError();
 }
 static hipError_t flux_list(const Dev& d, int dir, int stage, double dt, hipStream_t s) {
   if (d.order != 5 || !d.fuse_tanflux || !d.mlp || !d.smooth_flux || !d.tcount) return hipSuccess;
-  const long long nfaces = dir == 0 ? (long long)(d.NX + 1) * d.NY : (long long)d.NX * (d.NY + 1);
-  const unsigned nb = kListShards + (unsigned)((nfaces + kListFaces - 1) / kListFaces);
-  hipLaunchKernelGGL(k_list_prefix, dim3(1), dim3(64), 0, s, d, (int)KL_FLUX_X + dir, kListFaces);
-  if (dir == 0) hipLaunchKernelGGL(k_tanflux_list<0>, dim3(nb), dim3(2 * kListFaces), 0, s, d, stage, dt);
-  else hipLaunchKernelGGL(k_tanflux_list<1>, dim3(nb), dim3(2 * kListFaces), 0, s, d, stage, dt);
+  const long long nfx = (long long)(d.NX + 1) * d.NY, nfy = (long long)d.NX * (d.NY + 1);
+  const unsigned nbx = kListShards + (unsigned)((nfx + kListFaces - 1) / kListFaces);
+  const unsigned nby = kListShards + (unsigned)((nfy + kListFaces - 1) / kListFaces);
+  hipLaunchKernelGGL(k_list_prefix, dim3(1), dim3(64), 0, s, d, (int)KL_FLUX_X + (dir == 1 ? 1 : 0),
+                     dir == kBothDirs ? 2 : 1, kListFaces);
+  // (both: one prefix launch, then a launch per direction: one kernel
+  // holding both directions' flux bodies exceeds two waves' registers)
+  if (dir != 1) hipLaunchKernelGGL(k_tanflux_list<0>, dim3(nbx), dim3(2 * kListFaces), 0, s, d, stage, dt);
+  if (dir != 0) hipLaunchKernelGGL(k_tanflux_list<1>, dim3(nby), dim3(2 * kListFaces), 0, s, d, stage, dt);
   return hipGetLastError();
 }
 static hipError_t update(const Dev& d, int stage, hipStream_t s) {

@@ -80,6 +80,9 @@ constexpr int kListRecord = 48;
 // learned path: directions (bit 0 x, bit 1 y) whose smooth-face flux
 // launch forms the linear line values from the cell averages
 constexpr int kFusedLineDirs = 3;
+// the list launchers' direction argument for both directions' lists (one
+// prefix launch for the two)
+constexpr int kBothDirs = 2;
 // learned path: the smooth-face flux launch takes tiles of TF faces,
 // TF / kCheapCols rows (i) by kCheapCols faces (j)
 constexpr int kCheapCols = 16;
@@ -166,8 +169,8 @@ struct LaunchTable {
   hipError_t (*flux)(const Dev&, int dir, int stage, double dt, hipStream_t);
   // learned path: the listed normal cells (after normal) and flux faces
   // (after flux) of direction dir; nothing otherwise
-  hipError_t (*normal_list)(const Dev&, int dir, hipStream_t);
-  hipError_t (*flux_list)(const Dev&, int dir, int stage, double dt, hipStream_t);
+  hipError_t (*normal_list)(const Dev&, int dir, hipStream_t);  // dir kBothDirs: both lists
+  hipError_t (*flux_list)(const Dev&, int dir, int stage, double dt, hipStream_t);  // likewise
   hipError_t (*update)(const Dev&, int stage, hipStream_t);
   // the update, and the new state's blow-up test and CFL speed into stat
   // slot `slot` (as primvar's)
